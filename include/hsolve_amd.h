@@ -1,0 +1,277 @@
+/*
+ * hsolve_amd.h — C ABI of the MI355X HSolve engine.
+ *
+ * One engine holds a batch of HSolve instances ("cells").  Each cell is the
+ * model that one moose.HSolve object takes over in the reference: its
+ * compartments, HHChannels, HHGates, CaConcs, SpikeGens and Leakage channels,
+ * described in MOOSE object terms (creation order, message order) so that the
+ * engine can derive the Hines numbering, junctions and lookup tables the same
+ * way the reference does.  All per-step work runs on the GPU; the host only
+ * sets up, launches and moves fields between steps.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative HSG_E* code on
+ *     failure; hsg_last_error() then returns a thread-local message.  Nothing
+ *     here aborts the process (the reference prints to cerr and returns, or
+ *     asserts on an unknown Id: hsolve/HSolve.cpp:301-321,411-431,
+ *     hsolve/HSolveInterface.cpp:23-36).
+ *   - Object indices ("compartment 3", "channel 7", "pool 0") are positions in
+ *     the arrays of the hsg_cell_desc passed to hsg_add_cell, i.e. the caller's
+ *     own numbering, never the Hines numbering.
+ *   - Calls are made from one host thread per engine, between steps.
+ *
+ * Reference interfaces replaced (paths relative to the moose-core tree):
+ *   HSolve Cinfo and process/reinit ........ hsolve/HSolve.cpp:37-233
+ *   HSolve::setPath / setup ................. hsolve/HSolve.cpp:286-334
+ *   HSolveActive::step ...................... hsolve/HSolveActive.cpp:45-66
+ *   zombie field access (get/set) ........... hsolve/HSolveInterface.cpp:66-533
+ *   per-step inputs addInject/addGkEk/
+ *     addConc/iCa ........................... hsolve/HSolveInterface.cpp:213-259,469-480
+ *   SpikeGen spikeOut (via sendSpikes) ...... biophysics/SpikeGen.cpp:202-216,
+ *                                             hsolve/HSolveActive.cpp:342-347
+ */
+#ifndef HSOLVE_AMD_H
+#define HSOLVE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define HSG_OK            0
+#define HSG_E_ARG        -1   /* bad argument (null pointer, bad index, bad size) */
+#define HSG_E_MODEL      -2   /* the model cannot be taken over (see message)     */
+#define HSG_E_STATE      -3   /* call not valid in the engine's current state     */
+#define HSG_E_DEVICE     -4   /* a HIP runtime call failed                         */
+#define HSG_E_NOMEM      -5   /* device memory budget exceeded (admission)         */
+#define HSG_E_UNSUPPORTED -6  /* feature outside what the GPU path implements      */
+
+/* ---- precision forms --------------------------------------------------- */
+#define HSG_FP64   0   /* everything in FP64 (the reference's arithmetic)           */
+#define HSG_MIXED  1   /* gate tables, gate states, channel currents in FP32;
+                          Hines elimination, back-substitution and Vm in FP64      */
+
+/* ---- model description (MOOSE object level) ---------------------------- */
+
+/* One HHGate prototype (biophysics/HHGate.h): tables A (alpha) and B
+ * (alpha+beta) of divs+1 entries over [min, max].  `id` orders gates the way
+ * std::set<Id> does in HSolveActive::createLookupTables
+ * (hsolve/HSolveActiveSetup.cpp:339-358): ascending object id. */
+typedef struct hsg_gate_desc {
+    int64_t       id;
+    double        min;
+    double        max;
+    uint32_t      divs;
+    const double* tableA;
+    const double* tableB;
+} hsg_gate_desc;
+
+/* One Compartment (biophysics/CompartmentBase.cpp:191-235). */
+typedef struct hsg_compartment_desc {
+    double Ra, Rm, Cm, Em, initVm, Vm, inject;
+} hsg_compartment_desc;
+
+/* One HHChannel (biophysics/HHChannelBase.cpp:31-76, ChanBase.cpp:90-117).
+ * Channels of a compartment appear in the order of its "channel" messages.
+ * gate[0..2] index hsg_cell_desc.gates for the X, Y, Z gates (-1: none; a gate
+ * is only read when its power is > 0).  caTarget: the pool fed by IkOut (the
+ * first CaConc target), caDepend: the pool sending "concen" (the last one);
+ * -1 when absent (hsolve/HSolveActiveSetup.cpp:254-337). */
+typedef struct hsg_channel_desc {
+    int32_t compartment;
+    int32_t instant;
+    int32_t useConcentration;
+    int32_t gate[3];
+    int32_t caTarget;
+    int32_t caDepend;
+    double  Gbar, Ek, Xpower, Ypower, Zpower, X, Y, Z, modulation;
+} hsg_channel_desc;
+
+/* One CaConc pool (biophysics/CaConcBase.cpp:58-140). */
+typedef struct hsg_caconc_desc {
+    double Ca, CaBasal, tau, B, ceiling, floor;
+} hsg_caconc_desc;
+
+/* One SpikeGen fed by a compartment's VmOut (biophysics/SpikeGen.cpp:127-134). */
+typedef struct hsg_spikegen_desc {
+    int32_t compartment;
+    int32_t edgeTriggered;
+    double  threshold, refractT;
+} hsg_spikegen_desc;
+
+/* One Leakage channel on a compartment's "channel" message
+ * (biophysics/Leakage.cpp:90-102); folded into EmByRm at setup
+ * (hsolve/HSolvePassive.cpp:138-156). */
+typedef struct hsg_leak_desc {
+    int32_t compartment;
+    double  Gk, Ek;
+} hsg_leak_desc;
+
+/* One cell.  axial[2*k], axial[2*k+1] = (parent, child) of the k-th
+ * "axial"->"raxial" message, in creation order.  seed: the compartment
+ * HSolve.target resolves to (hsolve/HSolve.cpp:315-334). */
+typedef struct hsg_cell_desc {
+    int32_t                     n_compartments;
+    const hsg_compartment_desc* compartments;
+    int32_t                     n_axial;
+    const int32_t*              axial;
+    int32_t                     seed;
+    int32_t                     n_channels;
+    const hsg_channel_desc*     channels;
+    int32_t                     n_gates;
+    const hsg_gate_desc*        gates;
+    int32_t                     n_pools;
+    const hsg_caconc_desc*      pools;
+    int32_t                     n_spikegens;
+    const hsg_spikegen_desc*    spikegens;
+    int32_t                     n_leaks;
+    const hsg_leak_desc*        leaks;
+} hsg_cell_desc;
+
+/* ---- engine ------------------------------------------------------------ */
+
+typedef struct hsg_engine hsg_engine;
+
+typedef struct hsg_options {
+    int32_t precision;        /* HSG_FP64 or HSG_MIXED                              */
+    int32_t device;           /* HIP device ordinal                                 */
+    int64_t mem_budget;       /* bytes of device memory the engine may hold; 0 = no
+                                 limit.  hsg_add_cell fails with HSG_E_NOMEM beyond */
+    int32_t steps_per_launch; /* steps fused into one kernel launch (0 = default)   */
+    int32_t graph_chunk;      /* launches captured per hipGraph (0 = no graph)      */
+    int32_t block_threads;    /* threads per cell workgroup (0 = autotuned)         */
+} hsg_options;
+
+/* Creates an engine (replaces Shell::doCreate("HSolve") and the HSolve
+ * constructor, hsolve/HSolve.cpp:196-199). */
+int hsg_create(const hsg_options* opts, hsg_engine** out);
+
+/* Frees every device and host buffer (hsolve/HSolve.cpp:201-215). */
+int hsg_destroy(hsg_engine* e);
+
+/* Takes over one cell: the work of HSolve::setPath/setup with the solver's
+ * dt and caAdvance fields (hsolve/HSolve.cpp:286-334,409-431;
+ * hsolve/HSolveActiveSetup.cpp:17-41).  dt must be > 0.  The cell starts in
+ * the state the model objects held (their Vm, X/Y/Z, Ca); call hsg_reinit to
+ * reset it. *cell_id receives the engine-wide cell number. */
+int hsg_add_cell(hsg_engine* e, const hsg_cell_desc* desc, double dt,
+                 int32_t caAdvance, int32_t* cell_id);
+
+/* Releases a cell's slot (serving: a finished job leaves the batch). */
+int hsg_remove_cell(hsg_engine* e, int32_t cell);
+
+/* HSolve::reinit for every cell (hsolve/HSolve.cpp:229-233,
+ * hsolve/HSolveActiveSetup.cpp:43-160): V=initVm, Ca=CaBasal, gates at their
+ * steady state, spike generators re-armed, time reset to 0. dt is the clock
+ * tick used by later steps. */
+int hsg_reinit(hsg_engine* e, double dt);
+
+/* Same, for one cell only (a job admitted into a running batch). */
+int hsg_reinit_cell(hsg_engine* e, int32_t cell);
+
+/* Advances every live cell by nsteps ticks: HSolve::process per tick
+ * (hsolve/HSolve.cpp:221-227 -> hsolve/HSolveActive.cpp:45-66). */
+int hsg_step(hsg_engine* e, int64_t nsteps);
+
+/* Advances only the given cells by their own step counts (serving). */
+int hsg_step_cells(hsg_engine* e, const int32_t* cells, const int64_t* nsteps,
+                   int32_t n);
+
+/* ---- zombie fields ----------------------------------------------------- */
+/* Compartment fields (index = compartment) */
+#define HSG_F_VM        1   /* getVm/setVm           HSolveInterface.cpp:66-79   */
+#define HSG_F_IM        2   /* getIm (read-only)     HSolveInterface.cpp:161-182 */
+#define HSG_F_INJECT    3   /* get/setInject         HSolveInterface.cpp:190-211 */
+#define HSG_F_INITVM    4   /* get/setInitVm         HSolveInterface.cpp:146-158 */
+#define HSG_F_CM        5   /* get/setCm             HSolveInterface.cpp:81-96   */
+#define HSG_F_EM        6   /* get/setEm             HSolveInterface.cpp:98-113  */
+#define HSG_F_RM        7   /* get/setRm             HSolveInterface.cpp:115-130 */
+#define HSG_F_RA        8   /* get/setRa (no matrix rebuild) :132-144            */
+/* HHChannel fields (index = channel) */
+#define HSG_F_GBAR     20   /* HSolveInterface.cpp:287-300 */
+#define HSG_F_EK       21   /* :302-314 */
+#define HSG_F_GK       22   /* :316-328 */
+#define HSG_F_IK       23   /* read-only :330-339 */
+#define HSG_F_X        24   /* :341-367 */
+#define HSG_F_Y        25   /* :369-403 */
+#define HSG_F_Z        26   /* :405-443 */
+#define HSG_F_MODULATION 27 /* set only if > 0: :445-451 */
+#define HSG_F_INSTANT  28   /* :273-285 */
+/* CaConc fields (index = pool) */
+#define HSG_F_CA       40   /* :453-467 */
+#define HSG_F_CABASAL  41   /* :482-495 */
+#define HSG_F_CACEIL   42   /* :505-518 */
+#define HSG_F_CAFLOOR  43   /* :520-533 */
+
+/* Reads n values of `field` for objects idx[0..n-1] of `cell` into out. */
+int hsg_get_field(hsg_engine* e, int32_t cell, int32_t field,
+                  const int32_t* idx, int32_t n, double* out);
+/* Writes n values (same indexing). */
+int hsg_set_field(hsg_engine* e, int32_t cell, int32_t field,
+                  const int32_t* idx, int32_t n, const double* vals);
+
+/* ---- per-step inputs (consumed by the next step) ------------------------ */
+/* HSolve::addInject: injectVarying += value (HSolveInterface.cpp:213-219). */
+int hsg_add_inject(hsg_engine* e, int32_t cell, int32_t compartment, double value);
+/* HSolve::addGkEk: external channel current (HSolveInterface.cpp:246-252). */
+int hsg_add_gkek(hsg_engine* e, int32_t cell, int32_t compartment, double Gk, double Ek);
+/* HSolve::addConc: external [Ca] seen by a channel (HSolveInterface.cpp:254-259). */
+int hsg_add_conc(hsg_engine* e, int32_t cell, int32_t channel, double conc);
+/* HSolve::iCa: extra Ca-pool activation (HSolveInterface.cpp:469-480). */
+int hsg_add_ica(hsg_engine* e, int32_t cell, int32_t pool, double iCa);
+
+/* ---- outputs ------------------------------------------------------------- */
+typedef struct hsg_spike {
+    int32_t cell;
+    int32_t spikegen;   /* index into the cell's spikegens */
+    int64_t step;       /* 1-based tick index since reinit; time = step * dt */
+} hsg_spike;
+
+/* Copies out (and clears) the spikes recorded since the last call.  *n_out
+ * receives how many were written (at most cap); *dropped the number lost to
+ * ring-buffer overflow since the last call. */
+int hsg_read_spikes(hsg_engine* e, hsg_spike* out, int32_t cap, int32_t* n_out,
+                    int64_t* dropped);
+
+/* Records Vm of the listed compartments of `cell` at the end of every step
+ * into a device trace of `capacity` samples each (the Table-on-tick-8
+ * semantics of builtins/Table.cpp:255-285).  Replaces any earlier probe set of
+ * that cell. */
+int hsg_set_probes(hsg_engine* e, int32_t cell, const int32_t* compartments,
+                   int32_t n, int64_t capacity);
+/* Copies the recorded samples: out[k*n + j] = sample k of probe j. *n_samples
+ * receives the number of samples (<= capacity).  Clears the trace. */
+int hsg_read_probes(hsg_engine* e, int32_t cell, double* out, int64_t cap_samples,
+                    int64_t* n_samples);
+
+/* ---- introspection ------------------------------------------------------- */
+typedef struct hsg_cell_info {
+    int32_t n_compartments, n_channels, n_gates, n_pools, n_spikegens;
+    int32_t n_levels;         /* elimination levels of the parallel schedule */
+    int32_t max_group;        /* largest junction group (members)            */
+    int64_t device_bytes;     /* device memory attributed to the cell        */
+    int64_t steps_done;
+} hsg_cell_info;
+int hsg_cell_info_get(hsg_engine* e, int32_t cell, hsg_cell_info* out);
+
+/* Hines index of each compartment (out[i] for compartment i): the numbering of
+ * HSolvePassive::walkTree (hsolve/HSolvePassive.cpp:45-98). */
+int hsg_hines_order(hsg_engine* e, int32_t cell, int32_t* out);
+
+/* Blocks until all queued device work of the engine is done. */
+int hsg_sync(hsg_engine* e);
+
+/* Kernel timing of the most recent hsg_step/hsg_step_cells call, from HIP
+ * events recorded on the engine's stream around each launch: number of
+ * launches and their summed and mean duration in milliseconds. */
+int hsg_last_step_timing(hsg_engine* e, int32_t* launches, double* total_ms,
+                         double* mean_ms);
+
+const char* hsg_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HSOLVE_AMD_H */

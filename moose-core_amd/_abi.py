@@ -1,0 +1,133 @@
+"""ctypes mirror of include/hsolve_amd.h (structures and constants).
+
+The model description structures are the drop-in boundary: they carry a cell
+in MOOSE object terms (compartments in creation order, axial messages in
+creation order, channels in channel-message order, gate prototypes with their
+object ids), which is what HSolve reads at setup
+(hsolve/HSolvePassive.cpp:45-205, hsolve/HSolveActiveSetup.cpp:162-526).
+"""
+import ctypes as C
+
+import numpy as np
+
+HSG_OK = 0
+HSG_E_ARG = -1
+HSG_E_MODEL = -2
+HSG_E_STATE = -3
+HSG_E_DEVICE = -4
+HSG_E_NOMEM = -5
+HSG_E_UNSUPPORTED = -6
+
+HSG_FP64 = 0
+HSG_MIXED = 1
+
+FIELDS = {
+    "Vm": 1, "Im": 2, "inject": 3, "initVm": 4, "Cm": 5, "Em": 6, "Rm": 7, "Ra": 8,
+    "Gbar": 20, "Ek": 21, "Gk": 22, "Ik": 23, "X": 24, "Y": 25, "Z": 26,
+    "modulation": 27, "instant": 28,
+    "Ca": 40, "CaBasal": 41, "ceiling": 42, "floor": 43,
+}
+COMPARTMENT_FIELDS = {k for k, v in FIELDS.items() if v < 20}
+CHANNEL_FIELDS = {k for k, v in FIELDS.items() if 20 <= v < 40}
+POOL_FIELDS = {k for k, v in FIELDS.items() if v >= 40}
+
+
+class GateDesc(C.Structure):
+    _fields_ = [("id", C.c_int64), ("min", C.c_double), ("max", C.c_double),
+                ("divs", C.c_uint32), ("tableA", C.POINTER(C.c_double)),
+                ("tableB", C.POINTER(C.c_double))]
+
+
+class CellDesc(C.Structure):
+    _fields_ = [("n_compartments", C.c_int32), ("compartments", C.c_void_p),
+                ("n_axial", C.c_int32), ("axial", C.c_void_p),
+                ("seed", C.c_int32),
+                ("n_channels", C.c_int32), ("channels", C.c_void_p),
+                ("n_gates", C.c_int32), ("gates", C.POINTER(GateDesc)),
+                ("n_pools", C.c_int32), ("pools", C.c_void_p),
+                ("n_spikegens", C.c_int32), ("spikegens", C.c_void_p),
+                ("n_leaks", C.c_int32), ("leaks", C.c_void_p)]
+
+
+class Options(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("device", C.c_int32),
+                ("mem_budget", C.c_int64), ("steps_per_launch", C.c_int32),
+                ("graph_chunk", C.c_int32), ("block_threads", C.c_int32)]
+
+
+class Spike(C.Structure):
+    _fields_ = [("cell", C.c_int32), ("spikegen", C.c_int32), ("step", C.c_int64)]
+
+
+class CellInfo(C.Structure):
+    _fields_ = [("n_compartments", C.c_int32), ("n_channels", C.c_int32),
+                ("n_gates", C.c_int32), ("n_pools", C.c_int32),
+                ("n_spikegens", C.c_int32), ("n_levels", C.c_int32),
+                ("max_group", C.c_int32), ("device_bytes", C.c_int64),
+                ("steps_done", C.c_int64)]
+
+
+# numpy record layouts identical to the C structs (natural alignment)
+COMPARTMENT_DT = np.dtype([(k, "f8") for k in ("Ra", "Rm", "Cm", "Em", "initVm", "Vm", "inject")])
+CHANNEL_DT = np.dtype([("compartment", "i4"), ("instant", "i4"), ("useConcentration", "i4"),
+                       ("gate", "i4", (3,)), ("caTarget", "i4"), ("caDepend", "i4")]
+                      + [(k, "f8") for k in ("Gbar", "Ek", "Xpower", "Ypower", "Zpower",
+                                             "X", "Y", "Z", "modulation")], align=True)
+CACONC_DT = np.dtype([(k, "f8") for k in ("Ca", "CaBasal", "tau", "B", "ceiling", "floor")])
+SPIKEGEN_DT = np.dtype([("compartment", "i4"), ("edgeTriggered", "i4"),
+                        ("threshold", "f8"), ("refractT", "f8")], align=True)
+LEAK_DT = np.dtype([("compartment", "i4"), ("Gk", "f8"), ("Ek", "f8")], align=True)
+
+assert CHANNEL_DT.itemsize == 104 and SPIKEGEN_DT.itemsize == 24 and LEAK_DT.itemsize == 24
+
+
+def _ptr(a):
+    return a.ctypes.data if a is not None and len(a) else None
+
+
+class PackedCell:
+    """A cell description packed into C layout; keeps its arrays alive."""
+
+    def __init__(self, spec):
+        self.spec = spec
+        self.compartments = np.ascontiguousarray(spec.compartments, dtype=COMPARTMENT_DT)
+        self.axial = np.ascontiguousarray(spec.axial, dtype=np.int32).reshape(-1)
+        self.channels = np.ascontiguousarray(spec.channels, dtype=CHANNEL_DT)
+        self.pools = np.ascontiguousarray(spec.pools, dtype=CACONC_DT)
+        self.spikegens = np.ascontiguousarray(spec.spikegens, dtype=SPIKEGEN_DT)
+        self.leaks = np.ascontiguousarray(spec.leaks, dtype=LEAK_DT)
+        self._tables = []
+        gates = (GateDesc * max(1, len(spec.gates)))()
+        for i, g in enumerate(spec.gates):
+            a = np.ascontiguousarray(g.tableA, dtype=np.float64)
+            b = np.ascontiguousarray(g.tableB, dtype=np.float64)
+            if len(a) != g.divs + 1 or len(b) != g.divs + 1:
+                raise ValueError(f"gate {g.id}: tables need divs+1={g.divs + 1} entries")
+            self._tables += [a, b]
+            gates[i].id = g.id
+            gates[i].min = g.min
+            gates[i].max = g.max
+            gates[i].divs = g.divs
+            gates[i].tableA = a.ctypes.data_as(C.POINTER(C.c_double))
+            gates[i].tableB = b.ctypes.data_as(C.POINTER(C.c_double))
+        self._gates = gates
+        d = CellDesc()
+        d.n_compartments = len(self.compartments)
+        d.compartments = _ptr(self.compartments)
+        d.n_axial = len(self.axial) // 2
+        d.axial = _ptr(self.axial)
+        d.seed = spec.seed
+        d.n_channels = len(self.channels)
+        d.channels = _ptr(self.channels)
+        d.n_gates = len(spec.gates)
+        d.gates = gates
+        d.n_pools = len(self.pools)
+        d.pools = _ptr(self.pools)
+        d.n_spikegens = len(self.spikegens)
+        d.spikegens = _ptr(self.spikegens)
+        d.n_leaks = len(self.leaks)
+        d.leaks = _ptr(self.leaks)
+        self.desc = d
+
+    def ref(self):
+        return C.byref(self.desc)

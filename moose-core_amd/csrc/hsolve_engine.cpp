@@ -1,0 +1,1356 @@
+// hsolve_engine.cpp — the C ABI (include/hsolve_amd.h) over HIP.
+//
+// An engine owns one HIP stream and one device arena.  Each cell lives in its
+// own region of the arena (first-fit allocation, so cells can join and leave a
+// running batch).  hsg_step launches one workgroup per live cell and fuses
+// `steps_per_launch` ticks into each launch; optionally the steady-state
+// launches are captured once into a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/hsolve_amd.h"
+#include "hsolve_internal.h"
+#include "hsolve_setup.h"
+
+extern "C" hipError_t hsg_launch_step(const hsg::LaunchArgs* a, int nblocks, int threads, int fp32,
+                                     hipStream_t stream);
+extern "C" hipError_t hsg_launch_reinit(const hsg::LaunchArgs* a, int nblocks, int fp32,
+                                       hipStream_t stream);
+extern "C" hipError_t hsg_set_max_lds(int bytes);
+
+using namespace hsg;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return fail(HSG_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int64_t kAlign = 256;
+constexpr int kSpikeCap = 1 << 20;
+
+int64_t align_up(int64_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+
+// First-fit allocator over [0, cap) with coalescing free list.
+struct Arena {
+    char* base = nullptr;
+    int64_t cap = 0;
+    int64_t used = 0;
+    std::map<int64_t, int64_t> freeList;   // offset -> size
+
+    int64_t alloc(int64_t bytes) {
+        bytes = align_up(std::max<int64_t>(bytes, kAlign));
+        for (auto it = freeList.begin(); it != freeList.end(); ++it) {
+            if (it->second >= bytes) {
+                int64_t off = it->first;
+                int64_t rest = it->second - bytes;
+                freeList.erase(it);
+                if (rest > 0) freeList[off + bytes] = rest;
+                used += bytes;
+                return off;
+            }
+        }
+        return -1;
+    }
+    void release(int64_t off, int64_t bytes) {
+        bytes = align_up(std::max<int64_t>(bytes, kAlign));
+        used -= bytes;
+        auto it = freeList.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != freeList.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            freeList.erase(nx);
+        }
+        if (it != freeList.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                freeList.erase(it);
+            }
+        }
+    }
+};
+
+struct Block {
+    int64_t off = -1, bytes = 0;
+};
+
+struct HostCell {
+    int id = -1;
+    bool live = false;
+    CellSetup s;
+    CellDev dev{};
+    Block region;          // main region
+    Block inj, probe, extBlk; // side blocks
+    double hsolveDt = 0;   // HSolve::dt_ (setup dt until reinit)
+    int64_t bytes = 0;
+    // host mirrors of per-call inputs (reference: inject_ map, externalCurrent_)
+    std::vector<int> injIdx;           // Hines index per injection entry
+    std::vector<double> injBasal, injVar;
+    std::map<int, std::pair<double, double>> ext;   // Hines idx -> (sum Gk, sum Gk*Ek)
+    std::vector<double> prevExtG, prevExtGE;        // for getIm
+    std::map<int, double> ica;                      // desc pool -> pending iCa
+    std::vector<int> typeIds;                       // local type -> engine type
+    int tableSet = 0;
+    std::vector<int> probeCompt;
+    int64_t probeCap = 0;
+    int64_t stepsDone = 0;
+    bool injDirty = false;
+    // Gk as of the last step for channels whose X/Y/Z, Gbar or modulation was
+    // written since (the reference keeps current_[].Gk until the next step)
+    std::map<int, double> gkOverride;
+};
+
+}  // namespace
+
+struct hsg_engine {
+    hsg_options opt{};
+    int fp32 = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Arena arena;
+    std::vector<std::unique_ptr<HostCell>> cells;
+    std::vector<ChanType> types;
+    std::map<std::string, int> typeKey;
+    std::vector<TableSetDev> tsets;
+    std::map<std::string, int> tsetKey;
+    // device-side tables of the launch
+    CellDev* dCells = nullptr;
+    int dCellsCap = 0;
+    int32_t* dOrder = nullptr;
+    int dOrderCap = 0;
+    ChanType* dTypes = nullptr;
+    int dTypesCap = 0;
+    TableSetDev* dTsets = nullptr;
+    int dTsetsCap = 0;
+    SpikeRec* dSpikes = nullptr;
+    unsigned long long* dSpikeCount = nullptr;
+    int64_t spikesDropped = 0;
+    bool tablesDirty = true;
+    std::vector<int32_t> order;
+    int ldsLimit = 65536;
+    int maxLdsUsed = 0;
+    double dt = 0;                // clock tick
+    bool reinitDone = false;
+    // timing of the last step call
+    std::vector<hipEvent_t> evStart, evStop;
+    int lastLaunches = 0;
+    double lastTotalMs = 0;
+    // graph of the steady-state launches
+    hipGraphExec_t graphExec = nullptr;
+    int graphSteps = 0;
+    bool graphValid = false;
+};
+
+namespace {
+
+template <typename T>
+int upload(hsg_engine* e, int64_t off, const std::vector<T>& v) {
+    if (v.empty()) return 0;
+    HIPCHK(hipMemcpyAsync(e->arena.base + off, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice,
+                          e->stream));
+    return 0;
+}
+
+int grow_arena(hsg_engine* e, int64_t need) {
+    if (e->opt.mem_budget > 0) return fail(HSG_E_NOMEM, "device memory budget exceeded");
+    int64_t ncap = std::max<int64_t>(e->arena.cap * 2, e->arena.cap + align_up(need) + (64 << 20));
+    char* nb = nullptr;
+    HIPCHK(hipMalloc(&nb, ncap));
+    if (e->arena.base) {
+        HIPCHK(hipMemcpyAsync(nb, e->arena.base, e->arena.cap, hipMemcpyDeviceToDevice, e->stream));
+        HIPCHK(hipStreamSynchronize(e->stream));
+        HIPCHK(hipFree(e->arena.base));
+    }
+    // the new space at the end is free
+    int64_t oldCap = e->arena.cap;
+    e->arena.base = nb;
+    e->arena.cap = ncap;
+    e->arena.freeList[oldCap] = ncap - oldCap;
+    // coalesce with a trailing free block
+    auto it = e->arena.freeList.find(oldCap);
+    if (it != e->arena.freeList.begin()) {
+        auto pv = std::prev(it);
+        if (pv->first + pv->second == it->first) {
+            pv->second += it->second;
+            e->arena.freeList.erase(it);
+        }
+    }
+    e->graphValid = false;
+    return 0;
+}
+
+int arena_alloc(hsg_engine* e, int64_t bytes, int64_t* off) {
+    int64_t o = e->arena.alloc(bytes);
+    if (o < 0) {
+        int rc = grow_arena(e, bytes);
+        if (rc) return rc;
+        o = e->arena.alloc(bytes);
+        if (o < 0) return fail(HSG_E_NOMEM, "arena allocation failed");
+    }
+    *off = o;
+    return 0;
+}
+
+template <typename T>
+int grow_dev(T** p, int* cap, int need) {
+    if (need <= *cap) return 0;
+    int ncap = std::max(need, *cap * 2 + 16);
+    if (*p) HIPCHK(hipFree(*p));
+    HIPCHK(hipMalloc(p, sizeof(T) * (size_t)ncap));
+    *cap = ncap;
+    return 0;
+}
+
+int table_set_id(hsg_engine* e, const HostTableSet& t, int* id) {
+    auto it = e->tsetKey.find(t.key);
+    if (it != e->tsetKey.end()) { *id = it->second; return 0; }
+    TableSetDev ts{};
+    const HostTable* hs[2] = {&t.v, &t.ca};
+    TableDev* ds[2] = {&ts.v, &ts.ca};
+    for (int k = 0; k < 2; ++k) {
+        const HostTable& h = *hs[k];
+        TableDev& dv = *ds[k];
+        dv.min = h.min;
+        dv.max = h.max;
+        dv.dx = h.dx;
+        dv.invDx = 1.0 / h.dx;
+        dv.nPts = (int)h.nPts();
+        dv.nColumns = (int)h.nColumns();
+        int64_t o64 = 0, o32 = 0;
+        int64_t cnt = (int64_t)h.table.size();
+        int rc = arena_alloc(e, std::max<int64_t>(cnt, 1) * 8, &o64);
+        if (rc) return rc;
+        rc = arena_alloc(e, std::max<int64_t>(cnt, 1) * 4, &o32);
+        if (rc) return rc;
+        std::vector<float> f(h.table.begin(), h.table.end());
+        rc = upload(e, o64, h.table);
+        if (rc) return rc;
+        rc = upload(e, o32, f);
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(e->stream));
+        dv.off64 = o64;
+        dv.off32 = o32;
+    }
+    *id = (int)e->tsets.size();
+    e->tsets.push_back(ts);
+    e->tsetKey[t.key] = *id;
+    e->tablesDirty = true;
+    return 0;
+}
+
+int type_id(hsg_engine* e, const HostChanType& t, int tableSet) {
+    char kb[320];
+    snprintf(kb, sizeof kb, "%a|%a|%a|%d|%d|%d|%d|%d|%d|%d", t.Xpower, t.Ypower, t.Zpower, t.instant,
+             t.colX, t.colY, t.colZ, t.caDepLocal, t.caTgtLocal, tableSet);
+    auto it = e->typeKey.find(kb);
+    if (it != e->typeKey.end()) return it->second;
+    ChanType c{};
+    c.Xpower = t.Xpower;
+    c.Ypower = t.Ypower;
+    c.Zpower = t.Zpower;
+    c.instant = t.instant;
+    c.nGates = t.nGates;
+    c.colX = t.colX;
+    c.colY = t.colY;
+    c.colZ = t.colZ;
+    c.caDepLocal = t.caDepLocal;
+    c.caTgtLocal = t.caTgtLocal;
+    c.tableSet = tableSet;
+    int id = (int)e->types.size();
+    e->types.push_back(c);
+    e->typeKey[kb] = id;
+    e->tablesDirty = true;
+    return id;
+}
+
+int64_t work_bytes(const CellDev& c) {
+    return 8 * (2 * (int64_t)c.n + c.nHJ + 2 * (int64_t)c.nTerm);
+}
+
+// Lays out and uploads one cell's main region.
+int place_cell(hsg_engine* e, HostCell& hc) {
+    CellSetup& s = hc.s;
+    const int n = s.n;
+    const int gt = e->fp32 ? 4 : 8;
+    const int64_t CS = (int64_t)s.chanSlots * n, GS = (int64_t)s.gateSlots * n,
+                  PS = (int64_t)s.poolSlots * n;
+    // sizes of every array of the region
+    struct Item { int64_t* off; int64_t bytes; };
+    CellDev& d = hc.dev;
+    std::memset(&d, 0, sizeof d);
+    int64_t nGroups = (int64_t)s.groups.size(), nMem = 0;
+    for (auto& g : s.groups) nMem += (int64_t)g.mem.size();
+    int64_t workB = 8 * (2 * (int64_t)n + (int64_t)s.HJ0.size() + 2 * (int64_t)s.nTerm);
+    std::vector<Item> items = {
+        {&d.oV, 8 * n}, {&d.oCmByDt, 8 * n}, {&d.oEmByRm, 8 * n}, {&d.oDbase, 8 * n}, {&d.oOff, 8 * n},
+        {&d.oDefer, 8 * n},
+        {&d.oChType, 4 * CS}, {&d.oGbar, gt * CS}, {&d.oEk, gt * CS}, {&d.oMod, 8 * CS}, {&d.oExtCa, 8 * CS},
+        {&d.oState, gt * GS},
+        {&d.oPoolC, 8 * PS}, {&d.oPoolCa, 8 * PS}, {&d.oPoolBasal, 8 * PS}, {&d.oPoolF1, 8 * PS},
+        {&d.oPoolF2, 8 * PS}, {&d.oPoolCeil, 8 * PS}, {&d.oPoolFloor, 8 * PS}, {&d.oPoolICa, 8 * PS},
+        {&d.oPoolCount, 4 * n},
+        {&d.oLevelStart, 4 * (int64_t)s.levelStart.size()}, {&d.oUnits, 4 * (int64_t)s.units.size()},
+        {&d.oGrpMem, 4 * nMem}, {&d.oGrpInfo, 16 * nGroups}, {&d.oHJ0, 8 * (int64_t)s.HJ0.size()},
+        {&d.oWork, workB},
+        {&d.oSpkCompt, 4 * (int64_t)s.spkCompt.size()}, {&d.oSpkEdge, 4 * (int64_t)s.spkCompt.size()},
+        {&d.oSpkThr, 8 * (int64_t)s.spkCompt.size()}, {&d.oSpkRefr, 8 * (int64_t)s.spkCompt.size()},
+        {&d.oSpkLast, 8 * (int64_t)s.spkCompt.size()}, {&d.oSpkFired, 4 * (int64_t)s.spkCompt.size()},
+        {&d.oSteps, 8},
+    };
+    int64_t total = 0;
+    for (auto& it : items) total += align_up(std::max<int64_t>(it.bytes, 8));
+    int64_t base = 0;
+    int rc = arena_alloc(e, total, &base);
+    if (rc) return rc;
+    hc.region.off = base;
+    hc.region.bytes = total;
+    int64_t cur = base;
+    for (auto& it : items) {
+        *it.off = cur;
+        cur += align_up(std::max<int64_t>(it.bytes, 8));
+    }
+    hc.bytes = total;
+
+    d.n = n;
+    d.nChanSlots = s.chanSlots;
+    d.nGateSlots = s.gateSlots;
+    d.nPoolSlots = s.poolSlots;
+    d.hasModulation = s.hasModulation;
+    d.hasPools = s.poolSlots > 0;
+    d.caAdvance = s.caAdvance;
+    d.nLevels = (int)s.levelStart.size() - 1;
+    d.nUnits = (int)s.units.size();
+    d.nHJ = (int)s.HJ0.size();
+    d.nTerm = s.nTerm;
+    d.nGroups = (int)nGroups;
+    d.nSpike = (int)s.spkCompt.size();
+    d.cellId = hc.id;
+    d.live = 1;
+    // engine type ids
+    int tsid = 0;
+    rc = table_set_id(e, s.tables, &tsid);
+    if (rc) return rc;
+    hc.tableSet = tsid;
+    hc.typeIds.clear();
+    for (auto& t : s.localTypes) hc.typeIds.push_back(type_id(e, t, tsid));
+    std::vector<int32_t> chType(CS, -1);
+    for (int64_t k = 0; k < CS; ++k)
+        chType[k] = s.chTypeLocal[k] >= 0 ? hc.typeIds[s.chTypeLocal[k]] : -1;
+    d.uniform = s.uniform ? 1 : 0;
+    for (int k = 0; k < kMaxSlots; ++k)
+        d.slotType[k] = (k < s.chanSlots && s.uniform) ? chType[(int64_t)k * n] : -1;
+
+    // upload
+    std::vector<float> f;
+    auto up_gt = [&](int64_t off, const std::vector<double>& v) -> int {
+        if (e->fp32) {
+            f.assign(v.begin(), v.end());
+            return upload(e, off, f);
+        }
+        return upload(e, off, v);
+    };
+    std::vector<int32_t> defer(s.defer.begin(), s.defer.end());
+    std::vector<int32_t> levelStart(s.levelStart.begin(), s.levelStart.end());
+    std::vector<int32_t> units(s.units.begin(), s.units.end());
+    std::vector<int32_t> mem, info;
+    for (auto& g : s.groups) {
+        info.push_back((int32_t)mem.size());
+        info.push_back((int32_t)g.mem.size());
+        info.push_back(g.hjOff);
+        info.push_back(g.deferLast);
+        for (int x : g.mem) mem.push_back(x);
+    }
+    std::vector<int32_t> poolCount(s.poolCount.begin(), s.poolCount.end());
+    std::vector<double> zerosCS(CS, 0.0), zerosPS(PS, 0.0);
+    std::vector<int32_t> spkCompt(s.spkCompt.begin(), s.spkCompt.end());
+    std::vector<int32_t> spkEdge(s.spkEdge.begin(), s.spkEdge.end());
+    std::vector<double> spkLast(s.spkCompt.size(), 0.0);
+    std::vector<int32_t> spkFired(s.spkCompt.size(), 0);
+    std::vector<int64_t> steps(1, 0);
+    if ((rc = upload(e, d.oV, s.V)) || (rc = upload(e, d.oCmByDt, s.CmByDt)) ||
+        (rc = upload(e, d.oEmByRm, s.EmByRm)) || (rc = upload(e, d.oDbase, s.Dbase)) ||
+        (rc = upload(e, d.oOff, s.Off)) || (rc = upload(e, d.oDefer, defer)) ||
+        (rc = upload(e, d.oChType, chType)) || (rc = up_gt(d.oGbar, s.Gbar)) || (rc = up_gt(d.oEk, s.Ek)) ||
+        (rc = upload(e, d.oMod, s.mod)) || (rc = upload(e, d.oExtCa, zerosCS)) ||
+        (rc = up_gt(d.oState, s.state)) || (rc = upload(e, d.oPoolC, s.pC)) ||
+        (rc = upload(e, d.oPoolCa, s.pCa)) || (rc = upload(e, d.oPoolBasal, s.pBasal)) ||
+        (rc = upload(e, d.oPoolF1, s.pF1)) || (rc = upload(e, d.oPoolF2, s.pF2)) ||
+        (rc = upload(e, d.oPoolCeil, s.pCeil)) || (rc = upload(e, d.oPoolFloor, s.pFloor)) ||
+        (rc = upload(e, d.oPoolICa, zerosPS)) || (rc = upload(e, d.oPoolCount, poolCount)) ||
+        (rc = upload(e, d.oLevelStart, levelStart)) || (rc = upload(e, d.oUnits, units)) ||
+        (rc = upload(e, d.oGrpMem, mem)) || (rc = upload(e, d.oGrpInfo, info)) ||
+        (rc = upload(e, d.oHJ0, s.HJ0)) || (rc = upload(e, d.oSpkCompt, spkCompt)) ||
+        (rc = upload(e, d.oSpkEdge, spkEdge)) || (rc = upload(e, d.oSpkThr, s.spkThr)) ||
+        (rc = upload(e, d.oSpkRefr, s.spkRefr)) || (rc = upload(e, d.oSpkLast, spkLast)) ||
+        (rc = upload(e, d.oSpkFired, spkFired)) || (rc = upload(e, d.oSteps, steps)))
+        return rc;
+    HIPCHK(hipStreamSynchronize(e->stream));
+    // work arrays in LDS when they fit
+    d.workInLds = work_bytes(d) <= e->ldsLimit ? 1 : 0;
+    return 0;
+}
+
+int write_injections(hsg_engine* e, HostCell& hc) {
+    const int m = (int)hc.injIdx.size();
+    int64_t need = std::max(1, m) * (4 + 8 + 8) + 3 * kAlign;
+    if (hc.inj.off < 0 || hc.inj.bytes < need) {
+        if (hc.inj.off >= 0) e->arena.release(hc.inj.off, hc.inj.bytes);
+        int64_t off;
+        int rc = arena_alloc(e, need * 2, &off);
+        if (rc) return rc;
+        hc.inj.off = off;
+        hc.inj.bytes = need * 2;
+    }
+    hc.dev.oInjIdx = hc.inj.off;
+    hc.dev.oInjBasal = align_up(hc.inj.off + 4 * (int64_t)std::max(1, m));
+    hc.dev.oInjVar = align_up(hc.dev.oInjBasal + 8 * (int64_t)std::max(1, m));
+    std::vector<int32_t> idx(hc.injIdx.begin(), hc.injIdx.end());
+    int rc;
+    if ((rc = upload(e, hc.dev.oInjIdx, idx)) || (rc = upload(e, hc.dev.oInjBasal, hc.injBasal)) ||
+        (rc = upload(e, hc.dev.oInjVar, hc.injVar)))
+        return rc;
+    hc.dev.nInj = m;
+    hc.injDirty = false;
+    e->tablesDirty = true;
+    return 0;
+}
+
+int write_ext(hsg_engine* e, HostCell& hc) {
+    const int m = (int)hc.ext.size();
+    if (m == 0) {
+        if (hc.dev.nExt != 0) e->tablesDirty = true;
+        hc.dev.nExt = 0;
+        return 0;
+    }
+    int64_t need = (int64_t)m * 24 + 3 * kAlign;
+    if (hc.extBlk.off < 0 || hc.extBlk.bytes < need) {
+        if (hc.extBlk.off >= 0) e->arena.release(hc.extBlk.off, hc.extBlk.bytes);
+        int64_t off;
+        int rc = arena_alloc(e, need * 2, &off);
+        if (rc) return rc;
+        hc.extBlk.off = off;
+        hc.extBlk.bytes = need * 2;
+    }
+    hc.dev.oExtIdx = hc.extBlk.off;
+    hc.dev.oExtG = align_up(hc.extBlk.off + 4 * (int64_t)m);
+    hc.dev.oExtGE = align_up(hc.dev.oExtG + 8 * (int64_t)m);
+    std::vector<int32_t> idx;
+    std::vector<double> g, ge;
+    for (auto& kv : hc.ext) {
+        idx.push_back(kv.first);
+        g.push_back(kv.second.first);
+        ge.push_back(kv.second.second);
+    }
+    int rc;
+    if ((rc = upload(e, hc.dev.oExtIdx, idx)) || (rc = upload(e, hc.dev.oExtG, g)) ||
+        (rc = upload(e, hc.dev.oExtGE, ge)))
+        return rc;
+    hc.dev.nExt = m;
+    e->tablesDirty = true;
+    return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// helpers shared by the C ABI
+// ---------------------------------------------------------------------------
+namespace {
+
+template <typename T>
+int download(hsg_engine* e, int64_t off, size_t count, T* out) {
+    if (!count) return 0;
+    HIPCHK(hipMemcpyAsync(out, e->arena.base + off, count * sizeof(T), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+template <typename T>
+int put1(hsg_engine* e, int64_t off, size_t index, T v) {
+    HIPCHK(hipMemcpyAsync(e->arena.base + off + index * sizeof(T), &v, sizeof(T), hipMemcpyHostToDevice,
+                          e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+HostCell* get_cell(hsg_engine* e, int32_t cell) {
+    if (!e || cell < 0 || cell >= (int32_t)e->cells.size() || !e->cells[cell] || !e->cells[cell]->live)
+        return nullptr;
+    return e->cells[cell].get();
+}
+
+// state arrays of a cell as doubles
+int download_gt(hsg_engine* e, int64_t off, size_t count, std::vector<double>& out) {
+    out.assign(count, 0.0);
+    if (e->fp32) {
+        std::vector<float> f(count);
+        int rc = download(e, off, count, f.data());
+        if (rc) return rc;
+        for (size_t i = 0; i < count; ++i) out[i] = f[i];
+        return 0;
+    }
+    return download(e, off, count, out.data());
+}
+
+int put_gt(hsg_engine* e, int64_t off, size_t index, double v) {
+    if (e->fp32) return put1<float>(e, off, index, (float)v);
+    return put1<double>(e, off, index, v);
+}
+
+template <typename GT>
+GT host_power(GT x, double p) {
+    if (p == 1.0) return x;
+    if (p == 2.0) return x * x;
+    if (p == 3.0) return x * x * x;
+    if (p == 4.0) { GT y = x * x; return y * y; }
+    if (x > (GT)0) return (GT)std::exp((double)p * std::log((double)x));
+    return (GT)0;
+}
+
+// Gk of one channel from the current gate states, computed the way the kernel
+// does (ChannelStruct::process, hsolve/HSolveStruct.cpp:53-65).
+template <typename GT>
+double chan_gk(const HostCell& hc, int dk, const std::vector<double>& state) {
+    const CellSetup& s = hc.s;
+    const int n = s.n, i = s.chanCompt[dk], k = s.chanSlot[dk];
+    const size_t o = (size_t)k * n + i;
+    GT fraction = hc.dev.hasModulation ? (GT)s.mod[o] : (GT)1.0;
+    int js = s.chanState[dk];
+    if (s.chanXp[dk] > 0.0) fraction *= host_power<GT>((GT)state[(size_t)(js++) * n + i], s.chanXp[dk]);
+    if (s.chanYp[dk] > 0.0) fraction *= host_power<GT>((GT)state[(size_t)(js++) * n + i], s.chanYp[dk]);
+    if (s.chanZp[dk] > 0.0) fraction *= host_power<GT>((GT)state[(size_t)(js++) * n + i], s.chanZp[dk]);
+    return (double)((GT)s.Gbar[o] * fraction);
+}
+
+int upload_tables(hsg_engine* e) {
+    if (!e->tablesDirty) return 0;
+    const int nc = (int)e->cells.size();
+    std::vector<CellDev> devs(std::max(1, nc));
+    std::memset(devs.data(), 0, sizeof(CellDev) * devs.size());
+    e->order.clear();
+    e->maxLdsUsed = 0;
+    for (int i = 0; i < nc; ++i) {
+        if (!e->cells[i] || !e->cells[i]->live) continue;
+        devs[i] = e->cells[i]->dev;
+        e->order.push_back(i);
+        if (devs[i].workInLds) e->maxLdsUsed = std::max<int>(e->maxLdsUsed, (int)work_bytes(devs[i]));
+    }
+    std::stable_sort(e->order.begin(), e->order.end(),
+                     [&](int a, int b) { return devs[a].n > devs[b].n; });
+    int rc;
+    if ((rc = grow_dev(&e->dCells, &e->dCellsCap, (int)devs.size())) ||
+        (rc = grow_dev(&e->dOrder, &e->dOrderCap, std::max<int>(1, (int)e->order.size()))) ||
+        (rc = grow_dev(&e->dTypes, &e->dTypesCap, std::max<int>(1, (int)e->types.size()))) ||
+        (rc = grow_dev(&e->dTsets, &e->dTsetsCap, std::max<int>(1, (int)e->tsets.size()))))
+        return rc;
+    HIPCHK(hipMemcpyAsync(e->dCells, devs.data(), sizeof(CellDev) * devs.size(), hipMemcpyHostToDevice, e->stream));
+    if (!e->order.empty())
+        HIPCHK(hipMemcpyAsync(e->dOrder, e->order.data(), sizeof(int32_t) * e->order.size(),
+                              hipMemcpyHostToDevice, e->stream));
+    if (!e->types.empty())
+        HIPCHK(hipMemcpyAsync(e->dTypes, e->types.data(), sizeof(ChanType) * e->types.size(),
+                              hipMemcpyHostToDevice, e->stream));
+    if (!e->tsets.empty())
+        HIPCHK(hipMemcpyAsync(e->dTsets, e->tsets.data(), sizeof(TableSetDev) * e->tsets.size(),
+                              hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->tablesDirty = false;
+    e->graphValid = false;
+    return 0;
+}
+
+LaunchArgs make_args(hsg_engine* e, const int32_t* order, int nsteps, int first) {
+    LaunchArgs a{};
+    a.arena = e->arena.base;
+    a.cells = e->dCells;
+    a.order = order;
+    a.types = e->dTypes;
+    a.tsets = e->dTsets;
+    a.spikes = e->dSpikes;
+    a.spikeCount = e->dSpikeCount;
+    a.spikeCap = kSpikeCap;
+    a.nsteps = nsteps;
+    a.firstOfCall = first;
+    a.ldsBytes = e->maxLdsUsed;
+    a.dt = e->dt;
+    return a;
+}
+
+int block_threads(hsg_engine* e) {
+    int t = e->opt.block_threads > 0 ? e->opt.block_threads : 256;
+    t = std::max(64, std::min(1024, (t + 63) / 64 * 64));
+    return t;
+}
+
+// flushes one-shot inputs of every live cell to the device
+int flush_inputs(hsg_engine* e) {
+    for (auto& up : e->cells) {
+        if (!up || !up->live) continue;
+        HostCell& hc = *up;
+        int rc;
+        if (hc.injDirty && (rc = write_injections(e, hc))) return rc;
+        if ((rc = write_ext(e, hc))) return rc;
+        if (!hc.ica.empty()) {
+            for (auto& kv : hc.ica) {
+                int p = kv.first;
+                size_t o = (size_t)hc.s.poolSlot[p] * hc.s.n + hc.s.poolCompt[p];
+                if ((rc = put1<double>(e, hc.dev.oPoolICa, o, kv.second))) return rc;
+            }
+        }
+    }
+    return 0;
+}
+
+// bookkeeping after a step call: one-shot inputs are consumed
+void consume_inputs(hsg_engine* e, HostCell& hc, int64_t nsteps) {
+    const int n = hc.s.n;
+    hc.prevExtG.assign(n, 0.0);
+    hc.prevExtGE.assign(n, 0.0);
+    if (nsteps == 1)
+        for (auto& kv : hc.ext) {
+            hc.prevExtG[kv.first] = kv.second.first;
+            hc.prevExtGE[kv.first] = kv.second.second;
+        }
+    if (!hc.ext.empty()) {
+        hc.ext.clear();
+        hc.dev.nExt = 0;
+        e->tablesDirty = true;
+    }
+    hc.ica.clear();
+    hc.gkOverride.clear();
+    for (auto& v : hc.injVar) v = 0.0;
+    hc.stepsDone += nsteps;
+}
+
+int run_launches(hsg_engine* e, const std::vector<int32_t>& order, const std::vector<int64_t>& steps) {
+    // order: cells to run; steps[i]: steps for order[i]
+    const int K = e->opt.steps_per_launch > 0 ? e->opt.steps_per_launch : 50;
+    const int threads = block_threads(e);
+    std::vector<int64_t> left(steps);
+    std::vector<int32_t> cur;
+    int32_t* dSub = nullptr;
+    bool subset = false;
+    // whole live set with equal steps -> use the persistent order array
+    if (order.size() == e->order.size()) {
+        bool same = true;
+        for (size_t i = 0; i < order.size() && same; ++i) same = order[i] == e->order[i] && steps[i] == steps[0];
+        subset = !same;
+    } else subset = true;
+    if (subset) HIPCHK(hipMalloc(&dSub, sizeof(int32_t) * std::max<size_t>(1, order.size())));
+    e->lastLaunches = 0;
+    e->lastTotalMs = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
+    auto event_pair = [&](size_t k) -> std::pair<hipEvent_t, hipEvent_t> {
+        while (e->evStart.size() <= k) {
+            hipEvent_t a, b;
+            hipEventCreate(&a);
+            hipEventCreate(&b);
+            e->evStart.push_back(a);
+            e->evStop.push_back(b);
+        }
+        return {e->evStart[k], e->evStop[k]};
+    };
+    bool first = true;
+    int rc = 0;
+    while (true) {
+        cur.clear();
+        int64_t chunk = K;
+        for (size_t i = 0; i < order.size(); ++i)
+            if (left[i] > 0) { cur.push_back(order[i]); chunk = std::min(chunk, left[i]); }
+        if (cur.empty()) break;
+        const int32_t* dOrd = e->dOrder;
+        if (subset) {
+            hipError_t he = hipMemcpyAsync(dSub, cur.data(), sizeof(int32_t) * cur.size(), hipMemcpyHostToDevice, e->stream);
+            if (he != hipSuccess) { rc = fail(HSG_E_DEVICE, hipGetErrorString(he)); break; }
+            dOrd = dSub;
+        }
+        // steady state of the whole set: replay a captured graph when asked
+        const int G = e->opt.graph_chunk;
+        bool useGraph = !subset && !first && G > 0 && chunk == K;
+        if (useGraph) {
+            int64_t minLeft = INT64_MAX;
+            for (size_t i = 0; i < order.size(); ++i) minLeft = std::min(minLeft, left[i]);
+            useGraph = minLeft >= (int64_t)G * K;
+        }
+        auto ev = event_pair(evs.size());
+        hipEventRecord(ev.first, e->stream);
+        if (useGraph) {
+            if (!e->graphValid || e->graphSteps != K) {
+                if (e->graphExec) { hipGraphExecDestroy(e->graphExec); e->graphExec = nullptr; }
+                hipGraph_t g;
+                hipError_t he = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
+                if (he != hipSuccess) { rc = fail(HSG_E_DEVICE, hipGetErrorString(he)); break; }
+                LaunchArgs a = make_args(e, e->dOrder, K, 0);
+                for (int r = 0; r < G; ++r) hsg_launch_step(&a, (int)cur.size(), threads, e->fp32, e->stream);
+                he = hipStreamEndCapture(e->stream, &g);
+                if (he != hipSuccess) { rc = fail(HSG_E_DEVICE, hipGetErrorString(he)); break; }
+                he = hipGraphInstantiate(&e->graphExec, g, nullptr, nullptr, 0);
+                hipGraphDestroy(g);
+                if (he != hipSuccess) { rc = fail(HSG_E_DEVICE, hipGetErrorString(he)); break; }
+                e->graphValid = true;
+                e->graphSteps = K;
+            }
+            hipError_t he = hipGraphLaunch(e->graphExec, e->stream);
+            if (he != hipSuccess) { rc = fail(HSG_E_DEVICE, hipGetErrorString(he)); break; }
+            for (size_t i = 0; i < order.size(); ++i) left[i] -= (int64_t)G * K;
+            e->lastLaunches += G;
+        } else {
+            LaunchArgs a = make_args(e, dOrd, (int)chunk, first ? 1 : 0);
+            hipError_t he = hsg_launch_step(&a, (int)cur.size(), threads, e->fp32, e->stream);
+            if (he != hipSuccess) { rc = fail(HSG_E_DEVICE, std::string("step launch: ") + hipGetErrorString(he)); break; }
+            for (size_t i = 0; i < order.size(); ++i) if (left[i] > 0) left[i] -= chunk;
+            e->lastLaunches += 1;
+        }
+        hipEventRecord(ev.second, e->stream);
+        evs.push_back(ev);
+        first = false;
+    }
+    hipError_t he = hipStreamSynchronize(e->stream);
+    if (he != hipSuccess && !rc) rc = fail(HSG_E_DEVICE, std::string("step: ") + hipGetErrorString(he));
+    for (auto& ev : evs) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, ev.first, ev.second) == hipSuccess) e->lastTotalMs += ms;
+    }
+    if (dSub) hipFree(dSub);
+    return rc;
+}
+
+int reinit_cells_host(hsg_engine* e, HostCell& hc) {
+    CellSetup& s = hc.s;
+    const int n = s.n;
+    int rc;
+    if ((rc = upload(e, hc.dev.oV, s.initVm))) return rc;
+    // reinitCalcium: c = 0, ca = CaBasal, activation = 0
+    const size_t PS = (size_t)s.poolSlots * n;
+    std::vector<double> zeros(PS, 0.0);
+    if ((rc = upload(e, hc.dev.oPoolC, zeros)) || (rc = upload(e, hc.dev.oPoolCa, s.pBasal)) ||
+        (rc = upload(e, hc.dev.oPoolICa, zeros)))
+        return rc;
+    for (size_t o = 0; o < PS; ++o) s.pC[o] = 0.0;
+    // reinitSpikeGens: lastEvent = -refractT
+    std::vector<double> last(s.spkRefr.size());
+    for (size_t k = 0; k < last.size(); ++k) last[k] = -s.spkRefr[k];
+    if ((rc = upload(e, hc.dev.oSpkLast, last))) return rc;
+    std::vector<int64_t> zero1(1, 0);
+    if ((rc = upload(e, hc.dev.oSteps, zero1))) return rc;
+    if (hc.dev.nProbe > 0 && (rc = upload(e, hc.dev.oProbeCount, zero1))) return rc;
+    hc.ext.clear();
+    hc.ica.clear();
+    if (hc.dev.nExt) { hc.dev.nExt = 0; e->tablesDirty = true; }
+    hc.hsolveDt = e->dt;
+    hc.stepsDone = 0;
+    return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* hsg_last_error(void) { return g_err.c_str(); }
+
+int hsg_create(const hsg_options* opts, hsg_engine** out) {
+    if (!out) return fail(HSG_E_ARG, "null output pointer");
+    *out = nullptr;
+    std::unique_ptr<hsg_engine> e(new hsg_engine());
+    if (opts) e->opt = *opts;
+    if (e->opt.precision != HSG_FP64 && e->opt.precision != HSG_MIXED)
+        return fail(HSG_E_ARG, "precision must be HSG_FP64 or HSG_MIXED");
+    e->fp32 = e->opt.precision == HSG_MIXED;
+    e->device = e->opt.device;
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (e->device < 0 || e->device >= ndev) return fail(HSG_E_ARG, "no such HIP device");
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    int lds = 0;
+    HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, e->device));
+    e->ldsLimit = std::min(lds, 65536);
+    int64_t cap = e->opt.mem_budget > 0 ? e->opt.mem_budget : (int64_t)64 << 20;
+    HIPCHK(hipMalloc(&e->arena.base, cap));
+    e->arena.cap = cap;
+    e->arena.freeList[0] = cap;
+    HIPCHK(hipMalloc(&e->dSpikes, sizeof(SpikeRec) * kSpikeCap));
+    HIPCHK(hipMalloc(&e->dSpikeCount, sizeof(unsigned long long)));
+    HIPCHK(hipMemsetAsync(e->dSpikeCount, 0, sizeof(unsigned long long), e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    *out = e.release();
+    return 0;
+}
+
+int hsg_destroy(hsg_engine* e) {
+    if (!e) return 0;
+    hipSetDevice(e->device);
+    if (e->stream) hipStreamSynchronize(e->stream);
+    if (e->graphExec) hipGraphExecDestroy(e->graphExec);
+    for (auto ev : e->evStart) hipEventDestroy(ev);
+    for (auto ev : e->evStop) hipEventDestroy(ev);
+    hipFree(e->arena.base);
+    hipFree(e->dCells);
+    hipFree(e->dOrder);
+    hipFree(e->dTypes);
+    hipFree(e->dTsets);
+    hipFree(e->dSpikes);
+    hipFree(e->dSpikeCount);
+    if (e->stream) hipStreamDestroy(e->stream);
+    delete e;
+    return 0;
+}
+
+int hsg_add_cell(hsg_engine* e, const hsg_cell_desc* desc, double dt, int32_t caAdvance, int32_t* cell_id) {
+    if (!e || !desc || !cell_id) return fail(HSG_E_ARG, "null argument");
+    hipSetDevice(e->device);
+    std::unique_ptr<HostCell> hc(new HostCell());
+    std::string err;
+    int rc = setup_cell(desc, dt, caAdvance, hc->s, err);
+    if (rc) return fail(rc, err);
+    // find a free id
+    int id = -1;
+    for (int i = 0; i < (int)e->cells.size(); ++i)
+        if (!e->cells[i]) { id = i; break; }
+    if (id < 0) { id = (int)e->cells.size(); e->cells.emplace_back(); }
+    hc->id = id;
+    hc->hsolveDt = dt;
+    rc = place_cell(e, *hc);
+    if (rc) {
+        if (hc->region.off >= 0) e->arena.release(hc->region.off, hc->region.bytes);
+        return rc;
+    }
+    for (int i = 0; i < hc->s.n; ++i)
+        if (hc->s.hasInject[i]) {
+            hc->injIdx.push_back(i);
+            hc->injBasal.push_back(hc->s.injBasal[i]);
+            hc->injVar.push_back(0.0);
+        }
+    rc = write_injections(e, *hc);
+    if (rc) {
+        e->arena.release(hc->region.off, hc->region.bytes);
+        return rc;
+    }
+    hc->prevExtG.assign(hc->s.n, 0.0);
+    hc->prevExtGE.assign(hc->s.n, 0.0);
+    hc->live = true;
+    e->cells[id] = std::move(hc);
+    e->tablesDirty = true;
+    *cell_id = id;
+    return 0;
+}
+
+int hsg_remove_cell(hsg_engine* e, int32_t cell) {
+    HostCell* hc = get_cell(e, cell);
+    if (!hc) return fail(HSG_E_ARG, "no such cell");
+    hipSetDevice(e->device);
+    hipStreamSynchronize(e->stream);
+    e->arena.release(hc->region.off, hc->region.bytes);
+    if (hc->inj.off >= 0) e->arena.release(hc->inj.off, hc->inj.bytes);
+    if (hc->probe.off >= 0) e->arena.release(hc->probe.off, hc->probe.bytes);
+    if (hc->extBlk.off >= 0) e->arena.release(hc->extBlk.off, hc->extBlk.bytes);
+    e->cells[cell].reset();
+    e->tablesDirty = true;
+    return 0;
+}
+
+static int reinit_some(hsg_engine* e, const std::vector<int32_t>& ids) {
+    int rc;
+    for (int id : ids)
+        if ((rc = reinit_cells_host(e, *e->cells[id]))) return rc;
+    if ((rc = upload_tables(e))) return rc;
+    if (ids.empty()) return 0;
+    int32_t* dSub = nullptr;
+    HIPCHK(hipMalloc(&dSub, sizeof(int32_t) * ids.size()));
+    HIPCHK(hipMemcpyAsync(dSub, ids.data(), sizeof(int32_t) * ids.size(), hipMemcpyHostToDevice, e->stream));
+    LaunchArgs a = make_args(e, dSub, 0, 0);
+    hipError_t he = hsg_launch_reinit(&a, (int)ids.size(), e->fp32, e->stream);
+    hipError_t hs = hipStreamSynchronize(e->stream);
+    hipFree(dSub);
+    if (he != hipSuccess) return fail(HSG_E_DEVICE, std::string("reinit launch: ") + hipGetErrorString(he));
+    if (hs != hipSuccess) return fail(HSG_E_DEVICE, std::string("reinit: ") + hipGetErrorString(hs));
+    return 0;
+}
+
+int hsg_reinit(hsg_engine* e, double dt) {
+    if (!e) return fail(HSG_E_ARG, "null engine");
+    if (!(dt > 0.0)) return fail(HSG_E_ARG, "reinit: dt must be positive");
+    hipSetDevice(e->device);
+    e->dt = dt;
+    e->reinitDone = true;
+    std::vector<int32_t> ids;
+    for (auto& c : e->cells) if (c && c->live) ids.push_back(c->id);
+    HIPCHK(hipMemsetAsync(e->dSpikeCount, 0, sizeof(unsigned long long), e->stream));
+    return reinit_some(e, ids);
+}
+
+int hsg_reinit_cell(hsg_engine* e, int32_t cell) {
+    HostCell* hc = get_cell(e, cell);
+    if (!hc) return fail(HSG_E_ARG, "no such cell");
+    if (!(e->dt > 0.0)) return fail(HSG_E_STATE, "call hsg_reinit once before reinitialising single cells");
+    hipSetDevice(e->device);
+    return reinit_some(e, {cell});
+}
+
+int hsg_step(hsg_engine* e, int64_t nsteps) {
+    if (!e) return fail(HSG_E_ARG, "null engine");
+    if (nsteps < 0) return fail(HSG_E_ARG, "negative step count");
+    if (!(e->dt > 0.0)) return fail(HSG_E_STATE, "call hsg_reinit before stepping");
+    if (nsteps == 0) return 0;
+    hipSetDevice(e->device);
+    int rc;
+    if ((rc = flush_inputs(e)) || (rc = upload_tables(e))) return rc;
+    std::vector<int64_t> steps(e->order.size(), nsteps);
+    if (!e->order.empty() && (rc = run_launches(e, e->order, steps))) return rc;
+    for (auto& c : e->cells) if (c && c->live) consume_inputs(e, *c, nsteps);
+    return 0;
+}
+
+int hsg_step_cells(hsg_engine* e, const int32_t* cells, const int64_t* nsteps, int32_t n) {
+    if (!e || (n > 0 && (!cells || !nsteps))) return fail(HSG_E_ARG, "null argument");
+    if (!(e->dt > 0.0)) return fail(HSG_E_STATE, "call hsg_reinit before stepping");
+    hipSetDevice(e->device);
+    std::vector<int32_t> ord;
+    std::vector<int64_t> st;
+    for (int i = 0; i < n; ++i) {
+        if (!get_cell(e, cells[i])) return fail(HSG_E_ARG, "no such cell");
+        if (nsteps[i] < 0) return fail(HSG_E_ARG, "negative step count");
+        if (nsteps[i] > 0) { ord.push_back(cells[i]); st.push_back(nsteps[i]); }
+    }
+    int rc;
+    if ((rc = flush_inputs(e)) || (rc = upload_tables(e))) return rc;
+    if (!ord.empty() && (rc = run_launches(e, ord, st))) return rc;
+    for (size_t i = 0; i < ord.size(); ++i) consume_inputs(e, *e->cells[ord[i]], st[i]);
+    return 0;
+}
+
+int hsg_get_field(hsg_engine* e, int32_t cell, int32_t field, const int32_t* idx, int32_t n, double* out) {
+    HostCell* hc = get_cell(e, cell);
+    if (!hc) return fail(HSG_E_ARG, "no such cell");
+    if (n < 0 || (n > 0 && (!idx || !out))) return fail(HSG_E_ARG, "bad index list");
+    if (n == 0) return 0;
+    hipSetDevice(e->device);
+    CellSetup& s = hc->s;
+    const int nc = s.n;
+    std::vector<double> V, state, pool;
+    int rc;
+    auto needV = [&]() -> int { if (V.empty()) { V.resize(nc); return download(e, hc->dev.oV, nc, V.data()); } return 0; };
+    auto needState = [&]() -> int {
+        if (state.empty() && s.gateSlots > 0) return download_gt(e, hc->dev.oState, (size_t)s.gateSlots * nc, state);
+        return 0;
+    };
+    for (int q = 0; q < n; ++q) {
+        const int i = idx[q];
+        if (field >= 1 && field <= 8) {
+            if (i < 0 || i >= (int)s.descToHines.size() || s.descToHines[i] < 0)
+                return fail(HSG_E_ARG, "unknown compartment index");
+            const int h = s.descToHines[i];
+            switch (field) {
+            case HSG_F_VM: if ((rc = needV())) return rc; out[q] = V[h]; break;
+            case HSG_F_IM: {
+                // HSolve::getIm (hsolve/HSolveInterface.cpp:161-182)
+                if ((rc = needV()) || (rc = needState())) return rc;
+                double Im = s.EmByRm[h] - V[h] / s.Rm[h];
+                for (int dk = 0; dk < (int)s.chanCompt.size(); ++dk) {
+                    if (s.chanCompt[dk] != h) continue;
+                    auto ov = hc->gkOverride.find(dk);
+                    double gk = ov != hc->gkOverride.end() ? ov->second
+                              : e->fp32 ? chan_gk<float>(*hc, dk, state) : chan_gk<double>(*hc, dk, state);
+                    double ek = e->fp32 ? (double)(float)s.Ek[(size_t)s.chanSlot[dk] * nc + h]
+                                        : s.Ek[(size_t)s.chanSlot[dk] * nc + h];
+                    Im += (ek - V[h]) * gk;
+                }
+                Im += hc->prevExtGE[h] - hc->prevExtG[h] * V[h];
+                out[q] = Im;
+                break;
+            }
+            case HSG_F_INJECT: {
+                double v = 0.0;
+                for (size_t k = 0; k < hc->injIdx.size(); ++k) if (hc->injIdx[k] == h) v = hc->injBasal[k];
+                out[q] = v;
+                break;
+            }
+            case HSG_F_INITVM: out[q] = s.initVm[h]; break;
+            case HSG_F_CM: out[q] = s.Cm[h]; break;
+            case HSG_F_EM: out[q] = s.Em[h]; break;
+            case HSG_F_RM: out[q] = s.Rm[h]; break;
+            case HSG_F_RA: out[q] = s.Ra[h]; break;
+            }
+        } else if (field >= 20 && field <= 28) {
+            if (i < 0 || i >= (int)s.chanCompt.size() || s.chanCompt[i] < 0)
+                return fail(HSG_E_ARG, "unknown channel index");
+            const int h = s.chanCompt[i];
+            const size_t o = (size_t)s.chanSlot[i] * nc + h;
+            auto gt = [&](double v) { return e->fp32 ? (double)(float)v : v; };
+            auto stateAt = [&](int which) -> double {
+                int js = s.chanState[i];
+                if (which >= 1 && s.chanXp[i] > 0.0) ++js;
+                if (which >= 2 && s.chanYp[i] > 0.0) ++js;
+                return state[(size_t)js * nc + h];
+            };
+            switch (field) {
+            case HSG_F_GBAR: out[q] = gt(s.Gbar[o]); break;
+            case HSG_F_EK: out[q] = gt(s.Ek[o]); break;
+            case HSG_F_GK:
+            case HSG_F_IK: {
+                if ((rc = needState())) return rc;
+                auto ov = hc->gkOverride.find(i);
+                double gk = ov != hc->gkOverride.end() ? ov->second
+                          : e->fp32 ? chan_gk<float>(*hc, i, state) : chan_gk<double>(*hc, i, state);
+                if (field == HSG_F_GK) out[q] = gk;
+                else { if ((rc = needV())) return rc; out[q] = (gt(s.Ek[o]) - V[h]) * gk; }
+                break;
+            }
+            case HSG_F_X: if ((rc = needState())) return rc; out[q] = s.chanXp[i] == 0.0 ? 0.0 : stateAt(0); break;
+            case HSG_F_Y: if ((rc = needState())) return rc; out[q] = s.chanYp[i] == 0.0 ? 0.0 : stateAt(1); break;
+            case HSG_F_Z: if ((rc = needState())) return rc; out[q] = s.chanZp[i] == 0.0 ? 0.0 : stateAt(2); break;
+            case HSG_F_MODULATION: out[q] = s.mod[o]; break;
+            case HSG_F_INSTANT: {
+                int lt = s.chTypeLocal[o];
+                out[q] = lt >= 0 ? (double)s.localTypes[lt].instant : 0.0;
+                break;
+            }
+            default: return fail(HSG_E_ARG, "unknown field");
+            }
+        } else if (field >= 40 && field <= 43) {
+            if (i < 0 || i >= (int)s.poolCompt.size() || s.poolCompt[i] < 0)
+                return fail(HSG_E_ARG, "unknown pool index");
+            const size_t o = (size_t)s.poolSlot[i] * nc + s.poolCompt[i];
+            switch (field) {
+            case HSG_F_CA:
+                if (pool.empty()) {
+                    pool.resize((size_t)s.poolSlots * nc);
+                    if ((rc = download(e, hc->dev.oPoolCa, pool.size(), pool.data()))) return rc;
+                }
+                out[q] = pool[o];
+                break;
+            case HSG_F_CABASAL: out[q] = s.pBasal[o]; break;
+            case HSG_F_CACEIL: out[q] = s.pCeil[o]; break;
+            case HSG_F_CAFLOOR: out[q] = s.pFloor[o]; break;
+            }
+        } else {
+            return fail(HSG_E_ARG, "unknown field");
+        }
+    }
+    return 0;
+}
+
+int hsg_set_field(hsg_engine* e, int32_t cell, int32_t field, const int32_t* idx, int32_t n, const double* vals) {
+    HostCell* hc = get_cell(e, cell);
+    if (!hc) return fail(HSG_E_ARG, "no such cell");
+    if (n < 0 || (n > 0 && (!idx || !vals))) return fail(HSG_E_ARG, "bad index list");
+    hipSetDevice(e->device);
+    CellSetup& s = hc->s;
+    const int nc = s.n;
+    int rc;
+    for (int q = 0; q < n; ++q) {
+        const int i = idx[q];
+        const double x = vals[q];
+        if (field >= 1 && field <= 8) {
+            if (i < 0 || i >= (int)s.descToHines.size() || s.descToHines[i] < 0)
+                return fail(HSG_E_ARG, "unknown compartment index");
+            const int h = s.descToHines[i];
+            switch (field) {
+            case HSG_F_VM: if ((rc = put1<double>(e, hc->dev.oV, h, x))) return rc; break;
+            case HSG_F_IM: return fail(HSG_E_ARG, "Im is read-only");
+            case HSG_F_INJECT: {
+                bool found = false;
+                for (size_t k = 0; k < hc->injIdx.size(); ++k)
+                    if (hc->injIdx[k] == h) { hc->injBasal[k] = x; found = true; }
+                if (!found) { hc->injIdx.push_back(h); hc->injBasal.push_back(x); hc->injVar.push_back(0.0); }
+                hc->injDirty = true;
+                break;
+            }
+            case HSG_F_INITVM: s.initVm[h] = x; break;
+            case HSG_F_CM:
+                s.Cm[h] = x;
+                s.CmByDt[h] = 2.0 * x / hc->hsolveDt;
+                if ((rc = put1<double>(e, hc->dev.oCmByDt, h, s.CmByDt[h]))) return rc;
+                break;
+            case HSG_F_EM:
+                s.Em[h] = x;
+                s.EmByRm[h] = x / s.Rm[h];
+                if ((rc = put1<double>(e, hc->dev.oEmByRm, h, s.EmByRm[h]))) return rc;
+                break;
+            case HSG_F_RM:
+                s.Rm[h] = x;
+                s.EmByRm[h] = s.Em[h] / x;
+                if ((rc = put1<double>(e, hc->dev.oEmByRm, h, s.EmByRm[h]))) return rc;
+                break;
+            case HSG_F_RA: s.Ra[h] = x; break;   // no matrix rebuild, as the reference
+            }
+        } else if (field >= 20 && field <= 28) {
+            if (i < 0 || i >= (int)s.chanCompt.size() || s.chanCompt[i] < 0)
+                return fail(HSG_E_ARG, "unknown channel index");
+            const int h = s.chanCompt[i];
+            const size_t o = (size_t)s.chanSlot[i] * nc + h;
+            auto stateIdx = [&](int which) -> size_t {
+                int js = s.chanState[i];
+                if (which >= 1 && s.chanXp[i] > 0.0) ++js;
+                if (which >= 2 && s.chanYp[i] > 0.0) ++js;
+                return (size_t)js * nc + h;
+            };
+            if ((field == HSG_F_GBAR || field == HSG_F_X || field == HSG_F_Y || field == HSG_F_Z ||
+                 field == HSG_F_MODULATION) && !hc->gkOverride.count(i)) {
+                std::vector<double> st;
+                if (s.gateSlots > 0 && (rc = download_gt(e, hc->dev.oState, (size_t)s.gateSlots * nc, st))) return rc;
+                hc->gkOverride[i] = e->fp32 ? chan_gk<float>(*hc, i, st) : chan_gk<double>(*hc, i, st);
+            }
+            switch (field) {
+            case HSG_F_GBAR: s.Gbar[o] = x; if ((rc = put_gt(e, hc->dev.oGbar, o, x))) return rc; break;
+            case HSG_F_EK: s.Ek[o] = x; if ((rc = put_gt(e, hc->dev.oEk, o, x))) return rc; break;
+            case HSG_F_GK: case HSG_F_IK: return fail(HSG_E_UNSUPPORTED, "Gk/Ik are recomputed every step");
+            case HSG_F_X: if (s.chanXp[i] != 0.0 && (rc = put_gt(e, hc->dev.oState, stateIdx(0), x))) return rc; break;
+            case HSG_F_Y: if (s.chanYp[i] != 0.0 && (rc = put_gt(e, hc->dev.oState, stateIdx(1), x))) return rc; break;
+            case HSG_F_Z: if (s.chanZp[i] != 0.0 && (rc = put_gt(e, hc->dev.oState, stateIdx(2), x))) return rc; break;
+            case HSG_F_MODULATION:
+                if (x > 0.0) {   // HSolve::setHHmodulation ignores non-positive values
+                    s.mod[o] = x;
+                    if ((rc = put1<double>(e, hc->dev.oMod, o, x))) return rc;
+                    if (!hc->dev.hasModulation && x != 1.0) { hc->dev.hasModulation = 1; e->tablesDirty = true; }
+                }
+                break;
+            case HSG_F_INSTANT: {
+                int lt = s.chTypeLocal[o];
+                HostChanType t = s.localTypes[lt];
+                t.instant = (int)x;
+                int nl = -1;
+                for (size_t k = 0; k < s.localTypes.size(); ++k) {
+                    const HostChanType& u = s.localTypes[k];
+                    if (u.Xpower == t.Xpower && u.Ypower == t.Ypower && u.Zpower == t.Zpower && u.instant == t.instant &&
+                        u.colX == t.colX && u.colY == t.colY && u.colZ == t.colZ && u.caDepLocal == t.caDepLocal &&
+                        u.caTgtLocal == t.caTgtLocal) nl = (int)k;
+                }
+                if (nl < 0) {
+                    nl = (int)s.localTypes.size();
+                    s.localTypes.push_back(t);
+                    hc->typeIds.push_back(type_id(e, t, hc->tableSet));
+                }
+                s.chTypeLocal[o] = nl;
+                if ((rc = put1<int32_t>(e, hc->dev.oChType, o, hc->typeIds[nl]))) return rc;
+                if (hc->dev.uniform) { hc->dev.uniform = 0; }
+                e->tablesDirty = true;
+                break;
+            }
+            default: return fail(HSG_E_ARG, "unknown field");
+            }
+        } else if (field >= 40 && field <= 43) {
+            if (i < 0 || i >= (int)s.poolCompt.size() || s.poolCompt[i] < 0)
+                return fail(HSG_E_ARG, "unknown pool index");
+            const size_t o = (size_t)s.poolSlot[i] * nc + s.poolCompt[i];
+            switch (field) {
+            case HSG_F_CA:
+            case HSG_F_CABASAL:   // ZombieCaConc::vSetCaBasal calls setCa (hsolve/ZombieCaConc.cpp:62-65)
+                if ((rc = put1<double>(e, hc->dev.oPoolCa, o, x))) return rc;
+                s.pC[o] = x - s.pBasal[o];
+                if ((rc = put1<double>(e, hc->dev.oPoolC, o, s.pC[o]))) return rc;
+                break;
+            case HSG_F_CACEIL: s.pCeil[o] = x; if ((rc = put1<double>(e, hc->dev.oPoolCeil, o, x))) return rc; break;
+            case HSG_F_CAFLOOR: s.pFloor[o] = x; if ((rc = put1<double>(e, hc->dev.oPoolFloor, o, x))) return rc; break;
+            }
+        } else {
+            return fail(HSG_E_ARG, "unknown field");
+        }
+    }
+    return 0;
+}
+
+int hsg_add_inject(hsg_engine* e, int32_t cell, int32_t compartment, double value) {
+    HostCell* hc = get_cell(e, cell);
+    if (!hc) return fail(HSG_E_ARG, "no such cell");
+    const CellSetup& s = hc->s;
+    if (compartment < 0 || compartment >= (int)s.descToHines.size() || s.descToHines[compartment] < 0)
+        return fail(HSG_E_ARG, "unknown compartment index");
+    const int h = s.descToHines[compartment];
+    for (size_t k = 0; k < hc->injIdx.size(); ++k)
+        if (hc->injIdx[k] == h) { hc->injVar[k] += value; hc->injDirty = true; return 0; }
+    hc->injIdx.push_back(h);
+    hc->injBasal.push_back(0.0);
+    hc->injVar.push_back(0.0 + value);
+    hc->injDirty = true;
+    return 0;
+}
+
+int hsg_add_gkek(hsg_engine* e, int32_t cell, int32_t compartment, double Gk, double Ek) {
+    HostCell* hc = get_cell(e, cell);
+    if (!hc) return fail(HSG_E_ARG, "no such cell");
+    const CellSetup& s = hc->s;
+    if (compartment < 0 || compartment >= (int)s.descToHines.size() || s.descToHines[compartment] < 0)
+        return fail(HSG_E_ARG, "unknown compartment index");
+    auto& v = hc->ext[s.descToHines[compartment]];
+    v.first += Gk;
+    v.second += Gk * Ek;
+    return 0;
+}
+
+int hsg_add_conc(hsg_engine* e, int32_t cell, int32_t channel, double conc) {
+    HostCell* hc = get_cell(e, cell);
+    if (!hc) return fail(HSG_E_ARG, "no such cell");
+    const CellSetup& s = hc->s;
+    if (channel < 0 || channel >= (int)s.chanCompt.size() || s.chanCompt[channel] < 0)
+        return fail(HSG_E_ARG, "unknown channel index");
+    hipSetDevice(e->device);
+    return put1<double>(e, hc->dev.oExtCa, (size_t)s.chanSlot[channel] * s.n + s.chanCompt[channel], conc);
+}
+
+int hsg_add_ica(hsg_engine* e, int32_t cell, int32_t pool, double iCa) {
+    HostCell* hc = get_cell(e, cell);
+    if (!hc) return fail(HSG_E_ARG, "no such cell");
+    const CellSetup& s = hc->s;
+    if (pool < 0 || pool >= (int)s.poolCompt.size() || s.poolCompt[pool] < 0)
+        return fail(HSG_E_ARG, "unknown pool index");
+    hc->ica[pool] += iCa;
+    return 0;
+}
+
+int hsg_read_spikes(hsg_engine* e, hsg_spike* out, int32_t cap, int32_t* n_out, int64_t* dropped) {
+    if (!e || !n_out || (cap > 0 && !out)) return fail(HSG_E_ARG, "null argument");
+    hipSetDevice(e->device);
+    unsigned long long cnt = 0;
+    HIPCHK(hipMemcpyAsync(&cnt, e->dSpikeCount, sizeof cnt, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    const unsigned long long have = std::min<unsigned long long>(cnt, kSpikeCap);
+    std::vector<SpikeRec> recs(have);
+    if (have)
+        HIPCHK(hipMemcpyAsync(recs.data(), e->dSpikes, sizeof(SpikeRec) * have, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemsetAsync(e->dSpikeCount, 0, sizeof(unsigned long long), e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->spikesDropped += (int64_t)(cnt - have);
+    std::sort(recs.begin(), recs.end(), [](const SpikeRec& a, const SpikeRec& b) {
+        if (a.step != b.step) return a.step < b.step;
+        if (a.cell != b.cell) return a.cell < b.cell;
+        return a.gen < b.gen;
+    });
+    int32_t w = 0;
+    for (const SpikeRec& r : recs) {
+        if (w >= cap) { e->spikesDropped++; continue; }
+        HostCell* hc = get_cell(e, r.cell);
+        out[w].cell = r.cell;
+        out[w].spikegen = hc && r.gen < (int)hc->s.spkDesc.size() ? hc->s.spkDesc[r.gen] : r.gen;
+        out[w].step = r.step;
+        ++w;
+    }
+    *n_out = w;
+    if (dropped) { *dropped = e->spikesDropped; e->spikesDropped = 0; }
+    return 0;
+}
+
+int hsg_set_probes(hsg_engine* e, int32_t cell, const int32_t* compartments, int32_t n, int64_t capacity) {
+    HostCell* hc = get_cell(e, cell);
+    if (!hc) return fail(HSG_E_ARG, "no such cell");
+    if (n < 0 || capacity < 0 || (n > 0 && !compartments)) return fail(HSG_E_ARG, "bad probe list");
+    hipSetDevice(e->device);
+    const CellSetup& s = hc->s;
+    std::vector<int32_t> h(n);
+    for (int k = 0; k < n; ++k) {
+        int c = compartments[k];
+        if (c < 0 || c >= (int)s.descToHines.size() || s.descToHines[c] < 0)
+            return fail(HSG_E_ARG, "unknown compartment index");
+        h[k] = s.descToHines[c];
+    }
+    if (hc->probe.off >= 0) { e->arena.release(hc->probe.off, hc->probe.bytes); hc->probe.off = -1; }
+    hc->dev.nProbe = 0;
+    hc->dev.probeCap = 0;
+    e->tablesDirty = true;
+    if (n == 0 || capacity == 0) return 0;
+    int64_t bytes = align_up(4 * (int64_t)n) + align_up(8 * (int64_t)n * capacity) + kAlign;
+    int64_t off;
+    int rc = arena_alloc(e, bytes, &off);
+    if (rc) return rc;
+    hc->probe.off = off;
+    hc->probe.bytes = bytes;
+    hc->dev.oProbeCompt = off;
+    hc->dev.oProbeTrace = off + align_up(4 * (int64_t)n);
+    hc->dev.oProbeCount = hc->dev.oProbeTrace + align_up(8 * (int64_t)n * capacity);
+    std::vector<int64_t> zero1(1, 0);
+    if ((rc = upload(e, hc->dev.oProbeCompt, h)) || (rc = upload(e, hc->dev.oProbeCount, zero1))) return rc;
+    HIPCHK(hipStreamSynchronize(e->stream));
+    hc->dev.nProbe = n;
+    hc->dev.probeCap = capacity;
+    hc->probeCompt.assign(compartments, compartments + n);
+    hc->probeCap = capacity;
+    return 0;
+}
+
+int hsg_read_probes(hsg_engine* e, int32_t cell, double* out, int64_t cap_samples, int64_t* n_samples) {
+    HostCell* hc = get_cell(e, cell);
+    if (!hc || !n_samples) return fail(HSG_E_ARG, "no such cell");
+    *n_samples = 0;
+    if (hc->dev.nProbe == 0) return 0;
+    hipSetDevice(e->device);
+    int64_t cnt = 0;
+    int rc = download(e, hc->dev.oProbeCount, 1, &cnt);
+    if (rc) return rc;
+    int64_t k = std::min(std::min(cnt, hc->dev.probeCap), cap_samples);
+    if (k > 0 && out && (rc = download(e, hc->dev.oProbeTrace, (size_t)(k * hc->dev.nProbe), out))) return rc;
+    std::vector<int64_t> zero1(1, 0);
+    if ((rc = upload(e, hc->dev.oProbeCount, zero1))) return rc;
+    HIPCHK(hipStreamSynchronize(e->stream));
+    *n_samples = k;
+    return 0;
+}
+
+int hsg_cell_info_get(hsg_engine* e, int32_t cell, hsg_cell_info* out) {
+    HostCell* hc = get_cell(e, cell);
+    if (!hc || !out) return fail(HSG_E_ARG, "no such cell");
+    const CellSetup& s = hc->s;
+    out->n_compartments = s.n;
+    int nch = 0;
+    for (int c : s.chanCompt) nch += c >= 0;
+    out->n_channels = nch;
+    int ng = 0;
+    for (size_t k = 0; k < s.chanCompt.size(); ++k)
+        if (s.chanCompt[k] >= 0) ng += (s.chanXp[k] > 0) + (s.chanYp[k] > 0) + (s.chanZp[k] > 0);
+    out->n_gates = ng;
+    int np = 0;
+    for (int c : s.poolCompt) np += c >= 0;
+    out->n_pools = np;
+    out->n_spikegens = (int)s.spkCompt.size();
+    out->n_levels = (int)s.levelStart.size() - 1;
+    out->max_group = s.maxGroup;
+    out->device_bytes = hc->region.bytes + std::max<int64_t>(0, hc->inj.bytes) +
+                        std::max<int64_t>(0, hc->probe.bytes) + std::max<int64_t>(0, hc->extBlk.bytes);
+    out->steps_done = hc->stepsDone;
+    return 0;
+}
+
+int hsg_hines_order(hsg_engine* e, int32_t cell, int32_t* out) {
+    HostCell* hc = get_cell(e, cell);
+    if (!hc || !out) return fail(HSG_E_ARG, "no such cell");
+    for (size_t i = 0; i < hc->s.descToHines.size(); ++i) out[i] = hc->s.descToHines[i];
+    return 0;
+}
+
+int hsg_sync(hsg_engine* e) {
+    if (!e) return fail(HSG_E_ARG, "null engine");
+    hipSetDevice(e->device);
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+int hsg_last_step_timing(hsg_engine* e, int32_t* launches, double* total_ms, double* mean_ms) {
+    if (!e) return fail(HSG_E_ARG, "null engine");
+    if (launches) *launches = e->lastLaunches;
+    if (total_ms) *total_ms = e->lastTotalMs;
+    if (mean_ms) *mean_ms = e->lastLaunches ? e->lastTotalMs / e->lastLaunches : 0.0;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,123 @@
+// hsolve_internal.h — device-side layout shared by the host engine and the
+// HIP kernels.
+//
+// Every cell owns one contiguous region of the engine's device arena.  All
+// offsets below are byte offsets into that arena (so the arena can move).
+// Per-compartment arrays are indexed by the Hines index i (0..n-1).  Channel
+// and gate-state arrays are "slot-major": channel slot k of compartment i sits
+// at [k*n + i], gate-state slot j at [j*n + i], Ca-pool slot q at [q*n + i],
+// so a wavefront walking consecutive compartments issues coalesced loads.
+#pragma once
+#include <stdint.h>
+
+namespace hsg {
+
+constexpr int kMaxSlots = 8;      // channel slots per compartment
+constexpr int kMaxGateSlots = 24; // gate-state slots per compartment
+constexpr int kMaxPoolSlots = 4;  // Ca-pool slots per compartment
+
+// One channel type: everything about an HHChannel that does not vary per
+// instance (hsolve/HSolveStruct.h:53-108 ChannelStruct, plus the gate columns
+// and Ca wiring of hsolve/HSolveActiveSetup.cpp:235-526).
+struct ChanType {
+    double Xpower, Ypower, Zpower;
+    int32_t instant;        // INSTANT_X=1, INSTANT_Y=2, INSTANT_Z=4
+    int32_t nGates;         // state slots used (gates with power > 0)
+    int32_t colX, colY, colZ;   // column (2*species) in the cell's table set
+    int32_t caDepLocal;     // Z gate reads this local pool's row (-1: none)
+    int32_t caTgtLocal;     // IkOut feeds this local pool (-1: none)
+    int32_t tableSet;
+    int32_t pad;
+};
+
+// One lookup table (hsolve/RateLookup.h LookupTable).
+struct TableDev {
+    double  min, max, dx, invDx;
+    int32_t nPts, nColumns;
+    int64_t off64;          // f64 table [nPts][nColumns]
+    int64_t off32;          // f32 copy (mixed precision)
+};
+
+struct TableSetDev {
+    TableDev v, ca;
+};
+
+// Per-cell descriptor (read by every workgroup working on the cell).
+struct CellDev {
+    int32_t n;              // compartments
+    int32_t nChanSlots;     // channel slots per compartment (CHM)
+    int32_t nGateSlots;     // gate-state slots per compartment (GM)
+    int32_t nPoolSlots;     // Ca-pool slots per compartment (PM)
+    int32_t uniform;        // 1: every compartment has the channel types slotType[]
+    int32_t slotType[kMaxSlots];
+    int32_t hasModulation;  // 1: multiply by per-channel modulation
+    int32_t hasPools;
+    int32_t caAdvance;
+    int32_t nLevels;
+    int32_t nUnits;
+    int32_t nHJ;            // doubles of junction coefficients
+    int32_t nTerm;          // deferred elimination terms (pairs)
+    int32_t nInj;           // injection entries
+    int32_t nSpike;         // spike generators
+    int32_t nProbe;         // Vm probes
+    int32_t workInLds;      // 1: elimination work arrays live in LDS
+    int32_t live;
+    int32_t nGroups;
+    int32_t pad0;
+    int64_t probeCap;       // samples per probe
+    // compartment arrays (f64 unless noted)
+    int64_t oV, oCmByDt, oEmByRm, oDbase, oOff;
+    int64_t oDefer;         // int32[n]: first deferred-term pair of node, -1
+    // channels: slot-major [k*n + i]
+    int64_t oChType;        // int32 (non-uniform cells)
+    int64_t oGbar, oEk;     // GT
+    int64_t oMod;           // f64
+    int64_t oExtCa;         // f64
+    int64_t oState;         // GT, [j*n + i]
+    // pools: slot-major [q*n + i]
+    int64_t oPoolC, oPoolCa, oPoolBasal, oPoolF1, oPoolF2, oPoolCeil, oPoolFloor, oPoolICa;
+    int64_t oPoolCount;     // int32[n]
+    // Hines schedule
+    int64_t oLevelStart;    // int32[nLevels+1] into units
+    int64_t oUnits;         // int32: >=0 linear link i->i+1, <0 group -(u+1)
+    int64_t oGrpMem;        // int32 member lists
+    int64_t oGrpInfo;       // int32[4*nGroups]: memStart, size, hjOff, deferLast
+    int64_t oHJ0;           // f64[nHJ] initial junction coefficients (HJCopy_)
+    // work (global fallback when not in LDS): D[n], B[n], HJ[nHJ], term[2*nTerm]
+    int64_t oWork;
+    // injections: int32 idx[nInj] + f64 basal[nInj] + f64 varying[nInj]
+    int64_t oInjIdx, oInjBasal, oInjVar;
+    // spike generators: int32 compt, int32 edge, f64 thr, f64 refr, f64 last, int32 fired
+    int64_t oSpkCompt, oSpkEdge, oSpkThr, oSpkRefr, oSpkLast, oSpkFired;
+    // probes: int32 compt[nProbe], f64 trace[probeCap*nProbe], int64 count
+    int64_t oProbeCompt, oProbeTrace, oProbeCount;
+    // per-call external currents: int32 idx + f64 Gk + f64 GkEk (nExt entries)
+    int64_t oExtIdx, oExtG, oExtGE;
+    int32_t nExt;
+    int32_t cellId;
+    int64_t steps;          // steps done since reinit (device copy kept in oSteps)
+    int64_t oSteps;         // int64 counter
+};
+
+struct SpikeRec {
+    int32_t cell;
+    int32_t gen;
+    int64_t step;
+};
+
+struct LaunchArgs {
+    char*          arena;
+    const CellDev* cells;       // device array of CellDev
+    const int32_t* order;       // cells to run (indices into cells)
+    const ChanType* types;
+    const TableSetDev* tsets;
+    SpikeRec*      spikes;
+    unsigned long long* spikeCount;
+    int32_t        spikeCap;
+    int32_t        nsteps;      // steps in this launch
+    int32_t        firstOfCall; // 1: apply the call's one-shot inputs at step 0
+    int32_t        ldsBytes;    // dynamic LDS per workgroup
+    double         dt;          // clock tick dt
+};
+
+}  // namespace hsg

@@ -1,0 +1,527 @@
+// hsolve_kernels.hip — the per-step HSolve path on CDNA4 (gfx950).
+//
+// One workgroup advances one cell for `nsteps` consecutive ticks
+// (HSolveActive::step, hsolve/HSolveActive.cpp:45-66, repeated).  Cells are
+// independent, so a launch needs no grid-wide synchronisation between steps.
+// Per tick the workgroup runs:
+//   A  channels + assembly: one thread per compartment (strided), gate update
+//      from the lookup tables (advanceChannels, HSolveActive.cpp:218-332),
+//      Gk (calculateChannelCurrents, :68-83) and the matrix row
+//      (updateMatrix, :85-151) written to the elimination work arrays.
+//   F  forward elimination over the unit schedule, one unit per thread and a
+//      barrier per level (forwardEliminate, hsolve/HSolvePassive.cpp:246-323).
+//   B  back-substitution in reverse level order (backwardSubstitute,
+//      HSolvePassive.cpp:325-401) and V = 2*VMid - V.
+//   D  Ca pools (advanceCalcium, HSolveActive.cpp:153-216), spike generators
+//      (SpikeGen::process, biophysics/SpikeGen.cpp:202-216) and Vm probes.
+// A "unit" is either a linear link i -> i+1 or a junction group (all members
+// but the last eliminated, in member order, into the later members).  Units of
+// one level touch disjoint rows.  Terms that the reference applies to a row
+// only after that row's earlier group siblings are parked in a deferred-term
+// slot and applied in the reference's order, so that the FP64 form (built with
+// -ffp-contract=off and IEEE division) reproduces the reference bit for bit.
+//
+// GT is the gate/channel type: double (FP64 form) or float (mixed form, where
+// gate states, tables and channel currents are FP32 and the Hines solve stays
+// FP64).
+#include <hip/hip_runtime.h>
+
+#include "hsolve_internal.h"
+
+namespace hsg {
+
+template <typename T>
+__device__ __forceinline__ T* A_(char* arena, int64_t off) {
+    return reinterpret_cast<T*>(arena + off);
+}
+
+// LookupTable::row (hsolve/RateLookup.cpp:63-76)
+__device__ __forceinline__ void table_row(const TableDev& t, double x, int& base, double& frac) {
+    if (x < t.min) x = t.min;
+    else if (x > t.max) x = t.max;
+    double div = (x - t.min) / t.dx;
+    unsigned integer = (unsigned)div;
+    frac = div - integer;
+    base = (int)integer * t.nColumns;
+}
+
+// LookupTable::lookup (hsolve/RateLookup.cpp:78-103)
+template <typename GT>
+__device__ __forceinline__ void table_lookup(const GT* tab, int nCol, int base, int col, GT frac,
+                                             GT& C1, GT& C2) {
+    const GT* ap = tab + base + col;
+    const GT* bp = ap + nCol;
+    GT a = ap[0], b = bp[0];
+    C1 = a + (b - a) * frac;
+    a = ap[1];
+    b = bp[1];
+    C2 = a + (b - a) * frac;
+}
+
+template <typename GT>
+__device__ __forceinline__ const GT* table_ptr(char* arena, const TableDev& t);
+template <>
+__device__ __forceinline__ const double* table_ptr<double>(char* arena, const TableDev& t) {
+    return A_<const double>(arena, t.off64);
+}
+template <>
+__device__ __forceinline__ const float* table_ptr<float>(char* arena, const TableDev& t) {
+    return A_<const float>(arena, t.off32);
+}
+
+// ChannelStruct powers (hsolve/HSolveStruct.cpp:15-65, HSolveStruct.h:88-108)
+template <typename GT>
+__device__ __forceinline__ GT take_power(GT x, double p) {
+    if (p == 1.0) return x;
+    if (p == 2.0) return x * x;
+    if (p == 3.0) return x * x * x;
+    if (p == 4.0) {
+        GT y = x * x;
+        return y * y;
+    }
+    if (x > (GT)0) return (GT)exp((double)p * log((double)x));
+    return (GT)0;
+}
+
+// trapezoidal gate update, dx/dt = A - B x (HSolveActive.cpp:259-268)
+template <typename GT>
+__device__ __forceinline__ GT gate_step(GT x, GT C1, GT C2, GT dt, bool instant) {
+    if (instant) return C1 / C2;
+    GT temp = (GT)1.0 + dt / (GT)2.0 * C2;
+    return (x * ((GT)2.0 - temp) + dt * C1) / temp;
+}
+
+struct Work {
+    double* D;
+    double* B;     // becomes VMid during back-substitution
+    double* HJ;
+    double* T;     // deferred terms: pairs (tD, tB)
+};
+
+// ---------------------------------------------------------------------------
+// phase A: channels and matrix rows for compartment i
+// ---------------------------------------------------------------------------
+template <typename GT>
+__device__ __forceinline__ void channel_row(char* arena, const CellDev& c, const ChanType* types,
+                                            const TableSetDev* tsets, int i, GT dtg, Work w,
+                                            bool reinitOnly) {
+    const int n = c.n;
+    const double V = A_<const double>(arena, c.oV)[i];
+    GT* state = A_<GT>(arena, c.oState);
+    const GT* Gbar = A_<const GT>(arena, c.oGbar);
+    const GT* Ek = A_<const GT>(arena, c.oEk);
+    const int32_t* chType = c.uniform ? nullptr : A_<const int32_t>(arena, c.oChType);
+
+    // Ca rows of this compartment's pools (caRowCompt_), taken in the loop
+    // below from the cell's table set
+    int caBase[kMaxPoolSlots];
+    double caFrac[kMaxPoolSlots];
+    const int nPools = c.hasPools ? A_<const int32_t>(arena, c.oPoolCount)[i] : 0;
+
+    double GkSum = 0.0, GkEkSum = 0.0;
+    int js = 0;                       // gate-state slot
+    int vBase = -1;
+    double vFrac = 0.0;
+    int curSet = -1;
+    for (int k = 0; k < c.nChanSlots; ++k) {
+        const int t = c.uniform ? c.slotType[k] : chType[(size_t)k * n + i];
+        if (t < 0) continue;
+        const ChanType ct = types[t];
+        const TableSetDev& ts = tsets[ct.tableSet];
+        if (ct.tableSet != curSet) {
+            curSet = ct.tableSet;
+            table_row(ts.v, V, vBase, vFrac);
+            if (c.hasPools) {
+                const double* pca = A_<const double>(arena, c.oPoolCa);
+                for (int q = 0; q < kMaxPoolSlots; ++q)
+                    if (q < nPools) table_row(ts.ca, pca[(size_t)q * n + i], caBase[q], caFrac[q]);
+            }
+        }
+        const GT* vt = table_ptr<GT>(arena, ts.v);
+        const int vCol = ts.v.nColumns;
+        GT C1, C2;
+        GT frac = (GT)vFrac;
+        GT fraction = c.hasModulation ? (GT)A_<const double>(arena, c.oMod)[(size_t)k * n + i] : (GT)1.0;
+        if (ct.Xpower > 0.0) {
+            GT* sp = state + (size_t)js * n + i;
+            table_lookup<GT>(vt, vCol, vBase, ct.colX, frac, C1, C2);
+            GT x = reinitOnly ? C1 / C2 : gate_step<GT>(*sp, C1, C2, dtg, ct.instant & 1);
+            *sp = x;
+            fraction *= take_power<GT>(x, ct.Xpower);
+            ++js;
+        }
+        if (ct.Ypower > 0.0) {
+            GT* sp = state + (size_t)js * n + i;
+            table_lookup<GT>(vt, vCol, vBase, ct.colY, frac, C1, C2);
+            GT x = reinitOnly ? C1 / C2 : gate_step<GT>(*sp, C1, C2, dtg, ct.instant & 2);
+            *sp = x;
+            fraction *= take_power<GT>(x, ct.Ypower);
+            ++js;
+        }
+        if (ct.Zpower > 0.0) {
+            GT* sp = state + (size_t)js * n + i;
+            const GT* ctab = table_ptr<GT>(arena, ts.ca);
+            const int cCol = ts.ca.nColumns;
+            if (ct.caDepLocal >= 0 && ct.caDepLocal < nPools) {
+                int cb = 0;
+                double cf = 0.0;
+                for (int q = 0; q < kMaxPoolSlots; ++q)
+                    if (q == ct.caDepLocal) { cb = caBase[q]; cf = caFrac[q]; }
+                table_lookup<GT>(ctab, cCol, cb, ct.colZ, (GT)cf, C1, C2);
+            } else {
+                const double ext = A_<const double>(arena, c.oExtCa)[(size_t)k * n + i];
+                if (ext > 0) {
+                    int b; double f;
+                    table_row(ts.ca, ext, b, f);
+                    table_lookup<GT>(ctab, cCol, b, ct.colZ, (GT)f, C1, C2);
+                } else {
+                    table_lookup<GT>(vt, vCol, vBase, ct.colZ, frac, C1, C2);
+                }
+            }
+            GT x = reinitOnly ? C1 / C2 : gate_step<GT>(*sp, C1, C2, dtg, ct.instant & 4);
+            *sp = x;
+            fraction *= take_power<GT>(x, ct.Zpower);
+            ++js;
+        }
+        if (!reinitOnly) {
+            GT Gk = Gbar[(size_t)k * n + i] * fraction;
+            GkSum += (double)Gk;
+            GkEkSum += (double)Gk * (double)Ek[(size_t)k * n + i];
+        }
+    }
+    if (reinitOnly) return;
+    // updateMatrix row (HSolveActive.cpp:112-115)
+    w.D[i] = A_<const double>(arena, c.oDbase)[i] + GkSum;
+    w.B[i] = V * A_<const double>(arena, c.oCmByDt)[i] + A_<const double>(arena, c.oEmByRm)[i] + GkEkSum;
+}
+
+// ---------------------------------------------------------------------------
+// elimination units
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int hj_upper(int base, int p, int q) { (void)p; return base + 2 * (q - p - 1); }
+
+// HJ block of member p starts at sum_{p'<p} 2*(s-1-p')
+__device__ __forceinline__ int hj_base(int hjOff, int s, int p) { return hjOff + p * (2 * (s - 1) - (p - 1)); }
+
+__device__ void forward_unit(char* arena, const CellDev& c, int u, Work w) {
+    const int32_t* defer = A_<const int32_t>(arena, c.oDefer);
+    if (u >= 0) {
+        // linear link x -> x+1 (HSolvePassive.cpp:268-272)
+        const int x = u;
+        const double off = A_<const double>(arena, c.oOff)[x];
+        const double Dx = w.D[x];
+        const double tD = off / Dx * off;
+        const double tB = off / Dx * w.B[x];
+        const int d = defer[2 * (x + 1)];
+        if (d >= 0) {
+            w.T[2 * d] = tD;
+            w.T[2 * d + 1] = tB;
+        } else {
+            w.D[x + 1] -= tD;
+            w.B[x + 1] -= tB;
+        }
+        return;
+    }
+    const int g = -u - 1;
+    const int4 gi = A_<const int4>(arena, c.oGrpInfo)[g];
+    const int32_t* mem = A_<const int32_t>(arena, c.oGrpMem) + gi.x;
+    const int s = gi.y;
+    double* HJ = w.HJ;
+    const int deferLast = gi.w;
+    for (int p = 0; p + 1 < s; ++p) {
+        const int cn = mem[p];
+        if (p >= 1) {
+            const int d = defer[2 * cn];
+            if (d >= 0) {
+                const int cnt = defer[2 * cn + 1];
+                for (int t = 0; t < cnt; ++t) {
+                    w.D[cn] -= w.T[2 * (d + t)];
+                    w.B[cn] -= w.T[2 * (d + t) + 1];
+                }
+            }
+        }
+        const double pivot = w.D[cn];
+        const double Bc = w.B[cn];
+        const int base = hj_base(gi.z, s, p);
+        for (int q = p + 1; q < s; ++q) {
+            const int y = mem[q];
+            const double lower = HJ[base + 2 * (q - p - 1) + 1];
+            const double upper = HJ[base + 2 * (q - p - 1)];
+            const double tD = lower / pivot * upper;
+            const double tB = lower / pivot * Bc;
+            if (q == s - 1 && deferLast >= 0) {
+                w.T[2 * (deferLast + p)] = tD;
+                w.T[2 * (deferLast + p) + 1] = tB;
+            } else {
+                w.D[y] -= tD;
+                w.B[y] -= tB;
+            }
+            // fill among later members: A[y][z] -= A[y][c]/pivot * A[c][z]
+            for (int z = p + 1; z < s; ++z) {
+                if (z == q) continue;
+                const double above = HJ[base + 2 * (z - p - 1)];
+                int idx;
+                if (q < z) idx = hj_base(gi.z, s, q) + 2 * (z - q - 1);
+                else idx = hj_base(gi.z, s, z) + 2 * (q - z - 1) + 1;
+                HJ[idx] -= lower / pivot * above;
+            }
+        }
+    }
+}
+
+__device__ void backward_unit(char* arena, const CellDev& c, int u, Work w) {
+    if (u >= 0) {
+        const int x = u;
+        const double off = A_<const double>(arena, c.oOff)[x];
+        w.B[x] = (w.B[x] - off * w.B[x + 1]) / w.D[x];
+        return;
+    }
+    const int g = -u - 1;
+    const int4 gi = A_<const int4>(arena, c.oGrpInfo)[g];
+    const int32_t* mem = A_<const int32_t>(arena, c.oGrpMem) + gi.x;
+    const int s = gi.y;
+    for (int p = s - 2; p >= 0; --p) {
+        const int cn = mem[p];
+        const int base = hj_base(gi.z, s, p);
+        double acc = w.B[cn];
+        for (int q = s - 1; q > p; --q) acc -= w.B[mem[q]] * w.HJ[base + 2 * (q - p - 1)];
+        w.B[cn] = acc / w.D[cn];
+    }
+}
+
+// CaConcStruct::process (hsolve/HSolveStruct.cpp:138-155)
+__device__ __forceinline__ double caconc_process(double& cst, double basal, double f1, double f2,
+                                                double ceil, double floor, double act) {
+    cst = f1 * cst + f2 * act;
+    double ca = basal + cst;
+    if (ceil > 0 && ca > ceil) {
+        ca = ceil;
+        cst = ca - basal;
+    }
+    if (ca < floor) {
+        ca = floor;
+        cst = ca - basal;
+    }
+    return ca;
+}
+
+template <typename GT>
+__device__ void calcium_row(char* arena, const CellDev& c, const ChanType* types, int i, Work w,
+                            bool firstOfCall) {
+    const int n = c.n;
+    const int nPools = A_<const int32_t>(arena, c.oPoolCount)[i];
+    if (nPools == 0) return;
+    double act[kMaxPoolSlots];
+    double* ica = A_<double>(arena, c.oPoolICa);
+    for (int q = 0; q < kMaxPoolSlots; ++q) {
+        act[q] = 0.0;
+        if (q < nPools && firstOfCall) {
+            act[q] = 0.0 + ica[(size_t)q * n + i];
+            ica[(size_t)q * n + i] = 0.0;
+        }
+    }
+    const double vmid = w.B[i];
+    const double vnew = A_<const double>(arena, c.oV)[i];
+    const GT* state = A_<const GT>(arena, c.oState);
+    const GT* Gbar = A_<const GT>(arena, c.oGbar);
+    const GT* Ek = A_<const GT>(arena, c.oEk);
+    const int32_t* chType = c.uniform ? nullptr : A_<const int32_t>(arena, c.oChType);
+    int js = 0;
+    for (int k = 0; k < c.nChanSlots; ++k) {
+        const int t = c.uniform ? c.slotType[k] : chType[(size_t)k * n + i];
+        if (t < 0) continue;
+        const ChanType ct = types[t];
+        if (ct.caTgtLocal >= 0) {
+            GT fraction = c.hasModulation ? (GT)A_<const double>(arena, c.oMod)[(size_t)k * n + i] : (GT)1.0;
+            int jj = js;
+            if (ct.Xpower > 0.0) fraction *= take_power<GT>(state[(size_t)(jj++) * n + i], ct.Xpower);
+            if (ct.Ypower > 0.0) fraction *= take_power<GT>(state[(size_t)(jj++) * n + i], ct.Ypower);
+            if (ct.Zpower > 0.0) fraction *= take_power<GT>(state[(size_t)(jj++) * n + i], ct.Zpower);
+            const double Gk = (double)(Gbar[(size_t)k * n + i] * fraction);
+            const double ek = (double)Ek[(size_t)k * n + i];
+            for (int q = 0; q < kMaxPoolSlots; ++q)
+                if (q == ct.caTgtLocal && q < nPools) {
+                    if (c.caAdvance == 1) act[q] += Gk * (ek - vmid);
+                    else act[q] += Gk * (ek - (2 * vmid - vnew));
+                }
+        }
+        js += ct.nGates;
+    }
+    double* pc = A_<double>(arena, c.oPoolC);
+    double* pca = A_<double>(arena, c.oPoolCa);
+    const double* basal = A_<const double>(arena, c.oPoolBasal);
+    const double* f1 = A_<const double>(arena, c.oPoolF1);
+    const double* f2 = A_<const double>(arena, c.oPoolF2);
+    const double* ceil = A_<const double>(arena, c.oPoolCeil);
+    const double* flo = A_<const double>(arena, c.oPoolFloor);
+    for (int q = 0; q < kMaxPoolSlots; ++q) {
+        if (q >= nPools) break;
+        const size_t o = (size_t)q * n + i;
+        double cst = pc[o];
+        pca[o] = caconc_process(cst, basal[o], f1[o], f2[o], ceil[o], flo[o], act[q]);
+        pc[o] = cst;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// the step kernel
+// ---------------------------------------------------------------------------
+template <typename GT>
+__global__ void __launch_bounds__(1024) hsolve_step_kernel(LaunchArgs a) {
+    extern __shared__ double lds[];
+    const CellDev& c = a.cells[a.order[blockIdx.x]];
+    char* arena = a.arena;
+    const int n = c.n;
+    const int tid = threadIdx.x;
+    const int nt = blockDim.x;
+    Work w;
+    double* base = c.workInLds ? lds : A_<double>(arena, c.oWork);
+    w.D = base;
+    w.B = base + n;
+    w.HJ = base + 2 * n;
+    w.T = base + 2 * n + c.nHJ;
+    const GT dtg = (GT)a.dt;
+    const double* hj0 = A_<const double>(arena, c.oHJ0);
+    const int32_t* levelStart = A_<const int32_t>(arena, c.oLevelStart);
+    const int32_t* units = A_<const int32_t>(arena, c.oUnits);
+    int64_t* stepCounter = A_<int64_t>(arena, c.oSteps);
+    const int64_t step0 = *stepCounter;
+
+    for (int k = 0; k < a.nsteps; ++k) {
+        const bool first = a.firstOfCall && k == 0;
+        // ---- A: channels, currents, matrix rows
+        for (int i = tid; i < n; i += nt)
+            channel_row<GT>(arena, c, a.types, a.tsets, i, dtg, w, false);
+        for (int j = tid; j < c.nHJ; j += nt) w.HJ[j] = hj0[j];
+        __syncthreads();
+        if (tid == 0) {
+            // injections: HS[4i+3] += injectVarying + injectBasal (HSolveActive.cpp:119-128)
+            const int32_t* ii = A_<const int32_t>(arena, c.oInjIdx);
+            const double* ib = A_<const double>(arena, c.oInjBasal);
+            double* iv = A_<double>(arena, c.oInjVar);
+            for (int e = 0; e < c.nInj; ++e) {
+                const double var = first ? iv[e] : 0.0;
+                w.B[ii[e]] += var + ib[e];
+            }
+            if (first) {
+                for (int e = 0; e < c.nInj; ++e) iv[e] = 0.0;
+                // external channels (HSolveActive.cpp:142-148)
+                const int32_t* xi = A_<const int32_t>(arena, c.oExtIdx);
+                const double* xg = A_<const double>(arena, c.oExtG);
+                const double* xge = A_<const double>(arena, c.oExtGE);
+                for (int e = 0; e < c.nExt; ++e) {
+                    w.D[xi[e]] += xg[e];
+                    w.B[xi[e]] += xge[e];
+                }
+            }
+        }
+        __syncthreads();
+        // ---- F: forward elimination, level by level
+        for (int l = 0; l < c.nLevels; ++l) {
+            const int e = levelStart[l + 1];
+            for (int u = levelStart[l] + tid; u < e; u += nt) forward_unit(arena, c, units[u], w);
+            __syncthreads();
+        }
+        // ---- B: back-substitution (VMid overwrites B)
+        if (tid == 0) w.B[n - 1] = w.B[n - 1] / w.D[n - 1];
+        __syncthreads();
+        for (int l = c.nLevels - 1; l >= 0; --l) {
+            const int e = levelStart[l + 1];
+            for (int u = levelStart[l] + tid; u < e; u += nt) backward_unit(arena, c, units[u], w);
+            __syncthreads();
+        }
+        // ---- D: Vm, calcium
+        double* Vg = A_<double>(arena, c.oV);
+        for (int i = tid; i < n; i += nt) Vg[i] = 2 * w.B[i] - Vg[i];
+        if (c.hasPools) {
+            __syncthreads();
+            for (int i = tid; i < n; i += nt) calcium_row<GT>(arena, c, a.types, i, w, first);
+        }
+        __syncthreads();
+        const int64_t stepNo = step0 + k + 1;
+        // spike generators (SpikeGen::process at t = dt * step)
+        for (int sg = tid; sg < c.nSpike; sg += nt) {
+            const int comp = A_<const int32_t>(arena, c.oSpkCompt)[sg];
+            const double Vm = Vg[comp];
+            const double thr = A_<const double>(arena, c.oSpkThr)[sg];
+            int32_t* fired = A_<int32_t>(arena, c.oSpkFired);
+            if (Vm > thr) {
+                const double t = a.dt * (double)(unsigned long long)stepNo;
+                double* last = A_<double>(arena, c.oSpkLast);
+                const double refr = A_<const double>(arena, c.oSpkRefr)[sg];
+                const int edge = A_<const int32_t>(arena, c.oSpkEdge)[sg];
+                if ((t + a.dt / 2.0) >= (last[sg] + refr)) {
+                    if (!(edge && fired[sg])) {
+                        const unsigned long long slot = atomicAdd(a.spikeCount, 1ULL);
+                        if (slot < (unsigned long long)a.spikeCap) {
+                            SpikeRec r;
+                            r.cell = c.cellId;
+                            r.gen = sg;
+                            r.step = stepNo;
+                            a.spikes[slot] = r;
+                        }
+                        last[sg] = t;
+                        fired[sg] = 1;
+                    }
+                }
+            } else {
+                fired[sg] = 0;
+            }
+        }
+        // Vm probes (Table sampling after the solver's tick)
+        if (c.nProbe > 0) {
+            int64_t* pcount = A_<int64_t>(arena, c.oProbeCount);
+            const int64_t sample = *pcount;
+            if (sample < c.probeCap) {
+                const int32_t* pc = A_<const int32_t>(arena, c.oProbeCompt);
+                double* tr = A_<double>(arena, c.oProbeTrace);
+                for (int j = tid; j < c.nProbe; j += nt) tr[sample * c.nProbe + j] = Vg[pc[j]];
+            }
+            __syncthreads();
+            if (tid == 0) *pcount = sample + 1;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) *stepCounter = step0 + a.nsteps;
+}
+
+// HSolveActive::reinitChannels (hsolve/HSolveActiveSetup.cpp:79-160): gates at
+// A/B of the current V (and pool Ca).
+template <typename GT>
+__global__ void __launch_bounds__(256) hsolve_reinit_kernel(LaunchArgs a) {
+    const CellDev& c = a.cells[a.order[blockIdx.x]];
+    Work w{};
+    for (int i = threadIdx.x; i < c.n; i += blockDim.x)
+        channel_row<GT>(a.arena, c, a.types, a.tsets, i, (GT)a.dt, w, true);
+}
+
+}  // namespace hsg
+
+// host-side launchers (called from hsolve_engine.cpp)
+extern "C" hipError_t hsg_launch_step(const hsg::LaunchArgs* a, int nblocks, int threads, int fp32,
+                                     hipStream_t stream) {
+    if (fp32)
+        hipLaunchKernelGGL(hsg::hsolve_step_kernel<float>, dim3(nblocks), dim3(threads),
+                           a->ldsBytes, stream, *a);
+    else
+        hipLaunchKernelGGL(hsg::hsolve_step_kernel<double>, dim3(nblocks), dim3(threads),
+                           a->ldsBytes, stream, *a);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t hsg_launch_reinit(const hsg::LaunchArgs* a, int nblocks, int fp32,
+                                       hipStream_t stream) {
+    if (fp32)
+        hipLaunchKernelGGL(hsg::hsolve_reinit_kernel<float>, dim3(nblocks), dim3(256), 0, stream, *a);
+    else
+        hipLaunchKernelGGL(hsg::hsolve_reinit_kernel<double>, dim3(nblocks), dim3(256), 0, stream, *a);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t hsg_set_max_lds(int bytes) {
+    hipError_t e = hipFuncSetAttribute((const void*)hsg::hsolve_step_kernel<double>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute((const void*)hsg::hsolve_step_kernel<float>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}

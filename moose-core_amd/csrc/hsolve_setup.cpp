@@ -1,0 +1,639 @@
+// hsolve_setup.cpp — turns a cell described in MOOSE object terms into the
+// Hines-ordered arrays and the level schedule of the GPU path.
+//
+// The numbering, matrix coefficients, table construction and channel/pool
+// bookkeeping follow the reference's setup exactly (same arithmetic in the
+// same order), because the FP64 GPU form is meant to reproduce the
+// reference's numbers bit for bit:
+//   walkTree ............ hsolve/HSolvePassive.cpp:45-98
+//   initialize/storeTree  hsolve/HSolvePassive.cpp:100-205
+//   makeJunctions/Matrix  hsolve/HinesMatrix.cpp:85-222
+//   readHHChannels ...... hsolve/HSolveActiveSetup.cpp:162-233
+//   readGates/Calcium ... hsolve/HSolveActiveSetup.cpp:235-337
+//   createLookupTables .. hsolve/HSolveActiveSetup.cpp:339-526
+//   readSynapses ........ hsolve/HSolveActiveSetup.cpp:538-578 (SpikeGens)
+// The level schedule (which units may run concurrently) is this project's
+// own; see schedule() below.
+#include "hsolve_setup.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cstring>
+#include <set>
+
+namespace hsg {
+
+namespace {
+
+// Message lists per compartment, in message order.
+struct Adjacency {
+    std::vector<std::vector<int>> axial;    // children (axial targets)
+    std::vector<std::vector<int>> raxial;   // parents (raxial targets)
+};
+
+Adjacency make_adjacency(const hsg_cell_desc* d) {
+    Adjacency a;
+    a.axial.resize(d->n_compartments);
+    a.raxial.resize(d->n_compartments);
+    for (int k = 0; k < d->n_axial; ++k) {
+        int p = d->axial[2 * k], c = d->axial[2 * k + 1];
+        a.axial[p].push_back(c);
+        a.raxial[c].push_back(p);
+    }
+    return a;
+}
+
+// HSolveUtils::adjacent (hsolve/HSolveUtils.cpp:26-45) minus `exclude`.
+void adjacent(const Adjacency& a, int c, int exclude, std::vector<int>& out) {
+    for (int x : a.axial[c]) if (x != exclude) out.push_back(x);
+    for (int x : a.raxial[c]) if (x != exclude) out.push_back(x);
+}
+
+// HSolvePassive::walkTree: find a terminal by following the first neighbour,
+// then depth-first from it popping the last neighbour first; the visit order
+// reversed is the Hines numbering.
+int walk_tree(const hsg_cell_desc* d, const Adjacency& a, std::vector<int>& order, std::string& err) {
+    const int n = d->n_compartments;
+    int seed = d->seed;
+    std::vector<int> adj;
+    adjacent(a, seed, -2, adj);
+    if (adj.size() > 1) {
+        int guard = 0;
+        while (!adj.empty()) {
+            int prev = seed;
+            seed = adj[0];
+            adj.clear();
+            adjacent(a, seed, prev, adj);
+            if (++guard > n + 1) { err = "walkTree: the compartments form a loop"; return HSG_E_MODEL; }
+        }
+    }
+    std::vector<std::vector<int>> stack;
+    stack.push_back({seed});
+    int above = -2;
+    order.clear();
+    while (!stack.empty()) {
+        std::vector<int>& top = stack.back();
+        if (top.empty()) {
+            stack.pop_back();
+            if (!stack.empty()) stack.back().pop_back();
+            continue;
+        }
+        if (stack.size() > 1) above = stack[stack.size() - 2].back();
+        int cur = top.back();
+        if ((int)order.size() >= n) { err = "walkTree: the compartments form a loop"; return HSG_E_MODEL; }
+        order.push_back(cur);
+        std::vector<int> next;
+        adjacent(a, cur, above, next);
+        stack.push_back(std::move(next));
+    }
+    std::reverse(order.begin(), order.end());
+    return 0;
+}
+
+// HHGate::lookupBoth with interpolation forced on (biophysics/HHGate.cpp:258-281)
+void lookup_both(const hsg_gate_desc& g, double v, double& A, double& B) {
+    const double invDx = (double)g.divs / (g.max - g.min);
+    if (v <= g.min) { A = g.tableA[0]; B = g.tableB[0]; }
+    else if (v >= g.max) { A = g.tableA[g.divs]; B = g.tableB[g.divs]; }
+    else {
+        unsigned idx = (unsigned)((v - g.min) * invDx);
+        double frac = (v - g.min - idx / invDx) * invDx;
+        A = g.tableA[idx] * (1 - frac) + g.tableA[idx + 1] * frac;
+        B = g.tableB[idx] * (1 - frac) + g.tableB[idx + 1] * frac;
+    }
+}
+
+// HSolveUtils::rates (hsolve/HSolveUtils.cpp:205-252): copy when the grids
+// match, else resample.
+void rates(const hsg_gate_desc& g, double mn, double mx, unsigned divs, std::vector<double>& A,
+           std::vector<double>& B) {
+    A.assign(divs + 1, 0.0);
+    B.assign(divs + 1, 0.0);
+    if (mn == g.min && mx == g.max && divs == g.divs) {
+        std::copy(g.tableA, g.tableA + g.divs + 1, A.begin());
+        std::copy(g.tableB, g.tableB + g.divs + 1, B.begin());
+        return;
+    }
+    const double dx = (mx - mn) / divs;
+    for (unsigned i = 0; i < divs + 1; ++i) lookup_both(g, mn + dx * i, A[i], B[i]);
+}
+
+// LookupTable ctor + addColumns (hsolve/RateLookup.cpp:15-55)
+void make_table(HostTable& t, double mn, double mx, unsigned nDivs, unsigned nSpecies) {
+    t.min = mn;
+    t.max = mx;
+    t.nDivs = nDivs;
+    t.nSpecies = nSpecies;
+    t.dx = (mx - mn) / nDivs;
+    t.table.assign((size_t)t.nPts() * t.nColumns(), 0.0);
+}
+void add_columns(HostTable& t, int species, const std::vector<double>& A, const std::vector<double>& B) {
+    const uint32_t nc = t.nColumns();
+    for (uint32_t ig = 0; ig + 1 < t.nPts(); ++ig) {
+        t.table[(size_t)ig * nc + 2 * species] = A[ig];
+        t.table[(size_t)ig * nc + 2 * species + 1] = B[ig];
+    }
+    t.table[(size_t)(t.nPts() - 1) * nc + 2 * species] = A.back();
+    t.table[(size_t)(t.nPts() - 1) * nc + 2 * species + 1] = B.back();
+}
+
+uint64_t fnv(const void* p, size_t n, uint64_t h) {
+    const unsigned char* c = (const unsigned char*)p;
+    for (size_t i = 0; i < n; ++i) { h ^= c[i]; h *= 1099511628211ull; }
+    return h;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Level schedule of the elimination.
+//
+// Every node but the last (the Hines root, n-1) is eliminated exactly once:
+// by a linear unit (x -> x+1) or as a non-last member of one junction group.
+// The only row updates a node receives from outside its own eliminating unit
+// come from one "feeder": the linear unit of x-1, or the group in which the
+// node is the last member.  Units therefore form a tree (a unit's parent is
+// the unit that eliminates its target), and a level = height in that tree is
+// a valid parallel schedule with disjoint writes per level.
+//
+// Order of updates: for a group member c_k (k >= 1) the reference subtracts
+// the terms of its earlier siblings c_1..c_{k-1} first and the terms of its
+// feeder afterwards (the feeder's nodes have higher Hines indices than
+// c_{k-1}).  Such a feeder therefore parks its terms in a deferred slot that
+// the group applies right before it pivots c_k.
+// ---------------------------------------------------------------------------
+static int schedule(CellSetup& s, const std::vector<int>& junctionGroup, std::string& err) {
+    const int n = s.n;
+    // eliminating unit of every node; feeder unit of every node
+    std::vector<int> elimUnit(n, -1), feeder(n, -1);
+    std::vector<Unit> units;
+    for (int g = 0; g < (int)s.groups.size(); ++g) {
+        Unit u{1, -1, g, 0};
+        units.push_back(u);
+    }
+    for (int g = 0; g < (int)s.groups.size(); ++g) {
+        const auto& m = s.groups[g].mem;
+        for (size_t p = 0; p + 1 < m.size(); ++p) {
+            if (elimUnit[m[p]] >= 0) { err = "junction structure: node eliminated twice"; return HSG_E_UNSUPPORTED; }
+            elimUnit[m[p]] = g;
+        }
+    }
+    for (int x = 0; x + 1 < n; ++x) {
+        if (junctionGroup[x] >= 0) continue;
+        Unit u{0, x, -1, 0};
+        elimUnit[x] = (int)units.size();
+        units.push_back(u);
+    }
+    for (int x = 0; x + 1 < n; ++x)
+        if (elimUnit[x] < 0) { err = "junction structure: node never eliminated"; return HSG_E_UNSUPPORTED; }
+    if (n >= 1 && elimUnit[n - 1] >= 0) {
+        err = "junction structure: the last Hines node is not the root";
+        return HSG_E_UNSUPPORTED;
+    }
+    // feeders
+    for (int ui = 0; ui < (int)units.size(); ++ui) {
+        const Unit& u = units[ui];
+        int target = u.kind == 0 ? u.node + 1 : s.groups[u.group].mem.back();
+        if (feeder[target] >= 0) {
+            err = "junction structure: a row has two feeding units (not a walkTree order)";
+            return HSG_E_UNSUPPORTED;
+        }
+        feeder[target] = ui;
+    }
+    // levels: process units by their largest eliminated node
+    std::vector<int> key(units.size());
+    for (size_t ui = 0; ui < units.size(); ++ui) {
+        const Unit& u = units[ui];
+        key[ui] = u.kind == 0 ? u.node : s.groups[u.group].mem[s.groups[u.group].mem.size() - 2];
+    }
+    std::vector<int> ord(units.size());
+    for (size_t i = 0; i < ord.size(); ++i) ord[i] = (int)i;
+    std::sort(ord.begin(), ord.end(), [&](int a, int b) { return key[a] < key[b]; });
+    for (int ui : ord) {
+        Unit& u = units[ui];
+        int lv = 0;
+        auto feed = [&](int node) {
+            int f = feeder[node];
+            if (f >= 0 && f != ui) {
+                if (key[f] >= key[ui]) return false;
+                lv = std::max(lv, units[f].level + 1);
+            }
+            return true;
+        };
+        bool ok = true;
+        if (u.kind == 0) ok = feed(u.node);
+        else {
+            const auto& m = s.groups[u.group].mem;
+            for (size_t p = 0; p + 1 < m.size(); ++p) ok = ok && feed(m[p]);
+        }
+        if (!ok) { err = "junction structure: feeder after its target"; return HSG_E_UNSUPPORTED; }
+        u.level = lv;
+    }
+    // deferred terms
+    s.defer.assign(2 * (size_t)n, -1);
+    for (int x = 0; x < n; ++x) s.defer[2 * x + 1] = 0;
+    for (auto& g : s.groups) g.deferLast = -1;
+    s.nTerm = 0;
+    for (int g = 0; g < (int)s.groups.size(); ++g) {
+        const auto& m = s.groups[g].mem;
+        for (size_t p = 1; p + 1 < m.size(); ++p) {
+            int f = feeder[m[p]];
+            if (f < 0) continue;
+            int cnt = units[f].kind == 0 ? 1 : (int)s.groups[units[f].group].mem.size() - 1;
+            s.defer[2 * m[p]] = s.nTerm;
+            s.defer[2 * m[p] + 1] = cnt;
+            if (units[f].kind == 1) s.groups[units[f].group].deferLast = s.nTerm;
+            s.nTerm += cnt;
+        }
+    }
+    int nLevels = 0;
+    for (const Unit& u : units) nLevels = std::max(nLevels, u.level + 1);
+    s.levelStart.assign(nLevels + 1, 0);
+    for (const Unit& u : units) s.levelStart[u.level + 1]++;
+    for (int l = 0; l < nLevels; ++l) s.levelStart[l + 1] += s.levelStart[l];
+    s.units.assign(units.size(), 0);
+    std::vector<int> fill(s.levelStart.begin(), s.levelStart.end() - 1);
+    for (int ui : ord) {
+        const Unit& u = units[ui];
+        s.units[fill[u.level]++] = u.kind == 0 ? u.node : -(u.group + 1);
+    }
+    return 0;
+}
+
+int setup_cell(const hsg_cell_desc* d, double dt, int caAdvance, CellSetup& s, std::string& err) {
+    if (!d) { err = "null cell description"; return HSG_E_ARG; }
+    if (!(dt > 0.0)) { err = "HSolve: 'dt' must be set (positive) before 'target'"; return HSG_E_ARG; }
+    if (caAdvance != 0 && caAdvance != 1) { err = "HSolve: caAdvance should be either 0 or 1"; return HSG_E_ARG; }
+    const int nd = d->n_compartments;
+    if (nd <= 0 || !d->compartments) { err = "no compartments"; return HSG_E_MODEL; }
+    if (d->seed < 0 || d->seed >= nd) { err = "seed compartment out of range"; return HSG_E_ARG; }
+    for (int k = 0; k < d->n_axial; ++k) {
+        int p = d->axial[2 * k], c = d->axial[2 * k + 1];
+        if (p < 0 || p >= nd || c < 0 || c >= nd) { err = "axial message refers to a missing compartment"; return HSG_E_ARG; }
+    }
+    s.dt = dt;
+    s.caAdvance = caAdvance;
+    Adjacency adj = make_adjacency(d);
+    int rc = walk_tree(d, adj, s.hinesToDesc, err);
+    if (rc) return rc;
+    const int n = (int)s.hinesToDesc.size();
+    s.n = n;
+    s.descToHines.assign(nd, -1);
+    for (int i = 0; i < n; ++i) s.descToHines[s.hinesToDesc[i]] = i;
+
+    // initialize + storeTree
+    std::vector<std::vector<int>> leaks(nd);
+    for (int k = 0; k < d->n_leaks; ++k) {
+        int c = d->leaks[k].compartment;
+        if (c < 0 || c >= nd) { err = "leak on a missing compartment"; return HSG_E_ARG; }
+        leaks[c].push_back(k);
+    }
+    s.V.resize(n); s.CmByDt.resize(n); s.EmByRm.resize(n);
+    s.Ra.resize(n); s.Rm.resize(n); s.Cm.resize(n); s.Em.resize(n); s.initVm.resize(n);
+    s.injBasal.assign(n, 0.0); s.hasInject.assign(n, 0);
+    std::vector<std::vector<int>> children(n);
+    for (int i = 0; i < n; ++i) {
+        const int dc = s.hinesToDesc[i];
+        const hsg_compartment_desc& c = d->compartments[dc];
+        s.V[i] = c.Vm;
+        double EmGmThev = c.Em / c.Rm;
+        for (int k : leaks[dc]) EmGmThev += d->leaks[k].Ek * d->leaks[k].Gk;
+        s.CmByDt[i] = 2.0 * c.Cm / dt;
+        s.EmByRm[i] = EmGmThev;
+        if (c.inject != 0.0) { s.hasInject[i] = 1; s.injBasal[i] = c.inject; }
+        for (int ch : adj.axial[dc]) {
+            int h = s.descToHines[ch];
+            if (h < 0) { err = "child compartment outside the walked tree"; return HSG_E_MODEL; }
+            children[i].push_back(h);
+        }
+        s.Ra[i] = c.Ra; s.Rm[i] = c.Rm; s.Cm[i] = c.Cm; s.Em[i] = c.Em; s.initVm[i] = c.initVm;
+    }
+
+    // HinesMatrix::setup: Ga, makeJunctions, makeMatrix
+    std::vector<double> Ga(n);
+    for (int i = 0; i < n; ++i) Ga[i] = 2.0 / s.Ra[i];
+    std::vector<std::vector<int>> coupled;
+    for (int i = 0; i < n; ++i) {
+        const auto& c = children[i];
+        if (c.empty()) continue;
+        if (c.size() == 1 && (c[0] - i == 1 || c[0] - i == -1)) continue;
+        std::vector<int> g = c;
+        g.push_back(i);
+        std::sort(g.begin(), g.end());
+        coupled.push_back(std::move(g));
+    }
+    std::stable_sort(coupled.begin(), coupled.end(),
+                     [](const std::vector<int>& a, const std::vector<int>& b) { return a[0] < b[0]; });
+    std::vector<int> junctionGroup(n, -1);
+    for (int g = 0; g < (int)coupled.size(); ++g)
+        for (size_t p = 0; p + 1 < coupled[g].size(); ++p) junctionGroup[coupled[g][p]] = g;
+
+    std::vector<double> HS2(n);
+    s.Off.assign(n, 0.0);
+    for (int i = 0; i < n; ++i) HS2[i] = s.Cm[i] / (dt / 2.0) + 1.0 / s.Rm[i];
+    for (int i = 0; i + 1 < n; ++i) {
+        if (junctionGroup[i] >= 0) continue;
+        double gi = Ga[i], gj = Ga[i + 1];
+        double gij = gi * gj / (gi + gj);
+        s.Off[i] = -gij;
+        HS2[i] += gij;
+        HS2[i + 1] += gij;
+    }
+    for (const auto& g : coupled) {
+        double gsum = 0.0;
+        for (int x : g) gsum += Ga[x];
+        for (int x : g) { double gi = Ga[x]; HS2[x] += gi * (1.0 - gi / gsum); }
+    }
+    s.Dbase = HS2;
+    s.HJ0.clear();
+    s.groups.clear();
+    s.maxGroup = 0;
+    for (const auto& g : coupled) {
+        double gsum = 0.0;
+        for (int x : g) gsum += Ga[x];
+        Group G;
+        G.mem = g;
+        G.hjOff = (int)s.HJ0.size();
+        G.deferLast = -1;
+        for (size_t a = 0; a + 1 < g.size(); ++a)
+            for (size_t b = a + 1; b < g.size(); ++b) {
+                double gij = Ga[g[a]] * Ga[g[b]] / gsum;
+                s.HJ0.push_back(-gij);
+                s.HJ0.push_back(-gij);
+            }
+        s.maxGroup = std::max(s.maxGroup, (int)g.size());
+        s.groups.push_back(std::move(G));
+    }
+    rc = schedule(s, junctionGroup, err);
+    if (rc) return rc;
+
+    // readHHChannels / readGates
+    std::vector<std::vector<int>> chans(nd);
+    for (int k = 0; k < d->n_channels; ++k) {
+        int c = d->channels[k].compartment;
+        if (c < 0 || c >= nd) { err = "channel on a missing compartment"; return HSG_E_ARG; }
+        chans[c].push_back(k);
+    }
+    const int nch = d->n_channels;
+    s.chanCompt.assign(nch, -1); s.chanSlot.assign(nch, -1); s.chanState.assign(nch, -1);
+    s.chanXp.assign(nch, 0); s.chanYp.assign(nch, 0); s.chanZp.assign(nch, 0);
+    int maxSlots = 0, maxGates = 0;
+    std::vector<int> chanOrder;                 // channelId_ order
+    for (int i = 0; i < n; ++i) {
+        const auto& cl = chans[s.hinesToDesc[i]];
+        maxSlots = std::max(maxSlots, (int)cl.size());
+        int g = 0;
+        for (int k : cl) {
+            const hsg_channel_desc& c = d->channels[k];
+            g += (c.Xpower > 0.0) + (c.Ypower > 0.0) + (c.Zpower > 0.0);
+            chanOrder.push_back(k);
+        }
+        maxGates = std::max(maxGates, g);
+    }
+    if (maxSlots > 8) { err = "more than 8 HHChannels on one compartment"; return HSG_E_UNSUPPORTED; }
+    if (maxGates > 24) { err = "more than 24 gates on one compartment"; return HSG_E_UNSUPPORTED; }
+    s.chanSlots = maxSlots;
+    s.gateSlots = maxGates;
+    // gates in channelId_ order, gCaDepend_ marks a channel's last gate
+    std::vector<int> gateOf;        // desc gate index per gate entry
+    std::vector<char> caDep;
+    std::vector<int> gateChan, gateWhich;
+    for (int k : chanOrder) {
+        const hsg_channel_desc& c = d->channels[k];
+        const double pw[3] = {c.Xpower, c.Ypower, c.Zpower};
+        for (int w = 0; w < 3; ++w)
+            if (pw[w] > 0.0) {
+                if (c.gate[w] < 0 || c.gate[w] >= d->n_gates || !d->gates) {
+                    err = "channel gate with power > 0 has no HHGate";
+                    return HSG_E_MODEL;
+                }
+                gateOf.push_back(c.gate[w]);
+                caDep.push_back(0);
+                gateChan.push_back(k);
+                gateWhich.push_back(w);
+            }
+        if (c.useConcentration && !caDep.empty()) caDep.back() = 1;
+    }
+
+    // readCalcium
+    const int npd = d->n_pools;
+    s.poolCompt.assign(npd, -1);
+    s.poolSlot.assign(npd, -1);
+    s.poolCount.assign(n, 0);
+    std::vector<int> caTgtLocal(nch, -1), caDepLocal(nch, -1);
+    int maxPools = 0;
+    {
+        std::vector<int> caConcIndex(npd, -1);
+        for (int i = 0; i < n; ++i) {
+            for (int k : chans[s.hinesToDesc[i]]) {
+                const hsg_channel_desc& c = d->channels[k];
+                int ids[2], nid = 0;
+                if (c.caTarget >= 0) ids[nid++] = c.caTarget;
+                if (c.caDepend >= 0) ids[nid++] = c.caDepend;
+                for (int q = 0; q < nid; ++q) {
+                    int p = ids[q];
+                    if (p >= npd) { err = "channel refers to a missing CaConc"; return HSG_E_ARG; }
+                    if (caConcIndex[p] < 0) {
+                        caConcIndex[p] = s.poolCount[i]++;
+                        s.poolCompt[p] = i;
+                        s.poolSlot[p] = caConcIndex[p];
+                    } else if (s.poolCompt[p] != i) {
+                        err = "a CaConc pool shared by channels of two compartments";
+                        return HSG_E_UNSUPPORTED;
+                    }
+                }
+                if (c.caTarget >= 0) caTgtLocal[k] = caConcIndex[c.caTarget];
+                if (c.caDepend >= 0) caDepLocal[k] = caConcIndex[c.caDepend];
+            }
+            maxPools = std::max(maxPools, s.poolCount[i]);
+        }
+    }
+    if (maxPools > 4) { err = "more than 4 CaConc pools on one compartment"; return HSG_E_UNSUPPORTED; }
+    s.poolSlots = maxPools;
+    const size_t PS = (size_t)maxPools * n;
+    s.pC.assign(PS, 0.0); s.pCa.assign(PS, 0.0); s.pBasal.assign(PS, 0.0);
+    s.pF1.assign(PS, 0.0); s.pF2.assign(PS, 0.0); s.pCeil.assign(PS, 0.0); s.pFloor.assign(PS, 0.0);
+    for (int p = 0; p < npd; ++p) {
+        if (s.poolCompt[p] < 0) continue;
+        const hsg_caconc_desc& pc = d->pools[p];
+        const size_t o = (size_t)s.poolSlot[p] * n + s.poolCompt[p];
+        // CaConcStruct(Ca, CaBasal, tau, B, ceiling, floor, dt) (HSolveStruct.cpp:103-136)
+        s.pBasal[o] = pc.CaBasal;
+        s.pC[o] = pc.Ca - pc.CaBasal;
+        s.pF1[o] = 4.0 / (2.0 + dt / pc.tau) - 1.0;
+        s.pF2[o] = 2.0 * pc.B * dt / (2.0 + dt / pc.tau);
+        s.pCeil[o] = pc.ceiling;
+        s.pFloor[o] = pc.floor;
+        s.pCa[o] = 0.0;   // ca_ stays zero until reinit
+    }
+
+    // createLookupTables
+    std::vector<int> vGate, caGate;
+    for (size_t g = 0; g < gateOf.size(); ++g) (caDep[g] ? caGate : vGate).push_back(gateOf[g]);
+    auto uniq = [&](std::vector<int>& v) {
+        std::stable_sort(v.begin(), v.end(), [&](int a, int b) { return d->gates[a].id < d->gates[b].id; });
+        std::vector<int> u;
+        for (int x : v) if (u.empty() || d->gates[u.back()].id != d->gates[x].id) u.push_back(x);
+        v = u;
+    };
+    uniq(vGate);
+    uniq(caGate);
+    std::map<int64_t, int> species;
+    for (size_t i = 0; i < caGate.size(); ++i) species[d->gates[caGate[i]].id] = (int)i;
+    for (size_t i = 0; i < vGate.size(); ++i) species[d->gates[vGate[i]].id] = (int)i;
+    auto grid = [&](const std::vector<int>& gs, double& mn, double& mx, int& div) {
+        mn = DBL_MAX;
+        mx = DBL_MIN;
+        double ddx = DBL_MAX;
+        for (int g : gs) {
+            const hsg_gate_desc& G = d->gates[g];
+            if (G.divs == 0 || !G.tableA || !G.tableB) return false;
+            double dx = (G.max - G.min) / G.divs;
+            if (G.min < mn) mn = G.min;
+            if (G.max > mx) mx = G.max;
+            if (dx < ddx) ddx = dx;
+        }
+        double dv = (mx - mn) / ddx;
+        div = (int)(dv + 0.5);
+        return true;
+    };
+    double vMin, vMax, caMin, caMax;
+    int vDiv, caDiv;
+    if (!grid(caGate, caMin, caMax, caDiv) || !grid(vGate, vMin, vMax, vDiv)) {
+        err = "HHGate without tables";
+        return HSG_E_MODEL;
+    }
+    if (vDiv < 0 || vDiv > (1 << 22) || caDiv < 0 || caDiv > (1 << 22)) {
+        err = "lookup table grid out of range";
+        return HSG_E_MODEL;
+    }
+    make_table(s.tables.ca, caMin, caMax, (unsigned)caDiv, (unsigned)caGate.size());
+    make_table(s.tables.v, vMin, vMax, (unsigned)vDiv, (unsigned)vGate.size());
+    std::vector<double> A, B;
+    for (size_t i = 0; i < caGate.size(); ++i) {
+        rates(d->gates[caGate[i]], caMin, caMax, (unsigned)caDiv, A, B);
+        add_columns(s.tables.ca, (int)i, A, B);
+    }
+    for (size_t i = 0; i < vGate.size(); ++i) {
+        rates(d->gates[vGate[i]], vMin, vMax, (unsigned)vDiv, A, B);
+        add_columns(s.tables.v, (int)i, A, B);
+    }
+    {
+        uint64_t h = 1469598103934665603ull;
+        for (const HostTable* t : {&s.tables.v, &s.tables.ca}) {
+            h = fnv(&t->min, sizeof(double), h);
+            h = fnv(&t->max, sizeof(double), h);
+            h = fnv(&t->nDivs, sizeof(uint32_t), h);
+            h = fnv(&t->nSpecies, sizeof(uint32_t), h);
+            h = fnv(t->table.data(), t->table.size() * sizeof(double), h);
+        }
+        char buf[64];
+        snprintf(buf, sizeof buf, "%016llx:%zu", (unsigned long long)h, s.tables.v.table.size());
+        s.tables.key = buf;
+    }
+    // column of each gate entry, then per channel X/Y/Z columns
+    std::vector<int> colX(nch, 0), colY(nch, 0), colZ(nch, 0);
+    for (size_t g = 0; g < gateOf.size(); ++g) {
+        int col = 2 * species[d->gates[gateOf[g]].id];
+        int k = gateChan[g];
+        if (gateWhich[g] == 0) colX[k] = col;
+        else if (gateWhich[g] == 1) colY[k] = col;
+        else colZ[k] = col;
+    }
+    // the reference reads the vTable at a Ca column when a Ca-dependent Z gate
+    // has no pool and no external Ca (HSolveActive.cpp:304-312); that read
+    // must stay inside the vTable
+    for (int k = 0; k < nch; ++k) {
+        const hsg_channel_desc& c = d->channels[k];
+        if (c.Zpower > 0.0 && caDepLocal[k] < 0 && colZ[k] + 1 >= (int)s.tables.v.nColumns() &&
+            !(colZ[k] + 1 < (int)s.tables.ca.nColumns())) {
+            err = "Ca-dependent Z gate with no pool: its column lies outside both tables";
+            return HSG_E_UNSUPPORTED;
+        }
+        if (c.Xpower > 0.0 && colX[k] + 1 >= (int)s.tables.v.nColumns()) {
+            err = "X gate column outside the voltage table (Ca-dependent X gate)";
+            return HSG_E_UNSUPPORTED;
+        }
+        if (c.Ypower > 0.0 && colY[k] + 1 >= (int)s.tables.v.nColumns()) {
+            err = "Y gate column outside the voltage table (Ca-dependent Y gate)";
+            return HSG_E_UNSUPPORTED;
+        }
+    }
+
+    // channel slots, local types
+    const size_t CS = (size_t)maxSlots * n;
+    s.chTypeLocal.assign(CS, -1);
+    s.Gbar.assign(CS, 0.0); s.Ek.assign(CS, 0.0); s.mod.assign(CS, 1.0);
+    s.state.assign((size_t)maxGates * n, 0.0);
+    s.localTypes.clear();
+    std::map<std::string, int> typeIds;
+    s.hasModulation = false;
+    for (int i = 0; i < n; ++i) {
+        int k = 0, js = 0;
+        for (int dk : chans[s.hinesToDesc[i]]) {
+            const hsg_channel_desc& c = d->channels[dk];
+            HostChanType t;
+            t.Xpower = c.Xpower; t.Ypower = c.Ypower; t.Zpower = c.Zpower;
+            t.instant = c.instant;
+            t.nGates = (c.Xpower > 0.0) + (c.Ypower > 0.0) + (c.Zpower > 0.0);
+            t.colX = c.Xpower > 0.0 ? colX[dk] : 0;
+            t.colY = c.Ypower > 0.0 ? colY[dk] : 0;
+            t.colZ = c.Zpower > 0.0 ? colZ[dk] : 0;
+            t.caDepLocal = c.Zpower > 0.0 ? caDepLocal[dk] : -1;
+            t.caTgtLocal = caTgtLocal[dk];
+            t.tableKey = s.tables.key;
+            char kb[256];
+            snprintf(kb, sizeof kb, "%a|%a|%a|%d|%d|%d|%d|%d|%d", t.Xpower, t.Ypower, t.Zpower,
+                     t.instant, t.colX, t.colY, t.colZ, t.caDepLocal, t.caTgtLocal);
+            auto it = typeIds.find(kb);
+            int tid;
+            if (it == typeIds.end()) {
+                tid = (int)s.localTypes.size();
+                typeIds[kb] = tid;
+                s.localTypes.push_back(t);
+            } else tid = it->second;
+            const size_t o = (size_t)k * n + i;
+            s.chTypeLocal[o] = tid;
+            s.Gbar[o] = c.Gbar;
+            s.Ek[o] = c.Ek;
+            s.mod[o] = c.modulation;
+            if (c.modulation != 1.0) s.hasModulation = true;
+            s.chanCompt[dk] = i;
+            s.chanSlot[dk] = k;
+            s.chanState[dk] = js;
+            s.chanXp[dk] = c.Xpower; s.chanYp[dk] = c.Ypower; s.chanZp[dk] = c.Zpower;
+            if (c.Xpower > 0.0) s.state[(size_t)(js++) * n + i] = c.X;
+            if (c.Ypower > 0.0) s.state[(size_t)(js++) * n + i] = c.Y;
+            if (c.Zpower > 0.0) s.state[(size_t)(js++) * n + i] = c.Z;
+            ++k;
+        }
+    }
+    // uniform cell: every compartment carries the same type in every slot
+    s.uniform = true;
+    s.slotTypeUniform.assign(maxSlots, -1);
+    for (int k = 0; k < maxSlots && s.uniform; ++k) {
+        int t0 = s.chTypeLocal[(size_t)k * n];
+        for (int i = 1; i < n; ++i)
+            if (s.chTypeLocal[(size_t)k * n + i] != t0) { s.uniform = false; break; }
+        s.slotTypeUniform[k] = t0;
+    }
+
+    // SpikeGens in Hines order of their compartments
+    std::vector<std::vector<int>> spk(nd);
+    for (int k = 0; k < d->n_spikegens; ++k) {
+        int c = d->spikegens[k].compartment;
+        if (c < 0 || c >= nd) { err = "SpikeGen on a missing compartment"; return HSG_E_ARG; }
+        spk[c].push_back(k);
+    }
+    for (int i = 0; i < n; ++i)
+        for (int k : spk[s.hinesToDesc[i]]) {
+            s.spkCompt.push_back(i);
+            s.spkDesc.push_back(k);
+            s.spkEdge.push_back(d->spikegens[k].edgeTriggered);
+            s.spkThr.push_back(d->spikegens[k].threshold);
+            s.spkRefr.push_back(d->spikegens[k].refractT);
+        }
+    return 0;
+}
+
+}  // namespace hsg

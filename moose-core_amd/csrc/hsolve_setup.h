@@ -1,0 +1,94 @@
+// hsolve_setup.h — host-side model reading for one cell: the work of
+// HSolve::setup (hsolve/HSolve.cpp:286-293) turned into the flat, Hines-ordered
+// arrays and the parallel elimination schedule that the GPU kernels run.
+#pragma once
+#include <stdint.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/hsolve_amd.h"
+
+namespace hsg {
+
+// One lookup table as HSolveActive::createLookupTables builds it
+// (hsolve/HSolveActiveSetup.cpp:339-526, hsolve/RateLookup.cpp:15-55).
+struct HostTable {
+    double min = 0, max = 0, dx = 0;
+    uint32_t nDivs = 0, nSpecies = 0;
+    std::vector<double> table;   // [nDivs+2][2*nSpecies]
+    uint32_t nPts() const { return nDivs + 2; }
+    uint32_t nColumns() const { return 2 * nSpecies; }
+};
+
+struct HostTableSet {
+    HostTable v, ca;
+    std::string key;
+};
+
+struct HostChanType {
+    double Xpower, Ypower, Zpower;
+    int instant, nGates, colX, colY, colZ, caDepLocal, caTgtLocal;
+    std::string tableKey;
+};
+
+// Elimination unit: linear link (node) or junction group.
+struct Unit {
+    int kind;        // 0 linear, 1 group
+    int node;        // linear: eliminated node x (x -> x+1)
+    int group;       // group index
+    int level;
+};
+
+struct Group {
+    std::vector<int> mem;   // sorted Hines indices
+    int hjOff;
+    int deferLast;          // term offset for the last member, -1
+};
+
+struct CellSetup {
+    int n = 0;
+    double dt = 0;                    // setup dt (HinesMatrix::dt_)
+    int caAdvance = 1;
+    std::vector<int> hinesToDesc, descToHines;
+    // tree_ and compartment_ (hsolve/HSolvePassive.cpp:100-205)
+    std::vector<double> Ra, Rm, Cm, Em, initVm, V, CmByDt, EmByRm;
+    std::vector<double> injBasal;     // per Hines index
+    std::vector<char> hasInject;
+    // Hines matrix (hsolve/HinesMatrix.cpp:136-222)
+    std::vector<double> Dbase, Off, HJ0;
+    std::vector<Group> groups;
+    std::vector<int> units;           // encoded, level-ordered
+    std::vector<int> levelStart;
+    std::vector<int> defer;           // 2n: (offset, count)
+    int nTerm = 0;
+    int maxGroup = 0;
+    // channels, slot-major
+    int chanSlots = 0, gateSlots = 0, poolSlots = 0;
+    bool uniform = true;
+    std::vector<int> slotTypeUniform;        // per slot when uniform
+    std::vector<int> chType;                 // [k*n + i] engine-wide type id, -1
+    std::vector<double> Gbar, Ek, mod, state;  // slot-major
+    bool hasModulation = false;
+    // desc channel -> (compartment Hines index, slot k, first state slot j)
+    std::vector<int> chanCompt, chanSlot, chanState;
+    std::vector<double> chanXp, chanYp, chanZp;
+    // pools
+    std::vector<int> poolCount;              // per compartment
+    std::vector<int> poolCompt, poolSlot;    // per desc pool (-1 unused)
+    std::vector<double> pC, pCa, pBasal, pF1, pF2, pCeil, pFloor;  // slot-major
+    // spike generators (Hines order of their compartments)
+    std::vector<int> spkCompt, spkDesc;
+    std::vector<int> spkEdge;
+    std::vector<double> spkThr, spkRefr;
+    // types used (engine-wide ids filled by the engine)
+    std::vector<HostChanType> localTypes;    // indexed by local type id
+    std::vector<int> chTypeLocal;            // [k*n+i] local type id
+    HostTableSet tables;
+};
+
+// Reads one cell.  Returns 0 or an HSG_E_* code with *err set.
+int setup_cell(const hsg_cell_desc* d, double dt, int caAdvance, CellSetup& out, std::string& err);
+
+}  // namespace hsg

@@ -1,0 +1,246 @@
+"""GPU parity: the HIP path (called through the C ABI) against the oracle.
+
+FP64 form: bit-for-bit equal to the oracle (which is itself bit-exact with the
+reference's hsolve sources), i.e. max|dVm| = 0 and identical spike steps; the
+north-star bound is max|dVm| <= 1e-6 V with identical spike steps.
+Mixed form (FP32 gates/tables/currents, FP64 Hines): max|dVm| <= 1e-4 V
+(0.1 mV) and spike steps within one step of the oracle's.
+"""
+import numpy as np
+import pytest
+
+from moose_core_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-6     # V, north-star bound for the FP64 form (observed: 0)
+MIXED_TOL = 1e-4    # V, 0.1 mV
+
+
+def run_gpu(pkg, specs, dt, nsteps, precision="fp64", chunks=None, **opts):
+    eng = pkg.Engine(precision, **opts)
+    ids = [eng.add_cell(s, dt) for s in specs]
+    for cid, s in zip(ids, specs):
+        eng.set_probes(cid, np.arange(s.n), nsteps)
+    eng.reinit(dt)
+    for k in (chunks or [nsteps]):
+        eng.step(k)
+    traces = [eng.read_probes(cid) for cid in ids]
+    spikes = eng.read_spikes()
+    eng.close()
+    return ids, traces, spikes
+
+
+def run_oracle(oracle_mod, spec, dt, nsteps):
+    c = oracle_mod.OracleCell(spec, dt)
+    c.reinit(dt)
+    tr = c.run(dt, nsteps, np.arange(spec.n))
+    g, s = c.read_spikes()
+    return c, tr, s
+
+
+def test_fp64_ball_tree_bit_exact(pkg, oracle_mod):
+    print("case: ball tree 128 compartments, 1000 steps, fp64", flush=True)
+    spec = workloads.hh_ball_tree(inject=1e-9)
+    dt, n = 50e-6, 1000
+    _, (gpu,), spikes = run_gpu(pkg, [spec], dt, n)
+    _, ref, ref_spk = run_oracle(oracle_mod, spec, dt, n)
+    assert np.abs(gpu - ref).max() <= FP64_TOL
+    np.testing.assert_array_equal(gpu, ref)
+    assert list(spikes["step"]) == list(ref_spk)
+
+
+def mixed_batch():
+    rng = np.random.default_rng(7)
+    specs = [workloads.random_hh_cell(rng, int(n), inject=float(rng.uniform(0.3e-9, 1.5e-9)))
+             for n in (50, 77, 130, 260, 400, 1)]
+    specs += [workloads.random_ca_cell(rng, int(n), inject=float(rng.uniform(0.3e-9, 1.0e-9)))
+              for n in (40, 150, 333)]
+    specs.append(workloads.hh_ball_tree(inject=0.6e-9))
+    return specs
+
+
+def test_fp64_batch_bit_exact(pkg, oracle_mod):
+    print("case: 10-cell batch (HH + Ca), 600 steps, fp64", flush=True)
+    specs = mixed_batch()
+    dt, n = 25e-6, 600
+    ids, traces, spikes = run_gpu(pkg, specs, dt, n)
+    for cid, spec, gpu in zip(ids, specs, traces):
+        _, ref, ref_spk = run_oracle(oracle_mod, spec, dt, n)
+        assert np.abs(gpu - ref).max() <= FP64_TOL, cid
+        np.testing.assert_array_equal(gpu, ref)
+        mine = spikes[spikes["cell"] == cid]
+        assert list(mine["step"]) == list(ref_spk), cid
+
+
+def test_mixed_batch_within_tolerance(pkg, oracle_mod):
+    print("case: 10-cell batch, 600 steps, mixed", flush=True)
+    specs = mixed_batch()
+    dt, n = 25e-6, 600
+    ids, traces, spikes = run_gpu(pkg, specs, dt, n, precision="mixed")
+    worst = 0.0
+    for cid, spec, gpu in zip(ids, specs, traces):
+        _, ref, ref_spk = run_oracle(oracle_mod, spec, dt, n)
+        err = np.abs(gpu - ref).max()
+        worst = max(worst, err)
+        assert err <= MIXED_TOL, (cid, err)
+        mine = list(spikes[spikes["cell"] == cid]["step"])
+        assert len(mine) == len(ref_spk), cid
+        assert all(abs(a - b) <= 1 for a, b in zip(mine, ref_spk)), cid
+    print(f"mixed: worst max|dVm| = {worst:.3e} V")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_alone_equals_in_mix(pkg, precision):
+    print(f"case: alone vs in the mix, {precision}", flush=True)
+    specs = mixed_batch()
+    dt, n = 25e-6, 300
+    pick = 3
+    _, alone, _ = run_gpu(pkg, [specs[pick]], dt, n, precision=precision)
+    # another order and composition around it
+    mix = specs[::-1]
+    ids, traces, _ = run_gpu(pkg, mix, dt, n, precision=precision)
+    np.testing.assert_array_equal(alone[0], traces[mix.index(specs[pick])])
+
+
+@pytest.mark.parametrize("opts,chunks", [
+    ({"steps_per_launch": 7}, [100, 1, 199]),
+    ({"steps_per_launch": 10, "graph_chunk": 4}, [300]),
+    ({"block_threads": 64}, [300]),
+    ({"block_threads": 1024, "steps_per_launch": 300}, [150, 150]),
+])
+def test_launch_shapes_and_split_runs(pkg, oracle_mod, opts, chunks):
+    print(f"case: launch shape {opts} chunks {chunks}", flush=True)
+    specs = mixed_batch()[:5]
+    dt, n = 25e-6, sum(chunks)
+    ids, traces, _ = run_gpu(pkg, specs, dt, n, chunks=chunks, **opts)
+    for spec, gpu in zip(specs, traces):
+        _, ref, _ = run_oracle(oracle_mod, spec, dt, n)
+        np.testing.assert_array_equal(gpu, ref)
+
+
+def test_large_cells_global_work(pkg, oracle_mod):
+    """n = 2000 (work arrays in LDS) and n = 6000 (work arrays in HBM)."""
+    print("case: large cells 2000/6000 compartments, 100 steps, fp64", flush=True)
+    rng = np.random.default_rng(9)
+    specs = [workloads.random_hh_cell(rng, 2000, inject=2e-9), workloads.random_ca_cell(rng, 6000, inject=2e-9)]
+    dt, n = 25e-6, 100
+    ids, traces, _ = run_gpu(pkg, specs, dt, n)
+    for spec, gpu in zip(specs, traces):
+        _, ref, _ = run_oracle(oracle_mod, spec, dt, n)
+        np.testing.assert_array_equal(gpu, ref)
+
+
+def test_fields_and_inputs(pkg, oracle_mod):
+    """Zombie field reads and writes, and the per-step inputs, between steps."""
+    print("case: fields and inputs", flush=True)
+    rng = np.random.default_rng(17)
+    spec = workloads.random_ca_cell(rng, 80, inject=0.4e-9)
+    dt = 20e-6
+    eng = pkg.Engine("fp64")
+    cid = eng.add_cell(spec, dt)
+    eng.reinit(dt)
+    ref = oracle_mod.OracleCell(spec, dt)
+    ref.reinit(dt)
+    allc = np.arange(spec.n)
+    allch = np.arange(len(spec.channels))
+    pools = np.arange(len(spec.pools))
+
+    def compare():
+        for f in ("Vm", "Im", "inject", "initVm", "Cm", "Em", "Rm", "Ra"):
+            np.testing.assert_array_equal(eng.get(cid, f, allc), ref.get(f, allc), err_msg=f)
+        for f in ("Gbar", "Ek", "Gk", "Ik", "X", "Y", "Z", "modulation", "instant"):
+            np.testing.assert_array_equal(eng.get(cid, f, allch), ref.get(f, allch), err_msg=f)
+        for f in ("Ca", "CaBasal", "ceiling", "floor"):
+            np.testing.assert_array_equal(eng.get(cid, f, pools), ref.get(f, pools), err_msg=f)
+
+    def both(fn):
+        fn(eng, ref)
+
+    eng.step(50)
+    ref.run(dt, 50)
+    compare()
+    # per-step inputs: consumed by the next step only
+    eng.add_inject(cid, 3, 2e-10)
+    ref.add_inject(3, 2e-10)
+    eng.add_gkek(cid, 5, 1e-9, 0.02)
+    ref.add_gkek(5, 1e-9, 0.02)
+    eng.add_gkek(cid, 5, 2e-9, -0.05)
+    ref.add_gkek(5, 2e-9, -0.05)
+    eng.add_ica(cid, 2, 1e-12)
+    ref.add_ica(2, 1e-12)
+    eng.step(1)
+    ref.run(dt, 1)
+    compare()      # Im includes the previous step's external current
+    eng.step(30)
+    ref.run(dt, 30)
+    compare()
+    # field writes
+    eng.set(cid, "inject", [0, 7], [1e-9, 3e-10])
+    ref.set("inject", [0, 7], [1e-9, 3e-10])
+    eng.set(cid, "Vm", [1], [-0.04])
+    ref.set("Vm", [1], [-0.04])
+    eng.set(cid, "Cm", [2], [2e-11])
+    ref.set("Cm", [2], [2e-11])
+    eng.set(cid, "Em", [4], [-0.07])
+    ref.set("Em", [4], [-0.07])
+    eng.set(cid, "Rm", [6], [5e9])
+    ref.set("Rm", [6], [5e9])
+    eng.set(cid, "Gbar", [0, 9], [1e-9, 2e-9])
+    ref.set("Gbar", [0, 9], [1e-9, 2e-9])
+    eng.set(cid, "Ek", [1], [-0.09])
+    ref.set("Ek", [1], [-0.09])
+    eng.set(cid, "X", [0], [0.5])
+    ref.set("X", [0], [0.5])
+    eng.set(cid, "modulation", [3, 4], [1.5, -1.0])
+    ref.set("modulation", [3, 4], [1.5, -1.0])
+    eng.set(cid, "Ca", [0], [0.002])
+    ref.set("Ca", [0], [0.002])
+    eng.set(cid, "CaBasal", [1], [0.003])
+    ref.set("CaBasal", [1], [0.003])
+    eng.set(cid, "ceiling", [2], [0.004])
+    ref.set("ceiling", [2], [0.004])
+    eng.add_conc(cid, 4, 0.01)        # channel 4 is a K-AHP of compartment 0
+    ref.add_conc(4, 0.01)
+    eng.set(cid, "instant", [10], [1])
+    ref.set("instant", [10], [1])
+    compare()
+    eng.step(100)
+    ref.run(dt, 100)
+    compare()
+    eng.close()
+
+
+def test_hines_order_and_info(pkg, oracle_mod):
+    spec = workloads.random_ca_cell(np.random.default_rng(4), 300)
+    eng = pkg.Engine("fp64")
+    cid = eng.add_cell(spec, 25e-6)
+    assert np.array_equal(eng.hines_order(cid, spec.n), oracle_mod.OracleCell(spec, 25e-6).hines_order())
+    info = eng.cell_info(cid)
+    assert info["n_compartments"] == 300 and info["n_levels"] > 0 and info["max_group"] >= 3
+    eng.close()
+
+
+def test_remove_and_readmit(pkg, oracle_mod):
+    print("case: remove / re-admit cells", flush=True)
+    rng = np.random.default_rng(23)
+    specs = [workloads.random_hh_cell(rng, int(n), inject=1e-9) for n in (90, 150, 60)]
+    dt = 25e-6
+    eng = pkg.Engine("fp64")
+    a = eng.add_cell(specs[0], dt)
+    b = eng.add_cell(specs[1], dt)
+    eng.reinit(dt)
+    eng.step(40)
+    eng.remove_cell(a)
+    c = eng.add_cell(specs[2], dt)
+    eng.reinit_cell(c)
+    eng.set_probes(c, np.arange(specs[2].n), 200)
+    eng.set_probes(b, np.arange(specs[1].n), 200)
+    eng.step(200)
+    gc = eng.read_probes(c)
+    gb = eng.read_probes(b)
+    eng.close()
+    _, rc, _ = run_oracle(oracle_mod, specs[2], dt, 200)
+    np.testing.assert_array_equal(gc, rc)
+    _, rb, _ = run_oracle(oracle_mod, specs[1], dt, 240)
+    np.testing.assert_array_equal(gb, rb[40:])
