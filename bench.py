@@ -1,0 +1,212 @@
+"""Benchmark: compartment-steps/s of the batched HSolve ensemble on one MI355X.
+
+Workload (the batched-ensemble configuration): 4096 independent cells, each a
+random binary tree (recursive uniform splitting) with n ~ U{50..2000}
+compartments, branch lengths U[5, 50] um, diameters tapering 2 -> 0.5 um,
+squid HH Na/K in every compartment with Gbar scaled by U[0.8, 1.2] per cell,
+soma inject U[0, 0.3] nA, dt = 25 us.  All cells advance together in one
+persistent fused launch per `steps_per_launch` ticks.
+
+Timed: `--steps` ticks of the whole ensemble (default 5000, the full run),
+after `--warmup` untimed ticks, bracketed by device synchronisation.  The
+headline `value` is the mixed-precision form (FP32 gates/tables/currents, FP64
+Hines solve and Vm); the FP64 form (bit-identical to the oracle) is run the
+same way and reported beside it, with the accuracy of the mixed form against
+it (soma Vm of every cell, every step).
+
+The CPU baseline is the oracle (the bit-exact C restatement of the reference
+HSolve) on a bounded sample of the same cells, one cell after another on one
+core and sharded over all host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+import __graft_entry__  # noqa: E402
+
+pkg = __graft_entry__.import_package()
+from moose_core_amd import workloads  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per compartment-step (state-resident between steps,
+# tables excluded): V r+w 16, gate states r+w 2*w*3, Gbar/Ek 2*w*2,
+# CmByDt/EmByRm/diag/offdiag 32 (FP64), w = 8 (FP64) or 4 (mixed)
+BYTES_PER_CS = {"fp64": 16 + 2 * 8 * 3 + 2 * 8 * 2 + 32, "mixed": 16 + 2 * 4 * 3 + 2 * 4 * 2 + 32}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_cells(ncells, seed):
+    t = time.time()
+    cells = workloads.hh_ensemble(ncells, 50, 2000, seed)
+    log(f"[bench] built {ncells} cells ({sum(c.n for c in cells)} compartments) in {time.time() - t:.1f}s")
+    return cells
+
+
+def run_form(precision, packed, dt, steps, warmup, spl, graph, threads):
+    eng = pkg.Engine(precision, steps_per_launch=spl, graph_chunk=graph, block_threads=threads)
+    t = time.time()
+    ids = [eng.add_cell(p, dt) for p in packed]
+    log(f"[bench] {precision}: {len(ids)} cells set up in {time.time() - t:.1f}s")
+    for cid in ids:
+        eng.set_probes(cid, [0], steps)
+    eng.reinit(dt)
+    if warmup:
+        eng.step(warmup)
+        for cid in ids:
+            eng.read_probes(cid)
+        eng.read_spikes()
+    eng.sync()
+    t0 = time.perf_counter()
+    eng.step(steps)
+    eng.sync()
+    wall = time.perf_counter() - t0
+    launches, total_ms, mean_ms = eng.last_timing()
+    soma = np.stack([eng.read_probes(cid)[:, 0] for cid in ids], axis=1)
+    spikes = eng.read_spikes()
+    eng.close()
+    log(f"[bench] {precision}: {steps} steps in {wall:.3f}s, {launches} launches, mean launch {mean_ms:.3f} ms")
+    return wall, launches, total_ms, mean_ms, soma, spikes
+
+
+def spike_steps(spikes, ncells):
+    out = [[] for _ in range(ncells)]
+    for r in spikes:
+        out[int(r["cell"])].append(int(r["step"]))
+    return out
+
+
+def cpu_baseline(cells, dt, budget_s=15.0):
+    """Oracle on the host: single core (cells one after another) and all
+    cores (cells sharded over threads; ctypes releases the GIL)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    sample = cells[:64]
+    probe_steps = 20
+    # calibrate: compartment-steps/s on one core
+    c0 = oracle.OracleCell(sample[0], dt)
+    c0.reinit(dt)
+    t = time.perf_counter()
+    c0.run(dt, probe_steps)
+    rate1 = sample[0].n * probe_steps / max(time.perf_counter() - t, 1e-9)
+    total_cs = budget_s * rate1 / 2
+    nsteps = int(max(10, min(5000, total_cs / sum(c.n for c in sample))))
+    objs = []
+    for c in sample:
+        o = oracle.OracleCell(c, dt)
+        o.reinit(dt)
+        objs.append(o)
+    t = time.perf_counter()
+    for o in objs:
+        o.run(dt, nsteps)
+    single = sum(c.n for c in sample) * nsteps / (time.perf_counter() - t)
+    ncores = os.cpu_count() or 1
+    chunks = [objs[i::ncores] for i in range(ncores)]
+    ths = [threading.Thread(target=lambda ch: [o.run(dt, nsteps) for o in ch], args=(ch,)) for ch in chunks]
+    t = time.perf_counter()
+    for th in ths:
+        th.start()
+    for th in ths:
+        th.join()
+    multi = sum(c.n for c in sample) * nsteps / (time.perf_counter() - t)
+    return {
+        "value": multi, "unit": "compartment-steps/s", "cores": ncores, "kind": "port",
+        "single_core_value": single,
+        "sample": f"{len(sample)} of the ensemble's cells ({sum(c.n for c in sample)} compartments) "
+                  f"x {nsteps} steps, oracle (bit-exact C restatement of HSolve), -O2 -ffp-contract=off",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--cells", type=int, default=4096)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--steps-per-launch", type=int, default=50)
+    ap.add_argument("--graph-chunk", type=int, default=0)
+    ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="mixed,fp64")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                          "(cells are independent and would shard by bin-packing across GPUs)"}))
+        sys.exit(2)
+    dt = 25e-6
+    cells = build_cells(args.cells, args.seed)
+    packed = [c.pack() for c in cells]
+    ncomp = sum(c.n for c in cells)
+    res = {}
+    for form in args.forms.split(","):
+        res[form] = run_form(form, packed, dt, args.steps, args.warmup, args.steps_per_launch,
+                             args.graph_chunk, args.threads)
+    head = "mixed" if "mixed" in res else list(res)[0]
+    wall, launches, total_ms, mean_ms, soma, spikes = res[head]
+    value = ncomp * args.steps / wall
+    # dominant kernel: the fused step kernel; algorithmic bytes per launch
+    steps_per_launch = args.steps / max(1, launches)
+    bytes_per_launch = BYTES_PER_CS[head] * ncomp * steps_per_launch
+    achieved = bytes_per_launch / (mean_ms * 1e-3) / 1e9
+    out = {
+        "metric": "compartment-steps/s (batched cells, FP32-channel mixed form)",
+        "value": value,
+        "unit": "compartment-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-channels/f64-hines" if head == "mixed" else "f64",
+        "data": "synthetic (seeded random morphologies)",
+        "config": {"workload": "batched ensemble: 4096 random binary-tree HH cells, n~U{50..2000}, "
+                               "dt=25us, 5000 steps",
+                   "cells": args.cells, "compartments": ncomp, "dt": dt,
+                   "steps_per_launch": args.steps_per_launch, "graph_chunk": args.graph_chunk,
+                   "parallelism": "1 GPU, one workgroup per cell"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "hsolve_step_kernel", "bytes_per_compartment_step": BYTES_PER_CS[head],
+                     "mean_launch_ms": mean_ms},
+    }
+    if "fp64" in res and head != "fp64":
+        w64, l64, t64, m64, soma64, spk64 = res["fp64"]
+        out["fp64_value"] = ncomp * args.steps / w64
+        out["mixed_over_fp64"] = value / out["fp64_value"]
+        d = soma - soma64
+        s_mixed = spike_steps(spikes, len(cells))
+        s_64 = spike_steps(spk64, len(cells))
+        worst = 0
+        mismatched = 0
+        for a, b in zip(s_mixed, s_64):
+            if len(a) != len(b):
+                mismatched += 1
+                continue
+            if a:
+                worst = max(worst, int(np.max(np.abs(np.array(a) - np.array(b)))))
+        out["accuracy_mixed_vs_fp64"] = {
+            "max_abs_dVm": float(np.abs(d).max()), "rms_dVm": float(np.sqrt(np.mean(d * d))),
+            "spike_step_max_err": worst, "cells_with_spike_count_mismatch": mismatched,
+            "spikes": int(sum(len(a) for a in s_64)), "probe": "soma of every cell, every step"}
+        out["fp64_roofline_frac"] = (BYTES_PER_CS["fp64"] * ncomp * (args.steps / max(1, l64))
+                                     / (m64 * 1e-3) / 1e9) / HBM_PEAK_GBS
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle) ...")
+        out["cpu_baseline"] = cpu_baseline(cells, dt)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -170,7 +170,7 @@ __device__ __forceinline__ void channel_row(char* arena, const CellDev& c, const
                 table_lookup<GT>(ctab, cCol, cb, ct.colZ, (GT)cf, C1, C2);
             } else {
                 const double ext = A_<const double>(arena, c.oExtCa)[(size_t)k * n + i];
-                if (ext > 0) {
+                if (ext > 0 && ct.colZ + 1 < cCol) {
                     int b; double f;
                     table_row(ts.ca, ext, b, f);
                     table_lookup<GT>(ctab, cCol, b, ct.colZ, (GT)f, C1, C2);

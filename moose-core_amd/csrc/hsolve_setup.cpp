@@ -550,6 +550,10 @@ int setup_cell(const hsg_cell_desc* d, double dt, int caAdvance, CellSetup& s, s
             err = "Ca-dependent Z gate with no pool: its column lies outside both tables";
             return HSG_E_UNSUPPORTED;
         }
+        if (c.Zpower > 0.0 && caDepLocal[k] >= 0 && colZ[k] + 1 >= (int)s.tables.ca.nColumns()) {
+            err = "Z gate reads a CaConc but its column lies outside the Ca table (useConcentration unset?)";
+            return HSG_E_UNSUPPORTED;
+        }
         if (c.Xpower > 0.0 && colX[k] + 1 >= (int)s.tables.v.nColumns()) {
             err = "X gate column outside the voltage table (Ca-dependent X gate)";
             return HSG_E_UNSUPPORTED;

@@ -98,31 +98,36 @@ def hh_ball_tree(depth=7, inject=0.1e-9, soma_dia=20e-6, dend_len=10e-6, dend_di
 
 def random_tree_parents(rng, n, max_children=2):
     """Parent of each node 1..n-1 (node 0 is the root): a random tree by
-    recursive uniform splitting of the remaining nodes among up to
-    `max_children` subtrees (empty subtrees allowed, so chains occur)."""
-    parent = np.full(n, -1, dtype=np.int64)
-    depth = np.zeros(n, dtype=np.int64)
-    stack = [(0, n - 1)]          # (node, nodes still to place under it)
+    recursive uniform splitting of the remaining nodes among k subtrees
+    (k = 2, or k ~ U{2..max_children}); empty subtrees are allowed, so chains
+    occur.  Random draws are taken from pre-drawn uniform arrays."""
+    parent = [-1] * n
+    depth = [0] * n
+    ku = rng.random(n).tolist()                    # children-count draws
+    cu = rng.random(n * max(1, max_children - 1)).tolist()   # cut draws
+    ci = 0
+    stack = [(0, n - 1)]
     nxt = 1
     while stack:
         node, m = stack.pop()
         if m <= 0:
             continue
-        k = int(rng.integers(2, max_children + 1)) if max_children > 2 else 2
-        cuts = np.sort(rng.integers(0, m, size=k - 1)) if k > 1 else np.array([], dtype=np.int64)
-        sizes = np.diff(np.concatenate(([0], cuts, [m - 0])))
-        # each nonempty part gets a child node carrying (size-1) descendants
-        for s in sizes:
-            s = int(s)
-            if s <= 0:
+        k = 2 if max_children <= 2 else 2 + int(ku[node] * (max_children - 1))
+        cuts = sorted(int(cu[ci + j] * m) for j in range(k - 1))
+        ci += k - 1
+        prev = 0
+        for cut in cuts + [m]:
+            size = cut - prev
+            prev = cut
+            if size <= 0:
                 continue
             c = nxt
             nxt += 1
             parent[c] = node
             depth[c] = depth[node] + 1
-            stack.append((c, s - 1))
+            stack.append((c, size - 1))
     assert nxt == n
-    return parent, depth
+    return np.array(parent, dtype=np.int64), np.array(depth, dtype=np.int64)
 
 
 def random_hh_cell(rng, n, inject=None, spikegen=True):
