@@ -147,13 +147,22 @@ struct hsg_engine {
     bool tablesDirty = true;
     std::vector<int32_t> order;
     int ldsLimit = 65536;
-    int maxLdsUsed = 0;
     double dt = 0;                // clock tick
     bool reinitDone = false;
     // timing of the last step call
     std::vector<hipEvent_t> evStart, evStop;
     int lastLaunches = 0;
     double lastTotalMs = 0;
+    // size bins of the live set, each launched on its own stream
+    struct Bin {
+        std::vector<int32_t> cells;
+        int32_t* dOrder = nullptr;
+        int lds = 0;
+        int threads = 256;
+    };
+    std::vector<Bin> bins;
+    hipStream_t binStream[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t evFork = nullptr, evJoin[3] = {nullptr, nullptr, nullptr};
     // graph of the steady-state launches
     hipGraphExec_t graphExec = nullptr;
     int graphSteps = 0;
@@ -282,9 +291,9 @@ int type_id(hsg_engine* e, const HostChanType& t, int tableSet) {
     return id;
 }
 
-int64_t work_bytes(const CellDev& c) {
-    return 8 * (2 * (int64_t)c.n + c.nHJ + 2 * (int64_t)c.nTerm);
-}
+// LDS-resident part of the elimination work: D[n], B[n], deferred terms
+int64_t work_lds_bytes(const CellDev& c) { return 8 * (2 * (int64_t)c.n + 2 * (int64_t)c.nTerm); }
+int64_t level_bytes(const CellDev& c) { return 4 * ((int64_t)c.nLevels + 1); }
 
 // Lays out and uploads one cell's main region.
 int place_cell(hsg_engine* e, HostCell& hc) {
@@ -299,7 +308,7 @@ int place_cell(hsg_engine* e, HostCell& hc) {
     std::memset(&d, 0, sizeof d);
     int64_t nGroups = (int64_t)s.groups.size(), nMem = 0;
     for (auto& g : s.groups) nMem += (int64_t)g.mem.size();
-    int64_t workB = 8 * (2 * (int64_t)n + (int64_t)s.HJ0.size() + 2 * (int64_t)s.nTerm);
+    int64_t workB = 8 * (2 * (int64_t)n + 2 * (int64_t)s.nTerm + (s.anySlowGroup ? (int64_t)s.HJ0.size() : 0));
     std::vector<Item> items = {
         {&d.oV, 8 * n}, {&d.oCmByDt, 8 * n}, {&d.oEmByRm, 8 * n}, {&d.oDbase, 8 * n}, {&d.oOff, 8 * n},
         {&d.oDefer, 8 * n},
@@ -310,6 +319,7 @@ int place_cell(hsg_engine* e, HostCell& hc) {
         {&d.oPoolCount, 4 * n},
         {&d.oLevelStart, 4 * (int64_t)s.levelStart.size()}, {&d.oUnits, 4 * (int64_t)s.units.size()},
         {&d.oGrpMem, 4 * nMem}, {&d.oGrpInfo, 16 * nGroups}, {&d.oHJ0, 8 * (int64_t)s.HJ0.size()},
+        {&d.oDesc, 4 * (int64_t)s.udesc.size()},
         {&d.oWork, workB},
         {&d.oSpkCompt, 4 * (int64_t)s.spkCompt.size()}, {&d.oSpkEdge, 4 * (int64_t)s.spkCompt.size()},
         {&d.oSpkThr, 8 * (int64_t)s.spkCompt.size()}, {&d.oSpkRefr, 8 * (int64_t)s.spkCompt.size()},
@@ -398,14 +408,23 @@ int place_cell(hsg_engine* e, HostCell& hc) {
         (rc = upload(e, d.oPoolICa, zerosPS)) || (rc = upload(e, d.oPoolCount, poolCount)) ||
         (rc = upload(e, d.oLevelStart, levelStart)) || (rc = upload(e, d.oUnits, units)) ||
         (rc = upload(e, d.oGrpMem, mem)) || (rc = upload(e, d.oGrpInfo, info)) ||
-        (rc = upload(e, d.oHJ0, s.HJ0)) || (rc = upload(e, d.oSpkCompt, spkCompt)) ||
+        (rc = upload(e, d.oHJ0, s.HJ0)) || (rc = upload(e, d.oDesc, s.udesc)) ||
+        (rc = upload(e, d.oSpkCompt, spkCompt)) ||
         (rc = upload(e, d.oSpkEdge, spkEdge)) || (rc = upload(e, d.oSpkThr, s.spkThr)) ||
         (rc = upload(e, d.oSpkRefr, s.spkRefr)) || (rc = upload(e, d.oSpkLast, spkLast)) ||
         (rc = upload(e, d.oSpkFired, spkFired)) || (rc = upload(e, d.oSteps, steps)))
         return rc;
     HIPCHK(hipStreamSynchronize(e->stream));
-    // work arrays in LDS when they fit
-    d.workInLds = work_bytes(d) <= e->ldsLimit ? 1 : 0;
+    d.slowHJ = s.anySlowGroup ? 1 : 0;
+    // work arrays (and the level table) in LDS when they fit
+    const int64_t wl = work_lds_bytes(d), lb = level_bytes(d);
+    if (wl + lb <= e->ldsLimit) {
+        d.workInLds = 1;
+        d.ldsBytes = (int32_t)align_up(wl + lb);
+    } else {
+        d.workInLds = 0;
+        d.ldsBytes = lb <= 16384 ? (int32_t)align_up(lb) : 0;
+    }
     return 0;
 }
 
@@ -541,18 +560,18 @@ double chan_gk(const HostCell& hc, int dk, const std::vector<double>& state) {
     return (double)((GT)s.Gbar[o] * fraction);
 }
 
+int make_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg_engine::Bin>& bins);
+
 int upload_tables(hsg_engine* e) {
     if (!e->tablesDirty) return 0;
     const int nc = (int)e->cells.size();
     std::vector<CellDev> devs(std::max(1, nc));
     std::memset(devs.data(), 0, sizeof(CellDev) * devs.size());
     e->order.clear();
-    e->maxLdsUsed = 0;
     for (int i = 0; i < nc; ++i) {
         if (!e->cells[i] || !e->cells[i]->live) continue;
         devs[i] = e->cells[i]->dev;
         e->order.push_back(i);
-        if (devs[i].workInLds) e->maxLdsUsed = std::max<int>(e->maxLdsUsed, (int)work_bytes(devs[i]));
     }
     std::stable_sort(e->order.begin(), e->order.end(),
                      [&](int a, int b) { return devs[a].n > devs[b].n; });
@@ -573,8 +592,53 @@ int upload_tables(hsg_engine* e) {
         HIPCHK(hipMemcpyAsync(e->dTsets, e->tsets.data(), sizeof(TableSetDev) * e->tsets.size(),
                               hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
+    if ((rc = make_bins(e, e->order, e->bins))) return rc;
     e->tablesDirty = false;
     e->graphValid = false;
+    return 0;
+}
+
+// Threads per cell workgroup for a bin whose largest cell has n compartments.
+int default_threads(int n) { return n >= 1024 ? 256 : n >= 256 ? 128 : 64; }
+
+// Splits cells (largest first) into up to three bins by LDS need; each bin is
+// one launch with its own dynamic LDS size and workgroup size.
+int make_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg_engine::Bin>& bins) {
+    for (auto& b : bins) if (b.dOrder) hipFree(b.dOrder);
+    bins.clear();
+    if (cells.empty()) return 0;
+    std::vector<int32_t> sorted(cells);
+    std::stable_sort(sorted.begin(), sorted.end(), [&](int a, int b) {
+        const CellDev& x = e->cells[a]->dev;
+        const CellDev& y = e->cells[b]->dev;
+        if (x.ldsBytes != y.ldsBytes) return x.ldsBytes > y.ldsBytes;
+        return x.n > y.n;
+    });
+    const int thresholds[2] = {32768, 16384};
+    hsg_engine::Bin cur;
+    int tix = 0;
+    for (int id : sorted) {
+        const CellDev& d = e->cells[id]->dev;
+        while (tix < 2 && d.ldsBytes <= thresholds[tix]) {
+            if (!cur.cells.empty()) { bins.push_back(cur); cur = hsg_engine::Bin(); }
+            ++tix;
+        }
+        cur.cells.push_back(id);
+    }
+    if (!cur.cells.empty()) bins.push_back(cur);
+    for (auto& b : bins) {
+        int maxN = 0;
+        for (int id : b.cells) {
+            b.lds = std::max<int>(b.lds, e->cells[id]->dev.ldsBytes);
+            maxN = std::max(maxN, e->cells[id]->dev.n);
+        }
+        b.threads = e->opt.block_threads > 0 ? std::max(64, std::min(1024, (e->opt.block_threads + 63) / 64 * 64))
+                                             : default_threads(maxN);
+        HIPCHK(hipMalloc(&b.dOrder, sizeof(int32_t) * b.cells.size()));
+        HIPCHK(hipMemcpyAsync(b.dOrder, b.cells.data(), sizeof(int32_t) * b.cells.size(), hipMemcpyHostToDevice,
+                              e->stream));
+    }
+    HIPCHK(hipStreamSynchronize(e->stream));
     return 0;
 }
 
@@ -590,7 +654,7 @@ LaunchArgs make_args(hsg_engine* e, const int32_t* order, int nsteps, int first)
     a.spikeCap = kSpikeCap;
     a.nsteps = nsteps;
     a.firstOfCall = first;
-    a.ldsBytes = e->maxLdsUsed;
+    a.ldsBytes = 0;
     a.dt = e->dt;
     return a;
 }
@@ -641,21 +705,34 @@ void consume_inputs(hsg_engine* e, HostCell& hc, int64_t nsteps) {
     hc.stepsDone += nsteps;
 }
 
+// One round: every bin launched for `chunk` steps on its own stream, forked
+// from and joined back into the engine stream.
+int launch_round(hsg_engine* e, std::vector<hsg_engine::Bin>& bins, int chunk, int first) {
+    HIPCHK(hipEventRecord(e->evFork, e->stream));
+    for (size_t b = 0; b < bins.size(); ++b) {
+        hipStream_t st = e->binStream[b];
+        HIPCHK(hipStreamWaitEvent(st, e->evFork, 0));
+        LaunchArgs a = make_args(e, bins[b].dOrder, chunk, first);
+        a.ldsBytes = bins[b].lds;
+        HIPCHK(hsg_launch_step(&a, (int)bins[b].cells.size(), bins[b].threads, e->fp32, st));
+        HIPCHK(hipEventRecord(e->evJoin[b], st));
+        HIPCHK(hipStreamWaitEvent(e->stream, e->evJoin[b], 0));
+    }
+    return 0;
+}
+
 int run_launches(hsg_engine* e, const std::vector<int32_t>& order, const std::vector<int64_t>& steps) {
     // order: cells to run; steps[i]: steps for order[i]
     const int K = e->opt.steps_per_launch > 0 ? e->opt.steps_per_launch : 50;
-    const int threads = block_threads(e);
     std::vector<int64_t> left(steps);
-    std::vector<int32_t> cur;
-    int32_t* dSub = nullptr;
-    bool subset = false;
-    // whole live set with equal steps -> use the persistent order array
-    if (order.size() == e->order.size()) {
-        bool same = true;
-        for (size_t i = 0; i < order.size() && same; ++i) same = order[i] == e->order[i] && steps[i] == steps[0];
-        subset = !same;
-    } else subset = true;
-    if (subset) HIPCHK(hipMalloc(&dSub, sizeof(int32_t) * std::max<size_t>(1, order.size())));
+    bool whole = order.size() == e->order.size();
+    if (whole) {
+        std::vector<int32_t> a(order), b(e->order);
+        std::sort(a.begin(), a.end());
+        std::sort(b.begin(), b.end());
+        whole = a == b;
+        for (size_t i = 0; i < steps.size() && whole; ++i) whole = steps[i] == steps[0];
+    }
     e->lastLaunches = 0;
     e->lastTotalMs = 0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
@@ -669,23 +746,27 @@ int run_launches(hsg_engine* e, const std::vector<int32_t>& order, const std::ve
         }
         return {e->evStart[k], e->evStop[k]};
     };
+    std::vector<hsg_engine::Bin> tmpBins;
+    std::vector<int32_t> activePrev;
     bool first = true;
     int rc = 0;
     while (true) {
-        cur.clear();
+        std::vector<int32_t> active;
         int64_t chunk = K;
         for (size_t i = 0; i < order.size(); ++i)
-            if (left[i] > 0) { cur.push_back(order[i]); chunk = std::min(chunk, left[i]); }
-        if (cur.empty()) break;
-        const int32_t* dOrd = e->dOrder;
-        if (subset) {
-            hipError_t he = hipMemcpyAsync(dSub, cur.data(), sizeof(int32_t) * cur.size(), hipMemcpyHostToDevice, e->stream);
-            if (he != hipSuccess) { rc = fail(HSG_E_DEVICE, hipGetErrorString(he)); break; }
-            dOrd = dSub;
+            if (left[i] > 0) { active.push_back(order[i]); chunk = std::min(chunk, left[i]); }
+        if (active.empty()) break;
+        std::vector<hsg_engine::Bin>* bins = &e->bins;
+        if (!whole) {
+            if (active != activePrev) {
+                if ((rc = make_bins(e, active, tmpBins))) break;
+                activePrev = active;
+            }
+            bins = &tmpBins;
         }
         // steady state of the whole set: replay a captured graph when asked
         const int G = e->opt.graph_chunk;
-        bool useGraph = !subset && !first && G > 0 && chunk == K;
+        bool useGraph = whole && !first && G > 0 && chunk == K;
         if (useGraph) {
             int64_t minLeft = INT64_MAX;
             for (size_t i = 0; i < order.size(); ++i) minLeft = std::min(minLeft, left[i]);
@@ -699,9 +780,9 @@ int run_launches(hsg_engine* e, const std::vector<int32_t>& order, const std::ve
                 hipGraph_t g;
                 hipError_t he = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
                 if (he != hipSuccess) { rc = fail(HSG_E_DEVICE, hipGetErrorString(he)); break; }
-                LaunchArgs a = make_args(e, e->dOrder, K, 0);
-                for (int r = 0; r < G; ++r) hsg_launch_step(&a, (int)cur.size(), threads, e->fp32, e->stream);
+                for (int r = 0; r < G && !rc; ++r) rc = launch_round(e, e->bins, K, 0);
                 he = hipStreamEndCapture(e->stream, &g);
+                if (rc) break;
                 if (he != hipSuccess) { rc = fail(HSG_E_DEVICE, hipGetErrorString(he)); break; }
                 he = hipGraphInstantiate(&e->graphExec, g, nullptr, nullptr, 0);
                 hipGraphDestroy(g);
@@ -714,9 +795,7 @@ int run_launches(hsg_engine* e, const std::vector<int32_t>& order, const std::ve
             for (size_t i = 0; i < order.size(); ++i) left[i] -= (int64_t)G * K;
             e->lastLaunches += G;
         } else {
-            LaunchArgs a = make_args(e, dOrd, (int)chunk, first ? 1 : 0);
-            hipError_t he = hsg_launch_step(&a, (int)cur.size(), threads, e->fp32, e->stream);
-            if (he != hipSuccess) { rc = fail(HSG_E_DEVICE, std::string("step launch: ") + hipGetErrorString(he)); break; }
+            if ((rc = launch_round(e, *bins, (int)chunk, first ? 1 : 0))) break;
             for (size_t i = 0; i < order.size(); ++i) if (left[i] > 0) left[i] -= chunk;
             e->lastLaunches += 1;
         }
@@ -730,7 +809,7 @@ int run_launches(hsg_engine* e, const std::vector<int32_t>& order, const std::ve
         float ms = 0;
         if (hipEventElapsedTime(&ms, ev.first, ev.second) == hipSuccess) e->lastTotalMs += ms;
     }
-    if (dSub) hipFree(dSub);
+    for (auto& b : tmpBins) if (b.dOrder) hipFree(b.dOrder);
     return rc;
 }
 
@@ -786,7 +865,13 @@ int hsg_create(const hsg_options* opts, hsg_engine** out) {
     HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     int lds = 0;
     HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, e->device));
-    e->ldsLimit = std::min(lds, 65536);
+    e->ldsLimit = lds;
+    HIPCHK(hsg_set_max_lds(lds));
+    for (int b = 0; b < 3; ++b) {
+        HIPCHK(hipStreamCreateWithFlags(&e->binStream[b], hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&e->evJoin[b], hipEventDisableTiming));
+    }
+    HIPCHK(hipEventCreateWithFlags(&e->evFork, hipEventDisableTiming));
     int64_t cap = e->opt.mem_budget > 0 ? e->opt.mem_budget : (int64_t)64 << 20;
     HIPCHK(hipMalloc(&e->arena.base, cap));
     e->arena.cap = cap;
@@ -813,6 +898,12 @@ int hsg_destroy(hsg_engine* e) {
     hipFree(e->dTsets);
     hipFree(e->dSpikes);
     hipFree(e->dSpikeCount);
+    for (auto& b : e->bins) if (b.dOrder) hipFree(b.dOrder);
+    for (int b = 0; b < 3; ++b) {
+        if (e->binStream[b]) hipStreamDestroy(e->binStream[b]);
+        if (e->evJoin[b]) hipEventDestroy(e->evJoin[b]);
+    }
+    if (e->evFork) hipEventDestroy(e->evFork);
     if (e->stream) hipStreamDestroy(e->stream);
     delete e;
     return 0;

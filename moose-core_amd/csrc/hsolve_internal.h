@@ -83,6 +83,9 @@ struct CellDev {
     int64_t oGrpMem;        // int32 member lists
     int64_t oGrpInfo;       // int32[4*nGroups]: memStart, size, hjOff, deferLast
     int64_t oHJ0;           // f64[nHJ] initial junction coefficients (HJCopy_)
+    int64_t oDesc;          // int32[8*nUnits] unit descriptors, level order
+    int32_t slowHJ;         // 1: some group uses the slow path (HJ scratch in oWork)
+    int32_t ldsBytes;       // LDS this cell needs (work + level table)
     // work (global fallback when not in LDS): D[n], B[n], HJ[nHJ], term[2*nTerm]
     int64_t oWork;
     // injections: int32 idx[nInj] + f64 basal[nInj] + f64 varying[nInj]
@@ -98,6 +101,18 @@ struct CellDev {
     int64_t steps;          // steps done since reinit (device copy kept in oSteps)
     int64_t oSteps;         // int64 counter
 };
+
+// Elimination unit descriptor: 8 int32, read as two int4 loads.
+//   d0: bits 0-3 group size s (0 = linear link), bit 4 slow path,
+//       bits 8+4(p-1).. : number of deferred term pairs of member p (p=1..4)
+//   linear: d1 = x (eliminated into x+1), d2 = deferred slot of x+1 (-1: none),
+//           d4,d5 = the off-diagonal HS[4x+1] as raw double bits
+//   group : d1 = offset of its junction coefficients in HJ0 (slow path: the
+//           group index), d2 = deferred slot of the last member (-1: none),
+//           d3 = first deferred slot of its members, d4..d6 = members as
+//           16-bit pairs (m0 | m1<<16, m2 | m3<<16, m4 | m5<<16)
+constexpr int kFastGroupMax = 5;
+constexpr int kDescInts = 8;
 
 struct SpikeRec {
     int32_t cell;

@@ -198,35 +198,165 @@ __device__ __forceinline__ void channel_row(char* arena, const CellDev& c, const
 // ---------------------------------------------------------------------------
 // elimination units
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int hj_upper(int base, int p, int q) { (void)p; return base + 2 * (q - p - 1); }
+struct Desc {
+    int d0, d1, d2, d3, d4, d5, d6, d7;
+};
 
-// HJ block of member p starts at sum_{p'<p} 2*(s-1-p')
-__device__ __forceinline__ int hj_base(int hjOff, int s, int p) { return hjOff + p * (2 * (s - 1) - (p - 1)); }
+__device__ __forceinline__ Desc load_desc(const int4* __restrict__ p, int u) {
+    const int4 a = p[2 * u], b = p[2 * u + 1];
+    return Desc{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+}
 
-__device__ void forward_unit(char* arena, const CellDev& c, int u, Work w) {
-    const int32_t* defer = A_<const int32_t>(arena, c.oDefer);
-    if (u >= 0) {
-        // linear link x -> x+1 (HSolvePassive.cpp:268-272)
-        const int x = u;
-        const double off = A_<const double>(arena, c.oOff)[x];
-        const double Dx = w.D[x];
-        const double tD = off / Dx * off;
-        const double tB = off / Dx * w.B[x];
-        const int d = defer[2 * (x + 1)];
-        if (d >= 0) {
-            w.T[2 * d] = tD;
-            w.T[2 * d + 1] = tB;
-        } else {
-            w.D[x + 1] -= tD;
-            w.B[x + 1] -= tB;
-        }
-        return;
+__device__ __forceinline__ int member(const Desc& u, int p) {
+    const int w = p < 2 ? u.d4 : p < 4 ? u.d5 : u.d6;
+    return (p & 1) ? ((w >> 16) & 0xffff) : (w & 0xffff);
+}
+
+__device__ __forceinline__ double desc_off(const Desc& u) {
+    return __hiloint2double(u.d5, u.d4);
+}
+
+// start of member p's block in a group's junction coefficients (reference
+// HJ_ layout: for each non-last member, (A[c][y], A[y][c]) per later y)
+__host__ __device__ constexpr int hb(int S, int p) { return p * (2 * (S - 1) - (p - 1)); }
+
+// linear link x -> x+1 (HSolvePassive.cpp:268-272, 325-401)
+__device__ __forceinline__ void fwd_linear(const Desc& u, Work w) {
+    const int x = u.d1;
+    const double off = desc_off(u);
+    const double Dx = w.D[x];
+    const double tD = off / Dx * off;
+    const double tB = off / Dx * w.B[x];
+    if (u.d2 >= 0) {
+        w.T[2 * u.d2] = tD;
+        w.T[2 * u.d2 + 1] = tB;
+    } else {
+        w.D[x + 1] -= tD;
+        w.B[x + 1] -= tB;
     }
-    const int g = -u - 1;
+}
+
+__device__ __forceinline__ void bwd_linear(const Desc& u, Work w) {
+    const int x = u.d1;
+    const double off = desc_off(u);
+    w.B[x] = (w.B[x] - off * w.B[x + 1]) / w.D[x];
+}
+
+// Junction group of S members, all arithmetic in registers.  Member p < S-1
+// is eliminated into every later member (the operand sequences of
+// HinesMatrix::makeOperands, hsolve/HinesMatrix.cpp:225-365, and
+// HSolvePassive::forwardEliminate, hsolve/HSolvePassive.cpp:273-310).
+template <int S>
+__device__ void fwd_group(const Desc& u, Work w, const double* __restrict__ hj0) {
+    int m[S];
+    double hj[S * (S - 1)];
+    double D[S], B[S];
+#pragma unroll
+    for (int p = 0; p < S; ++p) m[p] = member(u, p);
+#pragma unroll
+    for (int k = 0; k < S * (S - 1); ++k) hj[k] = hj0[u.d1 + k];
+#pragma unroll
+    for (int p = 0; p < S; ++p) {
+        D[p] = w.D[m[p]];
+        B[p] = w.B[m[p]];
+    }
+    int tb = u.d3;
+#pragma unroll
+    for (int p = 0; p < S - 1; ++p) {
+        if (p >= 1) {
+            // the feeder's terms come after the earlier siblings' terms
+            const int cnt = (u.d0 >> (4 + 4 * p)) & 15;
+            for (int t = 0; t < cnt; ++t) {
+                D[p] -= w.T[2 * (tb + t)];
+                B[p] -= w.T[2 * (tb + t) + 1];
+            }
+            tb += cnt;
+        }
+        const double pivot = D[p];
+#pragma unroll
+        for (int q = p + 1; q < S; ++q) {
+            const int bq = hb(S, p) + 2 * (q - p - 1);
+            const double lower = hj[bq + 1];
+            const double tD = lower / pivot * hj[bq];
+            const double tB = lower / pivot * B[p];
+            if (q == S - 1 && u.d2 >= 0) {
+                w.T[2 * (u.d2 + p)] = tD;
+                w.T[2 * (u.d2 + p) + 1] = tB;
+            } else {
+                D[q] -= tD;
+                B[q] -= tB;
+            }
+#pragma unroll
+            for (int z = p + 1; z < S; ++z) {
+                if (z == q) continue;
+                const double above = hj[hb(S, p) + 2 * (z - p - 1)];
+                const int idx = q < z ? hb(S, q) + 2 * (z - q - 1) : hb(S, z) + 2 * (q - z - 1) + 1;
+                hj[idx] -= lower / pivot * above;
+            }
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < S - 1; ++p) {
+        w.D[m[p]] = D[p];
+        w.B[m[p]] = B[p];
+    }
+    if (u.d2 < 0) {
+        w.D[m[S - 1]] = D[S - 1];
+        w.B[m[S - 1]] = B[S - 1];
+    }
+}
+
+// Back-substitution of a group.  The fill of the junction coefficients is
+// recomputed from HJ0 and the stored pivots (same operations, same order, so
+// the same values as in the forward pass) instead of being kept.
+template <int S>
+__device__ void bwd_group(const Desc& u, Work w, const double* __restrict__ hj0) {
+    int m[S];
+    double hj[S * (S - 1)];
+    double D[S - 1], Vm[S];
+#pragma unroll
+    for (int p = 0; p < S; ++p) m[p] = member(u, p);
+#pragma unroll
+    for (int k = 0; k < S * (S - 1); ++k) hj[k] = hj0[u.d1 + k];
+#pragma unroll
+    for (int p = 0; p < S - 1; ++p) D[p] = w.D[m[p]];
+    Vm[S - 1] = w.B[m[S - 1]];
+#pragma unroll
+    for (int p = 0; p < S - 1; ++p) {
+        const double pivot = D[p];
+#pragma unroll
+        for (int q = p + 1; q < S; ++q) {
+            const double lower = hj[hb(S, p) + 2 * (q - p - 1) + 1];
+#pragma unroll
+            for (int z = p + 1; z < S; ++z) {
+                if (z == q) continue;
+                const double above = hj[hb(S, p) + 2 * (z - p - 1)];
+                const int idx = q < z ? hb(S, q) + 2 * (z - q - 1) : hb(S, z) + 2 * (q - z - 1) + 1;
+                hj[idx] -= lower / pivot * above;
+            }
+        }
+    }
+#pragma unroll
+    for (int p = S - 2; p >= 0; --p) {
+        double acc = w.B[m[p]];
+#pragma unroll
+        for (int q = S - 1; q > p; --q) acc -= Vm[q] * hj[hb(S, p) + 2 * (q - p - 1)];
+        Vm[p] = acc / D[p];
+        w.B[m[p]] = Vm[p];
+    }
+}
+
+// Groups larger than kFastGroupMax (or cells past 16-bit indices): member
+// lists, deferred slots and the working junction coefficients in memory.
+__device__ void fwd_group_slow(char* arena, const CellDev& c, const Desc& u, Work w) {
+    const int32_t* defer = A_<const int32_t>(arena, c.oDefer);
+    const int g = u.d1;
     const int4 gi = A_<const int4>(arena, c.oGrpInfo)[g];
     const int32_t* mem = A_<const int32_t>(arena, c.oGrpMem) + gi.x;
     const int s = gi.y;
     double* HJ = w.HJ;
+    const double* hj0 = A_<const double>(arena, c.oHJ0);
+    for (int k = 0; k < s * (s - 1); ++k) HJ[gi.z + k] = hj0[gi.z + k];
     const int deferLast = gi.w;
     for (int p = 0; p + 1 < s; ++p) {
         const int cn = mem[p];
@@ -242,12 +372,11 @@ __device__ void forward_unit(char* arena, const CellDev& c, int u, Work w) {
         }
         const double pivot = w.D[cn];
         const double Bc = w.B[cn];
-        const int base = hj_base(gi.z, s, p);
+        const int base = gi.z + hb(s, p);
         for (int q = p + 1; q < s; ++q) {
             const int y = mem[q];
             const double lower = HJ[base + 2 * (q - p - 1) + 1];
-            const double upper = HJ[base + 2 * (q - p - 1)];
-            const double tD = lower / pivot * upper;
+            const double tD = lower / pivot * HJ[base + 2 * (q - p - 1)];
             const double tB = lower / pivot * Bc;
             if (q == s - 1 && deferLast >= 0) {
                 w.T[2 * (deferLast + p)] = tD;
@@ -256,36 +385,51 @@ __device__ void forward_unit(char* arena, const CellDev& c, int u, Work w) {
                 w.D[y] -= tD;
                 w.B[y] -= tB;
             }
-            // fill among later members: A[y][z] -= A[y][c]/pivot * A[c][z]
             for (int z = p + 1; z < s; ++z) {
                 if (z == q) continue;
                 const double above = HJ[base + 2 * (z - p - 1)];
-                int idx;
-                if (q < z) idx = hj_base(gi.z, s, q) + 2 * (z - q - 1);
-                else idx = hj_base(gi.z, s, z) + 2 * (q - z - 1) + 1;
+                const int idx = q < z ? gi.z + hb(s, q) + 2 * (z - q - 1) : gi.z + hb(s, z) + 2 * (q - z - 1) + 1;
                 HJ[idx] -= lower / pivot * above;
             }
         }
     }
 }
 
-__device__ void backward_unit(char* arena, const CellDev& c, int u, Work w) {
-    if (u >= 0) {
-        const int x = u;
-        const double off = A_<const double>(arena, c.oOff)[x];
-        w.B[x] = (w.B[x] - off * w.B[x + 1]) / w.D[x];
-        return;
-    }
-    const int g = -u - 1;
+__device__ void bwd_group_slow(char* arena, const CellDev& c, const Desc& u, Work w) {
+    const int g = u.d1;
     const int4 gi = A_<const int4>(arena, c.oGrpInfo)[g];
     const int32_t* mem = A_<const int32_t>(arena, c.oGrpMem) + gi.x;
     const int s = gi.y;
     for (int p = s - 2; p >= 0; --p) {
         const int cn = mem[p];
-        const int base = hj_base(gi.z, s, p);
+        const int base = gi.z + hb(s, p);
         double acc = w.B[cn];
         for (int q = s - 1; q > p; --q) acc -= w.B[mem[q]] * w.HJ[base + 2 * (q - p - 1)];
         w.B[cn] = acc / w.D[cn];
+    }
+}
+
+__device__ __forceinline__ void forward_unit(char* arena, const CellDev& c, const Desc& u, Work w,
+                                             const double* hj0) {
+    switch (u.d0 & 31) {
+    case 0: fwd_linear(u, w); break;
+    case 2: fwd_group<2>(u, w, hj0); break;
+    case 3: fwd_group<3>(u, w, hj0); break;
+    case 4: fwd_group<4>(u, w, hj0); break;
+    case 5: fwd_group<5>(u, w, hj0); break;
+    default: fwd_group_slow(arena, c, u, w); break;
+    }
+}
+
+__device__ __forceinline__ void backward_unit(char* arena, const CellDev& c, const Desc& u, Work w,
+                                              const double* hj0) {
+    switch (u.d0 & 31) {
+    case 0: bwd_linear(u, w); break;
+    case 2: bwd_group<2>(u, w, hj0); break;
+    case 3: bwd_group<3>(u, w, hj0); break;
+    case 4: bwd_group<4>(u, w, hj0); break;
+    case 5: bwd_group<5>(u, w, hj0); break;
+    default: bwd_group_slow(arena, c, u, w); break;
     }
 }
 
@@ -367,32 +511,44 @@ __device__ void calcium_row(char* arena, const CellDev& c, const ChanType* types
 // the step kernel
 // ---------------------------------------------------------------------------
 template <typename GT>
-__global__ void __launch_bounds__(1024) hsolve_step_kernel(LaunchArgs a) {
+__global__ void __launch_bounds__(256) hsolve_step_kernel(LaunchArgs a) {
     extern __shared__ double lds[];
     const CellDev& c = a.cells[a.order[blockIdx.x]];
     char* arena = a.arena;
     const int n = c.n;
     const int tid = threadIdx.x;
     const int nt = blockDim.x;
+    // work arrays: D[n], B[n], T[2*nTerm] (LDS when they fit), HJ (slow groups)
+    double* gwork = A_<double>(arena, c.oWork);
+    double* base = c.workInLds ? lds : gwork;
     Work w;
-    double* base = c.workInLds ? lds : A_<double>(arena, c.oWork);
     w.D = base;
     w.B = base + n;
-    w.HJ = base + 2 * n;
-    w.T = base + 2 * n + c.nHJ;
+    w.T = base + 2 * n;
+    w.HJ = gwork + 2 * n + 2 * c.nTerm;
+    // level table in LDS after the work arrays (or read from HBM)
+    const int32_t* gLevel = A_<const int32_t>(arena, c.oLevelStart);
+    int32_t* sLevel = reinterpret_cast<int32_t*>(lds + (c.workInLds ? 2 * n + 2 * c.nTerm : 0));
+    const bool levelInLds = c.ldsBytes > 0;
+    if (levelInLds)
+        for (int l = tid; l <= c.nLevels; l += nt) sLevel[l] = gLevel[l];
+    const int32_t* level = levelInLds ? sLevel : gLevel;
     const GT dtg = (GT)a.dt;
     const double* hj0 = A_<const double>(arena, c.oHJ0);
-    const int32_t* levelStart = A_<const int32_t>(arena, c.oLevelStart);
-    const int32_t* units = A_<const int32_t>(arena, c.oUnits);
+    const int4* desc = A_<const int4>(arena, c.oDesc);
     int64_t* stepCounter = A_<int64_t>(arena, c.oSteps);
     const int64_t step0 = *stepCounter;
+    const int nUnits = c.nUnits;
+    __syncthreads();
 
     for (int k = 0; k < a.nsteps; ++k) {
         const bool first = a.firstOfCall && k == 0;
+        // first unit of level 0 for this thread, loaded while phase A runs
+        Desc cur{};
+        if (tid < nUnits) cur = load_desc(desc, tid);
         // ---- A: channels, currents, matrix rows
         for (int i = tid; i < n; i += nt)
             channel_row<GT>(arena, c, a.types, a.tsets, i, dtg, w, false);
-        for (int j = tid; j < c.nHJ; j += nt) w.HJ[j] = hj0[j];
         __syncthreads();
         if (tid == 0) {
             // injections: HS[4i+3] += injectVarying + injectBasal (HSolveActive.cpp:119-128)
@@ -416,18 +572,34 @@ __global__ void __launch_bounds__(1024) hsolve_step_kernel(LaunchArgs a) {
             }
         }
         __syncthreads();
-        // ---- F: forward elimination, level by level
+        // ---- F: forward elimination, level by level; the descriptor of the
+        // next level is fetched while this one is processed
         for (int l = 0; l < c.nLevels; ++l) {
-            const int e = levelStart[l + 1];
-            for (int u = levelStart[l] + tid; u < e; u += nt) forward_unit(arena, c, units[u], w);
+            const int s0 = level[l], s1 = level[l + 1];
+            Desc nxt{};
+            if (s1 + tid < nUnits) nxt = load_desc(desc, s1 + tid);
+            if (s0 + tid < s1) forward_unit(arena, c, cur, w, hj0);
+            for (int u = s0 + tid + nt; u < s1; u += nt) forward_unit(arena, c, load_desc(desc, u), w, hj0);
+            cur = nxt;
             __syncthreads();
         }
         // ---- B: back-substitution (VMid overwrites B)
         if (tid == 0) w.B[n - 1] = w.B[n - 1] / w.D[n - 1];
+        if (c.nLevels > 0) {
+            const int s0 = level[c.nLevels - 1];
+            if (s0 + tid < nUnits) cur = load_desc(desc, s0 + tid);
+        }
         __syncthreads();
         for (int l = c.nLevels - 1; l >= 0; --l) {
-            const int e = levelStart[l + 1];
-            for (int u = levelStart[l] + tid; u < e; u += nt) backward_unit(arena, c, units[u], w);
+            const int s0 = level[l], s1 = level[l + 1];
+            Desc nxt{};
+            if (l > 0) {
+                const int p0 = level[l - 1];
+                if (p0 + tid < s0) nxt = load_desc(desc, p0 + tid);
+            }
+            if (s0 + tid < s1) backward_unit(arena, c, cur, w, hj0);
+            for (int u = s0 + tid + nt; u < s1; u += nt) backward_unit(arena, c, load_desc(desc, u), w, hj0);
+            cur = nxt;
             __syncthreads();
         }
         // ---- D: Vm, calcium

@@ -16,6 +16,8 @@
 // own; see schedule() below.
 #include "hsolve_setup.h"
 
+#include "hsolve_internal.h"
+
 #include <algorithm>
 #include <cfloat>
 #include <cstring>
@@ -229,13 +231,15 @@ static int schedule(CellSetup& s, const std::vector<int>& junctionGroup, std::st
         if (!ok) { err = "junction structure: feeder after its target"; return HSG_E_UNSUPPORTED; }
         u.level = lv;
     }
-    // deferred terms
+    // deferred terms (a group's members get consecutive slots, in member order)
     s.defer.assign(2 * (size_t)n, -1);
     for (int x = 0; x < n; ++x) s.defer[2 * x + 1] = 0;
     for (auto& g : s.groups) g.deferLast = -1;
     s.nTerm = 0;
+    std::vector<int> termBase(s.groups.size(), 0);
     for (int g = 0; g < (int)s.groups.size(); ++g) {
         const auto& m = s.groups[g].mem;
+        termBase[g] = s.nTerm;
         for (size_t p = 1; p + 1 < m.size(); ++p) {
             int f = feeder[m[p]];
             if (f < 0) continue;
@@ -256,6 +260,48 @@ static int schedule(CellSetup& s, const std::vector<int>& junctionGroup, std::st
     for (int ui : ord) {
         const Unit& u = units[ui];
         s.units[fill[u.level]++] = u.kind == 0 ? u.node : -(u.group + 1);
+    }
+    // descriptors in the same (level) order
+    s.udesc.assign((size_t)kDescInts * s.units.size(), 0);
+    s.anySlowGroup = false;
+    for (size_t k = 0; k < s.units.size(); ++k) {
+        int32_t* d = &s.udesc[kDescInts * k];
+        const int code = s.units[k];
+        if (code >= 0) {
+            const int x = code;
+            d[0] = 0;
+            d[1] = x;
+            d[2] = s.defer[2 * (x + 1)];
+            if (d[2] >= 0 && s.defer[2 * (x + 1) + 1] != 1) { err = "internal: linear feeder with more than one term"; return HSG_E_UNSUPPORTED; }
+            std::memcpy(&d[4], &s.Off[x], sizeof(double));
+            continue;
+        }
+        const int g = -code - 1;
+        const auto& G = s.groups[g];
+        const int sz = (int)G.mem.size();
+        const bool slow = sz > kFastGroupMax || n > 65535;
+        int word = sz | (slow ? 16 : 0);
+        if (!slow) {
+            int expect = termBase[g];
+            for (int p = 1; p + 1 < sz; ++p) {
+                const int cnt = s.defer[2 * G.mem[p] + 1];
+                if (cnt > 15) { err = "internal: too many deferred terms"; return HSG_E_UNSUPPORTED; }
+                if (cnt > 0 && s.defer[2 * G.mem[p]] != expect) { err = "internal: deferred slots out of order"; return HSG_E_UNSUPPORTED; }
+                expect += cnt;
+                word |= cnt << (4 + 4 * p);
+            }
+        } else {
+            s.anySlowGroup = true;
+        }
+        d[0] = word;
+        d[1] = slow ? g : G.hjOff;
+        d[2] = G.deferLast;
+        d[3] = termBase[g];
+        if (!slow)
+            for (int p = 0; p < sz; ++p) {
+                int32_t& w = d[4 + p / 2];
+                w |= (G.mem[p] & 0xffff) << (16 * (p % 2));
+            }
     }
     return 0;
 }

@@ -61,6 +61,10 @@ struct CellSetup {
     std::vector<Group> groups;
     std::vector<int> units;           // encoded, level-ordered
     std::vector<int> levelStart;
+    // unit descriptors, 8 int32 per unit in level order (see UnitDesc in
+    // hsolve_internal.h)
+    std::vector<int32_t> udesc;
+    bool anySlowGroup = false;
     std::vector<int> defer;           // 2n: (offset, count)
     int nTerm = 0;
     int maxGroup = 0;

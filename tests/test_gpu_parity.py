@@ -107,7 +107,7 @@ def test_alone_equals_in_mix(pkg, precision):
     ({"steps_per_launch": 7}, [100, 1, 199]),
     ({"steps_per_launch": 10, "graph_chunk": 4}, [300]),
     ({"block_threads": 64}, [300]),
-    ({"block_threads": 1024, "steps_per_launch": 300}, [150, 150]),
+    ({"block_threads": 256, "steps_per_launch": 300}, [150, 150]),
 ])
 def test_launch_shapes_and_split_runs(pkg, oracle_mod, opts, chunks):
     print(f"case: launch shape {opts} chunks {chunks}", flush=True)
@@ -244,3 +244,39 @@ def test_remove_and_readmit(pkg, oracle_mod):
     np.testing.assert_array_equal(gc, rc)
     _, rb, _ = run_oracle(oracle_mod, specs[1], dt, 240)
     np.testing.assert_array_equal(gb, rb[40:])
+
+
+def star_cell(arms=9, arm_len=12, inject=1.5e-9):
+    """A soma with `arms` dendrites (a junction group of arms+1 members, past
+    the register fast path) plus a 6-member group half way down one arm."""
+    from moose_core_amd.cellspec import CellSpec, passive_compartments
+    n = 1 + arms * arm_len + 5
+    length = np.full(n, 15e-6)
+    dia = np.full(n, 1.5e-6)
+    length[0] = dia[0] = 20e-6
+    comp, area = passive_compartments(length, dia)
+    comp["inject"][0] = inject
+    axial = []
+    k = 1
+    for a in range(arms):
+        prev = 0
+        for j in range(arm_len):
+            axial.append((prev, k))
+            prev = k
+            k += 1
+    for j in range(5):            # five children on compartment 6 -> group of 6
+        axial.append((6, k))
+        k += 1
+    spec = CellSpec(comp, np.array(axial, dtype=np.int32), seed=0,
+                    channels=workloads.hh_channels(n, area), gates=workloads.hh_gates())
+    return spec
+
+
+def test_large_junction_groups(pkg, oracle_mod):
+    print("case: star cell (groups of 10 and 6), 400 steps, fp64", flush=True)
+    spec = star_cell()
+    dt, n = 25e-6, 400
+    ids, (gpu,), _ = run_gpu(pkg, [spec], dt, n)
+    _, ref, _ = run_oracle(oracle_mod, spec, dt, n)
+    np.testing.assert_array_equal(gpu, ref)
+    assert np.ptp(ref[:, 0]) > 0.05       # the soma fires
