@@ -102,17 +102,19 @@ struct CellDev {
     int64_t oSteps;         // int64 counter
 };
 
-// Elimination unit descriptor: 8 int32, read as two int4 loads.
-//   d0: bits 0-3 group size s (0 = linear link), bit 4 slow path,
-//       bits 8+4(p-1).. : number of deferred term pairs of member p (p=1..4)
-//   linear: d1 = x (eliminated into x+1), d2 = deferred slot of x+1 (-1: none),
-//           d4,d5 = the off-diagonal HS[4x+1] as raw double bits
-//   group : d1 = offset of its junction coefficients in HJ0 (slow path: the
-//           group index), d2 = deferred slot of the last member (-1: none),
-//           d3 = first deferred slot of its members, d4..d6 = members as
-//           16-bit pairs (m0 | m1<<16, m2 | m3<<16, m4 | m5<<16)
-constexpr int kFastGroupMax = 5;
-constexpr int kDescInts = 8;
+// Elimination unit descriptor: 12 int32 (48 bytes), read as three int4 loads.
+//   i0: bits 0-3 group size s (0 = linear link), bit 4 slow path,
+//       bits 8+4(p-1).. : number of deferred term pairs of member p (p=1)
+//   linear: i1 = x (eliminated into x+1), i2 = deferred slot of x+1 (-1: none),
+//           i4,i5 = the off-diagonal HS[4x+1] as raw double bits
+//   group : i1 = deferred slot of the last member (-1: none), i2 = first
+//           deferred slot of its members, i3 = m0 | m1<<16, i4 = m2,
+//           i6..i11 = Ga (= 2/Ra) of the members as doubles, from which the
+//           junction coefficients are recomputed in registers
+//   groups of more than kFastGroupMax members (or cells past 16-bit indices)
+//   take the slow path: i1 = group index, everything else from memory
+constexpr int kFastGroupMax = 3;
+constexpr int kDescInts = 12;
 
 struct SpikeRec {
     int32_t cell;

@@ -69,7 +69,14 @@ __device__ __forceinline__ const float* table_ptr<float>(char* arena, const Tabl
     return A_<const float>(arena, t.off32);
 }
 
-// ChannelStruct powers (hsolve/HSolveStruct.cpp:15-65, HSolveStruct.h:88-108)
+// ChannelStruct powers (hsolve/HSolveStruct.cpp:15-65, HSolveStruct.h:88-108).
+// Powers 1-4 by multiplication; any other power through exp(p*log(x)), kept
+// out of line so that its registers do not count against the hot path.
+__device__ __noinline__ double power_general(double x, double p) {
+    if (x > 0.0) return exp(p * log(x));
+    return 0.0;
+}
+
 template <typename GT>
 __device__ __forceinline__ GT take_power(GT x, double p) {
     if (p == 1.0) return x;
@@ -79,8 +86,7 @@ __device__ __forceinline__ GT take_power(GT x, double p) {
         GT y = x * x;
         return y * y;
     }
-    if (x > (GT)0) return (GT)exp((double)p * log((double)x));
-    return (GT)0;
+    return (GT)power_general((double)x, p);
 }
 
 // trapezoidal gate update, dx/dt = A - B x (HSolveActive.cpp:259-268)
@@ -101,7 +107,7 @@ struct Work {
 // ---------------------------------------------------------------------------
 // phase A: channels and matrix rows for compartment i
 // ---------------------------------------------------------------------------
-template <typename GT>
+template <typename GT, bool UNIFORM>
 __device__ __forceinline__ void channel_row(char* arena, const CellDev& c, const ChanType* types,
                                             const TableSetDev* tsets, int i, GT dtg, Work w,
                                             bool reinitOnly) {
@@ -110,12 +116,10 @@ __device__ __forceinline__ void channel_row(char* arena, const CellDev& c, const
     GT* state = A_<GT>(arena, c.oState);
     const GT* Gbar = A_<const GT>(arena, c.oGbar);
     const GT* Ek = A_<const GT>(arena, c.oEk);
-    const int32_t* chType = c.uniform ? nullptr : A_<const int32_t>(arena, c.oChType);
+    const int32_t* chType = A_<const int32_t>(arena, c.oChType);
 
-    // Ca rows of this compartment's pools (caRowCompt_), taken in the loop
-    // below from the cell's table set
-    int caBase[kMaxPoolSlots];
-    double caFrac[kMaxPoolSlots];
+    // pools of this compartment; the Ca row of a pool (caRowCompt_) is
+    // computed where a Z gate reads it
     const int nPools = c.hasPools ? A_<const int32_t>(arena, c.oPoolCount)[i] : 0;
 
     double GkSum = 0.0, GkEkSum = 0.0;
@@ -124,18 +128,15 @@ __device__ __forceinline__ void channel_row(char* arena, const CellDev& c, const
     double vFrac = 0.0;
     int curSet = -1;
     for (int k = 0; k < c.nChanSlots; ++k) {
-        const int t = c.uniform ? c.slotType[k] : chType[(size_t)k * n + i];
+        // uniform cells: the slot's type is the same for every compartment,
+        // so the type record is read with scalar loads
+        const int t = UNIFORM ? c.slotType[k] : chType[(size_t)k * n + i];
         if (t < 0) continue;
-        const ChanType ct = types[t];
+        const ChanType& ct = types[t];
         const TableSetDev& ts = tsets[ct.tableSet];
         if (ct.tableSet != curSet) {
             curSet = ct.tableSet;
             table_row(ts.v, V, vBase, vFrac);
-            if (c.hasPools) {
-                const double* pca = A_<const double>(arena, c.oPoolCa);
-                for (int q = 0; q < kMaxPoolSlots; ++q)
-                    if (q < nPools) table_row(ts.ca, pca[(size_t)q * n + i], caBase[q], caFrac[q]);
-            }
         }
         const GT* vt = table_ptr<GT>(arena, ts.v);
         const int vCol = ts.v.nColumns;
@@ -163,10 +164,9 @@ __device__ __forceinline__ void channel_row(char* arena, const CellDev& c, const
             const GT* ctab = table_ptr<GT>(arena, ts.ca);
             const int cCol = ts.ca.nColumns;
             if (ct.caDepLocal >= 0 && ct.caDepLocal < nPools) {
-                int cb = 0;
-                double cf = 0.0;
-                for (int q = 0; q < kMaxPoolSlots; ++q)
-                    if (q == ct.caDepLocal) { cb = caBase[q]; cf = caFrac[q]; }
+                int cb;
+                double cf;
+                table_row(ts.ca, A_<const double>(arena, c.oPoolCa)[(size_t)ct.caDepLocal * n + i], cb, cf);
                 table_lookup<GT>(ctab, cCol, cb, ct.colZ, (GT)cf, C1, C2);
             } else {
                 const double ext = A_<const double>(arena, c.oExtCa)[(size_t)k * n + i];
@@ -199,21 +199,50 @@ __device__ __forceinline__ void channel_row(char* arena, const CellDev& c, const
 // elimination units
 // ---------------------------------------------------------------------------
 struct Desc {
-    int d0, d1, d2, d3, d4, d5, d6, d7;
+    int4 a, b, c;
 };
 
 __device__ __forceinline__ Desc load_desc(const int4* __restrict__ p, int u) {
-    const int4 a = p[2 * u], b = p[2 * u + 1];
-    return Desc{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    return Desc{p[3 * u], p[3 * u + 1], p[3 * u + 2]};
+}
+
+__device__ __forceinline__ int dword(const Desc& u, int i) {
+    const int4& v = i < 4 ? u.a : i < 8 ? u.b : u.c;
+    const int j = i & 3;
+    return j == 0 ? v.x : j == 1 ? v.y : j == 2 ? v.z : v.w;
 }
 
 __device__ __forceinline__ int member(const Desc& u, int p) {
-    const int w = p < 2 ? u.d4 : p < 4 ? u.d5 : u.d6;
+    const int w = dword(u, 3 + p / 2);
     return (p & 1) ? ((w >> 16) & 0xffff) : (w & 0xffff);
 }
 
-__device__ __forceinline__ double desc_off(const Desc& u) {
-    return __hiloint2double(u.d5, u.d4);
+__device__ __forceinline__ double desc_double(const Desc& u, int i) {
+    return __hiloint2double(dword(u, i + 1), dword(u, i));
+}
+
+__device__ __forceinline__ double desc_off(const Desc& u) { return desc_double(u, 4); }
+
+// Junction coefficients of a group, as HinesMatrix::makeMatrix computes them
+// (hsolve/HinesMatrix.cpp:186-215): gsum = sum of Ga over the sorted members,
+// A[i][j] = A[j][i] = -Ga_i*Ga_j/gsum, in the reference's HJ_ layout.
+template <int S>
+__device__ __forceinline__ void group_coefficients(const Desc& u, double* hj) {
+    double Ga[S];
+#pragma unroll
+    for (int p = 0; p < S; ++p) Ga[p] = desc_double(u, 6 + 2 * p);
+    double gsum = 0.0;
+#pragma unroll
+    for (int p = 0; p < S; ++p) gsum += Ga[p];
+    int k = 0;
+#pragma unroll
+    for (int p = 0; p < S - 1; ++p)
+#pragma unroll
+        for (int q = p + 1; q < S; ++q) {
+            const double gij = Ga[p] * Ga[q] / gsum;
+            hj[k++] = -gij;
+            hj[k++] = -gij;
+        }
 }
 
 // start of member p's block in a group's junction coefficients (reference
@@ -222,14 +251,14 @@ __host__ __device__ constexpr int hb(int S, int p) { return p * (2 * (S - 1) - (
 
 // linear link x -> x+1 (HSolvePassive.cpp:268-272, 325-401)
 __device__ __forceinline__ void fwd_linear(const Desc& u, Work w) {
-    const int x = u.d1;
+    const int x = u.a.y;
     const double off = desc_off(u);
     const double Dx = w.D[x];
     const double tD = off / Dx * off;
     const double tB = off / Dx * w.B[x];
-    if (u.d2 >= 0) {
-        w.T[2 * u.d2] = tD;
-        w.T[2 * u.d2 + 1] = tB;
+    if (u.a.z >= 0) {
+        w.T[2 * u.a.z] = tD;
+        w.T[2 * u.a.z + 1] = tB;
     } else {
         w.D[x + 1] -= tD;
         w.B[x + 1] -= tB;
@@ -237,7 +266,7 @@ __device__ __forceinline__ void fwd_linear(const Desc& u, Work w) {
 }
 
 __device__ __forceinline__ void bwd_linear(const Desc& u, Work w) {
-    const int x = u.d1;
+    const int x = u.a.y;
     const double off = desc_off(u);
     w.B[x] = (w.B[x] - off * w.B[x + 1]) / w.D[x];
 }
@@ -247,25 +276,25 @@ __device__ __forceinline__ void bwd_linear(const Desc& u, Work w) {
 // HinesMatrix::makeOperands, hsolve/HinesMatrix.cpp:225-365, and
 // HSolvePassive::forwardEliminate, hsolve/HSolvePassive.cpp:273-310).
 template <int S>
-__device__ void fwd_group(const Desc& u, Work w, const double* __restrict__ hj0) {
+__device__ void fwd_group(const Desc& u, Work w) {
     int m[S];
     double hj[S * (S - 1)];
     double D[S], B[S];
 #pragma unroll
     for (int p = 0; p < S; ++p) m[p] = member(u, p);
-#pragma unroll
-    for (int k = 0; k < S * (S - 1); ++k) hj[k] = hj0[u.d1 + k];
+    group_coefficients<S>(u, hj);
+    const int deferLast = u.a.y;
 #pragma unroll
     for (int p = 0; p < S; ++p) {
         D[p] = w.D[m[p]];
         B[p] = w.B[m[p]];
     }
-    int tb = u.d3;
+    int tb = u.a.z;
 #pragma unroll
     for (int p = 0; p < S - 1; ++p) {
         if (p >= 1) {
             // the feeder's terms come after the earlier siblings' terms
-            const int cnt = (u.d0 >> (4 + 4 * p)) & 15;
+            const int cnt = (u.a.x >> (4 + 4 * p)) & 15;
             for (int t = 0; t < cnt; ++t) {
                 D[p] -= w.T[2 * (tb + t)];
                 B[p] -= w.T[2 * (tb + t) + 1];
@@ -279,9 +308,9 @@ __device__ void fwd_group(const Desc& u, Work w, const double* __restrict__ hj0)
             const double lower = hj[bq + 1];
             const double tD = lower / pivot * hj[bq];
             const double tB = lower / pivot * B[p];
-            if (q == S - 1 && u.d2 >= 0) {
-                w.T[2 * (u.d2 + p)] = tD;
-                w.T[2 * (u.d2 + p) + 1] = tB;
+            if (q == S - 1 && deferLast >= 0) {
+                w.T[2 * (deferLast + p)] = tD;
+                w.T[2 * (deferLast + p) + 1] = tB;
             } else {
                 D[q] -= tD;
                 B[q] -= tB;
@@ -300,24 +329,23 @@ __device__ void fwd_group(const Desc& u, Work w, const double* __restrict__ hj0)
         w.D[m[p]] = D[p];
         w.B[m[p]] = B[p];
     }
-    if (u.d2 < 0) {
+    if (deferLast < 0) {
         w.D[m[S - 1]] = D[S - 1];
         w.B[m[S - 1]] = B[S - 1];
     }
 }
 
 // Back-substitution of a group.  The fill of the junction coefficients is
-// recomputed from HJ0 and the stored pivots (same operations, same order, so
+// recomputed from the conductances and the stored pivots (same operations, same order, so
 // the same values as in the forward pass) instead of being kept.
 template <int S>
-__device__ void bwd_group(const Desc& u, Work w, const double* __restrict__ hj0) {
+__device__ void bwd_group(const Desc& u, Work w) {
     int m[S];
     double hj[S * (S - 1)];
     double D[S - 1], Vm[S];
 #pragma unroll
     for (int p = 0; p < S; ++p) m[p] = member(u, p);
-#pragma unroll
-    for (int k = 0; k < S * (S - 1); ++k) hj[k] = hj0[u.d1 + k];
+    group_coefficients<S>(u, hj);
 #pragma unroll
     for (int p = 0; p < S - 1; ++p) D[p] = w.D[m[p]];
     Vm[S - 1] = w.B[m[S - 1]];
@@ -350,7 +378,7 @@ __device__ void bwd_group(const Desc& u, Work w, const double* __restrict__ hj0)
 // lists, deferred slots and the working junction coefficients in memory.
 __device__ void fwd_group_slow(char* arena, const CellDev& c, const Desc& u, Work w) {
     const int32_t* defer = A_<const int32_t>(arena, c.oDefer);
-    const int g = u.d1;
+    const int g = u.a.y;
     const int4 gi = A_<const int4>(arena, c.oGrpInfo)[g];
     const int32_t* mem = A_<const int32_t>(arena, c.oGrpMem) + gi.x;
     const int s = gi.y;
@@ -396,7 +424,7 @@ __device__ void fwd_group_slow(char* arena, const CellDev& c, const Desc& u, Wor
 }
 
 __device__ void bwd_group_slow(char* arena, const CellDev& c, const Desc& u, Work w) {
-    const int g = u.d1;
+    const int g = u.a.y;
     const int4 gi = A_<const int4>(arena, c.oGrpInfo)[g];
     const int32_t* mem = A_<const int32_t>(arena, c.oGrpMem) + gi.x;
     const int s = gi.y;
@@ -409,26 +437,20 @@ __device__ void bwd_group_slow(char* arena, const CellDev& c, const Desc& u, Wor
     }
 }
 
-__device__ __forceinline__ void forward_unit(char* arena, const CellDev& c, const Desc& u, Work w,
-                                             const double* hj0) {
-    switch (u.d0 & 31) {
+__device__ __forceinline__ void forward_unit(char* arena, const CellDev& c, const Desc& u, Work w) {
+    switch (u.a.x & 31) {
     case 0: fwd_linear(u, w); break;
-    case 2: fwd_group<2>(u, w, hj0); break;
-    case 3: fwd_group<3>(u, w, hj0); break;
-    case 4: fwd_group<4>(u, w, hj0); break;
-    case 5: fwd_group<5>(u, w, hj0); break;
+    case 2: fwd_group<2>(u, w); break;
+    case 3: fwd_group<3>(u, w); break;
     default: fwd_group_slow(arena, c, u, w); break;
     }
 }
 
-__device__ __forceinline__ void backward_unit(char* arena, const CellDev& c, const Desc& u, Work w,
-                                              const double* hj0) {
-    switch (u.d0 & 31) {
+__device__ __forceinline__ void backward_unit(char* arena, const CellDev& c, const Desc& u, Work w) {
+    switch (u.a.x & 31) {
     case 0: bwd_linear(u, w); break;
-    case 2: bwd_group<2>(u, w, hj0); break;
-    case 3: bwd_group<3>(u, w, hj0); break;
-    case 4: bwd_group<4>(u, w, hj0); break;
-    case 5: bwd_group<5>(u, w, hj0); break;
+    case 2: bwd_group<2>(u, w); break;
+    case 3: bwd_group<3>(u, w); break;
     default: bwd_group_slow(arena, c, u, w); break;
     }
 }
@@ -455,54 +477,46 @@ __device__ void calcium_row(char* arena, const CellDev& c, const ChanType* types
     const int n = c.n;
     const int nPools = A_<const int32_t>(arena, c.oPoolCount)[i];
     if (nPools == 0) return;
-    double act[kMaxPoolSlots];
-    double* ica = A_<double>(arena, c.oPoolICa);
-    for (int q = 0; q < kMaxPoolSlots; ++q) {
-        act[q] = 0.0;
-        if (q < nPools && firstOfCall) {
-            act[q] = 0.0 + ica[(size_t)q * n + i];
-            ica[(size_t)q * n + i] = 0.0;
-        }
-    }
     const double vmid = w.B[i];
     const double vnew = A_<const double>(arena, c.oV)[i];
     const GT* state = A_<const GT>(arena, c.oState);
     const GT* Gbar = A_<const GT>(arena, c.oGbar);
     const GT* Ek = A_<const GT>(arena, c.oEk);
     const int32_t* chType = c.uniform ? nullptr : A_<const int32_t>(arena, c.oChType);
-    int js = 0;
-    for (int k = 0; k < c.nChanSlots; ++k) {
-        const int t = c.uniform ? c.slotType[k] : chType[(size_t)k * n + i];
-        if (t < 0) continue;
-        const ChanType ct = types[t];
-        if (ct.caTgtLocal >= 0) {
-            GT fraction = c.hasModulation ? (GT)A_<const double>(arena, c.oMod)[(size_t)k * n + i] : (GT)1.0;
-            int jj = js;
-            if (ct.Xpower > 0.0) fraction *= take_power<GT>(state[(size_t)(jj++) * n + i], ct.Xpower);
-            if (ct.Ypower > 0.0) fraction *= take_power<GT>(state[(size_t)(jj++) * n + i], ct.Ypower);
-            if (ct.Zpower > 0.0) fraction *= take_power<GT>(state[(size_t)(jj++) * n + i], ct.Zpower);
-            const double Gk = (double)(Gbar[(size_t)k * n + i] * fraction);
-            const double ek = (double)Ek[(size_t)k * n + i];
-            for (int q = 0; q < kMaxPoolSlots; ++q)
-                if (q == ct.caTgtLocal && q < nPools) {
-                    if (c.caAdvance == 1) act[q] += Gk * (ek - vmid);
-                    else act[q] += Gk * (ek - (2 * vmid - vnew));
-                }
-        }
-        js += ct.nGates;
-    }
+    double* ica = A_<double>(arena, c.oPoolICa);
     double* pc = A_<double>(arena, c.oPoolC);
     double* pca = A_<double>(arena, c.oPoolCa);
-    const double* basal = A_<const double>(arena, c.oPoolBasal);
-    const double* f1 = A_<const double>(arena, c.oPoolF1);
-    const double* f2 = A_<const double>(arena, c.oPoolF2);
-    const double* ceil = A_<const double>(arena, c.oPoolCeil);
-    const double* flo = A_<const double>(arena, c.oPoolFloor);
-    for (int q = 0; q < kMaxPoolSlots; ++q) {
-        if (q >= nPools) break;
+    for (int q = 0; q < nPools; ++q) {
         const size_t o = (size_t)q * n + i;
+        // caActivation_: iCa added between steps, then the channels feeding
+        // the pool in channel order (HSolveActive.cpp:167-214)
+        double act = 0.0;
+        if (firstOfCall) {
+            act = 0.0 + ica[o];
+            ica[o] = 0.0;
+        }
+        int js = 0;
+        for (int k = 0; k < c.nChanSlots; ++k) {
+            const int t = c.uniform ? c.slotType[k] : chType[(size_t)k * n + i];
+            if (t < 0) continue;
+            const ChanType& ct = types[t];
+            if (ct.caTgtLocal == q) {
+                GT fraction = c.hasModulation ? (GT)A_<const double>(arena, c.oMod)[(size_t)k * n + i] : (GT)1.0;
+                int jj = js;
+                if (ct.Xpower > 0.0) fraction *= take_power<GT>(state[(size_t)(jj++) * n + i], ct.Xpower);
+                if (ct.Ypower > 0.0) fraction *= take_power<GT>(state[(size_t)(jj++) * n + i], ct.Ypower);
+                if (ct.Zpower > 0.0) fraction *= take_power<GT>(state[(size_t)(jj++) * n + i], ct.Zpower);
+                const double Gk = (double)(Gbar[(size_t)k * n + i] * fraction);
+                const double ek = (double)Ek[(size_t)k * n + i];
+                if (c.caAdvance == 1) act += Gk * (ek - vmid);
+                else act += Gk * (ek - (2 * vmid - vnew));
+            }
+            js += ct.nGates;
+        }
         double cst = pc[o];
-        pca[o] = caconc_process(cst, basal[o], f1[o], f2[o], ceil[o], flo[o], act[q]);
+        pca[o] = caconc_process(cst, A_<const double>(arena, c.oPoolBasal)[o], A_<const double>(arena, c.oPoolF1)[o],
+                                A_<const double>(arena, c.oPoolF2)[o], A_<const double>(arena, c.oPoolCeil)[o],
+                                A_<const double>(arena, c.oPoolFloor)[o], act);
         pc[o] = cst;
     }
 }
@@ -534,7 +548,6 @@ __global__ void __launch_bounds__(256) hsolve_step_kernel(LaunchArgs a) {
         for (int l = tid; l <= c.nLevels; l += nt) sLevel[l] = gLevel[l];
     const int32_t* level = levelInLds ? sLevel : gLevel;
     const GT dtg = (GT)a.dt;
-    const double* hj0 = A_<const double>(arena, c.oHJ0);
     const int4* desc = A_<const int4>(arena, c.oDesc);
     int64_t* stepCounter = A_<int64_t>(arena, c.oSteps);
     const int64_t step0 = *stepCounter;
@@ -547,8 +560,10 @@ __global__ void __launch_bounds__(256) hsolve_step_kernel(LaunchArgs a) {
         Desc cur{};
         if (tid < nUnits) cur = load_desc(desc, tid);
         // ---- A: channels, currents, matrix rows
-        for (int i = tid; i < n; i += nt)
-            channel_row<GT>(arena, c, a.types, a.tsets, i, dtg, w, false);
+        if (c.uniform)
+            for (int i = tid; i < n; i += nt) channel_row<GT, true>(arena, c, a.types, a.tsets, i, dtg, w, false);
+        else
+            for (int i = tid; i < n; i += nt) channel_row<GT, false>(arena, c, a.types, a.tsets, i, dtg, w, false);
         __syncthreads();
         if (tid == 0) {
             // injections: HS[4i+3] += injectVarying + injectBasal (HSolveActive.cpp:119-128)
@@ -578,8 +593,8 @@ __global__ void __launch_bounds__(256) hsolve_step_kernel(LaunchArgs a) {
             const int s0 = level[l], s1 = level[l + 1];
             Desc nxt{};
             if (s1 + tid < nUnits) nxt = load_desc(desc, s1 + tid);
-            if (s0 + tid < s1) forward_unit(arena, c, cur, w, hj0);
-            for (int u = s0 + tid + nt; u < s1; u += nt) forward_unit(arena, c, load_desc(desc, u), w, hj0);
+            if (s0 + tid < s1) forward_unit(arena, c, cur, w);
+            for (int u = s0 + tid + nt; u < s1; u += nt) forward_unit(arena, c, load_desc(desc, u), w);
             cur = nxt;
             __syncthreads();
         }
@@ -597,8 +612,8 @@ __global__ void __launch_bounds__(256) hsolve_step_kernel(LaunchArgs a) {
                 const int p0 = level[l - 1];
                 if (p0 + tid < s0) nxt = load_desc(desc, p0 + tid);
             }
-            if (s0 + tid < s1) backward_unit(arena, c, cur, w, hj0);
-            for (int u = s0 + tid + nt; u < s1; u += nt) backward_unit(arena, c, load_desc(desc, u), w, hj0);
+            if (s0 + tid < s1) backward_unit(arena, c, cur, w);
+            for (int u = s0 + tid + nt; u < s1; u += nt) backward_unit(arena, c, load_desc(desc, u), w);
             cur = nxt;
             __syncthreads();
         }
@@ -664,7 +679,7 @@ __global__ void __launch_bounds__(256) hsolve_reinit_kernel(LaunchArgs a) {
     const CellDev& c = a.cells[a.order[blockIdx.x]];
     Work w{};
     for (int i = threadIdx.x; i < c.n; i += blockDim.x)
-        channel_row<GT>(a.arena, c, a.types, a.tsets, i, (GT)a.dt, w, true);
+        channel_row<GT, false>(a.arena, c, a.types, a.tsets, i, (GT)a.dt, w, true);
 }
 
 }  // namespace hsg

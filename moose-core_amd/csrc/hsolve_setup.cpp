@@ -164,7 +164,8 @@ uint64_t fnv(const void* p, size_t n, uint64_t h) {
 // c_{k-1}).  Such a feeder therefore parks its terms in a deferred slot that
 // the group applies right before it pivots c_k.
 // ---------------------------------------------------------------------------
-static int schedule(CellSetup& s, const std::vector<int>& junctionGroup, std::string& err) {
+static int schedule(CellSetup& s, const std::vector<int>& junctionGroup, const std::vector<double>& Ga,
+                    std::string& err) {
     const int n = s.n;
     // eliminating unit of every node; feeder unit of every node
     std::vector<int> elimUnit(n, -1), feeder(n, -1);
@@ -281,27 +282,28 @@ static int schedule(CellSetup& s, const std::vector<int>& junctionGroup, std::st
         const int sz = (int)G.mem.size();
         const bool slow = sz > kFastGroupMax || n > 65535;
         int word = sz | (slow ? 16 : 0);
-        if (!slow) {
-            int expect = termBase[g];
-            for (int p = 1; p + 1 < sz; ++p) {
-                const int cnt = s.defer[2 * G.mem[p] + 1];
-                if (cnt > 15) { err = "internal: too many deferred terms"; return HSG_E_UNSUPPORTED; }
-                if (cnt > 0 && s.defer[2 * G.mem[p]] != expect) { err = "internal: deferred slots out of order"; return HSG_E_UNSUPPORTED; }
-                expect += cnt;
-                word |= cnt << (4 + 4 * p);
-            }
-        } else {
+        if (slow) {
             s.anySlowGroup = true;
+            d[0] = word;
+            d[1] = g;
+            continue;
+        }
+        int expect = termBase[g];
+        for (int p = 1; p + 1 < sz; ++p) {
+            const int cnt = s.defer[2 * G.mem[p] + 1];
+            if (cnt > 15) { err = "internal: too many deferred terms"; return HSG_E_UNSUPPORTED; }
+            if (cnt > 0 && s.defer[2 * G.mem[p]] != expect) { err = "internal: deferred slots out of order"; return HSG_E_UNSUPPORTED; }
+            expect += cnt;
+            word |= cnt << (4 + 4 * p);
         }
         d[0] = word;
-        d[1] = slow ? g : G.hjOff;
-        d[2] = G.deferLast;
-        d[3] = termBase[g];
-        if (!slow)
-            for (int p = 0; p < sz; ++p) {
-                int32_t& w = d[4 + p / 2];
-                w |= (G.mem[p] & 0xffff) << (16 * (p % 2));
-            }
+        d[1] = G.deferLast;
+        d[2] = termBase[g];
+        for (int p = 0; p < sz; ++p) {
+            int32_t& w = d[3 + p / 2];
+            w |= (G.mem[p] & 0xffff) << (16 * (p % 2));
+            std::memcpy(&d[6 + 2 * p], &Ga[G.mem[p]], sizeof(double));
+        }
     }
     return 0;
 }
@@ -410,7 +412,7 @@ int setup_cell(const hsg_cell_desc* d, double dt, int caAdvance, CellSetup& s, s
         s.maxGroup = std::max(s.maxGroup, (int)g.size());
         s.groups.push_back(std::move(G));
     }
-    rc = schedule(s, junctionGroup, err);
+    rc = schedule(s, junctionGroup, Ga, err);
     if (rc) return rc;
 
     // readHHChannels / readGates
