@@ -129,5 +129,9 @@ class PackedCell:
         d.leaks = _ptr(self.leaks)
         self.desc = d
 
+    @property
+    def n(self):
+        return len(self.compartments)
+
     def ref(self):
         return C.byref(self.desc)

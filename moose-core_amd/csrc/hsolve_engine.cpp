@@ -360,6 +360,7 @@ int place_cell(hsg_engine* e, HostCell& hc) {
     rc = table_set_id(e, s.tables, &tsid);
     if (rc) return rc;
     hc.tableSet = tsid;
+    d.tableSet = tsid;
     hc.typeIds.clear();
     for (auto& t : s.localTypes) hc.typeIds.push_back(type_id(e, t, tsid));
     std::vector<int32_t> chType(CS, -1);
@@ -831,7 +832,8 @@ int reinit_cells_host(hsg_engine* e, HostCell& hc) {
     if ((rc = upload(e, hc.dev.oSpkLast, last))) return rc;
     std::vector<int64_t> zero1(1, 0);
     if ((rc = upload(e, hc.dev.oSteps, zero1))) return rc;
-    if (hc.dev.nProbe > 0 && (rc = upload(e, hc.dev.oProbeCount, zero1))) return rc;
+    hc.dev.probeBase = 0;
+    e->tablesDirty = true;
     hc.ext.clear();
     hc.ica.clear();
     if (hc.dev.nExt) { hc.dev.nExt = 0; e->tablesDirty = true; }
@@ -1370,11 +1372,11 @@ int hsg_set_probes(hsg_engine* e, int32_t cell, const int32_t* compartments, int
     hc->dev.oProbeCompt = off;
     hc->dev.oProbeTrace = off + align_up(4 * (int64_t)n);
     hc->dev.oProbeCount = hc->dev.oProbeTrace + align_up(8 * (int64_t)n * capacity);
-    std::vector<int64_t> zero1(1, 0);
-    if ((rc = upload(e, hc->dev.oProbeCompt, h)) || (rc = upload(e, hc->dev.oProbeCount, zero1))) return rc;
+    if ((rc = upload(e, hc->dev.oProbeCompt, h))) return rc;
     HIPCHK(hipStreamSynchronize(e->stream));
     hc->dev.nProbe = n;
     hc->dev.probeCap = capacity;
+    hc->dev.probeBase = hc->stepsDone;
     hc->probeCompt.assign(compartments, compartments + n);
     hc->probeCap = capacity;
     return 0;
@@ -1386,15 +1388,14 @@ int hsg_read_probes(hsg_engine* e, int32_t cell, double* out, int64_t cap_sample
     *n_samples = 0;
     if (hc->dev.nProbe == 0) return 0;
     hipSetDevice(e->device);
-    int64_t cnt = 0;
-    int rc = download(e, hc->dev.oProbeCount, 1, &cnt);
-    if (rc) return rc;
+    // samples are written at index (step - probeBase) while below capacity
+    const int64_t cnt = hc->stepsDone - hc->dev.probeBase;
     int64_t k = std::min(std::min(cnt, hc->dev.probeCap), cap_samples);
+    int rc;
     if (k > 0 && out && (rc = download(e, hc->dev.oProbeTrace, (size_t)(k * hc->dev.nProbe), out))) return rc;
-    std::vector<int64_t> zero1(1, 0);
-    if ((rc = upload(e, hc->dev.oProbeCount, zero1))) return rc;
-    HIPCHK(hipStreamSynchronize(e->stream));
-    *n_samples = k;
+    hc->dev.probeBase = hc->stepsDone;
+    e->tablesDirty = true;
+    *n_samples = std::max<int64_t>(0, k);
     return 0;
 }
 

@@ -63,8 +63,9 @@ struct CellDev {
     int32_t workInLds;      // 1: elimination work arrays live in LDS
     int32_t live;
     int32_t nGroups;
-    int32_t pad0;
+    int32_t tableSet;       // the cell's lookup-table set
     int64_t probeCap;       // samples per probe
+    int64_t probeBase;      // step count at which the current trace started
     // compartment arrays (f64 unless noted)
     int64_t oV, oCmByDt, oEmByRm, oDbase, oOff;
     int64_t oDefer;         // int32[n]: first deferred-term pair of node, -1

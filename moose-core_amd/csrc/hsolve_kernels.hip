@@ -21,6 +21,12 @@
 // slot and applied in the reference's order, so that the FP64 form (built with
 // -ffp-contract=off and IEEE division) reproduces the reference bit for bit.
 //
+// The solve is latency-bound, so: the elimination work arrays live in LDS,
+// the barriers between levels order LDS only (the next level's unit
+// descriptors, loaded from HBM, stay in flight across them), the cell's
+// fields and table parameters sit in registers, and the channel types of a
+// uniform cell sit in LDS.
+//
 // GT is the gate/channel type: double (FP64 form) or float (mixed form, where
 // gate states, tables and channel currents are FP32 and the Hines solve stays
 // FP64).
@@ -35,38 +41,59 @@ __device__ __forceinline__ T* A_(char* arena, int64_t off) {
     return reinterpret_cast<T*>(arena + off);
 }
 
+// Barrier that orders LDS only: global loads issued before it stay in flight.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// LDS-only barrier when the work arrays are in LDS, a full one otherwise.
+__device__ __forceinline__ void work_barrier(bool ldsWork) {
+    if (ldsWork) lds_barrier();
+    else __syncthreads();
+}
+
+// One lookup table in registers (hsolve/RateLookup.h LookupTable).
+template <typename GT>
+struct Table {
+    double min, max, dx;
+    int nCol;
+    const GT* __restrict__ tab;
+};
+
+template <typename GT>
+__device__ __forceinline__ Table<GT> table_view(char* arena, const TableDev& t) {
+    Table<GT> v;
+    v.min = t.min;
+    v.max = t.max;
+    v.dx = t.dx;
+    v.nCol = t.nColumns;
+    v.tab = reinterpret_cast<const GT*>(arena + (sizeof(GT) == 8 ? t.off64 : t.off32));
+    return v;
+}
+
 // LookupTable::row (hsolve/RateLookup.cpp:63-76)
-__device__ __forceinline__ void table_row(const TableDev& t, double x, int& base, double& frac) {
+template <typename GT>
+__device__ __forceinline__ void table_row(const Table<GT>& t, double x, int& base, double& frac) {
     if (x < t.min) x = t.min;
     else if (x > t.max) x = t.max;
     double div = (x - t.min) / t.dx;
     unsigned integer = (unsigned)div;
     frac = div - integer;
-    base = (int)integer * t.nColumns;
+    base = (int)integer * t.nCol;
 }
 
 // LookupTable::lookup (hsolve/RateLookup.cpp:78-103)
 template <typename GT>
-__device__ __forceinline__ void table_lookup(const GT* tab, int nCol, int base, int col, GT frac,
-                                             GT& C1, GT& C2) {
-    const GT* ap = tab + base + col;
-    const GT* bp = ap + nCol;
+__device__ __forceinline__ void table_lookup(const Table<GT>& t, int base, int col, GT frac, GT& C1, GT& C2) {
+    const GT* ap = t.tab + base + col;
+    const GT* bp = ap + t.nCol;
     GT a = ap[0], b = bp[0];
     C1 = a + (b - a) * frac;
     a = ap[1];
     b = bp[1];
     C2 = a + (b - a) * frac;
-}
-
-template <typename GT>
-__device__ __forceinline__ const GT* table_ptr(char* arena, const TableDev& t);
-template <>
-__device__ __forceinline__ const double* table_ptr<double>(char* arena, const TableDev& t) {
-    return A_<const double>(arena, t.off64);
-}
-template <>
-__device__ __forceinline__ const float* table_ptr<float>(char* arena, const TableDev& t) {
-    return A_<const float>(arena, t.off32);
 }
 
 // ChannelStruct powers (hsolve/HSolveStruct.cpp:15-65, HSolveStruct.h:88-108).
@@ -100,82 +127,113 @@ __device__ __forceinline__ GT gate_step(GT x, GT C1, GT C2, GT dt, bool instant)
 struct Work {
     double* D;
     double* B;     // becomes VMid during back-substitution
-    double* HJ;
+    double* HJ;    // working junction coefficients of slow-path groups (HBM)
     double* T;     // deferred terms: pairs (tD, tB)
 };
 
+// Everything a workgroup reads about its cell, hoisted into registers.
+template <typename GT>
+struct View {
+    int n, nChanSlots, uniform, hasModulation, hasPools, caAdvance;
+    double* __restrict__ V;
+    const double* __restrict__ CmByDt;
+    const double* __restrict__ EmByRm;
+    const double* __restrict__ Dbase;
+    GT* __restrict__ state;
+    const GT* __restrict__ Gbar;
+    const GT* __restrict__ Ek;
+    const double* __restrict__ mod;
+    const double* __restrict__ extCa;
+    const int32_t* __restrict__ chType;
+    const int32_t* __restrict__ poolCount;
+    double* __restrict__ poolCa;
+    Table<GT> vt, ct;
+};
+
+template <typename GT>
+__device__ __forceinline__ View<GT> make_view(char* arena, const CellDev& c, const TableSetDev* tsets) {
+    View<GT> v;
+    v.n = c.n;
+    v.nChanSlots = c.nChanSlots;
+    v.uniform = c.uniform;
+    v.hasModulation = c.hasModulation;
+    v.hasPools = c.hasPools;
+    v.caAdvance = c.caAdvance;
+    v.V = A_<double>(arena, c.oV);
+    v.CmByDt = A_<const double>(arena, c.oCmByDt);
+    v.EmByRm = A_<const double>(arena, c.oEmByRm);
+    v.Dbase = A_<const double>(arena, c.oDbase);
+    v.state = A_<GT>(arena, c.oState);
+    v.Gbar = A_<const GT>(arena, c.oGbar);
+    v.Ek = A_<const GT>(arena, c.oEk);
+    v.mod = A_<const double>(arena, c.oMod);
+    v.extCa = A_<const double>(arena, c.oExtCa);
+    v.chType = A_<const int32_t>(arena, c.oChType);
+    v.poolCount = A_<const int32_t>(arena, c.oPoolCount);
+    v.poolCa = A_<double>(arena, c.oPoolCa);
+    const TableSetDev& ts = tsets[c.tableSet];
+    v.vt = table_view<GT>(arena, ts.v);
+    v.ct = table_view<GT>(arena, ts.ca);
+    return v;
+}
+
 // ---------------------------------------------------------------------------
-// phase A: channels and matrix rows for compartment i
+// phase A: channels and matrix row of compartment i.  `types` is the cell's
+// slot-type table in LDS (uniform cells) or the engine's type table.
 // ---------------------------------------------------------------------------
 template <typename GT, bool UNIFORM>
-__device__ __forceinline__ void channel_row(char* arena, const CellDev& c, const ChanType* types,
-                                            const TableSetDev* tsets, int i, GT dtg, Work w,
-                                            bool reinitOnly) {
-    const int n = c.n;
-    const double V = A_<const double>(arena, c.oV)[i];
-    GT* state = A_<GT>(arena, c.oState);
-    const GT* Gbar = A_<const GT>(arena, c.oGbar);
-    const GT* Ek = A_<const GT>(arena, c.oEk);
-    const int32_t* chType = A_<const int32_t>(arena, c.oChType);
-
+__device__ __forceinline__ void channel_row(const View<GT>& v, const ChanType* __restrict__ types, int i,
+                                            GT dtg, Work w, bool reinitOnly) {
+    const int n = v.n;
+    const double V = v.V[i];
     // pools of this compartment; the Ca row of a pool (caRowCompt_) is
     // computed where a Z gate reads it
-    const int nPools = c.hasPools ? A_<const int32_t>(arena, c.oPoolCount)[i] : 0;
-
+    const int nPools = v.hasPools ? v.poolCount[i] : 0;
+    int vBase;
+    double vFrac;
+    table_row(v.vt, V, vBase, vFrac);
+    const GT frac = (GT)vFrac;
     double GkSum = 0.0, GkEkSum = 0.0;
-    int js = 0;                       // gate-state slot
-    int vBase = -1;
-    double vFrac = 0.0;
-    int curSet = -1;
-    for (int k = 0; k < c.nChanSlots; ++k) {
-        // uniform cells: the slot's type is the same for every compartment,
-        // so the type record is read with scalar loads
-        const int t = UNIFORM ? c.slotType[k] : chType[(size_t)k * n + i];
+    int js = 0;   // gate-state slot
+    for (int k = 0; k < v.nChanSlots; ++k) {
+        const int t = UNIFORM ? k : v.chType[(size_t)k * n + i];
         if (t < 0) continue;
         const ChanType& ct = types[t];
-        const TableSetDev& ts = tsets[ct.tableSet];
-        if (ct.tableSet != curSet) {
-            curSet = ct.tableSet;
-            table_row(ts.v, V, vBase, vFrac);
-        }
-        const GT* vt = table_ptr<GT>(arena, ts.v);
-        const int vCol = ts.v.nColumns;
+        if (UNIFORM && ct.nGates < 0) continue;   // empty slot
         GT C1, C2;
-        GT frac = (GT)vFrac;
-        GT fraction = c.hasModulation ? (GT)A_<const double>(arena, c.oMod)[(size_t)k * n + i] : (GT)1.0;
+        GT fraction = v.hasModulation ? (GT)v.mod[(size_t)k * n + i] : (GT)1.0;
         if (ct.Xpower > 0.0) {
-            GT* sp = state + (size_t)js * n + i;
-            table_lookup<GT>(vt, vCol, vBase, ct.colX, frac, C1, C2);
+            GT* sp = v.state + (size_t)js * n + i;
+            table_lookup<GT>(v.vt, vBase, ct.colX, frac, C1, C2);
             GT x = reinitOnly ? C1 / C2 : gate_step<GT>(*sp, C1, C2, dtg, ct.instant & 1);
             *sp = x;
             fraction *= take_power<GT>(x, ct.Xpower);
             ++js;
         }
         if (ct.Ypower > 0.0) {
-            GT* sp = state + (size_t)js * n + i;
-            table_lookup<GT>(vt, vCol, vBase, ct.colY, frac, C1, C2);
+            GT* sp = v.state + (size_t)js * n + i;
+            table_lookup<GT>(v.vt, vBase, ct.colY, frac, C1, C2);
             GT x = reinitOnly ? C1 / C2 : gate_step<GT>(*sp, C1, C2, dtg, ct.instant & 2);
             *sp = x;
             fraction *= take_power<GT>(x, ct.Ypower);
             ++js;
         }
         if (ct.Zpower > 0.0) {
-            GT* sp = state + (size_t)js * n + i;
-            const GT* ctab = table_ptr<GT>(arena, ts.ca);
-            const int cCol = ts.ca.nColumns;
+            GT* sp = v.state + (size_t)js * n + i;
             if (ct.caDepLocal >= 0 && ct.caDepLocal < nPools) {
                 int cb;
                 double cf;
-                table_row(ts.ca, A_<const double>(arena, c.oPoolCa)[(size_t)ct.caDepLocal * n + i], cb, cf);
-                table_lookup<GT>(ctab, cCol, cb, ct.colZ, (GT)cf, C1, C2);
+                table_row(v.ct, v.poolCa[(size_t)ct.caDepLocal * n + i], cb, cf);
+                table_lookup<GT>(v.ct, cb, ct.colZ, (GT)cf, C1, C2);
             } else {
-                const double ext = A_<const double>(arena, c.oExtCa)[(size_t)k * n + i];
-                if (ext > 0 && ct.colZ + 1 < cCol) {
-                    int b; double f;
-                    table_row(ts.ca, ext, b, f);
-                    table_lookup<GT>(ctab, cCol, b, ct.colZ, (GT)f, C1, C2);
+                const double ext = v.extCa[(size_t)k * n + i];
+                if (ext > 0 && ct.colZ + 1 < v.ct.nCol) {
+                    int b;
+                    double f;
+                    table_row(v.ct, ext, b, f);
+                    table_lookup<GT>(v.ct, b, ct.colZ, (GT)f, C1, C2);
                 } else {
-                    table_lookup<GT>(vt, vCol, vBase, ct.colZ, frac, C1, C2);
+                    table_lookup<GT>(v.vt, vBase, ct.colZ, frac, C1, C2);
                 }
             }
             GT x = reinitOnly ? C1 / C2 : gate_step<GT>(*sp, C1, C2, dtg, ct.instant & 4);
@@ -184,15 +242,15 @@ __device__ __forceinline__ void channel_row(char* arena, const CellDev& c, const
             ++js;
         }
         if (!reinitOnly) {
-            GT Gk = Gbar[(size_t)k * n + i] * fraction;
+            GT Gk = v.Gbar[(size_t)k * n + i] * fraction;
             GkSum += (double)Gk;
-            GkEkSum += (double)Gk * (double)Ek[(size_t)k * n + i];
+            GkEkSum += (double)Gk * (double)v.Ek[(size_t)k * n + i];
         }
     }
     if (reinitOnly) return;
     // updateMatrix row (HSolveActive.cpp:112-115)
-    w.D[i] = A_<const double>(arena, c.oDbase)[i] + GkSum;
-    w.B[i] = V * A_<const double>(arena, c.oCmByDt)[i] + A_<const double>(arena, c.oEmByRm)[i] + GkEkSum;
+    w.D[i] = v.Dbase[i] + GkSum;
+    w.B[i] = V * v.CmByDt[i] + v.EmByRm[i] + GkEkSum;
 }
 
 // ---------------------------------------------------------------------------
@@ -336,8 +394,8 @@ __device__ void fwd_group(const Desc& u, Work w) {
 }
 
 // Back-substitution of a group.  The fill of the junction coefficients is
-// recomputed from the conductances and the stored pivots (same operations, same order, so
-// the same values as in the forward pass) instead of being kept.
+// recomputed from the conductances and the stored pivots (same operations,
+// same order, so the same values as in the forward pass) instead of kept.
 template <int S>
 __device__ void bwd_group(const Desc& u, Work w) {
     int m[S];
@@ -471,54 +529,64 @@ __device__ __forceinline__ double caconc_process(double& cst, double basal, doub
     return ca;
 }
 
+// advanceCalcium for the pools of compartment i (HSolveActive.cpp:153-216):
+// activation = iCa added between steps, then the channels feeding the pool
+// in channel order.
 template <typename GT>
-__device__ void calcium_row(char* arena, const CellDev& c, const ChanType* types, int i, Work w,
-                            bool firstOfCall) {
-    const int n = c.n;
-    const int nPools = A_<const int32_t>(arena, c.oPoolCount)[i];
+__device__ void calcium_row(char* arena, const CellDev& c, const View<GT>& v, const ChanType* __restrict__ types,
+                            int i, Work w, bool firstOfCall) {
+    const int n = v.n;
+    const int nPools = v.poolCount[i];
     if (nPools == 0) return;
     const double vmid = w.B[i];
-    const double vnew = A_<const double>(arena, c.oV)[i];
-    const GT* state = A_<const GT>(arena, c.oState);
-    const GT* Gbar = A_<const GT>(arena, c.oGbar);
-    const GT* Ek = A_<const GT>(arena, c.oEk);
-    const int32_t* chType = c.uniform ? nullptr : A_<const int32_t>(arena, c.oChType);
+    const double vnew = v.V[i];
     double* ica = A_<double>(arena, c.oPoolICa);
     double* pc = A_<double>(arena, c.oPoolC);
-    double* pca = A_<double>(arena, c.oPoolCa);
     for (int q = 0; q < nPools; ++q) {
         const size_t o = (size_t)q * n + i;
-        // caActivation_: iCa added between steps, then the channels feeding
-        // the pool in channel order (HSolveActive.cpp:167-214)
         double act = 0.0;
         if (firstOfCall) {
             act = 0.0 + ica[o];
             ica[o] = 0.0;
         }
         int js = 0;
-        for (int k = 0; k < c.nChanSlots; ++k) {
-            const int t = c.uniform ? c.slotType[k] : chType[(size_t)k * n + i];
+        for (int k = 0; k < v.nChanSlots; ++k) {
+            const int t = v.uniform ? k : v.chType[(size_t)k * n + i];
             if (t < 0) continue;
             const ChanType& ct = types[t];
+            if (v.uniform && ct.nGates < 0) continue;
             if (ct.caTgtLocal == q) {
-                GT fraction = c.hasModulation ? (GT)A_<const double>(arena, c.oMod)[(size_t)k * n + i] : (GT)1.0;
+                GT fraction = v.hasModulation ? (GT)v.mod[(size_t)k * n + i] : (GT)1.0;
                 int jj = js;
-                if (ct.Xpower > 0.0) fraction *= take_power<GT>(state[(size_t)(jj++) * n + i], ct.Xpower);
-                if (ct.Ypower > 0.0) fraction *= take_power<GT>(state[(size_t)(jj++) * n + i], ct.Ypower);
-                if (ct.Zpower > 0.0) fraction *= take_power<GT>(state[(size_t)(jj++) * n + i], ct.Zpower);
-                const double Gk = (double)(Gbar[(size_t)k * n + i] * fraction);
-                const double ek = (double)Ek[(size_t)k * n + i];
-                if (c.caAdvance == 1) act += Gk * (ek - vmid);
+                if (ct.Xpower > 0.0) fraction *= take_power<GT>(v.state[(size_t)(jj++) * n + i], ct.Xpower);
+                if (ct.Ypower > 0.0) fraction *= take_power<GT>(v.state[(size_t)(jj++) * n + i], ct.Ypower);
+                if (ct.Zpower > 0.0) fraction *= take_power<GT>(v.state[(size_t)(jj++) * n + i], ct.Zpower);
+                const double Gk = (double)(v.Gbar[(size_t)k * n + i] * fraction);
+                const double ek = (double)v.Ek[(size_t)k * n + i];
+                if (v.caAdvance == 1) act += Gk * (ek - vmid);
                 else act += Gk * (ek - (2 * vmid - vnew));
             }
             js += ct.nGates;
         }
         double cst = pc[o];
-        pca[o] = caconc_process(cst, A_<const double>(arena, c.oPoolBasal)[o], A_<const double>(arena, c.oPoolF1)[o],
-                                A_<const double>(arena, c.oPoolF2)[o], A_<const double>(arena, c.oPoolCeil)[o],
-                                A_<const double>(arena, c.oPoolFloor)[o], act);
+        v.poolCa[o] = caconc_process(cst, A_<const double>(arena, c.oPoolBasal)[o],
+                                     A_<const double>(arena, c.oPoolF1)[o], A_<const double>(arena, c.oPoolF2)[o],
+                                     A_<const double>(arena, c.oPoolCeil)[o], A_<const double>(arena, c.oPoolFloor)[o],
+                                     act);
         pc[o] = cst;
     }
+}
+
+// Copies the slot types of a uniform cell to LDS (empty slots get
+// nGates = -1).  Returns the table to read channel types from.
+__device__ __forceinline__ const ChanType* stage_types(const CellDev& c, const ChanType* types, ChanType* sTypes) {
+    if (!c.uniform) return types;
+    for (int k = threadIdx.x; k < c.nChanSlots; k += blockDim.x) {
+        const int t = c.slotType[k];
+        if (t >= 0) sTypes[k] = types[t];
+        else sTypes[k].nGates = -1;
+    }
+    return sTypes;
 }
 
 // ---------------------------------------------------------------------------
@@ -527,31 +595,44 @@ __device__ void calcium_row(char* arena, const CellDev& c, const ChanType* types
 template <typename GT>
 __global__ void __launch_bounds__(256) hsolve_step_kernel(LaunchArgs a) {
     extern __shared__ double lds[];
+    __shared__ ChanType sTypes[kMaxSlots];
     const CellDev& c = a.cells[a.order[blockIdx.x]];
     char* arena = a.arena;
-    const int n = c.n;
+    const View<GT> v = make_view<GT>(arena, c, a.tsets);
+    const int n = v.n;
     const int tid = threadIdx.x;
     const int nt = blockDim.x;
+    const bool ldsWork = c.workInLds;
+    const int nTerm = c.nTerm;
+    const int nLevels = c.nLevels;
+    const int nUnits = c.nUnits;
+    const int nInj = c.nInj;
+    const int nExt = c.nExt;
+    const int nSpike = c.nSpike;
+    const int nProbe = c.nProbe;
+    const int64_t probeCap = c.probeCap;
+    const int64_t probeBase = c.probeBase;
     // work arrays: D[n], B[n], T[2*nTerm] (LDS when they fit), HJ (slow groups)
     double* gwork = A_<double>(arena, c.oWork);
-    double* base = c.workInLds ? lds : gwork;
+    double* base = ldsWork ? lds : gwork;
     Work w;
     w.D = base;
     w.B = base + n;
     w.T = base + 2 * n;
-    w.HJ = gwork + 2 * n + 2 * c.nTerm;
+    w.HJ = gwork + 2 * n + 2 * nTerm;
     // level table in LDS after the work arrays (or read from HBM)
     const int32_t* gLevel = A_<const int32_t>(arena, c.oLevelStart);
-    int32_t* sLevel = reinterpret_cast<int32_t*>(lds + (c.workInLds ? 2 * n + 2 * c.nTerm : 0));
+    int32_t* sLevel = reinterpret_cast<int32_t*>(lds + (ldsWork ? 2 * n + 2 * nTerm : 0));
     const bool levelInLds = c.ldsBytes > 0;
     if (levelInLds)
-        for (int l = tid; l <= c.nLevels; l += nt) sLevel[l] = gLevel[l];
+        for (int l = tid; l <= nLevels; l += nt) sLevel[l] = gLevel[l];
     const int32_t* level = levelInLds ? sLevel : gLevel;
+    const ChanType* types = stage_types(c, a.types, sTypes);
     const GT dtg = (GT)a.dt;
     const int4* desc = A_<const int4>(arena, c.oDesc);
     int64_t* stepCounter = A_<int64_t>(arena, c.oSteps);
     const int64_t step0 = *stepCounter;
-    const int nUnits = c.nUnits;
+    const bool uniform = v.uniform;
     __syncthreads();
 
     for (int k = 0; k < a.nsteps; ++k) {
@@ -560,52 +641,54 @@ __global__ void __launch_bounds__(256) hsolve_step_kernel(LaunchArgs a) {
         Desc cur{};
         if (tid < nUnits) cur = load_desc(desc, tid);
         // ---- A: channels, currents, matrix rows
-        if (c.uniform)
-            for (int i = tid; i < n; i += nt) channel_row<GT, true>(arena, c, a.types, a.tsets, i, dtg, w, false);
+        if (uniform)
+            for (int i = tid; i < n; i += nt) channel_row<GT, true>(v, types, i, dtg, w, false);
         else
-            for (int i = tid; i < n; i += nt) channel_row<GT, false>(arena, c, a.types, a.tsets, i, dtg, w, false);
-        __syncthreads();
-        if (tid == 0) {
-            // injections: HS[4i+3] += injectVarying + injectBasal (HSolveActive.cpp:119-128)
-            const int32_t* ii = A_<const int32_t>(arena, c.oInjIdx);
-            const double* ib = A_<const double>(arena, c.oInjBasal);
-            double* iv = A_<double>(arena, c.oInjVar);
-            for (int e = 0; e < c.nInj; ++e) {
-                const double var = first ? iv[e] : 0.0;
-                w.B[ii[e]] += var + ib[e];
-            }
-            if (first) {
-                for (int e = 0; e < c.nInj; ++e) iv[e] = 0.0;
-                // external channels (HSolveActive.cpp:142-148)
-                const int32_t* xi = A_<const int32_t>(arena, c.oExtIdx);
-                const double* xg = A_<const double>(arena, c.oExtG);
-                const double* xge = A_<const double>(arena, c.oExtGE);
-                for (int e = 0; e < c.nExt; ++e) {
-                    w.D[xi[e]] += xg[e];
-                    w.B[xi[e]] += xge[e];
+            for (int i = tid; i < n; i += nt) channel_row<GT, false>(v, types, i, dtg, w, false);
+        if (nInj > 0 || (first && nExt > 0)) {
+            work_barrier(ldsWork);
+            if (tid == 0) {
+                // injections: HS[4i+3] += injectVarying + injectBasal (HSolveActive.cpp:119-128)
+                const int32_t* ii = A_<const int32_t>(arena, c.oInjIdx);
+                const double* ib = A_<const double>(arena, c.oInjBasal);
+                double* iv = A_<double>(arena, c.oInjVar);
+                for (int e = 0; e < nInj; ++e) {
+                    const double var = first ? iv[e] : 0.0;
+                    w.B[ii[e]] += var + ib[e];
+                }
+                if (first) {
+                    for (int e = 0; e < nInj; ++e) iv[e] = 0.0;
+                    // external channels (HSolveActive.cpp:142-148)
+                    const int32_t* xi = A_<const int32_t>(arena, c.oExtIdx);
+                    const double* xg = A_<const double>(arena, c.oExtG);
+                    const double* xge = A_<const double>(arena, c.oExtGE);
+                    for (int e = 0; e < nExt; ++e) {
+                        w.D[xi[e]] += xg[e];
+                        w.B[xi[e]] += xge[e];
+                    }
                 }
             }
         }
-        __syncthreads();
+        work_barrier(ldsWork);
         // ---- F: forward elimination, level by level; the descriptor of the
         // next level is fetched while this one is processed
-        for (int l = 0; l < c.nLevels; ++l) {
+        for (int l = 0; l < nLevels; ++l) {
             const int s0 = level[l], s1 = level[l + 1];
             Desc nxt{};
             if (s1 + tid < nUnits) nxt = load_desc(desc, s1 + tid);
             if (s0 + tid < s1) forward_unit(arena, c, cur, w);
             for (int u = s0 + tid + nt; u < s1; u += nt) forward_unit(arena, c, load_desc(desc, u), w);
             cur = nxt;
-            __syncthreads();
+            work_barrier(ldsWork);
         }
         // ---- B: back-substitution (VMid overwrites B)
         if (tid == 0) w.B[n - 1] = w.B[n - 1] / w.D[n - 1];
-        if (c.nLevels > 0) {
-            const int s0 = level[c.nLevels - 1];
+        if (nLevels > 0) {
+            const int s0 = level[nLevels - 1];
             if (s0 + tid < nUnits) cur = load_desc(desc, s0 + tid);
         }
-        __syncthreads();
-        for (int l = c.nLevels - 1; l >= 0; --l) {
+        work_barrier(ldsWork);
+        for (int l = nLevels - 1; l >= 0; --l) {
             const int s0 = level[l], s1 = level[l + 1];
             Desc nxt{};
             if (l > 0) {
@@ -615,59 +698,57 @@ __global__ void __launch_bounds__(256) hsolve_step_kernel(LaunchArgs a) {
             if (s0 + tid < s1) backward_unit(arena, c, cur, w);
             for (int u = s0 + tid + nt; u < s1; u += nt) backward_unit(arena, c, load_desc(desc, u), w);
             cur = nxt;
-            __syncthreads();
+            work_barrier(ldsWork);
         }
         // ---- D: Vm, calcium
-        double* Vg = A_<double>(arena, c.oV);
-        for (int i = tid; i < n; i += nt) Vg[i] = 2 * w.B[i] - Vg[i];
-        if (c.hasPools) {
-            __syncthreads();
-            for (int i = tid; i < n; i += nt) calcium_row<GT>(arena, c, a.types, i, w, first);
-        }
-        __syncthreads();
+        for (int i = tid; i < n; i += nt) v.V[i] = 2 * w.B[i] - v.V[i];
+        if (v.hasPools)
+            for (int i = tid; i < n; i += nt) calcium_row<GT>(arena, c, v, types, i, w, first);
         const int64_t stepNo = step0 + k + 1;
-        // spike generators (SpikeGen::process at t = dt * step)
-        for (int sg = tid; sg < c.nSpike; sg += nt) {
-            const int comp = A_<const int32_t>(arena, c.oSpkCompt)[sg];
-            const double Vm = Vg[comp];
-            const double thr = A_<const double>(arena, c.oSpkThr)[sg];
-            int32_t* fired = A_<int32_t>(arena, c.oSpkFired);
-            if (Vm > thr) {
-                const double t = a.dt * (double)(unsigned long long)stepNo;
-                double* last = A_<double>(arena, c.oSpkLast);
-                const double refr = A_<const double>(arena, c.oSpkRefr)[sg];
-                const int edge = A_<const int32_t>(arena, c.oSpkEdge)[sg];
-                if ((t + a.dt / 2.0) >= (last[sg] + refr)) {
-                    if (!(edge && fired[sg])) {
-                        const unsigned long long slot = atomicAdd(a.spikeCount, 1ULL);
-                        if (slot < (unsigned long long)a.spikeCap) {
-                            SpikeRec r;
-                            r.cell = c.cellId;
-                            r.gen = sg;
-                            r.step = stepNo;
-                            a.spikes[slot] = r;
+        if (nSpike > 0 || nProbe > 0) {
+            __syncthreads();   // V written by other threads
+            // spike generators (SpikeGen::process at t = dt * step)
+            for (int sg = tid; sg < nSpike; sg += nt) {
+                const int comp = A_<const int32_t>(arena, c.oSpkCompt)[sg];
+                const double Vm = v.V[comp];
+                const double thr = A_<const double>(arena, c.oSpkThr)[sg];
+                int32_t* fired = A_<int32_t>(arena, c.oSpkFired);
+                if (Vm > thr) {
+                    const double t = a.dt * (double)(unsigned long long)stepNo;
+                    double* last = A_<double>(arena, c.oSpkLast);
+                    const double refr = A_<const double>(arena, c.oSpkRefr)[sg];
+                    const int edge = A_<const int32_t>(arena, c.oSpkEdge)[sg];
+                    if ((t + a.dt / 2.0) >= (last[sg] + refr)) {
+                        if (!(edge && fired[sg])) {
+                            const unsigned long long slot = atomicAdd(a.spikeCount, 1ULL);
+                            if (slot < (unsigned long long)a.spikeCap) {
+                                SpikeRec r;
+                                r.cell = c.cellId;
+                                r.gen = sg;
+                                r.step = stepNo;
+                                a.spikes[slot] = r;
+                            }
+                            last[sg] = t;
+                            fired[sg] = 1;
                         }
-                        last[sg] = t;
-                        fired[sg] = 1;
                     }
+                } else {
+                    fired[sg] = 0;
                 }
-            } else {
-                fired[sg] = 0;
+            }
+            // Vm probes (Table sampling after the solver's tick)
+            if (nProbe > 0) {
+                const int64_t sample = step0 + k - probeBase;
+                if (sample >= 0 && sample < probeCap) {
+                    const int32_t* pcmp = A_<const int32_t>(arena, c.oProbeCompt);
+                    double* tr = A_<double>(arena, c.oProbeTrace);
+                    for (int j = tid; j < nProbe; j += nt) tr[sample * nProbe + j] = v.V[pcmp[j]];
+                }
             }
         }
-        // Vm probes (Table sampling after the solver's tick)
-        if (c.nProbe > 0) {
-            int64_t* pcount = A_<int64_t>(arena, c.oProbeCount);
-            const int64_t sample = *pcount;
-            if (sample < c.probeCap) {
-                const int32_t* pc = A_<const int32_t>(arena, c.oProbeCompt);
-                double* tr = A_<double>(arena, c.oProbeTrace);
-                for (int j = tid; j < c.nProbe; j += nt) tr[sample * c.nProbe + j] = Vg[pc[j]];
-            }
-            __syncthreads();
-            if (tid == 0) *pcount = sample + 1;
-        }
-        __syncthreads();
+        // the next step's phase A overwrites D/B that other threads may still
+        // read; V and the gate states are touched only by their owner thread
+        work_barrier(ldsWork);
     }
     if (tid == 0) *stepCounter = step0 + a.nsteps;
 }
@@ -676,10 +757,16 @@ __global__ void __launch_bounds__(256) hsolve_step_kernel(LaunchArgs a) {
 // A/B of the current V (and pool Ca).
 template <typename GT>
 __global__ void __launch_bounds__(256) hsolve_reinit_kernel(LaunchArgs a) {
+    __shared__ ChanType sTypes[kMaxSlots];
     const CellDev& c = a.cells[a.order[blockIdx.x]];
+    const View<GT> v = make_view<GT>(a.arena, c, a.tsets);
+    const ChanType* types = stage_types(c, a.types, sTypes);
+    __syncthreads();
     Work w{};
-    for (int i = threadIdx.x; i < c.n; i += blockDim.x)
-        channel_row<GT, false>(a.arena, c, a.types, a.tsets, i, (GT)a.dt, w, true);
+    if (v.uniform)
+        for (int i = threadIdx.x; i < v.n; i += blockDim.x) channel_row<GT, true>(v, types, i, (GT)a.dt, w, true);
+    else
+        for (int i = threadIdx.x; i < v.n; i += blockDim.x) channel_row<GT, false>(v, types, i, (GT)a.dt, w, true);
 }
 
 }  // namespace hsg
