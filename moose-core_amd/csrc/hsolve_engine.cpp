@@ -291,8 +291,10 @@ int type_id(hsg_engine* e, const HostChanType& t, int tableSet) {
     return id;
 }
 
-// LDS-resident part of the elimination work: D[n], B[n], deferred terms
-int64_t work_lds_bytes(const CellDev& c) { return 8 * (2 * (int64_t)c.n + 2 * (int64_t)c.nTerm); }
+// LDS-resident part of the elimination work: D[n], B[n], deferred terms, fills
+int64_t work_lds_bytes(const CellDev& c) {
+    return 8 * (2 * (int64_t)c.n + 2 * (int64_t)c.nTerm + (int64_t)c.nFill);
+}
 int64_t level_bytes(const CellDev& c) { return 4 * ((int64_t)c.nLevels + 1); }
 
 // Lays out and uploads one cell's main region.
@@ -308,7 +310,8 @@ int place_cell(hsg_engine* e, HostCell& hc) {
     std::memset(&d, 0, sizeof d);
     int64_t nGroups = (int64_t)s.groups.size(), nMem = 0;
     for (auto& g : s.groups) nMem += (int64_t)g.mem.size();
-    int64_t workB = 8 * (2 * (int64_t)n + 2 * (int64_t)s.nTerm + (s.anySlowGroup ? (int64_t)s.HJ0.size() : 0));
+    int64_t workB = 8 * (2 * (int64_t)n + 2 * (int64_t)s.nTerm + (int64_t)s.nFill +
+                         (s.anySlowGroup ? (int64_t)s.HJ0.size() : 0));
     std::vector<Item> items = {
         {&d.oV, 8 * n}, {&d.oCmByDt, 8 * n}, {&d.oEmByRm, 8 * n}, {&d.oDbase, 8 * n}, {&d.oOff, 8 * n},
         {&d.oDefer, 8 * n},
@@ -351,6 +354,7 @@ int place_cell(hsg_engine* e, HostCell& hc) {
     d.nUnits = (int)s.units.size();
     d.nHJ = (int)s.HJ0.size();
     d.nTerm = s.nTerm;
+    d.nFill = s.nFill;
     d.nGroups = (int)nGroups;
     d.nSpike = (int)s.spkCompt.size();
     d.cellId = hc.id;
@@ -867,8 +871,15 @@ int hsg_create(const hsg_options* opts, hsg_engine** out) {
     HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     int lds = 0;
     HIPCHK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, e->device));
-    e->ldsLimit = lds;
-    HIPCHK(hsg_set_max_lds(lds));
+    // dynamic LDS per workgroup: the device maximum less the kernel's static
+    // LDS (the staged channel types), rounded down
+    e->ldsLimit = 0;
+    for (int cand : {lds - 4096, 65536, 32768}) {
+        if (cand <= 0) continue;
+        if (hsg_set_max_lds(cand) == hipSuccess) { e->ldsLimit = cand; break; }
+        (void)hipGetLastError();
+    }
+    if (e->ldsLimit == 0) return fail(HSG_E_DEVICE, "cannot configure dynamic LDS for the step kernel");
     for (int b = 0; b < 3; ++b) {
         HIPCHK(hipStreamCreateWithFlags(&e->binStream[b], hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&e->evJoin[b], hipEventDisableTiming));

@@ -57,6 +57,7 @@ struct CellDev {
     int32_t nUnits;
     int32_t nHJ;            // doubles of junction coefficients
     int32_t nTerm;          // deferred elimination terms (pairs)
+    int32_t nFill;          // fill slots (one per three-member group)
     int32_t nInj;           // injection entries
     int32_t nSpike;         // spike generators
     int32_t nProbe;         // Vm probes
@@ -104,14 +105,15 @@ struct CellDev {
 };
 
 // Elimination unit descriptor: 12 int32 (48 bytes), read as three int4 loads.
-//   i0: bits 0-3 group size s (0 = linear link), bit 4 slow path,
-//       bits 8+4(p-1).. : number of deferred term pairs of member p (p=1)
-//   linear: i1 = x (eliminated into x+1), i2 = deferred slot of x+1 (-1: none),
-//           i4,i5 = the off-diagonal HS[4x+1] as raw double bits
-//   group : i1 = deferred slot of the last member (-1: none), i2 = first
-//           deferred slot of its members, i3 = m0 | m1<<16, i4 = m2,
-//           i6..i11 = Ga (= 2/Ra) of the members as doubles, from which the
-//           junction coefficients are recomputed in registers
+// Units of up to three members share one code path: a linear link x -> x+1
+// is a two-member unit with members (x, x+1) and coefficient HS[4x+1].
+//   i0: bits 0-3 member count s, bit 4 slow path, bits 8-11 number of
+//       deferred term pairs of member 1 (applied before it is pivoted)
+//   i1: deferred slot for the terms of the last member (-1: apply directly)
+//   i2: first deferred slot of member 1
+//   i3: m0 | m1 << 16      i4: m2      i5: fill slot (s == 3)
+//   i6..i11: a01, a02, a12 as doubles (the initial off-diagonal entries,
+//            HJCopy_ / HS[4x+1], computed at setup as the reference does)
 //   groups of more than kFastGroupMax members (or cells past 16-bit indices)
 //   take the slow path: i1 = group index, everything else from memory
 constexpr int kFastGroupMax = 3;

@@ -129,6 +129,7 @@ struct Work {
     double* B;     // becomes VMid during back-substitution
     double* HJ;    // working junction coefficients of slow-path groups (HBM)
     double* T;     // deferred terms: pairs (tD, tB)
+    double* F;     // fill entry A[1][2] of each three-member unit
 };
 
 // Everything a workgroup reads about its cell, hoisted into registers.
@@ -281,156 +282,95 @@ __device__ __forceinline__ double desc_double(const Desc& u, int i) {
 
 __device__ __forceinline__ double desc_off(const Desc& u) { return desc_double(u, 4); }
 
-// Junction coefficients of a group, as HinesMatrix::makeMatrix computes them
-// (hsolve/HinesMatrix.cpp:186-215): gsum = sum of Ga over the sorted members,
-// A[i][j] = A[j][i] = -Ga_i*Ga_j/gsum, in the reference's HJ_ layout.
-template <int S>
-__device__ __forceinline__ void group_coefficients(const Desc& u, double* hj) {
-    double Ga[S];
-#pragma unroll
-    for (int p = 0; p < S; ++p) Ga[p] = desc_double(u, 6 + 2 * p);
-    double gsum = 0.0;
-#pragma unroll
-    for (int p = 0; p < S; ++p) gsum += Ga[p];
-    int k = 0;
-#pragma unroll
-    for (int p = 0; p < S - 1; ++p)
-#pragma unroll
-        for (int q = p + 1; q < S; ++q) {
-            const double gij = Ga[p] * Ga[q] / gsum;
-            hj[k++] = -gij;
-            hj[k++] = -gij;
+// Units of two or three members (every linear link, and the junction groups
+// of a binary tree) on one code path, all arithmetic in registers.  Member 0
+// is eliminated into the later members; in a three-member unit member 1 is
+// then pivoted and eliminated into member 2 (the operand sequences of
+// HinesMatrix::makeOperands, hsolve/HinesMatrix.cpp:225-365, and of
+// HSolvePassive::forwardEliminate, hsolve/HSolvePassive.cpp:262-298, ranks 1
+// and 2 and the linear run).
+__device__ __forceinline__ void fwd_small(const Desc& u, Work w) {
+    const bool three = (u.a.x & 15) == 3;
+    const int m0 = u.a.w & 0xffff, m1 = (u.a.w >> 16) & 0xffff, m2 = u.b.x;
+    const int deferLast = u.a.y;
+    const double a01 = desc_double(u, 6);
+    const double D0 = w.D[m0], B0 = w.B[m0];
+    double D1 = w.D[m1], B1 = w.B[m1];
+    const double l1 = a01 / D0;
+    const double tD1 = l1 * a01;
+    const double tB1 = l1 * B0;
+    // terms for the last member, in the reference's order
+    double lD1 = tD1, lB1 = tB1, lD2 = 0.0, lB2 = 0.0;
+    int last = m1;
+    if (three) {
+        const double a02 = desc_double(u, 8), a12 = desc_double(u, 10);
+        D1 -= tD1;
+        B1 -= tB1;
+        // member 1's feeder terms come after member 0's (deferred slot)
+        const int cnt = (u.a.x >> 8) & 15;
+        const int tb = u.a.z;
+        for (int t = 0; t < cnt; ++t) {
+            D1 -= w.T[2 * (tb + t)];
+            B1 -= w.T[2 * (tb + t) + 1];
         }
+        const double l2 = a02 / D0;
+        const double tD2 = l2 * a02;
+        const double tB2 = l2 * B0;
+        const double a12u = a12 - l1 * a02;   // A[1][2] after member 0
+        const double a12l = a12 - l2 * a01;   // A[2][1] after member 0
+        const double l = a12l / D1;
+        w.D[m1] = D1;
+        w.B[m1] = B1;
+        w.F[u.b.y] = a12u;
+        lD1 = tD2;
+        lB1 = tB2;
+        lD2 = l * a12u;
+        lB2 = l * B1;
+        last = m2;
+    }
+    if (deferLast >= 0) {
+        w.T[2 * deferLast] = lD1;
+        w.T[2 * deferLast + 1] = lB1;
+        if (three) {
+            w.T[2 * deferLast + 2] = lD2;
+            w.T[2 * deferLast + 3] = lB2;
+        }
+    } else {
+        double Dl = three ? w.D[last] : D1;
+        double Bl = three ? w.B[last] : B1;
+        Dl -= lD1;
+        Bl -= lB1;
+        if (three) {
+            Dl -= lD2;
+            Bl -= lB2;
+        }
+        w.D[last] = Dl;
+        w.B[last] = Bl;
+    }
+}
+
+// Back-substitution of a unit of two or three members (HSolvePassive.cpp:
+// 344-369): VMid of the last member is known; B holds VMid once solved.
+__device__ __forceinline__ void bwd_small(const Desc& u, Work w) {
+    const bool three = (u.a.x & 15) == 3;
+    const int m0 = u.a.w & 0xffff, m1 = (u.a.w >> 16) & 0xffff, m2 = u.b.x;
+    const double a01 = desc_double(u, 6);
+    double acc;
+    if (three) {
+        const double a02 = desc_double(u, 8);
+        const double v2 = w.B[m2];
+        const double v1 = (w.B[m1] - v2 * w.F[u.b.y]) / w.D[m1];
+        w.B[m1] = v1;
+        acc = (w.B[m0] - v2 * a02) - v1 * a01;
+    } else {
+        acc = w.B[m0] - a01 * w.B[m1];
+    }
+    w.B[m0] = acc / w.D[m0];
 }
 
 // start of member p's block in a group's junction coefficients (reference
 // HJ_ layout: for each non-last member, (A[c][y], A[y][c]) per later y)
 __host__ __device__ constexpr int hb(int S, int p) { return p * (2 * (S - 1) - (p - 1)); }
-
-// linear link x -> x+1 (HSolvePassive.cpp:268-272, 325-401)
-__device__ __forceinline__ void fwd_linear(const Desc& u, Work w) {
-    const int x = u.a.y;
-    const double off = desc_off(u);
-    const double Dx = w.D[x];
-    const double tD = off / Dx * off;
-    const double tB = off / Dx * w.B[x];
-    if (u.a.z >= 0) {
-        w.T[2 * u.a.z] = tD;
-        w.T[2 * u.a.z + 1] = tB;
-    } else {
-        w.D[x + 1] -= tD;
-        w.B[x + 1] -= tB;
-    }
-}
-
-__device__ __forceinline__ void bwd_linear(const Desc& u, Work w) {
-    const int x = u.a.y;
-    const double off = desc_off(u);
-    w.B[x] = (w.B[x] - off * w.B[x + 1]) / w.D[x];
-}
-
-// Junction group of S members, all arithmetic in registers.  Member p < S-1
-// is eliminated into every later member (the operand sequences of
-// HinesMatrix::makeOperands, hsolve/HinesMatrix.cpp:225-365, and
-// HSolvePassive::forwardEliminate, hsolve/HSolvePassive.cpp:273-310).
-template <int S>
-__device__ void fwd_group(const Desc& u, Work w) {
-    int m[S];
-    double hj[S * (S - 1)];
-    double D[S], B[S];
-#pragma unroll
-    for (int p = 0; p < S; ++p) m[p] = member(u, p);
-    group_coefficients<S>(u, hj);
-    const int deferLast = u.a.y;
-#pragma unroll
-    for (int p = 0; p < S; ++p) {
-        D[p] = w.D[m[p]];
-        B[p] = w.B[m[p]];
-    }
-    int tb = u.a.z;
-#pragma unroll
-    for (int p = 0; p < S - 1; ++p) {
-        if (p >= 1) {
-            // the feeder's terms come after the earlier siblings' terms
-            const int cnt = (u.a.x >> (4 + 4 * p)) & 15;
-            for (int t = 0; t < cnt; ++t) {
-                D[p] -= w.T[2 * (tb + t)];
-                B[p] -= w.T[2 * (tb + t) + 1];
-            }
-            tb += cnt;
-        }
-        const double pivot = D[p];
-#pragma unroll
-        for (int q = p + 1; q < S; ++q) {
-            const int bq = hb(S, p) + 2 * (q - p - 1);
-            const double lower = hj[bq + 1];
-            const double tD = lower / pivot * hj[bq];
-            const double tB = lower / pivot * B[p];
-            if (q == S - 1 && deferLast >= 0) {
-                w.T[2 * (deferLast + p)] = tD;
-                w.T[2 * (deferLast + p) + 1] = tB;
-            } else {
-                D[q] -= tD;
-                B[q] -= tB;
-            }
-#pragma unroll
-            for (int z = p + 1; z < S; ++z) {
-                if (z == q) continue;
-                const double above = hj[hb(S, p) + 2 * (z - p - 1)];
-                const int idx = q < z ? hb(S, q) + 2 * (z - q - 1) : hb(S, z) + 2 * (q - z - 1) + 1;
-                hj[idx] -= lower / pivot * above;
-            }
-        }
-    }
-#pragma unroll
-    for (int p = 0; p < S - 1; ++p) {
-        w.D[m[p]] = D[p];
-        w.B[m[p]] = B[p];
-    }
-    if (deferLast < 0) {
-        w.D[m[S - 1]] = D[S - 1];
-        w.B[m[S - 1]] = B[S - 1];
-    }
-}
-
-// Back-substitution of a group.  The fill of the junction coefficients is
-// recomputed from the conductances and the stored pivots (same operations,
-// same order, so the same values as in the forward pass) instead of kept.
-template <int S>
-__device__ void bwd_group(const Desc& u, Work w) {
-    int m[S];
-    double hj[S * (S - 1)];
-    double D[S - 1], Vm[S];
-#pragma unroll
-    for (int p = 0; p < S; ++p) m[p] = member(u, p);
-    group_coefficients<S>(u, hj);
-#pragma unroll
-    for (int p = 0; p < S - 1; ++p) D[p] = w.D[m[p]];
-    Vm[S - 1] = w.B[m[S - 1]];
-#pragma unroll
-    for (int p = 0; p < S - 1; ++p) {
-        const double pivot = D[p];
-#pragma unroll
-        for (int q = p + 1; q < S; ++q) {
-            const double lower = hj[hb(S, p) + 2 * (q - p - 1) + 1];
-#pragma unroll
-            for (int z = p + 1; z < S; ++z) {
-                if (z == q) continue;
-                const double above = hj[hb(S, p) + 2 * (z - p - 1)];
-                const int idx = q < z ? hb(S, q) + 2 * (z - q - 1) : hb(S, z) + 2 * (q - z - 1) + 1;
-                hj[idx] -= lower / pivot * above;
-            }
-        }
-    }
-#pragma unroll
-    for (int p = S - 2; p >= 0; --p) {
-        double acc = w.B[m[p]];
-#pragma unroll
-        for (int q = S - 1; q > p; --q) acc -= Vm[q] * hj[hb(S, p) + 2 * (q - p - 1)];
-        Vm[p] = acc / D[p];
-        w.B[m[p]] = Vm[p];
-    }
-}
 
 // Groups larger than kFastGroupMax (or cells past 16-bit indices): member
 // lists, deferred slots and the working junction coefficients in memory.
@@ -496,21 +436,13 @@ __device__ void bwd_group_slow(char* arena, const CellDev& c, const Desc& u, Wor
 }
 
 __device__ __forceinline__ void forward_unit(char* arena, const CellDev& c, const Desc& u, Work w) {
-    switch (u.a.x & 31) {
-    case 0: fwd_linear(u, w); break;
-    case 2: fwd_group<2>(u, w); break;
-    case 3: fwd_group<3>(u, w); break;
-    default: fwd_group_slow(arena, c, u, w); break;
-    }
+    if (u.a.x & 16) fwd_group_slow(arena, c, u, w);
+    else fwd_small(u, w);
 }
 
 __device__ __forceinline__ void backward_unit(char* arena, const CellDev& c, const Desc& u, Work w) {
-    switch (u.a.x & 31) {
-    case 0: bwd_linear(u, w); break;
-    case 2: bwd_group<2>(u, w); break;
-    case 3: bwd_group<3>(u, w); break;
-    default: bwd_group_slow(arena, c, u, w); break;
-    }
+    if (u.a.x & 16) bwd_group_slow(arena, c, u, w);
+    else bwd_small(u, w);
 }
 
 // CaConcStruct::process (hsolve/HSolveStruct.cpp:138-155)
@@ -612,17 +544,20 @@ __global__ void __launch_bounds__(256) hsolve_step_kernel(LaunchArgs a) {
     const int nProbe = c.nProbe;
     const int64_t probeCap = c.probeCap;
     const int64_t probeBase = c.probeBase;
-    // work arrays: D[n], B[n], T[2*nTerm] (LDS when they fit), HJ (slow groups)
+    // work arrays: D[n], B[n], T[2*nTerm], F[nFill] (LDS when they fit),
+    // HJ (slow groups, HBM)
+    const int nWork = 2 * n + 2 * nTerm + c.nFill;
     double* gwork = A_<double>(arena, c.oWork);
     double* base = ldsWork ? lds : gwork;
     Work w;
     w.D = base;
     w.B = base + n;
     w.T = base + 2 * n;
-    w.HJ = gwork + 2 * n + 2 * nTerm;
+    w.F = base + 2 * n + 2 * nTerm;
+    w.HJ = gwork + nWork;
     // level table in LDS after the work arrays (or read from HBM)
     const int32_t* gLevel = A_<const int32_t>(arena, c.oLevelStart);
-    int32_t* sLevel = reinterpret_cast<int32_t*>(lds + (ldsWork ? 2 * n + 2 * nTerm : 0));
+    int32_t* sLevel = reinterpret_cast<int32_t*>(lds + (ldsWork ? nWork : 0));
     const bool levelInLds = c.ldsBytes > 0;
     if (levelInLds)
         for (int l = tid; l <= nLevels; l += nt) sLevel[l] = gLevel[l];

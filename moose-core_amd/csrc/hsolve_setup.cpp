@@ -164,8 +164,7 @@ uint64_t fnv(const void* p, size_t n, uint64_t h) {
 // c_{k-1}).  Such a feeder therefore parks its terms in a deferred slot that
 // the group applies right before it pivots c_k.
 // ---------------------------------------------------------------------------
-static int schedule(CellSetup& s, const std::vector<int>& junctionGroup, const std::vector<double>& Ga,
-                    std::string& err) {
+static int schedule(CellSetup& s, const std::vector<int>& junctionGroup, std::string& err) {
     const int n = s.n;
     // eliminating unit of every node; feeder unit of every node
     std::vector<int> elimUnit(n, -1), feeder(n, -1);
@@ -262,48 +261,66 @@ static int schedule(CellSetup& s, const std::vector<int>& junctionGroup, const s
         const Unit& u = units[ui];
         s.units[fill[u.level]++] = u.kind == 0 ? u.node : -(u.group + 1);
     }
-    // descriptors in the same (level) order
+    // descriptors in the same (level) order; within a level, fast units first
     s.udesc.assign((size_t)kDescInts * s.units.size(), 0);
     s.anySlowGroup = false;
+    s.nFill = 0;
+    auto is_slow = [&](int code) {
+        if (code >= 0) return false;
+        const int sz = (int)s.groups[-code - 1].mem.size();
+        return sz > kFastGroupMax || n > 65535;
+    };
+    for (int l = 0; l + 1 < (int)s.levelStart.size(); ++l)
+        std::stable_partition(s.units.begin() + s.levelStart[l], s.units.begin() + s.levelStart[l + 1],
+                              [&](int code) { return !is_slow(code); });
     for (size_t k = 0; k < s.units.size(); ++k) {
         int32_t* d = &s.udesc[kDescInts * k];
         const int code = s.units[k];
+        double a[3] = {0.0, 0.0, 0.0};
         if (code >= 0) {
+            // linear link: a two-member unit (x, x+1) with coefficient HS[4x+1]
             const int x = code;
-            d[0] = 0;
-            d[1] = x;
-            d[2] = s.defer[2 * (x + 1)];
-            if (d[2] >= 0 && s.defer[2 * (x + 1) + 1] != 1) { err = "internal: linear feeder with more than one term"; return HSG_E_UNSUPPORTED; }
-            std::memcpy(&d[4], &s.Off[x], sizeof(double));
-            continue;
-        }
-        const int g = -code - 1;
-        const auto& G = s.groups[g];
-        const int sz = (int)G.mem.size();
-        const bool slow = sz > kFastGroupMax || n > 65535;
-        int word = sz | (slow ? 16 : 0);
-        if (slow) {
-            s.anySlowGroup = true;
+            if (s.defer[2 * (x + 1)] >= 0 && s.defer[2 * (x + 1) + 1] != 1) {
+                err = "internal: linear feeder with more than one term";
+                return HSG_E_UNSUPPORTED;
+            }
+            d[0] = 2;
+            d[1] = s.defer[2 * (x + 1)];
+            d[2] = -1;
+            d[3] = (x & 0xffff) | ((x + 1) << 16);
+            d[4] = x + 1;
+            d[5] = -1;
+            a[0] = s.Off[x];
+            if (n > 65535) { err = "cells above 65535 compartments are not supported"; return HSG_E_UNSUPPORTED; }
+        } else {
+            const int g = -code - 1;
+            const auto& G = s.groups[g];
+            const int sz = (int)G.mem.size();
+            if (is_slow(code)) {
+                s.anySlowGroup = true;
+                d[0] = sz | 16;
+                d[1] = g;
+                continue;
+            }
+            int word = sz;
+            if (sz == 3) {
+                const int cnt = s.defer[2 * G.mem[1] + 1];
+                if (cnt > 15) { err = "internal: too many deferred terms"; return HSG_E_UNSUPPORTED; }
+                if (cnt > 0 && s.defer[2 * G.mem[1]] != termBase[g]) { err = "internal: deferred slots out of order"; return HSG_E_UNSUPPORTED; }
+                word |= cnt << 8;
+            }
             d[0] = word;
-            d[1] = g;
-            continue;
+            d[1] = G.deferLast;
+            d[2] = termBase[g];
+            d[3] = (G.mem[0] & 0xffff) | (G.mem[1] << 16);
+            d[4] = sz == 3 ? G.mem[2] : G.mem[1];
+            d[5] = sz == 3 ? s.nFill++ : -1;
+            // the junction coefficients as HJCopy_ holds them: per pair (a,b)
+            // of the sorted members, -Ga_a*Ga_b/gsum
+            if (sz == 2) a[0] = s.HJ0[G.hjOff];
+            else { a[0] = s.HJ0[G.hjOff]; a[1] = s.HJ0[G.hjOff + 2]; a[2] = s.HJ0[G.hjOff + 4]; }
         }
-        int expect = termBase[g];
-        for (int p = 1; p + 1 < sz; ++p) {
-            const int cnt = s.defer[2 * G.mem[p] + 1];
-            if (cnt > 15) { err = "internal: too many deferred terms"; return HSG_E_UNSUPPORTED; }
-            if (cnt > 0 && s.defer[2 * G.mem[p]] != expect) { err = "internal: deferred slots out of order"; return HSG_E_UNSUPPORTED; }
-            expect += cnt;
-            word |= cnt << (4 + 4 * p);
-        }
-        d[0] = word;
-        d[1] = G.deferLast;
-        d[2] = termBase[g];
-        for (int p = 0; p < sz; ++p) {
-            int32_t& w = d[3 + p / 2];
-            w |= (G.mem[p] & 0xffff) << (16 * (p % 2));
-            std::memcpy(&d[6 + 2 * p], &Ga[G.mem[p]], sizeof(double));
-        }
+        std::memcpy(&d[6], a, sizeof a);
     }
     return 0;
 }
@@ -412,7 +429,7 @@ int setup_cell(const hsg_cell_desc* d, double dt, int caAdvance, CellSetup& s, s
         s.maxGroup = std::max(s.maxGroup, (int)g.size());
         s.groups.push_back(std::move(G));
     }
-    rc = schedule(s, junctionGroup, Ga, err);
+    rc = schedule(s, junctionGroup, err);
     if (rc) return rc;
 
     // readHHChannels / readGates

@@ -67,6 +67,7 @@ struct CellSetup {
     bool anySlowGroup = false;
     std::vector<int> defer;           // 2n: (offset, count)
     int nTerm = 0;
+    int nFill = 0;
     int maxGroup = 0;
     // channels, slot-major
     int chanSlots = 0, gateSlots = 0, poolSlots = 0;
