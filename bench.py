@@ -78,6 +78,17 @@ def run_form(precision, packed, dt, steps, warmup, spl, graph, threads):
     return wall, launches, total_ms, mean_ms, soma, spikes
 
 
+def measured_traffic(key, compartment_steps):
+    """HBM bytes per launch from the committed FETCH_SIZE/WRITE_SIZE profile
+    of this kernel and workload (profiles/traffic.json), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            t = json.load(f)[key]
+    except (OSError, KeyError, ValueError):
+        return None
+    return (t["fetch_bytes_per_compartment_step"] + t["write_bytes_per_compartment_step"]) * compartment_steps
+
+
 def spike_steps(spikes, ncells):
     out = [[] for _ in range(ncells)]
     for r in spikes:
@@ -126,6 +137,18 @@ def cpu_baseline(cells, dt, budget_s=15.0):
     }
 
 
+def copy_probe(mib):
+    """Plain device-to-device copy of a known size, for counter calibration."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4, "launches": 5}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,7 +161,13 @@ def main():
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="mixed,fp64")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (calibrates "
+                         "the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(cells are independent and would shard by bin-packing across GPUs)"}))
@@ -158,6 +187,7 @@ def main():
     steps_per_launch = args.steps / max(1, launches)
     bytes_per_launch = BYTES_PER_CS[head] * ncomp * steps_per_launch
     achieved = bytes_per_launch / (mean_ms * 1e-3) / 1e9
+    traffic = measured_traffic(f"hsolve_step_kernel/{head}/hh_ensemble_{args.cells}", ncomp * steps_per_launch)
     out = {
         "metric": "compartment-steps/s (batched cells, FP32-channel mixed form)",
         "value": value,
@@ -177,7 +207,7 @@ def main():
                    "steps_per_launch": args.steps_per_launch, "graph_chunk": args.graph_chunk,
                    "parallelism": "1 GPU, one workgroup per cell"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "hsolve_step_kernel", "bytes_per_compartment_step": BYTES_PER_CS[head],
                      "mean_launch_ms": mean_ms},
     }
@@ -196,8 +226,16 @@ def main():
                 continue
             if a:
                 worst = max(worst, int(np.max(np.abs(np.array(a) - np.array(b)))))
+        per_cell = np.abs(d).max(axis=0)
+        spike_ok = np.array([len(a) == len(b) and (not a or np.max(np.abs(np.array(a) - np.array(b))) <= 1)
+                             for a, b in zip(s_mixed, s_64)])
         out["accuracy_mixed_vs_fp64"] = {
-            "max_abs_dVm": float(np.abs(d).max()), "rms_dVm": float(np.sqrt(np.mean(d * d))),
+            "max_abs_dVm": float(per_cell.max()), "rms_dVm": float(np.sqrt(np.mean(d * d))),
+            "per_cell_max_abs_dVm_p50": float(np.percentile(per_cell, 50)),
+            "per_cell_max_abs_dVm_p99": float(np.percentile(per_cell, 99)),
+            "cells_within_0.1mV": int(np.sum(per_cell <= 1e-4)),
+            "cells_within_0.1mV_and_spikes_within_1dt": int(np.sum((per_cell <= 1e-4) & spike_ok)),
+            "cells": len(cells),
             "spike_step_max_err": worst, "cells_with_spike_count_mismatch": mismatched,
             "spikes": int(sum(len(a) for a in s_64)), "probe": "soma of every cell, every step"}
         out["fp64_roofline_frac"] = (BYTES_PER_CS["fp64"] * ncomp * (args.steps / max(1, l64))

@@ -171,13 +171,33 @@ struct hsg_engine {
 
 namespace {
 
+// Copies a host vector into the arena and waits, so that the caller may free
+// the vector as soon as this returns.
 template <typename T>
 int upload(hsg_engine* e, int64_t off, const std::vector<T>& v) {
     if (v.empty()) return 0;
     HIPCHK(hipMemcpyAsync(e->arena.base + off, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice,
                           e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
     return 0;
 }
+
+// Host image of one arena region: arrays are placed into it and the region
+// goes to the device in a single copy.
+struct Stager {
+    int64_t base;
+    std::vector<char> img;
+    Stager(int64_t b, int64_t bytes) : base(b), img((size_t)bytes, 0) {}
+    template <typename T>
+    void put(int64_t off, const std::vector<T>& v) {
+        if (!v.empty()) std::memcpy(img.data() + (off - base), v.data(), v.size() * sizeof(T));
+    }
+    int flush(hsg_engine* e) {
+        HIPCHK(hipMemcpyAsync(e->arena.base + base, img.data(), img.size(), hipMemcpyHostToDevice, e->stream));
+        HIPCHK(hipStreamSynchronize(e->stream));
+        return 0;
+    }
+};
 
 int grow_arena(hsg_engine* e, int64_t need) {
     if (e->opt.mem_budget > 0) return fail(HSG_E_NOMEM, "device memory budget exceeded");
@@ -313,7 +333,7 @@ int place_cell(hsg_engine* e, HostCell& hc) {
     int64_t workB = 8 * (2 * (int64_t)n + 2 * (int64_t)s.nTerm + (int64_t)s.nFill +
                          (s.anySlowGroup ? (int64_t)s.HJ0.size() : 0));
     std::vector<Item> items = {
-        {&d.oV, 8 * n}, {&d.oCmByDt, 8 * n}, {&d.oEmByRm, 8 * n}, {&d.oDbase, 8 * n}, {&d.oOff, 8 * n},
+        {&d.oV, 8 * n}, {&d.oInitVm, 8 * n}, {&d.oCmByDt, 8 * n}, {&d.oEmByRm, 8 * n}, {&d.oDbase, 8 * n}, {&d.oOff, 8 * n},
         {&d.oDefer, 8 * n},
         {&d.oChType, 4 * CS}, {&d.oGbar, gt * CS}, {&d.oEk, gt * CS}, {&d.oMod, 8 * CS}, {&d.oExtCa, 8 * CS},
         {&d.oState, gt * GS},
@@ -374,18 +394,13 @@ int place_cell(hsg_engine* e, HostCell& hc) {
     for (int k = 0; k < kMaxSlots; ++k)
         d.slotType[k] = (k < s.chanSlots && s.uniform) ? chType[(int64_t)k * n] : -1;
 
-    // upload
-    std::vector<float> f;
-    auto up_gt = [&](int64_t off, const std::vector<double>& v) -> int {
-        if (e->fp32) {
-            f.assign(v.begin(), v.end());
-            return upload(e, off, f);
-        }
-        return upload(e, off, v);
+    // upload: the whole region in one copy (zero-filled, so zero arrays need
+    // no staging)
+    Stager st(base, total);
+    auto put_gt = [&](int64_t off, const std::vector<double>& v) {
+        if (e->fp32) st.put(off, std::vector<float>(v.begin(), v.end()));
+        else st.put(off, v);
     };
-    std::vector<int32_t> defer(s.defer.begin(), s.defer.end());
-    std::vector<int32_t> levelStart(s.levelStart.begin(), s.levelStart.end());
-    std::vector<int32_t> units(s.units.begin(), s.units.end());
     std::vector<int32_t> mem, info;
     for (auto& g : s.groups) {
         info.push_back((int32_t)mem.size());
@@ -394,32 +409,38 @@ int place_cell(hsg_engine* e, HostCell& hc) {
         info.push_back(g.deferLast);
         for (int x : g.mem) mem.push_back(x);
     }
-    std::vector<int32_t> poolCount(s.poolCount.begin(), s.poolCount.end());
-    std::vector<double> zerosCS(CS, 0.0), zerosPS(PS, 0.0);
-    std::vector<int32_t> spkCompt(s.spkCompt.begin(), s.spkCompt.end());
-    std::vector<int32_t> spkEdge(s.spkEdge.begin(), s.spkEdge.end());
-    std::vector<double> spkLast(s.spkCompt.size(), 0.0);
-    std::vector<int32_t> spkFired(s.spkCompt.size(), 0);
-    std::vector<int64_t> steps(1, 0);
-    if ((rc = upload(e, d.oV, s.V)) || (rc = upload(e, d.oCmByDt, s.CmByDt)) ||
-        (rc = upload(e, d.oEmByRm, s.EmByRm)) || (rc = upload(e, d.oDbase, s.Dbase)) ||
-        (rc = upload(e, d.oOff, s.Off)) || (rc = upload(e, d.oDefer, defer)) ||
-        (rc = upload(e, d.oChType, chType)) || (rc = up_gt(d.oGbar, s.Gbar)) || (rc = up_gt(d.oEk, s.Ek)) ||
-        (rc = upload(e, d.oMod, s.mod)) || (rc = upload(e, d.oExtCa, zerosCS)) ||
-        (rc = up_gt(d.oState, s.state)) || (rc = upload(e, d.oPoolC, s.pC)) ||
-        (rc = upload(e, d.oPoolCa, s.pCa)) || (rc = upload(e, d.oPoolBasal, s.pBasal)) ||
-        (rc = upload(e, d.oPoolF1, s.pF1)) || (rc = upload(e, d.oPoolF2, s.pF2)) ||
-        (rc = upload(e, d.oPoolCeil, s.pCeil)) || (rc = upload(e, d.oPoolFloor, s.pFloor)) ||
-        (rc = upload(e, d.oPoolICa, zerosPS)) || (rc = upload(e, d.oPoolCount, poolCount)) ||
-        (rc = upload(e, d.oLevelStart, levelStart)) || (rc = upload(e, d.oUnits, units)) ||
-        (rc = upload(e, d.oGrpMem, mem)) || (rc = upload(e, d.oGrpInfo, info)) ||
-        (rc = upload(e, d.oHJ0, s.HJ0)) || (rc = upload(e, d.oDesc, s.udesc)) ||
-        (rc = upload(e, d.oSpkCompt, spkCompt)) ||
-        (rc = upload(e, d.oSpkEdge, spkEdge)) || (rc = upload(e, d.oSpkThr, s.spkThr)) ||
-        (rc = upload(e, d.oSpkRefr, s.spkRefr)) || (rc = upload(e, d.oSpkLast, spkLast)) ||
-        (rc = upload(e, d.oSpkFired, spkFired)) || (rc = upload(e, d.oSteps, steps)))
-        return rc;
-    HIPCHK(hipStreamSynchronize(e->stream));
+    auto i32 = [](const std::vector<int>& v) { return std::vector<int32_t>(v.begin(), v.end()); };
+    st.put(d.oV, s.V);
+    st.put(d.oInitVm, s.initVm);
+    st.put(d.oCmByDt, s.CmByDt);
+    st.put(d.oEmByRm, s.EmByRm);
+    st.put(d.oDbase, s.Dbase);
+    st.put(d.oOff, s.Off);
+    st.put(d.oDefer, i32(s.defer));
+    st.put(d.oChType, chType);
+    put_gt(d.oGbar, s.Gbar);
+    put_gt(d.oEk, s.Ek);
+    st.put(d.oMod, s.mod);
+    put_gt(d.oState, s.state);
+    st.put(d.oPoolC, s.pC);
+    st.put(d.oPoolCa, s.pCa);
+    st.put(d.oPoolBasal, s.pBasal);
+    st.put(d.oPoolF1, s.pF1);
+    st.put(d.oPoolF2, s.pF2);
+    st.put(d.oPoolCeil, s.pCeil);
+    st.put(d.oPoolFloor, s.pFloor);
+    st.put(d.oPoolCount, i32(s.poolCount));
+    st.put(d.oLevelStart, i32(s.levelStart));
+    st.put(d.oUnits, i32(s.units));
+    st.put(d.oGrpMem, mem);
+    st.put(d.oGrpInfo, info);
+    st.put(d.oHJ0, s.HJ0);
+    st.put(d.oDesc, s.udesc);
+    st.put(d.oSpkCompt, i32(s.spkCompt));
+    st.put(d.oSpkEdge, i32(s.spkEdge));
+    st.put(d.oSpkThr, s.spkThr);
+    st.put(d.oSpkRefr, s.spkRefr);
+    if ((rc = st.flush(e))) return rc;
     d.slowHJ = s.anySlowGroup ? 1 : 0;
     // work arrays (and the level table) in LDS when they fit
     const int64_t wl = work_lds_bytes(d), lb = level_bytes(d);
@@ -819,28 +840,15 @@ int run_launches(hsg_engine* e, const std::vector<int32_t>& order, const std::ve
 }
 
 int reinit_cells_host(hsg_engine* e, HostCell& hc) {
+    // the device state (V, Ca pools, spike generators, gates) is reset by
+    // hsolve_reinit_kernel; here only the host mirrors
     CellSetup& s = hc.s;
-    const int n = s.n;
-    int rc;
-    if ((rc = upload(e, hc.dev.oV, s.initVm))) return rc;
-    // reinitCalcium: c = 0, ca = CaBasal, activation = 0
-    const size_t PS = (size_t)s.poolSlots * n;
-    std::vector<double> zeros(PS, 0.0);
-    if ((rc = upload(e, hc.dev.oPoolC, zeros)) || (rc = upload(e, hc.dev.oPoolCa, s.pBasal)) ||
-        (rc = upload(e, hc.dev.oPoolICa, zeros)))
-        return rc;
-    for (size_t o = 0; o < PS; ++o) s.pC[o] = 0.0;
-    // reinitSpikeGens: lastEvent = -refractT
-    std::vector<double> last(s.spkRefr.size());
-    for (size_t k = 0; k < last.size(); ++k) last[k] = -s.spkRefr[k];
-    if ((rc = upload(e, hc.dev.oSpkLast, last))) return rc;
-    std::vector<int64_t> zero1(1, 0);
-    if ((rc = upload(e, hc.dev.oSteps, zero1))) return rc;
+    for (auto& c : s.pC) c = 0.0;
     hc.dev.probeBase = 0;
     e->tablesDirty = true;
     hc.ext.clear();
     hc.ica.clear();
-    if (hc.dev.nExt) { hc.dev.nExt = 0; e->tablesDirty = true; }
+    if (hc.dev.nExt) hc.dev.nExt = 0;
     hc.hsolveDt = e->dt;
     hc.stepsDone = 0;
     return 0;
@@ -1182,7 +1190,10 @@ int hsg_set_field(hsg_engine* e, int32_t cell, int32_t field, const int32_t* idx
                 hc->injDirty = true;
                 break;
             }
-            case HSG_F_INITVM: s.initVm[h] = x; break;
+            case HSG_F_INITVM:
+                s.initVm[h] = x;
+                if ((rc = put1<double>(e, hc->dev.oInitVm, h, x))) return rc;
+                break;
             case HSG_F_CM:
                 s.Cm[h] = x;
                 s.CmByDt[h] = 2.0 * x / hc->hsolveDt;

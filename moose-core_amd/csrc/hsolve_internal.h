@@ -68,7 +68,7 @@ struct CellDev {
     int64_t probeCap;       // samples per probe
     int64_t probeBase;      // step count at which the current trace started
     // compartment arrays (f64 unless noted)
-    int64_t oV, oCmByDt, oEmByRm, oDbase, oOff;
+    int64_t oV, oInitVm, oCmByDt, oEmByRm, oDbase, oOff;
     int64_t oDefer;         // int32[n]: first deferred-term pair of node, -1
     // channels: slot-major [k*n + i]
     int64_t oChType;        // int32 (non-uniform cells)

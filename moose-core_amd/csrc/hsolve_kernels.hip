@@ -696,6 +696,25 @@ __global__ void __launch_bounds__(256) hsolve_reinit_kernel(LaunchArgs a) {
     const CellDev& c = a.cells[a.order[blockIdx.x]];
     const View<GT> v = make_view<GT>(a.arena, c, a.tsets);
     const ChanType* types = stage_types(c, a.types, sTypes);
+    // HSolveActive::reinit order (hsolve/HSolveActiveSetup.cpp:43-76):
+    // spike generators, compartments, Ca pools, then the gates at the new V/Ca
+    {
+        const double* initVm = A_<const double>(a.arena, c.oInitVm);
+        for (int i = threadIdx.x; i < v.n; i += blockDim.x) v.V[i] = initVm[i];
+        const int PS = c.nPoolSlots * v.n;
+        double* pc = A_<double>(a.arena, c.oPoolC);
+        double* ica = A_<double>(a.arena, c.oPoolICa);
+        const double* basal = A_<const double>(a.arena, c.oPoolBasal);
+        for (int o = threadIdx.x; o < PS; o += blockDim.x) {
+            pc[o] = 0.0;
+            v.poolCa[o] = basal[o];
+            ica[o] = 0.0;
+        }
+        double* last = A_<double>(a.arena, c.oSpkLast);
+        const double* refr = A_<const double>(a.arena, c.oSpkRefr);
+        for (int k = threadIdx.x; k < c.nSpike; k += blockDim.x) last[k] = -refr[k];
+        if (threadIdx.x == 0) *A_<int64_t>(a.arena, c.oSteps) = 0;
+    }
     __syncthreads();
     Work w{};
     if (v.uniform)
