@@ -142,7 +142,14 @@ typedef struct hsg_options {
     int32_t steps_per_launch; /* steps fused into one kernel launch (0 = default)   */
     int32_t graph_chunk;      /* launches captured per hipGraph (0 = no graph)      */
     int32_t block_threads;    /* threads per cell workgroup (0 = autotuned)         */
+    int32_t kernel_mode;      /* HSG_KERNEL_*: how a tick is split into kernels     */
 } hsg_options;
+
+/* kernel_mode values */
+#define HSG_KERNEL_AUTO   0   /* engine's choice (currently FUSED)                          */
+#define HSG_KERNEL_FUSED  1   /* one workgroup per cell runs whole ticks, many per launch  */
+#define HSG_KERNEL_SPLIT  2   /* per tick: a batch-wide channel kernel, then the per-cell
+                                 solve kernel (Hines elimination, Vm, Ca, spikes)          */
 
 /* Creates an engine (replaces Shell::doCreate("HSolve") and the HSolve
  * constructor, hsolve/HSolve.cpp:196-199). */

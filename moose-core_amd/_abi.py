@@ -21,6 +21,11 @@ HSG_E_UNSUPPORTED = -6
 HSG_FP64 = 0
 HSG_MIXED = 1
 
+HSG_KERNEL_AUTO = 0
+HSG_KERNEL_FUSED = 1
+HSG_KERNEL_SPLIT = 2
+KERNEL_MODES = {"auto": HSG_KERNEL_AUTO, "fused": HSG_KERNEL_FUSED, "split": HSG_KERNEL_SPLIT}
+
 FIELDS = {
     "Vm": 1, "Im": 2, "inject": 3, "initVm": 4, "Cm": 5, "Em": 6, "Rm": 7, "Ra": 8,
     "Gbar": 20, "Ek": 21, "Gk": 22, "Ik": 23, "X": 24, "Y": 25, "Z": 26,
@@ -52,7 +57,8 @@ class CellDesc(C.Structure):
 class Options(C.Structure):
     _fields_ = [("precision", C.c_int32), ("device", C.c_int32),
                 ("mem_budget", C.c_int64), ("steps_per_launch", C.c_int32),
-                ("graph_chunk", C.c_int32), ("block_threads", C.c_int32)]
+                ("graph_chunk", C.c_int32), ("block_threads", C.c_int32),
+                ("kernel_mode", C.c_int32)]
 
 
 class Spike(C.Structure):

@@ -25,6 +25,10 @@ extern "C" hipError_t hsg_launch_step(const hsg::LaunchArgs* a, int nblocks, int
 extern "C" hipError_t hsg_launch_reinit(const hsg::LaunchArgs* a, int nblocks, int fp32,
                                        hipStream_t stream);
 extern "C" hipError_t hsg_set_max_lds(int bytes);
+extern "C" hipError_t hsg_launch_solve(const hsg::LaunchArgs* a, int nblocks, int threads, int fp32,
+                                      hipStream_t stream);
+extern "C" hipError_t hsg_launch_channel(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, int fp32,
+                                        hipStream_t stream);
 
 using namespace hsg;
 
@@ -157,10 +161,13 @@ struct hsg_engine {
     struct Bin {
         std::vector<int32_t> cells;
         int32_t* dOrder = nullptr;
+        int2* dTiles = nullptr;     // channel-kernel tiles (split mode)
+        int nTiles = 0;
         int lds = 0;
         int threads = 256;
     };
     std::vector<Bin> bins;
+    bool split = false;          // channel phase as its own kernel
     hipStream_t binStream[3] = {nullptr, nullptr, nullptr};
     hipEvent_t evFork = nullptr, evJoin[3] = {nullptr, nullptr, nullptr};
     // graph of the steady-state launches
@@ -448,8 +455,8 @@ int place_cell(hsg_engine* e, HostCell& hc) {
         d.workInLds = 1;
         d.ldsBytes = (int32_t)align_up(wl + lb);
     } else {
-        d.workInLds = 0;
-        d.ldsBytes = lb <= 16384 ? (int32_t)align_up(lb) : 0;
+        d.workInLds = 0;   // work arrays and level table read from HBM
+        d.ldsBytes = 0;
     }
     return 0;
 }
@@ -630,7 +637,10 @@ int default_threads(int n) { return n >= 1024 ? 256 : n >= 256 ? 128 : 64; }
 // Splits cells (largest first) into up to three bins by LDS need; each bin is
 // one launch with its own dynamic LDS size and workgroup size.
 int make_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg_engine::Bin>& bins) {
-    for (auto& b : bins) if (b.dOrder) hipFree(b.dOrder);
+    for (auto& b : bins) {
+        if (b.dOrder) hipFree(b.dOrder);
+        if (b.dTiles) hipFree(b.dTiles);
+    }
     bins.clear();
     if (cells.empty()) return 0;
     std::vector<int32_t> sorted(cells);
@@ -663,6 +673,14 @@ int make_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg_
         HIPCHK(hipMalloc(&b.dOrder, sizeof(int32_t) * b.cells.size()));
         HIPCHK(hipMemcpyAsync(b.dOrder, b.cells.data(), sizeof(int32_t) * b.cells.size(), hipMemcpyHostToDevice,
                               e->stream));
+        // channel-kernel tiles: (cell, first compartment), kChannelTile each
+        std::vector<int2> tiles;
+        for (int id : b.cells)
+            for (int i0 = 0; i0 < e->cells[id]->dev.n; i0 += kChannelTile) tiles.push_back(make_int2(id, i0));
+        b.nTiles = (int)tiles.size();
+        HIPCHK(hipMalloc(&b.dTiles, sizeof(int2) * std::max<size_t>(1, tiles.size())));
+        HIPCHK(hipMemcpyAsync(b.dTiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, e->stream));
+        HIPCHK(hipStreamSynchronize(e->stream));
     }
     HIPCHK(hipStreamSynchronize(e->stream));
     return 0;
@@ -738,9 +756,19 @@ int launch_round(hsg_engine* e, std::vector<hsg_engine::Bin>& bins, int chunk, i
     for (size_t b = 0; b < bins.size(); ++b) {
         hipStream_t st = e->binStream[b];
         HIPCHK(hipStreamWaitEvent(st, e->evFork, 0));
-        LaunchArgs a = make_args(e, bins[b].dOrder, chunk, first);
-        a.ldsBytes = bins[b].lds;
-        HIPCHK(hsg_launch_step(&a, (int)bins[b].cells.size(), bins[b].threads, e->fp32, st));
+        if (e->split) {
+            // per tick: the channel kernel over the bin's tiles, then the solve
+            for (int k = 0; k < chunk; ++k) {
+                LaunchArgs a = make_args(e, bins[b].dOrder, 1, first && k == 0);
+                a.ldsBytes = bins[b].lds;
+                HIPCHK(hsg_launch_channel(&a, bins[b].dTiles, bins[b].nTiles, e->fp32, st));
+                HIPCHK(hsg_launch_solve(&a, (int)bins[b].cells.size(), bins[b].threads, e->fp32, st));
+            }
+        } else {
+            LaunchArgs a = make_args(e, bins[b].dOrder, chunk, first);
+            a.ldsBytes = bins[b].lds;
+            HIPCHK(hsg_launch_step(&a, (int)bins[b].cells.size(), bins[b].threads, e->fp32, st));
+        }
         HIPCHK(hipEventRecord(e->evJoin[b], st));
         HIPCHK(hipStreamWaitEvent(e->stream, e->evJoin[b], 0));
     }
@@ -871,6 +899,9 @@ int hsg_create(const hsg_options* opts, hsg_engine** out) {
     if (e->opt.precision != HSG_FP64 && e->opt.precision != HSG_MIXED)
         return fail(HSG_E_ARG, "precision must be HSG_FP64 or HSG_MIXED");
     e->fp32 = e->opt.precision == HSG_MIXED;
+    if (e->opt.kernel_mode < HSG_KERNEL_AUTO || e->opt.kernel_mode > HSG_KERNEL_SPLIT)
+        return fail(HSG_E_ARG, "kernel_mode must be HSG_KERNEL_AUTO, HSG_KERNEL_FUSED or HSG_KERNEL_SPLIT");
+    e->split = e->opt.kernel_mode == HSG_KERNEL_SPLIT;
     e->device = e->opt.device;
     int ndev = 0;
     HIPCHK(hipGetDeviceCount(&ndev));

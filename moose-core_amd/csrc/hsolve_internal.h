@@ -117,6 +117,7 @@ struct CellDev {
 //   groups of more than kFastGroupMax members (or cells past 16-bit indices)
 //   take the slow path: i1 = group index, everything else from memory
 constexpr int kFastGroupMax = 3;
+constexpr int kChannelTile = 256;  // compartments per workgroup of the channel kernel
 constexpr int kDescInts = 12;
 
 struct SpikeRec {

@@ -524,17 +524,19 @@ __device__ __forceinline__ const ChanType* stage_types(const CellDev& c, const C
 // ---------------------------------------------------------------------------
 // the step kernel
 // ---------------------------------------------------------------------------
-template <typename GT>
-__global__ void __launch_bounds__(256) hsolve_step_kernel(LaunchArgs a) {
-    extern __shared__ double lds[];
-    __shared__ ChanType sTypes[kMaxSlots];
-    const CellDev& c = a.cells[a.order[blockIdx.x]];
+// K ticks of one cell (one workgroup).  SPLIT: phase A ran in
+// hsolve_channel_kernel, which left the matrix rows in the cell's HBM work
+// arrays; the workgroup only stages them into LDS.  WL: the elimination work
+// arrays and the level table live in LDS (else in the cell's HBM work area);
+// a compile-time choice so that every access is a plain LDS or global
+// instruction rather than a generic (flat) one.
+template <typename GT, bool SPLIT, bool WL>
+__device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c, double* lds, ChanType* sTypes) {
     char* arena = a.arena;
     const View<GT> v = make_view<GT>(arena, c, a.tsets);
     const int n = v.n;
     const int tid = threadIdx.x;
     const int nt = blockDim.x;
-    const bool ldsWork = c.workInLds;
     const int nTerm = c.nTerm;
     const int nLevels = c.nLevels;
     const int nUnits = c.nUnits;
@@ -544,24 +546,32 @@ __global__ void __launch_bounds__(256) hsolve_step_kernel(LaunchArgs a) {
     const int nProbe = c.nProbe;
     const int64_t probeCap = c.probeCap;
     const int64_t probeBase = c.probeBase;
-    // work arrays: D[n], B[n], T[2*nTerm], F[nFill] (LDS when they fit),
-    // HJ (slow groups, HBM)
+    // work arrays: D[n], B[n], T[2*nTerm], F[nFill]; HJ (slow groups, HBM)
     const int nWork = 2 * n + 2 * nTerm + c.nFill;
     double* gwork = A_<double>(arena, c.oWork);
-    double* base = ldsWork ? lds : gwork;
     Work w;
-    w.D = base;
-    w.B = base + n;
-    w.T = base + 2 * n;
-    w.F = base + 2 * n + 2 * nTerm;
+    if constexpr (WL) {
+        w.D = lds;
+        w.B = lds + n;
+        w.T = lds + 2 * n;
+        w.F = lds + 2 * n + 2 * nTerm;
+    } else {
+        w.D = gwork;
+        w.B = gwork + n;
+        w.T = gwork + 2 * n;
+        w.F = gwork + 2 * n + 2 * nTerm;
+    }
     w.HJ = gwork + nWork;
-    // level table in LDS after the work arrays (or read from HBM)
+    // level table: in LDS after the work arrays, or read from HBM
     const int32_t* gLevel = A_<const int32_t>(arena, c.oLevelStart);
-    int32_t* sLevel = reinterpret_cast<int32_t*>(lds + (ldsWork ? nWork : 0));
-    const bool levelInLds = c.ldsBytes > 0;
-    if (levelInLds)
+    const int32_t* level;
+    if constexpr (WL) {
+        int32_t* sLevel = reinterpret_cast<int32_t*>(lds + nWork);
         for (int l = tid; l <= nLevels; l += nt) sLevel[l] = gLevel[l];
-    const int32_t* level = levelInLds ? sLevel : gLevel;
+        level = sLevel;
+    } else {
+        level = gLevel;
+    }
     const ChanType* types = stage_types(c, a.types, sTypes);
     const GT dtg = (GT)a.dt;
     const int4* desc = A_<const int4>(arena, c.oDesc);
@@ -576,12 +586,20 @@ __global__ void __launch_bounds__(256) hsolve_step_kernel(LaunchArgs a) {
         Desc cur{};
         if (tid < nUnits) cur = load_desc(desc, tid);
         // ---- A: channels, currents, matrix rows
-        if (uniform)
-            for (int i = tid; i < n; i += nt) channel_row<GT, true>(v, types, i, dtg, w, false);
-        else
-            for (int i = tid; i < n; i += nt) channel_row<GT, false>(v, types, i, dtg, w, false);
+        if constexpr (SPLIT) {
+            if constexpr (WL)
+                for (int i = tid; i < n; i += nt) {
+                    w.D[i] = gwork[i];
+                    w.B[i] = gwork[n + i];
+                }
+        } else {
+            if (uniform)
+                for (int i = tid; i < n; i += nt) channel_row<GT, true>(v, types, i, dtg, w, false);
+            else
+                for (int i = tid; i < n; i += nt) channel_row<GT, false>(v, types, i, dtg, w, false);
+        }
         if (nInj > 0 || (first && nExt > 0)) {
-            work_barrier(ldsWork);
+            work_barrier(WL);
             if (tid == 0) {
                 // injections: HS[4i+3] += injectVarying + injectBasal (HSolveActive.cpp:119-128)
                 const int32_t* ii = A_<const int32_t>(arena, c.oInjIdx);
@@ -604,36 +622,45 @@ __global__ void __launch_bounds__(256) hsolve_step_kernel(LaunchArgs a) {
                 }
             }
         }
-        work_barrier(ldsWork);
-        // ---- F: forward elimination, level by level; the descriptor of the
-        // next level is fetched while this one is processed
-        for (int l = 0; l < nLevels; ++l) {
-            const int s0 = level[l], s1 = level[l + 1];
-            Desc nxt{};
-            if (s1 + tid < nUnits) nxt = load_desc(desc, s1 + tid);
-            if (s0 + tid < s1) forward_unit(arena, c, cur, w);
-            for (int u = s0 + tid + nt; u < s1; u += nt) forward_unit(arena, c, load_desc(desc, u), w);
-            cur = nxt;
-            work_barrier(ldsWork);
-        }
-        // ---- B: back-substitution (VMid overwrites B)
-        if (tid == 0) w.B[n - 1] = w.B[n - 1] / w.D[n - 1];
-        if (nLevels > 0) {
-            const int s0 = level[nLevels - 1];
-            if (s0 + tid < nUnits) cur = load_desc(desc, s0 + tid);
-        }
-        work_barrier(ldsWork);
-        for (int l = nLevels - 1; l >= 0; --l) {
-            const int s0 = level[l], s1 = level[l + 1];
-            Desc nxt{};
-            if (l > 0) {
-                const int p0 = level[l - 1];
-                if (p0 + tid < s0) nxt = load_desc(desc, p0 + tid);
+        work_barrier(WL);
+        // ---- F: forward elimination, level by level.  The descriptors of
+        // the next level and the bound after it are fetched while this level
+        // is processed.
+        {
+            int s0 = nLevels > 0 ? level[0] : 0;
+            int s1 = nLevels > 0 ? level[1] : 0;
+            for (int l = 0; l < nLevels; ++l) {
+                const int s2 = l + 2 <= nLevels ? level[l + 2] : s1;
+                Desc nxt{};
+                if (s1 + tid < nUnits) nxt = load_desc(desc, s1 + tid);
+                if (s0 + tid < s1) forward_unit(arena, c, cur, w);
+                for (int u = s0 + tid + nt; u < s1; u += nt) forward_unit(arena, c, load_desc(desc, u), w);
+                cur = nxt;
+                s0 = s1;
+                s1 = s2;
+                work_barrier(WL);
             }
-            if (s0 + tid < s1) backward_unit(arena, c, cur, w);
-            for (int u = s0 + tid + nt; u < s1; u += nt) backward_unit(arena, c, load_desc(desc, u), w);
-            cur = nxt;
-            work_barrier(ldsWork);
+        }
+        // ---- B: back-substitution (VMid overwrites B), in reverse level order
+        if (tid == 0) w.B[n - 1] = w.B[n - 1] / w.D[n - 1];
+        {
+            int s1 = nLevels > 0 ? level[nLevels] : 0;
+            int s0 = nLevels > 0 ? level[nLevels - 1] : 0;
+            int p0 = nLevels > 1 ? level[nLevels - 2] : 0;
+            if (s0 + tid < s1) cur = load_desc(desc, s0 + tid);
+            work_barrier(WL);
+            for (int l = nLevels - 1; l >= 0; --l) {
+                const int pp = l >= 2 ? level[l - 2] : 0;
+                Desc nxt{};
+                if (l > 0 && p0 + tid < s0) nxt = load_desc(desc, p0 + tid);
+                if (s0 + tid < s1) backward_unit(arena, c, cur, w);
+                for (int u = s0 + tid + nt; u < s1; u += nt) backward_unit(arena, c, load_desc(desc, u), w);
+                cur = nxt;
+                s1 = s0;
+                s0 = p0;
+                p0 = pp;
+                work_barrier(WL);
+            }
         }
         // ---- D: Vm, calcium
         for (int i = tid; i < n; i += nt) v.V[i] = 2 * w.B[i] - v.V[i];
@@ -683,9 +710,53 @@ __global__ void __launch_bounds__(256) hsolve_step_kernel(LaunchArgs a) {
         }
         // the next step's phase A overwrites D/B that other threads may still
         // read; V and the gate states are touched only by their owner thread
-        work_barrier(ldsWork);
+        work_barrier(WL);
     }
     if (tid == 0) *stepCounter = step0 + a.nsteps;
+}
+
+template <typename GT>
+__global__ void __launch_bounds__(256) hsolve_step_kernel(LaunchArgs a) {
+    extern __shared__ double lds[];
+    __shared__ ChanType sTypes[kMaxSlots];
+    const CellDev& c = a.cells[a.order[blockIdx.x]];
+    if (c.workInLds) cell_steps<GT, false, true>(a, c, lds, sTypes);
+    else cell_steps<GT, false, false>(a, c, lds, sTypes);
+}
+
+// The solve half of a tick when the channel phase runs as its own kernel.
+template <typename GT>
+__global__ void __launch_bounds__(256) hsolve_solve_kernel(LaunchArgs a) {
+    extern __shared__ double lds[];
+    __shared__ ChanType sTypes[kMaxSlots];
+    const CellDev& c = a.cells[a.order[blockIdx.x]];
+    if (c.workInLds) cell_steps<GT, true, true>(a, c, lds, sTypes);
+    else cell_steps<GT, true, false>(a, c, lds, sTypes);
+}
+
+// Phase A as a kernel of its own: advanceChannels, calculateChannelCurrents
+// and the updateMatrix row (HSolveActive.cpp:218-332, 68-83, 85-116) for one
+// tile of `blockDim.x` consecutive compartments of one cell per workgroup,
+// one compartment per thread, over the whole batch.  The rows go to the
+// cell's HBM work arrays (D at the start, B after it), where the solve
+// kernel of the same tick picks them up.
+template <typename GT>
+__global__ void __launch_bounds__(256) hsolve_channel_kernel(LaunchArgs a, const int2* __restrict__ tiles) {
+    __shared__ ChanType sTypes[kMaxSlots];
+    const int2 tile = tiles[blockIdx.x];
+    const CellDev& c = a.cells[tile.x];
+    const View<GT> v = make_view<GT>(a.arena, c, a.tsets);
+    const ChanType* types = stage_types(c, a.types, sTypes);
+    __syncthreads();
+    double* gwork = A_<double>(a.arena, c.oWork);
+    Work w{};
+    w.D = gwork;
+    w.B = gwork + v.n;
+    const int i = tile.y + threadIdx.x;
+    if (i < v.n) {
+        if (v.uniform) channel_row<GT, true>(v, types, i, (GT)a.dt, w, false);
+        else channel_row<GT, false>(v, types, i, (GT)a.dt, w, false);
+    }
 }
 
 // HSolveActive::reinitChannels (hsolve/HSolveActiveSetup.cpp:79-160): gates at
@@ -737,6 +808,28 @@ extern "C" hipError_t hsg_launch_step(const hsg::LaunchArgs* a, int nblocks, int
     return hipGetLastError();
 }
 
+extern "C" hipError_t hsg_launch_solve(const hsg::LaunchArgs* a, int nblocks, int threads, int fp32,
+                                      hipStream_t stream) {
+    if (fp32)
+        hipLaunchKernelGGL(hsg::hsolve_solve_kernel<float>, dim3(nblocks), dim3(threads),
+                           a->ldsBytes, stream, *a);
+    else
+        hipLaunchKernelGGL(hsg::hsolve_solve_kernel<double>, dim3(nblocks), dim3(threads),
+                           a->ldsBytes, stream, *a);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t hsg_launch_channel(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, int fp32,
+                                        hipStream_t stream) {
+    if (fp32)
+        hipLaunchKernelGGL(hsg::hsolve_channel_kernel<float>, dim3(ntiles), dim3(hsg::kChannelTile), 0, stream,
+                           *a, tiles);
+    else
+        hipLaunchKernelGGL(hsg::hsolve_channel_kernel<double>, dim3(ntiles), dim3(hsg::kChannelTile), 0, stream,
+                           *a, tiles);
+    return hipGetLastError();
+}
+
 extern "C" hipError_t hsg_launch_reinit(const hsg::LaunchArgs* a, int nblocks, int fp32,
                                        hipStream_t stream) {
     if (fp32)
@@ -747,9 +840,11 @@ extern "C" hipError_t hsg_launch_reinit(const hsg::LaunchArgs* a, int nblocks, i
 }
 
 extern "C" hipError_t hsg_set_max_lds(int bytes) {
-    hipError_t e = hipFuncSetAttribute((const void*)hsg::hsolve_step_kernel<double>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    if (e != hipSuccess) return e;
-    return hipFuncSetAttribute((const void*)hsg::hsolve_step_kernel<float>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    const void* kernels[4] = {(const void*)hsg::hsolve_step_kernel<double>, (const void*)hsg::hsolve_step_kernel<float>,
+                              (const void*)hsg::hsolve_solve_kernel<double>, (const void*)hsg::hsolve_solve_kernel<float>};
+    for (const void* k : kernels) {
+        hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }

@@ -92,7 +92,7 @@ class Engine:
     """A batch of HSolve cells on one GPU."""
 
     def __init__(self, precision="fp64", device=0, mem_budget=0, steps_per_launch=0,
-                 graph_chunk=0, block_threads=0):
+                 graph_chunk=0, block_threads=0, kernel_mode="auto"):
         self.lib = load_library()
         o = Options()
         o.precision = _abi.HSG_MIXED if precision == "mixed" else _abi.HSG_FP64
@@ -101,6 +101,9 @@ class Engine:
         o.steps_per_launch = int(steps_per_launch)
         o.graph_chunk = int(graph_chunk)
         o.block_threads = int(block_threads)
+        if kernel_mode not in _abi.KERNEL_MODES:
+            raise ValueError(f"kernel_mode must be one of {sorted(_abi.KERNEL_MODES)}")
+        o.kernel_mode = _abi.KERNEL_MODES[kernel_mode]
         self.precision = precision
         h = C.c_void_p()
         self._check(self.lib.hsg_create(C.byref(o), C.byref(h)))

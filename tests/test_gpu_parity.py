@@ -60,11 +60,12 @@ def mixed_batch():
     return specs
 
 
-def test_fp64_batch_bit_exact(pkg, oracle_mod):
-    print("case: 10-cell batch (HH + Ca), 600 steps, fp64", flush=True)
+@pytest.mark.parametrize("mode", ["fused", "split"])
+def test_fp64_batch_bit_exact(pkg, oracle_mod, mode):
+    print(f"case: 10-cell batch (HH + Ca), 600 steps, fp64, {mode}", flush=True)
     specs = mixed_batch()
     dt, n = 25e-6, 600
-    ids, traces, spikes = run_gpu(pkg, specs, dt, n)
+    ids, traces, spikes = run_gpu(pkg, specs, dt, n, kernel_mode=mode)
     for cid, spec, gpu in zip(ids, specs, traces):
         _, ref, ref_spk = run_oracle(oracle_mod, spec, dt, n)
         assert np.abs(gpu - ref).max() <= FP64_TOL, cid
@@ -73,11 +74,12 @@ def test_fp64_batch_bit_exact(pkg, oracle_mod):
         assert list(mine["step"]) == list(ref_spk), cid
 
 
-def test_mixed_batch_within_tolerance(pkg, oracle_mod):
-    print("case: 10-cell batch, 600 steps, mixed", flush=True)
+@pytest.mark.parametrize("mode", ["fused", "split"])
+def test_mixed_batch_within_tolerance(pkg, oracle_mod, mode):
+    print(f"case: 10-cell batch, 600 steps, mixed, {mode}", flush=True)
     specs = mixed_batch()
     dt, n = 25e-6, 600
-    ids, traces, spikes = run_gpu(pkg, specs, dt, n, precision="mixed")
+    ids, traces, spikes = run_gpu(pkg, specs, dt, n, precision="mixed", kernel_mode=mode)
     worst = 0.0
     for cid, spec, gpu in zip(ids, specs, traces):
         _, ref, ref_spk = run_oracle(oracle_mod, spec, dt, n)
@@ -108,6 +110,9 @@ def test_alone_equals_in_mix(pkg, precision):
     ({"steps_per_launch": 10, "graph_chunk": 4}, [300]),
     ({"block_threads": 64}, [300]),
     ({"block_threads": 256, "steps_per_launch": 300}, [150, 150]),
+    ({"kernel_mode": "split", "steps_per_launch": 7}, [100, 1, 199]),
+    ({"kernel_mode": "split", "steps_per_launch": 10, "graph_chunk": 4}, [300]),
+    ({"kernel_mode": "split", "block_threads": 64}, [300]),
 ])
 def test_launch_shapes_and_split_runs(pkg, oracle_mod, opts, chunks):
     print(f"case: launch shape {opts} chunks {chunks}", flush=True)
@@ -119,13 +124,14 @@ def test_launch_shapes_and_split_runs(pkg, oracle_mod, opts, chunks):
         np.testing.assert_array_equal(gpu, ref)
 
 
-def test_large_cells_global_work(pkg, oracle_mod):
+@pytest.mark.parametrize("mode", ["fused", "split"])
+def test_large_cells_global_work(pkg, oracle_mod, mode):
     """n = 2000 (work arrays in LDS) and n = 6000 (work arrays in HBM)."""
-    print("case: large cells 2000/6000 compartments, 100 steps, fp64", flush=True)
+    print(f"case: large cells 2000/6000 compartments, 100 steps, fp64, {mode}", flush=True)
     rng = np.random.default_rng(9)
     specs = [workloads.random_hh_cell(rng, 2000, inject=2e-9), workloads.random_ca_cell(rng, 6000, inject=2e-9)]
     dt, n = 25e-6, 100
-    ids, traces, _ = run_gpu(pkg, specs, dt, n)
+    ids, traces, _ = run_gpu(pkg, specs, dt, n, kernel_mode=mode)
     for spec, gpu in zip(specs, traces):
         _, ref, _ = run_oracle(oracle_mod, spec, dt, n)
         np.testing.assert_array_equal(gpu, ref)
