@@ -623,43 +623,58 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
             }
         }
         work_barrier(WL);
-        // ---- F: forward elimination, level by level.  The descriptors of
-        // the next level and the bound after it are fetched while this level
-        // is processed.
+        // ---- F: forward elimination, level by level.  Two descriptor
+        // buffers alternate: while one level is processed from one, the next
+        // level's descriptors load into the other, so the loads stay in
+        // flight across the barrier (no register copy forces them to land
+        // before it).  Level bounds are read one level ahead.
         {
             int s0 = nLevels > 0 ? level[0] : 0;
             int s1 = nLevels > 0 ? level[1] : 0;
-            for (int l = 0; l < nLevels; ++l) {
+            Desc nxt{};
+            for (int l = 0; l < nLevels; l += 2) {
                 const int s2 = l + 2 <= nLevels ? level[l + 2] : s1;
-                Desc nxt{};
                 if (s1 + tid < nUnits) nxt = load_desc(desc, s1 + tid);
                 if (s0 + tid < s1) forward_unit(arena, c, cur, w);
                 for (int u = s0 + tid + nt; u < s1; u += nt) forward_unit(arena, c, load_desc(desc, u), w);
-                cur = nxt;
-                s0 = s1;
-                s1 = s2;
                 work_barrier(WL);
+                if (l + 1 >= nLevels) break;
+                const int s3 = l + 3 <= nLevels ? level[l + 3] : s2;
+                if (s2 + tid < nUnits) cur = load_desc(desc, s2 + tid);
+                if (s1 + tid < s2) forward_unit(arena, c, nxt, w);
+                for (int u = s1 + tid + nt; u < s2; u += nt) forward_unit(arena, c, load_desc(desc, u), w);
+                work_barrier(WL);
+                s0 = s2;
+                s1 = s3;
             }
         }
-        // ---- B: back-substitution (VMid overwrites B), in reverse level order
+        // ---- B: back-substitution (VMid overwrites B), in reverse level
+        // order, with the same alternating prefetch
         if (tid == 0) w.B[n - 1] = w.B[n - 1] / w.D[n - 1];
         {
             int s1 = nLevels > 0 ? level[nLevels] : 0;
             int s0 = nLevels > 0 ? level[nLevels - 1] : 0;
             int p0 = nLevels > 1 ? level[nLevels - 2] : 0;
             if (s0 + tid < s1) cur = load_desc(desc, s0 + tid);
+            Desc nxt{};
             work_barrier(WL);
-            for (int l = nLevels - 1; l >= 0; --l) {
+            for (int l = nLevels - 1; l >= 0; l -= 2) {
+                // level l from cur, level l-1 (units p0..s0) into nxt
                 const int pp = l >= 2 ? level[l - 2] : 0;
-                Desc nxt{};
                 if (l > 0 && p0 + tid < s0) nxt = load_desc(desc, p0 + tid);
                 if (s0 + tid < s1) backward_unit(arena, c, cur, w);
                 for (int u = s0 + tid + nt; u < s1; u += nt) backward_unit(arena, c, load_desc(desc, u), w);
-                cur = nxt;
-                s1 = s0;
-                s0 = p0;
-                p0 = pp;
                 work_barrier(WL);
+                if (l == 0) break;
+                // level l-1 from nxt, level l-2 (units pp..p0) into cur
+                const int ppp = l >= 3 ? level[l - 3] : 0;
+                if (l > 1 && pp + tid < p0) cur = load_desc(desc, pp + tid);
+                if (p0 + tid < s0) backward_unit(arena, c, nxt, w);
+                for (int u = p0 + tid + nt; u < s0; u += nt) backward_unit(arena, c, load_desc(desc, u), w);
+                work_barrier(WL);
+                s1 = p0;
+                s0 = pp;
+                p0 = ppp;
             }
         }
         // ---- D: Vm, calcium
