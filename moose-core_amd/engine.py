@@ -88,6 +88,10 @@ def _f64(a):
     return a, a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+SPIKE_DT = np.dtype([("cell", "i4"), ("spikegen", "i4"), ("step", "i8")])
+assert SPIKE_DT.itemsize == C.sizeof(Spike)
+
+
 class Engine:
     """A batch of HSolve cells on one GPU."""
 
@@ -189,16 +193,16 @@ class Engine:
         self._check(self.lib.hsg_add_ica(self.h, int(cell), int(pool), float(iCa)))
 
     # ---- outputs
-    def read_spikes(self, cap=1 << 20):
-        buf = (Spike * cap)()
+    def read_spikes(self, cap=1 << 16):
+        """Spikes since the last call (and clears them): records of
+        (cell, spikegen, step)."""
+        out = np.zeros(cap, dtype=SPIKE_DT)
         n = C.c_int32()
         dropped = C.c_int64()
-        self._check(self.lib.hsg_read_spikes(self.h, buf, cap, C.byref(n), C.byref(dropped)))
-        out = np.zeros(n.value, dtype=[("cell", "i4"), ("spikegen", "i4"), ("step", "i8")])
-        for k in range(n.value):
-            out[k] = (buf[k].cell, buf[k].spikegen, buf[k].step)
+        self._check(self.lib.hsg_read_spikes(self.h, out.ctypes.data_as(C.POINTER(Spike)), cap,
+                                             C.byref(n), C.byref(dropped)))
         self.spikes_dropped = dropped.value
-        return out
+        return out[: n.value].copy()
 
     def set_probes(self, cell, compartments, capacity):
         c, cp = _i32(compartments)

@@ -1,0 +1,546 @@
+"""A pymoose-style object layer over the engine: build a model from
+Compartment, HHChannel, HHGate, CaConc, SpikeGen, Leakage and Table objects
+connected by messages, hand it to an HSolve through `target`, then
+`reinit()` / `start(t)` and read or write the (zombified) fields.
+
+This mirrors the part of the pymoose surface that an HSolve-managed model
+uses (python/moose/__init__.py:57-66 generates the classes from the Cinfos):
+
+  objects and paths        moose.Neutral/Compartment/...('/path'), element(),
+                           connect(src, 'srcField', dest, 'destField'), delete()
+  HSolve                   fields dt, target, caAdvance, seed, vDiv, vMin,
+                           vMax, caDiv, caMin, caMax (hsolve/HSolve.cpp:37-195);
+                           setting target runs setup (HSolve::setPath,
+                           hsolve/HSolve.cpp:313-334), which needs dt first
+  clock                    reinit(), start(runtime), setClock()
+  zombie fields            Compartment (biophysics/CompartmentBase.cpp:191-235),
+                           HHChannel (biophysics/ChanBase.cpp:90-117,
+                           HHChannelBase.cpp:31-76), CaConc
+                           (biophysics/CaConcBase.cpp:58-140): reads and writes
+                           go to the solver once the model is taken over
+                           (hsolve/HSolveInterface.cpp)
+
+Message order is creation order, and object ids grow with creation, so the
+solver sees the model exactly as MOOSE would (Hines numbering, channel order,
+gate order).  Only models run by an HSolve advance: `start` steps every
+HSolve's cells on the GPU; there is no per-object CPU integration here.
+"""
+import math
+
+import numpy as np
+
+from . import _abi
+from .cellspec import CellSpec, Gate
+
+FARADAY = 96485.3415      # basecode/header.h:30
+CA_VALENCE = 2            # biophysics/CaConcBase.cpp:15
+
+_objects = {}             # path -> element
+_next_id = [1]
+_msgs = []                # (src, srcField, dest, destField) in creation order
+_solvers = []             # HSolve elements with a target
+_engines = {}             # (precision, dt) -> Engine
+_precision = ["fp64"]
+
+
+def _norm(path):
+    if not path.startswith("/"):
+        path = "/" + path
+    return path.rstrip("/") or "/"
+
+
+class Neutral:
+    """A MOOSE element: a path, a unique id in creation order, children."""
+    _zfields = ()
+
+    def __init__(self, path="/"):
+        path = _norm(path)
+        if path in _objects:
+            old = _objects[path]
+            if type(old) is not type(self) and type(self) is not Neutral:
+                raise ValueError(f"{path} already exists as a {type(old).__name__}")
+            self.__dict__ = old.__dict__
+            return
+        parent = path.rsplit("/", 1)[0] or "/"
+        if path != "/" and parent not in _objects:
+            raise ValueError(f"cannot create {path}: parent {parent} does not exist")
+        d = self.__dict__
+        d["path"] = path
+        d["name"] = path.rsplit("/", 1)[1] if path != "/" else "/"
+        d["id"] = _next_id[0]
+        _next_id[0] += 1
+        d["children"] = []
+        d["_solver"] = None       # (HSolve, index) once zombified
+        if path != "/":
+            _objects[parent].children.append(self)
+        _objects[path] = self
+        self._init_fields()
+
+    def _init_fields(self):
+        pass
+
+    @property
+    def className(self):
+        return type(self).__name__
+
+    def isA(self, name):
+        return any(c.__name__ == name for c in type(self).__mro__)
+
+    def __repr__(self):
+        return f"<moose.{self.className}: id={self.id}, path={self.path}>"
+
+    # zombie field access: a taken-over object reads and writes the solver
+    def __getattr__(self, name):
+        d = self.__dict__
+        if name in type(self)._zfields and d.get("_solver") is not None:
+            hs, idx = d["_solver"]
+            return hs._get(self, name, idx)
+        raise AttributeError(f"{type(self).__name__} has no field {name}")
+
+    def __setattr__(self, name, value):
+        d = self.__dict__
+        if name in type(self)._zfields and d.get("_solver") is not None:
+            hs, idx = d["_solver"]
+            hs._set(self, name, idx, value)
+            return
+        self._set_local(name, value)
+
+    def _set_local(self, name, value):
+        object.__setattr__(self, name, value)
+
+
+class Compartment(Neutral):
+    _zfields = ("Vm", "Im", "inject", "initVm", "Cm", "Em", "Rm", "Ra")
+
+    def _init_fields(self):
+        d = self.__dict__
+        # biophysics/CompartmentBase.cpp defaults
+        d.update(Vm=-0.06, Cm=1.0, Em=-0.06, initVm=-0.06, Rm=1.0, Ra=1.0, inject=0.0, Im=0.0,
+                 length=0.0, diameter=0.0)
+
+
+class SymCompartment(Compartment):
+    pass
+
+
+class HHGate(Neutral):
+    """An HHGate: tables A (alpha) and B (alpha + beta) over [min, max]."""
+
+    def _init_fields(self):
+        d = self.__dict__
+        d.update(min=0.0, max=1.0, divs=0, tableA=np.zeros(1), tableB=np.zeros(1))
+
+    def _set_local(self, name, value):
+        if name in ("tableA", "tableB"):
+            value = np.array(value, dtype=np.float64)
+            object.__setattr__(self, "divs", len(value) - 1)
+        object.__setattr__(self, name, value)
+
+
+class HHChannel(Neutral):
+    _zfields = ("Gbar", "Ek", "Gk", "Ik", "X", "Y", "Z", "modulation", "instant")
+
+    def _init_fields(self):
+        d = self.__dict__
+        d.update(Gbar=0.0, Ek=0.0, Gk=0.0, Ik=0.0, Xpower=0.0, Ypower=0.0, Zpower=0.0,
+                 X=0.0, Y=0.0, Z=0.0, instant=0, useConcentration=0, modulation=1.0)
+
+    def _set_local(self, name, value):
+        object.__setattr__(self, name, value)
+        # setting a power creates the gate child (HHChannelBase::setXpower)
+        for p, g in (("Xpower", "gateX"), ("Ypower", "gateY"), ("Zpower", "gateZ")):
+            if name == p and value > 0 and _norm(self.path + "/" + g) not in _objects:
+                HHGate(self.path + "/" + g)
+
+    def gate(self, which):
+        """The gate prototype: a copied channel uses its original's gates
+        (HSolveUtils::gates with getOriginals, hsolve/HSolveUtils.cpp:75-120)."""
+        src = self.__dict__.get("_original") or self
+        return _objects.get(_norm(src.path + "/gate" + which))
+
+    @property
+    def gateX(self):
+        return self.gate("X")
+
+    @property
+    def gateY(self):
+        return self.gate("Y")
+
+    @property
+    def gateZ(self):
+        return self.gate("Z")
+
+
+class CaConc(Neutral):
+    _zfields = ("Ca", "CaBasal", "ceiling", "floor")
+
+    def _init_fields(self):
+        d = self.__dict__
+        d.update(Ca=0.0, CaBasal=0.0, tau=1.0, B=1.0, thick=0.0, ceiling=1e9, floor=0.0,
+                 length=0.0, diameter=0.0)
+
+    def shell_B(self):
+        """B as the solver takes it over.  Zombifying a CaConc re-applies
+        its geometry through CaConcBase::updateDimensions
+        (biophysics/CaConcBase.cpp:259-269, :385-388): B = 1/(2 F vol) of
+        the shell of thickness `thick`.  When length or diameter was never
+        set the reference reads uninitialised members there; this layer then
+        keeps the B the model set."""
+        if not (self.length > 0 and self.diameter > 0):
+            return self.B
+        vol = math.pi * self.diameter * self.diameter * self.length * 0.25
+        if 0 < self.thick < self.diameter / 2.0:
+            core = self.diameter / 2.0 - self.thick
+            vol -= math.pi * core * core * self.length
+        return 1.0 / (CA_VALENCE * FARADAY * vol)
+
+
+class SpikeGen(Neutral):
+    def _init_fields(self):
+        # biophysics/SpikeGen.cpp:127-134
+        self.__dict__.update(threshold=0.0, refractT=0.0, edgeTriggered=1)
+
+
+class Leakage(Neutral):
+    def _init_fields(self):
+        self.__dict__.update(Gk=0.0, Ek=0.0)
+
+
+class Table(Neutral):
+    """Records a compartment's Vm each step (requestOut -> getVm) or a
+    SpikeGen's spike times (spikeOut -> spike)."""
+
+    def _init_fields(self):
+        self.__dict__.update(_data=[])
+
+    @property
+    def vector(self):
+        return np.array(self._data, dtype=np.float64)
+
+    def clearVec(self):
+        self._data.clear()
+
+
+def element(path):
+    if isinstance(path, Neutral):
+        return path
+    p = _norm(path)
+    if p not in _objects:
+        raise ValueError(f"no element at {p}")
+    return _objects[p]
+
+
+def exists(path):
+    return _norm(path) in _objects
+
+
+def connect(src, srcField, dest, destField, msgType="Single"):
+    src, dest = element(src), element(dest)
+    _msgs.append((src, srcField, dest, destField))
+    return len(_msgs) - 1
+
+
+def copy(orig, parent, name=None):
+    """moose.copy of one element (fields only; a copied HHChannel keeps
+    reading its original's gates, as MOOSE's copies share the prototype
+    gates)."""
+    orig, parent = element(orig), element(parent)
+    name = name or orig.name
+    new = type(orig)(parent.path.rstrip("/") + "/" + name)
+    for k, v in orig.__dict__.items():
+        if k in ("path", "name", "id", "children", "_solver"):
+            continue
+        new.__dict__[k] = v.copy() if isinstance(v, (list, np.ndarray)) else v
+    if isinstance(orig, HHChannel):
+        new.__dict__["_original"] = orig.__dict__.get("_original") or orig
+    return new
+
+
+def delete(path):
+    obj = element(path)
+    for c in list(obj.children):
+        delete(c)
+    parent = _objects[obj.path.rsplit("/", 1)[0] or "/"]
+    parent.children.remove(obj)
+    del _objects[obj.path]
+    _msgs[:] = [m for m in _msgs if m[0] is not obj and m[2] is not obj]
+
+
+def reset():
+    """Forgets every element and solver (a fresh MOOSE session)."""
+    for eng in _engines.values():
+        eng.close()
+    _engines.clear()
+    _objects.clear()
+    _msgs.clear()
+    _solvers.clear()
+    _next_id[0] = 1
+    Neutral("/")
+
+
+def set_precision(precision):
+    """'fp64' (the reference's arithmetic) or 'mixed' (FP32 channels) for
+    solvers set up after this call."""
+    if precision not in ("fp64", "mixed"):
+        raise ValueError(precision)
+    _precision[0] = precision
+
+
+def _targets(obj, field, cls=None):
+    """Elements at the far end of messages leaving `obj` from `field`."""
+    return [d for s, sf, d, df in _msgs if s is obj and sf == field and (cls is None or d.isA(cls))]
+
+
+def _sources(obj, field, cls=None):
+    """Elements sending messages into `obj`'s `field`."""
+    return [s for s, sf, d, df in _msgs if d is obj and df == field and (cls is None or s.isA(cls))]
+
+
+def _adjacent(comp):
+    """HSolveUtils::adjacent (hsolve/HSolveUtils.cpp:36-45): axial targets,
+    then raxial targets (the parent, reached through the same message)."""
+    out = [d for s, sf, d, df in _msgs if s is comp and sf == "axial" and d.isA("Compartment")]
+    out += [s for s, sf, d, df in _msgs if d is comp and df == "raxial" and sf == "axial" and s.isA("Compartment")]
+    return out
+
+
+def _channels(comp, cls):
+    """Channels on a compartment's "channel" messages, in message order."""
+    out = []
+    for s, sf, d, df in _msgs:
+        if d is comp and df == "channel" and sf == "channel" and s.isA(cls):
+            out.append(s)
+        elif s is comp and sf == "channel" and df == "channel" and d.isA(cls):
+            out.append(d)
+    return out
+
+
+def _deep_search_compartment(base):
+    """HSolve::deepSearchForCompartment (hsolve/HSolve.cpp:346-393): depth
+    first, the last child of each level first."""
+    cstack = [[base]]
+    while cstack:
+        if not cstack[-1]:
+            cstack.pop()
+            if cstack:
+                cstack[-1].pop()
+            continue
+        cur = cstack[-1][-1]
+        if cur.isA("Compartment"):
+            return cur
+        cstack.append(list(cur.children))
+    return None
+
+
+class HSolve(Neutral):
+    """moose.HSolve: set dt, then target (a path at or above the cell's
+    compartments); the model is read, taken over and advanced on the GPU."""
+
+    def _init_fields(self):
+        self.__dict__.update(dt=0.0, caAdvance=1, vDiv=3000, vMin=-0.100, vMax=0.050,
+                             caDiv=3000, caMin=0.0, caMax=1000.0, _target="", seed=None,
+                             _cell=None, _engine=None, _objs=None, _tables=[], _spk=[])
+
+    @property
+    def target(self):
+        return self._target
+
+    @target.setter
+    def target(self, path):
+        if not self.dt > 0:
+            raise ValueError("HSolve::setPath(): Must set 'dt' first.")
+        seed = _deep_search_compartment(element(path))
+        if seed is None:
+            raise ValueError(f"HSolve::setPath(): No compartments found at or below '{path}'.")
+        object.__setattr__(self, "_target", path)
+        object.__setattr__(self, "seed", seed)
+        self._setup(seed)
+
+    def _set_local(self, name, value):
+        if name == "dt" and value < 0:
+            raise ValueError("HSolve: 'dt' must be positive.")
+        if name == "target":
+            HSolve.target.fset(self, value)
+            return
+        object.__setattr__(self, name, value)
+
+    # -- setup: the model in MOOSE object order --------------------------
+    def build_spec(self, seed=None):
+        """The cell description of the model reachable from `seed`, in
+        creation/message order (a CellSpec), plus the objects behind each
+        index."""
+        seed = seed or self.seed
+        comps, seen, todo = [], {seed.id}, [seed]
+        while todo:                           # everything connected to the seed
+            c = todo.pop()
+            comps.append(c)
+            for a in _adjacent(c):
+                if a.id not in seen:
+                    seen.add(a.id)
+                    todo.append(a)
+        comps.sort(key=lambda o: o.id)        # creation order
+        cidx = {c.id: i for i, c in enumerate(comps)}
+        ct = np.zeros(len(comps), _abi.COMPARTMENT_DT)
+        for i, c in enumerate(comps):
+            d = c.__dict__
+            ct[i] = (d["Ra"], d["Rm"], d["Cm"], d["Em"], d["initVm"], d["Vm"], d["inject"])
+        axial = [(cidx[s.id], cidx[d.id]) for s, sf, d, df in _msgs
+                 if sf == "axial" and df == "raxial" and s.id in cidx and d.id in cidx]
+        chans, gates, pools, leaks, spikes = [], {}, [], [], []
+        pidx = {}
+        for c in comps:
+            for ch in _channels(c, "HHChannel"):
+                chans.append((cidx[c.id], ch))
+            for lk in _channels(c, "Leakage"):
+                leaks.append((cidx[c.id], lk.__dict__["Gk"], lk.__dict__["Ek"]))
+            for sg in _targets(c, "VmOut", "SpikeGen"):
+                spikes.append((cidx[c.id], sg))
+        crec = np.zeros(len(chans), _abi.CHANNEL_DT)
+        for k, (ci, ch) in enumerate(chans):
+            d = ch.__dict__
+            g = []
+            for w in "XYZ":
+                gt = ch.gate(w) if d[w + "power"] > 0 else None
+                if gt is not None and gt.id not in gates:
+                    gates[gt.id] = gt
+                g.append(gt.id if gt is not None else -1)
+            tgt = _targets(ch, "IkOut", "CaConc")
+            dep = _targets(ch, "concen", "CaConc") + _sources(ch, "concen", "CaConc")
+            for p in tgt + dep:
+                if p.id not in pidx:
+                    pidx[p.id] = len(pools)
+                    pools.append(p)
+            crec[k] = (ci, d["instant"], d["useConcentration"], g,
+                       pidx[tgt[0].id] if tgt else -1, pidx[dep[-1].id] if dep else -1,
+                       d["Gbar"], d["Ek"], d["Xpower"], d["Ypower"], d["Zpower"],
+                       d["X"], d["Y"], d["Z"], d["modulation"])
+        gid = sorted(gates)
+        gpos = {g: i for i, g in enumerate(gid)}
+        for k in range(len(crec)):
+            crec[k]["gate"] = [gpos[g] if g >= 0 else -1 for g in crec[k]["gate"]]
+        glist = []
+        for g in gid:
+            gt = gates[g]
+            glist.append(Gate(g, gt.min, gt.max, int(gt.divs), np.asarray(gt.tableA, np.float64),
+                              np.asarray(gt.tableB, np.float64)))
+        prec = np.zeros(len(pools), _abi.CACONC_DT)
+        for q, p in enumerate(pools):
+            d = p.__dict__
+            prec[q] = (d["Ca"], d["CaBasal"], d["tau"], p.shell_B(), d["ceiling"], d["floor"])
+        srec = np.zeros(len(spikes), _abi.SPIKEGEN_DT)
+        for k, (ci, sg) in enumerate(spikes):
+            d = sg.__dict__
+            srec[k] = (ci, int(d["edgeTriggered"]), d["threshold"], d["refractT"])
+        lrec = np.zeros(len(leaks), _abi.LEAK_DT)
+        for k, rec in enumerate(leaks):
+            lrec[k] = rec
+        spec = CellSpec(ct, np.array(axial, dtype=np.int32).reshape(-1, 2), seed=cidx[seed.id],
+                        channels=crec, gates=glist, pools=prec, spikegens=srec, leaks=lrec)
+        objs = {"compartments": comps, "channels": [ch for _, ch in chans], "pools": pools,
+                "spikegens": [sg for _, sg in spikes]}
+        return spec, objs
+
+    def _setup(self, seed):
+        from .engine import Engine
+        spec, objs = self.build_spec(seed)
+        key = (_precision[0], float(self.dt))
+        if key not in _engines:
+            _engines[key] = Engine(_precision[0])
+        eng = _engines[key]
+        cell = eng.add_cell(spec, float(self.dt), int(self.caAdvance))
+        object.__setattr__(self, "_engine", eng)
+        object.__setattr__(self, "_cell", cell)
+        object.__setattr__(self, "_objs", objs)
+        object.__setattr__(self, "_spec", spec)
+        # zombify (hsolve/HSolve.cpp:235-262): fields now live in the solver
+        for kind in ("compartments", "channels", "pools"):
+            for i, o in enumerate(objs[kind]):
+                o.__dict__["_solver"] = (self, i)
+                for f in type(o)._zfields:       # reads now reach the solver
+                    o.__dict__.pop(f, None)
+        _solvers.append(self)
+
+    # -- zombie field access (hsolve/HSolveInterface.cpp) ----------------
+    def _get(self, obj, field, idx):
+        return float(self._engine.get(self._cell, field, [idx])[0])
+
+    def _set(self, obj, field, idx, value):
+        self._engine.set(self._cell, field, [idx], [float(value)])
+
+    # -- recording -------------------------------------------------------
+    def _recorders(self):
+        comps = self._objs["compartments"]
+        cpos = {c.id: i for i, c in enumerate(comps)}
+        vm = []
+        for s, sf, d, df in _msgs:
+            if sf == "requestOut" and df == "getVm" and isinstance(s, Table) and d.id in cpos:
+                vm.append((s, cpos[d.id]))
+        spk = []
+        sgs = self._objs["spikegens"]
+        spos = {g.id: i for i, g in enumerate(sgs)}
+        for s, sf, d, df in _msgs:
+            if sf == "spikeOut" and isinstance(d, Table) and s.id in spos:
+                spk.append((d, spos[s.id]))
+        return vm, spk
+
+
+def setClock(tick, dt):
+    """Accepted for compatibility; each HSolve advances at its own dt."""
+    return None
+
+
+def reinit():
+    """HSolve::reinit on every solver (hsolve/HSolveActiveSetup.cpp:43-52);
+    Tables record their t = 0 sample."""
+    done = set()
+    for hs in _solvers:
+        eng = hs._engine
+        if id(eng) not in done:
+            eng.reinit(float(hs.dt))
+            eng.read_spikes()
+            done.add(id(eng))
+    for hs in _solvers:
+        vm, spk = hs._recorders()
+        for t, i in vm:
+            t.clearVec()
+            t._data.append(float(hs._engine.get(hs._cell, "Vm", [i])[0]))
+        for t, _ in spk:
+            t.clearVec()
+        hs.__dict__["_time"] = 0.0
+        hs.__dict__["_steps"] = 0
+
+
+def start(runtime):
+    """Advances every HSolve by round(runtime / dt) steps on the GPU and
+    fills the Tables connected to its compartments and spike generators."""
+    byeng = {}
+    for hs in _solvers:
+        byeng.setdefault(id(hs._engine), []).append(hs)
+    for group in byeng.values():
+        eng = group[0]._engine
+        dt = float(group[0].dt)
+        nsteps = int(round(runtime / dt))
+        if nsteps <= 0:
+            continue
+        rec = []
+        for hs in group:
+            vm, spk = hs._recorders()
+            comps = sorted({i for _, i in vm})
+            if comps:
+                eng.set_probes(hs._cell, comps, nsteps)
+            rec.append((hs, vm, spk, comps))
+        eng.step(nsteps)
+        spikes = eng.read_spikes()
+        for hs, vm, spk, comps in rec:
+            if comps:
+                tr = eng.read_probes(hs._cell)
+                col = {c: j for j, c in enumerate(comps)}
+                for t, i in vm:
+                    t._data.extend(tr[:, col[i]].tolist())
+            mine = spikes[spikes["cell"] == hs._cell]
+            for t, g in spk:
+                t._data.extend((mine[mine["spikegen"] == g]["step"] * dt).tolist())
+            hs.__dict__["_steps"] = hs.__dict__.get("_steps", 0) + nsteps
+
+
+reset()
