@@ -52,8 +52,9 @@ def build_cells(ncells, seed):
     return cells
 
 
-def run_form(precision, packed, dt, steps, warmup, spl, graph, threads):
-    eng = pkg.Engine(precision, steps_per_launch=spl, graph_chunk=graph, block_threads=threads)
+def run_form(precision, packed, dt, steps, warmup, spl, graph, threads, mode="fused"):
+    eng = pkg.Engine(precision, steps_per_launch=spl, graph_chunk=graph, block_threads=threads,
+                     kernel_mode=mode)
     t = time.time()
     ids = [eng.add_cell(p, dt) for p in packed]
     log(f"[bench] {precision}: {len(ids)} cells set up in {time.time() - t:.1f}s")
@@ -159,6 +160,7 @@ def main():
     ap.add_argument("--steps-per-launch", type=int, default=50)
     ap.add_argument("--graph-chunk", type=int, default=0)
     ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--kernel-mode", default="fused", choices=["fused", "split", "auto"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="mixed,fp64")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -179,7 +181,7 @@ def main():
     res = {}
     for form in args.forms.split(","):
         res[form] = run_form(form, packed, dt, args.steps, args.warmup, args.steps_per_launch,
-                             args.graph_chunk, args.threads)
+                             args.graph_chunk, args.threads, args.kernel_mode)
     head = "mixed" if "mixed" in res else list(res)[0]
     wall, launches, total_ms, mean_ms, soma, spikes = res[head]
     value = ncomp * args.steps / wall

@@ -29,6 +29,8 @@ extern "C" hipError_t hsg_launch_solve(const hsg::LaunchArgs* a, int nblocks, in
                                       hipStream_t stream);
 extern "C" hipError_t hsg_launch_channel(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, int fp32,
                                         hipStream_t stream);
+extern "C" hipError_t hsg_launch_msg(const hsg::LaunchArgs* a, int nblocks, int threads, int fp32,
+                                    hipStream_t stream);
 
 using namespace hsg;
 
@@ -163,13 +165,15 @@ struct hsg_engine {
         int32_t* dOrder = nullptr;
         int2* dTiles = nullptr;     // channel-kernel tiles (split mode)
         int nTiles = 0;
+        bool msg = false;           // message-passing solve kernel (split mode)
         int lds = 0;
         int threads = 256;
     };
     std::vector<Bin> bins;
     bool split = false;          // channel phase as its own kernel
-    hipStream_t binStream[3] = {nullptr, nullptr, nullptr};
-    hipEvent_t evFork = nullptr, evJoin[3] = {nullptr, nullptr, nullptr};
+    static constexpr int kMaxBins = 6;
+    hipStream_t binStream[kMaxBins] = {};
+    hipEvent_t evFork = nullptr, evJoin[kMaxBins] = {};
     // graph of the steady-state launches
     hipGraphExec_t graphExec = nullptr;
     int graphSteps = 0;
@@ -339,6 +343,9 @@ int place_cell(hsg_engine* e, HostCell& hc) {
     for (auto& g : s.groups) nMem += (int64_t)g.mem.size();
     int64_t workB = 8 * (2 * (int64_t)n + 2 * (int64_t)s.nTerm + (int64_t)s.nFill +
                          (s.anySlowGroup ? (int64_t)s.HJ0.size() : 0));
+    // the message-passing solve needs its slots (and the level table) in LDS
+    const int64_t msgLds = align_up(8 * kSlotDoubles * ((int64_t)s.nSlots + 1) + 4 * (int64_t)s.levelStart.size());
+    const bool msgOn = e->split && s.msgOk && msgLds <= e->ldsLimit;
     std::vector<Item> items = {
         {&d.oV, 8 * n}, {&d.oInitVm, 8 * n}, {&d.oCmByDt, 8 * n}, {&d.oEmByRm, 8 * n}, {&d.oDbase, 8 * n}, {&d.oOff, 8 * n},
         {&d.oDefer, 8 * n},
@@ -355,6 +362,14 @@ int place_cell(hsg_engine* e, HostCell& hc) {
         {&d.oSpkThr, 8 * (int64_t)s.spkCompt.size()}, {&d.oSpkRefr, 8 * (int64_t)s.spkCompt.size()},
         {&d.oSpkLast, 8 * (int64_t)s.spkCompt.size()}, {&d.oSpkFired, 4 * (int64_t)s.spkCompt.size()},
         {&d.oSteps, 8},
+        // message-passing solve (split mode)
+        {&d.oMsgDesc, msgOn ? 4 * (int64_t)s.mdesc.size() : 0},
+        {&d.oRowPos, msgOn ? 4 * (int64_t)n : 0},
+        // rows in unit order, the root row, two dummy rows
+        {&d.oDA, msgOn ? 8 * (2 * (int64_t)s.units.size() + 3) : 0},
+        {&d.oBA, msgOn ? 8 * (2 * (int64_t)s.units.size() + 3) : 0},
+        {&d.oFillU, msgOn ? 8 * ((int64_t)s.units.size() + 1) : 0},
+        {&d.oVMid, msgOn ? 8 * ((int64_t)n + 1) : 0},
     };
     int64_t total = 0;
     for (auto& it : items) total += align_up(std::max<int64_t>(it.bytes, 8));
@@ -447,11 +462,22 @@ int place_cell(hsg_engine* e, HostCell& hc) {
     st.put(d.oSpkEdge, i32(s.spkEdge));
     st.put(d.oSpkThr, s.spkThr);
     st.put(d.oSpkRefr, s.spkRefr);
+    if (msgOn) {
+        st.put(d.oMsgDesc, s.mdesc);
+        st.put(d.oRowPos, s.rowPos);
+    }
     if ((rc = st.flush(e))) return rc;
+    d.msg = msgOn ? 1 : 0;
+    d.nSlots = s.nSlots;
+    d.rootSlot = s.rootSlot;
+    d.rootCnt = s.rootCnt;
     d.slowHJ = s.anySlowGroup ? 1 : 0;
     // work arrays (and the level table) in LDS when they fit
     const int64_t wl = work_lds_bytes(d), lb = level_bytes(d);
-    if (wl + lb <= e->ldsLimit) {
+    if (msgOn) {
+        d.workInLds = 1;
+        d.ldsBytes = (int32_t)msgLds;
+    } else if (wl + lb <= e->ldsLimit) {
         d.workInLds = 1;
         d.ldsBytes = (int32_t)align_up(wl + lb);
     } else {
@@ -643,22 +669,30 @@ int make_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg_
     }
     bins.clear();
     if (cells.empty()) return 0;
+    // split mode: message-passing cells by workgroup-size class (largest
+    // cell >= 1024: 256 threads, >= 256: 128, else 64); other cells (and
+    // every cell in fused mode) by LDS need (> 32 KB, > 16 KB, the rest)
+    auto cls = [&](int id) {
+        const CellDev& d = e->cells[id]->dev;
+        if (d.msg) return d.n >= 1024 ? 0 : d.n >= 256 ? 1 : 2;
+        return 3 + (d.ldsBytes > 32768 ? 0 : d.ldsBytes > 16384 ? 1 : 2);
+    };
     std::vector<int32_t> sorted(cells);
     std::stable_sort(sorted.begin(), sorted.end(), [&](int a, int b) {
-        const CellDev& x = e->cells[a]->dev;
-        const CellDev& y = e->cells[b]->dev;
-        if (x.ldsBytes != y.ldsBytes) return x.ldsBytes > y.ldsBytes;
-        return x.n > y.n;
+        const int ca = cls(a), cb = cls(b);
+        if (ca != cb) return ca < cb;
+        return e->cells[a]->dev.n > e->cells[b]->dev.n;
     });
-    const int thresholds[2] = {32768, 16384};
     hsg_engine::Bin cur;
-    int tix = 0;
+    int curCls = -1;
     for (int id : sorted) {
-        const CellDev& d = e->cells[id]->dev;
-        while (tix < 2 && d.ldsBytes <= thresholds[tix]) {
-            if (!cur.cells.empty()) { bins.push_back(cur); cur = hsg_engine::Bin(); }
-            ++tix;
+        const int k = cls(id);
+        if (k != curCls && !cur.cells.empty()) {
+            bins.push_back(cur);
+            cur = hsg_engine::Bin();
         }
+        curCls = k;
+        cur.msg = k < 3;
         cur.cells.push_back(id);
     }
     if (!cur.cells.empty()) bins.push_back(cur);
@@ -762,7 +796,8 @@ int launch_round(hsg_engine* e, std::vector<hsg_engine::Bin>& bins, int chunk, i
                 LaunchArgs a = make_args(e, bins[b].dOrder, 1, first && k == 0);
                 a.ldsBytes = bins[b].lds;
                 HIPCHK(hsg_launch_channel(&a, bins[b].dTiles, bins[b].nTiles, e->fp32, st));
-                HIPCHK(hsg_launch_solve(&a, (int)bins[b].cells.size(), bins[b].threads, e->fp32, st));
+                if (bins[b].msg) HIPCHK(hsg_launch_msg(&a, (int)bins[b].cells.size(), bins[b].threads, e->fp32, st));
+                else HIPCHK(hsg_launch_solve(&a, (int)bins[b].cells.size(), bins[b].threads, e->fp32, st));
             }
         } else {
             LaunchArgs a = make_args(e, bins[b].dOrder, chunk, first);
@@ -919,7 +954,7 @@ int hsg_create(const hsg_options* opts, hsg_engine** out) {
         (void)hipGetLastError();
     }
     if (e->ldsLimit == 0) return fail(HSG_E_DEVICE, "cannot configure dynamic LDS for the step kernel");
-    for (int b = 0; b < 3; ++b) {
+    for (int b = 0; b < hsg_engine::kMaxBins; ++b) {
         HIPCHK(hipStreamCreateWithFlags(&e->binStream[b], hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&e->evJoin[b], hipEventDisableTiming));
     }
@@ -951,7 +986,7 @@ int hsg_destroy(hsg_engine* e) {
     hipFree(e->dSpikes);
     hipFree(e->dSpikeCount);
     for (auto& b : e->bins) if (b.dOrder) hipFree(b.dOrder);
-    for (int b = 0; b < 3; ++b) {
+    for (int b = 0; b < hsg_engine::kMaxBins; ++b) {
         if (e->binStream[b]) hipStreamDestroy(e->binStream[b]);
         if (e->evJoin[b]) hipEventDestroy(e->evJoin[b]);
     }

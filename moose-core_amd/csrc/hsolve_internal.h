@@ -102,6 +102,18 @@ struct CellDev {
     int32_t cellId;
     int64_t steps;          // steps done since reinit (device copy kept in oSteps)
     int64_t oSteps;         // int64 counter
+    // message-passing solve (split mode, msg == 1)
+    int32_t msg;
+    int32_t nSlots;
+    int32_t rootSlot;
+    int32_t rootCnt;
+    int64_t oMsgDesc;       // int32[kMsgDescInts * nUnits]
+    int64_t oRowPos;        // int32[n]: unit-order position of each row
+    int64_t oDA, oBA;       // f64[2 * nUnits + 3]: rows in unit order, the root
+                            // row, two dummy rows; the eliminated rows after
+                            // the forward pass
+    int64_t oFillU;         // f64[nUnits + 1]: A[1][2] after member 0 (three-member units)
+    int64_t oVMid;          // f64[n + 1]
 };
 
 // Elimination unit descriptor: 12 int32 (48 bytes), read as three int4 loads.
@@ -116,6 +128,22 @@ struct CellDev {
 //            HJCopy_ / HS[4x+1], computed at setup as the reference does)
 //   groups of more than kFastGroupMax members (or cells past 16-bit indices)
 //   take the slow path: i1 = group index, everything else from memory
+// Message-passing unit descriptor (split mode): 12 int32 (48 bytes).
+//   i0: s (bits 0-3) | cnt0 << 4 | cnt1 << 6: term pairs the feeders of
+//       members 0 and 1 deliver (0: no feeder, 1: a two-member feeder,
+//       2: a three-member feeder)
+//   i1: slot of member 0 (its feeder's terms in, its VMid out), -1: none
+//   i2: slot of member 1 (three-member units), -1: none
+//   i3: slot of the last member (this unit's terms out, its VMid in)
+//   i4: m0 | m1 << 16 (Hines indices of the eliminated members; m1 = m0 in
+//       two-member units)
+//   i5: 0
+//   i6..i11: a01, a02, a12
+// The rows D, B of a compartment live at its unit-order position: 2u + k for
+// member k of the unit u that eliminates it, 2 * nUnits for the root.
+constexpr int kMsgDescInts = 12;
+constexpr int kSlotDoubles = 4;    // two (tD, tB) pairs; VMid reuses the first
+
 constexpr int kFastGroupMax = 3;
 constexpr int kChannelTile = 256;  // compartments per workgroup of the channel kernel
 constexpr int kDescInts = 12;

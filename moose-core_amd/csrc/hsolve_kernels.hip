@@ -184,7 +184,7 @@ __device__ __forceinline__ View<GT> make_view(char* arena, const CellDev& c, con
 // ---------------------------------------------------------------------------
 template <typename GT, bool UNIFORM>
 __device__ __forceinline__ void channel_row(const View<GT>& v, const ChanType* __restrict__ types, int i,
-                                            GT dtg, Work w, bool reinitOnly) {
+                                            GT dtg, Work w, bool reinitOnly, int dst) {
     const int n = v.n;
     const double V = v.V[i];
     // pools of this compartment; the Ca row of a pool (caRowCompt_) is
@@ -250,8 +250,8 @@ __device__ __forceinline__ void channel_row(const View<GT>& v, const ChanType* _
     }
     if (reinitOnly) return;
     // updateMatrix row (HSolveActive.cpp:112-115)
-    w.D[i] = v.Dbase[i] + GkSum;
-    w.B[i] = V * v.CmByDt[i] + v.EmByRm[i] + GkEkSum;
+    w.D[dst] = v.Dbase[i] + GkSum;
+    w.B[dst] = V * v.CmByDt[i] + v.EmByRm[i] + GkEkSum;
 }
 
 // ---------------------------------------------------------------------------
@@ -521,6 +521,49 @@ __device__ __forceinline__ const ChanType* stage_types(const CellDev& c, const C
     return sTypes;
 }
 
+// Per-tick outputs once V of the tick is complete: spike generators
+// (SpikeGen::process at t = dt * step, biophysics/SpikeGen.cpp:202-216, fed
+// by HSolveActive::sendSpikes, hsolve/HSolveActive.cpp:342-347) into the
+// engine's spike ring, and Vm probes (Table sampling after the solver's tick).
+__device__ __forceinline__ void tick_outputs(const LaunchArgs& a, const CellDev& c, const double* __restrict__ V,
+                                             int64_t stepNo, int64_t sample) {
+    char* arena = a.arena;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int sg = tid; sg < c.nSpike; sg += nt) {
+        const int comp = A_<const int32_t>(arena, c.oSpkCompt)[sg];
+        const double Vm = V[comp];
+        const double thr = A_<const double>(arena, c.oSpkThr)[sg];
+        int32_t* fired = A_<int32_t>(arena, c.oSpkFired);
+        if (Vm > thr) {
+            const double t = a.dt * (double)(unsigned long long)stepNo;
+            double* last = A_<double>(arena, c.oSpkLast);
+            const double refr = A_<const double>(arena, c.oSpkRefr)[sg];
+            const int edge = A_<const int32_t>(arena, c.oSpkEdge)[sg];
+            if ((t + a.dt / 2.0) >= (last[sg] + refr)) {
+                if (!(edge && fired[sg])) {
+                    const unsigned long long slot = atomicAdd(a.spikeCount, 1ULL);
+                    if (slot < (unsigned long long)a.spikeCap) {
+                        SpikeRec r;
+                        r.cell = c.cellId;
+                        r.gen = sg;
+                        r.step = stepNo;
+                        a.spikes[slot] = r;
+                    }
+                    last[sg] = t;
+                    fired[sg] = 1;
+                }
+            }
+        } else {
+            fired[sg] = 0;
+        }
+    }
+    if (c.nProbe > 0 && sample >= 0 && sample < c.probeCap) {
+        const int32_t* pcmp = A_<const int32_t>(arena, c.oProbeCompt);
+        double* tr = A_<double>(arena, c.oProbeTrace);
+        for (int j = tid; j < c.nProbe; j += nt) tr[sample * c.nProbe + j] = V[pcmp[j]];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // the step kernel
 // ---------------------------------------------------------------------------
@@ -584,7 +627,7 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
         const bool first = a.firstOfCall && k == 0;
         // first unit of level 0 for this thread, loaded while phase A runs
         Desc cur{};
-        if (tid < nUnits) cur = load_desc(desc, tid);
+        if (nUnits > 0) cur = load_desc(desc, min(tid, nUnits - 1));
         // ---- A: channels, currents, matrix rows
         if constexpr (SPLIT) {
             if constexpr (WL)
@@ -594,9 +637,9 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
                 }
         } else {
             if (uniform)
-                for (int i = tid; i < n; i += nt) channel_row<GT, true>(v, types, i, dtg, w, false);
+                for (int i = tid; i < n; i += nt) channel_row<GT, true>(v, types, i, dtg, w, false, i);
             else
-                for (int i = tid; i < n; i += nt) channel_row<GT, false>(v, types, i, dtg, w, false);
+                for (int i = tid; i < n; i += nt) channel_row<GT, false>(v, types, i, dtg, w, false, i);
         }
         if (nInj > 0 || (first && nExt > 0)) {
             work_barrier(WL);
@@ -634,13 +677,13 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
             Desc nxt{};
             for (int l = 0; l < nLevels; l += 2) {
                 const int s2 = l + 2 <= nLevels ? level[l + 2] : s1;
-                if (s1 + tid < nUnits) nxt = load_desc(desc, s1 + tid);
+                nxt = load_desc(desc, min(s1 + tid, nUnits - 1));
                 if (s0 + tid < s1) forward_unit(arena, c, cur, w);
                 for (int u = s0 + tid + nt; u < s1; u += nt) forward_unit(arena, c, load_desc(desc, u), w);
                 work_barrier(WL);
                 if (l + 1 >= nLevels) break;
                 const int s3 = l + 3 <= nLevels ? level[l + 3] : s2;
-                if (s2 + tid < nUnits) cur = load_desc(desc, s2 + tid);
+                cur = load_desc(desc, min(s2 + tid, nUnits - 1));
                 if (s1 + tid < s2) forward_unit(arena, c, nxt, w);
                 for (int u = s1 + tid + nt; u < s2; u += nt) forward_unit(arena, c, load_desc(desc, u), w);
                 work_barrier(WL);
@@ -655,20 +698,20 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
             int s1 = nLevels > 0 ? level[nLevels] : 0;
             int s0 = nLevels > 0 ? level[nLevels - 1] : 0;
             int p0 = nLevels > 1 ? level[nLevels - 2] : 0;
-            if (s0 + tid < s1) cur = load_desc(desc, s0 + tid);
+            if (nLevels > 0) cur = load_desc(desc, min(s0 + tid, nUnits - 1));
             Desc nxt{};
             work_barrier(WL);
             for (int l = nLevels - 1; l >= 0; l -= 2) {
                 // level l from cur, level l-1 (units p0..s0) into nxt
                 const int pp = l >= 2 ? level[l - 2] : 0;
-                if (l > 0 && p0 + tid < s0) nxt = load_desc(desc, p0 + tid);
+                nxt = load_desc(desc, min(p0 + tid, nUnits - 1));
                 if (s0 + tid < s1) backward_unit(arena, c, cur, w);
                 for (int u = s0 + tid + nt; u < s1; u += nt) backward_unit(arena, c, load_desc(desc, u), w);
                 work_barrier(WL);
                 if (l == 0) break;
                 // level l-1 from nxt, level l-2 (units pp..p0) into cur
                 const int ppp = l >= 3 ? level[l - 3] : 0;
-                if (l > 1 && pp + tid < p0) cur = load_desc(desc, pp + tid);
+                cur = load_desc(desc, min(pp + tid, nUnits - 1));
                 if (p0 + tid < s0) backward_unit(arena, c, nxt, w);
                 for (int u = p0 + tid + nt; u < s0; u += nt) backward_unit(arena, c, load_desc(desc, u), w);
                 work_barrier(WL);
@@ -684,44 +727,7 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
         const int64_t stepNo = step0 + k + 1;
         if (nSpike > 0 || nProbe > 0) {
             __syncthreads();   // V written by other threads
-            // spike generators (SpikeGen::process at t = dt * step)
-            for (int sg = tid; sg < nSpike; sg += nt) {
-                const int comp = A_<const int32_t>(arena, c.oSpkCompt)[sg];
-                const double Vm = v.V[comp];
-                const double thr = A_<const double>(arena, c.oSpkThr)[sg];
-                int32_t* fired = A_<int32_t>(arena, c.oSpkFired);
-                if (Vm > thr) {
-                    const double t = a.dt * (double)(unsigned long long)stepNo;
-                    double* last = A_<double>(arena, c.oSpkLast);
-                    const double refr = A_<const double>(arena, c.oSpkRefr)[sg];
-                    const int edge = A_<const int32_t>(arena, c.oSpkEdge)[sg];
-                    if ((t + a.dt / 2.0) >= (last[sg] + refr)) {
-                        if (!(edge && fired[sg])) {
-                            const unsigned long long slot = atomicAdd(a.spikeCount, 1ULL);
-                            if (slot < (unsigned long long)a.spikeCap) {
-                                SpikeRec r;
-                                r.cell = c.cellId;
-                                r.gen = sg;
-                                r.step = stepNo;
-                                a.spikes[slot] = r;
-                            }
-                            last[sg] = t;
-                            fired[sg] = 1;
-                        }
-                    }
-                } else {
-                    fired[sg] = 0;
-                }
-            }
-            // Vm probes (Table sampling after the solver's tick)
-            if (nProbe > 0) {
-                const int64_t sample = step0 + k - probeBase;
-                if (sample >= 0 && sample < probeCap) {
-                    const int32_t* pcmp = A_<const int32_t>(arena, c.oProbeCompt);
-                    double* tr = A_<double>(arena, c.oProbeTrace);
-                    for (int j = tid; j < nProbe; j += nt) tr[sample * nProbe + j] = v.V[pcmp[j]];
-                }
-            }
+            tick_outputs(a, c, v.V, stepNo, step0 + k - probeBase);
         }
         // the next step's phase A overwrites D/B that other threads may still
         // read; V and the gate states are touched only by their owner thread
@@ -754,7 +760,11 @@ __global__ void __launch_bounds__(256) hsolve_solve_kernel(LaunchArgs a) {
 // tile of `blockDim.x` consecutive compartments of one cell per workgroup,
 // one compartment per thread, over the whole batch.  The rows go to the
 // cell's HBM work arrays (D at the start, B after it), where the solve
-// kernel of the same tick picks them up.
+// kernel of the same tick picks them up.  For cells on the message-passing
+// solve the rows go to their unit-order positions instead, and the thread
+// of a compartment also adds its injection and (first tick of a call)
+// external currents, in the order of HSolveActive::updateMatrix
+// (HSolveActive.cpp:117-148).
 template <typename GT>
 __global__ void __launch_bounds__(256) hsolve_channel_kernel(LaunchArgs a, const int2* __restrict__ tiles) {
     __shared__ ChanType sTypes[kMaxSlots];
@@ -763,15 +773,291 @@ __global__ void __launch_bounds__(256) hsolve_channel_kernel(LaunchArgs a, const
     const View<GT> v = make_view<GT>(a.arena, c, a.tsets);
     const ChanType* types = stage_types(c, a.types, sTypes);
     __syncthreads();
-    double* gwork = A_<double>(a.arena, c.oWork);
-    Work w{};
-    w.D = gwork;
-    w.B = gwork + v.n;
     const int i = tile.y + threadIdx.x;
-    if (i < v.n) {
-        if (v.uniform) channel_row<GT, true>(v, types, i, (GT)a.dt, w, false);
-        else channel_row<GT, false>(v, types, i, (GT)a.dt, w, false);
+    if (i >= v.n) return;
+    Work w{};
+    int dst = i;
+    if (c.msg) {
+        w.D = A_<double>(a.arena, c.oDA);
+        w.B = A_<double>(a.arena, c.oBA);
+        dst = A_<const int32_t>(a.arena, c.oRowPos)[i];
+    } else {
+        double* gwork = A_<double>(a.arena, c.oWork);
+        w.D = gwork;
+        w.B = gwork + v.n;
     }
+    if (v.uniform) channel_row<GT, true>(v, types, i, (GT)a.dt, w, false, dst);
+    else channel_row<GT, false>(v, types, i, (GT)a.dt, w, false, dst);
+    if (c.msg) {
+        const int32_t* ii = A_<const int32_t>(a.arena, c.oInjIdx);
+        for (int e = 0; e < c.nInj; ++e)
+            if (ii[e] == i) {
+                double* iv = A_<double>(a.arena, c.oInjVar);
+                const double var = a.firstOfCall ? iv[e] : 0.0;
+                w.B[dst] += var + A_<const double>(a.arena, c.oInjBasal)[e];
+                if (a.firstOfCall) iv[e] = 0.0;
+            }
+        if (a.firstOfCall) {
+            const int32_t* xi = A_<const int32_t>(a.arena, c.oExtIdx);
+            for (int e = 0; e < c.nExt; ++e)
+                if (xi[e] == i) {
+                    w.D[dst] += A_<const double>(a.arena, c.oExtG)[e];
+                    w.B[dst] += A_<const double>(a.arena, c.oExtGE)[e];
+                }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Message-passing solve (split mode).  The rows of a cell sit in HBM in unit
+// order; a unit reads its members' rows at positions 2u, 2u+1 (prefetched a
+// level ahead with its descriptor, no dependency on the descriptor), takes
+// its feeders' terms from the members' LDS slots, writes its own terms for
+// its last member into that member's slot, and leaves the eliminated rows
+// (and the fill A[1][2]) in place for the way back.  Backward, a unit reads
+// its last member's VMid from that slot, solves its members, writes their
+// VMid into their slots for their feeders and updates V.  The operations
+// and their order are those of the in-place elimination (fwd_small /
+// bwd_small); only the home of the values in flight changes, so LDS holds
+// just the live slots and many more cells fit on a CU.
+// ---------------------------------------------------------------------------
+struct MsgFwd {
+    Desc d;
+    double D0, B0, D1, B1;
+};
+
+__device__ __forceinline__ MsgFwd load_msg_fwd(const int4* __restrict__ desc, const double* DA, const double* BA,
+                                               int u) {
+    MsgFwd m;
+    m.d = load_desc(desc, u);
+    m.D0 = DA[2 * u];
+    m.B0 = BA[2 * u];
+    m.D1 = DA[2 * u + 1];
+    m.B1 = BA[2 * u + 1];
+    return m;
+}
+
+// Every lane of a level issues the same global loads and stores (lanes past
+// the level's units work on a clamped copy and write to dummy slots and
+// positions), and the level loops have no per-lane branches around memory
+// operations: the waits the compiler places for the prefetched buffers then
+// count the outstanding operations exactly and never drain the next level's
+// prefetch.  Dummy targets: LDS slot nSlots, rows 2*nUnits+1 (and +2), fill
+// nUnits, VMid n.
+__device__ __forceinline__ void msg_forward(const MsgFwd& m, int u, bool on, double* slot, int dummySlot,
+                                            int dummyRow, double* DA, double* BA, double* FU) {
+    const bool three = (m.d.a.x & 15) == 3;
+    const int cnt0 = (m.d.a.x >> 4) & 3, cnt1 = (m.d.a.x >> 6) & 3;
+    const double a01 = desc_double(m.d, 6), a02 = desc_double(m.d, 8), a12 = desc_double(m.d, 10);
+    double D0 = m.D0, B0 = m.B0;
+    if (cnt0) {
+        const double* t = slot + kSlotDoubles * m.d.a.y;
+        D0 -= t[0];
+        B0 -= t[1];
+        if (cnt0 == 2) {
+            D0 -= t[2];
+            B0 -= t[3];
+        }
+    }
+    const double l1 = a01 / D0;
+    const double tD1 = l1 * a01;
+    const double tB1 = l1 * B0;
+    // second member (three-member units; computed and discarded otherwise)
+    double D1 = m.D1 - tD1, B1 = m.B1 - tB1;
+    if (cnt1) {
+        const double* t = slot + kSlotDoubles * m.d.a.z;
+        D1 -= t[0];
+        B1 -= t[1];
+        if (cnt1 == 2) {
+            D1 -= t[2];
+            B1 -= t[3];
+        }
+    }
+    const double l2 = a02 / D0;
+    const double tD2 = l2 * a02;
+    const double tB2 = l2 * B0;
+    const double a12u = a12 - l1 * a02;
+    const double a12l = a12 - l2 * a01;
+    const double l = a12l / D1;
+    double* o = slot + kSlotDoubles * (on ? m.d.a.w : dummySlot);
+    o[0] = three ? tD2 : tD1;
+    o[1] = three ? tB2 : tB1;
+    o[2] = l * a12u;
+    o[3] = l * B1;
+    const int r = on ? 2 * u : dummyRow;
+    DA[r] = D0;
+    BA[r] = B0;
+    DA[r + 1] = D1;     // a two-member unit's second position is unused
+    BA[r + 1] = B1;
+    FU[r >> 1] = a12u;
+}
+
+struct MsgBwd {
+    Desc d;
+    double D0, B0, D1, B1, F;
+};
+
+__device__ __forceinline__ MsgBwd load_msg_bwd(const int4* __restrict__ desc, const double* DA, const double* BA,
+                                               const double* FU, int u) {
+    MsgBwd m;
+    m.d = load_desc(desc, u);
+    m.D0 = DA[2 * u];
+    m.B0 = BA[2 * u];
+    m.D1 = DA[2 * u + 1];
+    m.B1 = BA[2 * u + 1];
+    m.F = FU[u];
+    return m;
+}
+
+__device__ __forceinline__ void msg_backward(const MsgBwd& m, bool on, double* slot, int dummySlot, double* VMid,
+                                             int dummyRow) {
+    const bool three = (m.d.a.x & 15) == 3;
+    // m1 == m0 for two-member units: the store of v1 is then overwritten
+    const int m0 = on ? (m.d.b.x & 0xffff) : dummyRow;
+    const int m1 = on ? ((m.d.b.x >> 16) & 0xffff) : dummyRow;
+    const double a01 = desc_double(m.d, 6), a02 = desc_double(m.d, 8);
+    const double vl = slot[kSlotDoubles * m.d.a.w];
+    const double v1 = (m.B1 - vl * m.F) / m.D1;
+    const double acc = three ? (m.B0 - vl * a02) - v1 * a01 : m.B0 - a01 * vl;
+    const double v0 = acc / m.D0;
+    const int o1 = on && three && m.d.a.z >= 0 ? m.d.a.z : dummySlot;
+    const int o0 = on && m.d.a.y >= 0 ? m.d.a.y : dummySlot;
+    slot[kSlotDoubles * o1] = v1;
+    slot[kSlotDoubles * o0] = v0;
+    VMid[m1] = v1;
+    VMid[m0] = v0;
+}
+
+template <typename GT>
+__global__ void __launch_bounds__(256) hsolve_msg_kernel(LaunchArgs a) {
+    extern __shared__ double lds[];
+    __shared__ ChanType sTypes[kMaxSlots];
+    const CellDev& c = a.cells[a.order[blockIdx.x]];
+    char* arena = a.arena;
+    const int n = c.n;
+    const int tid = threadIdx.x;
+    const int nt = blockDim.x;
+    const int nLevels = c.nLevels;
+    const int nUnits = c.nUnits;
+    const int dummySlot = c.nSlots;       // LDS slot nSlots
+    const int dummyRow = 2 * nUnits + 1;  // rows 2*nUnits+1, +2 and fill nUnits
+    double* slot = lds;
+    int32_t* level = reinterpret_cast<int32_t*>(lds + kSlotDoubles * (c.nSlots + 1));
+    {
+        const int32_t* gLevel = A_<const int32_t>(arena, c.oLevelStart);
+        for (int l = tid; l <= nLevels; l += nt) level[l] = gLevel[l];
+    }
+    const int4* desc = A_<const int4>(arena, c.oMsgDesc);
+    double* DA = A_<double>(arena, c.oDA);
+    double* BA = A_<double>(arena, c.oBA);
+    double* FU = A_<double>(arena, c.oFillU);
+    double* V = A_<double>(arena, c.oV);
+    double* VMid = A_<double>(arena, c.oVMid);
+    int64_t* stepCounter = A_<int64_t>(arena, c.oSteps);
+    const int64_t step0 = *stepCounter;
+    const int last = nUnits - 1;
+    __syncthreads();
+
+    for (int k = 0; k < a.nsteps; ++k) {
+        if (nLevels > 0) {
+            // ---- forward, two alternating prefetch buffers
+            int s0 = level[0], s1 = level[1];
+            MsgFwd cur = load_msg_fwd(desc, DA, BA, min(tid, last));
+            MsgFwd nxt;
+            for (int l = 0;; l += 2) {
+                const int s2 = l + 2 <= nLevels ? level[l + 2] : s1;
+                nxt = load_msg_fwd(desc, DA, BA, min(s1 + tid, last));
+                msg_forward(cur, s0 + tid, s0 + tid < s1, slot, dummySlot, dummyRow, DA, BA, FU);
+                for (int b = s0 + nt; b < s1; b += nt) {
+                    const int u = b + tid;
+                    msg_forward(load_msg_fwd(desc, DA, BA, min(u, last)), u, u < s1, slot, dummySlot, dummyRow, DA, BA, FU);
+                }
+                lds_barrier();
+                if (l + 1 >= nLevels) break;
+                const int s3 = l + 3 <= nLevels ? level[l + 3] : s2;
+                cur = load_msg_fwd(desc, DA, BA, min(s2 + tid, last));
+                msg_forward(nxt, s1 + tid, s1 + tid < s2, slot, dummySlot, dummyRow, DA, BA, FU);
+                for (int b = s1 + nt; b < s2; b += nt) {
+                    const int u = b + tid;
+                    msg_forward(load_msg_fwd(desc, DA, BA, min(u, last)), u, u < s2, slot, dummySlot, dummyRow, DA, BA, FU);
+                }
+                lds_barrier();
+                if (l + 2 >= nLevels) break;
+                s0 = s2;
+                s1 = s3;
+            }
+        }
+        // ---- the root row: its feeder's terms, then VMid = B / D
+        if (tid == 0) {
+            const int r = 2 * nUnits;
+            double D = DA[r], B = BA[r];
+            if (c.rootCnt) {
+                const double* t = slot + kSlotDoubles * c.rootSlot;
+                D -= t[0];
+                B -= t[1];
+                if (c.rootCnt == 2) {
+                    D -= t[2];
+                    B -= t[3];
+                }
+            }
+            const double vr = B / D;
+            if (c.rootCnt) slot[kSlotDoubles * c.rootSlot] = vr;
+            VMid[n - 1] = vr;
+        }
+        // the eliminated rows were written to HBM by other threads
+        __syncthreads();
+        if (nLevels > 0) {
+            // ---- backward, reverse level order
+            int s1 = level[nLevels], s0 = level[nLevels - 1];
+            int p0 = nLevels > 1 ? level[nLevels - 2] : 0;
+            MsgBwd cur = load_msg_bwd(desc, DA, BA, FU, min(s0 + tid, last));
+            MsgBwd nxt;
+            for (int l = nLevels - 1;; l -= 2) {
+                // level l from cur, level l-1 (units p0..s0) into nxt
+                const int pp = l >= 2 ? level[l - 2] : 0;
+                nxt = load_msg_bwd(desc, DA, BA, FU, min(p0 + tid, last));
+                msg_backward(cur, s0 + tid < s1, slot, dummySlot, VMid, n);
+                for (int b = s0 + nt; b < s1; b += nt) {
+                    const int u = b + tid;
+                    msg_backward(load_msg_bwd(desc, DA, BA, FU, min(u, last)), u < s1, slot, dummySlot, VMid, n);
+                }
+                lds_barrier();
+                if (l == 0) break;
+                // level l-1 from nxt, level l-2 (units pp..p0) into cur
+                const int ppp = l >= 3 ? level[l - 3] : 0;
+                cur = load_msg_bwd(desc, DA, BA, FU, min(pp + tid, last));
+                msg_backward(nxt, p0 + tid < s0, slot, dummySlot, VMid, n);
+                for (int b = p0 + nt; b < s0; b += nt) {
+                    const int u = b + tid;
+                    msg_backward(load_msg_bwd(desc, DA, BA, FU, min(u, last)), u < s0, slot, dummySlot, VMid, n);
+                }
+                lds_barrier();
+                if (l == 1) break;
+                s1 = p0;
+                s0 = pp;
+                p0 = ppp;
+            }
+        }
+        // VMid was written by other threads
+        __syncthreads();
+        for (int i = tid; i < n; i += nt) V[i] = 2 * VMid[i] - V[i];
+        const bool first = a.firstOfCall && k == 0;
+        if (c.hasPools) {
+            const View<GT> v = make_view<GT>(arena, c, a.tsets);
+            const ChanType* types = stage_types(c, a.types, sTypes);
+            __syncthreads();
+            Work w{};
+            w.B = VMid;
+            for (int i = tid; i < n; i += nt) calcium_row<GT>(arena, c, v, types, i, w, first);
+        }
+        const int64_t stepNo = step0 + k + 1;
+        if (c.nSpike > 0 || c.nProbe > 0) {
+            __syncthreads();   // V written by other threads
+            tick_outputs(a, c, V, stepNo, step0 + k - c.probeBase);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) *stepCounter = step0 + a.nsteps;
 }
 
 // HSolveActive::reinitChannels (hsolve/HSolveActiveSetup.cpp:79-160): gates at
@@ -804,9 +1090,9 @@ __global__ void __launch_bounds__(256) hsolve_reinit_kernel(LaunchArgs a) {
     __syncthreads();
     Work w{};
     if (v.uniform)
-        for (int i = threadIdx.x; i < v.n; i += blockDim.x) channel_row<GT, true>(v, types, i, (GT)a.dt, w, true);
+        for (int i = threadIdx.x; i < v.n; i += blockDim.x) channel_row<GT, true>(v, types, i, (GT)a.dt, w, true, i);
     else
-        for (int i = threadIdx.x; i < v.n; i += blockDim.x) channel_row<GT, false>(v, types, i, (GT)a.dt, w, true);
+        for (int i = threadIdx.x; i < v.n; i += blockDim.x) channel_row<GT, false>(v, types, i, (GT)a.dt, w, true, i);
 }
 
 }  // namespace hsg
@@ -834,6 +1120,15 @@ extern "C" hipError_t hsg_launch_solve(const hsg::LaunchArgs* a, int nblocks, in
     return hipGetLastError();
 }
 
+extern "C" hipError_t hsg_launch_msg(const hsg::LaunchArgs* a, int nblocks, int threads, int fp32,
+                                    hipStream_t stream) {
+    if (fp32)
+        hipLaunchKernelGGL(hsg::hsolve_msg_kernel<float>, dim3(nblocks), dim3(threads), a->ldsBytes, stream, *a);
+    else
+        hipLaunchKernelGGL(hsg::hsolve_msg_kernel<double>, dim3(nblocks), dim3(threads), a->ldsBytes, stream, *a);
+    return hipGetLastError();
+}
+
 extern "C" hipError_t hsg_launch_channel(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, int fp32,
                                         hipStream_t stream) {
     if (fp32)
@@ -855,8 +1150,9 @@ extern "C" hipError_t hsg_launch_reinit(const hsg::LaunchArgs* a, int nblocks, i
 }
 
 extern "C" hipError_t hsg_set_max_lds(int bytes) {
-    const void* kernels[4] = {(const void*)hsg::hsolve_step_kernel<double>, (const void*)hsg::hsolve_step_kernel<float>,
-                              (const void*)hsg::hsolve_solve_kernel<double>, (const void*)hsg::hsolve_solve_kernel<float>};
+    const void* kernels[6] = {(const void*)hsg::hsolve_step_kernel<double>, (const void*)hsg::hsolve_step_kernel<float>,
+                              (const void*)hsg::hsolve_solve_kernel<double>, (const void*)hsg::hsolve_solve_kernel<float>,
+                              (const void*)hsg::hsolve_msg_kernel<double>, (const void*)hsg::hsolve_msg_kernel<float>};
     for (const void* k : kernels) {
         hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
         if (e != hipSuccess) return e;

@@ -325,6 +325,89 @@ static int schedule(CellSetup& s, const std::vector<int>& junctionGroup, std::st
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Message-passing schedule (used by the split-mode solve kernel).  Every node
+// is eliminated by exactly one unit (its "consumer": the unit in which it is
+// member 0 or 1) and receives its subtree's terms from at most one unit (its
+// "feeder": the unit whose last member it is).  Instead of updating the
+// node's row in place, the feeder writes its terms into the node's slot and
+// the consumer applies them -- in the same order as the in-place form, so the
+// arithmetic is unchanged: member 0's row gets its feeder's pairs, member 1's
+// row first the term of member 0 and then its feeder's pairs.  On the way
+// back the consumer writes the node's VMid into the same slot for the
+// feeder.  A slot is live from the feeder's level to the consumer's level;
+// slots are shared by nodes whose intervals do not overlap.
+// ---------------------------------------------------------------------------
+static void schedule_msg(CellSetup& s) {
+    s.msgOk = !s.anySlowGroup && s.n <= 65535;
+    s.mdesc.clear();
+    s.rowPos.assign(s.n, 0);
+    s.nSlots = 0;
+    s.rootSlot = -1;
+    s.rootCnt = 0;
+    if (!s.msgOk) return;
+    const int n = s.n;
+    const int nU = (int)s.units.size();
+    const int nL = (int)s.levelStart.size() - 1;
+    std::vector<int> level(nU), feeder(n, -1), consumer(n, -1), cslot(n, -1);
+    for (int l = 0; l < nL; ++l)
+        for (int u = s.levelStart[l]; u < s.levelStart[l + 1]; ++u) level[u] = l;
+    auto dsc = [&](int u) { return &s.udesc[(size_t)kDescInts * u]; };
+    auto size_of = [&](int u) { return dsc(u)[0] & 15; };
+    auto mem = [&](int u, int k) {
+        const int32_t* d = dsc(u);
+        if (k == 0) return d[3] & 0xffff;
+        if (k == 1) return (d[3] >> 16) & 0xffff;
+        return d[4];
+    };
+    auto last_of = [&](int u) { return size_of(u) == 3 ? mem(u, 2) : mem(u, 1); };
+    for (int u = 0; u < nU; ++u) {
+        feeder[last_of(u)] = u;
+        consumer[mem(u, 0)] = u;
+        s.rowPos[mem(u, 0)] = 2 * u;
+        if (size_of(u) == 3) {
+            consumer[mem(u, 1)] = u;
+            s.rowPos[mem(u, 1)] = 2 * u + 1;
+        }
+    }
+    const int root = n - 1;
+    s.rowPos[root] = 2 * nU;
+    // slot intervals [feeder level, consumer level]; the root's consumer is
+    // the step after the last level
+    std::vector<std::pair<int, int>> iv;   // (start, node)
+    for (int m = 0; m < n; ++m)
+        if (feeder[m] >= 0) iv.push_back({level[feeder[m]], m});
+    std::sort(iv.begin(), iv.end());
+    std::vector<int> slotEnd;              // last level each slot is busy
+    for (auto& [start, m] : iv) {
+        const int end = m == root ? nL : level[consumer[m]];
+        int pick = -1;
+        for (int k = 0; k < (int)slotEnd.size(); ++k)
+            if (slotEnd[k] < start) { pick = k; break; }
+        if (pick < 0) { pick = (int)slotEnd.size(); slotEnd.push_back(end); }
+        else slotEnd[pick] = end;
+        cslot[m] = pick;
+    }
+    s.nSlots = (int)slotEnd.size();
+    auto cnt_of = [&](int m) { return feeder[m] < 0 ? 0 : (size_of(feeder[m]) == 3 ? 2 : 1); };
+    s.mdesc.assign((size_t)kMsgDescInts * nU, 0);
+    for (int u = 0; u < nU; ++u) {
+        const int32_t* d = dsc(u);
+        int32_t* o = &s.mdesc[(size_t)kMsgDescInts * u];
+        const int sz = size_of(u);
+        const int m0 = mem(u, 0), m1 = sz == 3 ? mem(u, 1) : m0;
+        o[0] = sz | cnt_of(m0) << 4 | (sz == 3 ? cnt_of(m1) : 0) << 6;
+        o[1] = cslot[m0];
+        o[2] = sz == 3 ? cslot[m1] : -1;
+        o[3] = cslot[last_of(u)];
+        o[4] = (m0 & 0xffff) | (m1 << 16);
+        o[5] = 0;
+        for (int k = 6; k < 12; ++k) o[k] = d[k];
+    }
+    s.rootSlot = cslot[root];
+    s.rootCnt = cnt_of(root);
+}
+
 int setup_cell(const hsg_cell_desc* d, double dt, int caAdvance, CellSetup& s, std::string& err) {
     if (!d) { err = "null cell description"; return HSG_E_ARG; }
     if (!(dt > 0.0)) { err = "HSolve: 'dt' must be set (positive) before 'target'"; return HSG_E_ARG; }
@@ -431,6 +514,7 @@ int setup_cell(const hsg_cell_desc* d, double dt, int caAdvance, CellSetup& s, s
     }
     rc = schedule(s, junctionGroup, err);
     if (rc) return rc;
+    schedule_msg(s);
 
     // readHHChannels / readGates
     std::vector<std::vector<int>> chans(nd);

@@ -65,6 +65,15 @@ struct CellSetup {
     // hsolve_internal.h)
     std::vector<int32_t> udesc;
     bool anySlowGroup = false;
+    // message-passing form of the schedule (every unit on the fast path):
+    // per-unit descriptors (kMsgDescInts each, level order), the position of
+    // each compartment's matrix row in unit order, and the LDS message slots
+    bool msgOk = false;
+    std::vector<int32_t> mdesc;
+    std::vector<int32_t> rowPos;      // per Hines index
+    int nSlots = 0;
+    int rootSlot = -1;
+    int rootCnt = 0;
     std::vector<int> defer;           // 2n: (offset, count)
     int nTerm = 0;
     int nFill = 0;

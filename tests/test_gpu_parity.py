@@ -137,13 +137,14 @@ def test_large_cells_global_work(pkg, oracle_mod, mode):
         np.testing.assert_array_equal(gpu, ref)
 
 
-def test_fields_and_inputs(pkg, oracle_mod):
+@pytest.mark.parametrize("mode", ["fused", "split"])
+def test_fields_and_inputs(pkg, oracle_mod, mode):
     """Zombie field reads and writes, and the per-step inputs, between steps."""
-    print("case: fields and inputs", flush=True)
+    print(f"case: fields and inputs, {mode}", flush=True)
     rng = np.random.default_rng(17)
     spec = workloads.random_ca_cell(rng, 80, inject=0.4e-9)
     dt = 20e-6
-    eng = pkg.Engine("fp64")
+    eng = pkg.Engine("fp64", kernel_mode=mode)
     cid = eng.add_cell(spec, dt)
     eng.reinit(dt)
     ref = oracle_mod.OracleCell(spec, dt)
@@ -278,11 +279,12 @@ def star_cell(arms=9, arm_len=12, inject=1.5e-9):
     return spec
 
 
-def test_large_junction_groups(pkg, oracle_mod):
-    print("case: star cell (groups of 10 and 6), 400 steps, fp64", flush=True)
+@pytest.mark.parametrize("mode", ["fused", "split"])
+def test_large_junction_groups(pkg, oracle_mod, mode):
+    print(f"case: star cell (groups of 10 and 6), 400 steps, fp64, {mode}", flush=True)
     spec = star_cell()
     dt, n = 25e-6, 400
-    ids, (gpu,), _ = run_gpu(pkg, [spec], dt, n)
+    ids, (gpu,), _ = run_gpu(pkg, [spec], dt, n, kernel_mode=mode)
     _, ref, _ = run_oracle(oracle_mod, spec, dt, n)
     np.testing.assert_array_equal(gpu, ref)
     assert np.ptp(ref[:, 0]) > 0.05       # the soma fires
