@@ -374,7 +374,7 @@ __host__ __device__ constexpr int hb(int S, int p) { return p * (2 * (S - 1) - (
 
 // Groups larger than kFastGroupMax (or cells past 16-bit indices): member
 // lists, deferred slots and the working junction coefficients in memory.
-__device__ void fwd_group_slow(char* arena, const CellDev& c, const Desc& u, Work w) {
+__device__ __noinline__ void fwd_group_slow(char* arena, const CellDev& c, Desc u, Work w) {
     const int32_t* defer = A_<const int32_t>(arena, c.oDefer);
     const int g = u.a.y;
     const int4 gi = A_<const int4>(arena, c.oGrpInfo)[g];
@@ -421,7 +421,7 @@ __device__ void fwd_group_slow(char* arena, const CellDev& c, const Desc& u, Wor
     }
 }
 
-__device__ void bwd_group_slow(char* arena, const CellDev& c, const Desc& u, Work w) {
+__device__ __noinline__ void bwd_group_slow(char* arena, const CellDev& c, Desc u, Work w) {
     const int g = u.a.y;
     const int4 gi = A_<const int4>(arena, c.oGrpInfo)[g];
     const int32_t* mem = A_<const int32_t>(arena, c.oGrpMem) + gi.x;
@@ -665,7 +665,9 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
                 }
             }
         }
-        work_barrier(WL);
+        // full barrier: it also lands every outstanding global load, so the
+        // level loops start from an empty queue
+        __syncthreads();
         // ---- F: forward elimination, level by level.  Two descriptor
         // buffers alternate: while one level is processed from one, the next
         // level's descriptors load into the other, so the loads stay in
@@ -676,14 +678,14 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
             int s1 = nLevels > 0 ? level[1] : 0;
             Desc nxt{};
             for (int l = 0; l < nLevels; l += 2) {
-                const int s2 = l + 2 <= nLevels ? level[l + 2] : s1;
                 nxt = load_desc(desc, min(s1 + tid, nUnits - 1));
+                const int s2 = l + 2 <= nLevels ? level[l + 2] : s1;
                 if (s0 + tid < s1) forward_unit(arena, c, cur, w);
                 for (int u = s0 + tid + nt; u < s1; u += nt) forward_unit(arena, c, load_desc(desc, u), w);
                 work_barrier(WL);
                 if (l + 1 >= nLevels) break;
-                const int s3 = l + 3 <= nLevels ? level[l + 3] : s2;
                 cur = load_desc(desc, min(s2 + tid, nUnits - 1));
+                const int s3 = l + 3 <= nLevels ? level[l + 3] : s2;
                 if (s1 + tid < s2) forward_unit(arena, c, nxt, w);
                 for (int u = s1 + tid + nt; u < s2; u += nt) forward_unit(arena, c, load_desc(desc, u), w);
                 work_barrier(WL);
@@ -698,20 +700,20 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
             int s1 = nLevels > 0 ? level[nLevels] : 0;
             int s0 = nLevels > 0 ? level[nLevels - 1] : 0;
             int p0 = nLevels > 1 ? level[nLevels - 2] : 0;
+            __syncthreads();
             if (nLevels > 0) cur = load_desc(desc, min(s0 + tid, nUnits - 1));
             Desc nxt{};
-            work_barrier(WL);
             for (int l = nLevels - 1; l >= 0; l -= 2) {
                 // level l from cur, level l-1 (units p0..s0) into nxt
-                const int pp = l >= 2 ? level[l - 2] : 0;
                 nxt = load_desc(desc, min(p0 + tid, nUnits - 1));
+                const int pp = l >= 2 ? level[l - 2] : 0;
                 if (s0 + tid < s1) backward_unit(arena, c, cur, w);
                 for (int u = s0 + tid + nt; u < s1; u += nt) backward_unit(arena, c, load_desc(desc, u), w);
                 work_barrier(WL);
                 if (l == 0) break;
                 // level l-1 from nxt, level l-2 (units pp..p0) into cur
-                const int ppp = l >= 3 ? level[l - 3] : 0;
                 cur = load_desc(desc, min(pp + tid, nUnits - 1));
+                const int ppp = l >= 3 ? level[l - 3] : 0;
                 if (p0 + tid < s0) backward_unit(arena, c, nxt, w);
                 for (int u = p0 + tid + nt; u < s0; u += nt) backward_unit(arena, c, load_desc(desc, u), w);
                 work_barrier(WL);
@@ -965,8 +967,8 @@ __global__ void __launch_bounds__(256) hsolve_msg_kernel(LaunchArgs a) {
             MsgFwd cur = load_msg_fwd(desc, DA, BA, min(tid, last));
             MsgFwd nxt;
             for (int l = 0;; l += 2) {
-                const int s2 = l + 2 <= nLevels ? level[l + 2] : s1;
                 nxt = load_msg_fwd(desc, DA, BA, min(s1 + tid, last));
+                const int s2 = l + 2 <= nLevels ? level[l + 2] : s1;
                 msg_forward(cur, s0 + tid, s0 + tid < s1, slot, dummySlot, dummyRow, DA, BA, FU);
                 for (int b = s0 + nt; b < s1; b += nt) {
                     const int u = b + tid;
@@ -974,8 +976,8 @@ __global__ void __launch_bounds__(256) hsolve_msg_kernel(LaunchArgs a) {
                 }
                 lds_barrier();
                 if (l + 1 >= nLevels) break;
-                const int s3 = l + 3 <= nLevels ? level[l + 3] : s2;
                 cur = load_msg_fwd(desc, DA, BA, min(s2 + tid, last));
+                const int s3 = l + 3 <= nLevels ? level[l + 3] : s2;
                 msg_forward(nxt, s1 + tid, s1 + tid < s2, slot, dummySlot, dummyRow, DA, BA, FU);
                 for (int b = s1 + nt; b < s2; b += nt) {
                     const int u = b + tid;
@@ -1014,8 +1016,8 @@ __global__ void __launch_bounds__(256) hsolve_msg_kernel(LaunchArgs a) {
             MsgBwd nxt;
             for (int l = nLevels - 1;; l -= 2) {
                 // level l from cur, level l-1 (units p0..s0) into nxt
-                const int pp = l >= 2 ? level[l - 2] : 0;
                 nxt = load_msg_bwd(desc, DA, BA, FU, min(p0 + tid, last));
+                const int pp = l >= 2 ? level[l - 2] : 0;
                 msg_backward(cur, s0 + tid < s1, slot, dummySlot, VMid, n);
                 for (int b = s0 + nt; b < s1; b += nt) {
                     const int u = b + tid;
@@ -1024,8 +1026,8 @@ __global__ void __launch_bounds__(256) hsolve_msg_kernel(LaunchArgs a) {
                 lds_barrier();
                 if (l == 0) break;
                 // level l-1 from nxt, level l-2 (units pp..p0) into cur
-                const int ppp = l >= 3 ? level[l - 3] : 0;
                 cur = load_msg_bwd(desc, DA, BA, FU, min(pp + tid, last));
+                const int ppp = l >= 3 ? level[l - 3] : 0;
                 msg_backward(nxt, p0 + tid < s0, slot, dummySlot, VMid, n);
                 for (int b = p0 + nt; b < s0; b += nt) {
                     const int u = b + tid;
