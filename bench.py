@@ -39,6 +39,11 @@ HBM_PEAK_GBS = 8000.0
 # tables excluded): V r+w 16, gate states r+w 2*w*3, Gbar/Ek 2*w*2,
 # CmByDt/EmByRm/diag/offdiag 32 (FP64), w = 8 (FP64) or 4 (mixed)
 BYTES_PER_CS = {"fp64": 16 + 2 * 8 * 3 + 2 * 8 * 2 + 32, "mixed": 16 + 2 * 4 * 3 + 2 * 4 * 2 + 32}
+# the large Ca cells: six gates and four HHChannels, a CaConc (Ca r+w, its
+# activation r+w, basal 8: 40) and an HHChannel2D (its 136-byte record, Z r+w,
+# Gk w, the external current w then r: 192) per compartment
+BYTES_PER_CS_LARGE = {"fp64": 16 + 2 * 8 * 6 + 2 * 8 * 4 + 32 + 40 + 192,
+                      "mixed": 16 + 2 * 4 * 6 + 2 * 4 * 4 + 32 + 40 + 192}
 
 
 def log(msg):
@@ -50,6 +55,29 @@ def build_cells(ncells, seed):
     cells = workloads.hh_ensemble(ncells, 50, 2000, seed)
     log(f"[bench] built {ncells} cells ({sum(c.n for c in cells)} compartments) in {time.time() - t:.1f}s")
     return cells
+
+
+def build_large_ca(ncells, seed, nmin=5000, nmax=30000):
+    """Large uneven cells with calcium: branching 2-4, Na/Kdr/KA/CaL, K-AHP as
+    an HHChannel2D on a 3000 x 500 Vm x [Ca] table, one CaConc per
+    compartment, n ~ U{5000..30000}."""
+    t = time.time()
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(nmin, nmax + 1, size=ncells)
+    cells = [workloads.random_ca_cell(rng, int(n), kahp2d=True) for n in sizes]
+    log(f"[bench] built {ncells} large Ca cells ({sum(c.n for c in cells)} compartments) in {time.time() - t:.1f}s")
+    return cells
+
+
+# workload name -> (description, default cells, dt, default steps, steps per
+# launch, graph chunk)
+WORKLOADS = {
+    "ensemble": ("batched ensemble: 4096 random binary-tree HH cells, n~U{50..2000}, dt=25us, 5000 steps",
+                 4096, 25e-6, 5000, 50, 0),
+    "large_ca": ("large uneven Ca cells: 256 random trees (branching 2-4), n~U{5000..30000}, Na/Kdr/KA/CaL + "
+                 "2-D K-AHP (Vm x [Ca], 3000 x 500), one CaConc per compartment, dt=20us, 10000 steps, "
+                 "graphs of 100 steps", 256, 20e-6, 10000, 100, 1),
+}
 
 
 def run_form(precision, packed, dt, steps, warmup, spl, graph, threads, mode="fused"):
@@ -121,7 +149,7 @@ def cpu_baseline(cells, dt, budget_s=15.0):
     for o in objs:
         o.run(dt, nsteps)
     single = sum(c.n for c in sample) * nsteps / (time.perf_counter() - t)
-    ncores = os.cpu_count() or 1
+    ncores = max(1, min(16, os.cpu_count() or 1))   # the host share of one GPU
     chunks = [objs[i::ncores] for i in range(ncores)]
     ths = [threading.Thread(target=lambda ch: [o.run(dt, nsteps) for o in ch], args=(ch,)) for ch in chunks]
     t = time.perf_counter()
@@ -153,12 +181,15 @@ def copy_probe(mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--workload", default="ensemble", choices=sorted(WORKLOADS),
+                    help="ensemble: the headline batched-ensemble line; large_ca: the large Ca cells "
+                         "with 2-D K-AHP channels")
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--cells", type=int, default=4096)
+    ap.add_argument("--cells", type=int, default=None)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--steps-per-launch", type=int, default=50)
-    ap.add_argument("--graph-chunk", type=int, default=0)
+    ap.add_argument("--steps-per-launch", type=int, default=None)
+    ap.add_argument("--graph-chunk", type=int, default=None)
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--kernel-mode", default="fused", choices=["fused", "split", "auto"])
     ap.add_argument("--no-cpu", action="store_true")
@@ -174,8 +205,15 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(cells are independent and would shard by bin-packing across GPUs)"}))
         sys.exit(2)
-    dt = 25e-6
-    cells = build_cells(args.cells, args.seed)
+    desc, dcells, dt, dsteps, dspl, dgraph = WORKLOADS[args.workload]
+    args.cells = dcells if args.cells is None else args.cells
+    args.steps = dsteps if args.steps is None else args.steps
+    args.steps_per_launch = dspl if args.steps_per_launch is None else args.steps_per_launch
+    args.graph_chunk = dgraph if args.graph_chunk is None else args.graph_chunk
+    if args.workload == "ensemble":
+        cells = build_cells(args.cells, args.seed)
+    else:
+        cells = build_large_ca(args.cells, args.seed)
     packed = [c.pack() for c in cells]
     ncomp = sum(c.n for c in cells)
     res = {}
@@ -186,10 +224,12 @@ def main():
     wall, launches, total_ms, mean_ms, soma, spikes = res[head]
     value = ncomp * args.steps / wall
     # dominant kernel: the fused step kernel; algorithmic bytes per launch
+    bpcs = BYTES_PER_CS if args.workload == "ensemble" else BYTES_PER_CS_LARGE
     steps_per_launch = args.steps / max(1, launches)
-    bytes_per_launch = BYTES_PER_CS[head] * ncomp * steps_per_launch
+    bytes_per_launch = bpcs[head] * ncomp * steps_per_launch
     achieved = bytes_per_launch / (mean_ms * 1e-3) / 1e9
-    traffic = measured_traffic(f"hsolve_step_kernel/{head}/hh_ensemble_{args.cells}", ncomp * steps_per_launch)
+    wl_key = f"hh_ensemble_{args.cells}" if args.workload == "ensemble" else f"large_ca_{args.cells}"
+    traffic = measured_traffic(f"hsolve_step_kernel/{head}/{wl_key}", ncomp * steps_per_launch)
     out = {
         "metric": "compartment-steps/s (batched cells, FP32-channel mixed form)",
         "value": value,
@@ -203,14 +243,12 @@ def main():
         "vs_baseline": None,
         "dtype": "f32-channels/f64-hines" if head == "mixed" else "f64",
         "data": "synthetic (seeded random morphologies)",
-        "config": {"workload": "batched ensemble: 4096 random binary-tree HH cells, n~U{50..2000}, "
-                               "dt=25us, 5000 steps",
-                   "cells": args.cells, "compartments": ncomp, "dt": dt,
+        "config": {"workload": desc, "cells": args.cells, "compartments": ncomp, "dt": dt,
                    "steps_per_launch": args.steps_per_launch, "graph_chunk": args.graph_chunk,
                    "parallelism": "1 GPU, one workgroup per cell"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "hsolve_step_kernel", "bytes_per_compartment_step": BYTES_PER_CS[head],
+                     "kernel": "hsolve_step_kernel", "bytes_per_compartment_step": bpcs[head],
                      "mean_launch_ms": mean_ms},
     }
     if "fp64" in res and head != "fp64":
@@ -240,7 +278,7 @@ def main():
             "cells": len(cells),
             "spike_step_max_err": worst, "cells_with_spike_count_mismatch": mismatched,
             "spikes": int(sum(len(a) for a in s_64)), "probe": "soma of every cell, every step"}
-        out["fp64_roofline_frac"] = (BYTES_PER_CS["fp64"] * ncomp * (args.steps / max(1, l64))
+        out["fp64_roofline_frac"] = (bpcs["fp64"] * ncomp * (args.steps / max(1, l64))
                                      / (m64 * 1e-3) / 1e9) / HBM_PEAK_GBS
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle) ...")

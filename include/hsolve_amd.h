@@ -109,6 +109,35 @@ typedef struct hsg_leak_desc {
     double  Gk, Ek;
 } hsg_leak_desc;
 
+/* One 2-D interpolation table (builtins/Interpol2D.h): (xdivs+1) x (ydivs+1)
+ * values, x-major (table[ix * (ydivs+1) + iy]), over [xmin, xmax] x
+ * [ymin, ymax]. */
+typedef struct hsg_table2d_desc {
+    uint32_t      xdivs, ydivs;
+    double        xmin, xmax, ymin, ymax;
+    const double* table;
+} hsg_table2d_desc;
+
+/* One HHChannel2D on a compartment's "channel" message
+ * (biophysics/HHChannel2D.cpp).  HSolve does not take it over: it advances
+ * on its own tick before the solver's, by exponential Euler
+ * (biophysics/HHChannelBase.cpp:268-274) from the Vm and [Ca] the solver sent
+ * at the end of the previous tick, and feeds its compartment through addGkEk
+ * (hsolve/HSolveInterface.cpp:246-252).  The engine runs it on the device in
+ * the same pass.  Per gate X, Y, Z: tables A and B (indices into
+ * hsg_cell_desc.tables2d, -1: no gate) and the quantity on each table axis
+ * (dep: 0 Vm, 1 conc1, 2 conc2, -1 none; biophysics/HHChannel2D.cpp:257-305).
+ * conc1/conc2: the pools whose concOut feeds "concen"/"concen2" (-1: none). */
+typedef struct hsg_chan2d_desc {
+    int32_t compartment;
+    int32_t instant;
+    int32_t tableA[3];
+    int32_t tableB[3];
+    int32_t dep[6];          /* dep[2*g], dep[2*g+1]: axes of gate g */
+    int32_t conc1, conc2;
+    double  Gbar, Ek, Xpower, Ypower, Zpower, X, Y, Z, modulation;
+} hsg_chan2d_desc;
+
 /* One cell.  axial[2*k], axial[2*k+1] = (parent, child) of the k-th
  * "axial"->"raxial" message, in creation order.  seed: the compartment
  * HSolve.target resolves to (hsolve/HSolve.cpp:315-334). */
@@ -128,6 +157,10 @@ typedef struct hsg_cell_desc {
     const hsg_spikegen_desc*    spikegens;
     int32_t                     n_leaks;
     const hsg_leak_desc*        leaks;
+    int32_t                     n_tables2d;
+    const hsg_table2d_desc*     tables2d;
+    int32_t                     n_chans2d;
+    const hsg_chan2d_desc*      chans2d;
 } hsg_cell_desc;
 
 /* ---- engine ------------------------------------------------------------ */
@@ -211,6 +244,15 @@ int hsg_step_cells(hsg_engine* e, const int32_t* cells, const int64_t* nsteps,
 #define HSG_F_CABASAL  41   /* :482-495 */
 #define HSG_F_CACEIL   42   /* :505-518 */
 #define HSG_F_CAFLOOR  43   /* :520-533 */
+/* HHChannel2D fields (index = chans2d entry; the object's own fields,
+ * biophysics/ChanBase.cpp:90-117, HHChannel2D.cpp) */
+#define HSG_F_2D_GBAR  60
+#define HSG_F_2D_EK    61
+#define HSG_F_2D_GK    62   /* read-only: Gk of the last tick */
+#define HSG_F_2D_X     64
+#define HSG_F_2D_Y     65
+#define HSG_F_2D_Z     66
+#define HSG_F_2D_MODULATION 67
 
 /* Reads n values of `field` for objects idx[0..n-1] of `cell` into out. */
 int hsg_get_field(hsg_engine* e, int32_t cell, int32_t field,

@@ -31,16 +31,24 @@ FIELDS = {
     "Gbar": 20, "Ek": 21, "Gk": 22, "Ik": 23, "X": 24, "Y": 25, "Z": 26,
     "modulation": 27, "instant": 28,
     "Ca": 40, "CaBasal": 41, "ceiling": 42, "floor": 43,
+    "Gbar2D": 60, "Ek2D": 61, "Gk2D": 62, "X2D": 64, "Y2D": 65, "Z2D": 66, "modulation2D": 67,
 }
 COMPARTMENT_FIELDS = {k for k, v in FIELDS.items() if v < 20}
 CHANNEL_FIELDS = {k for k, v in FIELDS.items() if 20 <= v < 40}
-POOL_FIELDS = {k for k, v in FIELDS.items() if v >= 40}
+POOL_FIELDS = {k for k, v in FIELDS.items() if 40 <= v < 60}
+CHAN2D_FIELDS = {k for k, v in FIELDS.items() if v >= 60}
 
 
 class GateDesc(C.Structure):
     _fields_ = [("id", C.c_int64), ("min", C.c_double), ("max", C.c_double),
                 ("divs", C.c_uint32), ("tableA", C.POINTER(C.c_double)),
                 ("tableB", C.POINTER(C.c_double))]
+
+
+class Table2DDesc(C.Structure):
+    _fields_ = [("xdivs", C.c_uint32), ("ydivs", C.c_uint32), ("xmin", C.c_double),
+                ("xmax", C.c_double), ("ymin", C.c_double), ("ymax", C.c_double),
+                ("table", C.POINTER(C.c_double))]
 
 
 class CellDesc(C.Structure):
@@ -51,7 +59,9 @@ class CellDesc(C.Structure):
                 ("n_gates", C.c_int32), ("gates", C.POINTER(GateDesc)),
                 ("n_pools", C.c_int32), ("pools", C.c_void_p),
                 ("n_spikegens", C.c_int32), ("spikegens", C.c_void_p),
-                ("n_leaks", C.c_int32), ("leaks", C.c_void_p)]
+                ("n_leaks", C.c_int32), ("leaks", C.c_void_p),
+                ("n_tables2d", C.c_int32), ("tables2d", C.POINTER(Table2DDesc)),
+                ("n_chans2d", C.c_int32), ("chans2d", C.c_void_p)]
 
 
 class Options(C.Structure):
@@ -83,8 +93,13 @@ CACONC_DT = np.dtype([(k, "f8") for k in ("Ca", "CaBasal", "tau", "B", "ceiling"
 SPIKEGEN_DT = np.dtype([("compartment", "i4"), ("edgeTriggered", "i4"),
                         ("threshold", "f8"), ("refractT", "f8")], align=True)
 LEAK_DT = np.dtype([("compartment", "i4"), ("Gk", "f8"), ("Ek", "f8")], align=True)
+CHAN2D_DT = np.dtype([("compartment", "i4"), ("instant", "i4"), ("tableA", "i4", (3,)),
+                      ("tableB", "i4", (3,)), ("dep", "i4", (6,)), ("conc1", "i4"), ("conc2", "i4")]
+                     + [(k, "f8") for k in ("Gbar", "Ek", "Xpower", "Ypower", "Zpower",
+                                            "X", "Y", "Z", "modulation")], align=True)
 
 assert CHANNEL_DT.itemsize == 104 and SPIKEGEN_DT.itemsize == 24 and LEAK_DT.itemsize == 24
+assert CHAN2D_DT.itemsize == 136
 
 
 def _ptr(a):
@@ -102,7 +117,18 @@ class PackedCell:
         self.pools = np.ascontiguousarray(spec.pools, dtype=CACONC_DT)
         self.spikegens = np.ascontiguousarray(spec.spikegens, dtype=SPIKEGEN_DT)
         self.leaks = np.ascontiguousarray(spec.leaks, dtype=LEAK_DT)
-        self._tables = []
+        self.chans2d = np.ascontiguousarray(spec.chans2d, dtype=CHAN2D_DT)
+        self._tables = []           # every table array the descriptors point into
+        t2 = (Table2DDesc * max(1, len(spec.tables2d)))()
+        for i, t in enumerate(spec.tables2d):
+            arr = np.ascontiguousarray(t.table, dtype=np.float64).reshape(-1)
+            if arr.size != (t.xdivs + 1) * (t.ydivs + 1):
+                raise ValueError(f"2-D table {i}: needs (xdivs+1)*(ydivs+1) values")
+            self._tables.append(arr)
+            t2[i].xdivs, t2[i].ydivs = t.xdivs, t.ydivs
+            t2[i].xmin, t2[i].xmax, t2[i].ymin, t2[i].ymax = t.xmin, t.xmax, t.ymin, t.ymax
+            t2[i].table = arr.ctypes.data_as(C.POINTER(C.c_double))
+        self._t2 = t2
         gates = (GateDesc * max(1, len(spec.gates)))()
         for i, g in enumerate(spec.gates):
             a = np.ascontiguousarray(g.tableA, dtype=np.float64)
@@ -133,6 +159,10 @@ class PackedCell:
         d.spikegens = _ptr(self.spikegens)
         d.n_leaks = len(self.leaks)
         d.leaks = _ptr(self.leaks)
+        d.n_tables2d = len(spec.tables2d)
+        d.tables2d = self._t2
+        d.n_chans2d = len(self.chans2d)
+        d.chans2d = _ptr(self.chans2d)
         self.desc = d
 
     @property

@@ -11,7 +11,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._abi import CACONC_DT, CHANNEL_DT, COMPARTMENT_DT, LEAK_DT, SPIKEGEN_DT, PackedCell
+from ._abi import CACONC_DT, CHAN2D_DT, CHANNEL_DT, COMPARTMENT_DT, LEAK_DT, SPIKEGEN_DT, PackedCell
 
 
 @dataclass
@@ -26,6 +26,18 @@ class Gate:
 
 
 @dataclass
+class Table2D:
+    """An Interpol2D: (xdivs+1) x (ydivs+1) values over [xmin,xmax] x [ymin,ymax]."""
+    xdivs: int
+    ydivs: int
+    xmin: float
+    xmax: float
+    ymin: float
+    ymax: float
+    table: np.ndarray
+
+
+@dataclass
 class CellSpec:
     compartments: np.ndarray                       # COMPARTMENT_DT
     axial: np.ndarray                              # int32 (m, 2): (parent, child)
@@ -35,6 +47,8 @@ class CellSpec:
     pools: np.ndarray = field(default_factory=lambda: np.zeros(0, CACONC_DT))
     spikegens: np.ndarray = field(default_factory=lambda: np.zeros(0, SPIKEGEN_DT))
     leaks: np.ndarray = field(default_factory=lambda: np.zeros(0, LEAK_DT))
+    tables2d: list = field(default_factory=list)
+    chans2d: np.ndarray = field(default_factory=lambda: np.zeros(0, CHAN2D_DT))
 
     @property
     def n(self):

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstddef>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -29,6 +30,7 @@ extern "C" hipError_t hsg_launch_solve(const hsg::LaunchArgs* a, int nblocks, in
                                       hipStream_t stream);
 extern "C" hipError_t hsg_launch_channel(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, int fp32,
                                         hipStream_t stream);
+extern "C" hipError_t hsg_launch_chan2d(const hsg::LaunchArgs* a, int nblocks, int reinit, hipStream_t stream);
 extern "C" hipError_t hsg_launch_msg(const hsg::LaunchArgs* a, int nblocks, int threads, int fp32,
                                     hipStream_t stream);
 
@@ -138,6 +140,10 @@ struct hsg_engine {
     std::map<std::string, int> typeKey;
     std::vector<TableSetDev> tsets;
     std::map<std::string, int> tsetKey;
+    std::vector<Table2DDev> tabs2d;              // Interpol2D tables, deduplicated
+    std::map<uint64_t, int> tab2dKey;
+    Table2DDev* dTabs2d = nullptr;
+    int dTabs2dCap = 0;
     // device-side tables of the launch
     CellDev* dCells = nullptr;
     int dCellsCap = 0;
@@ -166,12 +172,13 @@ struct hsg_engine {
         int2* dTiles = nullptr;     // channel-kernel tiles (split mode)
         int nTiles = 0;
         bool msg = false;           // message-passing solve kernel (split mode)
+        bool has2d = false;         // cells with HHChannel2D objects: launched tick by tick
         int lds = 0;
         int threads = 256;
     };
     std::vector<Bin> bins;
     bool split = false;          // channel phase as its own kernel
-    static constexpr int kMaxBins = 6;
+    static constexpr int kMaxBins = 8;
     hipStream_t binStream[kMaxBins] = {};
     hipEvent_t evFork = nullptr, evJoin[kMaxBins] = {};
     // graph of the steady-state launches
@@ -297,6 +304,24 @@ int table_set_id(hsg_engine* e, const HostTableSet& t, int* id) {
     return 0;
 }
 
+// Engine-wide id of a 2-D table (uploaded once per content).
+int table2d_id(hsg_engine* e, const HostTable2D& t, int* id) {
+    auto it = e->tab2dKey.find(t.hash);
+    if (it != e->tab2dKey.end()) { *id = it->second; return 0; }
+    Table2DDev d{};
+    d.xmin = t.xmin; d.xmax = t.xmax; d.ymin = t.ymin; d.ymax = t.ymax;
+    d.invDx = t.invDx; d.invDy = t.invDy;
+    d.nx = (int32_t)t.nx; d.ny = (int32_t)t.ny;
+    int rc = arena_alloc(e, std::max<int64_t>((int64_t)t.table.size(), 1) * 8, &d.off);
+    if (rc) return rc;
+    if ((rc = upload(e, d.off, t.table))) return rc;
+    *id = (int)e->tabs2d.size();
+    e->tabs2d.push_back(d);
+    e->tab2dKey[t.hash] = *id;
+    e->tablesDirty = true;
+    return 0;
+}
+
 int type_id(hsg_engine* e, const HostChanType& t, int tableSet) {
     char kb[320];
     snprintf(kb, sizeof kb, "%a|%a|%a|%d|%d|%d|%d|%d|%d|%d", t.Xpower, t.Ypower, t.Zpower, t.instant,
@@ -370,6 +395,12 @@ int place_cell(hsg_engine* e, HostCell& hc) {
         {&d.oBA, msgOn ? 8 * (2 * (int64_t)s.units.size() + 3) : 0},
         {&d.oFillU, msgOn ? 8 * ((int64_t)s.units.size() + 1) : 0},
         {&d.oVMid, msgOn ? 8 * ((int64_t)n + 1) : 0},
+        // HHChannel2D objects
+        {&d.o2dStart, s.ch2d.empty() ? 0 : 4 * ((int64_t)n + 1)},
+        {&d.o2dChan, (int64_t)sizeof(Chan2DDev) * (int64_t)s.ch2d.size()},
+        {&d.o2dState, 24 * (int64_t)s.ch2d.size()},
+        {&d.o2dGk, 8 * (int64_t)s.ch2d.size()},
+        {&d.o2dExt, s.ch2d.empty() ? 0 : 16 * (int64_t)n},
     };
     int64_t total = 0;
     for (auto& it : items) total += align_up(std::max<int64_t>(it.bytes, 8));
@@ -466,8 +497,35 @@ int place_cell(hsg_engine* e, HostCell& hc) {
         st.put(d.oMsgDesc, s.mdesc);
         st.put(d.oRowPos, s.rowPos);
     }
+    if (!s.ch2d.empty()) {
+        std::vector<int> tid(s.tables2d.size(), -1);
+        for (size_t k = 0; k < s.tables2d.size(); ++k)
+            if ((rc = table2d_id(e, s.tables2d[k], &tid[k]))) return rc;
+        std::vector<Chan2DDev> ch(s.ch2d.size());
+        std::vector<double> stt(3 * s.ch2d.size());
+        for (size_t k = 0; k < s.ch2d.size(); ++k) {
+            const HostChan2D& h = s.ch2d[k];
+            Chan2DDev& c = ch[k];
+            c.Gbar = h.Gbar; c.Ek = h.Ek; c.mod = h.mod;
+            for (int a = 0; a < 3; ++a) {
+                c.pw[a] = h.pw[a];
+                c.tabA[a] = h.tabA[a] >= 0 ? tid[h.tabA[a]] : -1;
+                c.tabB[a] = h.tabB[a] >= 0 ? tid[h.tabB[a]] : -1;
+                stt[3 * k + a] = h.st[a];
+            }
+            for (int q = 0; q < 6; ++q) c.dep[q] = h.dep[q];
+            c.conc1 = h.conc1; c.conc2 = h.conc2;
+            c.instant = h.instant;
+            c.compt = h.compt;
+        }
+        st.put(d.o2dStart, s.ch2dStart);
+        st.put(d.o2dChan, ch);
+        st.put(d.o2dState, stt);
+    }
     if ((rc = st.flush(e))) return rc;
     d.msg = msgOn ? 1 : 0;
+    d.n2d = (int32_t)s.ch2d.size();
+    d.values2dSent = 0;
     d.nSlots = s.nSlots;
     d.rootSlot = s.rootSlot;
     d.rootCnt = s.rootCnt;
@@ -638,7 +696,8 @@ int upload_tables(hsg_engine* e) {
     if ((rc = grow_dev(&e->dCells, &e->dCellsCap, (int)devs.size())) ||
         (rc = grow_dev(&e->dOrder, &e->dOrderCap, std::max<int>(1, (int)e->order.size()))) ||
         (rc = grow_dev(&e->dTypes, &e->dTypesCap, std::max<int>(1, (int)e->types.size()))) ||
-        (rc = grow_dev(&e->dTsets, &e->dTsetsCap, std::max<int>(1, (int)e->tsets.size()))))
+        (rc = grow_dev(&e->dTsets, &e->dTsetsCap, std::max<int>(1, (int)e->tsets.size()))) ||
+        (rc = grow_dev(&e->dTabs2d, &e->dTabs2dCap, std::max<int>(1, (int)e->tabs2d.size()))))
         return rc;
     HIPCHK(hipMemcpyAsync(e->dCells, devs.data(), sizeof(CellDev) * devs.size(), hipMemcpyHostToDevice, e->stream));
     if (!e->order.empty())
@@ -649,6 +708,9 @@ int upload_tables(hsg_engine* e) {
                               hipMemcpyHostToDevice, e->stream));
     if (!e->tsets.empty())
         HIPCHK(hipMemcpyAsync(e->dTsets, e->tsets.data(), sizeof(TableSetDev) * e->tsets.size(),
+                              hipMemcpyHostToDevice, e->stream));
+    if (!e->tabs2d.empty())
+        HIPCHK(hipMemcpyAsync(e->dTabs2d, e->tabs2d.data(), sizeof(Table2DDev) * e->tabs2d.size(),
                               hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     if ((rc = make_bins(e, e->order, e->bins))) return rc;
@@ -672,8 +734,11 @@ int make_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg_
     // split mode: message-passing cells by workgroup-size class (largest
     // cell >= 1024: 256 threads, >= 256: 128, else 64); other cells (and
     // every cell in fused mode) by LDS need (> 32 KB, > 16 KB, the rest)
+    // cells with HHChannel2D objects go to bins of their own (6, 7), which
+    // run one tick per launch after that tick's 2D-channel kernel
     auto cls = [&](int id) {
         const CellDev& d = e->cells[id]->dev;
+        if (d.n2d > 0) return d.msg ? 6 : 7;
         if (d.msg) return d.n >= 1024 ? 0 : d.n >= 256 ? 1 : 2;
         return 3 + (d.ldsBytes > 32768 ? 0 : d.ldsBytes > 16384 ? 1 : 2);
     };
@@ -692,7 +757,8 @@ int make_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg_
             cur = hsg_engine::Bin();
         }
         curCls = k;
-        cur.msg = k < 3;
+        cur.msg = k < 3 || k == 6;
+        cur.has2d = k >= 6;
         cur.cells.push_back(id);
     }
     if (!cur.cells.empty()) bins.push_back(cur);
@@ -727,6 +793,7 @@ LaunchArgs make_args(hsg_engine* e, const int32_t* order, int nsteps, int first)
     a.order = order;
     a.types = e->dTypes;
     a.tsets = e->dTsets;
+    a.tabs2d = e->dTabs2d;
     a.spikes = e->dSpikes;
     a.spikeCount = e->dSpikeCount;
     a.spikeCap = kSpikeCap;
@@ -790,14 +857,22 @@ int launch_round(hsg_engine* e, std::vector<hsg_engine::Bin>& bins, int chunk, i
     for (size_t b = 0; b < bins.size(); ++b) {
         hipStream_t st = e->binStream[b];
         HIPCHK(hipStreamWaitEvent(st, e->evFork, 0));
-        if (e->split) {
-            // per tick: the channel kernel over the bin's tiles, then the solve
+        const int nb = (int)bins[b].cells.size();
+        if (e->split || bins[b].has2d) {
+            // tick by tick: the HHChannel2D objects (bins that have them),
+            // then the channel kernel over the bin's tiles and the solve, or
+            // one fused tick
             for (int k = 0; k < chunk; ++k) {
                 LaunchArgs a = make_args(e, bins[b].dOrder, 1, first && k == 0);
                 a.ldsBytes = bins[b].lds;
+                if (bins[b].has2d) HIPCHK(hsg_launch_chan2d(&a, nb, 0, st));
+                if (!e->split) {
+                    HIPCHK(hsg_launch_step(&a, nb, bins[b].threads, e->fp32, st));
+                    continue;
+                }
                 HIPCHK(hsg_launch_channel(&a, bins[b].dTiles, bins[b].nTiles, e->fp32, st));
-                if (bins[b].msg) HIPCHK(hsg_launch_msg(&a, (int)bins[b].cells.size(), bins[b].threads, e->fp32, st));
-                else HIPCHK(hsg_launch_solve(&a, (int)bins[b].cells.size(), bins[b].threads, e->fp32, st));
+                if (bins[b].msg) HIPCHK(hsg_launch_msg(&a, nb, bins[b].threads, e->fp32, st));
+                else HIPCHK(hsg_launch_solve(&a, nb, bins[b].threads, e->fp32, st));
             }
         } else {
             LaunchArgs a = make_args(e, bins[b].dOrder, chunk, first);
@@ -983,6 +1058,7 @@ int hsg_destroy(hsg_engine* e) {
     hipFree(e->dOrder);
     hipFree(e->dTypes);
     hipFree(e->dTsets);
+    hipFree(e->dTabs2d);
     hipFree(e->dSpikes);
     hipFree(e->dSpikeCount);
     for (auto& b : e->bins) if (b.dOrder) hipFree(b.dOrder);
@@ -1059,9 +1135,21 @@ static int reinit_some(hsg_engine* e, const std::vector<int32_t>& ids) {
     HIPCHK(hipMalloc(&dSub, sizeof(int32_t) * ids.size()));
     HIPCHK(hipMemcpyAsync(dSub, ids.data(), sizeof(int32_t) * ids.size(), hipMemcpyHostToDevice, e->stream));
     LaunchArgs a = make_args(e, dSub, 0, 0);
-    hipError_t he = hsg_launch_reinit(&a, (int)ids.size(), e->fp32, e->stream);
+    // the HHChannel2D objects reinit before the solver, from the values it
+    // last sent
+    hipError_t he = hsg_launch_chan2d(&a, (int)ids.size(), 1, e->stream);
+    if (he == hipSuccess) he = hsg_launch_reinit(&a, (int)ids.size(), e->fp32, e->stream);
     hipError_t hs = hipStreamSynchronize(e->stream);
     hipFree(dSub);
+    // HSolveActive::reinit ends with sendValues: from now on the HHChannel2D
+    // objects see the solver's Vm and [Ca]
+    for (int id : ids) {
+        CellDev& d = e->cells[id]->dev;
+        if (d.n2d > 0 && !d.values2dSent) {
+            d.values2dSent = 1;
+            e->tablesDirty = true;
+        }
+    }
     if (he != hipSuccess) return fail(HSG_E_DEVICE, std::string("reinit launch: ") + hipGetErrorString(he));
     if (hs != hipSuccess) return fail(HSG_E_DEVICE, std::string("reinit: ") + hipGetErrorString(hs));
     return 0;
@@ -1155,7 +1243,17 @@ int hsg_get_field(hsg_engine* e, int32_t cell, int32_t field, const int32_t* idx
                                         : s.Ek[(size_t)s.chanSlot[dk] * nc + h];
                     Im += (ek - V[h]) * gk;
                 }
-                Im += hc->prevExtGE[h] - hc->prevExtG[h] * V[h];
+                double eg = hc->prevExtG[h], ege = hc->prevExtGE[h];
+                if (!s.ch2d.empty() && s.ch2dStart[h] < s.ch2dStart[h + 1]) {
+                    // the last tick's external current includes the HHChannel2D objects
+                    double ext[2];
+                    if ((rc = download(e, hc->dev.o2dExt + 16 * (int64_t)h, 2, ext))) return rc;
+                    if (hc->stepsDone > 0) {
+                        eg = ext[0];
+                        ege = ext[1];
+                    }
+                }
+                Im += ege - eg * V[h];
                 out[q] = Im;
                 break;
             }
@@ -1222,6 +1320,25 @@ int hsg_get_field(hsg_engine* e, int32_t cell, int32_t field, const int32_t* idx
             case HSG_F_CABASAL: out[q] = s.pBasal[o]; break;
             case HSG_F_CACEIL: out[q] = s.pCeil[o]; break;
             case HSG_F_CAFLOOR: out[q] = s.pFloor[o]; break;
+            }
+        } else if (field >= HSG_F_2D_GBAR && field <= HSG_F_2D_MODULATION) {
+            // the HHChannel2D object's own fields
+            if (i < 0 || i >= (int)s.ch2dOfDesc.size() || s.ch2dOfDesc[i] < 0)
+                return fail(HSG_E_ARG, "unknown HHChannel2D index");
+            const int k = s.ch2dOfDesc[i];
+            const HostChan2D& hcd = s.ch2d[k];
+            switch (field) {
+            case HSG_F_2D_GBAR: out[q] = hcd.Gbar; break;
+            case HSG_F_2D_EK: out[q] = hcd.Ek; break;
+            case HSG_F_2D_MODULATION: out[q] = hcd.mod; break;
+            case HSG_F_2D_GK:
+                if ((rc = download(e, hc->dev.o2dGk + 8 * (int64_t)k, 1, &out[q]))) return rc;
+                break;
+            case HSG_F_2D_X: case HSG_F_2D_Y: case HSG_F_2D_Z:
+                if ((rc = download(e, hc->dev.o2dState + 8 * (3 * (int64_t)k + (field - HSG_F_2D_X)), 1, &out[q])))
+                    return rc;
+                break;
+            default: return fail(HSG_E_ARG, "unknown field");
             }
         } else {
             return fail(HSG_E_ARG, "unknown field");
@@ -1345,6 +1462,31 @@ int hsg_set_field(hsg_engine* e, int32_t cell, int32_t field, const int32_t* idx
                 break;
             case HSG_F_CACEIL: s.pCeil[o] = x; if ((rc = put1<double>(e, hc->dev.oPoolCeil, o, x))) return rc; break;
             case HSG_F_CAFLOOR: s.pFloor[o] = x; if ((rc = put1<double>(e, hc->dev.oPoolFloor, o, x))) return rc; break;
+            }
+        } else if (field >= HSG_F_2D_GBAR && field <= HSG_F_2D_MODULATION) {
+            if (i < 0 || i >= (int)s.ch2dOfDesc.size() || s.ch2dOfDesc[i] < 0)
+                return fail(HSG_E_ARG, "unknown HHChannel2D index");
+            const int k = s.ch2dOfDesc[i];
+            HostChan2D& hcd = s.ch2d[k];
+            const int64_t rec = hc->dev.o2dChan + (int64_t)sizeof(Chan2DDev) * k;
+            switch (field) {
+            case HSG_F_2D_GBAR:
+                hcd.Gbar = x;
+                if ((rc = put1<double>(e, rec + offsetof(Chan2DDev, Gbar), 0, x))) return rc;
+                break;
+            case HSG_F_2D_EK:
+                hcd.Ek = x;
+                if ((rc = put1<double>(e, rec + offsetof(Chan2DDev, Ek), 0, x))) return rc;
+                break;
+            case HSG_F_2D_MODULATION:
+                hcd.mod = x;
+                if ((rc = put1<double>(e, rec + offsetof(Chan2DDev, mod), 0, x))) return rc;
+                break;
+            case HSG_F_2D_GK: return fail(HSG_E_UNSUPPORTED, "Gk is recomputed every step");
+            case HSG_F_2D_X: case HSG_F_2D_Y: case HSG_F_2D_Z:
+                if ((rc = put1<double>(e, hc->dev.o2dState, 3 * (size_t)k + (field - HSG_F_2D_X), x))) return rc;
+                break;
+            default: return fail(HSG_E_ARG, "unknown field");
             }
         } else {
             return fail(HSG_E_ARG, "unknown field");

@@ -42,6 +42,24 @@ struct TableSetDev {
     TableDev v, ca;
 };
 
+// One Interpol2D table (builtins/Interpol2D.cpp), shared engine-wide.
+struct Table2DDev {
+    double  xmin, xmax, ymin, ymax, invDx, invDy;
+    int32_t nx, ny;             // xdivs + 1, ydivs + 1
+    int64_t off;                // f64 [nx * ny], x-major
+};
+
+// One HHChannel2D (biophysics/HHChannel2D.cpp) run beside the solver.
+struct Chan2DDev {
+    double  Gbar, Ek, mod;
+    double  pw[3];
+    int32_t tabA[3], tabB[3];   // engine-wide Table2DDev ids (gate with power > 0)
+    int32_t dep[6];             // table axes of gate g: dep[2g], dep[2g+1] (0 Vm, 1 conc1, 2 conc2, -1)
+    int32_t conc1, conc2;       // slot-major pool offsets of the concentrations, -1
+    int32_t instant;
+    int32_t compt;              // Hines index
+};
+
 // Per-cell descriptor (read by every workgroup working on the cell).
 struct CellDev {
     int32_t n;              // compartments
@@ -114,6 +132,14 @@ struct CellDev {
                             // the forward pass
     int64_t oFillU;         // f64[nUnits + 1]: A[1][2] after member 0 (three-member units)
     int64_t oVMid;          // f64[n + 1]
+    // HHChannel2D objects beside the solver
+    int32_t n2d;
+    int32_t values2dSent;   // 1 once the solver has sent Vm/[Ca] (reinit or step)
+    int64_t o2dStart;       // int32[n + 1]: channels of compartment i
+    int64_t o2dChan;        // Chan2DDev[n2d]
+    int64_t o2dState;       // f64[3 * n2d]: X, Y, Z
+    int64_t o2dGk;          // f64[n2d]: Gk of the last tick
+    int64_t o2dExt;         // f64[2n]: external (Gk, Gk*Ek) of the last tick per compartment
 };
 
 // Elimination unit descriptor: 12 int32 (48 bytes), read as three int4 loads.
@@ -160,6 +186,7 @@ struct LaunchArgs {
     const int32_t* order;       // cells to run (indices into cells)
     const ChanType* types;
     const TableSetDev* tsets;
+    const Table2DDev* tabs2d;
     SpikeRec*      spikes;
     unsigned long long* spikeCount;
     int32_t        spikeCap;

@@ -255,6 +255,153 @@ __device__ __forceinline__ void channel_row(const View<GT>& v, const ChanType* _
 }
 
 // ---------------------------------------------------------------------------
+// HHChannel2D objects beside the solver (biophysics/HHChannel2D.cpp).  They
+// tick before the solver from the Vm and [Ca] it sent at the end of the
+// previous tick (zero before it first sends them), integrate their gates by
+// exponential Euler and add (Gk, Gk*Ek) to their compartment's external
+// current, which updateMatrix adds to the row in one sum
+// (HSolveActive.cpp:140-148).  Computed in FP64 in both forms.
+// ---------------------------------------------------------------------------
+
+// Interpol2D::innerLookup and ::interpolate (builtins/Interpol2D.cpp:508-590)
+__device__ __forceinline__ double interp2d(const Table2DDev& t, const double* __restrict__ tab, double x, double y) {
+    if (x < t.xmin) x = t.xmin;
+    if (x > t.xmax) x = t.xmax;
+    if (y < t.ymin) y = t.ymin;
+    if (y > t.ymax) y = t.ymax;
+    const unsigned long nx = (unsigned long)t.nx, ny = (unsigned long)t.ny;
+    const double xv = (x - t.xmin) * t.invDx;
+    unsigned long xi = (unsigned long)xv;
+    if (xi >= nx) xi = nx - 1;
+    const bool endX = xi == nx - 1;
+    const double xF = xv - xi;
+    const double yv = (y - t.ymin) * t.invDy;
+    unsigned long yi = (unsigned long)yv;
+    if (yi >= ny) yi = ny - 1;
+    const bool endY = yi == ny - 1;
+    const double yF = yv - yi;
+    const double xFyF = xF * yF;
+    const double z00 = tab[xi * ny + yi];
+    double z01 = 0.0, z10 = 0.0, z11 = 0.0;
+    if (!endX) {
+        z10 = tab[(xi + 1) * ny + yi];
+        if (!endY) {
+            z11 = tab[(xi + 1) * ny + yi + 1];
+            z01 = tab[xi * ny + yi + 1];
+        }
+    } else if (!endY) {
+        z01 = tab[xi * ny + yi + 1];
+    }
+    return z00 * (1 - xF - yF + xFyF) + z10 * (xF - xFyF) + z01 * (yF - xFyF) + z11 * xFyF;
+}
+
+// HHChannelBase::integrate (biophysics/HHChannelBase.cpp:268-274)
+__device__ __forceinline__ double integrate_ee(double state, double dt, double A, double B) {
+    if (B > 1.0e-10) {
+        const double x = exp(-B * dt);
+        return state * x + (A / B) * (1 - x);
+    }
+    return state + A * dt;
+}
+
+// HHChannel2D::depValue (HHChannel2D.cpp:257-271)
+__device__ __forceinline__ double dep2d(int dep, bool sent, double Vm, const Chan2DDev& ch,
+                                        const double* __restrict__ poolCa) {
+    if (!sent) return 0.0;
+    if (dep == 0) return Vm;
+    if (dep == 1) return ch.conc1 >= 0 ? poolCa[ch.conc1] : 0.0;
+    if (dep == 2) return ch.conc2 >= 0 ? poolCa[ch.conc2] : 0.0;
+    return 0.0;
+}
+
+// The HHChannel2D objects of compartment i for one tick
+// (HHChannel2D::vProcess, :325-368), in their message order, added to the
+// compartment's external current (HSolve::addGkEk): first the call's host
+// entries (first tick of a call), then the channels.  The sum is kept in
+// o2dExt for the row and for Im.  Run by its own kernel before the tick's
+// solver kernels, so the step kernels carry none of its registers.
+__device__ __forceinline__ void chan2d_external(char* arena, const Table2DDev* __restrict__ tabs2d, double dt,
+                                             bool first, const CellDev& c, int i, double Vm,
+                                             const double* __restrict__ poolCa) {
+    const int32_t* start = A_<const int32_t>(arena, c.o2dStart);
+    if (start[i] == start[i + 1]) return;
+    double G = 0.0, GE = 0.0;
+    if (first) {
+        const int32_t* xi = A_<const int32_t>(arena, c.oExtIdx);
+        for (int e = 0; e < c.nExt; ++e)
+            if (xi[e] == i) {
+                G += A_<const double>(arena, c.oExtG)[e];
+                GE += A_<const double>(arena, c.oExtGE)[e];
+            }
+    }
+    const Chan2DDev* chans = A_<const Chan2DDev>(arena, c.o2dChan);
+    double* st = A_<double>(arena, c.o2dState);
+    double* gk = A_<double>(arena, c.o2dGk);
+    const bool sent = c.values2dSent != 0;
+#pragma unroll 1
+    for (int k = start[i]; k < start[i + 1]; ++k) {
+        const Chan2DDev& ch = chans[k];
+        double g = 0.0;
+        g += ch.Gbar;
+#pragma unroll 1
+        for (int ax = 0; ax < 3; ++ax) {
+            if (!(ch.pw[ax] > 0.0)) continue;
+            const double u = dep2d(ch.dep[2 * ax], sent, Vm, ch, poolCa);
+            const double w = dep2d(ch.dep[2 * ax + 1], sent, Vm, ch, poolCa);
+            const Table2DDev& ta = tabs2d[ch.tabA[ax]];
+            const Table2DDev& tb = tabs2d[ch.tabB[ax]];
+            const double A = interp2d(ta, A_<const double>(arena, ta.off), u, w);
+            const double B = interp2d(tb, A_<const double>(arena, tb.off), u, w);
+            double x = st[3 * k + ax];
+            x = (ch.instant >> ax) & 1 ? A / B : integrate_ee(x, dt, A, B);
+            st[3 * k + ax] = x;
+            g *= take_power<double>(x, ch.pw[ax]);
+        }
+        const double Gk = g * ch.mod;
+        gk[k] = Gk;
+        G += Gk;
+        GE += Gk * ch.Ek;
+    }
+    double* ext = A_<double>(arena, c.o2dExt);
+    ext[2 * i] = G;
+    ext[2 * i + 1] = GE;
+}
+
+// HHChannel2D::vReinit (HHChannel2D.cpp:374-426), run before the solver's
+// reinit: gates at A/B of the values last received; a table B below
+// EPSILON stops that channel's reinit.
+__device__ __forceinline__ void chan2d_reinit(char* arena, const Table2DDev* __restrict__ tabs2d, const CellDev& c) {
+    const Chan2DDev* chans = A_<const Chan2DDev>(arena, c.o2dChan);
+    double* st = A_<double>(arena, c.o2dState);
+    double* gk = A_<double>(arena, c.o2dGk);
+    const double* V = A_<const double>(arena, c.oV);
+    const double* poolCa = A_<const double>(arena, c.oPoolCa);
+    const bool sent = c.values2dSent != 0;
+    for (int k = threadIdx.x; k < c.n2d; k += blockDim.x) {
+        const Chan2DDev& ch = chans[k];
+        double g = ch.Gbar;
+        bool stop = false;
+#pragma unroll 1
+        for (int ax = 0; ax < 3 && !stop; ++ax) {
+            if (!(ch.pw[ax] > 0.0)) continue;
+            const double u = dep2d(ch.dep[2 * ax], sent, V[ch.compt], ch, poolCa);
+            const double w = dep2d(ch.dep[2 * ax + 1], sent, V[ch.compt], ch, poolCa);
+            const Table2DDev& ta = tabs2d[ch.tabA[ax]];
+            const Table2DDev& tb = tabs2d[ch.tabB[ax]];
+            const double A = interp2d(ta, A_<const double>(arena, ta.off), u, w);
+            const double B = interp2d(tb, A_<const double>(arena, tb.off), u, w);
+            if (B < 1.0e-10) {
+                stop = true;
+                break;
+            }
+            st[3 * k + ax] = A / B;
+            g *= take_power<double>(st[3 * k + ax], ch.pw[ax]);
+        }
+        if (!stop) gk[k] = g * ch.mod;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // elimination units
 // ---------------------------------------------------------------------------
 struct Desc {
@@ -641,8 +788,9 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
             else
                 for (int i = tid; i < n; i += nt) channel_row<GT, false>(v, types, i, dtg, w, false, i);
         }
-        if (nInj > 0 || (first && nExt > 0)) {
+        if (nInj > 0 || (first && nExt > 0) || c.n2d > 0) {
             work_barrier(WL);
+            const int32_t* start2d = A_<const int32_t>(arena, c.o2dStart);
             if (tid == 0) {
                 // injections: HS[4i+3] += injectVarying + injectBasal (HSolveActive.cpp:119-128)
                 const int32_t* ii = A_<const int32_t>(arena, c.oInjIdx);
@@ -659,10 +807,22 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
                     const double* xg = A_<const double>(arena, c.oExtG);
                     const double* xge = A_<const double>(arena, c.oExtGE);
                     for (int e = 0; e < nExt; ++e) {
+                        if (c.n2d > 0 && start2d[xi[e]] < start2d[xi[e] + 1]) continue;   // in o2dExt
                         w.D[xi[e]] += xg[e];
                         w.B[xi[e]] += xge[e];
                     }
                 }
+            }
+            if (c.n2d > 0) {
+                // compartments with HHChannel2D objects: the whole external
+                // current, after the injection
+                work_barrier(WL);
+                const double* ext = A_<const double>(arena, c.o2dExt);
+                for (int i = tid; i < n; i += nt)
+                    if (start2d[i] < start2d[i + 1]) {
+                        w.D[i] += ext[2 * i];
+                        w.B[i] += ext[2 * i + 1];
+                    }
             }
         }
         // full barrier: it also lands every outstanding global load, so the
@@ -790,6 +950,7 @@ __global__ void __launch_bounds__(256) hsolve_channel_kernel(LaunchArgs a, const
     }
     if (v.uniform) channel_row<GT, true>(v, types, i, (GT)a.dt, w, false, dst);
     else channel_row<GT, false>(v, types, i, (GT)a.dt, w, false, dst);
+    const bool has2d = c.n2d > 0 && A_<const int32_t>(a.arena, c.o2dStart)[i] < A_<const int32_t>(a.arena, c.o2dStart)[i + 1];
     if (c.msg) {
         const int32_t* ii = A_<const int32_t>(a.arena, c.oInjIdx);
         for (int e = 0; e < c.nInj; ++e)
@@ -799,7 +960,11 @@ __global__ void __launch_bounds__(256) hsolve_channel_kernel(LaunchArgs a, const
                 w.B[dst] += var + A_<const double>(a.arena, c.oInjBasal)[e];
                 if (a.firstOfCall) iv[e] = 0.0;
             }
-        if (a.firstOfCall) {
+        if (has2d) {
+            const double* ext = A_<const double>(a.arena, c.o2dExt);
+            w.D[dst] += ext[2 * i];
+            w.B[dst] += ext[2 * i + 1];
+        } else if (a.firstOfCall) {
             const int32_t* xi = A_<const int32_t>(a.arena, c.oExtIdx);
             for (int e = 0; e < c.nExt; ++e)
                 if (xi[e] == i) {
@@ -1097,6 +1262,25 @@ __global__ void __launch_bounds__(256) hsolve_reinit_kernel(LaunchArgs a) {
         for (int i = threadIdx.x; i < v.n; i += blockDim.x) channel_row<GT, false>(v, types, i, (GT)a.dt, w, true, i);
 }
 
+// The HHChannel2D objects of the cells in `order` for one tick, one
+// workgroup per cell, one compartment per thread; the tick's step (or
+// channel) kernel then adds o2dExt to the rows.
+__global__ void __launch_bounds__(256) hsolve_chan2d_kernel(LaunchArgs a) {
+    const CellDev& c = a.cells[a.order[blockIdx.x]];
+    if (c.n2d == 0) return;
+    const double* V = A_<const double>(a.arena, c.oV);
+    const double* poolCa = A_<const double>(a.arena, c.oPoolCa);
+    for (int i = threadIdx.x; i < c.n; i += blockDim.x)
+        chan2d_external(a.arena, a.tabs2d, a.dt, a.firstOfCall != 0, c, i, V[i], poolCa);
+}
+
+// HHChannel2D::vReinit of the cells in `order`, launched before the
+// solver's reinit kernel.
+__global__ void __launch_bounds__(256) hsolve_chan2d_reinit_kernel(LaunchArgs a) {
+    const CellDev& c = a.cells[a.order[blockIdx.x]];
+    if (c.n2d > 0) chan2d_reinit(a.arena, a.tabs2d, c);
+}
+
 }  // namespace hsg
 
 // host-side launchers (called from hsolve_engine.cpp)
@@ -1148,6 +1332,14 @@ extern "C" hipError_t hsg_launch_reinit(const hsg::LaunchArgs* a, int nblocks, i
         hipLaunchKernelGGL(hsg::hsolve_reinit_kernel<float>, dim3(nblocks), dim3(256), 0, stream, *a);
     else
         hipLaunchKernelGGL(hsg::hsolve_reinit_kernel<double>, dim3(nblocks), dim3(256), 0, stream, *a);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t hsg_launch_chan2d(const hsg::LaunchArgs* a, int nblocks, int reinit, hipStream_t stream) {
+    if (reinit)
+        hipLaunchKernelGGL(hsg::hsolve_chan2d_reinit_kernel, dim3(nblocks), dim3(256), 0, stream, *a);
+    else
+        hipLaunchKernelGGL(hsg::hsolve_chan2d_kernel, dim3(nblocks), dim3(256), 0, stream, *a);
     return hipGetLastError();
 }
 

@@ -771,6 +771,84 @@ int setup_cell(const hsg_cell_desc* d, double dt, int caAdvance, CellSetup& s, s
         s.slotTypeUniform[k] = t0;
     }
 
+    // HHChannel2D objects (biophysics/HHChannel2D.cpp) and their tables
+    s.tables2d.clear();
+    for (int k = 0; k < (d->n_tables2d > 0 ? d->n_tables2d : 0); ++k) {
+        const hsg_table2d_desc& td = d->tables2d[k];
+        if (!td.table) { err = "2-D table without values"; return HSG_E_ARG; }
+        HostTable2D t;
+        t.nx = td.xdivs + 1;
+        t.ny = td.ydivs + 1;
+        t.xmin = td.xmin; t.xmax = td.xmax; t.ymin = td.ymin; t.ymax = td.ymax;
+        t.invDx = td.xmax != td.xmin ? td.xdivs / (td.xmax - td.xmin) : 1.0;
+        t.invDy = td.ymax != td.ymin ? td.ydivs / (td.ymax - td.ymin) : 1.0;
+        t.table.assign(td.table, td.table + (size_t)t.nx * t.ny);
+        uint64_t h = 1469598103934665603ULL;
+        h = fnv(&t.nx, sizeof t.nx, h);
+        h = fnv(&t.ny, sizeof t.ny, h);
+        h = fnv(&t.xmin, 4 * sizeof(double), h);
+        h = fnv(t.table.data(), t.table.size() * sizeof(double), h);
+        t.hash = h;
+        s.tables2d.push_back(std::move(t));
+    }
+    const int n2 = d->n_chans2d > 0 ? d->n_chans2d : 0;
+    std::vector<std::vector<int>> byCompt(n);
+    for (int k = 0; k < n2; ++k) {
+        const hsg_chan2d_desc& cd = d->chans2d[k];
+        if (cd.compartment < 0 || cd.compartment >= nd || s.descToHines[cd.compartment] < 0) {
+            err = "HHChannel2D on a compartment outside the cell";
+            return HSG_E_MODEL;
+        }
+        byCompt[s.descToHines[cd.compartment]].push_back(k);
+    }
+    s.ch2d.clear();
+    s.ch2dStart.assign(n + 1, 0);
+    s.ch2dOfDesc.assign(n2, -1);
+    for (int i = 0; i < n; ++i) {
+        s.ch2dStart[i] = (int32_t)s.ch2d.size();
+        for (int k : byCompt[i]) {
+            const hsg_chan2d_desc& cd = d->chans2d[k];
+            HostChan2D c{};
+            c.compt = i;
+            c.desc = k;
+            c.instant = cd.instant;
+            const double pw[3] = {cd.Xpower, cd.Ypower, cd.Zpower};
+            const double st[3] = {cd.X, cd.Y, cd.Z};
+            for (int a = 0; a < 3; ++a) {
+                c.pw[a] = pw[a];
+                c.st[a] = st[a];
+                c.tabA[a] = cd.tableA[a];
+                c.tabB[a] = cd.tableB[a];
+                if (pw[a] > 0.0 && (cd.tableA[a] < 0 || cd.tableA[a] >= (int)s.tables2d.size() ||
+                                    cd.tableB[a] < 0 || cd.tableB[a] >= (int)s.tables2d.size())) {
+                    err = "HHChannel2D gate without its tables";
+                    return HSG_E_MODEL;
+                }
+            }
+            for (int q = 0; q < 6; ++q) {
+                if (cd.dep[q] < -1 || cd.dep[q] > 2) { err = "HHChannel2D: unknown table dependency"; return HSG_E_ARG; }
+                c.dep[q] = cd.dep[q];
+            }
+            auto pool_at = [&](int p, int& out) {
+                out = -1;
+                if (p < 0) return true;
+                if (p >= npd || s.poolCompt[p] < 0) return false;   // a pool the solver does not run
+                out = s.poolSlot[p] * n + s.poolCompt[p];
+                return true;
+            };
+            if (!pool_at(cd.conc1, c.conc1) || !pool_at(cd.conc2, c.conc2)) {
+                err = "HHChannel2D reads a CaConc that no HHChannel of the cell uses";
+                return HSG_E_UNSUPPORTED;
+            }
+            c.Gbar = cd.Gbar;
+            c.Ek = cd.Ek;
+            c.mod = cd.modulation;
+            s.ch2dOfDesc[k] = (int)s.ch2d.size();
+            s.ch2d.push_back(c);
+        }
+    }
+    s.ch2dStart[n] = (int32_t)s.ch2d.size();
+
     // SpikeGens in Hines order of their compartments
     std::vector<std::vector<int>> spk(nd);
     for (int k = 0; k < d->n_spikegens; ++k) {

@@ -27,6 +27,25 @@ struct HostTableSet {
     std::string key;
 };
 
+// One Interpol2D (builtins/Interpol2D.cpp:173-191 for invDx/invDy).
+struct HostTable2D {
+    uint32_t nx = 0, ny = 0;          // xdivs + 1, ydivs + 1
+    double xmin = 0, xmax = 0, ymin = 0, ymax = 0, invDx = 1, invDy = 1;
+    std::vector<double> table;        // [ix * ny + iy]
+    uint64_t hash = 0;
+};
+
+// One HHChannel2D (device record; see Chan2DDev).
+struct HostChan2D {
+    int compt;                        // Hines index
+    int desc;                         // index in hsg_cell_desc.chans2d
+    int instant;
+    int tabA[3], tabB[3];             // local 2-D table indices, -1
+    int dep[6];
+    int conc1, conc2;                 // slot-major pool offsets (q*n + i), -1
+    double Gbar, Ek, mod, pw[3], st[3];
+};
+
 struct HostChanType {
     double Xpower, Ypower, Zpower;
     int instant, nGates, colX, colY, colZ, caDepLocal, caTgtLocal;
@@ -97,6 +116,12 @@ struct CellSetup {
     std::vector<int> spkEdge;
     std::vector<double> spkThr, spkRefr;
     // types used (engine-wide ids filled by the engine)
+    // HHChannel2D objects beside the solver, sorted by compartment (desc
+    // order within one), with the compartment -> channels index (n + 1)
+    std::vector<HostChan2D> ch2d;
+    std::vector<int32_t> ch2dStart;
+    std::vector<int> ch2dOfDesc;             // desc index -> position in ch2d
+    std::vector<HostTable2D> tables2d;
     std::vector<HostChanType> localTypes;    // indexed by local type id
     std::vector<int> chTypeLocal;            // [k*n+i] local type id
     HostTableSet tables;

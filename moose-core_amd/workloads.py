@@ -10,7 +10,9 @@
                       batched ensemble (dt = 25 us).
   random_ca_cell()    a random tree with 2-4 children per junction and six
                       channel types (Na, Kdr, KA, CaL, Ca-dependent K-AHP, plus
-                      leak through Rm) and one CaConc pool per compartment.
+                      leak through Rm) and one CaConc pool per compartment;
+                      with kahp2d the K-AHP is an HHChannel2D on a 3000 x 500
+                      Vm x [Ca] table.
 
 Passive parameters follow rdesigneur's buildCompt (RM = 1 ohm m^2,
 RA = 1 ohm m, CM = 0.01 F/m^2, Em = initVm = -65 mV); channel kinetics follow
@@ -18,8 +20,8 @@ python/rdesigneur/rdesigneurProtos.py:65-114 (squid HH, EREST_ACT_HH = -70 mV).
 """
 import numpy as np
 
-from .cellspec import CellSpec, Gate, channel_records, interleave_channels, passive_compartments
-from ._abi import CACONC_DT, SPIKEGEN_DT
+from .cellspec import CellSpec, Gate, Table2D, channel_records, interleave_channels, passive_compartments
+from ._abi import CACONC_DT, CHAN2D_DT, SPIKEGEN_DT
 
 PER_MS = 1e3
 EREST_ACT_HH = -0.070
@@ -196,7 +198,27 @@ def ca_gates(vmin=-0.100, vmax=0.050, divs=3000, camin=0.0, camax=1.0, cadivs=50
     return g
 
 
+def kahp2d_tables(vmin=-0.100, vmax=0.050, vdivs=3000, camin=0.0, camax=1.0, cadivs=500):
+    """The K-AHP gate as an HHChannel2D Z gate over Vm x [Ca]: the Ca-driven
+    alpha of ca_gates() scaled by a sigmoid in Vm (BK-like), beta 1/ms.
+    Returns the (A, B) Interpol2D tables, x = Vm, y = [Ca]."""
+    v = np.linspace(vmin, vmax, vdivs + 1)[:, None]
+    ca = np.linspace(camin, camax, cadivs + 1)[None, :]
+    a = np.minimum(20.0 * ca, 10.0) * (0.5 + 1.0 / (1.0 + np.exp(-(v + 0.030) / 0.008)))
+    b = np.ones_like(a)
+    mk = lambda t: Table2D(vdivs, cadivs, vmin, vmax, camin, camax, np.ascontiguousarray(t))
+    return [mk(a), mk(a + b)]
+
+
 _CA = None
+_KAHP2D = None
+
+
+def _kahp2d():
+    global _KAHP2D
+    if _KAHP2D is None:
+        _KAHP2D = kahp2d_tables()
+    return _KAHP2D
 
 
 def _ca():
@@ -206,9 +228,11 @@ def _ca():
     return _CA
 
 
-def random_ca_cell(rng, n, inject=None):
+def random_ca_cell(rng, n, inject=None, kahp2d=False):
     """A random tree with branching 2-4, six channel types and one Ca pool per
-    compartment (tau 20 ms, B from the shell volume, 1/(2 F vol))."""
+    compartment (tau 20 ms, B from the shell volume, 1/(2 F vol)).  kahp2d:
+    the K-AHP channel is an HHChannel2D over Vm x [Ca] (kahp2d_tables, 3000 x
+    500 divisions) instead of an HHChannel on [Ca]."""
     parent, depth = random_tree_parents(rng, n, 4)
     maxd = max(1, int(depth.max()))
     length = rng.uniform(5e-6, 50e-6, size=n)
@@ -227,7 +251,28 @@ def random_ca_cell(rng, n, inject=None):
                           caTarget=c)
     kahp = channel_records(n, c, 20.0 * area * scale, -0.080, Zpower=1.0, gates=(-1, -1, 6),
                            useConcentration=1, caDepend=c)
-    chans = interleave_channels([na, kdr, ka, cal, kahp])
+    chans2d = np.zeros(0, CHAN2D_DT)
+    tables2d = []
+    if kahp2d:
+        chans = interleave_channels([na, kdr, ka, cal])
+        tables2d = _kahp2d()
+        chans2d = np.zeros(n, CHAN2D_DT)
+        chans2d["compartment"] = c
+        chans2d["tableA"] = -1
+        chans2d["tableB"] = -1
+        chans2d["tableA"][:, 2] = 0
+        chans2d["tableB"][:, 2] = 1
+        chans2d["dep"] = -1
+        chans2d["dep"][:, 4] = 0                    # Z table x: Vm
+        chans2d["dep"][:, 5] = 1                    # Z table y: concen
+        chans2d["conc1"] = c
+        chans2d["conc2"] = -1
+        chans2d["Gbar"] = 20.0 * area * scale
+        chans2d["Ek"] = -0.080
+        chans2d["Zpower"] = 1.0
+        chans2d["modulation"] = 1.0
+    else:
+        chans = interleave_channels([na, kdr, ka, cal, kahp])
     pools = np.zeros(n, CACONC_DT)
     vol = PI * dia * dia * length / 4.0
     thick = np.minimum(0.1e-6, dia / 2.0)
@@ -242,4 +287,5 @@ def random_ca_cell(rng, n, inject=None):
     spk = np.zeros(1, SPIKEGEN_DT)
     spk["compartment"] = 0
     spk["edgeTriggered"] = 1
-    return CellSpec(comp, axial, seed=0, channels=chans, gates=_ca(), pools=pools, spikegens=spk)
+    return CellSpec(comp, axial, seed=0, channels=chans, gates=_ca(), pools=pools, spikegens=spk,
+                    tables2d=tables2d, chans2d=chans2d)

@@ -129,6 +129,23 @@ typedef struct {
     double  *injectVarying, *injectBasal;
 } or_hines;
 
+/* An Interpol2D (builtins/Interpol2D.cpp) and an HHChannel2D
+ * (biophysics/HHChannel2D.cpp) beside the solver. */
+typedef struct {
+    unsigned nx, ny;                  /* xdivs+1, ydivs+1 */
+    double   xmin, xmax, ymin, ymax, invDx, invDy;
+    const double* t;                  /* [ix*ny + iy], owned copy */
+} or_tab2d;
+
+typedef struct {
+    int    compt;                     /* Hines index */
+    int    instant;
+    int    tabA[3], tabB[3];
+    int    dep[6];
+    int    conc1, conc2;              /* oracle pool indices, -1 */
+    double Gbar, Ek, pw[3], st[3], mod, Gk;
+} or_chan2d;
+
 struct or_solver {
     or_hines h;
     double   hsolveDt;           /* HSolve::dt_ (shadows HinesMatrix::dt_) */
@@ -167,6 +184,11 @@ struct or_solver {
     ivec     spikeGenLog;        /* recorded spikes */
     long    *spikeStepLog; int spikeStepCap;
     long     stepCount;          /* steps since reinit */
+    /* HHChannel2D objects, desc order */
+    int        n2d, nTab2d;
+    or_chan2d* ch2d;
+    or_tab2d*  tab2d;
+    int        valuesSent;       /* the solver has sent Vm/[Ca] at least once */
 };
 
 /* ------------------------------------------------------------------------ */
@@ -1003,6 +1025,59 @@ int or_create(const hsg_cell_desc* d, double dt, int caAdvance, or_solver** out)
     /* readExternalChannels — HSolveActiveSetup.cpp:580-601 */
     s->externalCurrent = (double*)calloc(2 * (size_t)n, sizeof(double));
     s->prevExtCurr = (double*)calloc(2 * (size_t)n, sizeof(double));
+    /* HHChannel2D objects and their Interpol2D tables */
+    s->nTab2d = d->n_tables2d > 0 ? d->n_tables2d : 0;
+    s->tab2d = (or_tab2d*)calloc(s->nTab2d + 1, sizeof(or_tab2d));
+    for (int k = 0; k < s->nTab2d; ++k) {
+        const hsg_table2d_desc* td = &d->tables2d[k];
+        or_tab2d* t = &s->tab2d[k];
+        t->nx = td->xdivs + 1;
+        t->ny = td->ydivs + 1;
+        t->xmin = td->xmin; t->xmax = td->xmax; t->ymin = td->ymin; t->ymax = td->ymax;
+        /* Interpol2D ctor (builtins/Interpol2D.cpp:173-191) */
+        t->invDx = td->xmax != td->xmin ? td->xdivs / (td->xmax - td->xmin) : 1.0;
+        t->invDy = td->ymax != td->ymin ? td->ydivs / (td->ymax - td->ymin) : 1.0;
+        double* copy = (double*)malloc(sizeof(double) * (size_t)t->nx * t->ny);
+        memcpy(copy, td->table, sizeof(double) * (size_t)t->nx * t->ny);
+        t->t = copy;
+    }
+    s->n2d = d->n_chans2d > 0 ? d->n_chans2d : 0;
+    s->ch2d = (or_chan2d*)calloc(s->n2d + 1, sizeof(or_chan2d));
+    for (int k = 0; k < s->n2d; ++k) {
+        const hsg_chan2d_desc* cd = &d->chans2d[k];
+        or_chan2d* c = &s->ch2d[k];
+        if (cd->compartment < 0 || cd->compartment >= nd || s->descToHines[cd->compartment] < 0) {
+            or_destroy(s);
+            return HSG_E_MODEL;
+        }
+        c->compt = s->descToHines[cd->compartment];
+        c->instant = cd->instant;
+        const double pw[3] = {cd->Xpower, cd->Ypower, cd->Zpower};
+        const double st[3] = {cd->X, cd->Y, cd->Z};
+        for (int a = 0; a < 3; ++a) {
+            c->pw[a] = pw[a];
+            c->st[a] = st[a];
+            c->tabA[a] = cd->tableA[a];
+            c->tabB[a] = cd->tableB[a];
+            if (pw[a] > 0.0 && (cd->tableA[a] < 0 || cd->tableA[a] >= s->nTab2d ||
+                                cd->tableB[a] < 0 || cd->tableB[a] >= s->nTab2d)) {
+                or_destroy(s);
+                return HSG_E_MODEL;
+            }
+        }
+        for (int q = 0; q < 6; ++q) c->dep[q] = cd->dep[q];
+        c->conc1 = cd->conc1 >= 0 && cd->conc1 < d->n_pools ? s->descToPool[cd->conc1] : -1;
+        c->conc2 = cd->conc2 >= 0 && cd->conc2 < d->n_pools ? s->descToPool[cd->conc2] : -1;
+        if ((cd->conc1 >= 0 && c->conc1 < 0) || (cd->conc2 >= 0 && c->conc2 < 0)) {
+            or_destroy(s);                /* a pool the solver does not run */
+            return HSG_E_UNSUPPORTED;
+        }
+        c->Gbar = cd->Gbar;
+        c->Ek = cd->Ek;
+        c->mod = cd->modulation;
+        c->Gk = 0.0;
+    }
+    s->valuesSent = 0;
     *out = s;
     return 0;
 }
@@ -1022,6 +1097,9 @@ void or_destroy(or_solver* s) {
     free(s->externalCalcium); free(s->externalCurrent); free(s->prevExtCurr);
     free(s->spike); free(s->spikeToDesc);
     iv_free(&s->spikeGenLog); free(s->spikeStepLog);
+    if (s->tab2d)
+        for (int k = 0; k < s->nTab2d; ++k) free((void*)s->tab2d[k].t);
+    free(s->tab2d); free(s->ch2d);
     free(s);
 }
 
@@ -1066,8 +1144,11 @@ static void reinit_channels(or_solver* s) {
     }
 }
 
+static void chan2d_reinit(or_solver* s);
+
 void or_reinit(or_solver* s, double dt) {
     or_hines* h = &s->h;
+    chan2d_reinit(s);            /* their reinit runs before the solver's */
     s->hsolveDt = dt;
     for (unsigned i = 0; i < 2 * h->nCompt; ++i) s->externalCurrent[i] = 0.0;
     for (int k = 0; k < s->nSpike; ++k) s->spike[k].lastEvent = -s->spike[k].refractT;
@@ -1080,6 +1161,7 @@ void or_reinit(or_solver* s, double dt) {
     reinit_channels(s);
     s->stepCount = 0;
     s->spikeGenLog.n = 0;
+    s->valuesSent = 1;           /* HSolveActive::reinit ends with sendValues */
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1246,10 +1328,112 @@ static void send_spikes(or_solver* s, double dt, double t) {
     }
 }
 
+/* ------------------------------------------------------------------------ */
+/* HHChannel2D beside the solver                                             */
+/* ------------------------------------------------------------------------ */
+
+/* Interpol2D::innerLookup and ::interpolate (builtins/Interpol2D.cpp:508-590) */
+static double interp2d(const or_tab2d* t, double x, double y) {
+    if (t->nx == 0) return 0.0;
+    if (x < t->xmin) x = t->xmin;
+    if (x > t->xmax) x = t->xmax;
+    if (y < t->ymin) y = t->ymin;
+    if (y > t->ymax) y = t->ymax;
+    int endX = 0, endY = 0;
+    double z00 = 0.0, z01 = 0.0, z10 = 0.0, z11 = 0.0;
+    double xv = (x - t->xmin) * t->invDx;
+    unsigned long xi = (unsigned long)xv;
+    if (xi >= t->nx) xi = t->nx - 1;
+    if (xi == t->nx - 1) endX = 1;
+    double xF = xv - xi;
+    double yv = (y - t->ymin) * t->invDy;
+    unsigned long yi = (unsigned long)yv;
+    if (yi >= t->ny) yi = t->ny - 1;
+    if (yi == t->ny - 1) endY = 1;
+    double yF = yv - yi;
+    double xFyF = xF * yF;
+    z00 = t->t[xi * t->ny + yi];
+    if (!endX) {
+        z10 = t->t[(xi + 1) * t->ny + yi];
+        if (!endY) {
+            z11 = t->t[(xi + 1) * t->ny + yi + 1];
+            z01 = t->t[xi * t->ny + yi + 1];
+        }
+    } else if (!endY) {
+        z01 = t->t[xi * t->ny + yi + 1];
+    }
+    return z00 * (1 - xF - yF + xFyF) + z10 * (xF - xFyF) + z01 * (yF - xFyF) + z11 * xFyF;
+}
+
+/* HHChannelBase::integrate (biophysics/HHChannelBase.cpp:268-274) */
+static double integrate_ee(double state, double dt, double A, double B) {
+    if (B > 1.0e-10) {
+        double x = exp(-B * dt);
+        return state * x + (A / B) * (1 - x);
+    }
+    return state + A * dt;
+}
+
+/* HHChannel2D::depValue (HHChannel2D.cpp:257-271): the Vm and concentrations
+ * the channel last received (zero before the solver first sends them) */
+static double dep_value(const or_solver* s, const or_chan2d* c, int dep) {
+    if (!s->valuesSent) return 0.0;
+    switch (dep) {
+    case 0: return s->h.V[c->compt];
+    case 1: return c->conc1 >= 0 ? s->ca[c->conc1] : 0.0;
+    case 2: return c->conc2 >= 0 ? s->ca[c->conc2] : 0.0;
+    default: return 0.0;
+    }
+}
+
+/* HHChannel2D::vProcess (HHChannel2D.cpp:325-368), then the channel's
+ * channelOut -> HSolve::addGkEk (HSolveInterface.cpp:246-252) */
+static void chan2d_process(or_solver* s, double dt) {
+    for (int k = 0; k < s->n2d; ++k) {
+        or_chan2d* c = &s->ch2d[k];
+        double g = 0.0;
+        g += c->Gbar;
+        for (int a = 0; a < 3; ++a) {
+            if (!(c->pw[a] > 0.0)) continue;
+            const double u = dep_value(s, c, c->dep[2 * a]), w = dep_value(s, c, c->dep[2 * a + 1]);
+            const double A = interp2d(&s->tab2d[c->tabA[a]], u, w);
+            const double B = interp2d(&s->tab2d[c->tabB[a]], u, w);
+            if (c->instant & (1 << a)) c->st[a] = A / B;
+            else c->st[a] = integrate_ee(c->st[a], dt, A, B);
+            g *= take_power(c->st[a], c->pw[a]);
+        }
+        c->Gk = g * c->mod;
+        s->externalCurrent[2 * c->compt] += c->Gk;
+        s->externalCurrent[2 * c->compt + 1] += c->Gk * c->Ek;
+    }
+}
+
+/* HHChannel2D::vReinit (HHChannel2D.cpp:374-426): gates at A/B of the last
+ * received values; a table B below EPSILON stops the channel's reinit */
+static void chan2d_reinit(or_solver* s) {
+    for (int k = 0; k < s->n2d; ++k) {
+        or_chan2d* c = &s->ch2d[k];
+        double g = c->Gbar;
+        int stop = 0;
+        for (int a = 0; a < 3 && !stop; ++a) {
+            if (!(c->pw[a] > 0.0)) continue;
+            const double u = dep_value(s, c, c->dep[2 * a]), w = dep_value(s, c, c->dep[2 * a + 1]);
+            const double A = interp2d(&s->tab2d[c->tabA[a]], u, w);
+            const double B = interp2d(&s->tab2d[c->tabB[a]], u, w);
+            if (B < 1.0e-10) { stop = 1; break; }
+            c->st[a] = A / B;
+            g *= take_power(c->st[a], c->pw[a]);
+        }
+        if (!stop) c->Gk = g * c->mod;
+    }
+}
+
 void or_step(or_solver* s, double dt, double currTime) {
     or_hines* h = &s->h;
     if (h->nCompt <= 0) return;
     s->stepCount++;
+    /* the HHChannel2D objects tick before the solver (their tick 4) */
+    chan2d_process(s, dt);
     advance_channels(s, dt);
     calculate_channel_currents(s);
     update_matrix_active(s);
@@ -1259,6 +1443,7 @@ void or_step(or_solver* s, double dt, double currTime) {
     send_spikes(s, dt, currTime);
     memcpy(s->prevExtCurr, s->externalCurrent, sizeof(double) * 2 * h->nCompt);
     for (unsigned i = 0; i < 2 * h->nCompt; ++i) s->externalCurrent[i] = 0.0;
+    s->valuesSent = 1;           /* sendValues */
 }
 
 void or_run(or_solver* s, double dt, long nsteps, const int* probe, int nprobe, double* trace) {
@@ -1334,6 +1519,19 @@ int or_get_field(const or_solver* s, int field, const int* idx, int n, double* o
             case HSG_F_INSTANT: out[k] = (double)s->instant[c]; break;
             default: return HSG_E_ARG;
             }
+        } else if (field >= 60) {
+            if (i < 0 || i >= s->n2d) return HSG_E_ARG;
+            const or_chan2d* c = &s->ch2d[i];
+            switch (field) {
+            case HSG_F_2D_GBAR: out[k] = c->Gbar; break;
+            case HSG_F_2D_EK: out[k] = c->Ek; break;
+            case HSG_F_2D_GK: out[k] = c->Gk; break;
+            case HSG_F_2D_X: out[k] = c->st[0]; break;
+            case HSG_F_2D_Y: out[k] = c->st[1]; break;
+            case HSG_F_2D_Z: out[k] = c->st[2]; break;
+            case HSG_F_2D_MODULATION: out[k] = c->mod; break;
+            default: return HSG_E_ARG;
+            }
         } else {
             if (i < 0 || s->descToPool[i] < 0) return HSG_E_ARG;
             int p = s->descToPool[i];
@@ -1379,6 +1577,18 @@ int or_set_field(or_solver* s, int field, const int* idx, int n, const double* v
             case HSG_F_Z: if (s->Zpower[c] != 0.0) s->state[state_index(s, c, 2)] = x; break;
             case HSG_F_MODULATION: if (x > 0.0) s->modulation[c] = x; break;
             case HSG_F_INSTANT: s->instant[c] = (int)x; break;
+            default: return HSG_E_ARG;
+            }
+        } else if (field >= 60) {
+            if (i < 0 || i >= s->n2d) return HSG_E_ARG;
+            or_chan2d* c = &s->ch2d[i];
+            switch (field) {
+            case HSG_F_2D_GBAR: c->Gbar = x; break;
+            case HSG_F_2D_EK: c->Ek = x; break;
+            case HSG_F_2D_X: c->st[0] = x; break;
+            case HSG_F_2D_Y: c->st[1] = x; break;
+            case HSG_F_2D_Z: c->st[2] = x; break;
+            case HSG_F_2D_MODULATION: c->mod = x; break;
             default: return HSG_E_ARG;
             }
         } else {
@@ -1526,3 +1736,18 @@ double or_matrix_getV(const or_matrix* m, int row) { return m->h.V[row]; }
 void or_matrix_update(or_matrix* m) { hp_update_matrix(&m->h); }
 void or_matrix_forward(or_matrix* m) { hp_forward(&m->h); }
 void or_matrix_backward(or_matrix* m) { hp_backward(&m->h); }
+
+/* Single-lookup entry points of the 2-D table and the exponential-Euler
+ * step, for the closed-form checks of the tests. */
+double or_interp2d(const hsg_table2d_desc* td, double x, double y) {
+    or_tab2d t;
+    t.nx = td->xdivs + 1;
+    t.ny = td->ydivs + 1;
+    t.xmin = td->xmin; t.xmax = td->xmax; t.ymin = td->ymin; t.ymax = td->ymax;
+    t.invDx = td->xmax != td->xmin ? td->xdivs / (td->xmax - td->xmin) : 1.0;
+    t.invDy = td->ymax != td->ymin ? td->ydivs / (td->ymax - td->ymin) : 1.0;
+    t.t = (double*)td->table;
+    return interp2d(&t, x, y);
+}
+
+double or_integrate_ee(double state, double dt, double A, double B) { return integrate_ee(state, dt, A, B); }

@@ -64,6 +64,8 @@ def lib():
             "or_matrix_forward": ([vp], None),
             "or_matrix_backward": ([vp], None),
             "or_lookup_row": ([dbl, dbl, C.c_uint, dbl, P(C.c_uint)], dbl),
+            "or_interp2d": ([vp, dbl, dbl], dbl),
+            "or_integrate_ee": ([dbl, dbl, dbl, dbl], dbl),
         }
         for name, (a, r) in sig.items():
             f = getattr(L, name)
@@ -83,7 +85,8 @@ def _ip(a):
 
 FIELD = {"Vm": 1, "Im": 2, "inject": 3, "initVm": 4, "Cm": 5, "Em": 6, "Rm": 7, "Ra": 8,
          "Gbar": 20, "Ek": 21, "Gk": 22, "Ik": 23, "X": 24, "Y": 25, "Z": 26,
-         "modulation": 27, "instant": 28, "Ca": 40, "CaBasal": 41, "ceiling": 42, "floor": 43}
+         "modulation": 27, "instant": 28, "Ca": 40, "CaBasal": 41, "ceiling": 42, "floor": 43,
+         "Gbar2D": 60, "Ek2D": 61, "Gk2D": 62, "X2D": 64, "Y2D": 65, "Z2D": 66, "modulation2D": 67}
 
 
 class OracleCell:

@@ -49,13 +49,16 @@ int main(void) {
   printf("hsg_caconc_desc %zu\n", sizeof(hsg_caconc_desc));
   printf("hsg_spikegen_desc %zu\n", sizeof(hsg_spikegen_desc));
   printf("hsg_leak_desc %zu\n", sizeof(hsg_leak_desc));
+  printf("hsg_table2d_desc %zu\n", sizeof(hsg_table2d_desc));
+  printf("hsg_chan2d_desc %zu\n", sizeof(hsg_chan2d_desc));
   printf("hsg_cell_desc %zu\n", sizeof(hsg_cell_desc));
   printf("hsg_options %zu\n", sizeof(hsg_options));
   printf("hsg_spike %zu\n", sizeof(hsg_spike));
   printf("hsg_cell_info %zu\n", sizeof(hsg_cell_info));
   P(hsg_channel_desc, Gbar); P(hsg_channel_desc, caDepend); P(hsg_channel_desc, modulation);
   P(hsg_leak_desc, Gk); P(hsg_spikegen_desc, threshold); P(hsg_cell_desc, leaks);
-  P(hsg_gate_desc, tableA);
+  P(hsg_gate_desc, tableA); P(hsg_cell_desc, chans2d); P(hsg_chan2d_desc, Gbar);
+  P(hsg_chan2d_desc, conc2); P(hsg_table2d_desc, table);
   return 0;
 }
 '''
@@ -84,6 +87,12 @@ int main(void) {
     assert sizes["hsg_spikegen_desc.threshold"] == _abi.SPIKEGEN_DT.fields["threshold"][1]
     assert sizes["hsg_cell_desc.leaks"] == _abi.CellDesc.leaks.offset
     assert sizes["hsg_gate_desc.tableA"] == _abi.GateDesc.tableA.offset
+    assert sizes["hsg_table2d_desc"] == ctypes.sizeof(_abi.Table2DDesc)
+    assert sizes["hsg_chan2d_desc"] == _abi.CHAN2D_DT.itemsize
+    assert sizes["hsg_cell_desc.chans2d"] == _abi.CellDesc.chans2d.offset
+    assert sizes["hsg_chan2d_desc.Gbar"] == _abi.CHAN2D_DT.fields["Gbar"][1]
+    assert sizes["hsg_chan2d_desc.conc2"] == _abi.CHAN2D_DT.fields["conc2"][1]
+    assert sizes["hsg_table2d_desc.table"] == _abi.Table2DDesc.table.offset
 
 
 def test_engine_without_gpu_raises_cleanly():
