@@ -21,16 +21,17 @@
 #include "hsolve_internal.h"
 #include "hsolve_setup.h"
 
-extern "C" hipError_t hsg_launch_step(const hsg::LaunchArgs* a, int nblocks, int threads, int fp32,
+extern "C" hipError_t hsg_launch_step(const hsg::LaunchArgs* a, int nblocks, int threads, int fp32, int slow,
                                      hipStream_t stream);
 extern "C" hipError_t hsg_launch_reinit(const hsg::LaunchArgs* a, int nblocks, int fp32,
                                        hipStream_t stream);
 extern "C" hipError_t hsg_set_max_lds(int bytes);
-extern "C" hipError_t hsg_launch_solve(const hsg::LaunchArgs* a, int nblocks, int threads, int fp32,
+extern "C" hipError_t hsg_launch_solve(const hsg::LaunchArgs* a, int nblocks, int threads, int fp32, int slow,
                                       hipStream_t stream);
 extern "C" hipError_t hsg_launch_channel(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, int fp32,
                                         hipStream_t stream);
-extern "C" hipError_t hsg_launch_chan2d(const hsg::LaunchArgs* a, int nblocks, int reinit, hipStream_t stream);
+extern "C" hipError_t hsg_launch_chan2d(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, hipStream_t stream);
+extern "C" hipError_t hsg_launch_chan2d_reinit(const hsg::LaunchArgs* a, int nblocks, hipStream_t stream);
 extern "C" hipError_t hsg_launch_msg(const hsg::LaunchArgs* a, int nblocks, int threads, int fp32,
                                     hipStream_t stream);
 
@@ -173,12 +174,14 @@ struct hsg_engine {
         int nTiles = 0;
         bool msg = false;           // message-passing solve kernel (split mode)
         bool has2d = false;         // cells with HHChannel2D objects: launched tick by tick
+        bool split = false;         // channel kernel + solve kernel per tick
+        bool slow = false;          // cells with junction groups of four or more members
         int lds = 0;
         int threads = 256;
     };
     std::vector<Bin> bins;
     bool split = false;          // channel phase as its own kernel
-    static constexpr int kMaxBins = 8;
+    static constexpr int kMaxBins = 16;
     hipStream_t binStream[kMaxBins] = {};
     hipEvent_t evFork = nullptr, evJoin[kMaxBins] = {};
     // graph of the steady-state launches
@@ -370,7 +373,10 @@ int place_cell(hsg_engine* e, HostCell& hc) {
                          (s.anySlowGroup ? (int64_t)s.HJ0.size() : 0));
     // the message-passing solve needs its slots (and the level table) in LDS
     const int64_t msgLds = align_up(8 * kSlotDoubles * ((int64_t)s.nSlots + 1) + 4 * (int64_t)s.levelStart.size());
-    const bool msgOn = e->split && s.msgOk && msgLds <= e->ldsLimit;
+    // split mode: every cell that can; automatic mode: large cells, whose
+    // channel phase needs more threads than one workgroup brings
+    const bool wantSplit = e->split || (e->opt.kernel_mode == HSG_KERNEL_AUTO && n >= kAutoSplitMin);
+    const bool msgOn = wantSplit && s.msgOk && msgLds <= e->ldsLimit;
     std::vector<Item> items = {
         {&d.oV, 8 * n}, {&d.oInitVm, 8 * n}, {&d.oCmByDt, 8 * n}, {&d.oEmByRm, 8 * n}, {&d.oDbase, 8 * n}, {&d.oOff, 8 * n},
         {&d.oDefer, 8 * n},
@@ -722,8 +728,9 @@ int upload_tables(hsg_engine* e) {
 // Threads per cell workgroup for a bin whose largest cell has n compartments.
 int default_threads(int n) { return n >= 1024 ? 256 : n >= 256 ? 128 : 64; }
 
-// Splits cells (largest first) into up to three bins by LDS need; each bin is
-// one launch with its own dynamic LDS size and workgroup size.
+// Splits cells (largest first) into bins by kernel form and LDS need; each
+// bin is one launch (per round or per tick) on its own stream, with its own
+// dynamic LDS size and workgroup size.
 int make_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg_engine::Bin>& bins) {
     for (auto& b : bins) {
         if (b.dOrder) hipFree(b.dOrder);
@@ -736,11 +743,15 @@ int make_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg_
     // every cell in fused mode) by LDS need (> 32 KB, > 16 KB, the rest)
     // cells with HHChannel2D objects go to bins of their own (6, 7), which
     // run one tick per launch after that tick's 2D-channel kernel
+    // cells with junction groups of four or more members (slowHJ) in bins of
+    // their own too (8-15), launched with the kernels that carry the register
+    // form of those groups
     auto cls = [&](int id) {
         const CellDev& d = e->cells[id]->dev;
-        if (d.n2d > 0) return d.msg ? 6 : 7;
-        if (d.msg) return d.n >= 1024 ? 0 : d.n >= 256 ? 1 : 2;
-        return 3 + (d.ldsBytes > 32768 ? 0 : d.ldsBytes > 16384 ? 1 : 2);
+        const int s = d.slowHJ ? 8 : 0;
+        if (d.n2d > 0) return s + (d.msg ? 6 : 7);
+        if (d.msg) return s + (d.n >= 1024 ? 0 : d.n >= 256 ? 1 : 2);
+        return s + 3 + (d.ldsBytes > 32768 ? 0 : d.ldsBytes > 16384 ? 1 : 2);
     };
     std::vector<int32_t> sorted(cells);
     std::stable_sort(sorted.begin(), sorted.end(), [&](int a, int b) {
@@ -757,8 +768,10 @@ int make_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg_
             cur = hsg_engine::Bin();
         }
         curCls = k;
-        cur.msg = k < 3 || k == 6;
-        cur.has2d = k >= 6;
+        cur.msg = (k & 7) < 3 || (k & 7) == 6;
+        cur.has2d = (k & 7) >= 6;
+        cur.slow = k >= 8;
+        cur.split = e->split || cur.msg;
         cur.cells.push_back(id);
     }
     if (!cur.cells.empty()) bins.push_back(cur);
@@ -858,26 +871,27 @@ int launch_round(hsg_engine* e, std::vector<hsg_engine::Bin>& bins, int chunk, i
         hipStream_t st = e->binStream[b];
         HIPCHK(hipStreamWaitEvent(st, e->evFork, 0));
         const int nb = (int)bins[b].cells.size();
-        if (e->split || bins[b].has2d) {
+        const bool split = bins[b].split;
+        if (split || bins[b].has2d) {
             // tick by tick: the HHChannel2D objects (bins that have them),
             // then the channel kernel over the bin's tiles and the solve, or
             // one fused tick
             for (int k = 0; k < chunk; ++k) {
                 LaunchArgs a = make_args(e, bins[b].dOrder, 1, first && k == 0);
                 a.ldsBytes = bins[b].lds;
-                if (bins[b].has2d) HIPCHK(hsg_launch_chan2d(&a, nb, 0, st));
-                if (!e->split) {
-                    HIPCHK(hsg_launch_step(&a, nb, bins[b].threads, e->fp32, st));
+                if (bins[b].has2d) HIPCHK(hsg_launch_chan2d(&a, bins[b].dTiles, bins[b].nTiles, st));
+                if (!split) {
+                    HIPCHK(hsg_launch_step(&a, nb, bins[b].threads, e->fp32, bins[b].slow, st));
                     continue;
                 }
                 HIPCHK(hsg_launch_channel(&a, bins[b].dTiles, bins[b].nTiles, e->fp32, st));
                 if (bins[b].msg) HIPCHK(hsg_launch_msg(&a, nb, bins[b].threads, e->fp32, st));
-                else HIPCHK(hsg_launch_solve(&a, nb, bins[b].threads, e->fp32, st));
+                else HIPCHK(hsg_launch_solve(&a, nb, bins[b].threads, e->fp32, bins[b].slow, st));
             }
         } else {
             LaunchArgs a = make_args(e, bins[b].dOrder, chunk, first);
             a.ldsBytes = bins[b].lds;
-            HIPCHK(hsg_launch_step(&a, (int)bins[b].cells.size(), bins[b].threads, e->fp32, st));
+            HIPCHK(hsg_launch_step(&a, nb, bins[b].threads, e->fp32, bins[b].slow, st));
         }
         HIPCHK(hipEventRecord(e->evJoin[b], st));
         HIPCHK(hipStreamWaitEvent(e->stream, e->evJoin[b], 0));
@@ -1137,7 +1151,7 @@ static int reinit_some(hsg_engine* e, const std::vector<int32_t>& ids) {
     LaunchArgs a = make_args(e, dSub, 0, 0);
     // the HHChannel2D objects reinit before the solver, from the values it
     // last sent
-    hipError_t he = hsg_launch_chan2d(&a, (int)ids.size(), 1, e->stream);
+    hipError_t he = hsg_launch_chan2d_reinit(&a, (int)ids.size(), e->stream);
     if (he == hipSuccess) he = hsg_launch_reinit(&a, (int)ids.size(), e->fp32, e->stream);
     hipError_t hs = hipStreamSynchronize(e->stream);
     hipFree(dSub);

@@ -152,8 +152,13 @@ struct CellDev {
 //   i3: m0 | m1 << 16      i4: m2      i5: fill slot (s == 3)
 //   i6..i11: a01, a02, a12 as doubles (the initial off-diagonal entries,
 //            HJCopy_ / HS[4x+1], computed at setup as the reference does)
-//   groups of more than kFastGroupMax members (or cells past 16-bit indices)
-//   take the slow path: i1 = group index, everything else from memory
+//   groups of more than kFastGroupMax members take one of two other forms:
+//   up to kDescGroupMax members, everything in the descriptor (bit 5):
+//     i1: deferred slot for the last member's terms (-1: apply directly)
+//     i2: offset of the group's junction coefficients in HJ
+//     i3: m0 | m1 << 16   i4: m2 | m3 << 16   i5: m4
+//     i6..i8: deferred terms of members 1..3: slot | count << 24, -1: none
+//   larger groups (bit 4): i1 = group index, everything else from memory
 // Message-passing unit descriptor (split mode): 12 int32 (48 bytes).
 //   i0: s (bits 0-3) | cnt0 << 4 | cnt1 << 6: term pairs the feeders of
 //       members 0 and 1 deliver (0: no feeder, 1: a two-member feeder,
@@ -171,6 +176,8 @@ constexpr int kMsgDescInts = 12;
 constexpr int kSlotDoubles = 4;    // two (tD, tB) pairs; VMid reuses the first
 
 constexpr int kFastGroupMax = 3;
+constexpr int kDescGroupMax = 5;   // groups held in their descriptor (register form)
+constexpr int kAutoSplitMin = 4096;  // automatic mode: cells from this size run split
 constexpr int kChannelTile = 256;  // compartments per workgroup of the channel kernel
 constexpr int kDescInts = 12;
 

@@ -519,9 +519,110 @@ __device__ __forceinline__ void bwd_small(const Desc& u, Work w) {
 // HJ_ layout: for each non-last member, (A[c][y], A[y][c]) per later y)
 __host__ __device__ constexpr int hb(int S, int p) { return p * (2 * (S - 1) - (p - 1)); }
 
-// Groups larger than kFastGroupMax (or cells past 16-bit indices): member
-// lists, deferred slots and the working junction coefficients in memory.
-__device__ __noinline__ void fwd_group_slow(char* arena, const CellDev& c, Desc u, Work w) {
+// Junction groups of four or five members (descriptor form): the members,
+// the offset of the group's junction coefficients and the deferred term
+// slots come in the descriptor; the group's rows and coefficients are loaded
+// into registers at once, eliminated in registers in the operation order of
+// the in-memory form below, and stored back -- one memory round trip per
+// group instead of one per operation.
+template <int S>
+__device__ __forceinline__ void group_members(const Desc& u, int (&m)[S]) {
+    m[0] = u.a.w & 0xffff;
+    m[1] = (u.a.w >> 16) & 0xffff;
+    m[2] = u.b.x & 0xffff;
+    m[3] = (u.b.x >> 16) & 0xffff;
+    if constexpr (S == 5) m[4] = u.b.y;
+}
+
+template <int S>
+__device__ __forceinline__ void fwd_group_desc(const Desc& u, const double* __restrict__ hj0, Work w) {
+    int m[S];
+    group_members<S>(u, m);
+    const int deferLast = u.a.y;
+    const int hjOff = u.a.z;
+    const int dinfo[3] = {u.b.z, u.b.w, u.c.x};   // members 1..3
+    double HJ[S * (S - 1)];
+#pragma unroll
+    for (int k = 0; k < S * (S - 1); ++k) HJ[k] = hj0[hjOff + k];
+    double D[S], B[S];
+#pragma unroll
+    for (int p = 0; p < S; ++p) {
+        D[p] = w.D[m[p]];
+        B[p] = w.B[m[p]];
+    }
+#pragma unroll
+    for (int p = 0; p + 1 < S; ++p) {
+        if (p >= 1 && dinfo[p - 1] >= 0) {
+            const int d = dinfo[p - 1] & 0xffffff, cnt = dinfo[p - 1] >> 24;
+            for (int t = 0; t < cnt; ++t) {
+                D[p] -= w.T[2 * (d + t)];
+                B[p] -= w.T[2 * (d + t) + 1];
+            }
+        }
+        const double pivot = D[p];
+        const double Bc = B[p];
+        const int base = hb(S, p);
+#pragma unroll
+        for (int q = p + 1; q < S; ++q) {
+            const double lower = HJ[base + 2 * (q - p - 1) + 1];
+            const double tD = lower / pivot * HJ[base + 2 * (q - p - 1)];
+            const double tB = lower / pivot * Bc;
+            if (q == S - 1 && deferLast >= 0) {
+                w.T[2 * (deferLast + p)] = tD;
+                w.T[2 * (deferLast + p) + 1] = tB;
+            } else {
+                D[q] -= tD;
+                B[q] -= tB;
+            }
+#pragma unroll
+            for (int z = p + 1; z < S; ++z) {
+                if (z == q) continue;
+                const double above = HJ[base + 2 * (z - p - 1)];
+                const int idx = q < z ? hb(S, q) + 2 * (z - q - 1) : hb(S, z) + 2 * (q - z - 1) + 1;
+                HJ[idx] -= lower / pivot * above;
+            }
+        }
+    }
+    // members 1..S-2 always change; the last only without a deferred slot
+#pragma unroll
+    for (int p = 1; p < S; ++p) {
+        if (p == S - 1 && deferLast >= 0) break;
+        w.D[m[p]] = D[p];
+        w.B[m[p]] = B[p];
+    }
+#pragma unroll
+    for (int k = 0; k < S * (S - 1); ++k) w.HJ[hjOff + k] = HJ[k];
+}
+
+template <int S>
+__device__ __forceinline__ void bwd_group_desc(const Desc& u, Work w) {
+    int m[S];
+    group_members<S>(u, m);
+    const int hjOff = u.a.z;
+    double HJ[S * (S - 1)];
+#pragma unroll
+    for (int k = 0; k < S * (S - 1); ++k) HJ[k] = w.HJ[hjOff + k];
+    double V[S], D[S];
+#pragma unroll
+    for (int p = 0; p < S; ++p) {
+        V[p] = w.B[m[p]];
+        D[p] = w.D[m[p]];
+    }
+#pragma unroll
+    for (int p = S - 2; p >= 0; --p) {
+        const int base = hb(S, p);
+        double acc = V[p];
+#pragma unroll
+        for (int q = S - 1; q > p; --q) acc -= V[q] * HJ[base + 2 * (q - p - 1)];
+        V[p] = acc / D[p];
+    }
+#pragma unroll
+    for (int p = 0; p + 1 < S; ++p) w.B[m[p]] = V[p];
+}
+
+// Groups in memory: member lists, deferred slots and the working junction
+// coefficients read and written operation by operation.
+__device__ __noinline__ void fwd_group_mem(char* arena, const CellDev& c, Desc u, Work w) {
     const int32_t* defer = A_<const int32_t>(arena, c.oDefer);
     const int g = u.a.y;
     const int4 gi = A_<const int4>(arena, c.oGrpInfo)[g];
@@ -568,7 +669,7 @@ __device__ __noinline__ void fwd_group_slow(char* arena, const CellDev& c, Desc 
     }
 }
 
-__device__ __noinline__ void bwd_group_slow(char* arena, const CellDev& c, Desc u, Work w) {
+__device__ __noinline__ void bwd_group_mem(char* arena, const CellDev& c, Desc u, Work w) {
     const int g = u.a.y;
     const int4 gi = A_<const int4>(arena, c.oGrpInfo)[g];
     const int32_t* mem = A_<const int32_t>(arena, c.oGrpMem) + gi.x;
@@ -582,14 +683,48 @@ __device__ __noinline__ void bwd_group_slow(char* arena, const CellDev& c, Desc 
     }
 }
 
-__device__ __forceinline__ void forward_unit(char* arena, const CellDev& c, const Desc& u, Work w) {
-    if (u.a.x & 16) fwd_group_slow(arena, c, u, w);
-    else fwd_small(u, w);
+// The descriptor-form groups out of line.  Only the kernels for cells that
+// have such groups carry them (their registers would lower the occupancy of
+// every other cell's kernel).
+__device__ __noinline__ void fwd_group_regs(char* arena, const CellDev& c, Desc u, Work w) {
+    const double* hj0 = A_<const double>(arena, c.oHJ0);
+    if ((u.a.x & 15) == 4) fwd_group_desc<4>(u, hj0, w);
+    else fwd_group_desc<5>(u, hj0, w);
 }
 
+__device__ __noinline__ void bwd_group_regs(Desc u, Work w) {
+    if ((u.a.x & 15) == 4) bwd_group_desc<4>(u, w);
+    else bwd_group_desc<5>(u, w);
+}
+
+template <bool SLOWG>
+__device__ __forceinline__ void forward_unit(char* arena, const CellDev& c, const Desc& u, Work w) {
+    if constexpr (SLOWG) {
+        if (u.a.x & 32) {
+            fwd_group_regs(arena, c, u, w);
+            return;
+        }
+        if (u.a.x & 16) {
+            fwd_group_mem(arena, c, u, w);
+            return;
+        }
+    }
+    fwd_small(u, w);
+}
+
+template <bool SLOWG>
 __device__ __forceinline__ void backward_unit(char* arena, const CellDev& c, const Desc& u, Work w) {
-    if (u.a.x & 16) bwd_group_slow(arena, c, u, w);
-    else bwd_small(u, w);
+    if constexpr (SLOWG) {
+        if (u.a.x & 32) {
+            bwd_group_regs(u, w);
+            return;
+        }
+        if (u.a.x & 16) {
+            bwd_group_mem(arena, c, u, w);
+            return;
+        }
+    }
+    bwd_small(u, w);
 }
 
 // CaConcStruct::process (hsolve/HSolveStruct.cpp:138-155)
@@ -720,7 +855,7 @@ __device__ __forceinline__ void tick_outputs(const LaunchArgs& a, const CellDev&
 // arrays and the level table live in LDS (else in the cell's HBM work area);
 // a compile-time choice so that every access is a plain LDS or global
 // instruction rather than a generic (flat) one.
-template <typename GT, bool SPLIT, bool WL>
+template <typename GT, bool SPLIT, bool WL, bool SLOWG>
 __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c, double* lds, ChanType* sTypes) {
     char* arena = a.arena;
     const View<GT> v = make_view<GT>(arena, c, a.tsets);
@@ -840,14 +975,14 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
             for (int l = 0; l < nLevels; l += 2) {
                 nxt = load_desc(desc, min(s1 + tid, nUnits - 1));
                 const int s2 = l + 2 <= nLevels ? level[l + 2] : s1;
-                if (s0 + tid < s1) forward_unit(arena, c, cur, w);
-                for (int u = s0 + tid + nt; u < s1; u += nt) forward_unit(arena, c, load_desc(desc, u), w);
+                if (s0 + tid < s1) forward_unit<SLOWG>(arena, c, cur, w);
+                for (int u = s0 + tid + nt; u < s1; u += nt) forward_unit<SLOWG>(arena, c, load_desc(desc, u), w);
                 work_barrier(WL);
                 if (l + 1 >= nLevels) break;
                 cur = load_desc(desc, min(s2 + tid, nUnits - 1));
                 const int s3 = l + 3 <= nLevels ? level[l + 3] : s2;
-                if (s1 + tid < s2) forward_unit(arena, c, nxt, w);
-                for (int u = s1 + tid + nt; u < s2; u += nt) forward_unit(arena, c, load_desc(desc, u), w);
+                if (s1 + tid < s2) forward_unit<SLOWG>(arena, c, nxt, w);
+                for (int u = s1 + tid + nt; u < s2; u += nt) forward_unit<SLOWG>(arena, c, load_desc(desc, u), w);
                 work_barrier(WL);
                 s0 = s2;
                 s1 = s3;
@@ -867,15 +1002,15 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
                 // level l from cur, level l-1 (units p0..s0) into nxt
                 nxt = load_desc(desc, min(p0 + tid, nUnits - 1));
                 const int pp = l >= 2 ? level[l - 2] : 0;
-                if (s0 + tid < s1) backward_unit(arena, c, cur, w);
-                for (int u = s0 + tid + nt; u < s1; u += nt) backward_unit(arena, c, load_desc(desc, u), w);
+                if (s0 + tid < s1) backward_unit<SLOWG>(arena, c, cur, w);
+                for (int u = s0 + tid + nt; u < s1; u += nt) backward_unit<SLOWG>(arena, c, load_desc(desc, u), w);
                 work_barrier(WL);
                 if (l == 0) break;
                 // level l-1 from nxt, level l-2 (units pp..p0) into cur
                 cur = load_desc(desc, min(pp + tid, nUnits - 1));
                 const int ppp = l >= 3 ? level[l - 3] : 0;
-                if (p0 + tid < s0) backward_unit(arena, c, nxt, w);
-                for (int u = p0 + tid + nt; u < s0; u += nt) backward_unit(arena, c, load_desc(desc, u), w);
+                if (p0 + tid < s0) backward_unit<SLOWG>(arena, c, nxt, w);
+                for (int u = p0 + tid + nt; u < s0; u += nt) backward_unit<SLOWG>(arena, c, load_desc(desc, u), w);
                 work_barrier(WL);
                 s1 = p0;
                 s0 = pp;
@@ -898,23 +1033,25 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
     if (tid == 0) *stepCounter = step0 + a.nsteps;
 }
 
-template <typename GT>
+// SLOWG: the cells of the launch have junction groups of four or more
+// members (see fwd_group_regs).
+template <typename GT, bool SLOWG>
 __global__ void __launch_bounds__(256) hsolve_step_kernel(LaunchArgs a) {
     extern __shared__ double lds[];
     __shared__ ChanType sTypes[kMaxSlots];
     const CellDev& c = a.cells[a.order[blockIdx.x]];
-    if (c.workInLds) cell_steps<GT, false, true>(a, c, lds, sTypes);
-    else cell_steps<GT, false, false>(a, c, lds, sTypes);
+    if (c.workInLds) cell_steps<GT, false, true, SLOWG>(a, c, lds, sTypes);
+    else cell_steps<GT, false, false, SLOWG>(a, c, lds, sTypes);
 }
 
 // The solve half of a tick when the channel phase runs as its own kernel.
-template <typename GT>
+template <typename GT, bool SLOWG>
 __global__ void __launch_bounds__(256) hsolve_solve_kernel(LaunchArgs a) {
     extern __shared__ double lds[];
     __shared__ ChanType sTypes[kMaxSlots];
     const CellDev& c = a.cells[a.order[blockIdx.x]];
-    if (c.workInLds) cell_steps<GT, true, true>(a, c, lds, sTypes);
-    else cell_steps<GT, true, false>(a, c, lds, sTypes);
+    if (c.workInLds) cell_steps<GT, true, true, SLOWG>(a, c, lds, sTypes);
+    else cell_steps<GT, true, false, SLOWG>(a, c, lds, sTypes);
 }
 
 // Phase A as a kernel of its own: advanceChannels, calculateChannelCurrents
@@ -1262,16 +1399,18 @@ __global__ void __launch_bounds__(256) hsolve_reinit_kernel(LaunchArgs a) {
         for (int i = threadIdx.x; i < v.n; i += blockDim.x) channel_row<GT, false>(v, types, i, (GT)a.dt, w, true, i);
 }
 
-// The HHChannel2D objects of the cells in `order` for one tick, one
-// workgroup per cell, one compartment per thread; the tick's step (or
-// channel) kernel then adds o2dExt to the rows.
-__global__ void __launch_bounds__(256) hsolve_chan2d_kernel(LaunchArgs a) {
-    const CellDev& c = a.cells[a.order[blockIdx.x]];
-    if (c.n2d == 0) return;
+// The HHChannel2D objects of one tick over a bin's tiles of kChannelTile
+// consecutive compartments of one cell (as the channel kernel), one
+// compartment per thread; the tick's step (or channel) kernel then adds
+// o2dExt to the rows.
+__global__ void __launch_bounds__(256) hsolve_chan2d_kernel(LaunchArgs a, const int2* __restrict__ tiles) {
+    const int2 tile = tiles[blockIdx.x];
+    const CellDev& c = a.cells[tile.x];
+    const int i = tile.y + threadIdx.x;
+    if (c.n2d == 0 || i >= c.n) return;
     const double* V = A_<const double>(a.arena, c.oV);
     const double* poolCa = A_<const double>(a.arena, c.oPoolCa);
-    for (int i = threadIdx.x; i < c.n; i += blockDim.x)
-        chan2d_external(a.arena, a.tabs2d, a.dt, a.firstOfCall != 0, c, i, V[i], poolCa);
+    chan2d_external(a.arena, a.tabs2d, a.dt, a.firstOfCall != 0, c, i, V[i], poolCa);
 }
 
 // HHChannel2D::vReinit of the cells in `order`, launched before the
@@ -1284,25 +1423,23 @@ __global__ void __launch_bounds__(256) hsolve_chan2d_reinit_kernel(LaunchArgs a)
 }  // namespace hsg
 
 // host-side launchers (called from hsolve_engine.cpp)
-extern "C" hipError_t hsg_launch_step(const hsg::LaunchArgs* a, int nblocks, int threads, int fp32,
+extern "C" hipError_t hsg_launch_step(const hsg::LaunchArgs* a, int nblocks, int threads, int fp32, int slow,
                                      hipStream_t stream) {
-    if (fp32)
-        hipLaunchKernelGGL(hsg::hsolve_step_kernel<float>, dim3(nblocks), dim3(threads),
-                           a->ldsBytes, stream, *a);
-    else
-        hipLaunchKernelGGL(hsg::hsolve_step_kernel<double>, dim3(nblocks), dim3(threads),
-                           a->ldsBytes, stream, *a);
+    const dim3 g(nblocks), b(threads);
+    if (fp32 && slow) hipLaunchKernelGGL((hsg::hsolve_step_kernel<float, true>), g, b, a->ldsBytes, stream, *a);
+    else if (fp32) hipLaunchKernelGGL((hsg::hsolve_step_kernel<float, false>), g, b, a->ldsBytes, stream, *a);
+    else if (slow) hipLaunchKernelGGL((hsg::hsolve_step_kernel<double, true>), g, b, a->ldsBytes, stream, *a);
+    else hipLaunchKernelGGL((hsg::hsolve_step_kernel<double, false>), g, b, a->ldsBytes, stream, *a);
     return hipGetLastError();
 }
 
-extern "C" hipError_t hsg_launch_solve(const hsg::LaunchArgs* a, int nblocks, int threads, int fp32,
+extern "C" hipError_t hsg_launch_solve(const hsg::LaunchArgs* a, int nblocks, int threads, int fp32, int slow,
                                       hipStream_t stream) {
-    if (fp32)
-        hipLaunchKernelGGL(hsg::hsolve_solve_kernel<float>, dim3(nblocks), dim3(threads),
-                           a->ldsBytes, stream, *a);
-    else
-        hipLaunchKernelGGL(hsg::hsolve_solve_kernel<double>, dim3(nblocks), dim3(threads),
-                           a->ldsBytes, stream, *a);
+    const dim3 g(nblocks), b(threads);
+    if (fp32 && slow) hipLaunchKernelGGL((hsg::hsolve_solve_kernel<float, true>), g, b, a->ldsBytes, stream, *a);
+    else if (fp32) hipLaunchKernelGGL((hsg::hsolve_solve_kernel<float, false>), g, b, a->ldsBytes, stream, *a);
+    else if (slow) hipLaunchKernelGGL((hsg::hsolve_solve_kernel<double, true>), g, b, a->ldsBytes, stream, *a);
+    else hipLaunchKernelGGL((hsg::hsolve_solve_kernel<double, false>), g, b, a->ldsBytes, stream, *a);
     return hipGetLastError();
 }
 
@@ -1335,18 +1472,23 @@ extern "C" hipError_t hsg_launch_reinit(const hsg::LaunchArgs* a, int nblocks, i
     return hipGetLastError();
 }
 
-extern "C" hipError_t hsg_launch_chan2d(const hsg::LaunchArgs* a, int nblocks, int reinit, hipStream_t stream) {
-    if (reinit)
-        hipLaunchKernelGGL(hsg::hsolve_chan2d_reinit_kernel, dim3(nblocks), dim3(256), 0, stream, *a);
-    else
-        hipLaunchKernelGGL(hsg::hsolve_chan2d_kernel, dim3(nblocks), dim3(256), 0, stream, *a);
+extern "C" hipError_t hsg_launch_chan2d(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, hipStream_t stream) {
+    hipLaunchKernelGGL(hsg::hsolve_chan2d_kernel, dim3(ntiles), dim3(hsg::kChannelTile), 0, stream, *a, tiles);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t hsg_launch_chan2d_reinit(const hsg::LaunchArgs* a, int nblocks, hipStream_t stream) {
+    hipLaunchKernelGGL(hsg::hsolve_chan2d_reinit_kernel, dim3(nblocks), dim3(256), 0, stream, *a);
     return hipGetLastError();
 }
 
 extern "C" hipError_t hsg_set_max_lds(int bytes) {
-    const void* kernels[6] = {(const void*)hsg::hsolve_step_kernel<double>, (const void*)hsg::hsolve_step_kernel<float>,
-                              (const void*)hsg::hsolve_solve_kernel<double>, (const void*)hsg::hsolve_solve_kernel<float>,
-                              (const void*)hsg::hsolve_msg_kernel<double>, (const void*)hsg::hsolve_msg_kernel<float>};
+    const void* kernels[10] = {
+        (const void*)hsg::hsolve_step_kernel<double, false>, (const void*)hsg::hsolve_step_kernel<float, false>,
+        (const void*)hsg::hsolve_step_kernel<double, true>, (const void*)hsg::hsolve_step_kernel<float, true>,
+        (const void*)hsg::hsolve_solve_kernel<double, false>, (const void*)hsg::hsolve_solve_kernel<float, false>,
+        (const void*)hsg::hsolve_solve_kernel<double, true>, (const void*)hsg::hsolve_solve_kernel<float, true>,
+        (const void*)hsg::hsolve_msg_kernel<double>, (const void*)hsg::hsolve_msg_kernel<float>};
     for (const void* k : kernels) {
         hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
         if (e != hipSuccess) return e;

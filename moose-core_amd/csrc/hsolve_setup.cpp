@@ -298,6 +298,29 @@ static int schedule(CellSetup& s, const std::vector<int>& junctionGroup, std::st
             const int sz = (int)G.mem.size();
             if (is_slow(code)) {
                 s.anySlowGroup = true;
+                // four or five members: everything the register form needs in
+                // the descriptor (members, junction-coefficient offset, the
+                // deferred term slots of members 1..s-2)
+                bool inDesc = sz <= kDescGroupMax;
+                int32_t dinfo[3] = {-1, -1, -1};
+                for (int p = 1; inDesc && p + 1 < sz; ++p) {
+                    const int ds = s.defer[2 * G.mem[p]], cnt = s.defer[2 * G.mem[p] + 1];
+                    if (ds < 0) continue;
+                    if (ds >= (1 << 24) || cnt > 127) inDesc = false;
+                    else dinfo[p - 1] = ds | (cnt << 24);
+                }
+                if (inDesc) {
+                    d[0] = sz | 32;
+                    d[1] = G.deferLast;
+                    d[2] = G.hjOff;
+                    d[3] = (G.mem[0] & 0xffff) | (G.mem[1] << 16);
+                    d[4] = (G.mem[2] & 0xffff) | (G.mem[3] << 16);
+                    d[5] = sz == 5 ? G.mem[4] : 0;
+                    d[6] = dinfo[0];
+                    d[7] = dinfo[1];
+                    d[8] = dinfo[2];
+                    continue;
+                }
                 d[0] = sz | 16;
                 d[1] = g;
                 continue;
