@@ -191,7 +191,7 @@ def main():
     ap.add_argument("--steps-per-launch", type=int, default=None)
     ap.add_argument("--graph-chunk", type=int, default=None)
     ap.add_argument("--threads", type=int, default=0)
-    ap.add_argument("--kernel-mode", default="fused", choices=["fused", "split", "auto"])
+    ap.add_argument("--kernel-mode", default="auto", choices=["fused", "split", "auto"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="mixed,fp64")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",

@@ -179,8 +179,8 @@ typedef struct hsg_options {
 } hsg_options;
 
 /* kernel_mode values */
-#define HSG_KERNEL_AUTO   0   /* engine's choice: SPLIT for cells of 4096+ compartments,
-                                 FUSED for the rest                                        */
+#define HSG_KERNEL_AUTO   0   /* engine's choice per cell: SPLIT for cells of 4096+
+                                 compartments, FUSED for the rest                          */
 #define HSG_KERNEL_FUSED  1   /* one workgroup per cell runs whole ticks, many per launch  */
 #define HSG_KERNEL_SPLIT  2   /* per tick: a batch-wide channel kernel, then the per-cell
                                  solve kernel (Hines elimination, Vm, Ca, spikes)          */

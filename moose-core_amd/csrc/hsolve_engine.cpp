@@ -22,7 +22,7 @@
 #include "hsolve_setup.h"
 
 extern "C" hipError_t hsg_launch_step(const hsg::LaunchArgs* a, int nblocks, int threads, int fp32, int slow,
-                                     hipStream_t stream);
+                                     int fastms, hipStream_t stream);
 extern "C" hipError_t hsg_launch_reinit(const hsg::LaunchArgs* a, int nblocks, int fp32,
                                        hipStream_t stream);
 extern "C" hipError_t hsg_set_max_lds(int bytes);
@@ -31,6 +31,8 @@ extern "C" hipError_t hsg_launch_solve(const hsg::LaunchArgs* a, int nblocks, in
 extern "C" hipError_t hsg_launch_channel(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, int fp32,
                                         hipStream_t stream);
 extern "C" hipError_t hsg_launch_chan2d(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, hipStream_t stream);
+extern "C" hipError_t hsg_launch_channel_fast(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, int fp32,
+                                             int ms, hipStream_t stream);
 extern "C" hipError_t hsg_launch_chan2d_reinit(const hsg::LaunchArgs* a, int nblocks, hipStream_t stream);
 extern "C" hipError_t hsg_launch_msg(const hsg::LaunchArgs* a, int nblocks, int threads, int fp32,
                                     hipStream_t stream);
@@ -176,12 +178,14 @@ struct hsg_engine {
         bool has2d = false;         // cells with HHChannel2D objects: launched tick by tick
         bool split = false;         // channel kernel + solve kernel per tick
         bool slow = false;          // cells with junction groups of four or more members
+        int chanMS = 0;             // channel kernel's fast pass for up to chanMS slots (0: generic)
+        int64_t sumN = 0;           // compartments in the bin
         int lds = 0;
         int threads = 256;
     };
     std::vector<Bin> bins;
     bool split = false;          // channel phase as its own kernel
-    static constexpr int kMaxBins = 16;
+    static constexpr int kMaxBins = 32;
     hipStream_t binStream[kMaxBins] = {};
     hipEvent_t evFork = nullptr, evJoin[kMaxBins] = {};
     // graph of the steady-state launches
@@ -357,6 +361,8 @@ int64_t work_lds_bytes(const CellDev& c) {
 int64_t level_bytes(const CellDev& c) { return 4 * ((int64_t)c.nLevels + 1); }
 
 // Lays out and uploads one cell's main region.
+void fill_fast_slots(hsg_engine* e, CellDev& d);
+
 int place_cell(hsg_engine* e, HostCell& hc) {
     CellSetup& s = hc.s;
     const int n = s.n;
@@ -452,6 +458,7 @@ int place_cell(hsg_engine* e, HostCell& hc) {
     d.uniform = s.uniform ? 1 : 0;
     for (int k = 0; k < kMaxSlots; ++k)
         d.slotType[k] = (k < s.chanSlots && s.uniform) ? chType[(int64_t)k * n] : -1;
+    fill_fast_slots(e, d);
 
     // upload: the whole region in one copy (zero-filled, so zero arrays need
     // no staging)
@@ -725,6 +732,35 @@ int upload_tables(hsg_engine* e) {
     return 0;
 }
 
+// Packs the slot types of a uniform cell for the fast channel pass:
+// gate-state slots numbered in slot order, X, Y, Z.
+void fill_fast_slots(hsg_engine* e, CellDev& d) {
+    std::memset(d.fast, 0, sizeof d.fast);
+    d.fastOk = d.uniform && d.nChanSlots > 0;
+    int js = 0;
+    for (int k = 0; k < kMaxSlots && d.fastOk; ++k) {
+        const int t = d.slotType[k];
+        if (t < 0) continue;
+        const ChanType& ct = e->types[t];
+        const double pw[3] = {ct.Xpower, ct.Ypower, ct.Zpower};
+        const int col[3] = {ct.colX, ct.colY, ct.colZ};
+        int code[3] = {0, 0, 0}, gidx[3] = {0, 0, 0}, cl[3] = {0, 0, 0};
+        for (int ax = 0; ax < 3; ++ax) {
+            if (!(pw[ax] > 0.0)) continue;
+            if (pw[ax] != 1.0 && pw[ax] != 2.0 && pw[ax] != 3.0 && pw[ax] != 4.0) d.fastOk = 0;
+            code[ax] = (int)pw[ax];
+            cl[ax] = col[ax];
+            gidx[ax] = js++;
+        }
+        if (js > 31 || cl[0] > 0xffff || cl[1] > 0xffff || cl[2] > 0xffff || ct.caDepLocal > 14) d.fastOk = 0;
+        d.fast[k][0] = cl[0] | (cl[1] << 16);
+        d.fast[k][1] = (int32_t)((uint32_t)cl[2] | (uint32_t)gidx[0] << 16 | (uint32_t)gidx[1] << 21 |
+                                 (uint32_t)gidx[2] << 26 | 1u << 31);
+        d.fast[k][2] = code[0] | code[1] << 3 | code[2] << 6 | (ct.instant & 7) << 9 | (ct.caDepLocal + 1) << 12;
+    }
+    if (!d.fastOk) std::memset(d.fast, 0, sizeof d.fast);
+}
+
 // Threads per cell workgroup for a bin whose largest cell has n compartments.
 int default_threads(int n) { return n >= 1024 ? 256 : n >= 256 ? 128 : 64; }
 
@@ -746,9 +782,12 @@ int make_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg_
     // cells with junction groups of four or more members (slowHJ) in bins of
     // their own too (8-15), launched with the kernels that carry the register
     // form of those groups
+    // and, in automatic mode, large cells without the message-passing form
+    // in split bins of their own (16-31)
+    const bool autoMode = e->opt.kernel_mode == HSG_KERNEL_AUTO;
     auto cls = [&](int id) {
         const CellDev& d = e->cells[id]->dev;
-        const int s = d.slowHJ ? 8 : 0;
+        const int s = (d.slowHJ ? 8 : 0) + (autoMode && !d.msg && d.n >= kAutoSplitMin ? 16 : 0);
         if (d.n2d > 0) return s + (d.msg ? 6 : 7);
         if (d.msg) return s + (d.n >= 1024 ? 0 : d.n >= 256 ? 1 : 2);
         return s + 3 + (d.ldsBytes > 32768 ? 0 : d.ldsBytes > 16384 ? 1 : 2);
@@ -770,17 +809,24 @@ int make_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg_
         curCls = k;
         cur.msg = (k & 7) < 3 || (k & 7) == 6;
         cur.has2d = (k & 7) >= 6;
-        cur.slow = k >= 8;
-        cur.split = e->split || cur.msg;
+        cur.slow = (k & 8) != 0;
+        cur.split = e->split || cur.msg || (k & 16) != 0;
         cur.cells.push_back(id);
     }
     if (!cur.cells.empty()) bins.push_back(cur);
     for (auto& b : bins) {
-        int maxN = 0;
+        int maxN = 0, maxSlots = 0;
+        bool allUniform = true;
         for (int id : b.cells) {
-            b.lds = std::max<int>(b.lds, e->cells[id]->dev.ldsBytes);
-            maxN = std::max(maxN, e->cells[id]->dev.n);
+            const CellDev& d = e->cells[id]->dev;
+            b.lds = std::max<int>(b.lds, d.ldsBytes);
+            maxN = std::max(maxN, d.n);
+            maxSlots = std::max(maxSlots, d.nChanSlots);
+            allUniform = allUniform && d.fastOk;
         }
+        b.chanMS = allUniform ? (maxSlots <= 2 ? 2 : maxSlots <= 4 ? 4 : 8) : 0;
+        b.sumN = 0;
+        for (int id : b.cells) b.sumN += e->cells[id]->dev.n;
         b.threads = e->opt.block_threads > 0 ? std::max(64, std::min(1024, (e->opt.block_threads + 63) / 64 * 64))
                                              : default_threads(maxN);
         HIPCHK(hipMalloc(&b.dOrder, sizeof(int32_t) * b.cells.size()));
@@ -863,6 +909,18 @@ void consume_inputs(hsg_engine* e, HostCell& hc, int64_t nsteps) {
     hc.stepsDone += nsteps;
 }
 
+// The fused kernel's fast channel pass (two-slot uniform cells) holds more
+// registers, so fewer workgroups share a CU: it pays off where each thread
+// has many compartments to sweep (measured crossover on MI355X).
+#ifndef HSG_FAST_FUSED_MIN
+#define HSG_FAST_FUSED_MIN 6
+#endif
+int fused_fast(const hsg_engine::Bin& b) {
+    if (b.chanMS != 2 || b.slow || b.cells.empty()) return 0;
+    const double perThread = (double)b.sumN / (double)b.cells.size() / (double)b.threads;
+    return perThread >= HSG_FAST_FUSED_MIN ? 2 : 0;
+}
+
 // One round: every bin launched for `chunk` steps on its own stream, forked
 // from and joined back into the engine stream.
 int launch_round(hsg_engine* e, std::vector<hsg_engine::Bin>& bins, int chunk, int first) {
@@ -881,17 +939,20 @@ int launch_round(hsg_engine* e, std::vector<hsg_engine::Bin>& bins, int chunk, i
                 a.ldsBytes = bins[b].lds;
                 if (bins[b].has2d) HIPCHK(hsg_launch_chan2d(&a, bins[b].dTiles, bins[b].nTiles, st));
                 if (!split) {
-                    HIPCHK(hsg_launch_step(&a, nb, bins[b].threads, e->fp32, bins[b].slow, st));
+                    HIPCHK(hsg_launch_step(&a, nb, bins[b].threads, e->fp32, bins[b].slow, fused_fast(bins[b]), st));
                     continue;
                 }
-                HIPCHK(hsg_launch_channel(&a, bins[b].dTiles, bins[b].nTiles, e->fp32, st));
+                if (bins[b].chanMS > 0)
+                    HIPCHK(hsg_launch_channel_fast(&a, bins[b].dTiles, bins[b].nTiles, e->fp32, bins[b].chanMS, st));
+                else
+                    HIPCHK(hsg_launch_channel(&a, bins[b].dTiles, bins[b].nTiles, e->fp32, st));
                 if (bins[b].msg) HIPCHK(hsg_launch_msg(&a, nb, bins[b].threads, e->fp32, st));
                 else HIPCHK(hsg_launch_solve(&a, nb, bins[b].threads, e->fp32, bins[b].slow, st));
             }
         } else {
             LaunchArgs a = make_args(e, bins[b].dOrder, chunk, first);
             a.ldsBytes = bins[b].lds;
-            HIPCHK(hsg_launch_step(&a, nb, bins[b].threads, e->fp32, bins[b].slow, st));
+            HIPCHK(hsg_launch_step(&a, nb, bins[b].threads, e->fp32, bins[b].slow, fused_fast(bins[b]), st));
         }
         HIPCHK(hipEventRecord(e->evJoin[b], st));
         HIPCHK(hipStreamWaitEvent(e->stream, e->evJoin[b], 0));
@@ -1457,7 +1518,10 @@ int hsg_set_field(hsg_engine* e, int32_t cell, int32_t field, const int32_t* idx
                 }
                 s.chTypeLocal[o] = nl;
                 if ((rc = put1<int32_t>(e, hc->dev.oChType, o, hc->typeIds[nl]))) return rc;
-                if (hc->dev.uniform) { hc->dev.uniform = 0; }
+                if (hc->dev.uniform) {
+                    hc->dev.uniform = 0;
+                    hc->dev.fastOk = 0;
+                }
                 e->tablesDirty = true;
                 break;
             }

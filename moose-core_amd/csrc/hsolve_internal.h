@@ -61,6 +61,16 @@ struct Chan2DDev {
 };
 
 // Per-cell descriptor (read by every workgroup working on the cell).
+// One channel slot of a uniform cell for the fast channel pass, packed in
+// three words (read with scalar loads):
+//   w0: column of X | column of Y << 16
+//   w1: column of Z | gate-state slot of X << 16 | of Y << 21 | of Z << 26 |
+//       live << 31   (column and slot 0 when the gate is absent)
+//   w2: power code of X | of Y << 3 | of Z << 6 (0: no gate, 1-4: integer
+//       power) | instant bits << 9 | (Z's local pool + 1) << 12
+// Only cells whose powers are all integers 1-4 (fastOk) take the fast pass.
+constexpr int kFastWords = 3;
+
 struct CellDev {
     int32_t n;              // compartments
     int32_t nChanSlots;     // channel slots per compartment (CHM)
@@ -68,6 +78,8 @@ struct CellDev {
     int32_t nPoolSlots;     // Ca-pool slots per compartment (PM)
     int32_t uniform;        // 1: every compartment has the channel types slotType[]
     int32_t slotType[kMaxSlots];
+    int32_t fastOk;                // uniform, integer powers 1-4: fast channel pass
+    int32_t fast[kMaxSlots][kFastWords];
     int32_t hasModulation;  // 1: multiply by per-channel modulation
     int32_t hasPools;
     int32_t caAdvance;

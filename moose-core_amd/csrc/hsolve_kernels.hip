@@ -745,24 +745,26 @@ __device__ __forceinline__ double caconc_process(double& cst, double basal, doub
 
 // advanceCalcium for the pools of compartment i (HSolveActive.cpp:153-216):
 // activation = iCa added between steps, then the channels feeding the pool
-// in channel order.
+// in channel order.  vmid: VMid of the compartment, vnew: its new Vm.  Every
+// load of a pool is issued before its stores, so they travel together.
 template <typename GT>
-__device__ void calcium_row(char* arena, const CellDev& c, const View<GT>& v, const ChanType* __restrict__ types,
-                            int i, Work w, bool firstOfCall) {
+__device__ __forceinline__ void calcium_row(char* arena, const CellDev& c, const View<GT>& v,
+                                            const ChanType* __restrict__ types, int i, double vmid, double vnew,
+                                            bool firstOfCall) {
     const int n = v.n;
     const int nPools = v.poolCount[i];
-    if (nPools == 0) return;
-    const double vmid = w.B[i];
-    const double vnew = v.V[i];
     double* ica = A_<double>(arena, c.oPoolICa);
     double* pc = A_<double>(arena, c.oPoolC);
     for (int q = 0; q < nPools; ++q) {
         const size_t o = (size_t)q * n + i;
+        double cst = pc[o];
+        const double basal = A_<const double>(arena, c.oPoolBasal)[o];
+        const double f1 = A_<const double>(arena, c.oPoolF1)[o];
+        const double f2 = A_<const double>(arena, c.oPoolF2)[o];
+        const double ceil = A_<const double>(arena, c.oPoolCeil)[o];
+        const double floor = A_<const double>(arena, c.oPoolFloor)[o];
         double act = 0.0;
-        if (firstOfCall) {
-            act = 0.0 + ica[o];
-            ica[o] = 0.0;
-        }
+        if (firstOfCall) act = 0.0 + ica[o];
         int js = 0;
         for (int k = 0; k < v.nChanSlots; ++k) {
             const int t = v.uniform ? k : v.chType[(size_t)k * n + i];
@@ -782,12 +784,9 @@ __device__ void calcium_row(char* arena, const CellDev& c, const View<GT>& v, co
             }
             js += ct.nGates;
         }
-        double cst = pc[o];
-        v.poolCa[o] = caconc_process(cst, A_<const double>(arena, c.oPoolBasal)[o],
-                                     A_<const double>(arena, c.oPoolF1)[o], A_<const double>(arena, c.oPoolF2)[o],
-                                     A_<const double>(arena, c.oPoolCeil)[o], A_<const double>(arena, c.oPoolFloor)[o],
-                                     act);
+        v.poolCa[o] = caconc_process(cst, basal, f1, f2, ceil, floor, act);
         pc[o] = cst;
+        if (firstOfCall) ica[o] = 0.0;
     }
 }
 
@@ -855,7 +854,130 @@ __device__ __forceinline__ void tick_outputs(const LaunchArgs& a, const CellDev&
 // arrays and the level table live in LDS (else in the cell's HBM work area);
 // a compile-time choice so that every access is a plain LDS or global
 // instruction rather than a generic (flat) one.
-template <typename GT, bool SPLIT, bool WL, bool SLOWG>
+// The channel pass of a uniform cell with integer powers (every compartment
+// has the channel types slotType[], at most MS of them, packed in
+// CellDev::fast and read with scalar loads, so every branch on them is
+// uniform), with every global load of the compartment issued before its
+// arithmetic: gate states, Gbar, Ek, modulation, Ca, V and the passive terms
+// first, then the table rows of all gates (no load waits on a store: the new
+// states and the row are stored last).  The arithmetic is channel_row's, in
+// the same order.
+template <typename GT>
+__device__ __forceinline__ GT power_code(GT x, int code) {
+    if (code == 1) return x;
+    if (code == 2) return x * x;
+    if (code == 3) return x * x * x;
+    GT y = x * x;
+    return y * y;
+}
+
+template <typename GT, int MS>
+__device__ __forceinline__ void channel_row_fast(const View<GT>& v, const CellDev& c, int i, GT dtg, double& Dout,
+                                                 double& Bout) {
+    const int n = v.n;
+    const int S = v.nChanSlots;
+    // ---- loads (absent entries read valid addresses and are not used)
+    const double V = v.V[i];
+    const double Dbase = v.Dbase[i], CmByDt = v.CmByDt[i], EmByRm = v.EmByRm[i];
+    const int nPools = v.hasPools ? v.poolCount[i] : 0;
+    const double* __restrict__ caSrc = v.hasPools ? v.poolCa : v.V;
+    const GT* __restrict__ stSrc = c.nGateSlots > 0 ? v.state : reinterpret_cast<const GT*>(v.V);
+    GT st[MS][3], gb[MS], ek[MS];
+    double md[MS], cav[MS], ext[MS];
+#pragma unroll
+    for (int k = 0; k < MS; ++k) {
+        const int w1 = c.fast[k][1], w2 = c.fast[k][2];
+        const size_t kk = (size_t)(k < S ? k : S - 1) * n + i;
+        st[k][0] = stSrc[(size_t)((w1 >> 16) & 31) * n + i];
+        st[k][1] = stSrc[(size_t)((w1 >> 21) & 31) * n + i];
+        st[k][2] = stSrc[(size_t)((w1 >> 26) & 31) * n + i];
+        gb[k] = v.Gbar[kk];
+        ek[k] = v.Ek[kk];
+        md[k] = v.hasModulation ? v.mod[kk] : 1.0;
+        // Ca inputs only where a Z gate may read them (uniform condition)
+        const bool hasZ = ((w2 >> 6) & 7) != 0;
+        ext[k] = hasZ ? v.extCa[kk] : 0.0;
+        const int caDep = ((w2 >> 12) & 15) - 1;
+        cav[k] = hasZ && caDep >= 0 ? caSrc[(size_t)(v.hasPools && caDep > 0 ? caDep : 0) * n + i] : 0.0;
+    }
+    int vBase;
+    double vFrac;
+    table_row(v.vt, V, vBase, vFrac);
+    const GT frac = (GT)vFrac;
+    // table entries of every gate: (A, B) at the row and the next row, loaded
+    // here and interpolated below, so the loads of all gates travel together
+    // (Z: the Ca row of its pool, the channel's external Ca, or V, as
+    // channel_row chooses)
+    GT raw[MS][3][4];
+    GT zf[MS];
+#pragma unroll
+    for (int k = 0; k < MS; ++k) {
+        const int w0 = c.fast[k][0], w1 = c.fast[k][1], w2 = c.fast[k][2];
+        if (k >= S || w1 >= 0) continue;          // empty slot (live bit 31 clear)
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            if (((w2 >> (3 * ax)) & 7) == 0) continue;
+            const int col = ax == 0 ? (w0 & 0xffff) : ax == 1 ? ((w0 >> 16) & 0xffff) : (w1 & 0xffff);
+            const GT* tab = v.vt.tab;
+            int base = vBase, nCol = v.vt.nCol;
+            if (ax == 2) {
+                zf[k] = frac;
+                const int caDep = ((w2 >> 12) & 15) - 1;
+                const bool fromPool = caDep >= 0 && caDep < nPools;
+                const bool fromExt = !fromPool && ext[k] > 0 && col + 1 < v.ct.nCol;
+                if (fromPool || fromExt) {
+                    double cf;
+                    table_row(v.ct, fromPool ? cav[k] : ext[k], base, cf);
+                    zf[k] = (GT)cf;
+                    tab = v.ct.tab;
+                    nCol = v.ct.nCol;
+                }
+            }
+            const GT* ap = tab + base + col;
+            raw[k][ax][0] = ap[0];
+            raw[k][ax][1] = ap[nCol];
+            raw[k][ax][2] = ap[1];
+            raw[k][ax][3] = ap[nCol + 1];
+        }
+    }
+    double GkSum = 0.0, GkEkSum = 0.0;
+#pragma unroll
+    for (int k = 0; k < MS; ++k) {
+        const int w1 = c.fast[k][1], w2 = c.fast[k][2];
+        if (k >= S || w1 >= 0) continue;
+        GT fraction = v.hasModulation ? (GT)md[k] : (GT)1.0;
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            const int code = (w2 >> (3 * ax)) & 7;
+            if (code == 0) continue;
+            // LookupTable::lookup (RateLookup.cpp:78-103)
+            const GT fr = ax == 2 ? zf[k] : frac;
+            const GT C1 = raw[k][ax][0] + (raw[k][ax][1] - raw[k][ax][0]) * fr;
+            const GT C2 = raw[k][ax][2] + (raw[k][ax][3] - raw[k][ax][2]) * fr;
+            const GT x = gate_step<GT>(st[k][ax], C1, C2, dtg, (w2 >> (9 + ax)) & 1);
+            st[k][ax] = x;
+            fraction *= power_code<GT>(x, code);
+        }
+        const GT Gk = gb[k] * fraction;
+        GkSum += (double)Gk;
+        GkEkSum += (double)Gk * (double)ek[k];
+    }
+    // ---- stores
+#pragma unroll
+    for (int k = 0; k < MS; ++k) {
+        const int w1 = c.fast[k][1], w2 = c.fast[k][2];
+        if (k >= S || w1 >= 0) continue;
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax)
+            if (((w2 >> (3 * ax)) & 7) != 0) v.state[(size_t)((w1 >> (16 + 5 * ax)) & 31) * n + i] = st[k][ax];
+    }
+    Dout = Dbase + GkSum;
+    Bout = V * CmByDt + EmByRm + GkEkSum;
+}
+
+// FASTMS > 0: the cells are uniform with at most FASTMS channel slots and
+// phase A uses channel_row_fast.
+template <typename GT, bool SPLIT, bool WL, bool SLOWG, int FASTMS>
 __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c, double* lds, ChanType* sTypes) {
     char* arena = a.arena;
     const View<GT> v = make_view<GT>(arena, c, a.tsets);
@@ -917,13 +1039,17 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
                     w.D[i] = gwork[i];
                     w.B[i] = gwork[n + i];
                 }
+        } else if constexpr (FASTMS > 0) {
+#pragma unroll 1
+            for (int i = tid; i < n; i += nt) channel_row_fast<GT, FASTMS>(v, c, i, dtg, w.D[i], w.B[i]);
         } else {
             if (uniform)
                 for (int i = tid; i < n; i += nt) channel_row<GT, true>(v, types, i, dtg, w, false, i);
             else
                 for (int i = tid; i < n; i += nt) channel_row<GT, false>(v, types, i, dtg, w, false, i);
         }
-        if (nInj > 0 || (first && nExt > 0) || c.n2d > 0) {
+        // (split mode: the channel kernel has added them to the rows)
+        if (!SPLIT && (nInj > 0 || (first && nExt > 0) || c.n2d > 0)) {
             work_barrier(WL);
             const int32_t* start2d = A_<const int32_t>(arena, c.o2dStart);
             if (tid == 0) {
@@ -1018,9 +1144,15 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
             }
         }
         // ---- D: Vm, calcium
-        for (int i = tid; i < n; i += nt) v.V[i] = 2 * w.B[i] - v.V[i];
         if (v.hasPools)
-            for (int i = tid; i < n; i += nt) calcium_row<GT>(arena, c, v, types, i, w, first);
+            for (int i = tid; i < n; i += nt) {
+                const double vmid = w.B[i];
+                const double vnew = 2 * vmid - v.V[i];
+                v.V[i] = vnew;
+                calcium_row<GT>(arena, c, v, types, i, vmid, vnew, first);
+            }
+        else
+            for (int i = tid; i < n; i += nt) v.V[i] = 2 * w.B[i] - v.V[i];
         const int64_t stepNo = step0 + k + 1;
         if (nSpike > 0 || nProbe > 0) {
             __syncthreads();   // V written by other threads
@@ -1040,8 +1172,21 @@ __global__ void __launch_bounds__(256) hsolve_step_kernel(LaunchArgs a) {
     extern __shared__ double lds[];
     __shared__ ChanType sTypes[kMaxSlots];
     const CellDev& c = a.cells[a.order[blockIdx.x]];
-    if (c.workInLds) cell_steps<GT, false, true, SLOWG>(a, c, lds, sTypes);
-    else cell_steps<GT, false, false, SLOWG>(a, c, lds, sTypes);
+    if (c.workInLds) cell_steps<GT, false, true, SLOWG, 0>(a, c, lds, sTypes);
+    else cell_steps<GT, false, false, SLOWG, 0>(a, c, lds, sTypes);
+}
+
+// Uniform two-slot cells with many compartments per thread: phase A
+// through channel_row_fast (more registers, fewer workgroups per CU, the
+// channel loads of a compartment in two round trips instead of one per
+// gate).
+template <typename GT>
+__global__ void __launch_bounds__(256) hsolve_step_fast_kernel(LaunchArgs a) {
+    extern __shared__ double lds[];
+    __shared__ ChanType sTypes[kMaxSlots];
+    const CellDev& c = a.cells[a.order[blockIdx.x]];
+    if (c.workInLds) cell_steps<GT, false, true, false, 2>(a, c, lds, sTypes);
+    else cell_steps<GT, false, false, false, 2>(a, c, lds, sTypes);
 }
 
 // The solve half of a tick when the channel phase runs as its own kernel.
@@ -1050,8 +1195,47 @@ __global__ void __launch_bounds__(256) hsolve_solve_kernel(LaunchArgs a) {
     extern __shared__ double lds[];
     __shared__ ChanType sTypes[kMaxSlots];
     const CellDev& c = a.cells[a.order[blockIdx.x]];
-    if (c.workInLds) cell_steps<GT, true, true, SLOWG>(a, c, lds, sTypes);
-    else cell_steps<GT, true, false, SLOWG>(a, c, lds, sTypes);
+    if (c.workInLds) cell_steps<GT, true, true, SLOWG, 0>(a, c, lds, sTypes);
+    else cell_steps<GT, true, false, SLOWG, 0>(a, c, lds, sTypes);
+}
+
+// Rows in HBM (split mode): the row's destination for compartment i.
+__device__ __forceinline__ void row_dest(char* arena, const CellDev& c, int i, double*& D, double*& B) {
+    if (c.msg) {
+        const int dst = A_<const int32_t>(arena, c.oRowPos)[i];
+        D = A_<double>(arena, c.oDA) + dst;
+        B = A_<double>(arena, c.oBA) + dst;
+    } else {
+        double* gwork = A_<double>(arena, c.oWork);
+        D = gwork + i;
+        B = gwork + c.n + i;
+    }
+}
+
+// The rest of HSolveActive::updateMatrix for compartment i's row in split
+// mode: the injection, then the external current (HSolveActive.cpp:117-148).
+__device__ __forceinline__ void row_inputs(const LaunchArgs& a, const CellDev& c, int i, double& D, double& B) {
+    const bool has2d = c.n2d > 0 && A_<const int32_t>(a.arena, c.o2dStart)[i] < A_<const int32_t>(a.arena, c.o2dStart)[i + 1];
+    const int32_t* ii = A_<const int32_t>(a.arena, c.oInjIdx);
+    for (int e = 0; e < c.nInj; ++e)
+        if (ii[e] == i) {
+            double* iv = A_<double>(a.arena, c.oInjVar);
+            const double var = a.firstOfCall ? iv[e] : 0.0;
+            B += var + A_<const double>(a.arena, c.oInjBasal)[e];
+            if (a.firstOfCall) iv[e] = 0.0;
+        }
+    if (has2d) {
+        const double* ext = A_<const double>(a.arena, c.o2dExt);
+        D += ext[2 * i];
+        B += ext[2 * i + 1];
+    } else if (a.firstOfCall) {
+        const int32_t* xi = A_<const int32_t>(a.arena, c.oExtIdx);
+        for (int e = 0; e < c.nExt; ++e)
+            if (xi[e] == i) {
+                D += A_<const double>(a.arena, c.oExtG)[e];
+                B += A_<const double>(a.arena, c.oExtGE)[e];
+            }
+    }
 }
 
 // Phase A as a kernel of its own: advanceChannels, calculateChannelCurrents
@@ -1059,11 +1243,11 @@ __global__ void __launch_bounds__(256) hsolve_solve_kernel(LaunchArgs a) {
 // tile of `blockDim.x` consecutive compartments of one cell per workgroup,
 // one compartment per thread, over the whole batch.  The rows go to the
 // cell's HBM work arrays (D at the start, B after it), where the solve
-// kernel of the same tick picks them up.  For cells on the message-passing
-// solve the rows go to their unit-order positions instead, and the thread
-// of a compartment also adds its injection and (first tick of a call)
-// external currents, in the order of HSolveActive::updateMatrix
-// (HSolveActive.cpp:117-148).
+// kernel of the same tick picks them up (for cells on the message-passing
+// solve, at their unit-order positions).  The thread of a compartment also
+// adds its injection and then its external current (host addGkEk entries on
+// the first tick of a call, or the HHChannel2D sum), in the order of
+// HSolveActive::updateMatrix (HSolveActive.cpp:117-148).
 template <typename GT>
 __global__ void __launch_bounds__(256) hsolve_channel_kernel(LaunchArgs a, const int2* __restrict__ tiles) {
     __shared__ ChanType sTypes[kMaxSlots];
@@ -1087,29 +1271,24 @@ __global__ void __launch_bounds__(256) hsolve_channel_kernel(LaunchArgs a, const
     }
     if (v.uniform) channel_row<GT, true>(v, types, i, (GT)a.dt, w, false, dst);
     else channel_row<GT, false>(v, types, i, (GT)a.dt, w, false, dst);
-    const bool has2d = c.n2d > 0 && A_<const int32_t>(a.arena, c.o2dStart)[i] < A_<const int32_t>(a.arena, c.o2dStart)[i + 1];
-    if (c.msg) {
-        const int32_t* ii = A_<const int32_t>(a.arena, c.oInjIdx);
-        for (int e = 0; e < c.nInj; ++e)
-            if (ii[e] == i) {
-                double* iv = A_<double>(a.arena, c.oInjVar);
-                const double var = a.firstOfCall ? iv[e] : 0.0;
-                w.B[dst] += var + A_<const double>(a.arena, c.oInjBasal)[e];
-                if (a.firstOfCall) iv[e] = 0.0;
-            }
-        if (has2d) {
-            const double* ext = A_<const double>(a.arena, c.o2dExt);
-            w.D[dst] += ext[2 * i];
-            w.B[dst] += ext[2 * i + 1];
-        } else if (a.firstOfCall) {
-            const int32_t* xi = A_<const int32_t>(a.arena, c.oExtIdx);
-            for (int e = 0; e < c.nExt; ++e)
-                if (xi[e] == i) {
-                    w.D[dst] += A_<const double>(a.arena, c.oExtG)[e];
-                    w.B[dst] += A_<const double>(a.arena, c.oExtGE)[e];
-                }
-        }
-    }
+    row_inputs(a, c, i, w.D[dst], w.B[dst]);
+}
+
+// The channel kernel for bins of uniform cells with at most MS slots.
+template <typename GT, int MS>
+__global__ void __launch_bounds__(256) hsolve_channel_fast_kernel(LaunchArgs a, const int2* __restrict__ tiles) {
+    const int2 tile = tiles[blockIdx.x];
+    const CellDev& c = a.cells[tile.x];
+    const View<GT> v = make_view<GT>(a.arena, c, a.tsets);
+    const int i = tile.y + threadIdx.x;
+    if (i >= v.n) return;
+    double D, B;
+    channel_row_fast<GT, MS>(v, c, i, (GT)a.dt, D, B);
+    row_inputs(a, c, i, D, B);
+    double *Dp, *Bp;
+    row_dest(a.arena, c, i, Dp, Bp);
+    *Dp = D;
+    *Bp = B;
 }
 
 // ---------------------------------------------------------------------------
@@ -1344,15 +1523,19 @@ __global__ void __launch_bounds__(256) hsolve_msg_kernel(LaunchArgs a) {
         }
         // VMid was written by other threads
         __syncthreads();
-        for (int i = tid; i < n; i += nt) V[i] = 2 * VMid[i] - V[i];
         const bool first = a.firstOfCall && k == 0;
         if (c.hasPools) {
             const View<GT> v = make_view<GT>(arena, c, a.tsets);
             const ChanType* types = stage_types(c, a.types, sTypes);
             __syncthreads();
-            Work w{};
-            w.B = VMid;
-            for (int i = tid; i < n; i += nt) calcium_row<GT>(arena, c, v, types, i, w, first);
+            for (int i = tid; i < n; i += nt) {
+                const double vmid = VMid[i];
+                const double vnew = 2 * vmid - V[i];
+                V[i] = vnew;
+                calcium_row<GT>(arena, c, v, types, i, vmid, vnew, first);
+            }
+        } else {
+            for (int i = tid; i < n; i += nt) V[i] = 2 * VMid[i] - V[i];
         }
         const int64_t stepNo = step0 + k + 1;
         if (c.nSpike > 0 || c.nProbe > 0) {
@@ -1424,12 +1607,16 @@ __global__ void __launch_bounds__(256) hsolve_chan2d_reinit_kernel(LaunchArgs a)
 
 // host-side launchers (called from hsolve_engine.cpp)
 extern "C" hipError_t hsg_launch_step(const hsg::LaunchArgs* a, int nblocks, int threads, int fp32, int slow,
-                                     hipStream_t stream) {
+                                     int fastms, hipStream_t stream) {
     const dim3 g(nblocks), b(threads);
-    if (fp32 && slow) hipLaunchKernelGGL((hsg::hsolve_step_kernel<float, true>), g, b, a->ldsBytes, stream, *a);
-    else if (fp32) hipLaunchKernelGGL((hsg::hsolve_step_kernel<float, false>), g, b, a->ldsBytes, stream, *a);
-    else if (slow) hipLaunchKernelGGL((hsg::hsolve_step_kernel<double, true>), g, b, a->ldsBytes, stream, *a);
-    else hipLaunchKernelGGL((hsg::hsolve_step_kernel<double, false>), g, b, a->ldsBytes, stream, *a);
+    const size_t l = a->ldsBytes;
+    if (fastms == 2 && !slow) {
+        if (fp32) hipLaunchKernelGGL((hsg::hsolve_step_fast_kernel<float>), g, b, l, stream, *a);
+        else hipLaunchKernelGGL((hsg::hsolve_step_fast_kernel<double>), g, b, l, stream, *a);
+    } else if (fp32 && slow) hipLaunchKernelGGL((hsg::hsolve_step_kernel<float, true>), g, b, l, stream, *a);
+    else if (fp32) hipLaunchKernelGGL((hsg::hsolve_step_kernel<float, false>), g, b, l, stream, *a);
+    else if (slow) hipLaunchKernelGGL((hsg::hsolve_step_kernel<double, true>), g, b, l, stream, *a);
+    else hipLaunchKernelGGL((hsg::hsolve_step_kernel<double, false>), g, b, l, stream, *a);
     return hipGetLastError();
 }
 
@@ -1472,6 +1659,21 @@ extern "C" hipError_t hsg_launch_reinit(const hsg::LaunchArgs* a, int nblocks, i
     return hipGetLastError();
 }
 
+extern "C" hipError_t hsg_launch_channel_fast(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, int fp32,
+                                             int ms, hipStream_t stream) {
+    const dim3 g(ntiles), b(hsg::kChannelTile);
+    if (fp32) {
+        if (ms <= 2) hipLaunchKernelGGL((hsg::hsolve_channel_fast_kernel<float, 2>), g, b, 0, stream, *a, tiles);
+        else if (ms <= 4) hipLaunchKernelGGL((hsg::hsolve_channel_fast_kernel<float, 4>), g, b, 0, stream, *a, tiles);
+        else hipLaunchKernelGGL((hsg::hsolve_channel_fast_kernel<float, 8>), g, b, 0, stream, *a, tiles);
+    } else {
+        if (ms <= 2) hipLaunchKernelGGL((hsg::hsolve_channel_fast_kernel<double, 2>), g, b, 0, stream, *a, tiles);
+        else if (ms <= 4) hipLaunchKernelGGL((hsg::hsolve_channel_fast_kernel<double, 4>), g, b, 0, stream, *a, tiles);
+        else hipLaunchKernelGGL((hsg::hsolve_channel_fast_kernel<double, 8>), g, b, 0, stream, *a, tiles);
+    }
+    return hipGetLastError();
+}
+
 extern "C" hipError_t hsg_launch_chan2d(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, hipStream_t stream) {
     hipLaunchKernelGGL(hsg::hsolve_chan2d_kernel, dim3(ntiles), dim3(hsg::kChannelTile), 0, stream, *a, tiles);
     return hipGetLastError();
@@ -1483,9 +1685,10 @@ extern "C" hipError_t hsg_launch_chan2d_reinit(const hsg::LaunchArgs* a, int nbl
 }
 
 extern "C" hipError_t hsg_set_max_lds(int bytes) {
-    const void* kernels[10] = {
+    const void* kernels[12] = {
         (const void*)hsg::hsolve_step_kernel<double, false>, (const void*)hsg::hsolve_step_kernel<float, false>,
         (const void*)hsg::hsolve_step_kernel<double, true>, (const void*)hsg::hsolve_step_kernel<float, true>,
+        (const void*)hsg::hsolve_step_fast_kernel<double>, (const void*)hsg::hsolve_step_fast_kernel<float>,
         (const void*)hsg::hsolve_solve_kernel<double, false>, (const void*)hsg::hsolve_solve_kernel<float, false>,
         (const void*)hsg::hsolve_solve_kernel<double, true>, (const void*)hsg::hsolve_solve_kernel<float, true>,
         (const void*)hsg::hsolve_msg_kernel<double>, (const void*)hsg::hsolve_msg_kernel<float>};

@@ -137,6 +137,30 @@ def test_large_cells_global_work(pkg, oracle_mod, mode):
         np.testing.assert_array_equal(gpu, ref)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_fused_fast_channel_pass(pkg, oracle_mod, precision):
+    """Two-slot uniform cells with many compartments per thread take the
+    fused kernel's fast channel pass (all loads of a compartment first)."""
+    print(f"case: HH cells 1800-2000 compartments, 600 steps, {precision}, fused fast pass", flush=True)
+    rng = np.random.default_rng(23)
+    specs = [workloads.random_hh_cell(rng, int(n), inject=float(rng.uniform(1.0e-9, 3.0e-9)))
+             for n in (1800, 1900, 1950, 2000)]
+    dt, n = 25e-6, 600
+    ids, traces, spikes = run_gpu(pkg, specs, dt, n, precision=precision, kernel_mode="fused", block_threads=256)
+    total = 0
+    for cid, spec, gpu in zip(ids, specs, traces):
+        _, ref, ref_spk = run_oracle(oracle_mod, spec, dt, n)
+        mine = list(spikes[spikes["cell"] == cid]["step"])
+        total += len(ref_spk)
+        if precision == "fp64":
+            np.testing.assert_array_equal(gpu, ref)
+            assert mine == list(ref_spk)
+        else:
+            assert np.abs(gpu - ref).max() <= MIXED_TOL
+            assert len(mine) == len(ref_spk) and all(abs(a - b) <= 1 for a, b in zip(mine, ref_spk))
+    assert total > 0
+
+
 @pytest.mark.parametrize("mode", ["fused", "split"])
 def test_fields_and_inputs(pkg, oracle_mod, mode):
     """Zombie field reads and writes, and the per-step inputs, between steps."""
