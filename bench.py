@@ -166,6 +166,51 @@ def cpu_baseline(cells, dt, budget_s=15.0):
     }
 
 
+def run_serving(args):
+    """Poisson arrivals of simulation requests into one engine (continuous
+    batching, admission by device-memory footprint): each request a seeded
+    random Ca cell with a 2-D K-AHP (the large-cell channel set) of n ~
+    log-U[100, 30000] compartments, for a duration log-U[10 ms, 1 s] at
+    dt = 25 us, mixed precision.  Prints one JSON line with the p50/p99
+    request latency (arrival to result) and the sustained throughput."""
+    from moose_core_amd import serving
+    dt = 25e-6
+    rng = np.random.default_rng(args.seed)
+    count = args.cells
+    t = time.time()
+    reqs, arrival = [], 0.0
+    for k in range(count):
+        arrival += rng.exponential(1.0 / args.rate)
+        n = int(round(np.exp(rng.uniform(np.log(100), np.log(30000)))))
+        steps = int(round(np.exp(rng.uniform(np.log(0.010 / dt), np.log(1.0 / dt)))))
+        cell = workloads.random_ca_cell(rng, n, kahp2d=True).pack()
+        reqs.append(serving.Request(k, cell, steps, arrival))
+    log(f"[bench] serving: {count} requests built in {time.time() - t:.1f}s "
+        f"({sum(r.spec.n for r in reqs)} compartments, {sum(r.nsteps for r in reqs)} request-steps)")
+    srv = serving.Server("mixed", dt, mem_budget=args.mem_budget << 30, quantum=args.quantum,
+                         kernel_mode=args.kernel_mode)
+    # warm-up: one small request through the loop (kernels loaded, arena grown)
+    srv.run([serving.Request(-1, workloads.random_ca_cell(rng, 200, kahp2d=True).pack(), 200, 0.0)],
+            realtime=False)
+    res = srv.run(reqs, realtime=True)
+    summ = serving.summarize(res, srv.wall, srv.cs)
+    srv.close()
+    out = {
+        "metric": "serving: sustained compartment-steps/s with p50/p99 request latency",
+        "value": summ["sustained_cs_per_s"], "unit": "compartment-steps/s",
+        "n_gpus": 1, "steps": None, "warmup": 1, "ms_per_step": None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32-channels/f64-hines", "data": "synthetic (seeded random morphologies, Poisson arrivals)",
+        "config": {"workload": "serving loop: Poisson arrivals, n~logU[100,30000] Ca cells with 2-D K-AHP, "
+                               "duration logU[10ms,1s], dt=25us, mixed precision",
+                   "requests": count, "rate_per_s": args.rate, "quantum_steps": args.quantum,
+                   "mem_budget_gib": args.mem_budget, "max_active": srv.max_active, "quanta": srv.quanta},
+        "p50_latency_s": summ["p50_latency_s"], "p99_latency_s": summ["p99_latency_s"],
+        "wall_s": summ["wall_s"], "compartment_steps": summ["compartment_steps"],
+    }
+    print(json.dumps(out))
+
+
 def copy_probe(mib):
     """Plain device-to-device copy of a known size, for counter calibration."""
     import torch
@@ -181,9 +226,12 @@ def copy_probe(mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="ensemble", choices=sorted(WORKLOADS),
+    ap.add_argument("--workload", default="ensemble", choices=sorted(WORKLOADS) + ["serving"],
                     help="ensemble: the headline batched-ensemble line; large_ca: the large Ca cells "
-                         "with 2-D K-AHP channels")
+                         "with 2-D K-AHP channels; serving: the request loop (p50/p99 latency)")
+    ap.add_argument("--rate", type=float, default=20.0, help="serving: request arrivals per second")
+    ap.add_argument("--quantum", type=int, default=100, help="serving: steps per batch quantum")
+    ap.add_argument("--mem-budget", type=int, default=64, help="serving: device memory budget, GiB")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--cells", type=int, default=None)
@@ -205,6 +253,10 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(cells are independent and would shard by bin-packing across GPUs)"}))
         sys.exit(2)
+    if args.workload == "serving":
+        args.cells = 200 if args.cells is None else args.cells
+        run_serving(args)
+        return
     desc, dcells, dt, dsteps, dspl, dgraph = WORKLOADS[args.workload]
     args.cells = dcells if args.cells is None else args.cells
     args.steps = dsteps if args.steps is None else args.steps
