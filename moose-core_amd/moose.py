@@ -1,5 +1,6 @@
 """A pymoose-style object layer over the engine: build a model from
-Compartment, HHChannel, HHGate, CaConc, SpikeGen, Leakage and Table objects
+Compartment, HHChannel, HHGate, HHChannel2D, HHGate2D, CaConc, SpikeGen,
+Leakage and Table objects
 connected by messages, hand it to an HSolve through `target`, then
 `reinit()` / `start(t)` and read or write the (zombified) fields.
 
@@ -30,7 +31,7 @@ import math
 import numpy as np
 
 from . import _abi
-from .cellspec import CellSpec, Gate
+from .cellspec import CellSpec, Gate, Table2D
 
 FARADAY = 96485.3415      # basecode/header.h:30
 CA_VALENCE = 2            # biophysics/CaConcBase.cpp:15
@@ -171,6 +172,72 @@ class HHChannel(Neutral):
         return self.gate("Z")
 
 
+class HHGate2D(Neutral):
+    """An HHGate2D (biophysics/HHGate2D.cpp): two Interpol2D tables, A and B,
+    indexed [x][y] over [xmin, xmax] x [ymin, ymax]; setting a table sets
+    its divisions (Interpol2D::setTableVector)."""
+
+    def _init_fields(self):
+        d = self.__dict__
+        d.update(xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, xdivs=0, ydivs=0,
+                 tableA=np.zeros((1, 1)), tableB=np.zeros((1, 1)))
+
+    def _set_local(self, name, value):
+        if name in ("tableA", "tableB"):
+            value = np.array(value, dtype=np.float64)
+            if value.ndim != 2:
+                raise ValueError("HHGate2D tables are 2-D ([x][y])")
+            object.__setattr__(self, "xdivs", value.shape[0] - 1)
+            object.__setattr__(self, "ydivs", value.shape[1] - 1)
+        object.__setattr__(self, name, value)
+
+
+# HHChannel2D::dependency (biophysics/HHChannel2D.cpp:273-303): what the x and
+# y of a gate's tables read (0 Vm, 1 concen, 2 concen2, -1 nothing)
+_INDEX_DEPS = {"VOLT_INDEX": (0, -1), "C1_INDEX": (1, -1), "C2_INDEX": (2, -1),
+               "VOLT_C1_INDEX": (0, 1), "VOLT_C2_INDEX": (0, 2), "C1_C2_INDEX": (1, 2)}
+
+
+class HHChannel2D(Neutral):
+    """An HHChannel2D (biophysics/HHChannel2D.cpp): gates on 2-D tables of
+    Vm and up to two concentrations (Xindex/Yindex/Zindex), integrated by
+    exponential Euler; its current reaches the compartment through the
+    "channel" message.  Taken over with its cell, it runs on the GPU beside
+    the solver and its fields read and write there."""
+    _zfields = ("Gbar", "Ek", "Gk", "X", "Y", "Z", "modulation")
+    _zmap = {f: f + "2D" for f in _zfields}
+
+    def _init_fields(self):
+        d = self.__dict__
+        d.update(Gbar=0.0, Ek=0.0, Gk=0.0, Xpower=0.0, Ypower=0.0, Zpower=0.0,
+                 X=0.0, Y=0.0, Z=0.0, instant=0, modulation=1.0,
+                 Xindex="", Yindex="", Zindex="")
+
+    def _set_local(self, name, value):
+        if name in ("Xindex", "Yindex", "Zindex") and value not in _INDEX_DEPS and value != "":
+            raise ValueError(f"HHChannel2D: unknown index '{value}'")
+        object.__setattr__(self, name, value)
+        for p, g in (("Xpower", "gateX"), ("Ypower", "gateY"), ("Zpower", "gateZ")):
+            if name == p and value > 0 and _norm(self.path + "/" + g) not in _objects:
+                HHGate2D(self.path + "/" + g)
+
+    def gate(self, which):
+        src = self.__dict__.get("_original") or self
+        return _objects.get(_norm(src.path + "/gate" + which))
+
+    @property
+    def gateX(self):
+        return self.gate("X")
+
+    @property
+    def gateY(self):
+        return self.gate("Y")
+
+    @property
+    def gateZ(self):
+        return self.gate("Z")
+
+
 class CaConc(Neutral):
     _zfields = ("Ca", "CaBasal", "ceiling", "floor")
 
@@ -251,7 +318,7 @@ def copy(orig, parent, name=None):
         if k in ("path", "name", "id", "children", "_solver"):
             continue
         new.__dict__[k] = v.copy() if isinstance(v, (list, np.ndarray)) else v
-    if isinstance(orig, HHChannel):
+    if isinstance(orig, (HHChannel, HHChannel2D)):
         new.__dict__["_original"] = orig.__dict__.get("_original") or orig
     return new
 
@@ -427,6 +494,36 @@ class HSolve(Neutral):
         for q, p in enumerate(pools):
             d = p.__dict__
             prec[q] = (d["Ca"], d["CaBasal"], d["tau"], p.shell_B(), d["ceiling"], d["floor"])
+        # HHChannel2D objects on the compartments, in message order; each
+        # gate's A and B tables become Interpol2D entries (once per gate)
+        ch2, tabs, tpos = [], [], {}
+        for c in comps:
+            for ch in _channels(c, "HHChannel2D"):
+                ch2.append((cidx[c.id], ch))
+        c2rec = np.zeros(len(ch2), _abi.CHAN2D_DT)
+        for k, (ci, ch) in enumerate(ch2):
+            d = ch.__dict__
+            ta, tb, dep = [-1, -1, -1], [-1, -1, -1], [-1] * 6
+            for a, w in enumerate("XYZ"):
+                if not d[w + "power"] > 0:
+                    continue
+                gt = ch.gate(w)
+                if gt.id not in tpos:
+                    tpos[gt.id] = len(tabs)
+                    g = gt.__dict__
+                    for t in (g["tableA"], g["tableB"]):
+                        tabs.append(Table2D(int(g["xdivs"]), int(g["ydivs"]), g["xmin"], g["xmax"],
+                                            g["ymin"], g["ymax"], np.asarray(t, np.float64)))
+                ta[a], tb[a] = 2 * tpos[gt.id], 2 * tpos[gt.id] + 1
+                dep[2 * a], dep[2 * a + 1] = _INDEX_DEPS.get(d[w + "index"], (-1, -1))
+            concs = []
+            for field in ("concen", "concen2"):
+                src = _sources(ch, field, "CaConc")
+                if src and src[-1].id not in pidx:
+                    raise ValueError(f"{ch.path}: {field} from a CaConc the solver does not run")
+                concs.append(pidx[src[-1].id] if src else -1)
+            c2rec[k] = (ci, d["instant"], ta, tb, dep, concs[0], concs[1], d["Gbar"], d["Ek"],
+                        d["Xpower"], d["Ypower"], d["Zpower"], d["X"], d["Y"], d["Z"], d["modulation"])
         srec = np.zeros(len(spikes), _abi.SPIKEGEN_DT)
         for k, (ci, sg) in enumerate(spikes):
             d = sg.__dict__
@@ -435,9 +532,10 @@ class HSolve(Neutral):
         for k, rec in enumerate(leaks):
             lrec[k] = rec
         spec = CellSpec(ct, np.array(axial, dtype=np.int32).reshape(-1, 2), seed=cidx[seed.id],
-                        channels=crec, gates=glist, pools=prec, spikegens=srec, leaks=lrec)
+                        channels=crec, gates=glist, pools=prec, spikegens=srec, leaks=lrec,
+                        tables2d=tabs, chans2d=c2rec)
         objs = {"compartments": comps, "channels": [ch for _, ch in chans], "pools": pools,
-                "spikegens": [sg for _, sg in spikes]}
+                "spikegens": [sg for _, sg in spikes], "chans2d": [ch for _, ch in ch2]}
         return spec, objs
 
     def _setup(self, seed):
@@ -453,7 +551,7 @@ class HSolve(Neutral):
         object.__setattr__(self, "_objs", objs)
         object.__setattr__(self, "_spec", spec)
         # zombify (hsolve/HSolve.cpp:235-262): fields now live in the solver
-        for kind in ("compartments", "channels", "pools"):
+        for kind in ("compartments", "channels", "pools", "chans2d"):
             for i, o in enumerate(objs[kind]):
                 o.__dict__["_solver"] = (self, i)
                 for f in type(o)._zfields:       # reads now reach the solver
@@ -462,9 +560,11 @@ class HSolve(Neutral):
 
     # -- zombie field access (hsolve/HSolveInterface.cpp) ----------------
     def _get(self, obj, field, idx):
+        field = getattr(type(obj), "_zmap", {}).get(field, field)
         return float(self._engine.get(self._cell, field, [idx])[0])
 
     def _set(self, obj, field, idx, value):
+        field = getattr(type(obj), "_zmap", {}).get(field, field)
         self._engine.set(self._cell, field, [idx], [float(value)])
 
     # -- recording -------------------------------------------------------

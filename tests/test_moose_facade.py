@@ -157,3 +157,105 @@ def test_layer_runs_on_the_gpu_like_the_oracle(pkg, oracle_mod):
     np.testing.assert_allclose(spk.vector, np.array(steps, dtype=np.float64) * 50e-6, rtol=0, atol=0)
     assert len(spk.vector) > 0
     moose.reset()
+
+
+def add_ca_and_kc(moose, comps, kc_tables, gbar_kc=5e-9):
+    """Per compartment: a CaConc fed by a CaL HHChannel (IkOut -> CaConc)
+    and an HHChannel2D K(C) on Vm x [Ca] (Zindex VOLT_C1_INDEX) reading that
+    pool through concen, all in a MOOSE script's creation/message order."""
+    from moose_core_amd import workloads as W
+    cag = W.ca_gates()
+    cal = moose.HHChannel("/library/CaL")
+    cal.Ek = 0.080
+    cal.Xpower = 2.0
+    g = cag[5]
+    cal.gateX.min, cal.gateX.max = g.min, g.max
+    cal.gateX.tableA = g.tableA
+    cal.gateX.tableB = g.tableB
+    kc = moose.HHChannel2D("/library/KC")
+    kc.Ek = -0.080
+    kc.Zpower = 1.0
+    kc.Zindex = "VOLT_C1_INDEX"
+    A, B = kc_tables
+    for f in ("xmin", "xmax", "ymin", "ymax"):
+        setattr(kc.gateZ, f, getattr(A, f))
+    kc.gateZ.tableA = A.table
+    kc.gateZ.tableB = B.table
+    pools, kcs = [], []
+    for i, c in enumerate(comps):
+        p = moose.CaConc(c.path + "/ca")
+        p.CaBasal = p.Ca = 1e-4
+        p.tau = 0.020
+        p.B = 5e12
+        ch = moose.copy(cal, c, "CaL")
+        ch.Gbar = 1e-9
+        moose.connect(ch, "channel", c, "channel")
+        moose.connect(ch, "IkOut", p, "current")
+        k = moose.copy(kc, c, "KC")
+        k.Gbar = gbar_kc
+        moose.connect(k, "channel", c, "channel")
+        moose.connect(p, "concOut", k, "concen")
+        pools.append(p)
+        kcs.append(k)
+    return pools, kcs
+
+
+def small_kc_tables():
+    from moose_core_amd import workloads as W
+    return W.kahp2d_tables(vdivs=60, cadivs=25)
+
+
+def test_layer_builds_hhchannel2d(pkg, oracle_mod):
+    from moose_core_amd import moose
+    comps, _ = build_ball_tree(moose, depth=3)
+    tabs = small_kc_tables()
+    pools, kcs = add_ca_and_kc(moose, comps, tabs)
+    hs = moose.HSolve("/cell/hsolve")
+    spec, objs = hs.build_spec(moose._deep_search_compartment(moose.element("/cell")))
+    assert len(spec.chans2d) == len(comps) and len(spec.tables2d) == 2
+    np.testing.assert_array_equal(spec.tables2d[0].table, tabs[0].table)
+    np.testing.assert_array_equal(spec.tables2d[1].table, tabs[1].table)
+    cpos = {c.id: i for i, c in enumerate(objs["compartments"])}
+    ppos = {p.id: i for i, p in enumerate(objs["pools"])}
+    for k, ch in enumerate(objs["chans2d"]):
+        r = spec.chans2d[k]
+        comp = ch.path.rsplit("/", 1)[0]
+        assert r["compartment"] == cpos[moose.element(comp).id]
+        assert list(r["tableA"]) == [-1, -1, 0] and list(r["tableB"]) == [-1, -1, 1]
+        assert list(r["dep"]) == [-1, -1, -1, -1, 0, 1]
+        assert r["conc1"] == ppos[moose.element(comp + "/ca").id] and r["conc2"] == -1
+        assert r["Gbar"] == 5e-9 and r["Zpower"] == 1.0
+    # the oracle runs it and the 2-D channels conduct
+    dt = 50e-6
+    o = oracle_mod.OracleCell(spec, dt)
+    o.reinit(dt)
+    tr = o.run(dt, 200, np.arange(spec.n))
+    assert np.isfinite(tr).all()
+    assert (o.get("Gk2D", np.arange(len(kcs))) > 0).all()
+    with pytest.raises(ValueError):
+        kcs[0].Xindex = "NOT_AN_INDEX"
+
+
+@pytest.mark.gpu
+def test_layer_hhchannel2d_on_the_gpu(pkg, oracle_mod):
+    print("case: object layer, ball tree depth 3 + CaL/CaConc/HHChannel2D, fp64, 300 steps", flush=True)
+    from moose_core_amd import moose
+    comps, _ = build_ball_tree(moose, depth=3, inject=0.8e-9)
+    pools, kcs = add_ca_and_kc(moose, comps, small_kc_tables())
+    vm = moose.Table("/cell/vm")
+    moose.connect(vm, "requestOut", comps[0], "getVm")
+    hs = moose.HSolve("/cell/hsolve")
+    hs.dt = 50e-6
+    hs.target = "/cell"
+    spec = hs._spec
+    moose.reinit()
+    moose.start(300 * 50e-6)
+    ref = oracle_mod.OracleCell(spec, 50e-6)
+    ref.reinit(50e-6)
+    tr = ref.run(50e-6, 300, np.arange(spec.n))
+    assert np.abs(vm.vector[1:] - tr[:, 0]).max() <= 1e-6
+    k = hs._objs["chans2d"].index(kcs[3])
+    assert kcs[3].Gk == pytest.approx(ref.get("Gk2D", [k])[0], rel=1e-6)
+    kcs[3].Gbar = 2 * kcs[3].Gbar              # zombie write reaches the device
+    assert kcs[3].Gbar == pytest.approx(1e-8)
+    moose.reset()
