@@ -124,18 +124,276 @@ class SymCompartment(Compartment):
     pass
 
 
+SINGULARITY = 1.0e-6      # biophysics/HHGate.cpp:16
+
+
 class HHGate(Neutral):
-    """An HHGate: tables A (alpha) and B (alpha + beta) over [min, max]."""
+    """An HHGate (biophysics/HHGate.cpp): tables A (alpha) and B (alpha +
+    beta) over [min, max], set directly (tableA/tableB) or generated from
+    the GENESIS functional form (setupAlpha, setupTau, setupGate, the
+    alpha/beta/tau/mInfinity fields).  The table generation and the
+    range/size changes restate the reference's arithmetic step by step,
+    including its quirks, so a gate built either way holds the reference's
+    numbers."""
 
     def _init_fields(self):
         d = self.__dict__
-        d.update(min=0.0, max=1.0, divs=0, tableA=np.zeros(1), tableB=np.zeros(1))
+        # HHGate::HHGate(Id, Id) (HHGate.cpp:214-225)
+        d.update(_A=[0.0], _B=[0.0], _xmin=0.0, _xmax=1.0, _invDx=1.0, useInterpolation=False,
+                 _direct=False, _alpha=[], _beta=[], _tau=[], _minf=[])
+
+    # -- fields --------------------------------------------------------
+    @property
+    def tableA(self):
+        return np.array(self._A, dtype=np.float64)
+
+    @property
+    def tableB(self):
+        return np.array(self._B, dtype=np.float64)
+
+    @property
+    def min(self):
+        return self._xmin
+
+    @property
+    def max(self):
+        return self._xmax
+
+    @property
+    def divs(self):
+        return len(self._A) - 1
+
+    @property
+    def alpha(self):
+        return list(self._alpha)
+
+    @property
+    def beta(self):
+        return list(self._beta)
+
+    @property
+    def tau(self):
+        return list(self._tau)
+
+    @property
+    def mInfinity(self):
+        return list(self._minf)
+
+    @property
+    def alphaParms(self):
+        # HHGate::getAlphaParms (HHGate.cpp:498-508)
+        return list(self._alpha) + list(self._beta) + [float(len(self._A)), self._xmin, self._xmax]
 
     def _set_local(self, name, value):
-        if name in ("tableA", "tableB"):
-            value = np.array(value, dtype=np.float64)
-            object.__setattr__(self, "divs", len(value) - 1)
-        object.__setattr__(self, name, value)
+        d = self.__dict__
+        if name == "tableA":            # HHGate::setTableA (:436-448)
+            v = [float(x) for x in np.asarray(value, dtype=np.float64).reshape(-1)]
+            if len(v) < 2:
+                return
+            d["_direct"] = True
+            d["_A"] = v
+            d["_invDx"] = (len(v) - 1) / (d["_xmax"] - d["_xmin"])
+        elif name == "tableB":          # HHGate::setTableB (:456-468)
+            v = [float(x) for x in np.asarray(value, dtype=np.float64).reshape(-1)]
+            d["_direct"] = True
+            if len(v) != len(d["_A"]):
+                return
+            d["_B"] = v
+        elif name == "min":             # HHGate::setMin (:368-383)
+            d["_xmin"] = float(value)
+            xdivs = len(d["_A"]) - 1
+            if d["_direct"] and xdivs > 0:
+                d["_invDx"] = xdivs / (d["_xmax"] - d["_xmin"])
+                self._tab_fill("_A", xdivs, d["_xmin"], d["_xmax"])
+                self._tab_fill("_B", xdivs, d["_xmin"], d["_xmax"])
+            else:
+                self._update_tables()
+        elif name == "max":             # HHGate::setMax (:390-406)
+            d["_xmax"] = float(value)
+            xdivs = len(d["_A"]) - 1
+            if d["_direct"] and xdivs > 0:
+                d["_invDx"] = xdivs / (d["_xmax"] - d["_xmin"])
+                self._tab_fill("_A", xdivs, d["_xmin"], d["_xmax"])
+                self._tab_fill("_B", xdivs, d["_xmin"], d["_xmax"])
+            else:
+                self._update_tables()
+        elif name == "divs":            # HHGate::setDivs (:413-429)
+            val = int(value)
+            if d["_direct"]:
+                d["_invDx"] = val / (d["_xmax"] - d["_xmin"])
+                self._tab_fill("_A", val, d["_xmin"], d["_xmax"])
+                self._tab_fill("_B", val, d["_xmin"], d["_xmax"])
+            else:
+                d["_A"] = (d["_A"] + [0.0] * (val + 1))[:val + 1]
+                d["_B"] = (d["_B"] + [0.0] * (val + 1))[:val + 1]
+                d["_invDx"] = val / (d["_xmax"] - d["_xmin"])
+                self._update_tables()
+        elif name in ("alpha", "beta", "tau", "mInfinity"):   # :288-361
+            v = [float(x) for x in value]
+            if len(v) != 5:
+                return
+            key = {"alpha": "_alpha", "beta": "_beta", "tau": "_tau", "mInfinity": "_minf"}[name]
+            d[key] = v
+            self._update_tables()
+        else:
+            object.__setattr__(self, name, value)
+
+    # -- table generation (HHGate.cpp:481-795) ---------------------------
+    def setupAlpha(self, parms):
+        """HHGate::setupAlpha: 13 numbers, alpha A B C D F, beta A B C D F,
+        divs, min, max; tables A = alpha, B = alpha + beta."""
+        parms = [float(x) for x in parms]
+        if len(parms) != 13:
+            return
+        self._setup_tables(parms, False)
+        self.__dict__["_alpha"] = parms[0:5]
+        self.__dict__["_beta"] = parms[5:10]
+
+    def setupTau(self, parms):
+        """HHGate::setupTau: the same 13 numbers read as tau and mInfinity;
+        tables A = minf/tau, B = 1/tau."""
+        parms = [float(x) for x in parms]
+        if len(parms) != 13:
+            return
+        self._setup_tables(parms, True)
+
+    def tweakAlpha(self):
+        """HHGate::tweakAlpha is a no-op in this MOOSE version (:524-527)."""
+
+    def tweakTau(self):
+        """HHGate::tweakTau is a no-op in this MOOSE version (:529-532)."""
+
+    def setupGate(self, parms):
+        """HHGate::setupGate (:670-752): A B C D F size min max isbeta.  As
+        in the reference, x stays at min + dx/2 for every entry, and an
+        isbeta call converts the pair to (alpha, alpha + beta)."""
+        if len(parms) != 9:
+            return
+        A, B, C, D, F = (float(x) for x in parms[:5])
+        size, mn, mx, isBeta = int(parms[5]), float(parms[6]), float(parms[7]), bool(parms[8])
+        d = self.__dict__
+        key = "_B" if isBeta else "_A"
+        ip = d[key]
+        if size <= 0:
+            size = len(ip) - 1
+            if size <= 0:
+                return
+        else:
+            ip = (ip + [0.0] * (size + 1))[:size + 1]
+        dx = (mx - mn) / float(size)
+        x = mn + dx / 2.0
+        for i in range(size + 1):
+            if abs(F) < SINGULARITY:
+                ip[i] = 0.0
+            else:
+                temp2 = C + math.exp((x + D) / F)
+                if abs(temp2) < SINGULARITY:
+                    if i > 0:               # the reference reads ip[i - 1]
+                        ip[i] = ip[i - 1]
+                else:
+                    ip[i] = (A + B * x) / temp2
+        d[key] = ip
+        if isBeta:
+            if len(d["_A"]) != len(d["_B"]):
+                if len(d["_A"]) > len(d["_B"]):
+                    self._tab_fill("_B", len(d["_A"]) - 1, d["_xmin"], d["_xmax"])
+                else:
+                    self._tab_fill("_A", len(d["_B"]) - 1, d["_xmin"], d["_xmax"])
+            d["_B"] = [a + b for a, b in zip(d["_A"], d["_B"])]      # tweakTables(0)
+
+    def _update_tables(self):
+        # HHGate::updateTables (:784-795): the current table size as divs
+        d = self.__dict__
+        if not d["_alpha"] or not d["_beta"]:
+            return
+        self._setup_tables(d["_alpha"] + d["_beta"] + [float(len(d["_A"])), d["_xmin"], d["_xmax"]], False)
+
+    def _setup_tables(self, parms, doTau):
+        """HHGate::setupTables (:539-641)."""
+        if parms[10] < 1:
+            return
+        xdivs = int(parms[10])
+        d = self.__dict__
+        A = [0.0] * (xdivs + 1)
+        B = [0.0] * (xdivs + 1)
+        d["_xmin"], d["_xmax"] = parms[11], parms[12]
+        d["_invDx"] = xdivs / (d["_xmax"] - d["_xmin"])
+        dx = (d["_xmax"] - d["_xmin"]) / xdivs
+        x = d["_xmin"]
+        temp2 = 0.0
+        for i in range(xdivs + 1):
+            if abs(parms[4]) < SINGULARITY:
+                temp = 0.0
+                A[i] = temp
+            else:
+                temp2 = parms[2] + math.exp((x + parms[3]) / parms[4])
+                if abs(temp2) < SINGULARITY:
+                    temp2 = parms[2] + math.exp((x + dx / 10.0 + parms[3]) / parms[4])
+                    temp = (parms[0] + parms[1] * (x + dx / 10)) / temp2
+                    temp2 = parms[2] + math.exp((x - dx / 10.0 + parms[3]) / parms[4])
+                    temp += (parms[0] + parms[1] * (x - dx / 10)) / temp2
+                    temp /= 2.0
+                    A[i] = temp
+                else:
+                    temp = (parms[0] + parms[1] * x) / temp2
+                    A[i] = temp
+            if abs(parms[9]) < SINGULARITY:
+                B[i] = 0.0
+            else:
+                temp2 = parms[7] + math.exp((x + parms[8]) / parms[9])
+                if abs(temp2) < SINGULARITY:
+                    temp2 = parms[7] + math.exp((x + dx / 10.0 + parms[8]) / parms[9])
+                    temp = (parms[5] + parms[6] * (x + dx / 10)) / temp2
+                    temp2 = parms[7] + math.exp((x - dx / 10.0 + parms[8]) / parms[9])
+                    temp += (parms[5] + parms[6] * (x - dx / 10)) / temp2
+                    temp /= 2.0
+                    B[i] = temp
+                else:
+                    B[i] = (parms[5] + parms[6] * x) / temp2
+            if not doTau and abs(temp2) > SINGULARITY:
+                B[i] += temp
+            x += dx
+        if doTau:
+            pa = pb = 0.0
+            for i in range(xdivs + 1):
+                temp, temp2 = A[i], B[i]
+                if abs(temp) < SINGULARITY:
+                    A[i], B[i] = pa, pb
+                else:
+                    A[i] = temp2 / temp
+                    B[i] = 1.0 / temp
+                pa, pb = A[i], B[i]
+        d["_A"], d["_B"] = A, B
+
+    def _lookup_table(self, tab, v, interp):
+        # HHGate::lookupTable (:231-246)
+        d = self.__dict__
+        if v <= d["_xmin"]:
+            return tab[0]
+        if v >= d["_xmax"]:
+            return tab[-1]
+        index = int((v - d["_xmin"]) * d["_invDx"])
+        if interp:
+            frac = (v - d["_xmin"] - index / d["_invDx"]) * d["_invDx"]
+            # a point just below max can round onto the last entry; the
+            # reference then reads one past the end with frac ~ 0
+            nxt = tab[index + 1] if index + 1 < len(tab) else tab[index]
+            return tab[index] * (1 - frac) + nxt * frac
+        return tab[index]
+
+    def _tab_fill(self, key, newXdivs, newXmin, newXmax):
+        """HHGate::tabFill (:754-774): resizes, then refills each
+        entry by an interpolated lookup in the table being refilled, with
+        the gate's current range (as the reference does)."""
+        if newXdivs < 3:
+            return
+        d = self.__dict__
+        table = list(d[key])
+        newDx = (newXmax - newXmin) / newXdivs
+        table = (table + [0.0] * (newXdivs + 1))[:newXdivs + 1]
+        for i in range(newXdivs + 1):
+            table[i] = self._lookup_table(table, newXmin + i * newDx, True)
+        d[key] = table
 
 
 class HHChannel(Neutral):

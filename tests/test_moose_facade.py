@@ -259,3 +259,103 @@ def test_layer_hhchannel2d_on_the_gpu(pkg, oracle_mod):
     kcs[3].Gbar = 2 * kcs[3].Gbar              # zombie write reaches the device
     assert kcs[3].Gbar == pytest.approx(1e-8)
     moose.reset()
+
+
+# ---------------------------------------------------------------------------
+# HHGate table generation (biophysics/HHGate.cpp:481-795)
+# ---------------------------------------------------------------------------
+EREST = -0.070
+NA_M = [0.1e6 * (EREST + 0.025), -0.1e6, -1.0, -(EREST + 0.025), -0.010,    # alpha
+        4e3, 0.0, 0.0, -EREST, 0.018,                                       # beta
+        3000, -0.100, 0.050]
+
+
+def _alpha(p, x):
+    return (p[0] + p[1] * x) / (p[2] + np.exp((x + p[3]) / p[4]))
+
+
+def test_setup_alpha_tables(pkg):
+    from moose_core_amd import moose
+    moose.reset()
+    ch = moose.HHChannel("/na")
+    ch.Xpower = 3
+    g = ch.gateX
+    g.setupAlpha(NA_M)
+    A, B = g.tableA, g.tableB
+    assert len(A) == 3001 and g.min == -0.1 and g.max == 0.05 and g.divs == 3000
+    x = -0.1 + np.arange(3001) * (0.15 / 3000)
+    alpha, beta = _alpha(NA_M[:5], x), _alpha(NA_M[5:10], x)
+    # the alpha term has a 0/0 at x = -(EREST + 0.025) (x / (exp(x / k) - 1)):
+    # away from it the tables are alpha and alpha + beta; at it, the average
+    # of the two neighbours at +-dx/10, i.e. the limit 0.1e6 * 0.010 to O(dx^2)
+    sing = np.abs(NA_M[2] + np.exp((x + NA_M[3]) / NA_M[4])) < 1e-6
+    assert sing.sum() == 1
+    np.testing.assert_allclose(A[~sing], alpha[~sing], rtol=1e-9)
+    np.testing.assert_allclose(B[~sing] - A[~sing], beta[~sing], rtol=1e-9, atol=1e-9)
+    assert A[sing][0] == pytest.approx(0.1e6 * 0.010, rel=1e-6)
+    assert g.alpha == NA_M[:5] and g.beta == NA_M[5:10]
+    assert g.alphaParms[-3:] == [3001.0, -0.1, 0.05]
+
+
+def test_setup_tau_tables(pkg):
+    from moose_core_amd import moose
+    moose.reset()
+    ch = moose.HHChannel("/ka")
+    ch.Ypower = 1
+    g = ch.gateY
+    # tau = 0.02 (constant: B=0, C=1 with a huge F keeps exp() at 1 -> 0.04/2),
+    # mInfinity = 1/(1 + exp((x + 0.078)/0.006))
+    tau = [0.04, 0.0, 1.0, 0.0, 1e12]
+    minf = [1.0, 0.0, 1.0, 0.078, 0.006]
+    g.setupTau(tau + minf + [100, -0.1, 0.05])
+    x = -0.1 + np.arange(101) * 0.0015
+    t = _alpha(tau, x)
+    m = _alpha(minf, x)
+    np.testing.assert_allclose(g.tableA, m / t, rtol=1e-12)
+    np.testing.assert_allclose(g.tableB, 1.0 / t, rtol=1e-12)
+
+
+def test_gate_range_changes_and_field_forms(pkg):
+    from moose_core_amd import moose
+    moose.reset()
+    ch = moose.HHChannel("/k")
+    ch.Xpower = 4
+    g = ch.gateX
+    g.min, g.max = -0.1, 0.05          # before any table: nothing to rebuild
+    tab = np.linspace(0.0, 30.0, 301)
+    g.tableA = tab
+    g.tableB = 2 * tab
+    # a new range on a direct table refills it by lookups in itself at the
+    # new grid, which lands on its own entries: values stay, range moves
+    g.min = -0.12
+    assert g.min == -0.12
+    np.testing.assert_allclose(g.tableA, tab, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(g.tableB, 2 * tab, rtol=1e-12, atol=1e-12)
+    g.tableB = np.zeros(7)              # size mismatch: ignored
+    assert len(g.tableB) == 301
+    # the alpha/beta fields rebuild from the functional form with the
+    # current table size as the number of divisions
+    moose.reset()
+    ch = moose.HHChannel("/k2")
+    ch.Xpower = 1
+    g = ch.gateX
+    g.setupAlpha(NA_M[:10] + [50, -0.1, 0.05])
+    g.alpha = NA_M[:5]
+    assert g.divs == 51                 # HHGate::updateTables passes A_.size() as divs
+    x = -0.1 + np.arange(52) * (0.15 / 51)
+    ok = np.abs(NA_M[2] + np.exp((x + NA_M[3]) / NA_M[4])) >= 1e-6
+    np.testing.assert_allclose(g.tableA[ok], _alpha(NA_M[:5], x)[ok], rtol=1e-9)
+
+
+def test_setup_gate_form(pkg):
+    from moose_core_amd import moose
+    moose.reset()
+    ch = moose.HHChannel("/g")
+    ch.Xpower = 1
+    g = ch.gateX
+    g.min, g.max = -0.1, 0.05
+    g.setupGate([2.0, 0.0, 1.0, 0.0, 1e12, 10, -0.1, 0.05, 0])     # alpha = 1
+    g.setupGate([6.0, 0.0, 1.0, 0.0, 1e12, 10, -0.1, 0.05, 1])     # beta = 3
+    # A = alpha, B = alpha + beta after the beta call
+    np.testing.assert_allclose(g.tableA, np.ones(11))
+    np.testing.assert_allclose(g.tableB, np.full(11, 4.0))
