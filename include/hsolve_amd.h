@@ -291,6 +291,12 @@ int hsg_read_spikes(hsg_engine* e, hsg_spike* out, int32_t cap, int32_t* n_out,
  * that cell. */
 int hsg_set_probes(hsg_engine* e, int32_t cell, const int32_t* compartments,
                    int32_t n, int64_t capacity);
+/* As hsg_set_probes, sampling only every `every`-th step (steps counted from
+ * reinit): a Table on a clock tick `every` times the solver's dt, processed
+ * after the solver in the same step (Table::process, builtins/Table.cpp:254-
+ * 285; tick order and rates, scheduling/Clock.cpp:764-775). */
+int hsg_set_probes_every(hsg_engine* e, int32_t cell, const int32_t* compartments,
+                         int32_t n, int64_t capacity, int32_t every);
 /* Copies the recorded samples: out[k*n + j] = sample k of probe j. *n_samples
  * receives the number of samples (<= capacity).  Clears the trace. */
 int hsg_read_probes(hsg_engine* e, int32_t cell, double* out, int64_t cap_samples,

@@ -429,6 +429,7 @@ int place_cell(hsg_engine* e, HostCell& hc) {
     hc.bytes = total;
 
     d.n = n;
+    d.probeStride = 1;
     d.nChanSlots = s.chanSlots;
     d.nGateSlots = s.gateSlots;
     d.nPoolSlots = s.poolSlots;
@@ -1654,9 +1655,15 @@ int hsg_read_spikes(hsg_engine* e, hsg_spike* out, int32_t cap, int32_t* n_out, 
 }
 
 int hsg_set_probes(hsg_engine* e, int32_t cell, const int32_t* compartments, int32_t n, int64_t capacity) {
+    return hsg_set_probes_every(e, cell, compartments, n, capacity, 1);
+}
+
+int hsg_set_probes_every(hsg_engine* e, int32_t cell, const int32_t* compartments, int32_t n, int64_t capacity,
+                         int32_t every) {
     HostCell* hc = get_cell(e, cell);
     if (!hc) return fail(HSG_E_ARG, "no such cell");
     if (n < 0 || capacity < 0 || (n > 0 && !compartments)) return fail(HSG_E_ARG, "bad probe list");
+    if (every < 1) return fail(HSG_E_ARG, "probe interval must be at least one step");
     hipSetDevice(e->device);
     const CellSetup& s = hc->s;
     std::vector<int32_t> h(n);
@@ -1669,6 +1676,7 @@ int hsg_set_probes(hsg_engine* e, int32_t cell, const int32_t* compartments, int
     if (hc->probe.off >= 0) { e->arena.release(hc->probe.off, hc->probe.bytes); hc->probe.off = -1; }
     hc->dev.nProbe = 0;
     hc->dev.probeCap = 0;
+    hc->dev.probeStride = every;
     e->tablesDirty = true;
     if (n == 0 || capacity == 0) return 0;
     int64_t bytes = align_up(4 * (int64_t)n) + align_up(8 * (int64_t)n * capacity) + kAlign;
@@ -1696,8 +1704,10 @@ int hsg_read_probes(hsg_engine* e, int32_t cell, double* out, int64_t cap_sample
     *n_samples = 0;
     if (hc->dev.nProbe == 0) return 0;
     hipSetDevice(e->device);
-    // samples are written at index (step - probeBase) while below capacity
-    const int64_t cnt = hc->stepsDone - hc->dev.probeBase;
+    // sample k is the (k+1)-th multiple of the stride after probeBase,
+    // written while below capacity
+    const int64_t st = hc->dev.probeStride;
+    const int64_t cnt = hc->stepsDone / st - hc->dev.probeBase / st;
     int64_t k = std::min(std::min(cnt, hc->dev.probeCap), cap_samples);
     int rc;
     if (k > 0 && out && (rc = download(e, hc->dev.oProbeTrace, (size_t)(k * hc->dev.nProbe), out))) return rc;

@@ -97,6 +97,7 @@ struct CellDev {
     int32_t tableSet;       // the cell's lookup-table set
     int64_t probeCap;       // samples per probe
     int64_t probeBase;      // step count at which the current trace started
+    int64_t probeStride;    // a sample every probeStride steps (>= 1)
     // compartment arrays (f64 unless noted)
     int64_t oV, oInitVm, oCmByDt, oEmByRm, oDbase, oOff;
     int64_t oDefer;         // int32[n]: first deferred-term pair of node, -1

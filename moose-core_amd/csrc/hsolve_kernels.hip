@@ -807,7 +807,7 @@ __device__ __forceinline__ const ChanType* stage_types(const CellDev& c, const C
 // by HSolveActive::sendSpikes, hsolve/HSolveActive.cpp:342-347) into the
 // engine's spike ring, and Vm probes (Table sampling after the solver's tick).
 __device__ __forceinline__ void tick_outputs(const LaunchArgs& a, const CellDev& c, const double* __restrict__ V,
-                                             int64_t stepNo, int64_t sample) {
+                                             int64_t stepNo) {
     char* arena = a.arena;
     const int tid = threadIdx.x, nt = blockDim.x;
     for (int sg = tid; sg < c.nSpike; sg += nt) {
@@ -838,10 +838,16 @@ __device__ __forceinline__ void tick_outputs(const LaunchArgs& a, const CellDev&
             fired[sg] = 0;
         }
     }
-    if (c.nProbe > 0 && sample >= 0 && sample < c.probeCap) {
-        const int32_t* pcmp = A_<const int32_t>(arena, c.oProbeCompt);
-        double* tr = A_<double>(arena, c.oProbeTrace);
-        for (int j = tid; j < c.nProbe; j += nt) tr[sample * c.nProbe + j] = V[pcmp[j]];
+    // probes sample every probeStride-th step (a Table on a slower clock
+    // tick, Table::process after the solver's tick), sample 0 being the
+    // first such step after probeBase
+    if (c.nProbe > 0 && stepNo % c.probeStride == 0) {
+        const int64_t sample = stepNo / c.probeStride - c.probeBase / c.probeStride - 1;
+        if (sample >= 0 && sample < c.probeCap) {
+            const int32_t* pcmp = A_<const int32_t>(arena, c.oProbeCompt);
+            double* tr = A_<double>(arena, c.oProbeTrace);
+            for (int j = tid; j < c.nProbe; j += nt) tr[sample * c.nProbe + j] = V[pcmp[j]];
+        }
     }
 }
 
@@ -991,8 +997,6 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
     const int nExt = c.nExt;
     const int nSpike = c.nSpike;
     const int nProbe = c.nProbe;
-    const int64_t probeCap = c.probeCap;
-    const int64_t probeBase = c.probeBase;
     // work arrays: D[n], B[n], T[2*nTerm], F[nFill]; HJ (slow groups, HBM)
     const int nWork = 2 * n + 2 * nTerm + c.nFill;
     double* gwork = A_<double>(arena, c.oWork);
@@ -1156,7 +1160,7 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
         const int64_t stepNo = step0 + k + 1;
         if (nSpike > 0 || nProbe > 0) {
             __syncthreads();   // V written by other threads
-            tick_outputs(a, c, v.V, stepNo, step0 + k - probeBase);
+            tick_outputs(a, c, v.V, stepNo);
         }
         // the next step's phase A overwrites D/B that other threads may still
         // read; V and the gate states are touched only by their owner thread
@@ -1540,7 +1544,7 @@ __global__ void __launch_bounds__(256) hsolve_msg_kernel(LaunchArgs a) {
         const int64_t stepNo = step0 + k + 1;
         if (c.nSpike > 0 || c.nProbe > 0) {
             __syncthreads();   // V written by other threads
-            tick_outputs(a, c, V, stepNo, step0 + k - c.probeBase);
+            tick_outputs(a, c, V, stepNo);
         }
         __syncthreads();
     }

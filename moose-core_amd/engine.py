@@ -21,7 +21,7 @@ EXPORTS = [
     "hsg_create", "hsg_destroy", "hsg_add_cell", "hsg_remove_cell", "hsg_reinit",
     "hsg_reinit_cell", "hsg_step", "hsg_step_cells", "hsg_get_field", "hsg_set_field",
     "hsg_add_inject", "hsg_add_gkek", "hsg_add_conc", "hsg_add_ica", "hsg_read_spikes",
-    "hsg_set_probes", "hsg_read_probes", "hsg_cell_info_get", "hsg_hines_order",
+    "hsg_set_probes", "hsg_set_probes_every", "hsg_read_probes", "hsg_cell_info_get", "hsg_hines_order",
     "hsg_sync", "hsg_last_step_timing", "hsg_last_error",
 ]
 
@@ -62,6 +62,7 @@ def load_library(path=None):
         "hsg_add_ica": ([vp, i32, i32, dbl], C.c_int),
         "hsg_read_spikes": ([vp, P(Spike), i32, P(i32), P(i64)], C.c_int),
         "hsg_set_probes": ([vp, i32, P(i32), i32, i64], C.c_int),
+        "hsg_set_probes_every": ([vp, i32, P(i32), i32, i64, i32], C.c_int),
         "hsg_read_probes": ([vp, i32, P(dbl), i64, P(i64)], C.c_int),
         "hsg_cell_info_get": ([vp, i32, P(CellInfo)], C.c_int),
         "hsg_hines_order": ([vp, i32, P(i32)], C.c_int),
@@ -204,9 +205,14 @@ class Engine:
         self.spikes_dropped = dropped.value
         return out[: n.value].copy()
 
-    def set_probes(self, cell, compartments, capacity):
+    def set_probes(self, cell, compartments, capacity, every=1):
+        """Vm of `compartments` sampled every `every` steps (after the step),
+        up to `capacity` samples until read_probes."""
         c, cp = _i32(compartments)
-        self._check(self.lib.hsg_set_probes(self.h, int(cell), cp, len(c), int(capacity)))
+        if every == 1:
+            self._check(self.lib.hsg_set_probes(self.h, int(cell), cp, len(c), int(capacity)))
+        else:
+            self._check(self.lib.hsg_set_probes_every(self.h, int(cell), cp, len(c), int(capacity), int(every)))
         self._probe_n = getattr(self, "_probe_n", {})
         self._probe_n[int(cell)] = (len(c), int(capacity))
 

@@ -42,6 +42,11 @@ _msgs = []                # (src, srcField, dest, destField) in creation order
 _solvers = []             # HSolve elements with a target
 _engines = {}             # (precision, dt) -> Engine
 _precision = ["fp64"]
+# Clock tick dts (scheduling/Clock.cpp:999-1016); Tables run on tick 8
+_CLOCK_DT = {t: 50e-6 for t in range(8)}
+_CLOCK_DT.update({8: 1e-4, 9: 0.0, 10: 0.01})
+_CLOCK_DT.update({t: 0.1 for t in range(11, 20)})
+_clock_dt = dict(_CLOCK_DT)
 
 
 def _norm(path):
@@ -532,11 +537,13 @@ class Leakage(Neutral):
 
 
 class Table(Neutral):
-    """Records a compartment's Vm each step (requestOut -> getVm) or a
-    SpikeGen's spike times (spikeOut -> spike)."""
+    """Records a compartment's Vm (requestOut -> getVm) on its clock tick
+    (8 unless moved with useClock), after the solver's tick of the same step
+    (Table::process, builtins/Table.cpp:254-285), plus one sample at reinit;
+    or a SpikeGen's spike times (spikeOut -> spike)."""
 
     def _init_fields(self):
-        self.__dict__.update(_data=[])
+        self.__dict__.update(_data=[], tick=8)
 
     @property
     def vector(self):
@@ -599,6 +606,8 @@ def reset():
     _objects.clear()
     _msgs.clear()
     _solvers.clear()
+    _clock_dt.clear()
+    _clock_dt.update(_CLOCK_DT)
     _next_id[0] = 1
     Neutral("/")
 
@@ -843,8 +852,25 @@ class HSolve(Neutral):
 
 
 def setClock(tick, dt):
-    """Accepted for compatibility; each HSolve advances at its own dt."""
-    return None
+    """Sets a clock tick's dt.  Each HSolve advances at its own dt; Tables
+    sample every round(tick dt / solver dt) solver steps."""
+    _clock_dt[int(tick)] = float(dt)
+
+
+def getClock(tick):
+    return _clock_dt[int(tick)]
+
+
+def useClock(tick, path, field="process"):
+    """Puts the Table(s) at `path` on clock `tick`."""
+    obj = element(path)
+    if isinstance(obj, Table):
+        obj.__dict__["tick"] = int(tick)
+
+
+def _table_every(table, dt):
+    tdt = _clock_dt.get(table.__dict__["tick"], dt)
+    return max(1, int(round(tdt / dt))) if tdt > 0 else 1
 
 
 def reinit():
@@ -884,17 +910,25 @@ def start(runtime):
         for hs in group:
             vm, spk = hs._recorders()
             comps = sorted({i for _, i in vm})
+            # one sampling interval per cell on the device: the Tables'
+            # common one, or every step with the Tables decimated here
+            evs = {_table_every(t, dt) for t, _ in vm}
+            every = evs.pop() if len(evs) == 1 else 1
             if comps:
-                eng.set_probes(hs._cell, comps, nsteps)
-            rec.append((hs, vm, spk, comps))
+                eng.set_probes(hs._cell, comps, nsteps // every + 1, every=every)
+            rec.append((hs, vm, spk, comps, every, hs.__dict__.get("_steps", 0)))
         eng.step(nsteps)
         spikes = eng.read_spikes()
-        for hs, vm, spk, comps in rec:
+        for hs, vm, spk, comps, every, s0 in rec:
             if comps:
                 tr = eng.read_probes(hs._cell)
                 col = {c: j for j, c in enumerate(comps)}
+                # row r is step (s0 // every + 1 + r) * every
+                steps = (s0 // every + 1 + np.arange(len(tr))) * every
                 for t, i in vm:
-                    t._data.extend(tr[:, col[i]].tolist())
+                    te = _table_every(t, dt)
+                    keep = steps % te == 0
+                    t._data.extend(tr[keep, col[i]].tolist())
             mine = spikes[spikes["cell"] == hs._cell]
             for t, g in spk:
                 t._data.extend((mine[mine["spikegen"] == g]["step"] * dt).tolist())

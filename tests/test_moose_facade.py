@@ -127,6 +127,7 @@ def test_layer_runs_on_the_gpu_like_the_oracle(pkg, oracle_mod):
     print("case: object layer, ball tree, fp64, 400 + 200 steps", flush=True)
     from moose_core_amd import moose
     comps, sg = build_ball_tree(moose, inject=0.6e-9)
+    moose.setClock(8, 50e-6)                    # Tables every solver step
     vm = moose.Table("/cell/vm")
     moose.connect(vm, "requestOut", comps[0], "getVm")
     spk = moose.Table("/cell/spikes")
@@ -242,6 +243,7 @@ def test_layer_hhchannel2d_on_the_gpu(pkg, oracle_mod):
     from moose_core_amd import moose
     comps, _ = build_ball_tree(moose, depth=3, inject=0.8e-9)
     pools, kcs = add_ca_and_kc(moose, comps, small_kc_tables())
+    moose.setClock(8, 50e-6)                    # Tables every solver step
     vm = moose.Table("/cell/vm")
     moose.connect(vm, "requestOut", comps[0], "getVm")
     hs = moose.HSolve("/cell/hsolve")
@@ -359,3 +361,59 @@ def test_setup_gate_form(pkg):
     # A = alpha, B = alpha + beta after the beta call
     np.testing.assert_allclose(g.tableA, np.ones(11))
     np.testing.assert_allclose(g.tableB, np.full(11, 4.0))
+
+
+@pytest.mark.gpu
+def test_table_default_clock_on_device(pkg, oracle_mod):
+    """A Table on tick 8 (default 0.1 ms) with a 50 us solver: the device
+    samples every second step (probe interval 2)."""
+    print("case: object layer, Table on tick 8 default (every 2nd step), fp64", flush=True)
+    from moose_core_amd import moose
+    comps, _ = build_ball_tree(moose, depth=3, inject=0.8e-9)
+    t8 = moose.Table("/cell/vm8")
+    moose.connect(t8, "requestOut", comps[2], "getVm")
+    hs = moose.HSolve("/cell/hsolve")
+    hs.dt = 50e-6
+    hs.target = "/cell"
+    spec = hs._spec
+    moose.reinit()
+    moose.start(61 * 50e-6)
+    moose.start(40 * 50e-6)
+    ref = oracle_mod.OracleCell(spec, 50e-6)
+    ref.reinit(50e-6)
+    tr = ref.run(50e-6, 101, np.arange(spec.n))
+    steps = np.arange(1, 102)
+    np.testing.assert_array_equal(t8.vector[1:], tr[steps % 2 == 0, 2])
+    assert len(t8.vector) == 1 + 50
+    moose.reset()
+
+
+@pytest.mark.gpu
+def test_tables_sample_on_their_clock(pkg, oracle_mod):
+    """Tables on tick 8 (default 0.1 ms) sample every second 50 us step,
+    after the solver's step, plus the reinit sample; a Table moved to
+    another tick samples at its own rate, across start() calls."""
+    print("case: object layer, Table clocks (tick 8 default, tick 9 at 0.25 ms), fp64", flush=True)
+    from moose_core_amd import moose
+    comps, _ = build_ball_tree(moose, depth=4, inject=0.8e-9)
+    t8 = moose.Table("/cell/vm8")
+    moose.connect(t8, "requestOut", comps[0], "getVm")
+    t9 = moose.Table("/cell/vm9")
+    moose.connect(t9, "requestOut", comps[3], "getVm")
+    moose.setClock(9, 250e-6)
+    moose.useClock(9, "/cell/vm9", "process")
+    hs = moose.HSolve("/cell/hsolve")
+    hs.dt = 50e-6
+    hs.target = "/cell"
+    spec = hs._spec
+    moose.reinit()
+    moose.start(101 * 50e-6)
+    moose.start(99 * 50e-6)
+    ref = oracle_mod.OracleCell(spec, 50e-6)
+    ref.reinit(50e-6)
+    tr = ref.run(50e-6, 200, np.arange(spec.n))
+    steps = np.arange(1, 201)
+    np.testing.assert_array_equal(t8.vector[1:], tr[steps % 2 == 0, 0])
+    np.testing.assert_array_equal(t9.vector[1:], tr[steps % 5 == 0, 3])
+    assert t8.vector[0] == spec.compartments["initVm"][0] and len(t8.vector) == 101
+    moose.reset()
