@@ -851,6 +851,136 @@ class HSolve(Neutral):
         return vm, spk
 
 
+# ---------------------------------------------------------------------------
+# SWC morphologies (biophysics/ReadSwc.cpp, biophysics/SwcSegment.cpp,
+# shell/LoadModels.cpp:156-166)
+# ---------------------------------------------------------------------------
+_SWC_TYPE_NAME = ["undef", "soma", "axon", "dend", "apical", "dend_f", "dend_e", "custom", "bad",
+                  "undef", "axon_f", "axon_e", "apical_f", "apical_e"]
+_SWC_MIN_RADIUS = 0.04        # microns (ReadSwc.cpp:23)
+_SWC_ZERO_LENGTH = 1e-2       # microns (ReadSwc::cleanZeroLength)
+
+
+def _read_swc(fname):
+    """ReadSwc::ReadSwc: the segments (index, type, x, y, z, radius, parent,
+    kids) after validation, kid assignment, type refinement and the removal
+    of zero-length segments; then the branches (runs between forks/ends) in
+    the reference's order.  Returns (segs, branches) or raises ValueError."""
+    segs = []
+    with open(fname) as f:
+        for line in f:
+            t = line.strip()
+            if not t or t.startswith("#"):
+                continue
+            a = t.split()
+            if len(a) != 7:
+                continue                     # a bad segment is skipped
+            pa = int(a[6])
+            segs.append({"i": int(a[0]), "type": int(a[1]),
+                         "v": (float(a[2]), float(a[3]), float(a[4])),
+                         "r": float(a[5]), "pa": pa if pa > 0 else None, "kids": [], "bad": False})
+    # validate (ReadSwc::validate)
+    starts = sum(1 for s in segs if s["pa"] is None)
+    orphans = sum(1 for k, s in enumerate(segs) if s["pa"] is not None and s["pa"] > k)
+    badr = sum(1 for s in segs if s["r"] < _SWC_MIN_RADIUS)
+    if not (starts == 1 and orphans == 0 and badr == 0):
+        raise ValueError(f"ReadSwc: {fname} failed validation (starts {starts}, orphans {orphans}, "
+                         f"radius < {_SWC_MIN_RADIUS}: {badr})")
+    # assignKids + figureOutType
+    for k, s in enumerate(segs):
+        if s["pa"] is not None:
+            segs[s["pa"] - 1]["kids"].append(k + 1)
+    for s in segs:
+        n = len(s["kids"])
+        for base, fork, end in ((3, 5, 6), (4, 12, 13), (2, 10, 11)):
+            if s["type"] == base:
+                s["type"] = fork if n > 1 else end if n == 0 else base
+                break
+
+    def dist(a, b):
+        return math.sqrt(sum((x - y) ** 2 for x, y in zip(a["v"], b["v"])))
+
+    # cleanZeroLength
+    for k in range(1, len(segs)):
+        s = segs[k]
+        pa = segs[s["pa"] - 1]
+        if dist(s, pa) < _SWC_ZERO_LENGTH:
+            temp = [c for c in pa["kids"] if c != s["i"]]
+            for c in s["kids"]:
+                segs[c - 1]["pa"] = pa["i"]
+                temp.append(c)
+            pa["kids"] = temp
+            s["bad"] = True
+    # SwcSegment::OK (SwcSegment.h:34-37): not bad, not undef, not custom
+    for s in segs:
+        s["ok"] = not s["bad"] and s["type"] not in (0, 7)
+    # parseBranches: one branch per fork or end, back to the previous fork
+    branches = []
+    for s in segs:
+        if s["ok"] and len(s["kids"]) != 1:
+            cable = [s["i"]]
+            if s["pa"] is not None:
+                prev = s
+                while True:
+                    pa = segs[prev["pa"] - 1]
+                    cable.append(pa["i"])
+                    prev = pa
+                    if not (prev["pa"] is not None and len(prev["kids"]) == 1):
+                        break
+                cable.pop()
+            branches.append(cable[::-1])
+    return segs, branches
+
+
+def loadModel(filename, modelpath, solverClass="ee"):
+    """moose.loadModel for .swc files (Shell::doLoadModel, LoadModels.cpp:
+    156-166): a Neuron at `modelpath` holding one Compartment per segment,
+    built by ReadSwc::build with lambda 0.5 mm, RM 1, RA 1, CM 0.01, in the
+    reference's creation and message order."""
+    if not filename.lower().endswith(".swc"):
+        raise ValueError("loadModel: only .swc morphologies are read by this layer")
+    segs, branches = _read_swc(filename)
+    parent = Neutral(modelpath)
+    RM, RA, CM = 1.0, 1.0, 0.01
+    compts = {}
+    for bi, cable in enumerate(branches):
+        for j, si in enumerate(cable):
+            seg = segs[si - 1]
+            if seg["pa"] is not None and seg["pa"] not in compts:
+                # the reference asserts here: the parent point lies on no
+                # branch (a soma with exactly one child ends no branch)
+                raise ValueError(f"ReadSwc::build: point {si} has no parent compartment "
+                                 f"(point {seg['pa']} ends no branch)")
+            pa = segs[seg["pa"] - 1] if seg["pa"] is not None else seg
+            compts[si] = _make_swc_compartment(parent, seg, pa, RM, RA, CM, bi, j)
+            if seg["pa"] is not None:
+                connect(compts[seg["pa"]], "axial", compts[si], "raxial")
+    return parent
+
+
+def _make_swc_compartment(parent, seg, pa, RM, RA, CM, i, j):
+    # makeCompt (ReadSwc.cpp:234-280)
+    length = seg["r"] * 2.0
+    name = "soma"
+    if seg["pa"] is not None:
+        length = math.sqrt(sum((x - y) ** 2 for x, y in zip(seg["v"], pa["v"])))
+        name = f"{_SWC_TYPE_NAME[seg['type']]}_{i}_{j}"
+        x0 = pa["v"]
+    else:
+        x0 = (seg["v"][0] - length, seg["v"][1], seg["v"][2])
+    c = Compartment(parent.path.rstrip("/") + "/" + name)
+    xa = seg["r"] * seg["r"] * math.pi * 1e-12
+    length *= 1e-6
+    dia = seg["r"] * 2.0e-6
+    d = c.__dict__
+    d["Rm"] = RM / (length * dia * math.pi)
+    d["Ra"] = RA * length / xa
+    d["Cm"] = CM * (length * dia * math.pi)
+    d.update(diameter=dia, length=length, x0=x0[0] * 1e-6, y0=x0[1] * 1e-6, z0=x0[2] * 1e-6,
+             x=seg["v"][0] * 1e-6, y=seg["v"][1] * 1e-6, z=seg["v"][2] * 1e-6)
+    return c
+
+
 def setClock(tick, dt):
     """Sets a clock tick's dt.  Each HSolve advances at its own dt; Tables
     sample every round(tick dt / solver dt) solver steps."""

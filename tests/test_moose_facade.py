@@ -417,3 +417,128 @@ def test_tables_sample_on_their_clock(pkg, oracle_mod):
     np.testing.assert_array_equal(t9.vector[1:], tr[steps % 5 == 0, 3])
     assert t8.vector[0] == spec.compartments["initVm"][0] and len(t8.vector) == 101
     moose.reset()
+
+
+# ---------------------------------------------------------------------------
+# SWC morphologies (biophysics/ReadSwc.cpp)
+# ---------------------------------------------------------------------------
+TINY_SWC = """# soma with a dendrite that forks (one side with a zero-length point) and an axon
+1 1 0 0 0 5.0 -1
+2 3 10 0 0 1.0 1
+3 3 20 0 0 1.0 2
+4 3 30 5 0 0.8 3
+5 3 30 -5 0 0.8 3
+6 3 30 -5 0.001 0.8 5
+7 3 40 -5 0 0.6 6
+8 2 -10 0 0 0.5 1
+"""
+
+
+def test_swc_reader_structure(pkg, tmp_path):
+    """Hand-traced ReadSwc on a small tree: the zero-length point 6 (1 nm from
+    5) is dropped and 7 reparented to 5; branches are the runs ending at each
+    fork or end, in segment order; names are <type>_<branch>_<position>;
+    the soma is a cylinder of length 2r; compartments and axial messages are
+    created branch by branch."""
+    from moose_core_amd import moose
+    moose.reset()
+    f = tmp_path / "tiny.swc"
+    f.write_text(TINY_SWC)
+    segs, branches = moose._read_swc(str(f))
+    assert not segs[5]["ok"] and segs[6]["pa"] == 5 and segs[4]["kids"] == [7]
+    assert branches == [[1], [2, 3], [4], [5, 7], [8]]
+    moose.loadModel(str(f), "/cell")
+    names = [c.name for c in moose.element("/cell").children]
+    assert names == ["soma", "dend_1_0", "dend_f_1_1", "dend_e_2_0", "dend_3_0", "dend_e_3_1", "axon_e_4_0"]
+    soma = moose.element("/cell/soma")
+    assert soma.length == pytest.approx(10e-6) and soma.diameter == pytest.approx(10e-6)
+    # cylinder formulas with RM = 1, RA = 1, CM = 0.01 (LoadModels.cpp:165)
+    d = moose.element("/cell/dend_e_3_1")      # segment 7, 10 um from 5
+    L, dia = 10e-6, 1.2e-6
+    assert d.Rm == pytest.approx(1.0 / (L * dia * np.pi), rel=1e-12)
+    assert d.Ra == pytest.approx(L / (np.pi * (dia / 2) ** 2), rel=1e-12)
+    assert d.Cm == pytest.approx(0.01 * L * dia * np.pi, rel=1e-12)
+    axial = [(s.name, t.name) for s, sf, t, tf in moose._msgs if sf == "axial"]
+    assert axial == [("soma", "dend_1_0"), ("dend_1_0", "dend_f_1_1"), ("dend_f_1_1", "dend_e_2_0"),
+                     ("dend_f_1_1", "dend_3_0"), ("dend_3_0", "dend_e_3_1"), ("soma", "axon_e_4_0")]
+    one_kid = tmp_path / "one_kid.swc"         # the soma ends no branch: no parent compartment
+    one_kid.write_text("1 1 0 0 0 5 -1\n2 3 10 0 0 1 1\n3 3 20 0 0 1 2\n")
+    moose.reset()
+    with pytest.raises(ValueError, match="no parent compartment"):
+        moose.loadModel(str(one_kid), "/c1")
+    bad = tmp_path / "two_roots.swc"
+    bad.write_text("1 1 0 0 0 5 -1\n2 3 10 0 0 1 -1\n")
+    with pytest.raises(ValueError):
+        moose._read_swc(str(bad))
+
+
+def test_swc_h10_under_hsolve(pkg, oracle_mod):
+    """The reference's h10 test morphology: every valid point a compartment,
+    one tree, taken over by an HSolve (passive, Em = -65 mV) and run by the
+    oracle to the passive steady state of a soma injection."""
+    import os
+    from moose_core_amd import moose
+    moose.reset()
+    path = os.path.join(os.path.dirname(__file__), "golden", "h10.CNG.swc")
+    segs, branches = moose._read_swc(path)
+    cell = moose.loadModel(path, "/h10")
+    comps = [c for c in cell.children if isinstance(c, moose.Compartment)]
+    assert len(comps) == sum(1 for s in segs if s["ok"])
+    assert sum(len(b) for b in branches) == len(comps)
+    for c in comps:
+        c.Em = c.initVm = c.Vm = -0.065
+    comps[0].inject = 0.1e-9
+    hs = moose.HSolve("/h10/hsolve")
+    hs.dt = 50e-6
+    spec, _ = hs.build_spec(moose._deep_search_compartment(cell))
+    assert spec.n == len(comps) and len(spec.axial) == len(comps) - 1
+    o = oracle_mod.OracleCell(spec, 50e-6)
+    o.reinit(50e-6)
+    tr = o.run(50e-6, 4000, [0])
+    assert -0.065 < tr[-1, 0] < 0.0 and abs(tr[-1, 0] - tr[-2, 0]) < 1e-9
+    moose.reset()
+
+
+@pytest.mark.gpu
+def test_swc_h10_hh_on_the_gpu(pkg, oracle_mod):
+    """h10 with squid HH channels copied onto every compartment (Gbar by
+    area), run through HSolve on the GPU: bit-identical to the oracle."""
+    print("case: h10.CNG.swc with HH channels, fp64, 400 steps", flush=True)
+    import os
+    from moose_core_amd import moose
+    moose.reset()
+    moose.Neutral("/library")
+    gates = workloads.hh_gates()
+    na = moose.HHChannel("/library/Na")
+    na.Ek, na.Xpower, na.Ypower = 0.050, 3.0, 1.0
+    k = moose.HHChannel("/library/K")
+    k.Ek, k.Xpower = -0.077, 4.0
+    for gate, g in ((na.gateX, gates[0]), (na.gateY, gates[1]), (k.gateX, gates[2])):
+        gate.min, gate.max = g.min, g.max
+        gate.tableA = g.tableA
+        gate.tableB = g.tableB
+    cell = moose.loadModel(os.path.join(os.path.dirname(__file__), "golden", "h10.CNG.swc"), "/h10")
+    comps = [c for c in cell.children if isinstance(c, moose.Compartment)]
+    for c in comps:
+        c.Em = c.initVm = c.Vm = -0.065
+        area = np.pi * c.diameter * c.length
+        for proto, gbar in ((na, 1200.0), (k, 360.0)):
+            ch = moose.copy(proto, c, proto.name)
+            ch.Gbar = gbar * area
+            moose.connect(ch, "channel", c, "channel")
+    comps[0].inject = 2e-9
+    moose.setClock(8, 50e-6)
+    vm = moose.Table("/h10/vm")
+    moose.connect(vm, "requestOut", comps[0], "getVm")
+    hs = moose.HSolve("/h10/hsolve")
+    hs.dt = 50e-6
+    hs.target = "/h10"
+    moose.reinit()
+    moose.start(400 * 50e-6)
+    ref = oracle_mod.OracleCell(hs._spec, 50e-6)
+    ref.reinit(50e-6)
+    tr = ref.run(50e-6, 400, np.arange(hs._spec.n))
+    soma = hs._objs["compartments"].index(comps[0])
+    np.testing.assert_array_equal(vm.vector[1:], tr[:, soma])
+    assert vm.vector.max() > 0.0                # it fires
+    moose.reset()
