@@ -69,6 +69,14 @@ def build_large_ca(ncells, seed, nmin=5000, nmax=30000):
     return cells
 
 
+def build_uniform_hh(ncells, seed, n=2048):
+    """ncells random binary-tree HH cells of exactly n compartments."""
+    rng = np.random.default_rng(seed)
+    cells = [workloads.random_hh_cell(rng, n) for _ in range(ncells)]
+    log(f"[bench] built {ncells} HH cells of {n} compartments")
+    return cells
+
+
 # workload name -> (description, default cells, dt, default steps, steps per
 # launch, graph chunk)
 WORKLOADS = {
@@ -77,6 +85,8 @@ WORKLOADS = {
     "large_ca": ("large uneven Ca cells: 256 random trees (branching 2-4), n~U{5000..30000}, Na/Kdr/KA/CaL + "
                  "2-D K-AHP (Vm x [Ca], 3000 x 500), one CaConc per compartment, dt=20us, 10000 steps, "
                  "graphs of 100 steps", 256, 20e-6, 10000, 100, 1),
+    "hh_2048": ("uniform HH cells: 1953 random binary trees of 2048 compartments (4.0 M), dt=25us, "
+                "1000 steps", 1953, 25e-6, 1000, 50, 0),
 }
 
 
@@ -264,6 +274,8 @@ def main():
     args.graph_chunk = dgraph if args.graph_chunk is None else args.graph_chunk
     if args.workload == "ensemble":
         cells = build_cells(args.cells, args.seed)
+    elif args.workload == "hh_2048":
+        cells = build_uniform_hh(args.cells, args.seed)
     else:
         cells = build_large_ca(args.cells, args.seed)
     packed = [c.pack() for c in cells]
@@ -276,11 +288,12 @@ def main():
     wall, launches, total_ms, mean_ms, soma, spikes = res[head]
     value = ncomp * args.steps / wall
     # dominant kernel: the fused step kernel; algorithmic bytes per launch
-    bpcs = BYTES_PER_CS if args.workload == "ensemble" else BYTES_PER_CS_LARGE
+    bpcs = BYTES_PER_CS_LARGE if args.workload == "large_ca" else BYTES_PER_CS
     steps_per_launch = args.steps / max(1, launches)
     bytes_per_launch = bpcs[head] * ncomp * steps_per_launch
     achieved = bytes_per_launch / (mean_ms * 1e-3) / 1e9
-    wl_key = f"hh_ensemble_{args.cells}" if args.workload == "ensemble" else f"large_ca_{args.cells}"
+    wl_key = {"ensemble": f"hh_ensemble_{args.cells}", "large_ca": f"large_ca_{args.cells}",
+              "hh_2048": f"hh_2048_{args.cells}"}[args.workload]
     traffic = measured_traffic(f"hsolve_step_kernel/{head}/{wl_key}", ncomp * steps_per_launch)
     out = {
         "metric": "compartment-steps/s (batched cells, FP32-channel mixed form)",

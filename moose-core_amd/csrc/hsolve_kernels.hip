@@ -1169,15 +1169,26 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
     if (tid == 0) *stepCounter = step0 + a.nsteps;
 }
 
-// SLOWG: the cells of the launch have junction groups of four or more
-// members (see fwd_group_regs).
-template <typename GT, bool SLOWG>
-__global__ void __launch_bounds__(256) hsolve_step_kernel(LaunchArgs a) {
+// The fused kernel.  Its register budget is that of five waves per SIMD
+// (measured: the occupancy gained outweighs the few spills it costs).
+template <typename GT>
+__global__ void __launch_bounds__(256, 5) hsolve_step_kernel(LaunchArgs a) {
     extern __shared__ double lds[];
     __shared__ ChanType sTypes[kMaxSlots];
     const CellDev& c = a.cells[a.order[blockIdx.x]];
-    if (c.workInLds) cell_steps<GT, false, true, SLOWG, 0>(a, c, lds, sTypes);
-    else cell_steps<GT, false, false, SLOWG, 0>(a, c, lds, sTypes);
+    if (c.workInLds) cell_steps<GT, false, true, false, 0>(a, c, lds, sTypes);
+    else cell_steps<GT, false, false, false, 0>(a, c, lds, sTypes);
+}
+
+// The fused kernel for cells with junction groups of four or more members
+// (see fwd_group_regs).
+template <typename GT>
+__global__ void __launch_bounds__(256) hsolve_step_slow_kernel(LaunchArgs a) {
+    extern __shared__ double lds[];
+    __shared__ ChanType sTypes[kMaxSlots];
+    const CellDev& c = a.cells[a.order[blockIdx.x]];
+    if (c.workInLds) cell_steps<GT, false, true, true, 0>(a, c, lds, sTypes);
+    else cell_steps<GT, false, false, true, 0>(a, c, lds, sTypes);
 }
 
 // Uniform two-slot cells with many compartments per thread: phase A
@@ -1617,10 +1628,10 @@ extern "C" hipError_t hsg_launch_step(const hsg::LaunchArgs* a, int nblocks, int
     if (fastms == 2 && !slow) {
         if (fp32) hipLaunchKernelGGL((hsg::hsolve_step_fast_kernel<float>), g, b, l, stream, *a);
         else hipLaunchKernelGGL((hsg::hsolve_step_fast_kernel<double>), g, b, l, stream, *a);
-    } else if (fp32 && slow) hipLaunchKernelGGL((hsg::hsolve_step_kernel<float, true>), g, b, l, stream, *a);
-    else if (fp32) hipLaunchKernelGGL((hsg::hsolve_step_kernel<float, false>), g, b, l, stream, *a);
-    else if (slow) hipLaunchKernelGGL((hsg::hsolve_step_kernel<double, true>), g, b, l, stream, *a);
-    else hipLaunchKernelGGL((hsg::hsolve_step_kernel<double, false>), g, b, l, stream, *a);
+    } else if (fp32 && slow) hipLaunchKernelGGL((hsg::hsolve_step_slow_kernel<float>), g, b, l, stream, *a);
+    else if (fp32) hipLaunchKernelGGL((hsg::hsolve_step_kernel<float>), g, b, l, stream, *a);
+    else if (slow) hipLaunchKernelGGL((hsg::hsolve_step_slow_kernel<double>), g, b, l, stream, *a);
+    else hipLaunchKernelGGL((hsg::hsolve_step_kernel<double>), g, b, l, stream, *a);
     return hipGetLastError();
 }
 
@@ -1690,8 +1701,8 @@ extern "C" hipError_t hsg_launch_chan2d_reinit(const hsg::LaunchArgs* a, int nbl
 
 extern "C" hipError_t hsg_set_max_lds(int bytes) {
     const void* kernels[12] = {
-        (const void*)hsg::hsolve_step_kernel<double, false>, (const void*)hsg::hsolve_step_kernel<float, false>,
-        (const void*)hsg::hsolve_step_kernel<double, true>, (const void*)hsg::hsolve_step_kernel<float, true>,
+        (const void*)hsg::hsolve_step_kernel<double>, (const void*)hsg::hsolve_step_kernel<float>,
+        (const void*)hsg::hsolve_step_slow_kernel<double>, (const void*)hsg::hsolve_step_slow_kernel<float>,
         (const void*)hsg::hsolve_step_fast_kernel<double>, (const void*)hsg::hsolve_step_fast_kernel<float>,
         (const void*)hsg::hsolve_solve_kernel<double, false>, (const void*)hsg::hsolve_solve_kernel<float, false>,
         (const void*)hsg::hsolve_solve_kernel<double, true>, (const void*)hsg::hsolve_solve_kernel<float, true>,
