@@ -174,7 +174,7 @@ typedef struct hsg_options {
                                  limit.  hsg_add_cell fails with HSG_E_NOMEM beyond */
     int32_t steps_per_launch; /* steps fused into one kernel launch (0 = default)   */
     int32_t graph_chunk;      /* launches captured per hipGraph (0 = no graph)      */
-    int32_t block_threads;    /* threads per cell workgroup (0 = autotuned)         */
+    int32_t block_threads;    /* threads per cell workgroup, 64-256 (0 = by size)   */
     int32_t kernel_mode;      /* HSG_KERNEL_*: how a tick is split into kernels     */
 } hsg_options;
 

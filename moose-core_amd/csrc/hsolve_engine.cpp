@@ -828,7 +828,9 @@ int make_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg_
         b.chanMS = allUniform ? (maxSlots <= 2 ? 2 : maxSlots <= 4 ? 4 : 8) : 0;
         b.sumN = 0;
         for (int id : b.cells) b.sumN += e->cells[id]->dev.n;
-        b.threads = e->opt.block_threads > 0 ? std::max(64, std::min(1024, (e->opt.block_threads + 63) / 64 * 64))
+        // the per-cell kernels are built for at most 256 threads per
+        // workgroup (wider ones measured no faster, 512 slower)
+        b.threads = e->opt.block_threads > 0 ? std::max(64, std::min(256, (e->opt.block_threads + 63) / 64 * 64))
                                              : default_threads(maxN);
         HIPCHK(hipMalloc(&b.dOrder, sizeof(int32_t) * b.cells.size()));
         HIPCHK(hipMemcpyAsync(b.dOrder, b.cells.data(), sizeof(int32_t) * b.cells.size(), hipMemcpyHostToDevice,
@@ -862,12 +864,6 @@ LaunchArgs make_args(hsg_engine* e, const int32_t* order, int nsteps, int first)
     a.ldsBytes = 0;
     a.dt = e->dt;
     return a;
-}
-
-int block_threads(hsg_engine* e) {
-    int t = e->opt.block_threads > 0 ? e->opt.block_threads : 256;
-    t = std::max(64, std::min(1024, (t + 63) / 64 * 64));
-    return t;
 }
 
 // flushes one-shot inputs of every live cell to the device
