@@ -122,7 +122,7 @@ class Compartment(Neutral):
         d = self.__dict__
         # biophysics/CompartmentBase.cpp defaults
         d.update(Vm=-0.06, Cm=1.0, Em=-0.06, initVm=-0.06, Rm=1.0, Ra=1.0, inject=0.0, Im=0.0,
-                 length=0.0, diameter=0.0)
+                 length=0.0, diameter=0.0, x0=0.0, y0=0.0, z0=0.0, x=0.0, y=0.0, z=0.0)
 
 
 class SymCompartment(Compartment):
@@ -933,12 +933,16 @@ def _read_swc(fname):
 
 
 def loadModel(filename, modelpath, solverClass="ee"):
-    """moose.loadModel for .swc files (Shell::doLoadModel, LoadModels.cpp:
-    156-166): a Neuron at `modelpath` holding one Compartment per segment,
-    built by ReadSwc::build with lambda 0.5 mm, RM 1, RA 1, CM 0.01, in the
-    reference's creation and message order."""
-    if not filename.lower().endswith(".swc"):
-        raise ValueError("loadModel: only .swc morphologies are read by this layer")
+    """moose.loadModel for cell morphologies (Shell::doLoadModel,
+    shell/LoadModels.cpp:150-166): a Neuron at `modelpath`, filled from a
+    GENESIS .p file (ReadCell) or an SWC file (ReadSwc: one Compartment per
+    point, lambda 0.5 mm, RM 1, RA 1, CM 0.01), in the reference's creation
+    and message order."""
+    low = filename.lower()
+    if low.endswith(".p"):
+        return _ReadCell().read(filename, modelpath)
+    if not low.endswith(".swc"):
+        raise ValueError("loadModel: this layer reads .p and .swc cell files")
     segs, branches = _read_swc(filename)
     parent = Neutral(modelpath)
     RM, RA, CM = 1.0, 1.0, 0.01
@@ -979,6 +983,216 @@ def _make_swc_compartment(parent, seg, pa, RM, RA, CM, i, j):
     d.update(diameter=dia, length=length, x0=x0[0] * 1e-6, y0=x0[1] * 1e-6, z0=x0[2] * 1e-6,
              x=seg["v"][0] * 1e-6, y=seg["v"][1] * 1e-6, z=seg["v"][2] * 1e-6)
     return c
+
+
+# ---------------------------------------------------------------------------
+# GENESIS .p cell files (biophysics/ReadCell.cpp)
+# ---------------------------------------------------------------------------
+def _calc_surf(length, dia):
+    # calcSurf (ReadCell.cpp:617-626): a sphere when the length is zero
+    return dia * dia * math.pi if length == 0.0 else length * dia * math.pi
+
+
+class _ReadCell:
+    """ReadCell (biophysics/ReadCell.cpp): the GENESIS cell reader.  Supported:
+    comments (// and /* */), *cartesian/*polar, *relative/*absolute,
+    *asymmetric, *set_global/*set_compt_param (RM RA CM EREST_ACT ELEAK),
+    *start_cell (no graft), *double_endpoint[_off], *makeproto; compartment
+    lines `name parent x y z d [chan dens]...` with the parent '.', 'none'
+    or 'nil', the RA/RM/CM/Ra/Rm/Cm overrides, and channels, SpikeGens and
+    CaConcs copied from /library with their densities scaled as the
+    reference scales them.  Not supported (error): *symmetric
+    (SymCompartment messages), grafted cells, *compt prototypes; the
+    GENESIS 'addmsg' fields of prototypes are not applied."""
+
+    def __init__(self):
+        # ReadCell::ReadCell (ReadCell.cpp:22-105)
+        self.RM, self.CM, self.RA = 10.0, 0.01, 1.0
+        self.EREST_ACT, self.ELEAK = -0.065, -0.065
+        self.erestFlag = self.eleakFlag = False
+        self.polar = self.relative = self.doubleEnd = self.symmetric = False
+        self.last = None
+        lib = _objects.get("/library")
+        self.protos = {c.name: c for c in lib.children} if lib is not None else {}
+
+    def read(self, fname, path):
+        # ReadCell::read / innerRead (ReadCell.cpp:107-219)
+        self.cell = Neutral(path)
+        mode = "data"
+        with open(fname) as f:
+            for raw in f:
+                line = raw.strip()
+                if not line:
+                    continue
+                if line.startswith("//"):
+                    continue
+                if "//" in line:
+                    line = line[:line.index("//")]
+                if line.startswith("/*"):
+                    mode = "comment"
+                elif "*/" in line:
+                    mode = "data"
+                    continue
+                elif line[0] == "*":
+                    mode = "script"
+                if mode == "comment":
+                    pos = line.find("*/")
+                    if pos >= 0:
+                        mode = "data"
+                        if len(line) > pos + 2:
+                            line = line[pos + 2:]
+                if mode == "data":
+                    self._data(line.split())
+                elif mode == "script":
+                    self._script(line.split())
+                    mode = "data"
+        return self.cell
+
+    def _script(self, argv):
+        # ReadCell::readScript (ReadCell.cpp:220-306)
+        cmd = argv[0]
+        if cmd == "*cartesian":
+            self.polar = False
+        elif cmd == "*polar":
+            self.polar = True
+        elif cmd == "*relative":
+            self.relative = True
+        elif cmd == "*absolute":
+            self.relative = False
+        elif cmd == "*symmetric":
+            raise ValueError("ReadCell: *symmetric (SymCompartment) cells are not supported by this layer")
+        elif cmd == "*asymmetric":
+            self.symmetric = False
+        elif cmd in ("*set_global", "*set_compt_param"):
+            if len(argv) != 3:
+                return
+            v = float(argv[2])
+            if argv[1] == "RM":
+                self.RM = v
+            if argv[1] == "RA":
+                self.RA = v
+            if argv[1] == "CM":
+                self.CM = v
+            if argv[1] == "EREST_ACT":
+                self.EREST_ACT, self.erestFlag = v, True
+            if argv[1] == "ELEAK":
+                self.ELEAK, self.eleakFlag = v, True
+        elif cmd == "*start_cell":
+            if len(argv) != 1:
+                raise ValueError("ReadCell: grafted cells (*start_cell <path>) are not supported by this layer")
+        elif cmd == "*compt":
+            raise ValueError("ReadCell: *compt prototypes are not supported by this layer")
+        elif cmd == "*double_endpoint":
+            self.doubleEnd = True
+        elif cmd == "*double_endpoint_off":
+            self.doubleEnd = False
+
+    def _coords(self, a, b, c):
+        x, y, z = 1.0e-6 * float(a), float(b), float(c)
+        if self.polar:
+            r, th, ph = x, y * math.pi / 180.0, z * math.pi / 180.0
+            return r * math.sin(ph) * math.cos(th), r * math.sin(ph) * math.sin(th), r * math.cos(ph)
+        return x, y * 1.0e-6, z * 1.0e-6
+
+    def _data(self, argv):
+        # ReadCell::readData (ReadCell.cpp:308-375)
+        if len(argv) < 6:
+            return
+        off = 3 if self.doubleEnd else 0
+        x0 = y0 = z0 = 0.0
+        if self.doubleEnd:
+            x0, y0, z0 = self._coords(argv[2], argv[3], argv[4])
+        x, y, z = self._coords(argv[off + 2], argv[off + 3], argv[off + 4])
+        d = 1.0e-6 * float(argv[off + 5])
+        compt, length = self._compartment(argv[0], argv[1], x0, y0, z0, x, y, z, d)
+        self._channels(compt, argv, d, length)
+
+    def _compartment(self, name, parent, x0, y0, z0, x, y, z, d):
+        # ReadCell::buildCompartment (ReadCell.cpp:377-555)
+        if parent == ".":
+            pa = self.last
+        elif parent in ("none", "nil"):
+            pa = None
+        else:
+            pp = _norm(self.cell.path + "/" + parent)
+            if pp not in _objects:
+                raise ValueError(f"ReadCell: could not find parent compt '{parent}' for child '{name}'")
+            pa = _objects[pp]
+        c = Compartment(self.cell.path.rstrip("/") + "/" + name)
+        self.last = c
+        if pa is not None:
+            px, py, pz = pa.__dict__["x"], pa.__dict__["y"], pa.__dict__["z"]
+            if not self.doubleEnd:
+                x0, y0, z0 = px, py, pz
+            if self.relative:
+                x, y, z = x + px, y + py, z + pz
+                if self.doubleEnd:
+                    x0, y0, z0 = x0 + px, y0 + py, z0 + pz
+            dx, dy, dz = x - x0, y - y0, z - z0
+            length = math.sqrt(dx * dx + dy * dy + dz * dz)
+            connect(pa, "axial", c, "raxial")
+        else:
+            length = math.sqrt(x * x + y * y + z * z)
+        surf = _calc_surf(length, d)
+        Ra = self.RA * length * 4.0 / (d * d * math.pi) if length > 0 else self.RA * 8.0 / (d * math.pi)
+        eleak = self.EREST_ACT if (self.erestFlag and not self.eleakFlag) else self.ELEAK
+        erest = self.ELEAK if (not self.erestFlag and self.eleakFlag) else self.EREST_ACT
+        c.__dict__.update(x0=x0, y0=y0, z0=z0, x=x, y=y, z=z, diameter=d, length=length,
+                          Rm=self.RM / surf, Ra=Ra, Cm=self.CM * surf, initVm=erest, Em=eleak, Vm=erest)
+        return c, length
+
+    def _channels(self, compt, argv, dia, length):
+        # ReadCell::buildChannels / addChannel / addCanonicalChannel /
+        # addSpikeGen / addCaConc (ReadCell.cpp:628-890)
+        start = 9 if self.doubleEnd else 6
+        if (len(argv) % 2) != (1 if self.doubleEnd else 0):
+            return
+        j = start
+        while j < len(argv):
+            chan = argv[j]
+            j += 1
+            value = float(argv[j])
+            cd = compt.__dict__
+            if chan == "RA":
+                cd["Ra"] = (8.0 * value / (dia * math.pi) if length == 0.0
+                            else 4.0 * value * length / (dia * dia * math.pi))
+            elif chan == "RM":
+                cd["Rm"] = value * _calc_surf(length, dia)
+            elif chan == "CM":
+                cd["Cm"] = value * _calc_surf(length, dia)
+            elif chan in ("Rm", "Ra", "Cm"):
+                cd[chan] = value
+            elif chan == "kinModel":
+                j += 2
+            elif chan in ("m2c", "c2m"):
+                j += 4
+            elif chan in self.protos:
+                proto = self.protos[chan]
+                cp = copy(proto, compt, proto.name)
+                if isinstance(proto, (HHChannel, HHChannel2D)):
+                    connect(compt, "channel", cp, "channel")
+                    cp.__dict__["Gbar"] = value * _calc_surf(length, dia) if value > 0 else -value
+                elif isinstance(proto, SpikeGen):
+                    connect(compt, "VmOut", cp, "Vm")
+                    cp.__dict__["threshold"] = value
+                elif isinstance(proto, CaConc):
+                    thick = cp.__dict__["thick"]
+                    if thick > dia / 2.0:
+                        thick = 0.0
+                    if value > 0.0:
+                        if length > 0.0:
+                            vol = (math.pi * length * (dia - thick) * thick if thick > 0.0
+                                   else dia * dia * math.pi * length / 4.0)
+                        else:
+                            inner = dia - 2 * thick
+                            vol = (math.pi * (dia ** 3 - inner ** 3) / 6.0 if thick > 0.0
+                                   else math.pi * dia ** 3 / 6.0)
+                        if vol > 0.0:
+                            value /= vol
+                    else:
+                        value = -value
+                    cp.__dict__["B"] = value
+            j += 1
 
 
 def setClock(tick, dt):

@@ -542,3 +542,131 @@ def test_swc_h10_hh_on_the_gpu(pkg, oracle_mod):
     np.testing.assert_array_equal(vm.vector[1:], tr[:, soma])
     assert vm.vector.max() > 0.0                # it fires
     moose.reset()
+
+
+# ---------------------------------------------------------------------------
+# GENESIS .p cell files (biophysics/ReadCell.cpp)
+# ---------------------------------------------------------------------------
+def _golden(name):
+    import os
+    return os.path.join(os.path.dirname(__file__), "golden", name)
+
+
+def test_readcell_soma_p(pkg):
+    """The reference's tests/core/soma.p: globals RA 4, CM 0.03, RM 2.8,
+    EREST_ACT -80 mV, ELEAK -50 mV; one 16 x 16 um root compartment."""
+    from moose_core_amd import moose
+    moose.reset()
+    moose.Neutral("/library")
+    cell = moose.loadModel(_golden("soma.p"), "/neuron")
+    (soma,) = cell.children
+    L = d = 16e-6
+    surf = L * d * np.pi
+    assert soma.name == "soma" and soma.length == pytest.approx(L) and soma.diameter == pytest.approx(d)
+    assert soma.Cm == pytest.approx(0.03 * surf, rel=1e-12)
+    assert soma.Rm == pytest.approx(2.8 / surf, rel=1e-12)
+    assert soma.Ra == pytest.approx(4.0 * L * 4.0 / (d * d * np.pi), rel=1e-12)
+    assert soma.initVm == -0.080 and soma.Vm == -0.080 and soma.Em == -0.050
+
+
+def test_readcell_ca1_minimal(pkg):
+    """rdesigneur's ca1_minimal.p: *relative *cartesian coordinates, the '.'
+    parent, EREST_ACT only (so Em = initVm = -60 mV), lengths from the
+    relative offsets."""
+    from moose_core_amd import moose
+    moose.reset()
+    moose.Neutral("/library")
+    cell = moose.loadModel(_golden("ca1_minimal.p"), "/ca1")
+    names = [c.name for c in cell.children]
+    assert names[:6] == ["dend_3", "dend_5", "dend_6", "dend_8", "soma", "apical_10"]
+    soma = moose.element("/ca1/soma")
+    assert soma.y == pytest.approx((3 * 220 + 110 + 125) * 1e-6)     # chain of relative offsets
+    assert soma.length == pytest.approx(125e-6) and soma.diameter == pytest.approx(8.46e-6)
+    a14_2 = moose.element("/ca1/apical_14_2")                      # parent '.' = apical_14_1
+    assert a14_2.length == pytest.approx(10e-6)
+    for c in cell.children:
+        assert c.Em == -0.06 and c.initVm == -0.06
+        assert c.Cm == pytest.approx(0.03 * c.length * c.diameter * np.pi, rel=1e-12)
+    axial = [(s.name, t.name) for s, sf, t, tf in moose._msgs if sf == "axial"]
+    assert ("dend_8", "soma") in axial and ("apical_13_2", "apical_14_1") in axial
+    assert len(axial) == len(names) - 1
+    with pytest.raises(ValueError):
+        moose.loadModel(_golden("h10.CNG.swc")[:-4] + ".txt", "/x")
+
+
+P_FILE = """// a soma with HH channels and two dendrites
+*cartesian
+*relative
+*asymmetric
+*set_global RM 1.0
+*set_global RA 1.0
+*set_global CM 0.01
+*set_global EREST_ACT -0.065
+soma none 20 0 0 20 Na 1200 K 360 spike 0.0
+dend1 soma 50 0 0 2 Na 120 K 36
+dend2 . 50 0 0 1.5 RA 2.0 Na -1e-9 K 36
+dend3 soma 0 40 0 2 K 36
+"""
+
+
+def build_p_cell(moose, tmp_path, inject=1e-9):
+    moose.reset()
+    moose.Neutral("/library")
+    gates = workloads.hh_gates()
+    na = moose.HHChannel("/library/Na")
+    na.Ek, na.Xpower, na.Ypower = 0.050, 3.0, 1.0
+    k = moose.HHChannel("/library/K")
+    k.Ek, k.Xpower = -0.077, 4.0
+    for gate, g in ((na.gateX, gates[0]), (na.gateY, gates[1]), (k.gateX, gates[2])):
+        gate.min, gate.max = g.min, g.max
+        gate.tableA = g.tableA
+        gate.tableB = g.tableB
+    sg = moose.SpikeGen("/library/spike")
+    sg.refractT = 0.0
+    f = tmp_path / "cell.p"
+    f.write_text(P_FILE)
+    cell = moose.loadModel(str(f), "/pcell")
+    moose.element("/pcell/soma").inject = inject
+    return cell
+
+
+def test_readcell_channels(pkg, tmp_path):
+    from moose_core_amd import moose
+    cell = build_p_cell(moose, tmp_path)
+    soma = moose.element("/pcell/soma")
+    surf = 20e-6 * 20e-6 * np.pi
+    assert moose.element("/pcell/soma/Na").Gbar == pytest.approx(1200 * surf, rel=1e-12)
+    assert moose.element("/pcell/soma/spike").threshold == 0.0
+    d2 = moose.element("/pcell/dend2")
+    assert moose.element("/pcell/dend2/Na").Gbar == 1e-9               # negative: absolute
+    assert d2.Ra == pytest.approx(4.0 * 2.0 * 50e-6 / (1.5e-6 ** 2 * np.pi), rel=1e-12)
+    hs = moose.HSolve("/pcell/hsolve")
+    hs.dt = 50e-6
+    spec, objs = hs.build_spec(moose._deep_search_compartment(cell))
+    assert spec.n == 4 and len(spec.channels) == 7 and len(spec.spikegens) == 1
+
+
+@pytest.mark.gpu
+def test_readcell_cell_on_the_gpu(pkg, oracle_mod, tmp_path):
+    print("case: .p cell with HH channels and a SpikeGen, fp64, 600 steps", flush=True)
+    from moose_core_amd import moose
+    cell = build_p_cell(moose, tmp_path, inject=1.5e-10)
+    moose.setClock(8, 50e-6)
+    vm = moose.Table("/pcell/vm")
+    moose.connect(vm, "requestOut", moose.element("/pcell/soma"), "getVm")
+    spk = moose.Table("/pcell/spk")
+    moose.connect(moose.element("/pcell/soma/spike"), "spikeOut", spk, "spike")
+    hs = moose.HSolve("/pcell/hsolve")
+    hs.dt = 50e-6
+    hs.target = "/pcell"
+    moose.reinit()
+    moose.start(600 * 50e-6)
+    ref = oracle_mod.OracleCell(hs._spec, 50e-6)
+    ref.reinit(50e-6)
+    tr = ref.run(50e-6, 600, np.arange(hs._spec.n))
+    soma = hs._objs["compartments"].index(moose.element("/pcell/soma"))
+    np.testing.assert_array_equal(vm.vector[1:], tr[:, soma])
+    _, steps = ref.read_spikes()
+    assert len(steps) > 0
+    np.testing.assert_array_equal(spk.vector, np.array(steps, dtype=np.float64) * 50e-6)
+    moose.reset()
