@@ -138,6 +138,20 @@ typedef struct hsg_chan2d_desc {
     double  Gbar, Ek, Xpower, Ypower, Zpower, X, Y, Z, modulation;
 } hsg_chan2d_desc;
 
+/* A PulseGen whose outputOut feeds a compartment's injectMsg
+ * (device/PulseGen.cpp:359-433; HSolve::addInject,
+ * hsolve/HSolveInterface.cpp:213-219): free-running pulses (trigMode 0) of
+ * n_pulses (delay, width, level) entries over baseLevel, evaluated on the
+ * device before each tick at the tick's end time (PulseGen runs on clock tick
+ * 0, before HSolve's tick 6), and sent once at reinit (baseLevel). */
+#define HSG_MAX_PULSES 4
+typedef struct hsg_pulsegen_desc {
+    int32_t compartment;
+    int32_t n_pulses;        /* 1..HSG_MAX_PULSES (PulseGen::count)          */
+    double  baseLevel;
+    double  delay[HSG_MAX_PULSES], width[HSG_MAX_PULSES], level[HSG_MAX_PULSES];
+} hsg_pulsegen_desc;
+
 /* One cell.  axial[2*k], axial[2*k+1] = (parent, child) of the k-th
  * "axial"->"raxial" message, in creation order.  seed: the compartment
  * HSolve.target resolves to (hsolve/HSolve.cpp:315-334). */
@@ -161,6 +175,8 @@ typedef struct hsg_cell_desc {
     const hsg_table2d_desc*     tables2d;
     int32_t                     n_chans2d;
     const hsg_chan2d_desc*      chans2d;
+    int32_t                     n_pulsegens;
+    const hsg_pulsegen_desc*    pulsegens;
 } hsg_cell_desc;
 
 /* ---- engine ------------------------------------------------------------ */
@@ -271,6 +287,11 @@ int hsg_add_gkek(hsg_engine* e, int32_t cell, int32_t compartment, double Gk, do
 int hsg_add_conc(hsg_engine* e, int32_t cell, int32_t channel, double conc);
 /* HSolve::iCa: extra Ca-pool activation (HSolveInterface.cpp:469-480). */
 int hsg_add_ica(hsg_engine* e, int32_t cell, int32_t pool, double iCa);
+/* PulseGen field writes (setCount/setLevel/setWidth/setDelay/setBaseLevel,
+ * device/PulseGen.cpp:243-340) on the cell's pulsegen-th PulseGen, taking
+ * effect from the next step; its compartment must stay the one it was added
+ * with.  The current output level is kept. */
+int hsg_set_pulsegen(hsg_engine* e, int32_t cell, int32_t pulsegen, const hsg_pulsegen_desc* desc);
 
 /* ---- outputs ------------------------------------------------------------- */
 typedef struct hsg_spike {

@@ -61,7 +61,8 @@ class CellDesc(C.Structure):
                 ("n_spikegens", C.c_int32), ("spikegens", C.c_void_p),
                 ("n_leaks", C.c_int32), ("leaks", C.c_void_p),
                 ("n_tables2d", C.c_int32), ("tables2d", C.POINTER(Table2DDesc)),
-                ("n_chans2d", C.c_int32), ("chans2d", C.c_void_p)]
+                ("n_chans2d", C.c_int32), ("chans2d", C.c_void_p),
+                ("n_pulsegens", C.c_int32), ("pulsegens", C.c_void_p)]
 
 
 class Options(C.Structure):
@@ -99,7 +100,13 @@ CHAN2D_DT = np.dtype([("compartment", "i4"), ("instant", "i4"), ("tableA", "i4",
                                             "X", "Y", "Z", "modulation")], align=True)
 
 assert CHANNEL_DT.itemsize == 104 and SPIKEGEN_DT.itemsize == 24 and LEAK_DT.itemsize == 24
+MAX_PULSES = 4
+PULSEGEN_DT = np.dtype([("compartment", "i4"), ("n_pulses", "i4"), ("baseLevel", "f8"),
+                        ("delay", "f8", (MAX_PULSES,)), ("width", "f8", (MAX_PULSES,)),
+                        ("level", "f8", (MAX_PULSES,))], align=True)
+
 assert CHAN2D_DT.itemsize == 136
+assert PULSEGEN_DT.itemsize == 112
 
 
 def _ptr(a):
@@ -118,6 +125,8 @@ class PackedCell:
         self.spikegens = np.ascontiguousarray(spec.spikegens, dtype=SPIKEGEN_DT)
         self.leaks = np.ascontiguousarray(spec.leaks, dtype=LEAK_DT)
         self.chans2d = np.ascontiguousarray(spec.chans2d, dtype=CHAN2D_DT)
+        self.pulsegens = np.ascontiguousarray(getattr(spec, "pulsegens", np.zeros(0, PULSEGEN_DT)),
+                                              dtype=PULSEGEN_DT)
         self._tables = []           # every table array the descriptors point into
         t2 = (Table2DDesc * max(1, len(spec.tables2d)))()
         for i, t in enumerate(spec.tables2d):
@@ -163,6 +172,8 @@ class PackedCell:
         d.tables2d = self._t2
         d.n_chans2d = len(self.chans2d)
         d.chans2d = _ptr(self.chans2d)
+        d.n_pulsegens = len(self.pulsegens)
+        d.pulsegens = _ptr(self.pulsegens)
         self.desc = d
 
     @property

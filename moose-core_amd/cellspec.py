@@ -11,7 +11,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._abi import CACONC_DT, CHAN2D_DT, CHANNEL_DT, COMPARTMENT_DT, LEAK_DT, SPIKEGEN_DT, PackedCell
+from ._abi import (CACONC_DT, CHAN2D_DT, CHANNEL_DT, COMPARTMENT_DT, LEAK_DT, PULSEGEN_DT, SPIKEGEN_DT,
+                   PackedCell)
 
 
 @dataclass
@@ -49,6 +50,7 @@ class CellSpec:
     leaks: np.ndarray = field(default_factory=lambda: np.zeros(0, LEAK_DT))
     tables2d: list = field(default_factory=list)
     chans2d: np.ndarray = field(default_factory=lambda: np.zeros(0, CHAN2D_DT))
+    pulsegens: np.ndarray = field(default_factory=lambda: np.zeros(0, PULSEGEN_DT))
 
     @property
     def n(self):

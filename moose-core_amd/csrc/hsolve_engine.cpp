@@ -413,6 +413,8 @@ int place_cell(hsg_engine* e, HostCell& hc) {
         {&d.o2dState, 24 * (int64_t)s.ch2d.size()},
         {&d.o2dGk, 8 * (int64_t)s.ch2d.size()},
         {&d.o2dExt, s.ch2d.empty() ? 0 : 16 * (int64_t)n},
+        // PulseGens
+        {&d.oPulse, (int64_t)sizeof(PulseDev) * (int64_t)s.pulses.size()},
     };
     int64_t total = 0;
     for (auto& it : items) total += align_up(std::max<int64_t>(it.bytes, 8));
@@ -536,9 +538,29 @@ int place_cell(hsg_engine* e, HostCell& hc) {
         st.put(d.o2dChan, ch);
         st.put(d.o2dState, stt);
     }
+    if (!s.pulses.empty()) {
+        std::vector<PulseDev> pd(s.pulses.size());
+        for (size_t k = 0; k < s.pulses.size(); ++k) {
+            const HostPulse& h = s.pulses[k];
+            PulseDev& q = pd[k];
+            std::memset(&q, 0, sizeof q);
+            q.compt = h.compt;
+            q.entry = h.entry;
+            q.n = h.n;
+            q.base = h.base;
+            q.out = h.base;
+            for (int j = 0; j < h.n; ++j) {
+                q.delay[j] = h.delay[j];
+                q.width[j] = h.width[j];
+                q.level[j] = h.level[j];
+            }
+        }
+        st.put(d.oPulse, pd);
+    }
     if ((rc = st.flush(e))) return rc;
     d.msg = msgOn ? 1 : 0;
     d.n2d = (int32_t)s.ch2d.size();
+    d.nPulse = (int32_t)s.pulses.size();
     d.values2dSent = 0;
     d.nSlots = s.nSlots;
     d.rootSlot = s.rootSlot;
@@ -1050,10 +1072,19 @@ int run_launches(hsg_engine* e, const std::vector<int32_t>& order, const std::ve
 }
 
 int reinit_cells_host(hsg_engine* e, HostCell& hc) {
-    // the device state (V, Ca pools, spike generators, gates) is reset by
-    // hsolve_reinit_kernel; here only the host mirrors
+    // the device state (V, Ca pools, spike generators, gates, PulseGen
+    // outputs) is reset by hsolve_reinit_kernel; here only the host mirrors
     CellSetup& s = hc.s;
     for (auto& c : s.pC) c = 0.0;
+    // PulseGen::reinit sends baseLevel (device/PulseGen.cpp:426-433); the
+    // solver's reinit keeps injectVarying, so the first step adds it
+    for (const HostPulse& p : s.pulses) {
+        for (size_t k = 0; k < hc.injIdx.size(); ++k)
+            if (hc.injIdx[k] == p.compt) {
+                hc.injVar[k] += p.base;
+                hc.injDirty = true;
+            }
+    }
     hc.dev.probeBase = 0;
     e->tablesDirty = true;
     hc.ext.clear();
@@ -1583,6 +1614,42 @@ int hsg_add_inject(hsg_engine* e, int32_t cell, int32_t compartment, double valu
     hc->injBasal.push_back(0.0);
     hc->injVar.push_back(0.0 + value);
     hc->injDirty = true;
+    return 0;
+}
+
+int hsg_set_pulsegen(hsg_engine* e, int32_t cell, int32_t pulsegen, const hsg_pulsegen_desc* desc) {
+    HostCell* hc = get_cell(e, cell);
+    if (!hc) return fail(HSG_E_ARG, "no such cell");
+    CellSetup& s = hc->s;
+    if (pulsegen < 0 || pulsegen >= (int)s.pulses.size() || !desc) return fail(HSG_E_ARG, "no such PulseGen");
+    if (desc->n_pulses < 1 || desc->n_pulses > HSG_MAX_PULSES)
+        return fail(HSG_E_ARG, "PulseGen: 1 to HSG_MAX_PULSES pulses");
+    HostPulse& p = s.pulses[pulsegen];
+    if (desc->compartment < 0 || desc->compartment >= (int)s.descToHines.size() ||
+        s.descToHines[desc->compartment] != p.compt)
+        return fail(HSG_E_ARG, "a PulseGen keeps the compartment it was added with");
+    p.n = desc->n_pulses;
+    p.base = desc->baseLevel;
+    for (int j = 0; j < HSG_MAX_PULSES; ++j) {
+        p.delay[j] = j < p.n ? desc->delay[j] : 0.0;
+        p.width[j] = j < p.n ? desc->width[j] : 0.0;
+        p.level[j] = j < p.n ? desc->level[j] : 0.0;
+    }
+    hipSetDevice(e->device);
+    // everything but the output level, which the device holds
+    PulseDev q{};
+    const int64_t off = hc->dev.oPulse + (int64_t)sizeof(PulseDev) * pulsegen;
+    HIPCHK(hipMemcpyAsync(&q, e->arena.base + off, sizeof q, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    q.n = p.n;
+    q.base = p.base;
+    for (int j = 0; j < HSG_MAX_PULSES; ++j) {
+        q.delay[j] = p.delay[j];
+        q.width[j] = p.width[j];
+        q.level[j] = p.level[j];
+    }
+    HIPCHK(hipMemcpyAsync(e->arena.base + off, &q, sizeof q, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
     return 0;
 }
 

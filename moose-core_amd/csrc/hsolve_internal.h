@@ -10,6 +10,8 @@
 #pragma once
 #include <stdint.h>
 
+#include "../../include/hsolve_amd.h"
+
 namespace hsg {
 
 constexpr int kMaxSlots = 8;      // channel slots per compartment
@@ -58,6 +60,17 @@ struct Chan2DDev {
     int32_t conc1, conc2;       // slot-major pool offsets of the concentrations, -1
     int32_t instant;
     int32_t compt;              // Hines index
+};
+
+// A free-running PulseGen feeding a compartment's injection
+// (device/PulseGen.cpp:359-433).
+struct PulseDev {
+    int32_t compt;              // Hines index
+    int32_t entry;              // its injection entry (oInjIdx)
+    int32_t n;                  // pulses (1..HSG_MAX_PULSES)
+    int32_t pad;
+    double  base, out;          // baseLevel_, output_ (state)
+    double  delay[HSG_MAX_PULSES], width[HSG_MAX_PULSES], level[HSG_MAX_PULSES];
 };
 
 // Per-cell descriptor (read by every workgroup working on the cell).
@@ -153,6 +166,10 @@ struct CellDev {
     int64_t o2dState;       // f64[3 * n2d]: X, Y, Z
     int64_t o2dGk;          // f64[n2d]: Gk of the last tick
     int64_t o2dExt;         // f64[2n]: external (Gk, Gk*Ek) of the last tick per compartment
+    // PulseGens
+    int32_t nPulse;
+    int32_t pad2;
+    int64_t oPulse;         // PulseDev[nPulse]
 };
 
 // Elimination unit descriptor: 12 int32 (48 bytes), read as three int4 loads.

@@ -806,6 +806,48 @@ __device__ __forceinline__ const ChanType* stage_types(const CellDev& c, const C
 // (SpikeGen::process at t = dt * step, biophysics/SpikeGen.cpp:202-216, fed
 // by HSolveActive::sendSpikes, hsolve/HSolveActive.cpp:342-347) into the
 // engine's spike ring, and Vm probes (Table sampling after the solver's tick).
+// PulseGen::process in FREE_RUN mode (device/PulseGen.cpp:359-424) at time
+// t: updates the output state; false when the reference sends nothing
+// (phase past the period).
+__device__ __noinline__ bool pulse_tick(PulseDev& p, double t) {
+    double period = p.width[0] + p.delay[0];
+    for (int ii = 1; ii < p.n && (p.width[ii] > 0.0 || p.delay[ii] > 0.0); ++ii) {
+        const double incr = p.delay[ii] + p.width[ii] - p.width[ii - 1];
+        if (incr > 0) period += incr;
+    }
+    double phase = fmod(t, period);
+    if (phase >= period) {
+        p.out = p.base;
+        return false;
+    }
+    for (int ii = 0; ii < p.n; ++ii) {
+        if (phase < p.delay[ii]) {
+            p.out = p.base;
+            break;
+        } else if (phase < p.delay[ii] + p.width[ii]) {
+            p.out = p.level[ii];
+            break;
+        }
+        phase -= p.delay[ii];
+    }
+    return true;
+}
+
+// The varying injection of entry e for the tick ending at t: the call's host
+// additions (first tick), then what the entry's PulseGens send, in their
+// order (HSolve::addInject sums into injectVarying).
+__device__ __forceinline__ double inject_varying(char* arena, const CellDev& c, int e, bool first, double t) {
+    double* iv = A_<double>(arena, c.oInjVar);
+    double var = first ? iv[e] : 0.0;
+    if (first) iv[e] = 0.0;
+    if (c.nPulse > 0) {
+        PulseDev* pl = A_<PulseDev>(arena, c.oPulse);
+        for (int q = 0; q < c.nPulse; ++q)
+            if (pl[q].entry == e && pulse_tick(pl[q], t)) var += pl[q].out;
+    }
+    return var;
+}
+
 __device__ __forceinline__ void tick_outputs(const LaunchArgs& a, const CellDev& c, const double* __restrict__ V,
                                              int64_t stepNo) {
     char* arena = a.arena;
@@ -1060,13 +1102,12 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
                 // injections: HS[4i+3] += injectVarying + injectBasal (HSolveActive.cpp:119-128)
                 const int32_t* ii = A_<const int32_t>(arena, c.oInjIdx);
                 const double* ib = A_<const double>(arena, c.oInjBasal);
-                double* iv = A_<double>(arena, c.oInjVar);
+                const double t = a.dt * (double)(unsigned long long)(step0 + k + 1);
                 for (int e = 0; e < nInj; ++e) {
-                    const double var = first ? iv[e] : 0.0;
+                    const double var = inject_varying(arena, c, e, first, t);
                     w.B[ii[e]] += var + ib[e];
                 }
                 if (first) {
-                    for (int e = 0; e < nInj; ++e) iv[e] = 0.0;
                     // external channels (HSolveActive.cpp:142-148)
                     const int32_t* xi = A_<const int32_t>(arena, c.oExtIdx);
                     const double* xg = A_<const double>(arena, c.oExtG);
@@ -1234,10 +1275,11 @@ __device__ __forceinline__ void row_inputs(const LaunchArgs& a, const CellDev& c
     const int32_t* ii = A_<const int32_t>(a.arena, c.oInjIdx);
     for (int e = 0; e < c.nInj; ++e)
         if (ii[e] == i) {
-            double* iv = A_<double>(a.arena, c.oInjVar);
-            const double var = a.firstOfCall ? iv[e] : 0.0;
+            // this tick ends at step (steps done + 1); the solve kernel
+            // advances the counter after it
+            const double t = a.dt * (double)(unsigned long long)(*A_<const int64_t>(a.arena, c.oSteps) + 1);
+            const double var = inject_varying(a.arena, c, e, a.firstOfCall != 0, t);
             B += var + A_<const double>(a.arena, c.oInjBasal)[e];
-            if (a.firstOfCall) iv[e] = 0.0;
         }
     if (has2d) {
         const double* ext = A_<const double>(a.arena, c.o2dExt);
@@ -1587,6 +1629,8 @@ __global__ void __launch_bounds__(256) hsolve_reinit_kernel(LaunchArgs a) {
         double* last = A_<double>(a.arena, c.oSpkLast);
         const double* refr = A_<const double>(a.arena, c.oSpkRefr);
         for (int k = threadIdx.x; k < c.nSpike; k += blockDim.x) last[k] = -refr[k];
+        PulseDev* pl = A_<PulseDev>(a.arena, c.oPulse);
+        for (int k = threadIdx.x; k < c.nPulse; k += blockDim.x) pl[k].out = pl[k].base;
         if (threadIdx.x == 0) *A_<int64_t>(a.arena, c.oSteps) = 0;
     }
     __syncthreads();

@@ -872,6 +872,40 @@ int setup_cell(const hsg_cell_desc* d, double dt, int caAdvance, CellSetup& s, s
     }
     s.ch2dStart[n] = (int32_t)s.ch2d.size();
 
+    // PulseGens on the compartments' injectMsg: their targets get an
+    // injection entry, as HSolve::addInject creates one
+    // (hsolve/HSolveInterface.cpp:213-219)
+    s.pulses.clear();
+    for (int k = 0; k < (d->n_pulsegens > 0 ? d->n_pulsegens : 0); ++k) {
+        const hsg_pulsegen_desc& pd = d->pulsegens[k];
+        if (pd.compartment < 0 || pd.compartment >= nd || s.descToHines[pd.compartment] < 0) {
+            err = "PulseGen on a compartment outside the cell";
+            return HSG_E_MODEL;
+        }
+        if (pd.n_pulses < 1 || pd.n_pulses > HSG_MAX_PULSES) {
+            err = "PulseGen: 1 to HSG_MAX_PULSES pulses";
+            return HSG_E_ARG;
+        }
+        HostPulse p{};
+        p.compt = s.descToHines[pd.compartment];
+        p.n = pd.n_pulses;
+        p.base = pd.baseLevel;
+        for (int j = 0; j < p.n; ++j) {
+            p.delay[j] = pd.delay[j];
+            p.width[j] = pd.width[j];
+            p.level[j] = pd.level[j];
+        }
+        s.hasInject[p.compt] = 1;
+        s.pulses.push_back(p);
+    }
+    {
+        std::vector<int> entryOf(n, -1);
+        int m = 0;
+        for (int i = 0; i < n; ++i)
+            if (s.hasInject[i]) entryOf[i] = m++;
+        for (auto& p : s.pulses) p.entry = entryOf[p.compt];
+    }
+
     // SpikeGens in Hines order of their compartments
     std::vector<std::vector<int>> spk(nd);
     for (int k = 0; k < d->n_spikegens; ++k) {

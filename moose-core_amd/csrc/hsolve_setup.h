@@ -46,6 +46,14 @@ struct HostChan2D {
     double Gbar, Ek, mod, pw[3], st[3];
 };
 
+struct HostPulse {
+    int compt;                        // Hines index
+    int entry;                        // injection entry (Hines order of injected compartments)
+    int n;
+    double base;
+    double delay[HSG_MAX_PULSES], width[HSG_MAX_PULSES], level[HSG_MAX_PULSES];
+};
+
 struct HostChanType {
     double Xpower, Ypower, Zpower;
     int instant, nGates, colX, colY, colZ, caDepLocal, caTgtLocal;
@@ -119,6 +127,7 @@ struct CellSetup {
     // HHChannel2D objects beside the solver, sorted by compartment (desc
     // order within one), with the compartment -> channels index (n + 1)
     std::vector<HostChan2D> ch2d;
+    std::vector<HostPulse> pulses;    // PulseGens on injectMsg, desc order
     std::vector<int32_t> ch2dStart;
     std::vector<int> ch2dOfDesc;             // desc index -> position in ch2d
     std::vector<HostTable2D> tables2d;

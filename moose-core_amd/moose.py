@@ -531,6 +531,87 @@ class SpikeGen(Neutral):
         self.__dict__.update(threshold=0.0, refractT=0.0, edgeTriggered=1)
 
 
+class _PulseLookup:
+    """PulseGen's level/width/delay LookupValueFinfos: pg.delay[i] reads or
+    writes entry i (device/PulseGen.cpp:300-355; an index at or past count
+    is refused, as the reference warns and ignores)."""
+
+    def __init__(self, pg, key):
+        self._pg, self._key = pg, key
+
+    def __getitem__(self, i):
+        vals = self._pg.__dict__[self._key]
+        return vals[i] if 0 <= i < len(vals) else 0.0
+
+    def __setitem__(self, i, value):
+        vals = self._pg.__dict__[self._key]
+        if not 0 <= i < len(vals):
+            raise IndexError(f"PulseGen: index {i} past count {len(vals)}; set 'count' first")
+        vals[i] = float(value)
+
+    def __len__(self):
+        return len(self._pg.__dict__[self._key])
+
+
+class PulseGen(Neutral):
+    """A free-running pulse train (trigMode 0, device/PulseGen.cpp:359-424)
+    sent through 'output' to a compartment's 'injectMsg'.  Under an HSolve it
+    is evaluated on the GPU before each tick."""
+
+    def _init_fields(self):
+        # device/PulseGen.cpp:148-165: two pulses, all zero
+        self.__dict__.update(_level=[0.0, 0.0], _width=[0.0, 0.0], _delay=[0.0, 0.0],
+                             baseLevel=0.0, trigMode=0)
+
+    level = property(lambda self: _PulseLookup(self, "_level"))
+    width = property(lambda self: _PulseLookup(self, "_width"))
+    delay = property(lambda self: _PulseLookup(self, "_delay"))
+
+    @property
+    def count(self):
+        return len(self._level)
+
+    def _set_local(self, name, value):
+        d = self.__dict__
+        if name == "count":
+            n = int(value)
+            if n <= 0:
+                raise ValueError("PulseGen: invalid pulse count")
+            for k in ("_level", "_width", "_delay"):       # PulseGen::setCount
+                d[k] = (d[k] + [0.0] * n)[:n]
+            return
+        first = {"firstLevel": ("_level", 0), "firstWidth": ("_width", 0), "firstDelay": ("_delay", 0),
+                 "secondLevel": ("_level", 1), "secondWidth": ("_width", 1),
+                 "secondDelay": ("_delay", 1)}
+        if name in first:
+            key, i = first[name]
+            if i < len(d[key]):             # the second* setters need count >= 2
+                d[key][i] = float(value)
+            return
+        if name == "trigMode" and int(value) != 0:
+            raise NotImplementedError("PulseGen: only trigMode 0 (free running) is supported")
+        object.__setattr__(self, name, value)
+
+    def __getattr__(self, name):
+        first = {"firstLevel": ("_level", 0), "firstWidth": ("_width", 0), "firstDelay": ("_delay", 0),
+                 "secondLevel": ("_level", 1), "secondWidth": ("_width", 1),
+                 "secondDelay": ("_delay", 1)}
+        if name in first:
+            key, i = first[name]
+            vals = self.__dict__[key]
+            return vals[i] if i < len(vals) else 0.0
+        return Neutral.__getattr__(self, name)
+
+    def _record(self, compartment):
+        n = self.count
+        if n > _abi.MAX_PULSES:
+            raise ValueError(f"{self.path}: at most {_abi.MAX_PULSES} pulses under HSolve")
+        r = np.zeros(1, _abi.PULSEGEN_DT)[0]
+        r["compartment"], r["n_pulses"], r["baseLevel"] = compartment, n, self.baseLevel
+        r["delay"][:n], r["width"][:n], r["level"][:n] = self._delay, self._width, self._level
+        return r
+
+
 class Leakage(Neutral):
     def _init_fields(self):
         self.__dict__.update(Gk=0.0, Ek=0.0)
@@ -798,11 +879,20 @@ class HSolve(Neutral):
         lrec = np.zeros(len(leaks), _abi.LEAK_DT)
         for k, rec in enumerate(leaks):
             lrec[k] = rec
+        # PulseGens feeding the compartments' injectMsg, in creation order
+        # (the order clock tick 0 processes them)
+        pgs = sorted(((s, cidx[d.id]) for s, sf, d, df in _msgs
+                      if sf == "output" and df == "injectMsg" and isinstance(s, PulseGen)
+                      and d.id in cidx), key=lambda x: x[0].id)
+        grec = np.zeros(len(pgs), _abi.PULSEGEN_DT)
+        for k, (pg, ci) in enumerate(pgs):
+            grec[k] = pg._record(ci)
         spec = CellSpec(ct, np.array(axial, dtype=np.int32).reshape(-1, 2), seed=cidx[seed.id],
                         channels=crec, gates=glist, pools=prec, spikegens=srec, leaks=lrec,
-                        tables2d=tabs, chans2d=c2rec)
+                        tables2d=tabs, chans2d=c2rec, pulsegens=grec)
         objs = {"compartments": comps, "channels": [ch for _, ch in chans], "pools": pools,
-                "spikegens": [sg for _, sg in spikes], "chans2d": [ch for _, ch in ch2]}
+                "spikegens": [sg for _, sg in spikes], "chans2d": [ch for _, ch in ch2],
+                "pulsegens": pgs}
         return spec, objs
 
     def _setup(self, seed):
@@ -833,6 +923,23 @@ class HSolve(Neutral):
     def _set(self, obj, field, idx, value):
         field = getattr(type(obj), "_zmap", {}).get(field, field)
         self._engine.set(self._cell, field, [idx], [float(value)])
+
+    def _push_pulsegens(self):
+        """PulseGen fields written since setup reach the device before the
+        next step, as the reference's PulseGen reads them every tick."""
+        sent = self._spec.pulsegens
+        cidx = {c.id: i for i, c in enumerate(self._objs["compartments"])}
+        now = sorted(((s.id, cidx[d.id]) for s, sf, d, df in _msgs
+                      if sf == "output" and df == "injectMsg" and isinstance(s, PulseGen)
+                      and d.id in cidx))
+        if now != [(pg.id, ci) for pg, ci in self._objs["pulsegens"]]:
+            raise ValueError("PulseGen messages to the solver's compartments changed after "
+                             "HSolve.target was set; connect them first")
+        for k, (pg, ci) in enumerate(self._objs["pulsegens"]):
+            r = pg._record(ci)
+            if r.tobytes() != sent[k].tobytes():
+                self._engine.set_pulsegen(self._cell, k, r)
+                sent[k] = r
 
     # -- recording -------------------------------------------------------
     def _recorders(self):
@@ -1222,6 +1329,8 @@ def reinit():
     Tables record their t = 0 sample."""
     done = set()
     for hs in _solvers:
+        hs._push_pulsegens()
+    for hs in _solvers:
         eng = hs._engine
         if id(eng) not in done:
             eng.reinit(float(hs.dt))
@@ -1261,6 +1370,7 @@ def start(runtime):
             if comps:
                 eng.set_probes(hs._cell, comps, nsteps // every + 1, every=every)
             rec.append((hs, vm, spk, comps, every, hs.__dict__.get("_steps", 0)))
+            hs._push_pulsegens()
         eng.step(nsteps)
         spikes = eng.read_spikes()
         for hs, vm, spk, comps, every, s0 in rec:

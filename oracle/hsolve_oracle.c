@@ -129,6 +129,15 @@ typedef struct {
     double  *injectVarying, *injectBasal;
 } or_hines;
 
+/* A free-running PulseGen (device/PulseGen.cpp) sending to a compartment's
+ * injectMsg. */
+typedef struct {
+    int    ic;                        /* Hines index of the target */
+    int    n;                         /* entries of delay/width/level */
+    double base, out;                 /* baseLevel_, output_ */
+    double delay[HSG_MAX_PULSES], width[HSG_MAX_PULSES], level[HSG_MAX_PULSES];
+} or_pulse;
+
 /* An Interpol2D (builtins/Interpol2D.cpp) and an HHChannel2D
  * (biophysics/HHChannel2D.cpp) beside the solver. */
 typedef struct {
@@ -189,6 +198,9 @@ struct or_solver {
     or_chan2d* ch2d;
     or_tab2d*  tab2d;
     int        valuesSent;       /* the solver has sent Vm/[Ca] at least once */
+    /* PulseGens feeding injectMsg, desc order */
+    int        nPulse;
+    or_pulse*  pulse;
 };
 
 /* ------------------------------------------------------------------------ */
@@ -1078,8 +1090,58 @@ int or_create(const hsg_cell_desc* d, double dt, int caAdvance, or_solver** out)
         c->Gk = 0.0;
     }
     s->valuesSent = 0;
+    /* PulseGens: an injection entry on their target (HSolve::addInject
+     * creates it, hsolve/HSolveInterface.cpp:213-219) */
+    s->nPulse = d->n_pulsegens > 0 ? d->n_pulsegens : 0;
+    s->pulse = (or_pulse*)calloc(s->nPulse + 1, sizeof(or_pulse));
+    for (int k = 0; k < s->nPulse; ++k) {
+        const hsg_pulsegen_desc* pd = &d->pulsegens[k];
+        if (pd->compartment < 0 || pd->compartment >= d->n_compartments || s->descToHines[pd->compartment] < 0 ||
+            pd->n_pulses < 1 || pd->n_pulses > HSG_MAX_PULSES) {
+            or_destroy(s);
+            return HSG_E_ARG;
+        }
+        or_pulse* p = &s->pulse[k];
+        p->ic = s->descToHines[pd->compartment];
+        p->n = pd->n_pulses;
+        p->base = pd->baseLevel;
+        p->out = 0.0;
+        for (int j = 0; j < p->n; ++j) {
+            p->delay[j] = pd->delay[j];
+            p->width[j] = pd->width[j];
+            p->level[j] = pd->level[j];
+        }
+        s->h.hasInject[p->ic] = 1;
+    }
     *out = s;
     return 0;
+}
+
+/* PulseGen::process in FREE_RUN mode (device/PulseGen.cpp:359-424): the
+ * output at time t; returns 0 when the reference sends nothing (phase past
+ * the period), leaving out at baseLevel. */
+static int pulse_process(or_pulse* p, double t) {
+    double period = p->width[0] + p->delay[0];
+    for (int ii = 1; ii < p->n && (p->width[ii] > 0.0 || p->delay[ii] > 0.0); ++ii) {
+        double incr = p->delay[ii] + p->width[ii] - p->width[ii - 1];
+        if (incr > 0) period += incr;
+    }
+    double phase = fmod(t, period);
+    if (phase >= period) {
+        p->out = p->base;
+        return 0;
+    }
+    for (int ii = 0; ii < p->n; ++ii) {
+        if (phase < p->delay[ii]) {
+            p->out = p->base;
+            break;
+        } else if (phase < p->delay[ii] + p->width[ii]) {
+            p->out = p->level[ii];
+            break;
+        }
+        phase -= p->delay[ii];
+    }
+    return 1;
 }
 
 void or_destroy(or_solver* s) {
@@ -1099,7 +1161,7 @@ void or_destroy(or_solver* s) {
     iv_free(&s->spikeGenLog); free(s->spikeStepLog);
     if (s->tab2d)
         for (int k = 0; k < s->nTab2d; ++k) free((void*)s->tab2d[k].t);
-    free(s->tab2d); free(s->ch2d);
+    free(s->tab2d); free(s->ch2d); free(s->pulse);
     free(s);
 }
 
@@ -1148,6 +1210,13 @@ static void chan2d_reinit(or_solver* s);
 
 void or_reinit(or_solver* s, double dt) {
     or_hines* h = &s->h;
+    /* PulseGen::reinit (device/PulseGen.cpp:426-433) on tick 0 sends
+     * baseLevel; the solver's reinit leaves injectVarying alone, so it is
+     * added to the first step */
+    for (int k = 0; k < s->nPulse; ++k) {
+        s->pulse[k].out = s->pulse[k].base;
+        h->injectVarying[s->pulse[k].ic] += s->pulse[k].base;
+    }
     chan2d_reinit(s);            /* their reinit runs before the solver's */
     s->hsolveDt = dt;
     for (unsigned i = 0; i < 2 * h->nCompt; ++i) s->externalCurrent[i] = 0.0;
@@ -1432,7 +1501,10 @@ void or_step(or_solver* s, double dt, double currTime) {
     or_hines* h = &s->h;
     if (h->nCompt <= 0) return;
     s->stepCount++;
-    /* the HHChannel2D objects tick before the solver (their tick 4) */
+    /* PulseGens (tick 0), then the HHChannel2D objects (tick 4), before the
+     * solver (tick 6) */
+    for (int k = 0; k < s->nPulse; ++k)
+        if (pulse_process(&s->pulse[k], currTime)) h->injectVarying[s->pulse[k].ic] += s->pulse[k].out;
     chan2d_process(s, dt);
     advance_channels(s, dt);
     calculate_channel_currents(s);

@@ -59,6 +59,9 @@ int main(void) {
   P(hsg_leak_desc, Gk); P(hsg_spikegen_desc, threshold); P(hsg_cell_desc, leaks);
   P(hsg_gate_desc, tableA); P(hsg_cell_desc, chans2d); P(hsg_chan2d_desc, Gbar);
   P(hsg_chan2d_desc, conc2); P(hsg_table2d_desc, table);
+  printf("hsg_pulsegen_desc %zu\n", sizeof(hsg_pulsegen_desc));
+  P(hsg_pulsegen_desc, baseLevel); P(hsg_pulsegen_desc, width); P(hsg_pulsegen_desc, level);
+  P(hsg_cell_desc, pulsegens);
   return 0;
 }
 '''
@@ -93,6 +96,10 @@ int main(void) {
     assert sizes["hsg_chan2d_desc.Gbar"] == _abi.CHAN2D_DT.fields["Gbar"][1]
     assert sizes["hsg_chan2d_desc.conc2"] == _abi.CHAN2D_DT.fields["conc2"][1]
     assert sizes["hsg_table2d_desc.table"] == _abi.Table2DDesc.table.offset
+    assert sizes["hsg_pulsegen_desc"] == _abi.PULSEGEN_DT.itemsize
+    for f in ("baseLevel", "width", "level"):
+        assert sizes["hsg_pulsegen_desc." + f] == _abi.PULSEGEN_DT.fields[f][1]
+    assert sizes["hsg_cell_desc.pulsegens"] == _abi.CellDesc.pulsegens.offset
 
 
 def test_engine_without_gpu_raises_cleanly():

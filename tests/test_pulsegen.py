@@ -1,0 +1,341 @@
+"""PulseGen stimuli under the solver: free-running pulse trains
+(device/PulseGen.cpp:359-424, trigMode 0) sent through outputOut to a
+compartment's injectMsg, which the solver sums into injectVarying
+(HSolve::addInject, hsolve/HSolveInterface.cpp:213-219).  PulseGens run on
+clock tick 0, before the solver's tick 6, at the tick's end time
+t = dt * step (scheduling/Clock.cpp:726-729); PulseGen::reinit sends
+baseLevel once (device/PulseGen.cpp:426-433), and the solver's reinit keeps
+injectVarying, so the first step sees it.
+
+CPU tests pin the oracle: a PulseGen is the same, bit for bit, as a host
+addInject before every tick of the value the pulse train's documented shape
+gives (pulse i starts delay[i] after pulse i-1 starts, lasts width[i] at
+level[i], baseLevel elsewhere, repeating after the last pulse ends), written
+here from that shape and not from the reference's loop.  GPU tests compare
+the HIP path (through the C ABI) with the oracle: FP64 within 1e-6 V with
+identical spike steps, mixed within 0.1 mV and one step.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from moose_core_amd import workloads
+from moose_core_amd._abi import PULSEGEN_DT
+
+FP64_TOL = 1e-6
+MIXED_TOL = 1e-4
+
+
+def train(compartment, base, delay, width, level):
+    r = np.zeros(1, PULSEGEN_DT)[0]
+    n = len(delay)
+    r["compartment"], r["n_pulses"], r["baseLevel"] = compartment, n, base
+    r["delay"][:n], r["width"][:n], r["level"][:n] = delay, width, level
+    return r
+
+
+def shape_output(t, base, delay, width, level):
+    """The train's value at time t from its shape: onsets are cumulative
+    delays; the train repeats with period = last onset + last width.  Also
+    returns the distance from t's phase to the nearest edge, so that a test
+    can stay clear of edges (where two roundings of the same edge differ)."""
+    onsets = np.cumsum(delay)
+    period = onsets[-1] + width[-1]
+    if period == 0.0:          # no pulses: the output stays at its reinit value
+        return base, math.inf
+    ph = math.fmod(t, period)
+    edges = np.concatenate([onsets, onsets + np.asarray(width), [0.0, period]])
+    margin = float(np.min(np.abs(edges - ph)))
+    for o, w, lv in zip(onsets, width, level):
+        if o <= ph < o + w:
+            return lv, margin
+    return base, margin
+
+
+TRAINS = [
+    # one pulse over a base level
+    dict(base=0.05e-9, delay=[0.73137e-3], width=[1.21519e-3], level=[0.9e-9]),
+    # two pulses, the second one 2.1 ms after the first starts, then a gap
+    dict(base=0.0, delay=[0.41113e-3, 2.10371e-3, 1.30017e-3], width=[0.61979e-3, 0.37711e-3, 0.52223e-3],
+         level=[0.7e-9, -0.3e-9, 1.1e-9]),
+    # the reference test's pattern: one pulse, the next one never reached
+    dict(base=0.0, delay=[1.5071e-3, 1e9], width=[2.5e-3, 0.0], level=[0.6e-9, 0.0]),
+    # a default PulseGen (two zero pulses) with a base level: base every tick
+    dict(base=0.2e-9, delay=[0.0, 0.0], width=[0.0, 0.0], level=[0.0, 0.0]),
+]
+
+
+def cell_with_trains(spec, trains, comps):
+    spec.pulsegens = np.array([train(c, **tr) for c, tr in zip(comps, trains)], PULSEGEN_DT)
+    return spec
+
+
+def test_pulse_train_is_a_host_inject_sequence(oracle_mod):
+    """Four PulseGens (two on one compartment) against a copy of the cell
+    with no PulseGens that receives, through addInject, every train's
+    baseLevel after reinit and its shape value before every tick, in
+    creation order: the traces agree bit for bit."""
+    dt, steps = 25e-6, 600
+    rng = np.random.default_rng(5)
+    base = workloads.random_hh_cell(rng, 30, inject=0.0)
+    comps = [0, 7, 7, 19]
+    a = oracle_mod.OracleCell(cell_with_trains(workloads.random_hh_cell(np.random.default_rng(5), 30, inject=0.0),
+                                               TRAINS, comps), dt)
+    a.reinit(dt)
+    ta = a.run(dt, steps, np.arange(base.n))
+    b = oracle_mod.OracleCell(base, dt)
+    b.reinit(dt)
+    for c, tr in zip(comps, TRAINS):
+        b.add_inject(c, tr["base"])
+    tb, worst = [], math.inf
+    for s in range(1, steps + 1):
+        for c, tr in zip(comps, TRAINS):
+            v, margin = shape_output(dt * s, **tr)
+            worst = min(worst, margin)
+            b.add_inject(c, v)
+        tb.append(b.run(dt, 1, np.arange(base.n))[0])
+    assert worst > 1e-9, "a tick lands on a pulse edge; move the train"
+    np.testing.assert_array_equal(ta, np.array(tb))
+    # the stimulus did something: the soma fired
+    assert len(a.read_spikes()[1]) > 0
+
+
+def test_reinit_sends_base_level_once(oracle_mod):
+    """A train that always outputs L (level = baseLevel = L) is a basal
+    inject of L, except on the first tick after each reinit, which also
+    carries the L PulseGen::reinit sent."""
+    dt = 25e-6
+    L = 0.4e-9
+    tr = dict(base=L, delay=[0.3e-3], width=[0.5e-3], level=[L])
+    a = oracle_mod.OracleCell(cell_with_trains(workloads.random_hh_cell(np.random.default_rng(4), 20, inject=0.0),
+                                               [tr], [3]), dt)
+    spec_b = workloads.random_hh_cell(np.random.default_rng(4), 20, inject=0.0)
+    spec_b.compartments["inject"][3] = L
+    b = oracle_mod.OracleCell(spec_b, dt)
+    for _ in range(2):
+        a.reinit(dt)
+        b.reinit(dt)
+        b.add_inject(3, L)
+        np.testing.assert_array_equal(a.run(dt, 300, np.arange(20)), b.run(dt, 300, np.arange(20)))
+
+
+# ---------------------------------------------------------------------------
+# object layer
+# ---------------------------------------------------------------------------
+def test_pulsegen_fields(pkg):
+    """PulseGen fields as the reference's setters define them: two zero
+    pulses by default; count resizes keeping the first entries; the second*
+    fields need count >= 2; lookups past count are refused."""
+    from moose_core_amd import moose
+    moose.reset()
+    pg = moose.PulseGen("/pg")
+    assert pg.count == 2 and pg.baseLevel == 0.0 and pg.trigMode == 0
+    pg.firstDelay, pg.firstWidth, pg.firstLevel = 1e-3, 2e-3, 1e-9
+    pg.secondDelay = 5e-3
+    assert (pg.delay[0], pg.width[0], pg.level[0], pg.delay[1]) == (1e-3, 2e-3, 1e-9, 5e-3)
+    pg.count = 3
+    pg.delay[2] = 7e-3
+    assert [pg.delay[i] for i in range(3)] == [1e-3, 5e-3, 7e-3]
+    pg.count = 1
+    pg.secondLevel = 3.0                       # ignored: only one pulse
+    assert pg.count == 1 and pg.secondLevel == 0.0
+    with pytest.raises(IndexError):
+        pg.level[1] = 1.0
+    with pytest.raises(ValueError):
+        pg.count = 0
+    with pytest.raises(NotImplementedError):
+        pg.trigMode = 1
+    moose.reset()
+
+
+def build_pulse_model(moose, inject_soma=0.0):
+    from test_moose_facade import build_ball_tree
+    comps, sg = build_ball_tree(moose, depth=4, inject=inject_soma)
+    p1 = moose.PulseGen("/cell/p1")
+    p1.firstDelay, p1.firstWidth, p1.firstLevel = 1.01013e-3, 2.02017e-3, 1.0e-9
+    p1.delay[1] = 1e9
+    p2 = moose.PulseGen("/cell/p2")
+    p2.count = 3
+    p2.baseLevel = 0.02e-9
+    for i, (d, w, lv) in enumerate(((0.5137e-3, 0.3111e-3, 0.4e-9), (1.7013e-3, 0.2203e-3, -0.2e-9),
+                                    (0.9019e-3, 0.4417e-3, 0.3e-9))):
+        p2.delay[i], p2.width[i], p2.level[i] = d, w, lv
+    moose.connect(p1, "output", comps[0], "injectMsg")
+    moose.connect(p2, "output", comps[5], "injectMsg")
+    return comps, sg, (p1, p2)
+
+
+def test_layer_collects_pulsegens(pkg):
+    from moose_core_amd import moose
+    comps, _, (p1, p2) = build_pulse_model(moose)
+    moose.PulseGen("/cell/unconnected")         # sends nowhere: not the solver's
+    hs = moose.HSolve("/cell/hsolve")
+    hs.dt = 50e-6
+    spec, objs = hs.build_spec(moose._deep_search_compartment(moose.element("/cell")))
+    pg = spec.pulsegens
+    assert [p for p, _ in objs["pulsegens"]] == [p1, p2]
+    assert list(pg["compartment"]) == [0, 5] and list(pg["n_pulses"]) == [2, 3]
+    assert pg[0]["delay"][1] == 1e9 and pg[1]["baseLevel"] == 0.02e-9
+    np.testing.assert_array_equal(pg[1]["level"][:3], [0.4e-9, -0.2e-9, 0.3e-9])
+    moose.reset()
+
+
+# ---------------------------------------------------------------------------
+# GPU parity
+# ---------------------------------------------------------------------------
+def pulse_batch():
+    rng = np.random.default_rng(23)
+    specs = []
+    for n, comps in ((30, [0, 7, 7, 19]), (140, [0, 60, 60, 139]), (333, [3, 3, 200, 0])):
+        s = workloads.random_hh_cell(rng, n, inject=0.0)
+        specs.append(cell_with_trains(s, TRAINS, comps))
+    s = workloads.random_ca_cell(rng, 120, inject=0.2e-9)
+    specs.append(cell_with_trains(s, TRAINS[:2], [0, 50]))
+    specs.append(workloads.hh_ball_tree(inject=0.6e-9))          # no PulseGen
+    return specs
+
+
+def run_both(pkg, oracle_mod, specs, dt, nsteps, precision, mode):
+    eng = pkg.Engine(precision, kernel_mode=mode)
+    ids = [eng.add_cell(s, dt) for s in specs]
+    for cid, s in zip(ids, specs):
+        eng.set_probes(cid, np.arange(s.n), nsteps)
+    eng.reinit(dt)
+    eng.step(nsteps // 3)
+    eng.step(nsteps - nsteps // 3)
+    out = []
+    spikes = eng.read_spikes()
+    for cid, s in zip(ids, specs):
+        r = oracle_mod.OracleCell(s, dt)
+        r.reinit(dt)
+        ref = r.run(dt, nsteps, np.arange(s.n))
+        out.append((eng.read_probes(cid), ref, list(spikes[spikes["cell"] == cid]["step"]),
+                    list(r.read_spikes()[1])))
+    eng.close()
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["fused", "split"])
+def test_gpu_fp64_pulsegen_batch(pkg, oracle_mod, mode):
+    print(f"case: PulseGen batch (4 cells with trains + 1 plain), 800 steps, fp64, {mode}", flush=True)
+    res = run_both(pkg, oracle_mod, pulse_batch(), 25e-6, 800, "fp64", mode)
+    for gpu, ref, s_gpu, s_ref in res:
+        assert np.abs(gpu - ref).max() <= FP64_TOL
+        assert s_gpu == s_ref
+    assert sum(len(r[3]) for r in res[:3]) > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["fused", "split"])
+def test_gpu_mixed_pulsegen_batch(pkg, oracle_mod, mode):
+    print(f"case: PulseGen batch, 800 steps, mixed, {mode}", flush=True)
+    for gpu, ref, s_gpu, s_ref in run_both(pkg, oracle_mod, pulse_batch(), 25e-6, 800, "mixed", mode):
+        assert np.abs(gpu - ref).max() <= MIXED_TOL
+        assert len(s_gpu) == len(s_ref)
+        assert all(abs(a - b) <= 1 for a, b in zip(s_gpu, s_ref))
+
+
+@pytest.mark.gpu
+def test_gpu_pulsegen_rewritten_between_runs(pkg, oracle_mod):
+    """hsg_set_pulsegen between two runs, a host addInject on the same
+    compartment, and a second reinit: against an oracle cell without
+    PulseGens fed the shape values through addInject."""
+    print("case: 60-compartment HH cell, PulseGen rewritten mid-run + reinit, fp64", flush=True)
+    dt = 25e-6
+    first = dict(base=0.01e-9, delay=[0.61313e-3], width=[0.91717e-3], level=[0.8e-9])
+    second = dict(base=-0.01e-9, delay=[0.21111e-3, 0.93137e-3], width=[0.41919e-3, 0.30103e-3],
+                  level=[0.5e-9, 0.9e-9])
+    plain = workloads.random_hh_cell(np.random.default_rng(9), 60, inject=0.0)
+    spec = cell_with_trains(workloads.random_hh_cell(np.random.default_rng(9), 60, inject=0.0), [first], [4])
+    eng = pkg.Engine("fp64")
+    cid = eng.add_cell(spec, dt)
+    ref = oracle_mod.OracleCell(plain, dt)
+    worst = math.inf
+
+    def ref_steps(tr, s0, n, extra=None):
+        nonlocal worst
+        for s in range(s0 + 1, s0 + n + 1):
+            if extra is not None and s == s0 + 1:
+                ref.add_inject(4, extra)
+            v, m = shape_output(dt * s, **tr)
+            worst = min(worst, m)
+            ref.add_inject(4, v)
+            ref.run(dt, 1)
+
+    eng.reinit(dt)
+    ref.reinit(dt)
+    ref.add_inject(4, first["base"])
+    eng.step(300)
+    ref_steps(first, 0, 300)
+    eng.set_pulsegen(cid, 0, train(4, **second))
+    eng.add_inject(cid, 4, 0.3e-9)
+    eng.step(250)
+    ref_steps(second, 300, 250, extra=0.3e-9)
+    v_gpu = eng.get(cid, "Vm", np.arange(60))
+    v_ref = ref.get("Vm", np.arange(60))
+    assert np.abs(v_gpu - v_ref).max() <= FP64_TOL
+    # reinit: the new baseLevel is sent once more, time restarts at 0
+    eng.reinit(dt)
+    ref.reinit(dt)
+    ref.add_inject(4, second["base"])
+    eng.step(200)
+    ref_steps(second, 0, 200)
+    assert worst > 1e-9
+    v_gpu = eng.get(cid, "Vm", np.arange(60))
+    v_ref = ref.get("Vm", np.arange(60))
+    assert np.abs(v_gpu - v_ref).max() <= FP64_TOL
+    with pytest.raises(pkg.HSolveError):
+        eng.set_pulsegen(cid, 0, train(5, **second))      # another compartment
+    eng.close()
+
+
+@pytest.mark.gpu
+def test_gpu_layer_pulsegen(pkg, oracle_mod):
+    """PulseGens through the object layer, with a field written between two
+    start() calls.  The first run is compared bit for bit with the oracle on
+    the layer's description; the whole Vm Table with an oracle cell without
+    PulseGens fed both trains' shape values through addInject, the first
+    train's level changing at step 121."""
+    print("case: object layer, 2 PulseGens on a depth-4 ball tree, fp64", flush=True)
+    from moose_core_amd import moose
+    comps, _, (p1, p2) = build_pulse_model(moose)
+    tab = moose.Table("/cell/vm")
+    moose.connect(tab, "requestOut", comps[5], "getVm")
+    moose.setClock(8, 50e-6)
+    hs = moose.HSolve("/cell/hsolve")
+    hs.dt = 50e-6
+    hs.target = "/cell"
+    spec = hs._spec
+    trains = [dict(base=r["baseLevel"], delay=list(r["delay"][:r["n_pulses"]]),
+                   width=list(r["width"][:r["n_pulses"]]), level=list(r["level"][:r["n_pulses"]]))
+              for r in spec.pulsegens]
+    tcomp = list(spec.pulsegens["compartment"])
+    ref = oracle_mod.OracleCell(spec, 50e-6)
+    ref.reinit(50e-6)
+    tr1 = ref.run(50e-6, 120, [5])[:, 0]
+    moose.reinit()
+    moose.start(120 * 50e-6)
+    np.testing.assert_array_equal(tab.vector[1:121], tr1)
+    p1.firstLevel = 0.5e-9
+    moose.start(80 * 50e-6)
+    assert len(tab.vector) == 201
+    plain = spec
+    plain.pulsegens = plain.pulsegens[:0]
+    ref = oracle_mod.OracleCell(plain, 50e-6)
+    ref.reinit(50e-6)
+    for c, tr in zip(tcomp, trains):
+        ref.add_inject(c, tr["base"])
+    out, worst = [], math.inf
+    for s in range(1, 201):
+        if s == 121:
+            trains[0]["level"][0] = 0.5e-9
+        for c, tr in zip(tcomp, trains):
+            v, m = shape_output(50e-6 * s, **tr)
+            worst = min(worst, m)
+            ref.add_inject(c, v)
+        out.append(ref.run(50e-6, 1, [5])[0, 0])
+    assert worst > 1e-9
+    assert np.abs(tab.vector[1:] - np.array(out)).max() <= FP64_TOL
+    moose.reset()
