@@ -101,13 +101,40 @@ typedef struct hsg_spikegen_desc {
     double  threshold, refractT;
 } hsg_spikegen_desc;
 
-/* One Leakage channel on a compartment's "channel" message
- * (biophysics/Leakage.cpp:90-102); folded into EmByRm at setup
- * (hsolve/HSolvePassive.cpp:138-156). */
+/* One Leakage object on a compartment's "channel" message.  Gk is its Gk
+ * field when the solver reads the model (0 until the object has been
+ * reinitialised or processed): the solver folds Ek * Gk into EmByRm, the
+ * right-hand side only (hsolve/HSolvePassive.cpp:138-156).  The object is
+ * not taken over: on every tick (clock tick 4, before the solver's tick 6)
+ * it sends (Gbar * modulation, Ek) (biophysics/Leakage.cpp:90-95), which the
+ * solver adds to the compartment's external current (HSolve::addGkEk,
+ * hsolve/HSolveInterface.cpp:246-252); what it sends at reinit is cleared by
+ * the solver's own reinit (hsolve/HSolveActiveSetup.cpp:43-46). */
 typedef struct hsg_leak_desc {
     int32_t compartment;
     double  Gk, Ek;
+    double  Gbar, modulation;
 } hsg_leak_desc;
+
+/* One SynChan (biophysics/SynChan.cpp) on a compartment's "channel" message,
+ * with the SimpleSynHandler whose activationOut feeds it
+ * (synapse/SimpleSynHandler.cpp): n_synapses synapses, the next n_synapses
+ * entries of hsg_cell_desc.synapses (synchans in array order).  Per tick
+ * the handler pops the events due (time <= t) and sends sum(weight / dt)
+ * (clock tick 1), then the channel advances X and Y and sends
+ * (Y * norm * modulation, Ek) to the compartment's external current
+ * (clock tick 4), before the solver's tick 6 (SynChan.cpp:200-227). */
+typedef struct hsg_synchan_desc {
+    int32_t compartment;
+    int32_t n_synapses;
+    double  Gbar, Ek, tau1, tau2, modulation;
+} hsg_synchan_desc;
+
+/* One synapse of a SimpleSynHandler (synapse/Synapse.cpp:73-113): a spike
+ * at time t arriving through it becomes the event (t + delay, weight). */
+typedef struct hsg_synapse_desc {
+    double weight, delay;
+} hsg_synapse_desc;
 
 /* One 2-D interpolation table (builtins/Interpol2D.h): (xdivs+1) x (ydivs+1)
  * values, x-major (table[ix * (ydivs+1) + iy]), over [xmin, xmax] x
@@ -177,6 +204,9 @@ typedef struct hsg_cell_desc {
     const hsg_chan2d_desc*      chans2d;
     int32_t                     n_pulsegens;
     const hsg_pulsegen_desc*    pulsegens;
+    int32_t                     n_synchans;
+    const hsg_synchan_desc*     synchans;
+    const hsg_synapse_desc*     synapses;   /* sum of the synchans' n_synapses */
 } hsg_cell_desc;
 
 /* ---- engine ------------------------------------------------------------ */
@@ -270,6 +300,25 @@ int hsg_step_cells(hsg_engine* e, const int32_t* cells, const int64_t* nsteps,
 #define HSG_F_2D_Y     65
 #define HSG_F_2D_Z     66
 #define HSG_F_2D_MODULATION 67
+/* SynChan fields (index = synchans entry; biophysics/SynChan.cpp:117-160,
+ * ChanBase.cpp:90-117).  Writing tau1, tau2 or Gbar recomputes the
+ * integration constants and norm at the current dt, as the setters do. */
+#define HSG_F_SYN_GBAR  80
+#define HSG_F_SYN_EK    81
+#define HSG_F_SYN_GK    82   /* read-only: Gk of the last tick */
+#define HSG_F_SYN_TAU1  83
+#define HSG_F_SYN_TAU2  84
+#define HSG_F_SYN_MODULATION 85
+#define HSG_F_SYN_X     86   /* read-only: the rise state X */
+#define HSG_F_SYN_Y     87   /* read-only: the conductance state Y */
+/* Synapse fields (index = synapses entry) */
+#define HSG_F_SYNAPSE_WEIGHT 90
+#define HSG_F_SYNAPSE_DELAY  91
+/* Leakage fields (index = leaks entry): the per-tick send; Gk and the
+ * fold into EmByRm stay as set up */
+#define HSG_F_LEAK_GBAR 95
+#define HSG_F_LEAK_EK   96
+#define HSG_F_LEAK_MODULATION 97
 
 /* Reads n values of `field` for objects idx[0..n-1] of `cell` into out. */
 int hsg_get_field(hsg_engine* e, int32_t cell, int32_t field,
@@ -287,6 +336,19 @@ int hsg_add_gkek(hsg_engine* e, int32_t cell, int32_t compartment, double Gk, do
 int hsg_add_conc(hsg_engine* e, int32_t cell, int32_t channel, double conc);
 /* HSolve::iCa: extra Ca-pool activation (HSolveInterface.cpp:469-480). */
 int hsg_add_ica(hsg_engine* e, int32_t cell, int32_t pool, double iCa);
+/* Synapse::addSpike (synapse/Synapse.cpp:104-113): a spike at `time` through
+ * synapse `synapse` of synchan `synchan` (both indices into the cell's
+ * description) queues the event (time + delay, weight) on the synchan's
+ * handler.  Events stay queued until a tick whose end time reaches them. */
+int hsg_add_spike(hsg_engine* e, int32_t cell, int32_t synchan, int32_t synapse, double time);
+/* A SpikeGen's spikeOut -> Synapse addSpike message between cells of the
+ * engine (or within one): every spike of spikegen `spikegen` of src_cell at
+ * time t = dt * step becomes an event on dst_cell's synchan at t + delay.
+ * hsg_step then advances the batch in stretches no longer than the shortest
+ * such connection's delay allows (at least one tick), routing the spikes of
+ * each stretch before the next. */
+int hsg_connect_spike(hsg_engine* e, int32_t src_cell, int32_t spikegen, int32_t dst_cell,
+                      int32_t synchan, int32_t synapse);
 /* PulseGen field writes (setCount/setLevel/setWidth/setDelay/setBaseLevel,
  * device/PulseGen.cpp:243-340) on the cell's pulsegen-th PulseGen, taking
  * effect from the next step; its compartment must stay the one it was added

@@ -32,11 +32,17 @@ FIELDS = {
     "modulation": 27, "instant": 28,
     "Ca": 40, "CaBasal": 41, "ceiling": 42, "floor": 43,
     "Gbar2D": 60, "Ek2D": 61, "Gk2D": 62, "X2D": 64, "Y2D": 65, "Z2D": 66, "modulation2D": 67,
+    "GbarSyn": 80, "EkSyn": 81, "GkSyn": 82, "tau1": 83, "tau2": 84, "modulationSyn": 85,
+    "XSyn": 86, "YSyn": 87, "weight": 90, "delay": 91,
+    "GbarLeak": 95, "EkLeak": 96, "modulationLeak": 97,
 }
 COMPARTMENT_FIELDS = {k for k, v in FIELDS.items() if v < 20}
 CHANNEL_FIELDS = {k for k, v in FIELDS.items() if 20 <= v < 40}
 POOL_FIELDS = {k for k, v in FIELDS.items() if 40 <= v < 60}
-CHAN2D_FIELDS = {k for k, v in FIELDS.items() if v >= 60}
+CHAN2D_FIELDS = {k for k, v in FIELDS.items() if 60 <= v < 80}
+SYNCHAN_FIELDS = {k for k, v in FIELDS.items() if 80 <= v < 90}
+SYNAPSE_FIELDS = {k for k, v in FIELDS.items() if 90 <= v < 95}
+LEAK_FIELDS = {k for k, v in FIELDS.items() if v >= 95}
 
 
 class GateDesc(C.Structure):
@@ -62,7 +68,8 @@ class CellDesc(C.Structure):
                 ("n_leaks", C.c_int32), ("leaks", C.c_void_p),
                 ("n_tables2d", C.c_int32), ("tables2d", C.POINTER(Table2DDesc)),
                 ("n_chans2d", C.c_int32), ("chans2d", C.c_void_p),
-                ("n_pulsegens", C.c_int32), ("pulsegens", C.c_void_p)]
+                ("n_pulsegens", C.c_int32), ("pulsegens", C.c_void_p),
+                ("n_synchans", C.c_int32), ("synchans", C.c_void_p), ("synapses", C.c_void_p)]
 
 
 class Options(C.Structure):
@@ -93,13 +100,18 @@ CHANNEL_DT = np.dtype([("compartment", "i4"), ("instant", "i4"), ("useConcentrat
 CACONC_DT = np.dtype([(k, "f8") for k in ("Ca", "CaBasal", "tau", "B", "ceiling", "floor")])
 SPIKEGEN_DT = np.dtype([("compartment", "i4"), ("edgeTriggered", "i4"),
                         ("threshold", "f8"), ("refractT", "f8")], align=True)
-LEAK_DT = np.dtype([("compartment", "i4"), ("Gk", "f8"), ("Ek", "f8")], align=True)
+LEAK_DT = np.dtype([("compartment", "i4"), ("Gk", "f8"), ("Ek", "f8"), ("Gbar", "f8"),
+                    ("modulation", "f8")], align=True)
+SYNCHAN_DT = np.dtype([("compartment", "i4"), ("n_synapses", "i4")]
+                      + [(k, "f8") for k in ("Gbar", "Ek", "tau1", "tau2", "modulation")], align=True)
+SYNAPSE_DT = np.dtype([("weight", "f8"), ("delay", "f8")])
 CHAN2D_DT = np.dtype([("compartment", "i4"), ("instant", "i4"), ("tableA", "i4", (3,)),
                       ("tableB", "i4", (3,)), ("dep", "i4", (6,)), ("conc1", "i4"), ("conc2", "i4")]
                      + [(k, "f8") for k in ("Gbar", "Ek", "Xpower", "Ypower", "Zpower",
                                             "X", "Y", "Z", "modulation")], align=True)
 
-assert CHANNEL_DT.itemsize == 104 and SPIKEGEN_DT.itemsize == 24 and LEAK_DT.itemsize == 24
+assert CHANNEL_DT.itemsize == 104 and SPIKEGEN_DT.itemsize == 24 and LEAK_DT.itemsize == 40
+assert SYNCHAN_DT.itemsize == 48 and SYNAPSE_DT.itemsize == 16
 MAX_PULSES = 4
 PULSEGEN_DT = np.dtype([("compartment", "i4"), ("n_pulses", "i4"), ("baseLevel", "f8"),
                         ("delay", "f8", (MAX_PULSES,)), ("width", "f8", (MAX_PULSES,)),
@@ -113,6 +125,21 @@ def _ptr(a):
     return a.ctypes.data if a is not None and len(a) else None
 
 
+def as_records(a, dt):
+    """a as a contiguous array of dt; a structured array of another layout
+    (an older fixture) is copied field by field, missing fields zero."""
+    a = np.asarray(a)
+    if a.dtype == dt:
+        return np.ascontiguousarray(a)
+    if a.dtype.names is None:
+        return np.ascontiguousarray(a, dtype=dt)
+    out = np.zeros(a.shape, dt)
+    for name in a.dtype.names:
+        if name in dt.names:
+            out[name] = a[name]
+    return out
+
+
 class PackedCell:
     """A cell description packed into C layout; keeps its arrays alive."""
 
@@ -123,7 +150,11 @@ class PackedCell:
         self.channels = np.ascontiguousarray(spec.channels, dtype=CHANNEL_DT)
         self.pools = np.ascontiguousarray(spec.pools, dtype=CACONC_DT)
         self.spikegens = np.ascontiguousarray(spec.spikegens, dtype=SPIKEGEN_DT)
-        self.leaks = np.ascontiguousarray(spec.leaks, dtype=LEAK_DT)
+        self.leaks = as_records(spec.leaks, LEAK_DT)
+        self.synchans = as_records(getattr(spec, "synchans", np.zeros(0, SYNCHAN_DT)), SYNCHAN_DT)
+        self.synapses = as_records(getattr(spec, "synapses", np.zeros(0, SYNAPSE_DT)), SYNAPSE_DT)
+        if len(self.synapses) != int(self.synchans["n_synapses"].sum()):
+            raise ValueError("synapses: need the sum of the synchans' n_synapses entries")
         self.chans2d = np.ascontiguousarray(spec.chans2d, dtype=CHAN2D_DT)
         self.pulsegens = np.ascontiguousarray(getattr(spec, "pulsegens", np.zeros(0, PULSEGEN_DT)),
                                               dtype=PULSEGEN_DT)
@@ -174,6 +205,9 @@ class PackedCell:
         d.chans2d = _ptr(self.chans2d)
         d.n_pulsegens = len(self.pulsegens)
         d.pulsegens = _ptr(self.pulsegens)
+        d.n_synchans = len(self.synchans)
+        d.synchans = _ptr(self.synchans)
+        d.synapses = _ptr(self.synapses)
         self.desc = d
 
     @property

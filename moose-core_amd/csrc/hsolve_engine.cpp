@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstddef>
@@ -30,7 +31,7 @@ extern "C" hipError_t hsg_launch_solve(const hsg::LaunchArgs* a, int nblocks, in
                                       hipStream_t stream);
 extern "C" hipError_t hsg_launch_channel(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, int fp32,
                                         hipStream_t stream);
-extern "C" hipError_t hsg_launch_chan2d(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, hipStream_t stream);
+extern "C" hipError_t hsg_launch_external(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, hipStream_t stream);
 extern "C" hipError_t hsg_launch_channel_fast(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, int fp32,
                                              int ms, hipStream_t stream);
 extern "C" hipError_t hsg_launch_chan2d_reinit(const hsg::LaunchArgs* a, int nblocks, hipStream_t stream);
@@ -104,6 +105,75 @@ struct Block {
     int64_t off = -1, bytes = 0;
 };
 
+// SynChan's integration constants and norm at step dt (setTau1, setTau2 and
+// normalizeGbar, biophysics/SynChan.cpp:123-198; reinit, :229-251).
+struct SynConst {
+    double dt = 25.0e-6;       // SynChan's dt_ until a reinit
+    double xconst1 = 0, xconst2 = 0, yconst1 = 1, yconst2 = 0, norm = 0;
+};
+
+// doubleEq (basecode/doubleEq.cpp:16-22)
+bool double_eq(double x, double y) {
+    double denom = std::fabs(x) + std::fabs(y);
+    if (denom < 1e-12) denom = 1e-6;
+    return std::fabs(x - y) / denom < 1e-6;
+}
+
+SynConst syn_constants(const HostSyn& q, double dt) {
+    SynConst c;
+    c.dt = dt;
+    c.xconst1 = q.tau1 * (1.0 - std::exp(-dt / q.tau1));
+    c.xconst2 = std::exp(-dt / q.tau1);
+    if (double_eq(q.tau2, 0.0)) {
+        c.yconst1 = 1.0;
+        c.yconst2 = 0.0;
+    } else {
+        c.yconst1 = q.tau2 * (1.0 - std::exp(-dt / q.tau2));
+        c.yconst2 = std::exp(-dt / q.tau2);
+    }
+    if (double_eq(q.tau2, 0.0)) {
+        c.norm = q.Gbar;
+    } else if (double_eq(q.tau1, q.tau2)) {
+        c.norm = q.Gbar * std::exp(1.0) / q.tau1;
+    } else {
+        const double tpeak = q.tau1 * q.tau2 * std::log(q.tau1 / q.tau2) / (q.tau1 - q.tau2);
+        c.norm = q.Gbar * (q.tau1 - q.tau2) /
+                 (q.tau1 * q.tau2 * (std::exp(-tpeak / q.tau1) - std::exp(-tpeak / q.tau2)));
+    }
+    return c;
+}
+
+std::vector<LeakDev> leak_records(const CellSetup& s) {
+    std::vector<LeakDev> out(s.leaks.size());
+    for (size_t k = 0; k < s.leaks.size(); ++k) {
+        const double G = s.leaks[k].Gbar * s.leaks[k].mod;
+        out[k].G = G;
+        out[k].GE = G * s.leaks[k].Ek;
+    }
+    return out;
+}
+
+std::vector<SynDev> syn_records(const CellSetup& s, const std::vector<SynConst>& cst,
+                                const std::vector<std::array<double, 3>>& state) {
+    std::vector<SynDev> out(s.syn.size());
+    for (size_t k = 0; k < s.syn.size(); ++k) {
+        SynDev& d = out[k];
+        std::memset(&d, 0, sizeof d);
+        d.compt = s.syn[k].compt;
+        d.Ek = s.syn[k].Ek;
+        d.mod = s.syn[k].mod;
+        d.xconst1 = cst[k].xconst1;
+        d.xconst2 = cst[k].xconst2;
+        d.yconst1 = cst[k].yconst1;
+        d.yconst2 = cst[k].yconst2;
+        d.norm = cst[k].norm;
+        d.X = state[k][0];
+        d.Y = state[k][1];
+        d.Gk = state[k][2];
+    }
+    return out;
+}
+
 struct HostCell {
     int id = -1;
     bool live = false;
@@ -128,6 +198,13 @@ struct HostCell {
     // Gk as of the last step for channels whose X/Y/Z, Gbar or modulation was
     // written since (the reference keeps current_[].Gk until the next step)
     std::map<int, double> gkOverride;
+    // SynChans (s.syn order): integration constants at their dt, the state
+    // when it was last uploaded, and the handlers' queues of (time, weight),
+    // sorted by time, equal times in arrival order
+    std::vector<SynConst> synConst;
+    std::vector<std::array<double, 3>> synState;   // X, Y, Gk
+    std::vector<std::vector<std::pair<double, double>>> synQ;
+    Block act;                                     // the call's activation list
 };
 
 }  // namespace
@@ -175,7 +252,7 @@ struct hsg_engine {
         int2* dTiles = nullptr;     // channel-kernel tiles (split mode)
         int nTiles = 0;
         bool msg = false;           // message-passing solve kernel (split mode)
-        bool has2d = false;         // cells with HHChannel2D objects: launched tick by tick
+        bool hasExt = false;        // cells with objects sending every tick: launched tick by tick
         bool split = false;         // channel kernel + solve kernel per tick
         bool slow = false;          // cells with junction groups of four or more members
         int chanMS = 0;             // channel kernel's fast pass for up to chanMS slots (0: generic)
@@ -188,6 +265,12 @@ struct hsg_engine {
     static constexpr int kMaxBins = 32;
     hipStream_t binStream[kMaxBins] = {};
     hipEvent_t evFork = nullptr, evJoin[kMaxBins] = {};
+    // SpikeGen spikeOut -> Synapse addSpike messages between cells: (source
+    // cell, spikegen desc index) -> (target cell, synchan desc index,
+    // synapse index within it), in connection order
+    struct SpikeTarget { int32_t cell, synchan, synapse; };
+    std::map<std::pair<int32_t, int32_t>, std::vector<SpikeTarget>> conns;
+    std::vector<SpikeRec> spikeBuf;   // spikes read for routing, not yet handed out
     // graph of the steady-state launches
     hipGraphExec_t graphExec = nullptr;
     int graphSteps = 0;
@@ -383,6 +466,7 @@ int place_cell(hsg_engine* e, HostCell& hc) {
     // channel phase needs more threads than one workgroup brings
     const bool wantSplit = e->split || (e->opt.kernel_mode == HSG_KERNEL_AUTO && n >= kAutoSplitMin);
     const bool msgOn = wantSplit && s.msgOk && msgLds <= e->ldsLimit;
+    const int64_t nX = (int64_t)(s.ch2d.size() + s.leaks.size() + s.syn.size());
     std::vector<Item> items = {
         {&d.oV, 8 * n}, {&d.oInitVm, 8 * n}, {&d.oCmByDt, 8 * n}, {&d.oEmByRm, 8 * n}, {&d.oDbase, 8 * n}, {&d.oOff, 8 * n},
         {&d.oDefer, 8 * n},
@@ -412,9 +496,17 @@ int place_cell(hsg_engine* e, HostCell& hc) {
         {&d.o2dChan, (int64_t)sizeof(Chan2DDev) * (int64_t)s.ch2d.size()},
         {&d.o2dState, 24 * (int64_t)s.ch2d.size()},
         {&d.o2dGk, 8 * (int64_t)s.ch2d.size()},
-        {&d.o2dExt, s.ch2d.empty() ? 0 : 16 * (int64_t)n},
+        {&d.o2dExt, nX == 0 ? 0 : 16 * (int64_t)n},
         // PulseGens
         {&d.oPulse, (int64_t)sizeof(PulseDev) * (int64_t)s.pulses.size()},
+        // Leakage sends and SynChans
+        {&d.oXFlag, nX == 0 ? 0 : 4 * (int64_t)n},
+        {&d.oLeakStart, s.leaks.empty() ? 0 : 4 * ((int64_t)n + 1)},
+        {&d.oLeak, (int64_t)sizeof(LeakDev) * (int64_t)s.leaks.size()},
+        {&d.oSynStart, s.syn.empty() ? 0 : 4 * ((int64_t)n + 1)},
+        {&d.oSyn, (int64_t)sizeof(SynDev) * (int64_t)s.syn.size()},
+        {&d.oActCur, 4 * (int64_t)s.syn.size()},
+        {&d.oActEnd, 4 * (int64_t)s.syn.size()},
     };
     int64_t total = 0;
     for (auto& it : items) total += align_up(std::max<int64_t>(it.bytes, 8));
@@ -557,7 +649,27 @@ int place_cell(hsg_engine* e, HostCell& hc) {
         }
         st.put(d.oPulse, pd);
     }
+    if (nX > 0) {
+        std::vector<int32_t> flag(n, 0);
+        for (int i = 0; i < n; ++i)
+            flag[i] = (!s.ch2d.empty() && s.ch2dStart[i] < s.ch2dStart[i + 1]) ||
+                      (!s.leaks.empty() && s.leakStart[i] < s.leakStart[i + 1]) ||
+                      (!s.syn.empty() && s.synStart[i] < s.synStart[i + 1]);
+        st.put(d.oXFlag, flag);
+    }
+    if (!s.leaks.empty()) {
+        st.put(d.oLeakStart, s.leakStart);
+        st.put(d.oLeak, leak_records(s));
+    }
+    if (!s.syn.empty()) {
+        st.put(d.oSynStart, s.synStart);
+        st.put(d.oSyn, syn_records(s, hc.synConst, hc.synState));
+    }
     if ((rc = st.flush(e))) return rc;
+    d.nXObj = (int32_t)nX;
+    d.nLeak = (int32_t)s.leaks.size();
+    d.nSyn = (int32_t)s.syn.size();
+    d.oAct = 0;
     d.msg = msgOn ? 1 : 0;
     d.n2d = (int32_t)s.ch2d.size();
     d.nPulse = (int32_t)s.pulses.size();
@@ -811,7 +923,7 @@ int make_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg_
     auto cls = [&](int id) {
         const CellDev& d = e->cells[id]->dev;
         const int s = (d.slowHJ ? 8 : 0) + (autoMode && !d.msg && d.n >= kAutoSplitMin ? 16 : 0);
-        if (d.n2d > 0) return s + (d.msg ? 6 : 7);
+        if (d.nXObj > 0) return s + (d.msg ? 6 : 7);
         if (d.msg) return s + (d.n >= 1024 ? 0 : d.n >= 256 ? 1 : 2);
         return s + 3 + (d.ldsBytes > 32768 ? 0 : d.ldsBytes > 16384 ? 1 : 2);
     };
@@ -831,7 +943,7 @@ int make_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg_
         }
         curCls = k;
         cur.msg = (k & 7) < 3 || (k & 7) == 6;
-        cur.has2d = (k & 7) >= 6;
+        cur.hasExt = (k & 7) >= 6;
         cur.slow = (k & 8) != 0;
         cur.split = e->split || cur.msg || (k & 16) != 0;
         cur.cells.push_back(id);
@@ -928,6 +1040,131 @@ void consume_inputs(hsg_engine* e, HostCell& hc, int64_t nsteps) {
     hc.stepsDone += nsteps;
 }
 
+// SimpleSynHandler::vProcess (synapse/SimpleSynHandler.cpp:104-112) for the
+// next nsteps ticks of the cell: per SynChan, the ticks whose end time
+// reaches queued events and the activation sum(weight / dt) each sends, in
+// the order the queue pops them.  The list goes to the device, where the
+// SynChan step reads it tick by tick.  Events arriving later (from spikes
+// of these ticks) are due after them: hsg_step keeps each call within the
+// shortest connection delay.
+int write_activations(hsg_engine* e, HostCell& hc, int64_t nsteps) {
+    const CellSetup& s = hc.s;
+    const int ns = (int)s.syn.size();
+    if (ns == 0) return 0;
+    const double dt = e->dt;
+    const int64_t s0 = hc.stepsDone;
+    std::vector<ActDev> acts;
+    std::vector<int32_t> cur(ns), end(ns);
+    for (int k = 0; k < ns; ++k) {
+        cur[k] = (int32_t)acts.size();
+        auto& q = hc.synQ[k];
+        size_t p = 0;
+        while (p < q.size()) {
+            // the first tick of this call whose end time reaches event p
+            const double tq = q[p].first;
+            int64_t st = s0 + 1;
+            const double est = std::ceil(tq / dt);
+            if (est > (double)st) st = est > (double)(s0 + nsteps + 1) ? s0 + nsteps + 1 : (int64_t)est;
+            while (st > s0 + 1 && tq <= dt * (double)(unsigned long long)(st - 1)) --st;
+            while (st <= s0 + nsteps && !(tq <= dt * (double)(unsigned long long)st)) ++st;
+            if (st > s0 + nsteps) break;
+            const double t = dt * (double)(unsigned long long)st;
+            double a = 0.0;
+            while (p < q.size() && q[p].first <= t) {
+                a += q[p].second / dt;
+                ++p;
+            }
+            if (a != 0.0) acts.push_back(ActDev{st, a});
+        }
+        q.erase(q.begin(), q.begin() + (std::ptrdiff_t)p);
+        end[k] = (int32_t)acts.size();
+    }
+    const int64_t need = std::max<int64_t>(1, (int64_t)acts.size()) * (int64_t)sizeof(ActDev);
+    if (hc.act.off < 0 || hc.act.bytes < need) {
+        if (hc.act.off >= 0) e->arena.release(hc.act.off, hc.act.bytes);
+        hc.act.off = -1;
+        int64_t off;
+        int rc = arena_alloc(e, 2 * need, &off);
+        if (rc) return rc;
+        hc.act.off = off;
+        hc.act.bytes = 2 * need;
+    }
+    int rc;
+    if (!acts.empty() && (rc = upload(e, hc.act.off, acts))) return rc;
+    if ((rc = upload(e, hc.dev.oActCur, cur)) || (rc = upload(e, hc.dev.oActEnd, end))) return rc;
+    if (hc.dev.oAct != hc.act.off) {
+        hc.dev.oAct = hc.act.off;
+        e->tablesDirty = true;
+    }
+    return 0;
+}
+
+// An event (time, weight) into a handler's queue: after the events of equal
+// or earlier time (SimpleSynHandler::addSpike, :93-97).
+void queue_event(std::vector<std::pair<double, double>>& q, double time, double weight) {
+    auto it = std::upper_bound(q.begin(), q.end(), time,
+                               [](double t, const std::pair<double, double>& x) { return t < x.first; });
+    q.insert(it, {time, weight});
+}
+
+// Moves the spikes of the last launches from the device ring into the
+// engine's buffer and delivers each along its connections
+// (Synapse::addSpike, synapse/Synapse.cpp:104-113): the spike at t = dt *
+// step becomes the event (t + delay, weight).  Spikes are delivered by
+// step, then cell, then spike generator (the order the solvers send them).
+int route_spikes(hsg_engine* e) {
+    unsigned long long cnt = 0;
+    HIPCHK(hipMemcpyAsync(&cnt, e->dSpikeCount, sizeof cnt, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (cnt == 0) return 0;
+    const unsigned long long have = std::min<unsigned long long>(cnt, kSpikeCap);
+    std::vector<SpikeRec> recs(have);
+    HIPCHK(hipMemcpyAsync(recs.data(), e->dSpikes, sizeof(SpikeRec) * have, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemsetAsync(e->dSpikeCount, 0, sizeof(unsigned long long), e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (cnt > have) return fail(HSG_E_STATE, "spike ring overflow between routed stretches");
+    std::sort(recs.begin(), recs.end(), [](const SpikeRec& a, const SpikeRec& b) {
+        if (a.step != b.step) return a.step < b.step;
+        if (a.cell != b.cell) return a.cell < b.cell;
+        return a.gen < b.gen;
+    });
+    for (const SpikeRec& r : recs) {
+        e->spikeBuf.push_back(r);
+        HostCell* src = get_cell(e, r.cell);
+        if (!src) continue;
+        auto it = e->conns.find({r.cell, src->s.spkDesc[r.gen]});
+        if (it == e->conns.end()) continue;
+        const double t = e->dt * (double)(unsigned long long)r.step;
+        for (const auto& tg : it->second) {
+            HostCell* dst = get_cell(e, tg.cell);
+            if (!dst) continue;
+            const int q = dst->s.synOfDesc[tg.synchan];
+            const int j = dst->s.syn[q].firstSyn + tg.synapse;
+            queue_event(dst->synQ[q], t + dst->s.synDelay[j], dst->s.synW[j]);
+        }
+    }
+    return 0;
+}
+
+// Longest stretch of ticks whose spikes are all due after it: a spike at
+// step s through delay d is due at the first step s' > s with
+// dt * s + d <= dt * s', so stretches of floor(d_min / dt) ticks (at least
+// one) never need a spike of their own.
+int64_t routed_stretch(hsg_engine* e) {
+    double dmin = -1.0;
+    for (auto& kv : e->conns)
+        for (const auto& tg : kv.second) {
+            HostCell* dst = get_cell(e, tg.cell);
+            if (!dst) continue;
+            const int q = dst->s.synOfDesc[tg.synchan];
+            const double d = dst->s.synDelay[dst->s.syn[q].firstSyn + tg.synapse];
+            if (dmin < 0.0 || d < dmin) dmin = d;
+        }
+    if (dmin < 0.0) return INT64_MAX;
+    const double k = std::floor(dmin / e->dt);
+    return k < 1.0 ? 1 : k > 1e12 ? (int64_t)1e12 : (int64_t)k;
+}
+
 // The fused kernel's fast channel pass (two-slot uniform cells) holds more
 // registers, so fewer workgroups share a CU: it pays off where each thread
 // has many compartments to sweep (measured crossover on MI355X).
@@ -949,14 +1186,14 @@ int launch_round(hsg_engine* e, std::vector<hsg_engine::Bin>& bins, int chunk, i
         HIPCHK(hipStreamWaitEvent(st, e->evFork, 0));
         const int nb = (int)bins[b].cells.size();
         const bool split = bins[b].split;
-        if (split || bins[b].has2d) {
+        if (split || bins[b].hasExt) {
             // tick by tick: the HHChannel2D objects (bins that have them),
             // then the channel kernel over the bin's tiles and the solve, or
             // one fused tick
             for (int k = 0; k < chunk; ++k) {
                 LaunchArgs a = make_args(e, bins[b].dOrder, 1, first && k == 0);
                 a.ldsBytes = bins[b].lds;
-                if (bins[b].has2d) HIPCHK(hsg_launch_chan2d(&a, bins[b].dTiles, bins[b].nTiles, st));
+                if (bins[b].hasExt) HIPCHK(hsg_launch_external(&a, bins[b].dTiles, bins[b].nTiles, st));
                 if (!split) {
                     HIPCHK(hsg_launch_step(&a, nb, bins[b].threads, e->fp32, bins[b].slow, fused_fast(bins[b]), st));
                     continue;
@@ -1085,6 +1322,19 @@ int reinit_cells_host(hsg_engine* e, HostCell& hc) {
                 hc.injDirty = true;
             }
     }
+    // SynChan::vReinit at the clock's dt, handlers' queues dropped
+    // (SimpleSynHandler::vReinit)
+    if (!s.syn.empty()) {
+        for (size_t k = 0; k < s.syn.size(); ++k) {
+            hc.synConst[k] = syn_constants(s.syn[k], e->dt);
+            hc.synState[k] = {0.0, 0.0, 0.0};
+            hc.synQ[k].clear();
+        }
+        int rc = upload(e, hc.dev.oSyn, syn_records(s, hc.synConst, hc.synState));
+        if (rc) return rc;
+        std::vector<int32_t> zero(s.syn.size(), 0);
+        if ((rc = upload(e, hc.dev.oActCur, zero)) || (rc = upload(e, hc.dev.oActEnd, zero))) return rc;
+    }
     hc.dev.probeBase = 0;
     e->tablesDirty = true;
     hc.ext.clear();
@@ -1189,6 +1439,11 @@ int hsg_add_cell(hsg_engine* e, const hsg_cell_desc* desc, double dt, int32_t ca
     if (id < 0) { id = (int)e->cells.size(); e->cells.emplace_back(); }
     hc->id = id;
     hc->hsolveDt = dt;
+    // SynChans as the objects hold them before any reinit: constants at
+    // their default dt_, X = Y = Gk = 0, no queued events
+    for (const HostSyn& q : hc->s.syn) hc->synConst.push_back(syn_constants(q, 25.0e-6));
+    hc->synState.assign(hc->s.syn.size(), {0.0, 0.0, 0.0});
+    hc->synQ.assign(hc->s.syn.size(), {});
     rc = place_cell(e, *hc);
     if (rc) {
         if (hc->region.off >= 0) e->arena.release(hc->region.off, hc->region.bytes);
@@ -1223,6 +1478,15 @@ int hsg_remove_cell(hsg_engine* e, int32_t cell) {
     if (hc->inj.off >= 0) e->arena.release(hc->inj.off, hc->inj.bytes);
     if (hc->probe.off >= 0) e->arena.release(hc->probe.off, hc->probe.bytes);
     if (hc->extBlk.off >= 0) e->arena.release(hc->extBlk.off, hc->extBlk.bytes);
+    if (hc->act.off >= 0) e->arena.release(hc->act.off, hc->act.bytes);
+    // its spike connections go with it
+    for (auto it = e->conns.begin(); it != e->conns.end();) {
+        auto& v = it->second;
+        v.erase(std::remove_if(v.begin(), v.end(), [&](const hsg_engine::SpikeTarget& t) { return t.cell == cell; }),
+                v.end());
+        if (it->first.first == cell || v.empty()) it = e->conns.erase(it);
+        else ++it;
+    }
     e->cells[cell].reset();
     e->tablesDirty = true;
     return 0;
@@ -1267,6 +1531,7 @@ int hsg_reinit(hsg_engine* e, double dt) {
     std::vector<int32_t> ids;
     for (auto& c : e->cells) if (c && c->live) ids.push_back(c->id);
     HIPCHK(hipMemsetAsync(e->dSpikeCount, 0, sizeof(unsigned long long), e->stream));
+    e->spikeBuf.clear();
     return reinit_some(e, ids);
 }
 
@@ -1285,10 +1550,25 @@ int hsg_step(hsg_engine* e, int64_t nsteps) {
     if (nsteps == 0) return 0;
     hipSetDevice(e->device);
     int rc;
-    if ((rc = flush_inputs(e)) || (rc = upload_tables(e))) return rc;
-    std::vector<int64_t> steps(e->order.size(), nsteps);
-    if (!e->order.empty() && (rc = run_launches(e, e->order, steps))) return rc;
-    for (auto& c : e->cells) if (c && c->live) consume_inputs(e, *c, nsteps);
+    bool anySyn = false;
+    for (auto& c : e->cells) if (c && c->live && !c->s.syn.empty()) anySyn = true;
+    // with SynChans: stretches short enough that no spike of a stretch is
+    // due within it, the handlers' activations prepared before each and its
+    // spikes routed after it
+    const int64_t L = anySyn ? routed_stretch(e) : nsteps;
+    for (int64_t done = 0; done < nsteps;) {
+        const int64_t m = std::min(nsteps - done, L);
+        if ((rc = flush_inputs(e))) return rc;
+        if (anySyn)
+            for (auto& c : e->cells)
+                if (c && c->live && (rc = write_activations(e, *c, m))) return rc;
+        if ((rc = upload_tables(e))) return rc;
+        std::vector<int64_t> steps(e->order.size(), m);
+        if (!e->order.empty() && (rc = run_launches(e, e->order, steps))) return rc;
+        for (auto& c : e->cells) if (c && c->live) consume_inputs(e, *c, m);
+        if (!e->conns.empty() && (rc = route_spikes(e))) return rc;
+        done += m;
+    }
     return 0;
 }
 
@@ -1303,8 +1583,12 @@ int hsg_step_cells(hsg_engine* e, const int32_t* cells, const int64_t* nsteps, i
         if (nsteps[i] < 0) return fail(HSG_E_ARG, "negative step count");
         if (nsteps[i] > 0) { ord.push_back(cells[i]); st.push_back(nsteps[i]); }
     }
+    if (!e->conns.empty()) return fail(HSG_E_UNSUPPORTED, "cells joined by spike connections step with hsg_step");
     int rc;
-    if ((rc = flush_inputs(e)) || (rc = upload_tables(e))) return rc;
+    if ((rc = flush_inputs(e))) return rc;
+    for (size_t i = 0; i < ord.size(); ++i)
+        if ((rc = write_activations(e, *e->cells[ord[i]], st[i]))) return rc;
+    if ((rc = upload_tables(e))) return rc;
     if (!ord.empty() && (rc = run_launches(e, ord, st))) return rc;
     for (size_t i = 0; i < ord.size(); ++i) consume_inputs(e, *e->cells[ord[i]], st[i]);
     return 0;
@@ -1347,8 +1631,11 @@ int hsg_get_field(hsg_engine* e, int32_t cell, int32_t field, const int32_t* idx
                     Im += (ek - V[h]) * gk;
                 }
                 double eg = hc->prevExtG[h], ege = hc->prevExtGE[h];
-                if (!s.ch2d.empty() && s.ch2dStart[h] < s.ch2dStart[h + 1]) {
-                    // the last tick's external current includes the HHChannel2D objects
+                if ((!s.ch2d.empty() && s.ch2dStart[h] < s.ch2dStart[h + 1]) ||
+                    (!s.leaks.empty() && s.leakStart[h] < s.leakStart[h + 1]) ||
+                    (!s.syn.empty() && s.synStart[h] < s.synStart[h + 1])) {
+                    // the last tick's external current includes the objects
+                    // sending every tick
                     double ext[2];
                     if ((rc = download(e, hc->dev.o2dExt + 16 * (int64_t)h, 2, ext))) return rc;
                     if (hc->stepsDone > 0) {
@@ -1443,6 +1730,36 @@ int hsg_get_field(hsg_engine* e, int32_t cell, int32_t field, const int32_t* idx
                 break;
             default: return fail(HSG_E_ARG, "unknown field");
             }
+        } else if (field >= HSG_F_SYN_GBAR && field <= HSG_F_SYN_Y) {
+            if (i < 0 || i >= (int)s.synOfDesc.size()) return fail(HSG_E_ARG, "unknown SynChan index");
+            const int k = s.synOfDesc[i];
+            const HostSyn& hs = s.syn[k];
+            const int64_t rec = hc->dev.oSyn + (int64_t)sizeof(SynDev) * k;
+            switch (field) {
+            case HSG_F_SYN_GBAR: out[q] = hs.Gbar; break;
+            case HSG_F_SYN_EK: out[q] = hs.Ek; break;
+            case HSG_F_SYN_TAU1: out[q] = hs.tau1; break;
+            case HSG_F_SYN_TAU2: out[q] = hs.tau2; break;
+            case HSG_F_SYN_MODULATION: out[q] = hs.mod; break;
+            case HSG_F_SYN_GK:
+                if ((rc = download(e, rec + offsetof(SynDev, Gk), 1, &out[q]))) return rc;
+                break;
+            case HSG_F_SYN_X:
+                if ((rc = download(e, rec + offsetof(SynDev, X), 1, &out[q]))) return rc;
+                break;
+            case HSG_F_SYN_Y:
+                if ((rc = download(e, rec + offsetof(SynDev, Y), 1, &out[q]))) return rc;
+                break;
+            default: return fail(HSG_E_ARG, "unknown field");
+            }
+        } else if (field == HSG_F_SYNAPSE_WEIGHT || field == HSG_F_SYNAPSE_DELAY) {
+            if (i < 0 || i >= (int)s.synW.size()) return fail(HSG_E_ARG, "unknown synapse index");
+            out[q] = field == HSG_F_SYNAPSE_WEIGHT ? s.synW[i] : s.synDelay[i];
+        } else if (field >= HSG_F_LEAK_GBAR && field <= HSG_F_LEAK_MODULATION) {
+            if (i < 0 || i >= (int)s.leakOfDesc.size() || s.leakOfDesc[i] < 0)
+                return fail(HSG_E_ARG, "unknown Leakage index");
+            const HostLeak& l = s.leaks[s.leakOfDesc[i]];
+            out[q] = field == HSG_F_LEAK_GBAR ? l.Gbar : field == HSG_F_LEAK_EK ? l.Ek : l.mod;
         } else {
             return fail(HSG_E_ARG, "unknown field");
         }
@@ -1594,6 +1911,49 @@ int hsg_set_field(hsg_engine* e, int32_t cell, int32_t field, const int32_t* idx
                 break;
             default: return fail(HSG_E_ARG, "unknown field");
             }
+        } else if (field >= HSG_F_SYN_GBAR && field <= HSG_F_SYN_Y) {
+            if (i < 0 || i >= (int)s.synOfDesc.size()) return fail(HSG_E_ARG, "unknown SynChan index");
+            const int k = s.synOfDesc[i];
+            HostSyn& hs = s.syn[k];
+            const int64_t rec = hc->dev.oSyn + (int64_t)sizeof(SynDev) * k;
+            switch (field) {
+            case HSG_F_SYN_EK:
+                hs.Ek = x;
+                if ((rc = put1<double>(e, rec + offsetof(SynDev, Ek), 0, x))) return rc;
+                break;
+            case HSG_F_SYN_MODULATION:
+                hs.mod = x;
+                if ((rc = put1<double>(e, rec + offsetof(SynDev, mod), 0, x))) return rc;
+                break;
+            case HSG_F_SYN_GBAR: case HSG_F_SYN_TAU1: case HSG_F_SYN_TAU2: {
+                if (field == HSG_F_SYN_TAU1 && !(x > 0.0)) return fail(HSG_E_ARG, "SynChan: tau1 must be > 0");
+                if (field == HSG_F_SYN_TAU2 && !(x >= 0.0)) return fail(HSG_E_ARG, "SynChan: tau2 must be >= 0");
+                (field == HSG_F_SYN_GBAR ? hs.Gbar : field == HSG_F_SYN_TAU1 ? hs.tau1 : hs.tau2) = x;
+                // the setters recompute the constants at the object's dt_
+                SynConst& c = hc->synConst[k];
+                c = syn_constants(hs, c.dt);
+                const double v[5] = {c.xconst1, c.xconst2, c.yconst1, c.yconst2, c.norm};
+                HIPCHK(hipMemcpyAsync(e->arena.base + rec + offsetof(SynDev, xconst1), v, sizeof v,
+                                      hipMemcpyHostToDevice, e->stream));
+                HIPCHK(hipStreamSynchronize(e->stream));
+                break;
+            }
+            default: return fail(HSG_E_ARG, "read-only SynChan field");
+            }
+        } else if (field == HSG_F_SYNAPSE_WEIGHT || field == HSG_F_SYNAPSE_DELAY) {
+            if (i < 0 || i >= (int)s.synW.size()) return fail(HSG_E_ARG, "unknown synapse index");
+            if (field == HSG_F_SYNAPSE_DELAY && !(x >= 0.0)) return fail(HSG_E_ARG, "synapse delay must be >= 0");
+            (field == HSG_F_SYNAPSE_WEIGHT ? s.synW[i] : s.synDelay[i]) = x;
+        } else if (field >= HSG_F_LEAK_GBAR && field <= HSG_F_LEAK_MODULATION) {
+            if (i < 0 || i >= (int)s.leakOfDesc.size() || s.leakOfDesc[i] < 0)
+                return fail(HSG_E_ARG, "unknown Leakage index");
+            const int k = s.leakOfDesc[i];
+            HostLeak& l = s.leaks[k];
+            (field == HSG_F_LEAK_GBAR ? l.Gbar : field == HSG_F_LEAK_EK ? l.Ek : l.mod) = x;
+            const LeakDev rec = leak_records(s)[k];
+            HIPCHK(hipMemcpyAsync(e->arena.base + hc->dev.oLeak + (int64_t)sizeof(LeakDev) * k, &rec, sizeof rec,
+                                  hipMemcpyHostToDevice, e->stream));
+            HIPCHK(hipStreamSynchronize(e->stream));
         } else {
             return fail(HSG_E_ARG, "unknown field");
         }
@@ -1653,6 +2013,33 @@ int hsg_set_pulsegen(hsg_engine* e, int32_t cell, int32_t pulsegen, const hsg_pu
     return 0;
 }
 
+int hsg_add_spike(hsg_engine* e, int32_t cell, int32_t synchan, int32_t synapse, double time) {
+    HostCell* hc = get_cell(e, cell);
+    if (!hc) return fail(HSG_E_ARG, "no such cell");
+    const CellSetup& s = hc->s;
+    if (synchan < 0 || synchan >= (int)s.synOfDesc.size()) return fail(HSG_E_ARG, "no such SynChan");
+    const int q = s.synOfDesc[synchan];
+    if (synapse < 0 || synapse >= s.syn[q].nSyn) return fail(HSG_E_ARG, "no such synapse");
+    const int j = s.syn[q].firstSyn + synapse;
+    queue_event(hc->synQ[q], time + s.synDelay[j], s.synW[j]);
+    return 0;
+}
+
+int hsg_connect_spike(hsg_engine* e, int32_t src_cell, int32_t spikegen, int32_t dst_cell, int32_t synchan,
+                      int32_t synapse) {
+    HostCell* src = get_cell(e, src_cell);
+    HostCell* dst = get_cell(e, dst_cell);
+    if (!src || !dst) return fail(HSG_E_ARG, "no such cell");
+    bool found = false;
+    for (int g : src->s.spkDesc) found = found || g == spikegen;
+    if (!found) return fail(HSG_E_ARG, "no such SpikeGen in the source cell");
+    const CellSetup& s = dst->s;
+    if (synchan < 0 || synchan >= (int)s.synOfDesc.size()) return fail(HSG_E_ARG, "no such SynChan");
+    if (synapse < 0 || synapse >= s.syn[s.synOfDesc[synchan]].nSyn) return fail(HSG_E_ARG, "no such synapse");
+    e->conns[{src_cell, spikegen}].push_back(hsg_engine::SpikeTarget{dst_cell, synchan, synapse});
+    return 0;
+}
+
 int hsg_add_gkek(hsg_engine* e, int32_t cell, int32_t compartment, double Gk, double Ek) {
     HostCell* hc = get_cell(e, cell);
     if (!hc) return fail(HSG_E_ARG, "no such cell");
@@ -1698,6 +2085,8 @@ int hsg_read_spikes(hsg_engine* e, hsg_spike* out, int32_t cap, int32_t* n_out, 
     HIPCHK(hipMemsetAsync(e->dSpikeCount, 0, sizeof(unsigned long long), e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     e->spikesDropped += (int64_t)(cnt - have);
+    recs.insert(recs.end(), e->spikeBuf.begin(), e->spikeBuf.end());
+    e->spikeBuf.clear();
     std::sort(recs.begin(), recs.end(), [](const SpikeRec& a, const SpikeRec& b) {
         if (a.step != b.step) return a.step < b.step;
         if (a.cell != b.cell) return a.cell < b.cell;

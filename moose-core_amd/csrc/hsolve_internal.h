@@ -73,6 +73,28 @@ struct PulseDev {
     double  delay[HSG_MAX_PULSES], width[HSG_MAX_PULSES], level[HSG_MAX_PULSES];
 };
 
+// A Leakage object's per-tick send: (G, G * Ek) with G = Gbar * modulation
+// (biophysics/Leakage.cpp:90-95; HSolve::addGkEk).
+struct LeakDev {
+    double G, GE;
+};
+
+// A SynChan (biophysics/SynChan.cpp:200-227) run beside the solver.
+struct SynDev {
+    int32_t compt;              // Hines index
+    int32_t pad;
+    double  Ek, mod;
+    double  xconst1, xconst2, yconst1, yconst2, norm;
+    double  X, Y, Gk;           // state; Gk of the last tick
+};
+
+// One tick's activation of a SynChan (what its SimpleSynHandler sent,
+// synapse/SimpleSynHandler.cpp:104-112), prepared by the host per call.
+struct ActDev {
+    int64_t step;               // 1-based tick since reinit
+    double  value;
+};
+
 // Per-cell descriptor (read by every workgroup working on the cell).
 // One channel slot of a uniform cell for the fast channel pass, packed in
 // three words (read with scalar loads):
@@ -170,6 +192,21 @@ struct CellDev {
     int32_t nPulse;
     int32_t pad2;
     int64_t oPulse;         // PulseDev[nPulse]
+    // objects sending (Gk, Ek) every tick: Leakage, HHChannel2D, SynChan.
+    // Compartments that have any (oXFlag) take their external current from
+    // o2dExt, which the external-objects kernel fills before each tick
+    int32_t nXObj;          // n2d + nLeak + nSyn
+    int32_t nLeak;
+    int32_t nSyn;
+    int32_t pad3;
+    int64_t oXFlag;         // int32[n]
+    int64_t oLeakStart;     // int32[n + 1]
+    int64_t oLeak;          // LeakDev[nLeak]
+    int64_t oSynStart;      // int32[n + 1]
+    int64_t oSyn;           // SynDev[nSyn]
+    int64_t oActCur;        // int32[nSyn]: next activation entry of the call
+    int64_t oActEnd;        // int32[nSyn]
+    int64_t oAct;           // ActDev[]: the call's activations, by synchan then step
 };
 
 // Elimination unit descriptor: 12 int32 (48 bytes), read as three int4 loads.

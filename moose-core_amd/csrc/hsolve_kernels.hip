@@ -314,17 +314,21 @@ __device__ __forceinline__ double dep2d(int dep, bool sent, double Vm, const Cha
     return 0.0;
 }
 
-// The HHChannel2D objects of compartment i for one tick
-// (HHChannel2D::vProcess, :325-368), in their message order, added to the
-// compartment's external current (HSolve::addGkEk): first the call's host
-// entries (first tick of a call), then the channels.  The sum is kept in
-// o2dExt for the row and for Im.  Run by its own kernel before the tick's
-// solver kernels, so the step kernels carry none of its registers.
-__device__ __forceinline__ void chan2d_external(char* arena, const Table2DDev* __restrict__ tabs2d, double dt,
-                                             bool first, const CellDev& c, int i, double Vm,
-                                             const double* __restrict__ poolCa) {
-    const int32_t* start = A_<const int32_t>(arena, c.o2dStart);
-    if (start[i] == start[i + 1]) return;
+// The objects that send (Gk, Ek) to compartment i on every tick before the
+// solver's tick (clock tick 4), summed into the compartment's external
+// current (HSolve::addGkEk, hsolve/HSolveInterface.cpp:246-252) after the
+// call's host entries (first tick of a call): the Leakage objects
+// (Leakage::vProcess, biophysics/Leakage.cpp:90-95), the HHChannel2D
+// objects (HHChannel2D::vProcess, :325-368) and the SynChans
+// (SynChan::vProcess, biophysics/SynChan.cpp:200-227, with the activation
+// their handlers sent on tick 1), each kind in description order.  The sum
+// is kept in o2dExt for the row and for Im.  Run by its own kernel before
+// the tick's solver kernels, so the step kernels carry none of its
+// registers.
+__device__ __forceinline__ void external_objects(char* arena, const Table2DDev* __restrict__ tabs2d, double dt,
+                                                 bool first, const CellDev& c, int i, double Vm,
+                                                 const double* __restrict__ poolCa, int64_t stepNo) {
+    if (!A_<const int32_t>(arena, c.oXFlag)[i]) return;
     double G = 0.0, GE = 0.0;
     if (first) {
         const int32_t* xi = A_<const int32_t>(arena, c.oExtIdx);
@@ -334,33 +338,70 @@ __device__ __forceinline__ void chan2d_external(char* arena, const Table2DDev* _
                 GE += A_<const double>(arena, c.oExtGE)[e];
             }
     }
-    const Chan2DDev* chans = A_<const Chan2DDev>(arena, c.o2dChan);
-    double* st = A_<double>(arena, c.o2dState);
-    double* gk = A_<double>(arena, c.o2dGk);
-    const bool sent = c.values2dSent != 0;
+    if (c.nLeak > 0) {
+        const int32_t* ls = A_<const int32_t>(arena, c.oLeakStart);
+        const LeakDev* lk = A_<const LeakDev>(arena, c.oLeak);
 #pragma unroll 1
-    for (int k = start[i]; k < start[i + 1]; ++k) {
-        const Chan2DDev& ch = chans[k];
-        double g = 0.0;
-        g += ch.Gbar;
-#pragma unroll 1
-        for (int ax = 0; ax < 3; ++ax) {
-            if (!(ch.pw[ax] > 0.0)) continue;
-            const double u = dep2d(ch.dep[2 * ax], sent, Vm, ch, poolCa);
-            const double w = dep2d(ch.dep[2 * ax + 1], sent, Vm, ch, poolCa);
-            const Table2DDev& ta = tabs2d[ch.tabA[ax]];
-            const Table2DDev& tb = tabs2d[ch.tabB[ax]];
-            const double A = interp2d(ta, A_<const double>(arena, ta.off), u, w);
-            const double B = interp2d(tb, A_<const double>(arena, tb.off), u, w);
-            double x = st[3 * k + ax];
-            x = (ch.instant >> ax) & 1 ? A / B : integrate_ee(x, dt, A, B);
-            st[3 * k + ax] = x;
-            g *= take_power<double>(x, ch.pw[ax]);
+        for (int k = ls[i]; k < ls[i + 1]; ++k) {
+            G += lk[k].G;
+            GE += lk[k].GE;
         }
-        const double Gk = g * ch.mod;
-        gk[k] = Gk;
-        G += Gk;
-        GE += Gk * ch.Ek;
+    }
+    if (c.n2d > 0) {
+        const int32_t* start = A_<const int32_t>(arena, c.o2dStart);
+        const Chan2DDev* chans = A_<const Chan2DDev>(arena, c.o2dChan);
+        double* st = A_<double>(arena, c.o2dState);
+        double* gk = A_<double>(arena, c.o2dGk);
+        const bool sent = c.values2dSent != 0;
+#pragma unroll 1
+        for (int k = start[i]; k < start[i + 1]; ++k) {
+            const Chan2DDev& ch = chans[k];
+            double g = 0.0;
+            g += ch.Gbar;
+#pragma unroll 1
+            for (int ax = 0; ax < 3; ++ax) {
+                if (!(ch.pw[ax] > 0.0)) continue;
+                const double u = dep2d(ch.dep[2 * ax], sent, Vm, ch, poolCa);
+                const double w = dep2d(ch.dep[2 * ax + 1], sent, Vm, ch, poolCa);
+                const Table2DDev& ta = tabs2d[ch.tabA[ax]];
+                const Table2DDev& tb = tabs2d[ch.tabB[ax]];
+                const double A = interp2d(ta, A_<const double>(arena, ta.off), u, w);
+                const double B = interp2d(tb, A_<const double>(arena, tb.off), u, w);
+                double x = st[3 * k + ax];
+                x = (ch.instant >> ax) & 1 ? A / B : integrate_ee(x, dt, A, B);
+                st[3 * k + ax] = x;
+                g *= take_power<double>(x, ch.pw[ax]);
+            }
+            const double Gk = g * ch.mod;
+            gk[k] = Gk;
+            G += Gk;
+            GE += Gk * ch.Ek;
+        }
+    }
+    if (c.nSyn > 0) {
+        const int32_t* ss = A_<const int32_t>(arena, c.oSynStart);
+        SynDev* sy = A_<SynDev>(arena, c.oSyn);
+        int32_t* cur = A_<int32_t>(arena, c.oActCur);
+        const int32_t* end = A_<const int32_t>(arena, c.oActEnd);
+        const ActDev* act = A_<const ActDev>(arena, c.oAct);
+#pragma unroll 1
+        for (int k = ss[i]; k < ss[i + 1]; ++k) {
+            SynDev& q = sy[k];
+            double a = 0.0;
+            const int j = cur[k];
+            if (j < end[k] && act[j].step == stepNo) {
+                a = act[j].value;
+                cur[k] = j + 1;
+            }
+            const double X = a * q.xconst1 + q.X * q.xconst2;
+            const double Y = X * q.yconst1 + q.Y * q.yconst2;
+            const double Gk = Y * q.norm * q.mod;
+            q.X = X;
+            q.Y = Y;
+            q.Gk = Gk;
+            G += Gk;
+            GE += Gk * q.Ek;
+        }
     }
     double* ext = A_<double>(arena, c.o2dExt);
     ext[2 * i] = G;
@@ -802,10 +843,6 @@ __device__ __forceinline__ const ChanType* stage_types(const CellDev& c, const C
     return sTypes;
 }
 
-// Per-tick outputs once V of the tick is complete: spike generators
-// (SpikeGen::process at t = dt * step, biophysics/SpikeGen.cpp:202-216, fed
-// by HSolveActive::sendSpikes, hsolve/HSolveActive.cpp:342-347) into the
-// engine's spike ring, and Vm probes (Table sampling after the solver's tick).
 // PulseGen::process in FREE_RUN mode (device/PulseGen.cpp:359-424) at time
 // t: updates the output state; false when the reference sends nothing
 // (phase past the period).
@@ -848,6 +885,10 @@ __device__ __forceinline__ double inject_varying(char* arena, const CellDev& c, 
     return var;
 }
 
+// Per-tick outputs once V of the tick is complete: spike generators
+// (SpikeGen::process at t = dt * step, biophysics/SpikeGen.cpp:202-216, fed
+// by HSolveActive::sendSpikes, hsolve/HSolveActive.cpp:342-347) into the
+// engine's spike ring, and Vm probes (Table sampling after the solver's tick).
 __device__ __forceinline__ void tick_outputs(const LaunchArgs& a, const CellDev& c, const double* __restrict__ V,
                                              int64_t stepNo) {
     char* arena = a.arena;
@@ -1095,9 +1136,9 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
                 for (int i = tid; i < n; i += nt) channel_row<GT, false>(v, types, i, dtg, w, false, i);
         }
         // (split mode: the channel kernel has added them to the rows)
-        if (!SPLIT && (nInj > 0 || (first && nExt > 0) || c.n2d > 0)) {
+        if (!SPLIT && (nInj > 0 || (first && nExt > 0) || c.nXObj > 0)) {
             work_barrier(WL);
-            const int32_t* start2d = A_<const int32_t>(arena, c.o2dStart);
+            const int32_t* xflag = A_<const int32_t>(arena, c.oXFlag);
             if (tid == 0) {
                 // injections: HS[4i+3] += injectVarying + injectBasal (HSolveActive.cpp:119-128)
                 const int32_t* ii = A_<const int32_t>(arena, c.oInjIdx);
@@ -1113,19 +1154,19 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
                     const double* xg = A_<const double>(arena, c.oExtG);
                     const double* xge = A_<const double>(arena, c.oExtGE);
                     for (int e = 0; e < nExt; ++e) {
-                        if (c.n2d > 0 && start2d[xi[e]] < start2d[xi[e] + 1]) continue;   // in o2dExt
+                        if (c.nXObj > 0 && xflag[xi[e]]) continue;   // in o2dExt
                         w.D[xi[e]] += xg[e];
                         w.B[xi[e]] += xge[e];
                     }
                 }
             }
-            if (c.n2d > 0) {
-                // compartments with HHChannel2D objects: the whole external
-                // current, after the injection
+            if (c.nXObj > 0) {
+                // compartments with objects sending every tick: the whole
+                // external current, after the injection
                 work_barrier(WL);
                 const double* ext = A_<const double>(arena, c.o2dExt);
                 for (int i = tid; i < n; i += nt)
-                    if (start2d[i] < start2d[i + 1]) {
+                    if (xflag[i]) {
                         w.D[i] += ext[2 * i];
                         w.B[i] += ext[2 * i + 1];
                     }
@@ -1271,7 +1312,7 @@ __device__ __forceinline__ void row_dest(char* arena, const CellDev& c, int i, d
 // The rest of HSolveActive::updateMatrix for compartment i's row in split
 // mode: the injection, then the external current (HSolveActive.cpp:117-148).
 __device__ __forceinline__ void row_inputs(const LaunchArgs& a, const CellDev& c, int i, double& D, double& B) {
-    const bool has2d = c.n2d > 0 && A_<const int32_t>(a.arena, c.o2dStart)[i] < A_<const int32_t>(a.arena, c.o2dStart)[i + 1];
+    const bool has2d = c.nXObj > 0 && A_<const int32_t>(a.arena, c.oXFlag)[i];
     const int32_t* ii = A_<const int32_t>(a.arena, c.oInjIdx);
     for (int e = 0; e < c.nInj; ++e)
         if (ii[e] == i) {
@@ -1641,18 +1682,21 @@ __global__ void __launch_bounds__(256) hsolve_reinit_kernel(LaunchArgs a) {
         for (int i = threadIdx.x; i < v.n; i += blockDim.x) channel_row<GT, false>(v, types, i, (GT)a.dt, w, true, i);
 }
 
-// The HHChannel2D objects of one tick over a bin's tiles of kChannelTile
-// consecutive compartments of one cell (as the channel kernel), one
-// compartment per thread; the tick's step (or channel) kernel then adds
-// o2dExt to the rows.
-__global__ void __launch_bounds__(256) hsolve_chan2d_kernel(LaunchArgs a, const int2* __restrict__ tiles) {
+// The objects sending (Gk, Ek) every tick (Leakage, HHChannel2D, SynChan)
+// over a bin's tiles of kChannelTile consecutive compartments of one cell
+// (as the channel kernel), one compartment per thread; the tick's step (or
+// channel) kernel then adds o2dExt to the rows.
+__global__ void __launch_bounds__(256) hsolve_external_kernel(LaunchArgs a, const int2* __restrict__ tiles) {
     const int2 tile = tiles[blockIdx.x];
     const CellDev& c = a.cells[tile.x];
     const int i = tile.y + threadIdx.x;
-    if (c.n2d == 0 || i >= c.n) return;
+    if (c.nXObj == 0 || i >= c.n) return;
     const double* V = A_<const double>(a.arena, c.oV);
     const double* poolCa = A_<const double>(a.arena, c.oPoolCa);
-    chan2d_external(a.arena, a.tabs2d, a.dt, a.firstOfCall != 0, c, i, V[i], poolCa);
+    // this tick ends at step (steps done + 1); the tick's solver kernel
+    // advances the counter after it
+    const int64_t stepNo = *A_<const int64_t>(a.arena, c.oSteps) + 1;
+    external_objects(a.arena, a.tabs2d, a.dt, a.firstOfCall != 0, c, i, V[i], poolCa, stepNo);
 }
 
 // HHChannel2D::vReinit of the cells in `order`, launched before the
@@ -1733,8 +1777,8 @@ extern "C" hipError_t hsg_launch_channel_fast(const hsg::LaunchArgs* a, const in
     return hipGetLastError();
 }
 
-extern "C" hipError_t hsg_launch_chan2d(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, hipStream_t stream) {
-    hipLaunchKernelGGL(hsg::hsolve_chan2d_kernel, dim3(ntiles), dim3(hsg::kChannelTile), 0, stream, *a, tiles);
+extern "C" hipError_t hsg_launch_external(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, hipStream_t stream) {
+    hipLaunchKernelGGL(hsg::hsolve_external_kernel, dim3(ntiles), dim3(hsg::kChannelTile), 0, stream, *a, tiles);
     return hipGetLastError();
 }
 

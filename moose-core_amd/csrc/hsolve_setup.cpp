@@ -906,6 +906,67 @@ int setup_cell(const hsg_cell_desc* d, double dt, int caAdvance, CellSetup& s, s
         for (auto& p : s.pulses) p.entry = entryOf[p.compt];
     }
 
+    // Leakage objects (their per-tick sends) and SynChans, grouped by
+    // compartment in Hines order, desc order within one
+    {
+        std::vector<std::vector<int>> byC(n);
+        for (int k = 0; k < d->n_leaks; ++k) {
+            const int h = s.descToHines[d->leaks[k].compartment];
+            if (h >= 0) byC[h].push_back(k);   // a leak off the tree is not the solver's
+        }
+        s.leaks.clear();
+        s.leakStart.assign(n + 1, 0);
+        s.leakOfDesc.assign(d->n_leaks > 0 ? d->n_leaks : 0, -1);
+        for (int i = 0; i < n; ++i) {
+            s.leakStart[i] = (int32_t)s.leaks.size();
+            for (int k : byC[i]) {
+                const hsg_leak_desc& ld = d->leaks[k];
+                s.leakOfDesc[k] = (int)s.leaks.size();
+                s.leaks.push_back(HostLeak{i, k, ld.Gbar, ld.Ek, ld.modulation});
+            }
+        }
+        s.leakStart[n] = (int32_t)s.leaks.size();
+    }
+    {
+        const int ns = d->n_synchans > 0 ? d->n_synchans : 0;
+        if (ns > 0 && !d->synchans) { err = "synchans: null array"; return HSG_E_ARG; }
+        std::vector<std::vector<int>> byC(n);
+        std::vector<int> first(ns, 0);
+        int nsyn = 0;
+        for (int k = 0; k < ns; ++k) {
+            const hsg_synchan_desc& sd = d->synchans[k];
+            if (sd.compartment < 0 || sd.compartment >= nd || s.descToHines[sd.compartment] < 0) {
+                err = "SynChan on a compartment outside the cell";
+                return HSG_E_MODEL;
+            }
+            if (sd.n_synapses < 0) { err = "SynChan: negative n_synapses"; return HSG_E_ARG; }
+            if (!(sd.tau1 > 0.0) || !(sd.tau2 >= 0.0)) { err = "SynChan: tau1 must be > 0, tau2 >= 0"; return HSG_E_ARG; }
+            first[k] = nsyn;
+            nsyn += sd.n_synapses;
+            byC[s.descToHines[sd.compartment]].push_back(k);
+        }
+        if (nsyn > 0 && !d->synapses) { err = "synapses: null array"; return HSG_E_ARG; }
+        s.synW.resize(nsyn);
+        s.synDelay.resize(nsyn);
+        for (int j = 0; j < nsyn; ++j) {
+            s.synW[j] = d->synapses[j].weight;
+            s.synDelay[j] = d->synapses[j].delay;
+        }
+        s.syn.clear();
+        s.synStart.assign(n + 1, 0);
+        s.synOfDesc.assign(ns, -1);
+        for (int i = 0; i < n; ++i) {
+            s.synStart[i] = (int32_t)s.syn.size();
+            for (int k : byC[i]) {
+                const hsg_synchan_desc& sd = d->synchans[k];
+                s.synOfDesc[k] = (int)s.syn.size();
+                s.syn.push_back(HostSyn{i, k, first[k], sd.n_synapses, sd.Gbar, sd.Ek, sd.tau1, sd.tau2,
+                                        sd.modulation});
+            }
+        }
+        s.synStart[n] = (int32_t)s.syn.size();
+    }
+
     // SpikeGens in Hines order of their compartments
     std::vector<std::vector<int>> spk(nd);
     for (int k = 0; k < d->n_spikegens; ++k) {

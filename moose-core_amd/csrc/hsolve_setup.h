@@ -54,6 +54,22 @@ struct HostPulse {
     double delay[HSG_MAX_PULSES], width[HSG_MAX_PULSES], level[HSG_MAX_PULSES];
 };
 
+// A Leakage object's per-tick send (biophysics/Leakage.cpp:90-95).
+struct HostLeak {
+    int compt;                        // Hines index
+    int desc;                         // index in hsg_cell_desc.leaks
+    double Gbar, Ek, mod;
+};
+
+// A SynChan with its handler (biophysics/SynChan.cpp,
+// synapse/SimpleSynHandler.cpp).
+struct HostSyn {
+    int compt;                        // Hines index
+    int desc;                         // index in hsg_cell_desc.synchans
+    int firstSyn, nSyn;               // its synapses (desc order)
+    double Gbar, Ek, tau1, tau2, mod;
+};
+
 struct HostChanType {
     double Xpower, Ypower, Zpower;
     int instant, nGates, colX, colY, colZ, caDepLocal, caTgtLocal;
@@ -128,6 +144,15 @@ struct CellSetup {
     // order within one), with the compartment -> channels index (n + 1)
     std::vector<HostChan2D> ch2d;
     std::vector<HostPulse> pulses;    // PulseGens on injectMsg, desc order
+    // Leakage sends and SynChans, sorted by compartment (desc order within
+    // one), with their compartment -> objects indexes (n + 1)
+    std::vector<HostLeak> leaks;
+    std::vector<int32_t> leakStart;
+    std::vector<int> leakOfDesc;
+    std::vector<HostSyn> syn;
+    std::vector<int32_t> synStart;
+    std::vector<int> synOfDesc;
+    std::vector<double> synW, synDelay;      // synapses, desc order
     std::vector<int32_t> ch2dStart;
     std::vector<int> ch2dOfDesc;             // desc index -> position in ch2d
     std::vector<HostTable2D> tables2d;

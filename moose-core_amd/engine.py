@@ -20,7 +20,7 @@ _LIB = None
 EXPORTS = [
     "hsg_create", "hsg_destroy", "hsg_add_cell", "hsg_remove_cell", "hsg_reinit",
     "hsg_reinit_cell", "hsg_step", "hsg_step_cells", "hsg_get_field", "hsg_set_field",
-    "hsg_add_inject", "hsg_add_gkek", "hsg_add_conc", "hsg_add_ica", "hsg_set_pulsegen", "hsg_read_spikes",
+    "hsg_add_inject", "hsg_add_gkek", "hsg_add_conc", "hsg_add_ica", "hsg_set_pulsegen", "hsg_add_spike", "hsg_connect_spike", "hsg_read_spikes",
     "hsg_set_probes", "hsg_set_probes_every", "hsg_read_probes", "hsg_cell_info_get", "hsg_hines_order",
     "hsg_sync", "hsg_last_step_timing", "hsg_last_error",
 ]
@@ -61,6 +61,8 @@ def load_library(path=None):
         "hsg_add_conc": ([vp, i32, i32, dbl], C.c_int),
         "hsg_add_ica": ([vp, i32, i32, dbl], C.c_int),
         "hsg_set_pulsegen": ([vp, i32, i32, vp], C.c_int),
+        "hsg_add_spike": ([vp, i32, i32, i32, dbl], C.c_int),
+        "hsg_connect_spike": ([vp, i32, i32, i32, i32, i32], C.c_int),
         "hsg_read_spikes": ([vp, P(Spike), i32, P(i32), P(i64)], C.c_int),
         "hsg_set_probes": ([vp, i32, P(i32), i32, i64], C.c_int),
         "hsg_set_probes_every": ([vp, i32, P(i32), i32, i64, i32], C.c_int),
@@ -193,6 +195,15 @@ class Engine:
 
     def add_ica(self, cell, pool, iCa):
         self._check(self.lib.hsg_add_ica(self.h, int(cell), int(pool), float(iCa)))
+
+    def add_spike(self, cell, synchan, synapse, time):
+        """Synapse::addSpike: a spike at `time` through the synapse."""
+        self._check(self.lib.hsg_add_spike(self.h, int(cell), int(synchan), int(synapse), float(time)))
+
+    def connect_spike(self, src_cell, spikegen, dst_cell, synchan, synapse):
+        """spikeOut of a SpikeGen -> addSpike of a synapse (any two cells)."""
+        self._check(self.lib.hsg_connect_spike(self.h, int(src_cell), int(spikegen), int(dst_cell),
+                                               int(synchan), int(synapse)))
 
     def set_pulsegen(self, cell, index, rec):
         """New parameters for the cell's index-th PulseGen (one PULSEGEN_DT

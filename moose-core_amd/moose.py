@@ -1,6 +1,7 @@
 """A pymoose-style object layer over the engine: build a model from
 Compartment, HHChannel, HHGate, HHChannel2D, HHGate2D, CaConc, SpikeGen,
-Leakage and Table objects
+Leakage, PulseGen, SynChan, SimpleSynHandler (with its synapses) and Table
+objects
 connected by messages, hand it to an HSolve through `target`, then
 `reinit()` / `start(t)` and read or write the (zombified) fields.
 
@@ -613,8 +614,83 @@ class PulseGen(Neutral):
 
 
 class Leakage(Neutral):
+    """A Leakage channel (biophysics/Leakage.cpp): not taken over by the
+    solver.  Its Gk field (Gbar * modulation once reinit has run) is folded
+    into the compartment's EmByRm when the solver reads the model, and on
+    every tick it sends (Gbar * modulation, Ek) to the compartment, which
+    under the solver becomes external current (run on the GPU)."""
+
     def _init_fields(self):
-        self.__dict__.update(Gk=0.0, Ek=0.0)
+        self.__dict__.update(Gk=0.0, Ek=0.0, Gbar=0.0, modulation=1.0)
+
+
+class Synapse:
+    """One synapse of a SimpleSynHandler (synapse/Synapse.cpp): weight,
+    delay; a spike arriving on addSpike at time t queues the event
+    (t + delay, weight)."""
+
+    def __init__(self, handler, index):
+        self.handler, self.index = handler, index
+        self.weight, self.delay = 1.0, 0.0
+        self.id = _next_id[0]
+        _next_id[0] += 1
+        self.path = f"{handler.path}/synapse[{index}]"
+
+    def isA(self, name):
+        return name == "Synapse"
+
+    def addSpike(self, time):
+        """Synapse::addSpike from the host (a spike at `time` seconds)."""
+        hs, k = self.handler._solver_of()
+        hs._engine.add_spike(hs._cell, k, self.index, float(time))
+
+
+class SimpleSynHandler(Neutral):
+    """A SimpleSynHandler (synapse/SimpleSynHandler.cpp): synapses as field
+    elements (numSynapses, synapse[i]); on each tick it sends the activation
+    sum(weight / dt) of the events due to the SynChan on its activationOut.
+    Under a solver its queue and the SynChan run beside it (queue on the
+    host, SynChan on the GPU)."""
+
+    def _init_fields(self):
+        self.__dict__.update(synapse=[])
+
+    @property
+    def numSynapses(self):
+        return len(self.synapse)
+
+    def _set_local(self, name, value):
+        if name == "numSynapses":
+            n = int(value)
+            syn = self.__dict__["synapse"]
+            while len(syn) < n:
+                syn.append(Synapse(self, len(syn)))
+            del syn[n:]
+            return
+        object.__setattr__(self, name, value)
+
+    def _solver_of(self):
+        for chan in _targets(self, "activationOut", "SynChan"):
+            z = chan.__dict__.get("_solver")
+            if z is not None:
+                return z
+        raise RuntimeError(f"{self.path}: its SynChan is not under an HSolve")
+
+
+class SynChan(Neutral):
+    """A SynChan (biophysics/SynChan.cpp): dual-exponential conductance
+    (tau1, tau2; normalised so the peak of a unit activation is Gbar),
+    driven by its SimpleSynHandler's activation.  Taken over with its cell,
+    it runs on the GPU beside the solver and its fields read and write
+    there."""
+    _zfields = ("Gbar", "Ek", "Gk", "tau1", "tau2", "modulation", "X", "Y")
+    _zmap = {"Gbar": "GbarSyn", "Ek": "EkSyn", "Gk": "GkSyn", "modulation": "modulationSyn",
+             "X": "XSyn", "Y": "YSyn"}
+
+    def _init_fields(self):
+        # biophysics/SynChan.cpp:95-108, ChanCommon.cpp:13-20
+        self.__dict__.update(Gbar=0.0, Ek=0.0, Gk=0.0, tau1=1.0e-3, tau2=1.0e-3, modulation=1.0,
+                             X=0.0, Y=0.0, normalizeWeights=0)
 
 
 class Table(Neutral):
@@ -635,7 +711,7 @@ class Table(Neutral):
 
 
 def element(path):
-    if isinstance(path, Neutral):
+    if isinstance(path, (Neutral, Synapse)):
         return path
     p = _norm(path)
     if p not in _objects:
@@ -807,7 +883,8 @@ class HSolve(Neutral):
             for ch in _channels(c, "HHChannel"):
                 chans.append((cidx[c.id], ch))
             for lk in _channels(c, "Leakage"):
-                leaks.append((cidx[c.id], lk.__dict__["Gk"], lk.__dict__["Ek"]))
+                d = lk.__dict__
+                leaks.append((cidx[c.id], d["Gk"], d["Ek"], d["Gbar"], d["modulation"]))
             for sg in _targets(c, "VmOut", "SpikeGen"):
                 spikes.append((cidx[c.id], sg))
         crec = np.zeros(len(chans), _abi.CHANNEL_DT)
@@ -879,6 +956,21 @@ class HSolve(Neutral):
         lrec = np.zeros(len(leaks), _abi.LEAK_DT)
         for k, rec in enumerate(leaks):
             lrec[k] = rec
+        # SynChans on the compartments, in message order, each with the
+        # synapses of the handler feeding it
+        syns, synrec, synapses = [], [], []
+        for c in comps:
+            for sc in _channels(c, "SynChan"):
+                hs_ = _sources(sc, "activation", "SimpleSynHandler")
+                if len(hs_) > 1:
+                    raise ValueError(f"{sc.path}: more than one SynHandler feeds it")
+                syn = hs_[0].synapse if hs_ else []
+                d = sc.__dict__
+                synrec.append((cidx[c.id], len(syn), d["Gbar"], d["Ek"], d["tau1"], d["tau2"], d["modulation"]))
+                synapses += [(y.weight, y.delay) for y in syn]
+                syns.append(sc)
+        screc = np.array(synrec, _abi.SYNCHAN_DT) if synrec else np.zeros(0, _abi.SYNCHAN_DT)
+        sprec = np.array(synapses, _abi.SYNAPSE_DT) if synapses else np.zeros(0, _abi.SYNAPSE_DT)
         # PulseGens feeding the compartments' injectMsg, in creation order
         # (the order clock tick 0 processes them)
         pgs = sorted(((s, cidx[d.id]) for s, sf, d, df in _msgs
@@ -889,10 +981,11 @@ class HSolve(Neutral):
             grec[k] = pg._record(ci)
         spec = CellSpec(ct, np.array(axial, dtype=np.int32).reshape(-1, 2), seed=cidx[seed.id],
                         channels=crec, gates=glist, pools=prec, spikegens=srec, leaks=lrec,
-                        tables2d=tabs, chans2d=c2rec, pulsegens=grec)
+                        tables2d=tabs, chans2d=c2rec, pulsegens=grec, synchans=screc, synapses=sprec)
         objs = {"compartments": comps, "channels": [ch for _, ch in chans], "pools": pools,
                 "spikegens": [sg for _, sg in spikes], "chans2d": [ch for _, ch in ch2],
-                "pulsegens": pgs}
+                "pulsegens": pgs, "synchans": syns,
+                "leaks": [lk for c in comps for lk in _channels(c, "Leakage")]}
         return spec, objs
 
     def _setup(self, seed):
@@ -908,7 +1001,7 @@ class HSolve(Neutral):
         object.__setattr__(self, "_objs", objs)
         object.__setattr__(self, "_spec", spec)
         # zombify (hsolve/HSolve.cpp:235-262): fields now live in the solver
-        for kind in ("compartments", "channels", "pools", "chans2d"):
+        for kind in ("compartments", "channels", "pools", "chans2d", "synchans"):
             for i, o in enumerate(objs[kind]):
                 o.__dict__["_solver"] = (self, i)
                 for f in type(o)._zfields:       # reads now reach the solver
@@ -923,6 +1016,31 @@ class HSolve(Neutral):
     def _set(self, obj, field, idx, value):
         field = getattr(type(obj), "_zmap", {}).get(field, field)
         self._engine.set(self._cell, field, [idx], [float(value)])
+
+    def _push_leaks(self):
+        """Leakage fields written since setup reach the device's per-tick
+        send (the object itself keeps running beside the solver)."""
+        sent = self._spec.leaks
+        for k, lk in enumerate(self._objs["leaks"]):
+            d = lk.__dict__
+            for f, name in (("Gbar", "GbarLeak"), ("Ek", "EkLeak"), ("modulation", "modulationLeak")):
+                if d[f] != sent[k][f]:
+                    self._engine.set(self._cell, name, [k], [d[f]])
+                    sent[k][f] = d[f]
+
+    def _push_synapses(self):
+        """Synapse weights and delays written since setup."""
+        sent = self._spec.synapses
+        j = 0
+        for sc in self._objs["synchans"]:
+            hs_ = _sources(sc, "activation", "SimpleSynHandler")
+            for y in (hs_[0].synapse if hs_ else []):
+                for f, name in (("weight", "weight"), ("delay", "delay")):
+                    v = getattr(y, f)
+                    if v != sent[j][f]:
+                        self._engine.set(self._cell, name, [j], [v])
+                        sent[j][f] = v
+                j += 1
 
     def _push_pulsegens(self):
         """PulseGen fields written since setup reach the device before the
@@ -1324,11 +1442,46 @@ def _table_every(table, dt):
     return max(1, int(round(tdt / dt))) if tdt > 0 else 1
 
 
+def _wire_spikes():
+    """SpikeGen spikeOut -> synapse addSpike messages between the solvers'
+    cells become engine connections (routed on the host between stretches
+    no longer than the shortest delay)."""
+    where = {}
+    for hs in _solvers:
+        for g, sg in enumerate(hs._objs["spikegens"]):
+            where.setdefault(sg.id, []).append((hs, g))
+    for s, sf, d, df in _msgs:
+        if sf != "spikeOut" or df != "addSpike" or not isinstance(d, Synapse):
+            continue
+        try:
+            hs_dst, k = d.handler._solver_of()
+        except RuntimeError:
+            continue
+        srcs = where.get(s.id)
+        if not srcs:
+            raise ValueError(f"{s.path} -> {d.path}: the SpikeGen is not under an HSolve")
+        for hs_src, g in srcs:
+            if hs_src._engine is not hs_dst._engine:
+                raise ValueError(f"{s.path} -> {d.path}: solvers with different dt or precision")
+            key = (s.id, d.id, hs_src._cell, hs_dst._cell)
+            wired = hs_src._engine.__dict__.setdefault("_wired", set())
+            if key not in wired:
+                hs_src._engine.connect_spike(hs_src._cell, g, hs_dst._cell, k, d.index)
+                wired.add(key)
+
+
 def reinit():
     """HSolve::reinit on every solver (hsolve/HSolveActiveSetup.cpp:43-52);
-    Tables record their t = 0 sample."""
+    Tables record their t = 0 sample; Leakage objects set Gk = Gbar *
+    modulation (Leakage::vReinit)."""
+    for o in list(_objects.values()):
+        if isinstance(o, Leakage):
+            o.__dict__["Gk"] = o.__dict__["Gbar"] * o.__dict__["modulation"]
+    _wire_spikes()
     done = set()
     for hs in _solvers:
+        hs._push_leaks()
+        hs._push_synapses()
         hs._push_pulsegens()
     for hs in _solvers:
         eng = hs._engine
@@ -1353,6 +1506,7 @@ def start(runtime):
     byeng = {}
     for hs in _solvers:
         byeng.setdefault(id(hs._engine), []).append(hs)
+    _wire_spikes()
     for group in byeng.values():
         eng = group[0]._engine
         dt = float(group[0].dt)
@@ -1371,6 +1525,8 @@ def start(runtime):
                 eng.set_probes(hs._cell, comps, nsteps // every + 1, every=every)
             rec.append((hs, vm, spk, comps, every, hs.__dict__.get("_steps", 0)))
             hs._push_pulsegens()
+            hs._push_leaks()
+            hs._push_synapses()
         eng.step(nsteps)
         spikes = eng.read_spikes()
         for hs, vm, spk, comps, every, s0 in rec:

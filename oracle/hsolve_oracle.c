@@ -155,6 +155,62 @@ typedef struct {
     double Gbar, Ek, pw[3], st[3], mod, Gk;
 } or_chan2d;
 
+/* doubleEq (basecode/doubleEq.cpp:16-22) */
+static int double_eq(double x, double y) {
+    double denom = fabs(x) + fabs(y);
+    if (denom < 1e-12) denom = 1e-6;
+    return (fabs(x - y) / denom) < 1e-6;
+}
+
+/* A Leakage object's per-tick send (biophysics/Leakage.cpp:90-95). */
+typedef struct {
+    int    ic;                        /* Hines index */
+    double Gbar, Ek, mod;
+} or_leak;
+
+/* A pending synaptic event (synapse/SynEvent.h). */
+typedef struct {
+    double time, weight;
+} or_event;
+
+/* A SynChan (biophysics/SynChan.cpp) and its SimpleSynHandler
+ * (synapse/SimpleSynHandler.cpp): the handler's queue holds events sorted
+ * by time, equal times in arrival order. */
+typedef struct {
+    int    ic;                        /* Hines index */
+    int    firstSyn, nSyn;
+    double Gbar, Ek, tau1, tau2, mod;
+    double xconst1, xconst2, yconst1, yconst2, norm, dt;
+    double X, Y, Gk, activation;
+    or_event* ev;
+    int    nEv, capEv;
+} or_synchan;
+
+/* SynChan's integration constants and norm at step dt: setTau1/setTau2
+ * and normalizeGbar (biophysics/SynChan.cpp:123-198), as reinit computes
+ * them (:229-251). */
+static void synchan_constants(or_synchan* c, double dt) {
+    c->dt = dt;
+    c->xconst1 = c->tau1 * (1.0 - exp(-dt / c->tau1));
+    c->xconst2 = exp(-dt / c->tau1);
+    if (double_eq(c->tau2, 0.0)) {
+        c->yconst1 = 1.0;
+        c->yconst2 = 0.0;
+    } else {
+        c->yconst1 = c->tau2 * (1.0 - exp(-dt / c->tau2));
+        c->yconst2 = exp(-dt / c->tau2);
+    }
+    if (double_eq(c->tau2, 0.0)) {
+        c->norm = c->Gbar;
+    } else if (double_eq(c->tau1, c->tau2)) {
+        c->norm = c->Gbar * exp(1.0) / c->tau1;
+    } else {
+        double tpeak = c->tau1 * c->tau2 * log(c->tau1 / c->tau2) / (c->tau1 - c->tau2);
+        c->norm = c->Gbar * (c->tau1 - c->tau2) /
+                  (c->tau1 * c->tau2 * (exp(-tpeak / c->tau1) - exp(-tpeak / c->tau2)));
+    }
+}
+
 struct or_solver {
     or_hines h;
     double   hsolveDt;           /* HSolve::dt_ (shadows HinesMatrix::dt_) */
@@ -201,6 +257,12 @@ struct or_solver {
     /* PulseGens feeding injectMsg, desc order */
     int        nPulse;
     or_pulse*  pulse;
+    /* Leakage objects and SynChans sending to the compartments, desc order */
+    int         nLeak;
+    or_leak*    leak;
+    int         nSyn, nSynapse;
+    or_synchan* syn;
+    double     *synW, *synDelay;      /* synapses, desc order */
 };
 
 /* ------------------------------------------------------------------------ */
@@ -1113,6 +1175,49 @@ int or_create(const hsg_cell_desc* d, double dt, int caAdvance, or_solver** out)
         }
         s->h.hasInject[p->ic] = 1;
     }
+    /* Leakage objects keep sending on their own tick */
+    s->nLeak = d->n_leaks > 0 ? d->n_leaks : 0;
+    s->leak = (or_leak*)calloc(s->nLeak + 1, sizeof(or_leak));
+    for (int k = 0; k < s->nLeak; ++k) {
+        const hsg_leak_desc* ld = &d->leaks[k];
+        if (ld->compartment < 0 || ld->compartment >= d->n_compartments) {
+            or_destroy(s);
+            return HSG_E_ARG;
+        }
+        s->leak[k].ic = s->descToHines[ld->compartment];   /* -1: off the tree, not the solver's */
+        s->leak[k].Gbar = ld->Gbar;
+        s->leak[k].Ek = ld->Ek;
+        s->leak[k].mod = ld->modulation;
+    }
+    /* SynChans with their handlers' synapses */
+    s->nSyn = d->n_synchans > 0 ? d->n_synchans : 0;
+    s->syn = (or_synchan*)calloc(s->nSyn + 1, sizeof(or_synchan));
+    s->nSynapse = 0;
+    for (int k = 0; k < s->nSyn; ++k) {
+        const hsg_synchan_desc* sd = &d->synchans[k];
+        if (sd->compartment < 0 || sd->compartment >= d->n_compartments || s->descToHines[sd->compartment] < 0 ||
+            sd->n_synapses < 0) {
+            or_destroy(s);
+            return HSG_E_ARG;
+        }
+        or_synchan* c = &s->syn[k];
+        c->ic = s->descToHines[sd->compartment];
+        c->firstSyn = s->nSynapse;
+        c->nSyn = sd->n_synapses;
+        s->nSynapse += sd->n_synapses;
+        c->Gbar = sd->Gbar;
+        c->Ek = sd->Ek;
+        c->tau1 = sd->tau1;
+        c->tau2 = sd->tau2;
+        c->mod = sd->modulation;
+        synchan_constants(c, 25.0e-6);   /* SynChan's dt_ before any reinit */
+    }
+    s->synW = (double*)calloc(s->nSynapse + 1, sizeof(double));
+    s->synDelay = (double*)calloc(s->nSynapse + 1, sizeof(double));
+    for (int j = 0; j < s->nSynapse; ++j) {
+        s->synW[j] = d->synapses[j].weight;
+        s->synDelay[j] = d->synapses[j].delay;
+    }
     *out = s;
     return 0;
 }
@@ -1161,7 +1266,9 @@ void or_destroy(or_solver* s) {
     iv_free(&s->spikeGenLog); free(s->spikeStepLog);
     if (s->tab2d)
         for (int k = 0; k < s->nTab2d; ++k) free((void*)s->tab2d[k].t);
-    free(s->tab2d); free(s->ch2d); free(s->pulse);
+    free(s->tab2d); free(s->ch2d); free(s->pulse); free(s->leak);
+    for (int k = 0; k < s->nSyn; ++k) free(s->syn[k].ev);
+    free(s->syn); free(s->synW); free(s->synDelay);
     free(s);
 }
 
@@ -1218,6 +1325,15 @@ void or_reinit(or_solver* s, double dt) {
         h->injectVarying[s->pulse[k].ic] += s->pulse[k].base;
     }
     chan2d_reinit(s);            /* their reinit runs before the solver's */
+    /* SimpleSynHandler::vReinit drops the queue; SynChan::vReinit
+     * (SynChan.cpp:229-251) zeroes X, Y, Gk, activation at the new dt */
+    for (int k = 0; k < s->nSyn; ++k) {
+        or_synchan* c = &s->syn[k];
+        c->nEv = 0;
+        c->activation = 0.0;
+        c->X = c->Y = c->Gk = 0.0;
+        synchan_constants(c, dt);
+    }
     s->hsolveDt = dt;
     for (unsigned i = 0; i < 2 * h->nCompt; ++i) s->externalCurrent[i] = 0.0;
     for (int k = 0; k < s->nSpike; ++k) s->spike[k].lastEvent = -s->spike[k].refractT;
@@ -1505,7 +1621,40 @@ void or_step(or_solver* s, double dt, double currTime) {
      * solver (tick 6) */
     for (int k = 0; k < s->nPulse; ++k)
         if (pulse_process(&s->pulse[k], currTime)) h->injectVarying[s->pulse[k].ic] += s->pulse[k].out;
+    /* SimpleSynHandler::vProcess (tick 1, SimpleSynHandler.cpp:104-112) */
+    for (int k = 0; k < s->nSyn; ++k) {
+        or_synchan* c = &s->syn[k];
+        double activation = 0.0;
+        int q = 0;
+        while (q < c->nEv && c->ev[q].time <= currTime) {
+            activation += c->ev[q].weight / dt;
+            ++q;
+        }
+        if (q > 0) {
+            memmove(c->ev, c->ev + q, sizeof(or_event) * (size_t)(c->nEv - q));
+            c->nEv -= q;
+        }
+        if (activation != 0.0) c->activation += activation;
+    }
+    /* tick 4: Leakage (Leakage.cpp:90-95), the HHChannel2D objects, the
+     * SynChans (SynChan.cpp:200-227), each sending (Gk, Ek) to addGkEk */
+    for (int k = 0; k < s->nLeak; ++k) {
+        const or_leak* l = &s->leak[k];
+        if (l->ic < 0) continue;
+        const double Gk = l->Gbar * l->mod;
+        s->externalCurrent[2 * l->ic] += Gk;
+        s->externalCurrent[2 * l->ic + 1] += Gk * l->Ek;
+    }
     chan2d_process(s, dt);
+    for (int k = 0; k < s->nSyn; ++k) {
+        or_synchan* c = &s->syn[k];
+        c->X = c->activation * c->xconst1 + c->X * c->xconst2;
+        c->Y = c->X * c->yconst1 + c->Y * c->yconst2;
+        c->activation = 0.0;
+        c->Gk = c->Y * c->norm * c->mod;
+        s->externalCurrent[2 * c->ic] += c->Gk;
+        s->externalCurrent[2 * c->ic + 1] += c->Gk * c->Ek;
+    }
     advance_channels(s, dt);
     calculate_channel_currents(s);
     update_matrix_active(s);
@@ -1591,6 +1740,36 @@ int or_get_field(const or_solver* s, int field, const int* idx, int n, double* o
             case HSG_F_INSTANT: out[k] = (double)s->instant[c]; break;
             default: return HSG_E_ARG;
             }
+        } else if (field >= 95) {
+            if (i < 0 || i >= s->nLeak) return HSG_E_ARG;
+            const or_leak* l = &s->leak[i];
+            switch (field) {
+            case HSG_F_LEAK_GBAR: out[k] = l->Gbar; break;
+            case HSG_F_LEAK_EK: out[k] = l->Ek; break;
+            case HSG_F_LEAK_MODULATION: out[k] = l->mod; break;
+            default: return HSG_E_ARG;
+            }
+        } else if (field >= 90) {
+            if (i < 0 || i >= s->nSynapse) return HSG_E_ARG;
+            switch (field) {
+            case HSG_F_SYNAPSE_WEIGHT: out[k] = s->synW[i]; break;
+            case HSG_F_SYNAPSE_DELAY: out[k] = s->synDelay[i]; break;
+            default: return HSG_E_ARG;
+            }
+        } else if (field >= 80) {
+            if (i < 0 || i >= s->nSyn) return HSG_E_ARG;
+            const or_synchan* c = &s->syn[i];
+            switch (field) {
+            case HSG_F_SYN_GBAR: out[k] = c->Gbar; break;
+            case HSG_F_SYN_EK: out[k] = c->Ek; break;
+            case HSG_F_SYN_GK: out[k] = c->Gk; break;
+            case HSG_F_SYN_TAU1: out[k] = c->tau1; break;
+            case HSG_F_SYN_TAU2: out[k] = c->tau2; break;
+            case HSG_F_SYN_MODULATION: out[k] = c->mod; break;
+            case HSG_F_SYN_X: out[k] = c->X; break;
+            case HSG_F_SYN_Y: out[k] = c->Y; break;
+            default: return HSG_E_ARG;
+            }
         } else if (field >= 60) {
             if (i < 0 || i >= s->n2d) return HSG_E_ARG;
             const or_chan2d* c = &s->ch2d[i];
@@ -1651,6 +1830,35 @@ int or_set_field(or_solver* s, int field, const int* idx, int n, const double* v
             case HSG_F_INSTANT: s->instant[c] = (int)x; break;
             default: return HSG_E_ARG;
             }
+        } else if (field >= 95) {
+            if (i < 0 || i >= s->nLeak) return HSG_E_ARG;
+            or_leak* l = &s->leak[i];
+            switch (field) {
+            case HSG_F_LEAK_GBAR: l->Gbar = x; break;
+            case HSG_F_LEAK_EK: l->Ek = x; break;
+            case HSG_F_LEAK_MODULATION: l->mod = x; break;
+            default: return HSG_E_ARG;
+            }
+        } else if (field >= 90) {
+            if (i < 0 || i >= s->nSynapse) return HSG_E_ARG;
+            switch (field) {
+            case HSG_F_SYNAPSE_WEIGHT: s->synW[i] = x; break;
+            case HSG_F_SYNAPSE_DELAY: s->synDelay[i] = x; break;
+            default: return HSG_E_ARG;
+            }
+        } else if (field >= 80) {
+            if (i < 0 || i >= s->nSyn) return HSG_E_ARG;
+            or_synchan* c = &s->syn[i];
+            switch (field) {
+            /* the setters recompute the constants at the object's dt_
+             * (SynChan.cpp:117-156) */
+            case HSG_F_SYN_GBAR: c->Gbar = x; synchan_constants(c, c->dt); break;
+            case HSG_F_SYN_TAU1: c->tau1 = x; synchan_constants(c, c->dt); break;
+            case HSG_F_SYN_TAU2: c->tau2 = x; synchan_constants(c, c->dt); break;
+            case HSG_F_SYN_EK: c->Ek = x; break;
+            case HSG_F_SYN_MODULATION: c->mod = x; break;
+            default: return HSG_E_ARG;
+            }
         } else if (field >= 60) {
             if (i < 0 || i >= s->n2d) return HSG_E_ARG;
             or_chan2d* c = &s->ch2d[i];
@@ -1688,6 +1896,28 @@ int or_add_inject(or_solver* s, int compartment, double value) {
     s->h.injectVarying[ic] += value;
     return 0;
 }
+/* Synapse::addSpike (synapse/Synapse.cpp:104-113) into the handler's
+ * priority queue (SimpleSynHandler::addSpike, :93-97) */
+int or_add_spike(or_solver* s, int synchan, int synapse, double time) {
+    if (synchan < 0 || synchan >= s->nSyn) return HSG_E_ARG;
+    or_synchan* c = &s->syn[synchan];
+    if (synapse < 0 || synapse >= c->nSyn) return HSG_E_ARG;
+    const int j = c->firstSyn + synapse;
+    or_event e = {time + s->synDelay[j], s->synW[j]};
+    if (c->nEv == c->capEv) {
+        c->capEv = c->capEv ? 2 * c->capEv : 16;
+        c->ev = (or_event*)realloc(c->ev, sizeof(or_event) * (size_t)c->capEv);
+    }
+    int q = c->nEv;
+    while (q > 0 && c->ev[q - 1].time > e.time) {
+        c->ev[q] = c->ev[q - 1];
+        --q;
+    }
+    c->ev[q] = e;
+    c->nEv++;
+    return 0;
+}
+
 int or_add_gkek(or_solver* s, int compartment, double Gk, double Ek) {
     if (compartment < 0 || s->descToHines[compartment] < 0) return HSG_E_ARG;
     int ic = s->descToHines[compartment];

@@ -59,6 +59,7 @@ int  or_get_field(const or_solver* s, int field, const int* idx, int n, double* 
 int  or_set_field(or_solver* s, int field, const int* idx, int n, const double* v);
 int  or_add_inject(or_solver* s, int compartment, double value);
 int  or_add_gkek(or_solver* s, int compartment, double Gk, double Ek);
+int  or_add_spike(or_solver* s, int synchan, int synapse, double time);
 int  or_add_conc(or_solver* s, int channel, double conc);
 int  or_add_ica(or_solver* s, int pool, double iCa);
 /* Spikes recorded since the last call: (spikegen index, step number). */

@@ -46,6 +46,7 @@ def lib():
             "or_set_field": ([vp, i32, P(i32), i32, P(dbl)], i32),
             "or_add_inject": ([vp, i32, dbl], i32),
             "or_add_gkek": ([vp, i32, dbl, dbl], i32),
+            "or_add_spike": ([vp, i32, i32, dbl], i32),
             "or_add_conc": ([vp, i32, dbl], i32),
             "or_add_ica": ([vp, i32, dbl], i32),
             "or_read_spikes": ([vp, P(i32), P(C.c_long), i32], i32),
@@ -86,7 +87,10 @@ def _ip(a):
 FIELD = {"Vm": 1, "Im": 2, "inject": 3, "initVm": 4, "Cm": 5, "Em": 6, "Rm": 7, "Ra": 8,
          "Gbar": 20, "Ek": 21, "Gk": 22, "Ik": 23, "X": 24, "Y": 25, "Z": 26,
          "modulation": 27, "instant": 28, "Ca": 40, "CaBasal": 41, "ceiling": 42, "floor": 43,
-         "Gbar2D": 60, "Ek2D": 61, "Gk2D": 62, "X2D": 64, "Y2D": 65, "Z2D": 66, "modulation2D": 67}
+         "Gbar2D": 60, "Ek2D": 61, "Gk2D": 62, "X2D": 64, "Y2D": 65, "Z2D": 66, "modulation2D": 67,
+         "GbarSyn": 80, "EkSyn": 81, "GkSyn": 82, "tau1": 83, "tau2": 84, "modulationSyn": 85,
+         "XSyn": 86, "YSyn": 87, "weight": 90, "delay": 91,
+         "GbarLeak": 95, "EkLeak": 96, "modulationLeak": 97}
 
 
 class OracleCell:
@@ -147,6 +151,10 @@ class OracleCell:
 
     def add_inject(self, c, v):
         self.L.or_add_inject(self.h, int(c), float(v))
+
+    def add_spike(self, synchan, synapse, time):
+        if self.L.or_add_spike(self.h, int(synchan), int(synapse), float(time)):
+            raise IndexError("oracle add_spike: no such synchan or synapse")
 
     def add_gkek(self, c, g, e):
         self.L.or_add_gkek(self.h, int(c), float(g), float(e))
