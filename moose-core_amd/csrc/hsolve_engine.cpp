@@ -252,7 +252,7 @@ struct hsg_engine {
         int2* dTiles = nullptr;     // channel-kernel tiles (split mode)
         int nTiles = 0;
         bool msg = false;           // message-passing solve kernel (split mode)
-        bool hasExt = false;        // cells with objects sending every tick: launched tick by tick
+        bool hasExt = false;        // cells with HHChannel2D objects: the external kernel before every tick
         bool split = false;         // channel kernel + solve kernel per tick
         bool slow = false;          // cells with junction groups of four or more members
         int chanMS = 0;             // channel kernel's fast pass for up to chanMS slots (0: generic)
@@ -923,7 +923,7 @@ int make_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg_
     auto cls = [&](int id) {
         const CellDev& d = e->cells[id]->dev;
         const int s = (d.slowHJ ? 8 : 0) + (autoMode && !d.msg && d.n >= kAutoSplitMin ? 16 : 0);
-        if (d.nXObj > 0) return s + (d.msg ? 6 : 7);
+        if (d.n2d > 0) return s + (d.msg ? 6 : 7);
         if (d.msg) return s + (d.n >= 1024 ? 0 : d.n >= 256 ? 1 : 2);
         return s + 3 + (d.ldsBytes > 32768 ? 0 : d.ldsBytes > 16384 ? 1 : 2);
     };

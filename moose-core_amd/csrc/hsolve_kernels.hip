@@ -325,10 +325,10 @@ __device__ __forceinline__ double dep2d(int dep, bool sent, double Vm, const Cha
 // is kept in o2dExt for the row and for Im.  Run by its own kernel before
 // the tick's solver kernels, so the step kernels carry none of its
 // registers.
-__device__ __forceinline__ void external_objects(char* arena, const Table2DDev* __restrict__ tabs2d, double dt,
-                                                 bool first, const CellDev& c, int i, double Vm,
-                                                 const double* __restrict__ poolCa, int64_t stepNo) {
-    if (!A_<const int32_t>(arena, c.oXFlag)[i]) return;
+template <bool WITH2D>
+__device__ __forceinline__ double2 external_sum(char* arena, const Table2DDev* __restrict__ tabs2d, double dt,
+                                                bool first, const CellDev& c, int i, double Vm,
+                                                const double* __restrict__ poolCa, int64_t stepNo) {
     double G = 0.0, GE = 0.0;
     if (first) {
         const int32_t* xi = A_<const int32_t>(arena, c.oExtIdx);
@@ -347,7 +347,7 @@ __device__ __forceinline__ void external_objects(char* arena, const Table2DDev* 
             GE += lk[k].GE;
         }
     }
-    if (c.n2d > 0) {
+    if (WITH2D && c.n2d > 0) {
         const int32_t* start = A_<const int32_t>(arena, c.o2dStart);
         const Chan2DDev* chans = A_<const Chan2DDev>(arena, c.o2dChan);
         double* st = A_<double>(arena, c.o2dState);
@@ -406,6 +406,35 @@ __device__ __forceinline__ void external_objects(char* arena, const Table2DDev* 
     double* ext = A_<double>(arena, c.o2dExt);
     ext[2 * i] = G;
     ext[2 * i + 1] = GE;
+    return make_double2(G, GE);
+}
+
+// The external kernel's form (every kind of object).
+__device__ __forceinline__ void external_objects(char* arena, const Table2DDev* __restrict__ tabs2d, double dt,
+                                                 bool first, const CellDev& c, int i, double Vm,
+                                                 const double* __restrict__ poolCa, int64_t stepNo) {
+    if (!A_<const int32_t>(arena, c.oXFlag)[i]) return;
+    external_sum<true>(arena, tabs2d, dt, first, c, i, Vm, poolCa, stepNo);
+}
+
+// The step kernels' form, for cells without HHChannel2D objects (Leakage
+// and SynChan only): called where the tick's row takes its external
+// current, so these cells keep multi-tick launches.  Not inlined, so the
+// step kernels' register budget does not grow.
+__device__ __noinline__ double2 external_light(char* arena, bool first, const CellDev& c, int i, int64_t stepNo) {
+    return external_sum<false>(arena, nullptr, 0.0, first, c, i, 0.0, nullptr, stepNo);
+}
+
+// The external current of compartment i (which has objects sending every
+// tick) for the tick ending at step stepNo: computed here, or read from
+// o2dExt where the external kernel ran before the tick (HHChannel2D cells).
+__device__ __forceinline__ double2 external_current(char* arena, bool first, const CellDev& c, int i,
+                                                    int64_t stepNo) {
+    if (c.n2d > 0) {
+        const double* ext = A_<const double>(arena, c.o2dExt);
+        return make_double2(ext[2 * i], ext[2 * i + 1]);
+    }
+    return external_light(arena, first, c, i, stepNo);
 }
 
 // HHChannel2D::vReinit (HHChannel2D.cpp:374-426), run before the solver's
@@ -1164,11 +1193,11 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
                 // compartments with objects sending every tick: the whole
                 // external current, after the injection
                 work_barrier(WL);
-                const double* ext = A_<const double>(arena, c.o2dExt);
                 for (int i = tid; i < n; i += nt)
                     if (xflag[i]) {
-                        w.D[i] += ext[2 * i];
-                        w.B[i] += ext[2 * i + 1];
+                        const double2 g = external_current(arena, first, c, i, step0 + k + 1);
+                        w.D[i] += g.x;
+                        w.B[i] += g.y;
                     }
             }
         }
@@ -1323,9 +1352,10 @@ __device__ __forceinline__ void row_inputs(const LaunchArgs& a, const CellDev& c
             B += var + A_<const double>(a.arena, c.oInjBasal)[e];
         }
     if (has2d) {
-        const double* ext = A_<const double>(a.arena, c.o2dExt);
-        D += ext[2 * i];
-        B += ext[2 * i + 1];
+        const double2 g = external_current(a.arena, a.firstOfCall != 0, c, i,
+                                           *A_<const int64_t>(a.arena, c.oSteps) + 1);
+        D += g.x;
+        B += g.y;
     } else if (a.firstOfCall) {
         const int32_t* xi = A_<const int32_t>(a.arena, c.oExtIdx);
         for (int e = 0; e < c.nExt; ++e)

@@ -283,6 +283,33 @@ def test_gpu_fp64_synapses_host_spikes(pkg, oracle_mod, mode):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("mode", ["fused", "split"])
+def test_gpu_fp64_synapses_beside_chan2d(pkg, oracle_mod, mode):
+    """Cells whose compartments carry HHChannel2D objects as well as
+    SynChans and Leakage objects (the external kernel sums all three kinds),
+    next to cells with only SynChans and leaks (summed in the step kernels)."""
+    print(f"case: K-AHP 2-D cells + SynChans + leaks, host spikes, 800 steps, fp64, {mode}", flush=True)
+    rng = np.random.default_rng(43)
+    specs = []
+    for n in (40, 150):
+        s = workloads.random_ca_cell(rng, n, inject=0.3e-9, kahp2d=True)
+        extra = syn_hh_cell(rng, n)
+        s.synchans, s.synapses, s.leaks = extra.synchans, extra.synapses, extra.leaks
+        # one SynChan and one leak on a compartment that has a 2-D channel
+        s.synchans["compartment"][0] = s.chans2d["compartment"][0]
+        s.leaks["compartment"][0] = s.chans2d["compartment"][0]
+        specs.append(s)
+    specs.append(syn_hh_cell(rng, 90))
+    host = []
+    for c, s in enumerate(specs):
+        host += [(c, k, j, t) for (k, j, t) in poisson_inputs(rng, s, 300.0, 0.02)]
+    gpu, ref, gs, rs = run_network(pkg, oracle_mod, specs, [], host, 25e-6, 800, "fp64", mode, (0.5, 1.0))
+    for g, r in zip(gpu, ref):
+        assert np.abs(g - r).max() <= FP64_TOL
+    assert gs == rs
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["fused", "split"])
 def test_gpu_mixed_synapses_host_spikes(pkg, oracle_mod, mode):
     print(f"case: SynChan cells, host spikes, 1000 steps, mixed, {mode}", flush=True)
     rng = np.random.default_rng(41)
