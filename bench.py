@@ -77,6 +77,33 @@ def build_uniform_hh(ncells, seed, n=2048):
     return cells
 
 
+def build_network(ncells, seed, fan_in=10):
+    """A spiking network: random binary-tree HH cells (n ~ U{50..500}, soma
+    inject U[0, 0.3] nA), each with a SynChan on its soma (tau 1/3 ms, Ek 0)
+    whose fan_in synapses (weight U[0.5, 1.5], delay U[1, 5] ms) receive the
+    SpikeGens of random other cells.  Returns the cells and the connections
+    (source cell, spikegen, target cell, synchan, synapse)."""
+    from moose_core_amd._abi import SYNAPSE_DT, SYNCHAN_DT
+    rng = np.random.default_rng(seed)
+    cells, conns = [], []
+    for c in range(ncells):
+        s = workloads.random_hh_cell(rng, int(rng.integers(50, 501)))
+        sc = np.zeros(1, SYNCHAN_DT)
+        sc["compartment"], sc["n_synapses"] = 0, fan_in
+        sc["Gbar"], sc["Ek"], sc["tau1"], sc["tau2"], sc["modulation"] = 3e-9, 0.0, 1e-3, 3e-3, 1.0
+        s.synchans = sc
+        syn = np.zeros(fan_in, SYNAPSE_DT)
+        syn["weight"] = rng.uniform(0.5, 1.5, fan_in)
+        syn["delay"] = rng.uniform(1e-3, 5e-3, fan_in)
+        s.synapses = syn
+        cells.append(s)
+        for j, src in enumerate(rng.choice(ncells, size=fan_in, replace=False)):
+            conns.append((int(src), 0, c, 0, j))
+    log(f"[bench] built a {ncells}-cell network ({sum(c.n for c in cells)} compartments, "
+        f"{len(conns)} connections)")
+    return cells, conns
+
+
 # workload name -> (description, default cells, dt, default steps, steps per
 # launch, graph chunk)
 WORKLOADS = {
@@ -87,14 +114,19 @@ WORKLOADS = {
                  "graphs of 100 steps", 256, 20e-6, 10000, 100, 1),
     "hh_2048": ("uniform HH cells: 1953 random binary trees of 2048 compartments (4.0 M), dt=25us, "
                 "1000 steps", 1953, 25e-6, 1000, 50, 0),
+    "network": ("spiking network: 4096 random binary-tree HH cells, n~U{50..500}, a somatic SynChan each "
+                "with 10 synapses from random cells' SpikeGens (delays 1-5 ms), dt=25us, 4000 steps",
+                4096, 25e-6, 4000, 50, 0),
 }
 
 
-def run_form(precision, packed, dt, steps, warmup, spl, graph, threads, mode="fused"):
+def run_form(precision, packed, dt, steps, warmup, spl, graph, threads, mode="fused", conns=()):
     eng = pkg.Engine(precision, steps_per_launch=spl, graph_chunk=graph, block_threads=threads,
                      kernel_mode=mode)
     t = time.time()
     ids = [eng.add_cell(p, dt) for p in packed]
+    for (a, g, b, k, j) in conns:
+        eng.connect_spike(ids[a], g, ids[b], k, j)
     log(f"[bench] {precision}: {len(ids)} cells set up in {time.time() - t:.1f}s")
     for cid in ids:
         eng.set_probes(cid, [0], steps)
@@ -272,8 +304,11 @@ def main():
     args.steps = dsteps if args.steps is None else args.steps
     args.steps_per_launch = dspl if args.steps_per_launch is None else args.steps_per_launch
     args.graph_chunk = dgraph if args.graph_chunk is None else args.graph_chunk
+    conns = ()
     if args.workload == "ensemble":
         cells = build_cells(args.cells, args.seed)
+    elif args.workload == "network":
+        cells, conns = build_network(args.cells, args.seed)
     elif args.workload == "hh_2048":
         cells = build_uniform_hh(args.cells, args.seed)
     else:
@@ -283,7 +318,7 @@ def main():
     res = {}
     for form in args.forms.split(","):
         res[form] = run_form(form, packed, dt, args.steps, args.warmup, args.steps_per_launch,
-                             args.graph_chunk, args.threads, args.kernel_mode)
+                             args.graph_chunk, args.threads, args.kernel_mode, conns)
     head = "mixed" if "mixed" in res else list(res)[0]
     wall, launches, total_ms, mean_ms, soma, spikes = res[head]
     value = ncomp * args.steps / wall
@@ -293,7 +328,7 @@ def main():
     bytes_per_launch = bpcs[head] * ncomp * steps_per_launch
     achieved = bytes_per_launch / (mean_ms * 1e-3) / 1e9
     wl_key = {"ensemble": f"hh_ensemble_{args.cells}", "large_ca": f"large_ca_{args.cells}",
-              "hh_2048": f"hh_2048_{args.cells}"}[args.workload]
+              "hh_2048": f"hh_2048_{args.cells}", "network": f"network_{args.cells}"}[args.workload]
     traffic = measured_traffic(f"hsolve_step_kernel/{head}/{wl_key}", ncomp * steps_per_launch)
     out = {
         "metric": "compartment-steps/s (batched cells, FP32-channel mixed form)",
@@ -310,7 +345,8 @@ def main():
         "data": "synthetic (seeded random morphologies)",
         "config": {"workload": desc, "cells": args.cells, "compartments": ncomp, "dt": dt,
                    "steps_per_launch": args.steps_per_launch, "graph_chunk": args.graph_chunk,
-                   "parallelism": "1 GPU, one workgroup per cell"},
+                   "parallelism": "1 GPU, one workgroup per cell"}
+                  | ({"connections": len(conns), "spikes_routed": int(len(spikes))} if conns else {}),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "hsolve_step_kernel", "bytes_per_compartment_step": bpcs[head],

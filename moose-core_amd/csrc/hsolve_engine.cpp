@@ -188,6 +188,7 @@ struct HostCell {
     std::vector<double> injBasal, injVar;
     std::map<int, std::pair<double, double>> ext;   // Hines idx -> (sum Gk, sum Gk*Ek)
     std::vector<double> prevExtG, prevExtGE;        // for getIm
+    bool prevExtSet = false;                        // some prevExt entry is nonzero
     std::map<int, double> ica;                      // desc pool -> pending iCa
     std::vector<int> typeIds;                       // local type -> engine type
     int tableSet = 0;
@@ -271,6 +272,9 @@ struct hsg_engine {
     struct SpikeTarget { int32_t cell, synchan, synapse; };
     std::map<std::pair<int32_t, int32_t>, std::vector<SpikeTarget>> conns;
     std::vector<SpikeRec> spikeBuf;   // spikes read for routing, not yet handed out
+    Block actBlk;                     // the SynChans' activation list of the current call
+    bool actEmpty = false;            // the list last copied held no events
+    size_t actCells = 0;              // cells it covered
     // graph of the steady-state launches
     hipGraphExec_t graphExec = nullptr;
     int graphSteps = 0;
@@ -1022,12 +1026,16 @@ int flush_inputs(hsg_engine* e) {
 // bookkeeping after a step call: one-shot inputs are consumed
 void consume_inputs(hsg_engine* e, HostCell& hc, int64_t nsteps) {
     const int n = hc.s.n;
-    hc.prevExtG.assign(n, 0.0);
-    hc.prevExtGE.assign(n, 0.0);
+    if (hc.prevExtSet) {
+        hc.prevExtG.assign(n, 0.0);
+        hc.prevExtGE.assign(n, 0.0);
+        hc.prevExtSet = false;
+    }
     if (nsteps == 1)
         for (auto& kv : hc.ext) {
             hc.prevExtG[kv.first] = kv.second.first;
             hc.prevExtGE[kv.first] = kv.second.second;
+            hc.prevExtSet = true;
         }
     if (!hc.ext.empty()) {
         hc.ext.clear();
@@ -1041,61 +1049,82 @@ void consume_inputs(hsg_engine* e, HostCell& hc, int64_t nsteps) {
 }
 
 // SimpleSynHandler::vProcess (synapse/SimpleSynHandler.cpp:104-112) for the
-// next nsteps ticks of the cell: per SynChan, the ticks whose end time
+// next nsteps ticks of each given cell: per SynChan, the ticks whose end time
 // reaches queued events and the activation sum(weight / dt) each sends, in
-// the order the queue pops them.  The list goes to the device, where the
-// SynChan step reads it tick by tick.  Events arriving later (from spikes
-// of these ticks) are due after them: hsg_step keeps each call within the
+// the order the queue pops them.  One list for all cells goes to the device
+// in one copy (events, then per-SynChan first and end positions); the
+// SynChan step reads it tick by tick.  Events arriving later (from spikes of
+// these ticks) are due after them: hsg_step keeps each call within the
 // shortest connection delay.
-int write_activations(hsg_engine* e, HostCell& hc, int64_t nsteps) {
-    const CellSetup& s = hc.s;
-    const int ns = (int)s.syn.size();
-    if (ns == 0) return 0;
+int write_activations(hsg_engine* e, const std::vector<HostCell*>& cells, const std::vector<int64_t>& nsteps) {
     const double dt = e->dt;
-    const int64_t s0 = hc.stepsDone;
     std::vector<ActDev> acts;
-    std::vector<int32_t> cur(ns), end(ns);
-    for (int k = 0; k < ns; ++k) {
-        cur[k] = (int32_t)acts.size();
-        auto& q = hc.synQ[k];
-        size_t p = 0;
-        while (p < q.size()) {
-            // the first tick of this call whose end time reaches event p
-            const double tq = q[p].first;
-            int64_t st = s0 + 1;
-            const double est = std::ceil(tq / dt);
-            if (est > (double)st) st = est > (double)(s0 + nsteps + 1) ? s0 + nsteps + 1 : (int64_t)est;
-            while (st > s0 + 1 && tq <= dt * (double)(unsigned long long)(st - 1)) --st;
-            while (st <= s0 + nsteps && !(tq <= dt * (double)(unsigned long long)st)) ++st;
-            if (st > s0 + nsteps) break;
-            const double t = dt * (double)(unsigned long long)st;
-            double a = 0.0;
-            while (p < q.size() && q[p].first <= t) {
-                a += q[p].second / dt;
-                ++p;
+    std::vector<int32_t> cur, end;
+    std::vector<int64_t> base(cells.size());
+    for (size_t c = 0; c < cells.size(); ++c) {
+        HostCell& hc = *cells[c];
+        base[c] = (int64_t)cur.size();
+        const int64_t s0 = hc.stepsDone, s1 = hc.stepsDone + nsteps[c];
+        for (size_t k = 0; k < hc.s.syn.size(); ++k) {
+            cur.push_back((int32_t)acts.size());
+            auto& q = hc.synQ[k];
+            size_t p = 0;
+            while (p < q.size()) {
+                // the first tick of this call whose end time reaches event p
+                const double tq = q[p].first;
+                int64_t st = s0 + 1;
+                const double est = std::ceil(tq / dt);
+                if (est > (double)st) st = est > (double)(s1 + 1) ? s1 + 1 : (int64_t)est;
+                while (st > s0 + 1 && tq <= dt * (double)(unsigned long long)(st - 1)) --st;
+                while (st <= s1 && !(tq <= dt * (double)(unsigned long long)st)) ++st;
+                if (st > s1) break;
+                const double t = dt * (double)(unsigned long long)st;
+                double a = 0.0;
+                while (p < q.size() && q[p].first <= t) {
+                    a += q[p].second / dt;
+                    ++p;
+                }
+                if (a != 0.0) acts.push_back(ActDev{st, a});
             }
-            if (a != 0.0) acts.push_back(ActDev{st, a});
+            q.erase(q.begin(), q.begin() + (std::ptrdiff_t)p);
+            end.push_back((int32_t)acts.size());
         }
-        q.erase(q.begin(), q.begin() + (std::ptrdiff_t)p);
-        end[k] = (int32_t)acts.size();
     }
-    const int64_t need = std::max<int64_t>(1, (int64_t)acts.size()) * (int64_t)sizeof(ActDev);
-    if (hc.act.off < 0 || hc.act.bytes < need) {
-        if (hc.act.off >= 0) e->arena.release(hc.act.off, hc.act.bytes);
-        hc.act.off = -1;
+    if (cur.empty()) return 0;
+    // an empty list after an empty list: the device cursors already say so
+    if (acts.empty() && e->actEmpty && e->actCells == cells.size()) return 0;
+    const int64_t nA = (int64_t)acts.size(), nS = (int64_t)cur.size();
+    const int64_t offCur = align_up(nA * (int64_t)sizeof(ActDev));
+    const int64_t offEnd = offCur + align_up(4 * nS);
+    const int64_t need = offEnd + align_up(4 * nS);
+    if (e->actBlk.off < 0 || e->actBlk.bytes < need) {
+        if (e->actBlk.off >= 0) e->arena.release(e->actBlk.off, e->actBlk.bytes);
+        e->actBlk.off = -1;
         int64_t off;
         int rc = arena_alloc(e, 2 * need, &off);
         if (rc) return rc;
-        hc.act.off = off;
-        hc.act.bytes = 2 * need;
+        e->actBlk.off = off;
+        e->actBlk.bytes = 2 * need;
     }
-    int rc;
-    if (!acts.empty() && (rc = upload(e, hc.act.off, acts))) return rc;
-    if ((rc = upload(e, hc.dev.oActCur, cur)) || (rc = upload(e, hc.dev.oActEnd, end))) return rc;
-    if (hc.dev.oAct != hc.act.off) {
-        hc.dev.oAct = hc.act.off;
-        e->tablesDirty = true;
+    std::vector<char> blob((size_t)need, 0);
+    if (nA) std::memcpy(blob.data(), acts.data(), sizeof(ActDev) * (size_t)nA);
+    std::memcpy(blob.data() + offCur, cur.data(), 4 * (size_t)nS);
+    std::memcpy(blob.data() + offEnd, end.data(), 4 * (size_t)nS);
+    int rc = upload(e, e->actBlk.off, blob);
+    if (rc) return rc;
+    for (size_t c = 0; c < cells.size(); ++c) {
+        CellDev& d = cells[c]->dev;
+        const int64_t a = e->actBlk.off, ac = e->actBlk.off + offCur + 4 * base[c],
+                      ae = e->actBlk.off + offEnd + 4 * base[c];
+        if (d.oAct != a || d.oActCur != ac || d.oActEnd != ae) {
+            d.oAct = a;
+            d.oActCur = ac;
+            d.oActEnd = ae;
+            e->tablesDirty = true;
+        }
     }
+    e->actEmpty = acts.empty();
+    e->actCells = cells.size();
     return 0;
 }
 
@@ -1228,8 +1257,6 @@ int run_launches(hsg_engine* e, const std::vector<int32_t>& order, const std::ve
         whole = a == b;
         for (size_t i = 0; i < steps.size() && whole; ++i) whole = steps[i] == steps[0];
     }
-    e->lastLaunches = 0;
-    e->lastTotalMs = 0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> evs;
     auto event_pair = [&](size_t k) -> std::pair<hipEvent_t, hipEvent_t> {
         while (e->evStart.size() <= k) {
@@ -1556,12 +1583,14 @@ int hsg_step(hsg_engine* e, int64_t nsteps) {
     // due within it, the handlers' activations prepared before each and its
     // spikes routed after it
     const int64_t L = anySyn ? routed_stretch(e) : nsteps;
+    std::vector<HostCell*> synCells;
+    for (auto& c : e->cells) if (c && c->live && !c->s.syn.empty()) synCells.push_back(c.get());
+    e->lastLaunches = 0;
+    e->lastTotalMs = 0;
     for (int64_t done = 0; done < nsteps;) {
         const int64_t m = std::min(nsteps - done, L);
         if ((rc = flush_inputs(e))) return rc;
-        if (anySyn)
-            for (auto& c : e->cells)
-                if (c && c->live && (rc = write_activations(e, *c, m))) return rc;
+        if (anySyn && (rc = write_activations(e, synCells, std::vector<int64_t>(synCells.size(), m)))) return rc;
         if ((rc = upload_tables(e))) return rc;
         std::vector<int64_t> steps(e->order.size(), m);
         if (!e->order.empty() && (rc = run_launches(e, e->order, steps))) return rc;
@@ -1586,8 +1615,16 @@ int hsg_step_cells(hsg_engine* e, const int32_t* cells, const int64_t* nsteps, i
     if (!e->conns.empty()) return fail(HSG_E_UNSUPPORTED, "cells joined by spike connections step with hsg_step");
     int rc;
     if ((rc = flush_inputs(e))) return rc;
+    std::vector<HostCell*> synCells;
+    std::vector<int64_t> synSteps;
     for (size_t i = 0; i < ord.size(); ++i)
-        if ((rc = write_activations(e, *e->cells[ord[i]], st[i]))) return rc;
+        if (!e->cells[ord[i]]->s.syn.empty()) {
+            synCells.push_back(e->cells[ord[i]].get());
+            synSteps.push_back(st[i]);
+        }
+    if ((rc = write_activations(e, synCells, synSteps))) return rc;
+    e->lastLaunches = 0;
+    e->lastTotalMs = 0;
     if ((rc = upload_tables(e))) return rc;
     if (!ord.empty() && (rc = run_launches(e, ord, st))) return rc;
     for (size_t i = 0; i < ord.size(); ++i) consume_inputs(e, *e->cells[ord[i]], st[i]);
