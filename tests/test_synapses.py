@@ -513,3 +513,36 @@ def test_gpu_layer_synapse_pair(pkg, oracle_mod):
     # b fired from a's input
     assert tabs["b"].vector.max() > 0.0
     moose.reset()
+
+
+@pytest.mark.gpu
+def test_gpu_step_cells_with_synapses(pkg, oracle_mod):
+    """hsg_step_cells (the serving path) on SynChan cells advancing by
+    different step counts: each cell's handler queue is popped over its own
+    ticks.  Spike connections are refused there."""
+    print("case: step_cells on 3 SynChan cells with different step counts, fp64", flush=True)
+    rng = np.random.default_rng(61)
+    specs = [syn_hh_cell(rng, n) for n in (40, 90, 160)]
+    dt = 25e-6
+    eng = pkg.Engine("fp64")
+    ids = [eng.add_cell(s, dt) for s in specs]
+    eng.reinit(dt)
+    refs = [oracle_mod.OracleCell(s, dt) for s in specs]
+    for r in refs:
+        r.reinit(dt)
+    for c, s in enumerate(specs):
+        for (k, j, t) in poisson_inputs(rng, s, 400.0, 0.02):
+            eng.add_spike(ids[c], k, j, t)
+            refs[c].add_spike(k, j, t)
+    for steps in ((100, 250, 40), (300, 0, 500)):
+        eng.step_cells(ids, list(steps))
+        for r, m in zip(refs, steps):
+            if m:
+                r.run(dt, m)
+    for cid, s, r in zip(ids, specs, refs):
+        v = eng.get(cid, "Vm", np.arange(s.n))
+        assert np.abs(v - r.get("Vm", np.arange(s.n))).max() <= FP64_TOL
+    eng.connect_spike(ids[0], 0, ids[1], 0, 0)
+    with pytest.raises(pkg.HSolveError):
+        eng.step_cells(ids, [1, 1, 1])
+    eng.close()
