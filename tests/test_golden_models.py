@@ -1,28 +1,39 @@
 """End-to-end golden vectors from the reference's own test suite.
 
-tests/rdesigneur/test_32_squid_axon_propgn.py (moose-core) builds a squid
-axon with rdesigneur and checks 41 soma Vm samples (numpy.allclose).  The
-model is rebuilt here through the object layer as rdesigneur would build it:
+Four rdesigneur tests of moose-core hold numbers this path must reproduce:
 
-* compartments as rdesigneur's buildCompt makes them
+* tests/rdesigneur/test_1_minimalModel.py: the fields of rdesigneur's
+  default soma (makePassiveHHsoma);
+* tests/rdesigneur/test_20_currentPulse.py: that soma's Vm under a 20 nA
+  pulse, as mean, std and a 10-bin histogram of the plot;
+* tests/rdesigneur/test_32_squid_axon_propgn.py: a 200-segment squid axon
+  with a propagating spike, 41 soma Vm samples;
+* tests/rdesigneur/test_41_ballAndStick.py: a ball-and-stick HH cell, mean
+  and std of the soma plot.
+
+The models are rebuilt through the object layer as rdesigneur builds them:
+
+* compartments as buildCompt makes them
   (python/rdesigneur/rdesigneurProtos.py:620-638: length from dx, dy; Ra, Rm,
-  Cm from RA, RM, CM with that file's PI = 3.14159265359; Em = initVm =
-  -65 mV), a 10 um soma and 200 axon segments of 10 um x 1 um bending as the
-  test's makeAxonProto does;
-* HH Na and K channels with tables as make_HH_Na / make_HH_K fill them
+  Cm from RA, RM, CM with that file's PI = 3.14159265359), the passive soma
+  as makePassiveHHsoma (:794-808), the ball-and-stick and the test's axon as
+  their builders lay them out (rdesigneur.py:505-521);
+* HH Na and K tables as make_HH_Na / make_HH_K fill them
   (rdesigneurProtos.py:61-108: 3000 divisions over -110..50 mV and
-  -120..40 mV, the squid rate functions of V - EREST_ACT_HH), distributed to
-  every compartment with Gbar = density * len * dia * PI
-  (biophysics/Neuron.cpp:703-712, PI = 3.141592653589793), Na before K;
+  -120..40 mV, the squid rate functions of V - EREST_ACT_HH), distributed
+  with Gbar = density * len * dia * PI (biophysics/Neuron.cpp:703-712,
+  PI = 3.141592653589793), in chanDistrib order;
 * an HSolve at dt = 50 us on the soma, a Table on the 0.1 ms plot clock
-  (tick 8), and the stimulus "(t>0.005 && t<0.2) * 2e-11" into the soma's
-  inject: a Function on tick 12 at 0.1 ms (rdesigneur.py:1252-1330,
-  scheduling/Clock.cpp:928), which runs after the solver's tick 6 at
-  t = dt * step for even steps, so the injection starts with step 103 (the
-  first even step with t > 5 ms is 102; 50e-6 * 100 == 0.005 exactly).
+  (tick 8, a sample at reinit and after every second tick), and the stimulus
+  expression into the soma's inject from a Function on tick 12 at 0.1 ms
+  (rdesigneur.py:1252-1330, scheduling/Clock.cpp:928): it runs after the
+  solver's tick 6 at t = dt * step on even steps, so a value it computes at
+  step s is first used by step s + 1 (e.g. "t > 0.01" first holds at step
+  202: 50e-6 * 200 == 0.01 exactly).
 
-The oracle (CPU) and the GPU path (FP64 and mixed) are each checked against
-the reference's 41 values with the reference test's own numpy.allclose.
+The oracle (CPU) and the GPU path in FP64 are checked with the reference
+tests' own comparisons (numpy.allclose); the mixed form against its own
+0.1 mV bar where the reference compares traces.
 """
 import json
 import os
@@ -31,20 +42,22 @@ import numpy as np
 import pytest
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-GOLD = json.load(open(os.path.join(HERE, "golden", "squid_axon_propagation.json")))
+
+
+def golden(name):
+    return json.load(open(os.path.join(HERE, "golden", name)))
+
 
 PI_PROTOS = 3.14159265359        # rdesigneurProtos.py's PI (buildCompt)
 PI = 3.141592653589793           # basecode/header.h (channel distribution)
 EREST_ACT_HH = -0.070
 DT = 50e-6
-ONSET_STEP = 102                 # the Function's first firing with t > 5 ms
-NSTEPS = 800                     # 40 ms
 
 
 def squid_tables(kind):
     """Gate tables (A = alpha, B = alpha + beta, per second) of the squid
-    channels over the test model's voltage grids, in the arithmetic order of
-    the model's rate functions."""
+    channels over the test models' voltage grids, in the arithmetic order of
+    the models' rate functions."""
     if kind == "Na":
         v = np.linspace(-110e-3, 50e-3, 3001) - EREST_ACT_HH
         m_alpha = 1e3 * (25 - v * 1e3) / (10 * (np.exp((25 - v * 1e3) / 10) - 1))
@@ -58,10 +71,9 @@ def squid_tables(kind):
     return [(-120e-3, 40e-3, n_alpha, n_alpha + n_beta)]
 
 
-def build_squid_axon(moose, nseg=200, comptLen=10e-6, comptDia=1e-6, RM=1.0, RA=10.0, CM=0.01):
-    moose.reset()
+def squid_protos(moose):
     moose.Neutral("/library")
-    protos = []
+    protos = {}
     for name, ek, xp, yp in (("Na", 50e-3, 3, 1), ("K", -77e-3, 4, 0)):
         ch = moose.HHChannel(f"/library/{name}")
         ch.Ek = ek
@@ -72,50 +84,143 @@ def build_squid_axon(moose, nseg=200, comptLen=10e-6, comptDia=1e-6, RM=1.0, RA=
             gate.min, gate.max, gate.divs = vmin, vmax, 3000
             gate.tableA = A
             gate.tableB = B
-        protos.append(ch)
+        protos[name] = ch
+    return protos
+
+
+def build_compt(moose, path, RM=1.0, RA=1.0, CM=0.01, dia=1.0e-6, x=0.0, y=0.0, dx=10e-6, dy=0.0,
+                Em=-0.065, initVm=-0.065):
+    length = np.sqrt(dx * dx + dy * dy)
+    c = moose.Compartment(path)
+    c.x0, c.y0, c.x, c.y = x, y, dx + x, dy + y
+    c.diameter, c.length = dia, length
+    xa = dia * dia * PI_PROTOS / 4.0
+    sa = length * dia * PI_PROTOS
+    c.Ra = length * RA / xa
+    c.Rm = RM / sa
+    c.Cm = CM * sa
+    c.Em = Em
+    c.initVm = initVm
+    return c
+
+
+def distribute(moose, proto, comps, density):
+    for c in comps:
+        ch = moose.copy(proto, c, proto.name)
+        ch.Gbar = density * c.length * c.diameter * PI
+        moose.connect(ch, "channel", c, "channel")
+
+
+def chain(moose, comps):
+    for a, b in zip(comps, comps[1:]):
+        moose.connect(a, "axial", b, "raxial")
+
+
+def build_passive_soma(moose):
+    moose.reset()
     moose.Neutral("/model")
     moose.Neutral("/model/elec")
+    dia = 500e-6
+    soma = build_compt(moose, "/model/elec/soma", dx=dia, dia=dia, x=0.0, RM=0.33333333, RA=3000, CM=0.01)
+    soma.initVm = -65e-3
+    soma.Em = -54.4e-3
+    return [soma]
 
-    def build_compt(name, dia, x, y, dx, dy):
-        length = np.sqrt(dx * dx + dy * dy)
-        c = moose.Compartment(f"/model/elec/{name}")
-        c.diameter, c.length = dia, length
-        xa = dia * dia * PI_PROTOS / 4.0
-        sa = length * dia * PI_PROTOS
-        c.Ra = length * RA / xa
-        c.Rm = RM / sa
-        c.Cm = CM * sa
-        c.Em = c.initVm = -0.065
-        return c
 
-    comps = [build_compt("soma", 10e-6, 0.0, 0.0, comptLen, 0.0)]
+def build_ball_and_stick(moose):
+    moose.reset()
+    protos = squid_protos(moose)
+    moose.Neutral("/model")
+    moose.Neutral("/model/elec")
+    soma = build_compt(moose, "/model/elec/soma", dia=20e-6, dx=20e-6)
+    comps, x, dx = [soma], soma.x, 500e-6 / 10
+    for i in range(10):
+        comps.append(build_compt(moose, f"/model/elec/dend{i}", x=x, dx=dx, dia=4e-6))
+        x += dx
+    chain(moose, comps)
+    distribute(moose, protos["Na"], comps[:1], 1200.0)
+    distribute(moose, protos["K"], comps[:1], 360.0)
+    distribute(moose, protos["Na"], comps[1:], 400.0)
+    distribute(moose, protos["K"], comps[1:], 120.0)
+    return comps
+
+
+def build_squid_axon(moose, nseg=200, comptLen=10e-6, comptDia=1e-6, RM=1.0, RA=10.0, CM=0.01):
+    moose.reset()
+    protos = squid_protos(moose)
+    moose.Neutral("/model")
+    moose.Neutral("/model/elec")
+    comps = [build_compt(moose, "/model/elec/soma", RM=RM, RA=RA, CM=CM, dia=10e-6, x=0, dx=comptLen)]
     theta, x, y = 0.0, comptLen, 0.0
     for i in range(nseg):
         dx = comptLen * np.cos(theta)
         dy = comptLen * np.sin(theta)
         r = np.sqrt(x * x + y * y)
         theta += comptLen / r
-        c = build_compt(f"axon{i}", comptDia, x, y, dx, dy)
-        moose.connect(comps[-1], "axial", c, "raxial")
-        comps.append(c)
+        comps.append(build_compt(moose, f"/model/elec/axon{i}", RM=RM, RA=RA, CM=CM, x=x, y=y, dx=dx, dy=dy,
+                                 dia=comptDia))
         x += dx
         y += dy
-    for proto, dens in zip(protos, (1200.0, 360.0)):
-        for c in comps:
-            ch = moose.copy(proto, c, proto.name)
-            ch.Gbar = dens * c.length * c.diameter * PI
-            moose.connect(ch, "channel", c, "channel")
+    chain(moose, comps)
+    distribute(moose, protos["Na"], comps, 1200.0)
+    distribute(moose, protos["K"], comps, 360.0)
     return comps
 
 
-def samples_ok(v):
-    return np.allclose(v, np.array(GOLD["Vm"]))
+def stimulus_schedule(expr, nsteps):
+    """(step, value): the Function's output changes, computed on its even
+    steps at t = dt * step; the solver sees each from the next step."""
+    out, cur = [], expr(0.0)            # doEvalAtReinit
+    for s in range(2, nsteps + 1, 2):
+        v = expr(DT * s)
+        if v != cur:
+            out.append((s, v))
+            cur = v
+    return out
 
 
-def test_squid_axon_oracle(pkg, oracle_mod):
-    """The oracle on the rebuilt model reproduces the reference's 41 samples."""
+MODELS = {
+    # name: (builder, stimulus, steps)
+    "current_pulse": (build_passive_soma, lambda t: float(t > 0.1 and t < 0.2) * 2e-8, 6000),
+    "ball_and_stick": (build_ball_and_stick, lambda t: float(t > 0.01 and t < 0.05) * 1e-9, 1200),
+    "squid_axon": (build_squid_axon, lambda t: float(t > 0.005 and t < 0.2) * 2e-11, 800),
+}
+
+
+def check(name, v, mixed=False):
+    """The reference test's own comparison of the soma plot v (samples at
+    reinit and every second tick)."""
+    if name == "squid_axon":
+        ref = np.array(golden("squid_axon_propagation.json")["Vm"])
+        dv = np.abs(v[::10] - ref).max()
+        return (dv <= 1e-4) if mixed else np.allclose(v[::10], ref), dv
+    if name == "ball_and_stick":
+        g = golden("ball_and_stick.json")
+        got = [np.mean(v), np.std(v)]
+        return np.allclose(got, [g["mean"], g["std"]], rtol=1e-3 if mixed else 1e-5), got
+    g = golden("current_pulse_soma.json")
+    counts, edges = np.histogram(v)
+    ok = (np.allclose(np.mean(v), g["mean"]) and np.allclose(np.std(v), g["std"])
+          and np.allclose(counts, g["hist_counts"]) and np.allclose(edges, g["hist_edges"]))
+    return ok, (np.mean(v), np.std(v), counts.tolist())
+
+
+def test_passive_soma_fields(pkg):
+    """buildCompt as restated gives the reference's printed soma fields."""
     from moose_core_amd import moose
-    comps = build_squid_axon(moose)
+    soma = build_passive_soma(moose)[0]
+    g = golden("passive_hh_soma_fields.json")
+    for f in ("Cm", "Em", "Ra", "Rm", "initVm", "diameter", "length"):
+        assert getattr(soma, f) == g[f], f
+    moose.reset()
+
+
+@pytest.mark.parametrize("name", sorted(MODELS))
+def test_reference_model_oracle(pkg, oracle_mod, name):
+    """The oracle on the rebuilt model passes the reference test's check."""
+    from moose_core_amd import moose
+    build, expr, nsteps = MODELS[name]
+    comps = build(moose)
     hs = moose.HSolve("/model/elec/hsolve")
     hs.dt = DT
     spec, objs = hs.build_spec(comps[0])
@@ -123,26 +228,34 @@ def test_squid_axon_oracle(pkg, oracle_mod):
     o = oracle_mod.OracleCell(spec, DT)
     o.reinit(DT)
     v = [o.get("Vm", [soma])[0]]
-    tr = o.run(DT, ONSET_STEP, [soma])[:, 0]
-    o.set("inject", [soma], [2e-11])
-    tr = np.concatenate([tr, o.run(DT, NSTEPS - ONSET_STEP, [soma])[:, 0]])
-    v = np.array(v + list(tr[19::20]))
-    assert len(v) == 41
-    print("max |dV| vs the reference's samples:", np.abs(v - np.array(GOLD["Vm"])).max())
-    assert samples_ok(v)
+    done = 0
+    trace = []
+    for s, val in stimulus_schedule(expr, nsteps) + [(nsteps, None)]:
+        trace.append(o.run(DT, s - done, [soma])[:, 0])
+        done = s
+        if val is not None:
+            o.set("inject", [soma], [val])
+    tr = np.concatenate(trace)
+    v = np.array(v + list(tr[1::2]))
+    ok, got = check(name, v)
+    print(name, got)
+    assert ok, got
     moose.reset()
 
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_squid_axon_gpu(pkg, precision):
-    """The same model through the object layer on the GPU: Table on the
-    0.1 ms plot clock, the stimulus written at the Function's firing."""
-    print(f"case: squid axon (rdesigneur test_32), 201 compartments, {precision}", flush=True)
+@pytest.mark.parametrize("name", sorted(MODELS))
+def test_reference_model_gpu(pkg, name, precision):
+    """The same models through the object layer on the GPU: the soma Table
+    on the 0.1 ms plot clock, the stimulus written at the Function's
+    firings."""
+    print(f"case: reference model {name}, {precision}", flush=True)
     from moose_core_amd import moose
     moose.set_precision(precision)
     try:
-        comps = build_squid_axon(moose)
+        build, expr, nsteps = MODELS[name]
+        comps = build(moose)
         moose.Neutral("/model/graphs")
         tab = moose.Table("/model/graphs/plot0")
         moose.connect(tab, "requestOut", comps[0], "getVm")
@@ -151,13 +264,17 @@ def test_squid_axon_gpu(pkg, precision):
         hs.dt = DT
         hs.target = "/model/elec/soma"
         moose.reinit()
-        moose.start(ONSET_STEP * DT)
-        comps[0].inject = 2e-11
-        moose.start((NSTEPS - ONSET_STEP) * DT)
+        done = 0
+        for s, val in stimulus_schedule(expr, nsteps) + [(nsteps, None)]:
+            moose.start((s - done) * DT)
+            done = s
+            if val is not None:
+                comps[0].inject = val
         v = tab.vector
-        assert len(v) == 401
-        print("max |dV| vs the reference's samples:", np.abs(v[::10] - np.array(GOLD["Vm"])).max(), flush=True)
-        assert samples_ok(v[::10])
+        assert len(v) == nsteps // 2 + 1
+        ok, got = check(name, v, mixed=precision == "mixed")
+        print(name, precision, got, flush=True)
+        assert ok, got
     finally:
         moose.set_precision("fp64")
         moose.reset()
