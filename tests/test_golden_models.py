@@ -1,6 +1,6 @@
 """End-to-end golden vectors from the reference's own test suite.
 
-Four rdesigneur tests of moose-core hold numbers this path must reproduce:
+Five rdesigneur tests of moose-core hold numbers this path must reproduce:
 
 * tests/rdesigneur/test_1_minimalModel.py: the fields of rdesigneur's
   default soma (makePassiveHHsoma);
@@ -9,7 +9,9 @@ Four rdesigneur tests of moose-core hold numbers this path must reproduce:
 * tests/rdesigneur/test_32_squid_axon_propgn.py: a 200-segment squid axon
   with a propagating spike, 41 soma Vm samples;
 * tests/rdesigneur/test_41_ballAndStick.py: a ball-and-stick HH cell, mean
-  and std of the soma plot.
+  and std of the soma plot;
+* tests/rdesigneur/test_51_periodic_syn_input.py: a soma with a glu SynChan
+  driven at 50 Hz by a periodic RandSpike, mean and std of the soma plot.
 
 The models are rebuilt through the object layer as rdesigneur builds them:
 
@@ -179,12 +181,62 @@ def stimulus_schedule(expr, nsteps):
     return out
 
 
+def build_periodic_syn(moose):
+    """rdesigneur's somaProto (20 um x 200 um) with make_glu's SynChan on it
+    (tau1 2 ms, tau2 9 ms, Ek 0, Gbar 1 S/m^2 by chanDistrib) and its
+    handler's synapse[0] at weight 0.5 (rdesigneurProtos.py:448-459;
+    rdesigneur.py:582-597)."""
+    moose.reset()
+    moose.Neutral("/model")
+    moose.Neutral("/model/elec")
+    soma = build_compt(moose, "/model/elec/soma", dia=20e-6, dx=200e-6)
+    glu = moose.SynChan("/model/elec/soma/glu")
+    glu.Ek, glu.tau1, glu.tau2 = 0.0, 2.0e-3, 9.0e-3
+    glu.Gbar = 1.0 * soma.length * soma.diameter * PI
+    moose.connect(glu, "channel", soma, "channel")
+    sh = moose.SimpleSynHandler("/model/elec/soma/glu/sh")
+    moose.connect(sh, "activationOut", glu, "activation")
+    sh.numSynapses = 1
+    sh.synapse[0].weight = 0.5
+    return [soma]
+
+
+def periodic_spikes(rate, nsteps, refractT=5e-3):
+    """Steps at which a periodic RandSpike fires (biophysics/RandSpike.cpp:
+    204-229) on tick 1 at t = dt * step: its rate is still 0 at its reinit
+    (the rate Function reinits later, on tick 12), so lastEvent starts at 0
+    and no random number is drawn.  The handler processes after it on the
+    same tick and pops the event at once (with the other order the plot's
+    std is 8e-4 off the reference's; with this one mean and std are equal
+    to the last digit)."""
+    out, last = [], 0.0
+    for s in range(1, nsteps + 1):
+        t = DT * s
+        if refractT > t - last:
+            continue
+        if (t - last) > 1.0 / rate:
+            out.append(s)
+            last = t
+    return out
+
+
 MODELS = {
     # name: (builder, stimulus, steps)
     "current_pulse": (build_passive_soma, lambda t: float(t > 0.1 and t < 0.2) * 2e-8, 6000),
     "ball_and_stick": (build_ball_and_stick, lambda t: float(t > 0.01 and t < 0.05) * 1e-9, 1200),
     "squid_axon": (build_squid_axon, lambda t: float(t > 0.005 and t < 0.2) * 2e-11, 800),
+    "periodic_syn": (build_periodic_syn, None, 6000),
 }
+
+
+def events(name):
+    """(step, kind, value) after which the stimulus changes: 'inject' (the
+    Function's output) or 'spike' (a synaptic event at t = dt * step)."""
+    build, expr, nsteps = MODELS[name]
+    if expr is not None:
+        return [(s, "inject", v) for s, v in stimulus_schedule(expr, nsteps)]
+    # queued before the tick that fires it, which pops it
+    return [(s - 1, "spike", DT * s) for s in periodic_spikes(50.0, nsteps)]
 
 
 def check(name, v, mixed=False):
@@ -194,6 +246,12 @@ def check(name, v, mixed=False):
         ref = np.array(golden("squid_axon_propagation.json")["Vm"])
         dv = np.abs(v[::10] - ref).max()
         return (dv <= 1e-4) if mixed else np.allclose(v[::10], ref), dv
+    if name == "periodic_syn":
+        g = golden("periodic_syn_input.json")
+        got = [np.mean(v), np.std(v)]
+        # the reference test's rtol (1e-3) for the mixed form; its printed
+        # values to 1e-9 relative otherwise
+        return np.allclose([g["mean"], g["std"]], got, rtol=1e-3 if mixed else 1e-9), got
     if name == "ball_and_stick":
         g = golden("ball_and_stick.json")
         got = [np.mean(v), np.std(v)]
@@ -230,11 +288,13 @@ def test_reference_model_oracle(pkg, oracle_mod, name):
     v = [o.get("Vm", [soma])[0]]
     done = 0
     trace = []
-    for s, val in stimulus_schedule(expr, nsteps) + [(nsteps, None)]:
+    for s, kind, val in events(name) + [(nsteps, None, None)]:
         trace.append(o.run(DT, s - done, [soma])[:, 0])
         done = s
-        if val is not None:
+        if kind == "inject":
             o.set("inject", [soma], [val])
+        elif kind == "spike":
+            o.add_spike(0, 0, val)
     tr = np.concatenate(trace)
     v = np.array(v + list(tr[1::2]))
     ok, got = check(name, v)
@@ -265,11 +325,13 @@ def test_reference_model_gpu(pkg, name, precision):
         hs.target = "/model/elec/soma"
         moose.reinit()
         done = 0
-        for s, val in stimulus_schedule(expr, nsteps) + [(nsteps, None)]:
+        for s, kind, val in events(name) + [(nsteps, None, None)]:
             moose.start((s - done) * DT)
             done = s
-            if val is not None:
+            if kind == "inject":
                 comps[0].inject = val
+            elif kind == "spike":
+                moose.element("/model/elec/soma/glu/sh").synapse[0].addSpike(val)
         v = tab.vector
         assert len(v) == nsteps // 2 + 1
         ok, got = check(name, v, mixed=precision == "mixed")
