@@ -101,6 +101,21 @@ typedef struct hsg_spikegen_desc {
     double  threshold, refractT;
 } hsg_spikegen_desc;
 
+/* A VClamp (biophysics/VClamp.cpp) on a compartment: it senses the
+ * compartment's Vm (VmOut -> sensedIn, the Vm the solver sent at the end of
+ * the previous tick) and sends its PID current to the compartment's
+ * injectMsg on every tick (clock tick 0, before the solver's tick 6),
+ * after the PulseGens.  mode 0: PID on the error; 1: derivative on Vm;
+ * 2: proportional and derivative on Vm.  gain is Kp (0: Cm / dt at reinit);
+ * ti = 0 becomes dt, td < 0 becomes 0, tau = 0 becomes 5 dt at reinit
+ * (VClamp::reinit, :313-347).  The command input (commandIn) is the field
+ * HSG_F_VCLAMP_COMMAND. */
+typedef struct hsg_vclamp_desc {
+    int32_t compartment;
+    int32_t mode;
+    double  tau, ti, td, gain;
+} hsg_vclamp_desc;
+
 /* One Leakage object on a compartment's "channel" message.  Gk is its Gk
  * field when the solver reads the model (0 until the object has been
  * reinitialised or processed): the solver folds Ek * Gk into EmByRm, the
@@ -207,6 +222,8 @@ typedef struct hsg_cell_desc {
     int32_t                     n_synchans;
     const hsg_synchan_desc*     synchans;
     const hsg_synapse_desc*     synapses;   /* sum of the synchans' n_synapses */
+    int32_t                     n_vclamps;
+    const hsg_vclamp_desc*      vclamps;
 } hsg_cell_desc;
 
 /* ---- engine ------------------------------------------------------------ */
@@ -314,6 +331,9 @@ int hsg_step_cells(hsg_engine* e, const int32_t* cells, const int64_t* nsteps,
 /* Synapse fields (index = synapses entry) */
 #define HSG_F_SYNAPSE_WEIGHT 90
 #define HSG_F_SYNAPSE_DELAY  91
+/* VClamp fields (index = vclamps entry; biophysics/VClamp.cpp:212-285) */
+#define HSG_F_VCLAMP_COMMAND 100   /* commandIn: the command voltage input */
+#define HSG_F_VCLAMP_CURRENT 101   /* read-only: the current last sent */
 /* Leakage fields (index = leaks entry): the per-tick send; Gk and the
  * fold into EmByRm stay as set up */
 #define HSG_F_LEAK_GBAR 95

@@ -35,6 +35,7 @@ FIELDS = {
     "GbarSyn": 80, "EkSyn": 81, "GkSyn": 82, "tau1": 83, "tau2": 84, "modulationSyn": 85,
     "XSyn": 86, "YSyn": 87, "weight": 90, "delay": 91,
     "GbarLeak": 95, "EkLeak": 96, "modulationLeak": 97,
+    "command": 100, "current": 101,
 }
 COMPARTMENT_FIELDS = {k for k, v in FIELDS.items() if v < 20}
 CHANNEL_FIELDS = {k for k, v in FIELDS.items() if 20 <= v < 40}
@@ -42,7 +43,8 @@ POOL_FIELDS = {k for k, v in FIELDS.items() if 40 <= v < 60}
 CHAN2D_FIELDS = {k for k, v in FIELDS.items() if 60 <= v < 80}
 SYNCHAN_FIELDS = {k for k, v in FIELDS.items() if 80 <= v < 90}
 SYNAPSE_FIELDS = {k for k, v in FIELDS.items() if 90 <= v < 95}
-LEAK_FIELDS = {k for k, v in FIELDS.items() if v >= 95}
+LEAK_FIELDS = {k for k, v in FIELDS.items() if 95 <= v < 100}
+VCLAMP_FIELDS = {k for k, v in FIELDS.items() if v >= 100}
 
 
 class GateDesc(C.Structure):
@@ -69,7 +71,8 @@ class CellDesc(C.Structure):
                 ("n_tables2d", C.c_int32), ("tables2d", C.POINTER(Table2DDesc)),
                 ("n_chans2d", C.c_int32), ("chans2d", C.c_void_p),
                 ("n_pulsegens", C.c_int32), ("pulsegens", C.c_void_p),
-                ("n_synchans", C.c_int32), ("synchans", C.c_void_p), ("synapses", C.c_void_p)]
+                ("n_synchans", C.c_int32), ("synchans", C.c_void_p), ("synapses", C.c_void_p),
+                ("n_vclamps", C.c_int32), ("vclamps", C.c_void_p)]
 
 
 class Options(C.Structure):
@@ -105,13 +108,14 @@ LEAK_DT = np.dtype([("compartment", "i4"), ("Gk", "f8"), ("Ek", "f8"), ("Gbar", 
 SYNCHAN_DT = np.dtype([("compartment", "i4"), ("n_synapses", "i4")]
                       + [(k, "f8") for k in ("Gbar", "Ek", "tau1", "tau2", "modulation")], align=True)
 SYNAPSE_DT = np.dtype([("weight", "f8"), ("delay", "f8")])
+VCLAMP_DT = np.dtype([("compartment", "i4"), ("mode", "i4")] + [(k, "f8") for k in ("tau", "ti", "td", "gain")])
 CHAN2D_DT = np.dtype([("compartment", "i4"), ("instant", "i4"), ("tableA", "i4", (3,)),
                       ("tableB", "i4", (3,)), ("dep", "i4", (6,)), ("conc1", "i4"), ("conc2", "i4")]
                      + [(k, "f8") for k in ("Gbar", "Ek", "Xpower", "Ypower", "Zpower",
                                             "X", "Y", "Z", "modulation")], align=True)
 
 assert CHANNEL_DT.itemsize == 104 and SPIKEGEN_DT.itemsize == 24 and LEAK_DT.itemsize == 40
-assert SYNCHAN_DT.itemsize == 48 and SYNAPSE_DT.itemsize == 16
+assert SYNCHAN_DT.itemsize == 48 and SYNAPSE_DT.itemsize == 16 and VCLAMP_DT.itemsize == 40
 MAX_PULSES = 4
 PULSEGEN_DT = np.dtype([("compartment", "i4"), ("n_pulses", "i4"), ("baseLevel", "f8"),
                         ("delay", "f8", (MAX_PULSES,)), ("width", "f8", (MAX_PULSES,)),
@@ -153,6 +157,7 @@ class PackedCell:
         self.leaks = as_records(spec.leaks, LEAK_DT)
         self.synchans = as_records(getattr(spec, "synchans", np.zeros(0, SYNCHAN_DT)), SYNCHAN_DT)
         self.synapses = as_records(getattr(spec, "synapses", np.zeros(0, SYNAPSE_DT)), SYNAPSE_DT)
+        self.vclamps = as_records(getattr(spec, "vclamps", np.zeros(0, VCLAMP_DT)), VCLAMP_DT)
         if len(self.synapses) != int(self.synchans["n_synapses"].sum()):
             raise ValueError("synapses: need the sum of the synchans' n_synapses entries")
         self.chans2d = np.ascontiguousarray(spec.chans2d, dtype=CHAN2D_DT)
@@ -208,6 +213,8 @@ class PackedCell:
         d.n_synchans = len(self.synchans)
         d.synchans = _ptr(self.synchans)
         d.synapses = _ptr(self.synapses)
+        d.n_vclamps = len(self.vclamps)
+        d.vclamps = _ptr(self.vclamps)
         self.desc = d
 
     @property

@@ -12,7 +12,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from ._abi import (CACONC_DT, CHAN2D_DT, CHANNEL_DT, COMPARTMENT_DT, LEAK_DT, PULSEGEN_DT, SPIKEGEN_DT,
-                   SYNAPSE_DT, SYNCHAN_DT, PackedCell)
+                   SYNAPSE_DT, SYNCHAN_DT, VCLAMP_DT, PackedCell)
 
 
 @dataclass
@@ -53,6 +53,7 @@ class CellSpec:
     pulsegens: np.ndarray = field(default_factory=lambda: np.zeros(0, PULSEGEN_DT))
     synchans: np.ndarray = field(default_factory=lambda: np.zeros(0, SYNCHAN_DT))
     synapses: np.ndarray = field(default_factory=lambda: np.zeros(0, SYNAPSE_DT))
+    vclamps: np.ndarray = field(default_factory=lambda: np.zeros(0, VCLAMP_DT))
 
     @property
     def n(self):

@@ -503,6 +503,7 @@ int place_cell(hsg_engine* e, HostCell& hc) {
         {&d.o2dExt, nX == 0 ? 0 : 16 * (int64_t)n},
         // PulseGens
         {&d.oPulse, (int64_t)sizeof(PulseDev) * (int64_t)s.pulses.size()},
+        {&d.oVClamp, (int64_t)sizeof(VClampDev) * (int64_t)s.vclamps.size()},
         // Leakage sends and SynChans
         {&d.oXFlag, nX == 0 ? 0 : 4 * (int64_t)n},
         {&d.oLeakStart, s.leaks.empty() ? 0 : 4 * ((int64_t)n + 1)},
@@ -653,6 +654,18 @@ int place_cell(hsg_engine* e, HostCell& hc) {
         }
         st.put(d.oPulse, pd);
     }
+    if (!s.vclamps.empty()) {
+        std::vector<VClampDev> vd(s.vclamps.size());
+        for (size_t k = 0; k < s.vclamps.size(); ++k) {
+            VClampDev& q = vd[k];
+            std::memset(&q, 0, sizeof q);
+            q.compt = s.vclamps[k].compt;
+            q.entry = s.vclamps[k].entry;
+            q.mode = s.vclamps[k].mode;
+            q.Kp = s.vclamps[k].gain;
+        }
+        st.put(d.oVClamp, vd);
+    }
     if (nX > 0) {
         std::vector<int32_t> flag(n, 0);
         for (int i = 0; i < n; ++i)
@@ -677,6 +690,7 @@ int place_cell(hsg_engine* e, HostCell& hc) {
     d.msg = msgOn ? 1 : 0;
     d.n2d = (int32_t)s.ch2d.size();
     d.nPulse = (int32_t)s.pulses.size();
+    d.nVClamp = (int32_t)s.vclamps.size();
     d.values2dSent = 0;
     d.nSlots = s.nSlots;
     d.rootSlot = s.rootSlot;
@@ -1349,6 +1363,32 @@ int reinit_cells_host(hsg_engine* e, HostCell& hc) {
                 hc.injDirty = true;
             }
     }
+    // VClamp::reinit (biophysics/VClamp.cpp:313-347, clock tick 0): the
+    // constants at the clock's dt, the command filter and error history
+    // restarted at the compartment's initVm; the current carries over
+    if (!s.vclamps.empty()) {
+        std::vector<VClampDev> vd(s.vclamps.size());
+        int rc = download(e, hc.dev.oVClamp, vd.size() * sizeof(VClampDev) / sizeof(double),
+                          reinterpret_cast<double*>(vd.data()));
+        if (rc) return rc;
+        const double dt = e->dt;
+        for (size_t k = 0; k < s.vclamps.size(); ++k) {
+            HostVClamp& v = s.vclamps[k];
+            VClampDev& q = vd[k];
+            if (v.ti == 0.0) v.ti = dt;
+            if (v.td < 0.0) v.td = 0.0;
+            if (v.tau == 0.0) v.tau = 5.0 * dt;
+            q.expt = (dt / v.tau > 1e-15) ? std::exp(-dt / v.tau) : 1.0 - dt / v.tau;
+            q.tauByDt = v.tau / dt;
+            q.dtByTi = dt / v.ti;
+            q.tdByDt = v.td / dt;
+            if (v.gain == 0.0) v.gain = s.Cm[v.compt] / dt;
+            q.Kp = v.gain;
+            q.v1 = q.e1 = q.e2 = 0.0;
+            q.command = q.cmdIn = q.oldCmdIn = s.initVm[v.compt];
+        }
+        if ((rc = upload(e, hc.dev.oVClamp, vd))) return rc;
+    }
     // SynChan::vReinit at the clock's dt, handlers' queues dropped
     // (SimpleSynHandler::vReinit)
     if (!s.syn.empty()) {
@@ -1797,6 +1837,11 @@ int hsg_get_field(hsg_engine* e, int32_t cell, int32_t field, const int32_t* idx
                 return fail(HSG_E_ARG, "unknown Leakage index");
             const HostLeak& l = s.leaks[s.leakOfDesc[i]];
             out[q] = field == HSG_F_LEAK_GBAR ? l.Gbar : field == HSG_F_LEAK_EK ? l.Ek : l.mod;
+        } else if (field == HSG_F_VCLAMP_COMMAND || field == HSG_F_VCLAMP_CURRENT) {
+            if (i < 0 || i >= (int)s.vclamps.size()) return fail(HSG_E_ARG, "unknown VClamp index");
+            const int64_t rec = hc->dev.oVClamp + (int64_t)sizeof(VClampDev) * i;
+            const int64_t off = field == HSG_F_VCLAMP_COMMAND ? offsetof(VClampDev, cmdIn) : offsetof(VClampDev, current);
+            if ((rc = download(e, rec + off, 1, &out[q]))) return rc;
         } else {
             return fail(HSG_E_ARG, "unknown field");
         }
@@ -1991,6 +2036,11 @@ int hsg_set_field(hsg_engine* e, int32_t cell, int32_t field, const int32_t* idx
             HIPCHK(hipMemcpyAsync(e->arena.base + hc->dev.oLeak + (int64_t)sizeof(LeakDev) * k, &rec, sizeof rec,
                                   hipMemcpyHostToDevice, e->stream));
             HIPCHK(hipStreamSynchronize(e->stream));
+        } else if (field == HSG_F_VCLAMP_COMMAND) {
+            // VClamp::setCommand: cmdIn_
+            if (i < 0 || i >= (int)s.vclamps.size()) return fail(HSG_E_ARG, "unknown VClamp index");
+            const int64_t rec = hc->dev.oVClamp + (int64_t)sizeof(VClampDev) * i;
+            if ((rc = put1<double>(e, rec + offsetof(VClampDev, cmdIn), 0, x))) return rc;
         } else {
             return fail(HSG_E_ARG, "unknown field");
         }

@@ -73,6 +73,17 @@ struct PulseDev {
     double  delay[HSG_MAX_PULSES], width[HSG_MAX_PULSES], level[HSG_MAX_PULSES];
 };
 
+// A VClamp (biophysics/VClamp.cpp:287-347): constants set at reinit, then
+// the command filter, error history and current, per tick.
+struct VClampDev {
+    int32_t compt;              // Hines index
+    int32_t entry;              // its injection entry
+    int32_t mode;
+    int32_t pad;
+    double  Kp, expt, tauByDt, dtByTi, tdByDt;
+    double  cmdIn, oldCmdIn, command, e1, e2, v1, current;
+};
+
 // A Leakage object's per-tick send: (G, G * Ek) with G = Gbar * modulation
 // (biophysics/Leakage.cpp:90-95; HSolve::addGkEk).
 struct LeakDev {
@@ -207,6 +218,10 @@ struct CellDev {
     int64_t oActCur;        // int32[nSyn]: next activation entry of the call
     int64_t oActEnd;        // int32[nSyn]
     int64_t oAct;           // ActDev[]: the call's activations, by synchan then step
+    // VClamps
+    int32_t nVClamp;
+    int32_t pad4;
+    int64_t oVClamp;        // VClampDev[nVClamp]
 };
 
 // Elimination unit descriptor: 12 int32 (48 bytes), read as three int4 loads.

@@ -899,9 +899,36 @@ __device__ __noinline__ bool pulse_tick(PulseDev& p, double t) {
     return true;
 }
 
+// VClamp::process (biophysics/VClamp.cpp:287-311) with the sensed Vm vIn
+// (what the solver sent at the end of the previous tick): the low-pass
+// filtered command, the error and the PID current increment; returns the
+// current it sends.
+__device__ __noinline__ double vclamp_tick(VClampDev& c, double vIn) {
+    const double dCmd = c.cmdIn - c.oldCmdIn;
+    c.command = c.cmdIn + dCmd * (1 - c.tauByDt) + (c.command - c.cmdIn + dCmd * c.tauByDt) * c.expt;
+    c.oldCmdIn = c.cmdIn;
+    const double e = c.command - vIn;
+    if (c.mode == 0) {
+        c.current += c.Kp * ((1 + c.dtByTi + c.tdByDt) * e - (1 + 2 * c.tdByDt) * c.e1 + c.tdByDt * c.e2);
+        c.e2 = c.e1;
+        c.e1 = e;
+    } else if (c.mode == 1) {
+        c.current += c.Kp * ((1 + c.dtByTi) * e - c.e1 + c.tdByDt * (vIn - 2 * c.v1 + c.e2));
+        c.e2 = c.v1;
+        c.v1 = vIn;
+        c.e1 = e;
+    } else {
+        c.current += c.Kp * (vIn - c.v1 + c.dtByTi * e + c.tdByDt * (vIn - 2 * c.v1 + c.e2));
+        c.e2 = c.v1;
+        c.v1 = vIn;
+    }
+    return c.current;
+}
+
 // The varying injection of entry e for the tick ending at t: the call's host
-// additions (first tick), then what the entry's PulseGens send, in their
-// order (HSolve::addInject sums into injectVarying).
+// additions (first tick), then what the entry's PulseGens and VClamps send
+// (clock tick 0), in their order (HSolve::addInject sums into
+// injectVarying).
 __device__ __forceinline__ double inject_varying(char* arena, const CellDev& c, int e, bool first, double t) {
     double* iv = A_<double>(arena, c.oInjVar);
     double var = first ? iv[e] : 0.0;
@@ -910,6 +937,12 @@ __device__ __forceinline__ double inject_varying(char* arena, const CellDev& c, 
         PulseDev* pl = A_<PulseDev>(arena, c.oPulse);
         for (int q = 0; q < c.nPulse; ++q)
             if (pl[q].entry == e && pulse_tick(pl[q], t)) var += pl[q].out;
+    }
+    if (c.nVClamp > 0) {
+        VClampDev* vc = A_<VClampDev>(arena, c.oVClamp);
+        const double* V = A_<const double>(arena, c.oV);
+        for (int q = 0; q < c.nVClamp; ++q)
+            if (vc[q].entry == e) var += vclamp_tick(vc[q], V[vc[q].compt]);
     }
     return var;
 }
@@ -1166,7 +1199,10 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
         }
         // (split mode: the channel kernel has added them to the rows)
         if (!SPLIT && (nInj > 0 || (first && nExt > 0) || c.nXObj > 0)) {
-            work_barrier(WL);
+            // VClamps read V written by other threads in the last tick: a
+            // full barrier then (its release covers those global stores)
+            if (c.nVClamp > 0) __syncthreads();
+            else work_barrier(WL);
             const int32_t* xflag = A_<const int32_t>(arena, c.oXFlag);
             if (tid == 0) {
                 // injections: HS[4i+3] += injectVarying + injectBasal (HSolveActive.cpp:119-128)

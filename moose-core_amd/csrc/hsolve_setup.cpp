@@ -898,12 +898,35 @@ int setup_cell(const hsg_cell_desc* d, double dt, int caAdvance, CellSetup& s, s
         s.hasInject[p.compt] = 1;
         s.pulses.push_back(p);
     }
+    // VClamps likewise (their currentOut goes to injectMsg)
+    s.vclamps.clear();
+    for (int k = 0; k < (d->n_vclamps > 0 ? d->n_vclamps : 0); ++k) {
+        const hsg_vclamp_desc& vd = d->vclamps[k];
+        if (vd.compartment < 0 || vd.compartment >= nd || s.descToHines[vd.compartment] < 0) {
+            err = "VClamp on a compartment outside the cell";
+            return HSG_E_MODEL;
+        }
+        if (vd.mode < 0 || vd.mode > 2) {
+            err = "VClamp: mode must be 0, 1 or 2";
+            return HSG_E_ARG;
+        }
+        HostVClamp v{};
+        v.compt = s.descToHines[vd.compartment];
+        v.mode = vd.mode;
+        v.tau = vd.tau;
+        v.ti = vd.ti;
+        v.td = vd.td;
+        v.gain = vd.gain;
+        s.hasInject[v.compt] = 1;
+        s.vclamps.push_back(v);
+    }
     {
         std::vector<int> entryOf(n, -1);
         int m = 0;
         for (int i = 0; i < n; ++i)
             if (s.hasInject[i]) entryOf[i] = m++;
         for (auto& p : s.pulses) p.entry = entryOf[p.compt];
+        for (auto& v : s.vclamps) v.entry = entryOf[v.compt];
     }
 
     // Leakage objects (their per-tick sends) and SynChans, grouped by

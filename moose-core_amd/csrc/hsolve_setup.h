@@ -54,6 +54,14 @@ struct HostPulse {
     double delay[HSG_MAX_PULSES], width[HSG_MAX_PULSES], level[HSG_MAX_PULSES];
 };
 
+// A VClamp (biophysics/VClamp.cpp) sending to its compartment's injectMsg.
+struct HostVClamp {
+    int compt;                        // Hines index
+    int entry;                        // injection entry
+    int mode;
+    double tau, ti, td, gain;
+};
+
 // A Leakage object's per-tick send (biophysics/Leakage.cpp:90-95).
 struct HostLeak {
     int compt;                        // Hines index
@@ -144,6 +152,7 @@ struct CellSetup {
     // order within one), with the compartment -> channels index (n + 1)
     std::vector<HostChan2D> ch2d;
     std::vector<HostPulse> pulses;    // PulseGens on injectMsg, desc order
+    std::vector<HostVClamp> vclamps;  // VClamps on injectMsg, desc order
     // Leakage sends and SynChans, sorted by compartment (desc order within
     // one), with their compartment -> objects indexes (n + 1)
     std::vector<HostLeak> leaks;

@@ -613,6 +613,19 @@ class PulseGen(Neutral):
         return r
 
 
+class VClamp(Neutral):
+    """A voltage clamp (biophysics/VClamp.cpp): senses its compartment's Vm
+    (VmOut -> sensedIn), low-pass filters the command (commandIn or the
+    'command' field) and sends a PID current to the compartment's injectMsg
+    on clock tick 0.  Under an HSolve it runs on the GPU; 'command' writes
+    and 'current' reads go there."""
+    _zfields = ("command", "current")
+
+    def _init_fields(self):
+        # biophysics/VClamp.cpp:196-205
+        self.__dict__.update(mode=0, tau=0.0, ti=0.0, td=-1.0, gain=0.0, command=0.0, current=0.0)
+
+
 class Leakage(Neutral):
     """A Leakage channel (biophysics/Leakage.cpp): not taken over by the
     solver.  Its Gk field (Gbar * modulation once reinit has run) is folded
@@ -971,6 +984,18 @@ class HSolve(Neutral):
                 syns.append(sc)
         screc = np.array(synrec, _abi.SYNCHAN_DT) if synrec else np.zeros(0, _abi.SYNCHAN_DT)
         sprec = np.array(synapses, _abi.SYNAPSE_DT) if synapses else np.zeros(0, _abi.SYNAPSE_DT)
+        # VClamps sending to the compartments' injectMsg, in creation order;
+        # each senses its own compartment
+        vcs = sorted(((src, cidx[d.id]) for src, sf, d, df in _msgs
+                      if sf == "currentOut" and df == "injectMsg" and isinstance(src, VClamp)
+                      and d.id in cidx), key=lambda x: x[0].id)
+        vrec = np.zeros(len(vcs), _abi.VCLAMP_DT)
+        for k, (vc, ci) in enumerate(vcs):
+            sensed = _sources(vc, "sensedIn")
+            if [x.id for x in sensed] != [comps[ci].id]:
+                raise ValueError(f"{vc.path}: a VClamp under HSolve senses the compartment it drives")
+            d = vc.__dict__
+            vrec[k] = (ci, int(d["mode"]), d["tau"], d["ti"], d["td"], d["gain"])
         # PulseGens feeding the compartments' injectMsg, in creation order
         # (the order clock tick 0 processes them)
         pgs = sorted(((s, cidx[d.id]) for s, sf, d, df in _msgs
@@ -981,10 +1006,11 @@ class HSolve(Neutral):
             grec[k] = pg._record(ci)
         spec = CellSpec(ct, np.array(axial, dtype=np.int32).reshape(-1, 2), seed=cidx[seed.id],
                         channels=crec, gates=glist, pools=prec, spikegens=srec, leaks=lrec,
-                        tables2d=tabs, chans2d=c2rec, pulsegens=grec, synchans=screc, synapses=sprec)
+                        tables2d=tabs, chans2d=c2rec, pulsegens=grec, synchans=screc, synapses=sprec,
+                        vclamps=vrec)
         objs = {"compartments": comps, "channels": [ch for _, ch in chans], "pools": pools,
                 "spikegens": [sg for _, sg in spikes], "chans2d": [ch for _, ch in ch2],
-                "pulsegens": pgs, "synchans": syns,
+                "pulsegens": pgs, "synchans": syns, "vclamps": [vc for vc, _ in vcs],
                 "leaks": [lk for c in comps for lk in _channels(c, "Leakage")]}
         return spec, objs
 
@@ -1001,7 +1027,7 @@ class HSolve(Neutral):
         object.__setattr__(self, "_objs", objs)
         object.__setattr__(self, "_spec", spec)
         # zombify (hsolve/HSolve.cpp:235-262): fields now live in the solver
-        for kind in ("compartments", "channels", "pools", "chans2d", "synchans"):
+        for kind in ("compartments", "channels", "pools", "chans2d", "synchans", "vclamps"):
             for i, o in enumerate(objs[kind]):
                 o.__dict__["_solver"] = (self, i)
                 for f in type(o)._zfields:       # reads now reach the solver
@@ -1074,6 +1100,26 @@ class HSolve(Neutral):
             if sf == "spikeOut" and isinstance(d, Table) and s.id in spos:
                 spk.append((d, spos[s.id]))
         return vm, spk
+
+    def _field_recorders(self, dt):
+        """Tables reading another field of an object this solver runs (a
+        VClamp's current, a channel's Gk, ...): (table, object, field,
+        sampling interval in steps).  The host reads these after the ticks
+        they sample."""
+        out = []
+        for s, sf, d, df in _msgs:
+            if sf != "requestOut" or not isinstance(s, Table) or df == "getVm" or not df.startswith("get"):
+                continue
+            z = d.__dict__.get("_solver") if isinstance(d, Neutral) else None
+            if z is None or z[0] is not self:
+                continue
+            for f in (df[3:], df[3].lower() + df[4:]):
+                if f in type(d)._zfields:
+                    out.append((s, d, f, _table_every(s, dt)))
+                    break
+            else:
+                raise ValueError(f"{s.path}: cannot record {df} of {d.path} under HSolve")
+        return out
 
 
 # ---------------------------------------------------------------------------
@@ -1496,6 +1542,9 @@ def reinit():
             t._data.append(float(hs._engine.get(hs._cell, "Vm", [i])[0]))
         for t, _ in spk:
             t.clearVec()
+        for t, obj, f, _ in hs._field_recorders(float(hs.dt)):
+            t.clearVec()
+            t._data.append(float(getattr(obj, f)))
         hs.__dict__["_time"] = 0.0
         hs.__dict__["_steps"] = 0
 
@@ -1527,7 +1576,24 @@ def start(runtime):
             hs._push_pulsegens()
             hs._push_leaks()
             hs._push_synapses()
-        eng.step(nsteps)
+        # tables of other fields: stepped in stretches ending on their
+        # sampling steps, read by the host
+        ftabs = [(hs, r) for hs in group for r in hs._field_recorders(dt)]
+        if ftabs:
+            grid = 0
+            for _, (_, _, _, e) in ftabs:
+                grid = math.gcd(grid, e)
+            s0 = group[0].__dict__.get("_steps", 0)
+            done = 0
+            while done < nsteps:
+                m = min(grid - (s0 + done) % grid, nsteps - done)
+                eng.step(m)
+                done += m
+                for _, (t, obj, f, e) in ftabs:
+                    if (s0 + done) % e == 0:
+                        t._data.append(float(getattr(obj, f)))
+        else:
+            eng.step(nsteps)
         spikes = eng.read_spikes()
         for hs, vm, spk, comps, every, s0 in rec:
             if comps:

@@ -155,6 +155,16 @@ typedef struct {
     double Gbar, Ek, pw[3], st[3], mod, Gk;
 } or_chan2d;
 
+/* A VClamp (biophysics/VClamp.cpp) sensing a compartment's Vm and sending
+ * its current to the compartment's injectMsg. */
+typedef struct {
+    int    ic;                        /* Hines index */
+    int    mode;
+    double tau, ti, td, Kp;           /* fields (Kp = gain) */
+    double expt, tauByDt, dtByTi, tdByDt;
+    double vIn, v1, cmdIn, oldCmdIn, command, e, e1, e2, current;
+} or_vclamp;
+
 /* doubleEq (basecode/doubleEq.cpp:16-22) */
 static int double_eq(double x, double y) {
     double denom = fabs(x) + fabs(y);
@@ -263,6 +273,9 @@ struct or_solver {
     int         nSyn, nSynapse;
     or_synchan* syn;
     double     *synW, *synDelay;      /* synapses, desc order */
+    /* VClamps on the compartments, desc order */
+    int         nVc;
+    or_vclamp*  vc;
 };
 
 /* ------------------------------------------------------------------------ */
@@ -1212,6 +1225,26 @@ int or_create(const hsg_cell_desc* d, double dt, int caAdvance, or_solver** out)
         c->mod = sd->modulation;
         synchan_constants(c, 25.0e-6);   /* SynChan's dt_ before any reinit */
     }
+    /* VClamps: an injection entry on their compartment, as addInject
+     * creates one */
+    s->nVc = d->n_vclamps > 0 ? d->n_vclamps : 0;
+    s->vc = (or_vclamp*)calloc(s->nVc + 1, sizeof(or_vclamp));
+    for (int k = 0; k < s->nVc; ++k) {
+        const hsg_vclamp_desc* vd = &d->vclamps[k];
+        if (vd->compartment < 0 || vd->compartment >= d->n_compartments || s->descToHines[vd->compartment] < 0 ||
+            vd->mode < 0 || vd->mode > 2) {
+            or_destroy(s);
+            return HSG_E_ARG;
+        }
+        or_vclamp* c = &s->vc[k];
+        c->ic = s->descToHines[vd->compartment];
+        c->mode = vd->mode;
+        c->tau = vd->tau;
+        c->ti = vd->ti;
+        c->td = vd->td;
+        c->Kp = vd->gain;
+        s->h.hasInject[c->ic] = 1;
+    }
     s->synW = (double*)calloc(s->nSynapse + 1, sizeof(double));
     s->synDelay = (double*)calloc(s->nSynapse + 1, sizeof(double));
     for (int j = 0; j < s->nSynapse; ++j) {
@@ -1220,6 +1253,29 @@ int or_create(const hsg_cell_desc* d, double dt, int caAdvance, or_solver** out)
     }
     *out = s;
     return 0;
+}
+
+/* VClamp::process (biophysics/VClamp.cpp:287-311): the low-pass filtered
+ * command, the error against the sensed Vm and the PID current increment */
+static void vclamp_process(or_vclamp* c) {
+    double dCmd = c->cmdIn - c->oldCmdIn;
+    c->command = c->cmdIn + dCmd * (1 - c->tauByDt) + (c->command - c->cmdIn + dCmd * c->tauByDt) * c->expt;
+    c->oldCmdIn = c->cmdIn;
+    c->e = c->command - c->vIn;
+    if (c->mode == 0) {
+        c->current += c->Kp * ((1 + c->dtByTi + c->tdByDt) * c->e - (1 + 2 * c->tdByDt) * c->e1 + c->tdByDt * c->e2);
+        c->e2 = c->e1;
+        c->e1 = c->e;
+    } else if (c->mode == 1) {
+        c->current += c->Kp * ((1 + c->dtByTi) * c->e - c->e1 + c->tdByDt * (c->vIn - 2 * c->v1 + c->e2));
+        c->e2 = c->v1;
+        c->v1 = c->vIn;
+        c->e1 = c->e;
+    } else {
+        c->current += c->Kp * (c->vIn - c->v1 + c->dtByTi * c->e + c->tdByDt * (c->vIn - 2 * c->v1 + c->e2));
+        c->e2 = c->v1;
+        c->v1 = c->vIn;
+    }
 }
 
 /* PulseGen::process in FREE_RUN mode (device/PulseGen.cpp:359-424): the
@@ -1268,7 +1324,7 @@ void or_destroy(or_solver* s) {
         for (int k = 0; k < s->nTab2d; ++k) free((void*)s->tab2d[k].t);
     free(s->tab2d); free(s->ch2d); free(s->pulse); free(s->leak);
     for (int k = 0; k < s->nSyn; ++k) free(s->syn[k].ev);
-    free(s->syn); free(s->synW); free(s->synDelay);
+    free(s->syn); free(s->synW); free(s->synDelay); free(s->vc);
     free(s);
 }
 
@@ -1323,6 +1379,24 @@ void or_reinit(or_solver* s, double dt) {
     for (int k = 0; k < s->nPulse; ++k) {
         s->pulse[k].out = s->pulse[k].base;
         h->injectVarying[s->pulse[k].ic] += s->pulse[k].base;
+    }
+    /* VClamp::reinit (VClamp.cpp:313-347, tick 0): the error terms and
+     * the command filter restart at the compartment's initVm; the current
+     * is kept */
+    for (int k = 0; k < s->nVc; ++k) {
+        or_vclamp* c = &s->vc[k];
+        c->vIn = 0.0;
+        c->v1 = 0.0;
+        c->command = c->cmdIn = c->oldCmdIn = c->e = c->e1 = c->e2 = 0.0;
+        if (c->ti == 0.0) c->ti = dt;
+        if (c->td < 0.0) c->td = 0.0;
+        if (c->tau == 0.0) c->tau = 5.0 * dt;
+        c->expt = (dt / c->tau > 1e-15) ? exp(-dt / c->tau) : 1.0 - dt / c->tau;
+        c->tauByDt = c->tau / dt;
+        c->dtByTi = dt / c->ti;
+        c->tdByDt = c->td / dt;
+        if (c->Kp == 0.0) c->Kp = h->Cm[c->ic] / dt;
+        c->command = c->cmdIn = c->oldCmdIn = h->initVm[c->ic];
     }
     chan2d_reinit(s);            /* their reinit runs before the solver's */
     /* SimpleSynHandler::vReinit drops the queue; SynChan::vReinit
@@ -1621,6 +1695,14 @@ void or_step(or_solver* s, double dt, double currTime) {
      * solver (tick 6) */
     for (int k = 0; k < s->nPulse; ++k)
         if (pulse_process(&s->pulse[k], currTime)) h->injectVarying[s->pulse[k].ic] += s->pulse[k].out;
+    /* VClamps (tick 0, after the PulseGens), from the Vm the solver sent at
+     * the end of the previous tick */
+    for (int k = 0; k < s->nVc; ++k) {
+        or_vclamp* c = &s->vc[k];
+        c->vIn = h->V[c->ic];
+        vclamp_process(c);
+        h->injectVarying[c->ic] += c->current;
+    }
     /* SimpleSynHandler::vProcess (tick 1, SimpleSynHandler.cpp:104-112) */
     for (int k = 0; k < s->nSyn; ++k) {
         or_synchan* c = &s->syn[k];
@@ -1740,6 +1822,13 @@ int or_get_field(const or_solver* s, int field, const int* idx, int n, double* o
             case HSG_F_INSTANT: out[k] = (double)s->instant[c]; break;
             default: return HSG_E_ARG;
             }
+        } else if (field >= 100) {
+            if (i < 0 || i >= s->nVc) return HSG_E_ARG;
+            switch (field) {
+            case HSG_F_VCLAMP_COMMAND: out[k] = s->vc[i].cmdIn; break;
+            case HSG_F_VCLAMP_CURRENT: out[k] = s->vc[i].current; break;
+            default: return HSG_E_ARG;
+            }
         } else if (field >= 95) {
             if (i < 0 || i >= s->nLeak) return HSG_E_ARG;
             const or_leak* l = &s->leak[i];
@@ -1828,6 +1917,12 @@ int or_set_field(or_solver* s, int field, const int* idx, int n, const double* v
             case HSG_F_Z: if (s->Zpower[c] != 0.0) s->state[state_index(s, c, 2)] = x; break;
             case HSG_F_MODULATION: if (x > 0.0) s->modulation[c] = x; break;
             case HSG_F_INSTANT: s->instant[c] = (int)x; break;
+            default: return HSG_E_ARG;
+            }
+        } else if (field >= 100) {
+            if (i < 0 || i >= s->nVc) return HSG_E_ARG;
+            switch (field) {
+            case HSG_F_VCLAMP_COMMAND: s->vc[i].cmdIn = x; break;   /* VClamp::setCommand */
             default: return HSG_E_ARG;
             }
         } else if (field >= 95) {

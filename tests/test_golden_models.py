@@ -1,11 +1,14 @@
 """End-to-end golden vectors from the reference's own test suite.
 
-Five rdesigneur tests of moose-core hold numbers this path must reproduce:
+Six rdesigneur tests of moose-core hold numbers this path must reproduce:
 
 * tests/rdesigneur/test_1_minimalModel.py: the fields of rdesigneur's
   default soma (makePassiveHHsoma);
 * tests/rdesigneur/test_20_currentPulse.py: that soma's Vm under a 20 nA
   pulse, as mean, std and a 10-bin histogram of the plot;
+* tests/rdesigneur/test_21_vclamp.py: that soma under rdesigneur's voltage
+  clamp (a VClamp, biophysics/VClamp.cpp), means and stds of its Vm and of
+  the clamp current;
 * tests/rdesigneur/test_32_squid_axon_propgn.py: a 200-segment squid axon
   with a propagating spike, 41 soma Vm samples;
 * tests/rdesigneur/test_41_ballAndStick.py: a ball-and-stick HH cell, mean
@@ -220,12 +223,27 @@ def periodic_spikes(rate, nsteps, refractT=5e-3):
     return out
 
 
+def build_vclamp_soma(moose):
+    """The default passive soma with rdesigneur's voltage clamp on it
+    (make_vclamp, rdesigneurProtos.py:699-715: mode 0, tau 0.2 ms, ti 20 us,
+    td 5 us; _buildVclampOnCompt, rdesigneur.py:568-580: gain = Cm * 1e4,
+    VmOut -> sensedIn, currentOut -> injectMsg)."""
+    comps = build_passive_soma(moose)
+    vc = moose.VClamp("/model/elec/soma/vclamp")
+    vc.mode, vc.tau, vc.ti, vc.td = 0, 0.2e-3, 20e-6, 5e-6
+    vc.gain = comps[0].Cm * 1e4
+    moose.connect(comps[0], "VmOut", vc, "sensedIn")
+    moose.connect(vc, "currentOut", comps[0], "injectMsg")
+    return comps
+
+
 MODELS = {
     # name: (builder, stimulus, steps)
     "current_pulse": (build_passive_soma, lambda t: float(t > 0.1 and t < 0.2) * 2e-8, 6000),
     "ball_and_stick": (build_ball_and_stick, lambda t: float(t > 0.01 and t < 0.05) * 1e-9, 1200),
     "squid_axon": (build_squid_axon, lambda t: float(t > 0.005 and t < 0.2) * 2e-11, 800),
     "periodic_syn": (build_periodic_syn, None, 6000),
+    "vclamp": (build_vclamp_soma, lambda t: -0.065 + float(t > 0.1 and t < 0.2) * 0.02, 6000),
 }
 
 
@@ -233,6 +251,8 @@ def events(name):
     """(step, kind, value) after which the stimulus changes: 'inject' (the
     Function's output) or 'spike' (a synaptic event at t = dt * step)."""
     build, expr, nsteps = MODELS[name]
+    if name == "vclamp":
+        return [(s, "command", v) for s, v in stimulus_schedule(expr, nsteps)]
     if expr is not None:
         return [(s, "inject", v) for s, v in stimulus_schedule(expr, nsteps)]
     # queued before the tick that fires it, which pops it
@@ -246,6 +266,12 @@ def check(name, v, mixed=False):
         ref = np.array(golden("squid_axon_propagation.json")["Vm"])
         dv = np.abs(v[::10] - ref).max()
         return (dv <= 1e-4) if mixed else np.allclose(v[::10], ref), dv
+    if name == "vclamp":
+        g = golden("vclamp_soma.json")
+        got = [np.mean(v, axis=1).tolist(), np.std(v, axis=1).tolist()]
+        if mixed:
+            return np.allclose(got, [g["mean"], g["std"]], rtol=1e-3), got
+        return np.allclose(got[0], g["mean"]) and np.allclose(got[1], g["std"]), got
     if name == "periodic_syn":
         g = golden("periodic_syn_input.json")
         got = [np.mean(v), np.std(v)]
@@ -285,6 +311,24 @@ def test_reference_model_oracle(pkg, oracle_mod, name):
     soma = objs["compartments"].index(comps[0])
     o = oracle_mod.OracleCell(spec, DT)
     o.reinit(DT)
+    if name == "vclamp":
+        # the command Function's reinit value, then Vm and the clamp
+        # current on the plot clock
+        o.set("command", [0], [expr(0.0)])
+        changes = dict((s, val) for s, _, val in events(name))
+        vm, cur = [o.get("Vm", [soma])[0]], [o.get("current", [0])[0]]
+        for step in range(1, nsteps + 1):
+            o.run(DT, 1)
+            if step % 2 == 0:
+                vm.append(o.get("Vm", [soma])[0])
+                cur.append(o.get("current", [0])[0])
+            if step in changes:
+                o.set("command", [0], [changes[step]])
+        ok, got = check(name, np.array([vm, cur]))
+        print(name, got)
+        assert ok, got
+        moose.reset()
+        return
     v = [o.get("Vm", [soma])[0]]
     done = 0
     trace = []
@@ -319,21 +363,29 @@ def test_reference_model_gpu(pkg, name, precision):
         moose.Neutral("/model/graphs")
         tab = moose.Table("/model/graphs/plot0")
         moose.connect(tab, "requestOut", comps[0], "getVm")
+        tabs = [tab]
+        if name == "vclamp":
+            tabs.append(moose.Table("/model/graphs/plot1"))
+            moose.connect(tabs[1], "requestOut", moose.element("/model/elec/soma/vclamp"), "getCurrent")
         moose.setClock(8, 0.1e-3)
         hs = moose.HSolve("/model/elec/hsolve")
         hs.dt = DT
         hs.target = "/model/elec/soma"
         moose.reinit()
+        if name == "vclamp":
+            moose.element("/model/elec/soma/vclamp").command = expr(0.0)   # the Function's reinit
         done = 0
         for s, kind, val in events(name) + [(nsteps, None, None)]:
             moose.start((s - done) * DT)
             done = s
             if kind == "inject":
                 comps[0].inject = val
+            elif kind == "command":
+                moose.element("/model/elec/soma/vclamp").command = val
             elif kind == "spike":
                 moose.element("/model/elec/soma/glu/sh").synapse[0].addSpike(val)
-        v = tab.vector
-        assert len(v) == nsteps // 2 + 1
+        v = tab.vector if len(tabs) == 1 else np.array([t.vector for t in tabs])
+        assert v.shape[-1] == nsteps // 2 + 1
         ok, got = check(name, v, mixed=precision == "mixed")
         print(name, precision, got, flush=True)
         assert ok, got

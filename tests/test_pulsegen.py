@@ -339,3 +339,81 @@ def test_gpu_layer_pulsegen(pkg, oracle_mod):
     assert worst > 1e-9
     assert np.abs(tab.vector[1:] - np.array(out)).max() <= FP64_TOL
     moose.reset()
+
+
+# ---------------------------------------------------------------------------
+# VClamp (biophysics/VClamp.cpp), also on injectMsg at clock tick 0
+# ---------------------------------------------------------------------------
+def vclamps(recs):
+    from moose_core_amd._abi import VCLAMP_DT
+    out = np.zeros(len(recs), VCLAMP_DT)
+    for k, (comp, mode, tau, ti, td, gain) in enumerate(recs):
+        out[k] = (comp, mode, tau, ti, td, gain)
+    return out
+
+
+def test_vclamp_holds_the_command(oracle_mod):
+    """A clamp (rdesigneur's settings: tau 0.2 ms, ti 20 us, td 5 us, gain
+    Cm * 1e4) settles its passive compartment at the command in modes 0
+    and 1, and at the steady state the current it sends is what the whole
+    passive cell's membranes pass (axial currents cancel in the sum):
+    current = -sum(Im)."""
+    dt = 25e-6
+    for mode in (0, 1):
+        spec = workloads.random_hh_cell(np.random.default_rng(12), 15, inject=0.0)
+        spec.channels = spec.channels[:0]
+        spec.spikegens = spec.spikegens[:0]
+        spec.vclamps = vclamps([(0, mode, 0.2e-3, 20e-6, 5e-6, spec.compartments["Cm"][0] * 1e4)])
+        o = oracle_mod.OracleCell(spec, dt)
+        o.reinit(dt)
+        o.set("command", [0], [-0.040])
+        o.run(dt, 4000)
+        assert abs(o.get("Vm", [0])[0] - (-0.040)) < 1e-9, mode
+        assert o.get("command", [0])[0] == -0.040
+        i = o.get("current", [0])[0]
+        assert i > 0 and abs(o.get("Im", np.arange(spec.n)).sum() + i) < 1e-6 * i
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["fused", "split"])
+def test_gpu_vclamp_batch(pkg, oracle_mod, mode):
+    """VClamps in modes 0, 1 and 2 (each of the first three sharing its
+    compartment with a PulseGen) on HH cells, the command stepped between
+    runs: FP64 traces within 1e-6 V of the oracle, equal currents."""
+    print(f"case: VClamp batch (3 cells, modes 0/1/2, with a PulseGen), 600 steps, fp64, {mode}", flush=True)
+    rng = np.random.default_rng(31)
+    specs = []
+    for n, vm in ((25, 0), (80, 1), (190, 0)):
+        s = workloads.random_hh_cell(rng, n, inject=0.0)
+        c = int(n // 3)
+        s.vclamps = vclamps([(c, vm, 0.2e-3, 20e-6, 5e-6, s.compartments["Cm"][c] * 1e4)])
+        s.pulsegens = np.array([train(c, **TRAINS[0])], PULSEGEN_DT)
+        specs.append(s)
+    # mode 2 (proportional and derivative on Vm) with a gain that keeps it
+    # bounded over the run
+    s = workloads.random_hh_cell(rng, 40, inject=0.0)
+    s.vclamps = vclamps([(0, 2, 0.2e-3, 20e-6, 5e-6, s.compartments["Cm"][0] * 1e2)])
+    specs.append(s)
+    dt = 25e-6
+    eng = pkg.Engine("fp64", kernel_mode=mode)
+    ids = [eng.add_cell(s, dt) for s in specs]
+    for cid, s in zip(ids, specs):
+        eng.set_probes(cid, np.arange(s.n), 600)
+    eng.reinit(dt)
+    refs = [oracle_mod.OracleCell(s, dt) for s in specs]
+    for r in refs:
+        r.reinit(dt)
+    for cmd, steps in ((-0.060, 200), (-0.030, 250), (-0.065, 150)):
+        for cid, r in zip(ids, refs):
+            eng.set(cid, "command", [0], [cmd])
+            r.set("command", [0], [cmd])
+        eng.step(steps)
+        for r in refs:
+            r.run(dt, steps)
+    spikes = eng.read_spikes()
+    for cid, s, r in zip(ids, specs, refs):
+        v = eng.get(cid, "Vm", np.arange(s.n))
+        assert np.abs(v - r.get("Vm", np.arange(s.n))).max() <= FP64_TOL
+        np.testing.assert_allclose(eng.get(cid, "current", [0]), r.get("current", [0]), rtol=1e-6, atol=1e-18)
+        assert eng.get(cid, "command", [0])[0] == -0.065
+    eng.close()
