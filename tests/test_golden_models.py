@@ -1,6 +1,6 @@
 """End-to-end golden vectors from the reference's own test suite.
 
-Six rdesigneur tests of moose-core hold numbers this path must reproduce:
+Seven rdesigneur tests of moose-core hold numbers this path must reproduce:
 
 * tests/rdesigneur/test_1_minimalModel.py: the fields of rdesigneur's
   default soma (makePassiveHHsoma);
@@ -9,6 +9,9 @@ Six rdesigneur tests of moose-core hold numbers this path must reproduce:
 * tests/rdesigneur/test_21_vclamp.py: that soma under rdesigneur's voltage
   clamp (a VClamp, biophysics/VClamp.cpp), means and stds of its Vm and of
   the clamp current;
+* tests/rdesigneur/test_30_squid_currentPulse.py: that soma with the squid
+  channels under a 10 nA pulse, the doctest's first and last samples and
+  the plot's mean and std;
 * tests/rdesigneur/test_32_squid_axon_propgn.py: a 200-segment squid axon
   with a propagating spike, 41 soma Vm samples;
 * tests/rdesigneur/test_41_ballAndStick.py: a ball-and-stick HH cell, mean
@@ -132,6 +135,16 @@ def build_passive_soma(moose):
     return [soma]
 
 
+def build_squid_soma(moose):
+    """The default passive soma with the squid channels on it
+    (test_30: chanDistrib Na 1200, K 360 on 'soma')."""
+    comps = build_passive_soma(moose)
+    protos = squid_protos(moose)
+    distribute(moose, protos["Na"], comps, 1200.0)
+    distribute(moose, protos["K"], comps, 360.0)
+    return comps
+
+
 def build_ball_and_stick(moose):
     moose.reset()
     protos = squid_protos(moose)
@@ -244,6 +257,7 @@ MODELS = {
     "squid_axon": (build_squid_axon, lambda t: float(t > 0.005 and t < 0.2) * 2e-11, 800),
     "periodic_syn": (build_periodic_syn, None, 6000),
     "vclamp": (build_vclamp_soma, lambda t: -0.065 + float(t > 0.1 and t < 0.2) * 0.02, 6000),
+    "squid_pulse": (build_squid_soma, lambda t: float(t > 0.1 and t < 0.2) * 1e-8, 6000),
 }
 
 
@@ -272,6 +286,13 @@ def check(name, v, mixed=False):
         if mixed:
             return np.allclose(got, [g["mean"], g["std"]], rtol=1e-3), got
         return np.allclose(got[0], g["mean"]) and np.allclose(got[1], g["std"]), got
+    if name == "squid_pulse":
+        g = golden("squid_current_pulse.json")
+        got = [np.mean(v), np.std(v), v[:3].tolist(), v[-3:].tolist()]
+        rt = 1e-3 if mixed else 1e-5
+        ok = (np.allclose(got[:2], [g["mean"], g["std"]], rtol=rt)
+              and np.allclose(v[:3], g["first"], rtol=rt) and np.allclose(v[-3:], g["last"], rtol=rt))
+        return ok, got
     if name == "periodic_syn":
         g = golden("periodic_syn_input.json")
         got = [np.mean(v), np.std(v)]
