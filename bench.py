@@ -270,7 +270,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="ensemble", choices=sorted(WORKLOADS) + ["serving"],
                     help="ensemble: the headline batched-ensemble line; large_ca: the large Ca cells "
-                         "with 2-D K-AHP channels; serving: the request loop (p50/p99 latency)")
+                         "with 2-D K-AHP channels; hh_2048: uniform 2048-compartment HH cells; network: "
+                         "4096 HH cells joined by SpikeGen -> SynChan synapses; serving: the request loop "
+                         "(p50/p99 latency)")
     ap.add_argument("--rate", type=float, default=20.0, help="serving: request arrivals per second")
     ap.add_argument("--quantum", type=int, default=100, help="serving: steps per batch quantum")
     ap.add_argument("--mem-budget", type=int, default=64, help="serving: device memory budget, GiB")
