@@ -35,6 +35,9 @@ extern "C" hipError_t hsg_launch_external(const hsg::LaunchArgs* a, const int2* 
 extern "C" hipError_t hsg_launch_channel_fast(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, int fp32,
                                              int ms, hipStream_t stream);
 extern "C" hipError_t hsg_launch_chan2d_reinit(const hsg::LaunchArgs* a, int nblocks, hipStream_t stream);
+extern "C" hipError_t hsg_launch_update(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, int fp32,
+                                       hipStream_t stream);
+extern "C" hipError_t hsg_launch_outputs(const hsg::LaunchArgs* a, int nblocks, hipStream_t stream);
 extern "C" hipError_t hsg_launch_msg(const hsg::LaunchArgs* a, int nblocks, int threads, int fp32,
                                     hipStream_t stream);
 
@@ -1245,8 +1248,13 @@ int launch_round(hsg_engine* e, std::vector<hsg_engine::Bin>& bins, int chunk, i
                     HIPCHK(hsg_launch_channel_fast(&a, bins[b].dTiles, bins[b].nTiles, e->fp32, bins[b].chanMS, st));
                 else
                     HIPCHK(hsg_launch_channel(&a, bins[b].dTiles, bins[b].nTiles, e->fp32, st));
+                // the solve stops at VMid; V, Ca and the tick's outputs
+                // follow over the batch
+                a.deferD = 1;
                 if (bins[b].msg) HIPCHK(hsg_launch_msg(&a, nb, bins[b].threads, e->fp32, st));
                 else HIPCHK(hsg_launch_solve(&a, nb, bins[b].threads, e->fp32, bins[b].slow, st));
+                HIPCHK(hsg_launch_update(&a, bins[b].dTiles, bins[b].nTiles, e->fp32, st));
+                HIPCHK(hsg_launch_outputs(&a, nb, st));
             }
         } else {
             LaunchArgs a = make_args(e, bins[b].dOrder, chunk, first);

@@ -283,6 +283,9 @@ struct LaunchArgs {
     int32_t        firstOfCall; // 1: apply the call's one-shot inputs at step 0
     int32_t        ldsBytes;    // dynamic LDS per workgroup
     double         dt;          // clock tick dt
+    int32_t        deferD;      // split mode: the solve kernels stop at VMid; the
+                                // update and outputs kernels finish the tick
+    int32_t        pad;
 };
 
 }  // namespace hsg

@@ -1295,19 +1295,25 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
             }
         }
         // ---- D: Vm, calcium
-        if (v.hasPools)
-            for (int i = tid; i < n; i += nt) {
-                const double vmid = w.B[i];
-                const double vnew = 2 * vmid - v.V[i];
-                v.V[i] = vnew;
-                calcium_row<GT>(arena, c, v, types, i, vmid, vnew, first);
+        if (SPLIT && a.deferD) {
+            // the update kernel does D over the whole batch: VMid to HBM
+            if constexpr (WL)
+                for (int i = tid; i < n; i += nt) gwork[n + i] = w.B[i];
+        } else {
+            if (v.hasPools)
+                for (int i = tid; i < n; i += nt) {
+                    const double vmid = w.B[i];
+                    const double vnew = 2 * vmid - v.V[i];
+                    v.V[i] = vnew;
+                    calcium_row<GT>(arena, c, v, types, i, vmid, vnew, first);
+                }
+            else
+                for (int i = tid; i < n; i += nt) v.V[i] = 2 * w.B[i] - v.V[i];
+            const int64_t stepNo = step0 + k + 1;
+            if (nSpike > 0 || nProbe > 0) {
+                __syncthreads();   // V written by other threads
+                tick_outputs(a, c, v.V, stepNo);
             }
-        else
-            for (int i = tid; i < n; i += nt) v.V[i] = 2 * w.B[i] - v.V[i];
-        const int64_t stepNo = step0 + k + 1;
-        if (nSpike > 0 || nProbe > 0) {
-            __syncthreads();   // V written by other threads
-            tick_outputs(a, c, v.V, stepNo);
         }
         // the next step's phase A overwrites D/B that other threads may still
         // read; V and the gate states are touched only by their owner thread
@@ -1688,7 +1694,9 @@ __global__ void __launch_bounds__(256) hsolve_msg_kernel(LaunchArgs a) {
         // VMid was written by other threads
         __syncthreads();
         const bool first = a.firstOfCall && k == 0;
-        if (c.hasPools) {
+        if (a.deferD) {
+            // the update kernel does D over the whole batch
+        } else if (c.hasPools) {
             const View<GT> v = make_view<GT>(arena, c, a.tsets);
             const ChanType* types = stage_types(c, a.types, sTypes);
             __syncthreads();
@@ -1702,13 +1710,48 @@ __global__ void __launch_bounds__(256) hsolve_msg_kernel(LaunchArgs a) {
             for (int i = tid; i < n; i += nt) V[i] = 2 * VMid[i] - V[i];
         }
         const int64_t stepNo = step0 + k + 1;
-        if (c.nSpike > 0 || c.nProbe > 0) {
+        if (!a.deferD && (c.nSpike > 0 || c.nProbe > 0)) {
             __syncthreads();   // V written by other threads
             tick_outputs(a, c, V, stepNo);
         }
         __syncthreads();
     }
     if (tid == 0) *stepCounter = step0 + a.nsteps;
+}
+
+// Phase D of a split tick as a kernel of its own (the solve kernels left
+// VMid in HBM): V = 2 VMid - V (HSolvePassive::backwardSubstitute's last
+// step, hsolve/HSolvePassive.cpp:395-400) and advanceCalcium
+// (HSolveActive.cpp:153-216) for one compartment per thread over the bin's
+// 256-compartment tiles, so that a large cell's sweep is spread over the
+// whole GPU rather than one workgroup.
+template <typename GT>
+__global__ void __launch_bounds__(256) hsolve_update_kernel(LaunchArgs a, const int2* __restrict__ tiles) {
+    __shared__ ChanType sTypes[kMaxSlots];
+    const int2 tile = tiles[blockIdx.x];
+    const CellDev& c = a.cells[tile.x];
+    const ChanType* types = stage_types(c, a.types, sTypes);
+    __syncthreads();
+    const int i = tile.y + threadIdx.x;
+    if (i >= c.n) return;
+    const double* VMid = c.msg ? A_<const double>(a.arena, c.oVMid) : A_<const double>(a.arena, c.oWork) + c.n;
+    double* V = A_<double>(a.arena, c.oV);
+    const double vmid = VMid[i];
+    const double vnew = 2 * vmid - V[i];
+    V[i] = vnew;
+    if (c.hasPools) {
+        const View<GT> v = make_view<GT>(a.arena, c, a.tsets);
+        calcium_row<GT>(a.arena, c, v, types, i, vmid, vnew, a.firstOfCall != 0);
+    }
+}
+
+// The tick's outputs (spike generators, probes) once the update kernel has
+// written V; the solve kernel has advanced the step counter to this tick.
+__global__ void __launch_bounds__(256) hsolve_outputs_kernel(LaunchArgs a) {
+    const CellDev& c = a.cells[a.order[blockIdx.x]];
+    if (c.nSpike == 0 && c.nProbe == 0) return;
+    const int64_t stepNo = *A_<const int64_t>(a.arena, c.oSteps);
+    tick_outputs(a, c, A_<const double>(a.arena, c.oV), stepNo);
 }
 
 // HSolveActive::reinitChannels (hsolve/HSolveActiveSetup.cpp:79-160): gates at
@@ -1850,6 +1893,22 @@ extern "C" hipError_t hsg_launch_external(const hsg::LaunchArgs* a, const int2* 
 
 extern "C" hipError_t hsg_launch_chan2d_reinit(const hsg::LaunchArgs* a, int nblocks, hipStream_t stream) {
     hipLaunchKernelGGL(hsg::hsolve_chan2d_reinit_kernel, dim3(nblocks), dim3(256), 0, stream, *a);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t hsg_launch_update(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, int fp32,
+                                       hipStream_t stream) {
+    if (fp32)
+        hipLaunchKernelGGL(hsg::hsolve_update_kernel<float>, dim3(ntiles), dim3(hsg::kChannelTile), 0, stream, *a,
+                           tiles);
+    else
+        hipLaunchKernelGGL(hsg::hsolve_update_kernel<double>, dim3(ntiles), dim3(hsg::kChannelTile), 0, stream, *a,
+                           tiles);
+    return hipGetLastError();
+}
+
+extern "C" hipError_t hsg_launch_outputs(const hsg::LaunchArgs* a, int nblocks, hipStream_t stream) {
+    hipLaunchKernelGGL(hsg::hsolve_outputs_kernel, dim3(nblocks), dim3(256), 0, stream, *a);
     return hipGetLastError();
 }
 
