@@ -402,6 +402,14 @@ int hsg_set_probes_every(hsg_engine* e, int32_t cell, const int32_t* compartment
                          int32_t n, int64_t capacity, int32_t every);
 /* Copies the recorded samples: out[k*n + j] = sample k of probe j. *n_samples
  * receives the number of samples (<= capacity).  Clears the trace. */
+/* Probes on other fields as well, sampled on the device like Vm probes
+ * (a Table on CaConc getCa, VClamp getCurrent or SynChan getGk,
+ * builtins/Table.cpp:254-285): fields[k] is HSG_F_VM (idx = compartment),
+ * HSG_F_CA (idx = pool), HSG_F_VCLAMP_CURRENT (idx = vclamp) or
+ * HSG_F_SYN_GK (idx = synchan).  Replaces the cell's probe set;
+ * hsg_read_probes returns the columns in this order. */
+int hsg_set_probe_fields(hsg_engine* e, int32_t cell, const int32_t* fields, const int32_t* idx,
+                         int32_t n, int64_t capacity, int32_t every);
 int hsg_read_probes(hsg_engine* e, int32_t cell, double* out, int64_t cap_samples,
                     int64_t* n_samples);
 

@@ -2205,44 +2205,78 @@ int hsg_set_probes(hsg_engine* e, int32_t cell, const int32_t* compartments, int
     return hsg_set_probes_every(e, cell, compartments, n, capacity, 1);
 }
 
-int hsg_set_probes_every(hsg_engine* e, int32_t cell, const int32_t* compartments, int32_t n, int64_t capacity,
-                         int32_t every) {
-    HostCell* hc = get_cell(e, cell);
-    if (!hc) return fail(HSG_E_ARG, "no such cell");
-    if (n < 0 || capacity < 0 || (n > 0 && !compartments)) return fail(HSG_E_ARG, "bad probe list");
-    if (every < 1) return fail(HSG_E_ARG, "probe interval must be at least one step");
-    hipSetDevice(e->device);
-    const CellSetup& s = hc->s;
-    std::vector<int32_t> h(n);
-    for (int k = 0; k < n; ++k) {
-        int c = compartments[k];
-        if (c < 0 || c >= (int)s.descToHines.size() || s.descToHines[c] < 0)
-            return fail(HSG_E_ARG, "unknown compartment index");
-        h[k] = s.descToHines[c];
-    }
+// A probe set: the arena offsets of the doubles sampled every `every` steps.
+static int set_probe_offsets(hsg_engine* e, HostCell* hc, const std::vector<int64_t>& offs, int64_t capacity,
+                             int32_t every) {
+    const int n = (int)offs.size();
     if (hc->probe.off >= 0) { e->arena.release(hc->probe.off, hc->probe.bytes); hc->probe.off = -1; }
     hc->dev.nProbe = 0;
     hc->dev.probeCap = 0;
     hc->dev.probeStride = every;
     e->tablesDirty = true;
     if (n == 0 || capacity == 0) return 0;
-    int64_t bytes = align_up(4 * (int64_t)n) + align_up(8 * (int64_t)n * capacity) + kAlign;
+    int64_t bytes = align_up(8 * (int64_t)n) + align_up(8 * (int64_t)n * capacity) + kAlign;
     int64_t off;
     int rc = arena_alloc(e, bytes, &off);
     if (rc) return rc;
     hc->probe.off = off;
     hc->probe.bytes = bytes;
-    hc->dev.oProbeCompt = off;
-    hc->dev.oProbeTrace = off + align_up(4 * (int64_t)n);
+    hc->dev.oProbeOff = off;
+    hc->dev.oProbeTrace = off + align_up(8 * (int64_t)n);
     hc->dev.oProbeCount = hc->dev.oProbeTrace + align_up(8 * (int64_t)n * capacity);
-    if ((rc = upload(e, hc->dev.oProbeCompt, h))) return rc;
+    if ((rc = upload(e, hc->dev.oProbeOff, offs))) return rc;
     HIPCHK(hipStreamSynchronize(e->stream));
     hc->dev.nProbe = n;
     hc->dev.probeCap = capacity;
     hc->dev.probeBase = hc->stepsDone;
-    hc->probeCompt.assign(compartments, compartments + n);
+    hc->probeCompt.assign(n, 0);
     hc->probeCap = capacity;
     return 0;
+}
+
+int hsg_set_probes_every(hsg_engine* e, int32_t cell, const int32_t* compartments, int32_t n, int64_t capacity,
+                         int32_t every) {
+    if (n < 0 || (n > 0 && !compartments)) return fail(HSG_E_ARG, "bad probe list");
+    std::vector<int32_t> fields(n > 0 ? n : 0, HSG_F_VM);
+    return hsg_set_probe_fields(e, cell, fields.data(), compartments, n, capacity, every);
+}
+
+int hsg_set_probe_fields(hsg_engine* e, int32_t cell, const int32_t* fields, const int32_t* idx, int32_t n,
+                         int64_t capacity, int32_t every) {
+    HostCell* hc = get_cell(e, cell);
+    if (!hc) return fail(HSG_E_ARG, "no such cell");
+    if (n < 0 || capacity < 0 || (n > 0 && (!fields || !idx))) return fail(HSG_E_ARG, "bad probe list");
+    if (every < 1) return fail(HSG_E_ARG, "probe interval must be at least one step");
+    hipSetDevice(e->device);
+    const CellSetup& s = hc->s;
+    const CellDev& d = hc->dev;
+    std::vector<int64_t> offs(n);
+    for (int k = 0; k < n; ++k) {
+        const int i = idx[k];
+        switch (fields[k]) {
+        case HSG_F_VM:
+            if (i < 0 || i >= (int)s.descToHines.size() || s.descToHines[i] < 0)
+                return fail(HSG_E_ARG, "unknown compartment index");
+            offs[k] = d.oV + 8 * (int64_t)s.descToHines[i];
+            break;
+        case HSG_F_CA:
+            if (i < 0 || i >= (int)s.poolCompt.size() || s.poolCompt[i] < 0)
+                return fail(HSG_E_ARG, "unknown pool index");
+            offs[k] = d.oPoolCa + 8 * ((int64_t)s.poolSlot[i] * s.n + s.poolCompt[i]);
+            break;
+        case HSG_F_VCLAMP_CURRENT:
+            if (i < 0 || i >= (int)s.vclamps.size()) return fail(HSG_E_ARG, "unknown VClamp index");
+            offs[k] = d.oVClamp + (int64_t)sizeof(VClampDev) * i + (int64_t)offsetof(VClampDev, current);
+            break;
+        case HSG_F_SYN_GK:
+            if (i < 0 || i >= (int)s.synOfDesc.size()) return fail(HSG_E_ARG, "unknown SynChan index");
+            offs[k] = d.oSyn + (int64_t)sizeof(SynDev) * s.synOfDesc[i] + (int64_t)offsetof(SynDev, Gk);
+            break;
+        default:
+            return fail(HSG_E_ARG, "a probe samples Vm, Ca, a VClamp's current or a SynChan's Gk");
+        }
+    }
+    return set_probe_offsets(e, hc, offs, capacity, every);
 }
 
 int hsg_read_probes(hsg_engine* e, int32_t cell, double* out, int64_t cap_samples, int64_t* n_samples) {

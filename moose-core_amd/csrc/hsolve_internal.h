@@ -171,8 +171,9 @@ struct CellDev {
     int64_t oInjIdx, oInjBasal, oInjVar;
     // spike generators: int32 compt, int32 edge, f64 thr, f64 refr, f64 last, int32 fired
     int64_t oSpkCompt, oSpkEdge, oSpkThr, oSpkRefr, oSpkLast, oSpkFired;
-    // probes: int32 compt[nProbe], f64 trace[probeCap*nProbe], int64 count
-    int64_t oProbeCompt, oProbeTrace, oProbeCount;
+    // probes: int64 arena offset of the sampled double [nProbe], f64
+    // trace[probeCap*nProbe], int64 count
+    int64_t oProbeOff, oProbeTrace, oProbeCount;
     // per-call external currents: int32 idx + f64 Gk + f64 GkEk (nExt entries)
     int64_t oExtIdx, oExtG, oExtGE;
     int32_t nExt;

@@ -989,9 +989,9 @@ __device__ __forceinline__ void tick_outputs(const LaunchArgs& a, const CellDev&
     if (c.nProbe > 0 && stepNo % c.probeStride == 0) {
         const int64_t sample = stepNo / c.probeStride - c.probeBase / c.probeStride - 1;
         if (sample >= 0 && sample < c.probeCap) {
-            const int32_t* pcmp = A_<const int32_t>(arena, c.oProbeCompt);
+            const int64_t* poff = A_<const int64_t>(arena, c.oProbeOff);
             double* tr = A_<double>(arena, c.oProbeTrace);
-            for (int j = tid; j < c.nProbe; j += nt) tr[sample * c.nProbe + j] = V[pcmp[j]];
+            for (int j = tid; j < c.nProbe; j += nt) tr[sample * c.nProbe + j] = *A_<const double>(arena, poff[j]);
         }
     }
 }

@@ -21,7 +21,7 @@ EXPORTS = [
     "hsg_create", "hsg_destroy", "hsg_add_cell", "hsg_remove_cell", "hsg_reinit",
     "hsg_reinit_cell", "hsg_step", "hsg_step_cells", "hsg_get_field", "hsg_set_field",
     "hsg_add_inject", "hsg_add_gkek", "hsg_add_conc", "hsg_add_ica", "hsg_set_pulsegen", "hsg_add_spike", "hsg_connect_spike", "hsg_read_spikes",
-    "hsg_set_probes", "hsg_set_probes_every", "hsg_read_probes", "hsg_cell_info_get", "hsg_hines_order",
+    "hsg_set_probes", "hsg_set_probes_every", "hsg_set_probe_fields", "hsg_read_probes", "hsg_cell_info_get", "hsg_hines_order",
     "hsg_sync", "hsg_last_step_timing", "hsg_last_error",
 ]
 
@@ -66,6 +66,7 @@ def load_library(path=None):
         "hsg_read_spikes": ([vp, P(Spike), i32, P(i32), P(i64)], C.c_int),
         "hsg_set_probes": ([vp, i32, P(i32), i32, i64], C.c_int),
         "hsg_set_probes_every": ([vp, i32, P(i32), i32, i64, i32], C.c_int),
+        "hsg_set_probe_fields": ([vp, i32, P(i32), P(i32), i32, i64, i32], C.c_int),
         "hsg_read_probes": ([vp, i32, P(dbl), i64, P(i64)], C.c_int),
         "hsg_cell_info_get": ([vp, i32, P(CellInfo)], C.c_int),
         "hsg_hines_order": ([vp, i32, P(i32)], C.c_int),
@@ -233,6 +234,18 @@ class Engine:
             self._check(self.lib.hsg_set_probes_every(self.h, int(cell), cp, len(c), int(capacity), int(every)))
         self._probe_n = getattr(self, "_probe_n", {})
         self._probe_n[int(cell)] = (len(c), int(capacity))
+
+    def set_probe_fields(self, cell, fields, idx, capacity, every=1):
+        """Probes on fields ("Vm", "Ca", "current", "GkSyn") of objects
+        `idx`, sampled on the device every `every` steps; read_probes
+        returns one column per probe."""
+        f, fp = _i32([FIELDS[x] for x in fields])
+        i, ip = _i32(idx)
+        if len(f) != len(i):
+            raise ValueError("one index per field")
+        self._check(self.lib.hsg_set_probe_fields(self.h, int(cell), fp, ip, len(i), int(capacity), int(every)))
+        self._probe_n = getattr(self, "_probe_n", {})
+        self._probe_n[int(cell)] = (len(i), int(capacity))
 
     def read_probes(self, cell):
         n, cap = self._probe_n[int(cell)]

@@ -1086,13 +1086,21 @@ class HSolve(Neutral):
                 sent[k] = r
 
     # -- recording -------------------------------------------------------
+    # Table sources the device samples: getter -> (object kind, engine field)
+    _DEVICE_PROBES = {"getVm": ("compartments", "Vm"), "getCa": ("pools", "Ca"),
+                      "getCurrent": ("vclamps", "current"), "getGk": ("synchans", "GkSyn")}
+
     def _recorders(self):
-        comps = self._objs["compartments"]
-        cpos = {c.id: i for i, c in enumerate(comps)}
+        """Tables the device samples: (table, (engine field, index)); and
+        Tables of spike times: (table, spikegen index)."""
+        pos = {kind: {o.id: i for i, o in enumerate(self._objs[kind])}
+               for kind, _ in self._DEVICE_PROBES.values()}
         vm = []
         for s, sf, d, df in _msgs:
-            if sf == "requestOut" and df == "getVm" and isinstance(s, Table) and d.id in cpos:
-                vm.append((s, cpos[d.id]))
+            if sf == "requestOut" and isinstance(s, Table) and df in self._DEVICE_PROBES:
+                kind, field = self._DEVICE_PROBES[df]
+                if isinstance(d, Neutral) and d.id in pos[kind]:
+                    vm.append((s, (field, pos[kind][d.id])))
         spk = []
         sgs = self._objs["spikegens"]
         spos = {g.id: i for i, g in enumerate(sgs)}
@@ -1108,10 +1116,12 @@ class HSolve(Neutral):
         they sample."""
         out = []
         for s, sf, d, df in _msgs:
-            if sf != "requestOut" or not isinstance(s, Table) or df == "getVm" or not df.startswith("get"):
+            if sf != "requestOut" or not isinstance(s, Table) or not df.startswith("get"):
                 continue
             z = d.__dict__.get("_solver") if isinstance(d, Neutral) else None
             if z is None or z[0] is not self:
+                continue
+            if df in self._DEVICE_PROBES and d in self._objs[self._DEVICE_PROBES[df][0]]:
                 continue
             for f in (df[3:], df[3].lower() + df[4:]):
                 if f in type(d)._zfields:
@@ -1537,9 +1547,9 @@ def reinit():
             done.add(id(eng))
     for hs in _solvers:
         vm, spk = hs._recorders()
-        for t, i in vm:
+        for t, (field, i) in vm:
             t.clearVec()
-            t._data.append(float(hs._engine.get(hs._cell, "Vm", [i])[0]))
+            t._data.append(float(hs._engine.get(hs._cell, field, [i])[0]))
         for t, _ in spk:
             t.clearVec()
         for t, obj, f, _ in hs._field_recorders(float(hs.dt)):
@@ -1565,13 +1575,14 @@ def start(runtime):
         rec = []
         for hs in group:
             vm, spk = hs._recorders()
-            comps = sorted({i for _, i in vm})
+            comps = sorted({key for _, key in vm})
             # one sampling interval per cell on the device: the Tables'
             # common one, or every step with the Tables decimated here
             evs = {_table_every(t, dt) for t, _ in vm}
             every = evs.pop() if len(evs) == 1 else 1
             if comps:
-                eng.set_probes(hs._cell, comps, nsteps // every + 1, every=every)
+                eng.set_probe_fields(hs._cell, [f for f, _ in comps], [i for _, i in comps],
+                                     nsteps // every + 1, every=every)
             rec.append((hs, vm, spk, comps, every, hs.__dict__.get("_steps", 0)))
             hs._push_pulsegens()
             hs._push_leaks()

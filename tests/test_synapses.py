@@ -546,3 +546,47 @@ def test_gpu_step_cells_with_synapses(pkg, oracle_mod):
     with pytest.raises(pkg.HSolveError):
         eng.step_cells(ids, [1, 1, 1])
     eng.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["fused", "split"])
+def test_gpu_probes_of_other_fields(pkg, oracle_mod, mode):
+    """Device probes on a CaConc's Ca, a SynChan's Gk and a VClamp's
+    current beside Vm, every third step: equal to the oracle's values read
+    after the same steps."""
+    print(f"case: probes of Vm/Ca/GkSyn/current on a Ca cell, 300 steps every 3, fp64, {mode}", flush=True)
+    from moose_core_amd._abi import VCLAMP_DT
+    rng = np.random.default_rng(77)
+    spec = workloads.random_ca_cell(rng, 70, inject=0.4e-9)
+    extra = syn_hh_cell(rng, 70)
+    spec.synchans, spec.synapses = extra.synchans, extra.synapses
+    vc = np.zeros(1, VCLAMP_DT)
+    vc[0] = (5, 0, 0.2e-3, 20e-6, 5e-6, spec.compartments["Cm"][5] * 1e4)
+    spec.vclamps = vc
+    dt, nsteps, every = 25e-6, 300, 3
+    fields = ["Vm", "Ca", "GkSyn", "current", "Vm"]
+    idx = [0, 3, 1, 0, 40]
+    host = poisson_inputs(rng, spec, 500.0, nsteps * dt)
+    eng = pkg.Engine("fp64", kernel_mode=mode)
+    cid = eng.add_cell(spec, dt)
+    eng.reinit(dt)
+    eng.set(cid, "command", [0], [-0.05])
+    for (k, j, t) in host:
+        eng.add_spike(cid, k, j, t)
+    eng.set_probe_fields(cid, fields, idx, nsteps // every, every)
+    eng.step(nsteps)
+    got = eng.read_probes(cid)
+    eng.close()
+    ref = oracle_mod.OracleCell(spec, dt)
+    ref.reinit(dt)
+    ref.set("command", [0], [-0.05])
+    for (k, j, t) in host:
+        ref.add_spike(k, j, t)
+    want = []
+    for step in range(1, nsteps + 1):
+        ref.run(dt, 1)
+        if step % every == 0:
+            want.append([ref.get(f, [i])[0] for f, i in zip(fields, idx)])
+    want = np.array(want)
+    assert got.shape == want.shape
+    np.testing.assert_allclose(got, want, rtol=1e-6, atol=1e-12)
