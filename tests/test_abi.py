@@ -62,6 +62,13 @@ int main(void) {
   printf("hsg_pulsegen_desc %zu\n", sizeof(hsg_pulsegen_desc));
   P(hsg_pulsegen_desc, baseLevel); P(hsg_pulsegen_desc, width); P(hsg_pulsegen_desc, level);
   P(hsg_cell_desc, pulsegens);
+  printf("hsg_synchan_desc %zu\n", sizeof(hsg_synchan_desc));
+  printf("hsg_synapse_desc %zu\n", sizeof(hsg_synapse_desc));
+  printf("hsg_vclamp_desc %zu\n", sizeof(hsg_vclamp_desc));
+  P(hsg_leak_desc, Gbar); P(hsg_leak_desc, modulation);
+  P(hsg_synchan_desc, Gbar); P(hsg_synchan_desc, tau2); P(hsg_synchan_desc, modulation);
+  P(hsg_vclamp_desc, tau); P(hsg_vclamp_desc, gain);
+  P(hsg_cell_desc, synchans); P(hsg_cell_desc, synapses); P(hsg_cell_desc, vclamps);
   return 0;
 }
 '''
@@ -100,6 +107,17 @@ int main(void) {
     for f in ("baseLevel", "width", "level"):
         assert sizes["hsg_pulsegen_desc." + f] == _abi.PULSEGEN_DT.fields[f][1]
     assert sizes["hsg_cell_desc.pulsegens"] == _abi.CellDesc.pulsegens.offset
+    assert sizes["hsg_synchan_desc"] == _abi.SYNCHAN_DT.itemsize
+    assert sizes["hsg_synapse_desc"] == _abi.SYNAPSE_DT.itemsize
+    assert sizes["hsg_vclamp_desc"] == _abi.VCLAMP_DT.itemsize
+    for f in ("Gbar", "modulation"):
+        assert sizes["hsg_leak_desc." + f] == _abi.LEAK_DT.fields[f][1]
+    for f in ("Gbar", "tau2", "modulation"):
+        assert sizes["hsg_synchan_desc." + f] == _abi.SYNCHAN_DT.fields[f][1]
+    for f in ("tau", "gain"):
+        assert sizes["hsg_vclamp_desc." + f] == _abi.VCLAMP_DT.fields[f][1]
+    for f in ("synchans", "synapses", "vclamps"):
+        assert sizes["hsg_cell_desc." + f] == getattr(_abi.CellDesc, f).offset
 
 
 def test_engine_without_gpu_raises_cleanly():
