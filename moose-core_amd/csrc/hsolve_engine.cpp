@@ -2032,7 +2032,8 @@ int hsg_set_field(hsg_engine* e, int32_t cell, int32_t field, const int32_t* idx
             }
         } else if (field == HSG_F_SYNAPSE_WEIGHT || field == HSG_F_SYNAPSE_DELAY) {
             if (i < 0 || i >= (int)s.synW.size()) return fail(HSG_E_ARG, "unknown synapse index");
-            if (field == HSG_F_SYNAPSE_DELAY && !(x >= 0.0)) return fail(HSG_E_ARG, "synapse delay must be >= 0");
+            if (!std::isfinite(x) || (field == HSG_F_SYNAPSE_DELAY && !(x >= 0.0)))
+                return fail(HSG_E_ARG, "synapse weight must be finite, delay finite and >= 0");
             (field == HSG_F_SYNAPSE_WEIGHT ? s.synW[i] : s.synDelay[i]) = x;
         } else if (field >= HSG_F_LEAK_GBAR && field <= HSG_F_LEAK_MODULATION) {
             if (i < 0 || i >= (int)s.leakOfDesc.size() || s.leakOfDesc[i] < 0)
@@ -2115,6 +2116,7 @@ int hsg_add_spike(hsg_engine* e, int32_t cell, int32_t synchan, int32_t synapse,
     if (synchan < 0 || synchan >= (int)s.synOfDesc.size()) return fail(HSG_E_ARG, "no such SynChan");
     const int q = s.synOfDesc[synchan];
     if (synapse < 0 || synapse >= s.syn[q].nSyn) return fail(HSG_E_ARG, "no such synapse");
+    if (!std::isfinite(time)) return fail(HSG_E_ARG, "spike time must be finite");
     const int j = s.syn[q].firstSyn + synapse;
     queue_event(hc->synQ[q], time + s.synDelay[j], s.synW[j]);
     return 0;

@@ -19,6 +19,7 @@
 #include "hsolve_internal.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cfloat>
 #include <cstring>
 #include <set>
@@ -974,6 +975,10 @@ int setup_cell(const hsg_cell_desc* d, double dt, int caAdvance, CellSetup& s, s
         for (int j = 0; j < nsyn; ++j) {
             s.synW[j] = d->synapses[j].weight;
             s.synDelay[j] = d->synapses[j].delay;
+            if (!std::isfinite(s.synW[j]) || !(s.synDelay[j] >= 0.0) || !std::isfinite(s.synDelay[j])) {
+                err = "synapse: weight must be finite and delay finite and >= 0";
+                return HSG_E_ARG;
+            }
         }
         s.syn.clear();
         s.synStart.assign(n + 1, 0);

@@ -625,6 +625,11 @@ class VClamp(Neutral):
         # biophysics/VClamp.cpp:196-205
         self.__dict__.update(mode=0, tau=0.0, ti=0.0, td=-1.0, gain=0.0, command=0.0, current=0.0)
 
+    def _set_local(self, name, value):
+        if name in ("mode", "tau", "ti", "td", "gain") and self.__dict__.get("_solver") is not None:
+            raise NotImplementedError(f"{self.path}: set VClamp {name} before HSolve.target")
+        object.__setattr__(self, name, value)
+
 
 class Leakage(Neutral):
     """A Leakage channel (biophysics/Leakage.cpp): not taken over by the
