@@ -507,6 +507,11 @@ int place_cell(hsg_engine* e, HostCell& hc) {
         // PulseGens
         {&d.oPulse, (int64_t)sizeof(PulseDev) * (int64_t)s.pulses.size()},
         {&d.oVClamp, (int64_t)sizeof(VClampDev) * (int64_t)s.vclamps.size()},
+        {&d.oPulseStart, 4 * (int64_t)s.pulseStart.size()},
+        {&d.oPulseIdx, 4 * (int64_t)s.pulseIdx.size()},
+        {&d.oVcStart, 4 * (int64_t)s.vcStart.size()},
+        {&d.oVcIdx, 4 * (int64_t)s.vcIdx.size()},
+        {&d.oInjOf, 4 * (int64_t)n},
         // Leakage sends and SynChans
         {&d.oXFlag, nX == 0 ? 0 : 4 * (int64_t)n},
         {&d.oLeakStart, s.leaks.empty() ? 0 : 4 * ((int64_t)n + 1)},
@@ -657,6 +662,10 @@ int place_cell(hsg_engine* e, HostCell& hc) {
         }
         st.put(d.oPulse, pd);
     }
+    st.put(d.oPulseStart, s.pulseStart);
+    st.put(d.oPulseIdx, s.pulseIdx);
+    st.put(d.oVcStart, s.vcStart);
+    st.put(d.oVcIdx, s.vcIdx);
     if (!s.vclamps.empty()) {
         std::vector<VClampDev> vd(s.vclamps.size());
         for (size_t k = 0; k < s.vclamps.size(); ++k) {
@@ -694,6 +703,7 @@ int place_cell(hsg_engine* e, HostCell& hc) {
     d.n2d = (int32_t)s.ch2d.size();
     d.nPulse = (int32_t)s.pulses.size();
     d.nVClamp = (int32_t)s.vclamps.size();
+    d.nSetupEntries = s.nSetupEntries;
     d.values2dSent = 0;
     d.nSlots = s.nSlots;
     d.rootSlot = s.rootSlot;
@@ -729,9 +739,12 @@ int write_injections(hsg_engine* e, HostCell& hc) {
     hc.dev.oInjBasal = align_up(hc.inj.off + 4 * (int64_t)std::max(1, m));
     hc.dev.oInjVar = align_up(hc.dev.oInjBasal + 8 * (int64_t)std::max(1, m));
     std::vector<int32_t> idx(hc.injIdx.begin(), hc.injIdx.end());
+    // compartment -> entry, for the split kernels' per-compartment lookup
+    std::vector<int32_t> of(hc.s.n, -1);
+    for (int k = 0; k < m; ++k) of[hc.injIdx[k]] = k;
     int rc;
     if ((rc = upload(e, hc.dev.oInjIdx, idx)) || (rc = upload(e, hc.dev.oInjBasal, hc.injBasal)) ||
-        (rc = upload(e, hc.dev.oInjVar, hc.injVar)))
+        (rc = upload(e, hc.dev.oInjVar, hc.injVar)) || (rc = upload(e, hc.dev.oInjOf, of)))
         return rc;
     hc.dev.nInj = m;
     hc.injDirty = false;

@@ -221,8 +221,12 @@ struct CellDev {
     int64_t oAct;           // ActDev[]: the call's activations, by synchan then step
     // VClamps
     int32_t nVClamp;
-    int32_t pad4;
+    int32_t nSetupEntries;  // injection entries at setup (the ones objects send to)
     int64_t oVClamp;        // VClampDev[nVClamp]
+    // per setup entry e, the PulseGens / VClamps sending to it:
+    // idx[start[e] .. start[e+1]) (desc order)
+    int64_t oPulseStart, oPulseIdx, oVcStart, oVcIdx;
+    int64_t oInjOf;         // int32[n]: injection entry of compartment i, -1
 };
 
 // Elimination unit descriptor: 12 int32 (48 bytes), read as three int4 loads.

@@ -933,16 +933,20 @@ __device__ __forceinline__ double inject_varying(char* arena, const CellDev& c, 
     double* iv = A_<double>(arena, c.oInjVar);
     double var = first ? iv[e] : 0.0;
     if (first) iv[e] = 0.0;
+    if (e >= c.nSetupEntries) return var;   // an entry added later: host inputs only
     if (c.nPulse > 0) {
         PulseDev* pl = A_<PulseDev>(arena, c.oPulse);
-        for (int q = 0; q < c.nPulse; ++q)
-            if (pl[q].entry == e && pulse_tick(pl[q], t)) var += pl[q].out;
+        const int32_t* st = A_<const int32_t>(arena, c.oPulseStart);
+        const int32_t* ix = A_<const int32_t>(arena, c.oPulseIdx);
+        for (int q = st[e]; q < st[e + 1]; ++q)
+            if (pulse_tick(pl[ix[q]], t)) var += pl[ix[q]].out;
     }
     if (c.nVClamp > 0) {
         VClampDev* vc = A_<VClampDev>(arena, c.oVClamp);
+        const int32_t* st = A_<const int32_t>(arena, c.oVcStart);
+        const int32_t* ix = A_<const int32_t>(arena, c.oVcIdx);
         const double* V = A_<const double>(arena, c.oV);
-        for (int q = 0; q < c.nVClamp; ++q)
-            if (vc[q].entry == e) var += vclamp_tick(vc[q], V[vc[q].compt]);
+        for (int q = st[e]; q < st[e + 1]; ++q) var += vclamp_tick(vc[ix[q]], V[vc[ix[q]].compt]);
     }
     return var;
 }
@@ -1204,25 +1208,29 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
             if (c.nVClamp > 0) __syncthreads();
             else work_barrier(WL);
             const int32_t* xflag = A_<const int32_t>(arena, c.oXFlag);
-            if (tid == 0) {
-                // injections: HS[4i+3] += injectVarying + injectBasal (HSolveActive.cpp:119-128)
+            // injections: HS[4i+3] += injectVarying + injectBasal
+            // (HSolveActive.cpp:119-128); one entry per compartment, so
+            // the entries are independent
+            {
                 const int32_t* ii = A_<const int32_t>(arena, c.oInjIdx);
                 const double* ib = A_<const double>(arena, c.oInjBasal);
                 const double t = a.dt * (double)(unsigned long long)(step0 + k + 1);
-                for (int e = 0; e < nInj; ++e) {
+                for (int e = tid; e < nInj; e += nt) {
                     const double var = inject_varying(arena, c, e, first, t);
                     w.B[ii[e]] += var + ib[e];
                 }
-                if (first) {
-                    // external channels (HSolveActive.cpp:142-148)
-                    const int32_t* xi = A_<const int32_t>(arena, c.oExtIdx);
-                    const double* xg = A_<const double>(arena, c.oExtG);
-                    const double* xge = A_<const double>(arena, c.oExtGE);
-                    for (int e = 0; e < nExt; ++e) {
-                        if (c.nXObj > 0 && xflag[xi[e]]) continue;   // in o2dExt
-                        w.D[xi[e]] += xg[e];
-                        w.B[xi[e]] += xge[e];
-                    }
+            }
+            if (first && nExt > 0) {
+                // external channels after the injection (HSolveActive.cpp:142-148);
+                // one host entry per compartment
+                work_barrier(WL);
+                const int32_t* xi = A_<const int32_t>(arena, c.oExtIdx);
+                const double* xg = A_<const double>(arena, c.oExtG);
+                const double* xge = A_<const double>(arena, c.oExtGE);
+                for (int e = tid; e < nExt; e += nt) {
+                    if (c.nXObj > 0 && xflag[xi[e]]) continue;   // in o2dExt
+                    w.D[xi[e]] += xg[e];
+                    w.B[xi[e]] += xge[e];
                 }
             }
             if (c.nXObj > 0) {
@@ -1384,15 +1392,14 @@ __device__ __forceinline__ void row_dest(char* arena, const CellDev& c, int i, d
 // mode: the injection, then the external current (HSolveActive.cpp:117-148).
 __device__ __forceinline__ void row_inputs(const LaunchArgs& a, const CellDev& c, int i, double& D, double& B) {
     const bool has2d = c.nXObj > 0 && A_<const int32_t>(a.arena, c.oXFlag)[i];
-    const int32_t* ii = A_<const int32_t>(a.arena, c.oInjIdx);
-    for (int e = 0; e < c.nInj; ++e)
-        if (ii[e] == i) {
-            // this tick ends at step (steps done + 1); the solve kernel
-            // advances the counter after it
-            const double t = a.dt * (double)(unsigned long long)(*A_<const int64_t>(a.arena, c.oSteps) + 1);
-            const double var = inject_varying(a.arena, c, e, a.firstOfCall != 0, t);
-            B += var + A_<const double>(a.arena, c.oInjBasal)[e];
-        }
+    const int e = c.nInj > 0 ? A_<const int32_t>(a.arena, c.oInjOf)[i] : -1;
+    if (e >= 0) {
+        // this tick ends at step (steps done + 1); the solve kernel
+        // advances the counter after it
+        const double t = a.dt * (double)(unsigned long long)(*A_<const int64_t>(a.arena, c.oSteps) + 1);
+        const double var = inject_varying(a.arena, c, e, a.firstOfCall != 0, t);
+        B += var + A_<const double>(a.arena, c.oInjBasal)[e];
+    }
     if (has2d) {
         const double2 g = external_current(a.arena, a.firstOfCall != 0, c, i,
                                            *A_<const int64_t>(a.arena, c.oSteps) + 1);

@@ -928,6 +928,20 @@ int setup_cell(const hsg_cell_desc* d, double dt, int caAdvance, CellSetup& s, s
             if (s.hasInject[i]) entryOf[i] = m++;
         for (auto& p : s.pulses) p.entry = entryOf[p.compt];
         for (auto& v : s.vclamps) v.entry = entryOf[v.compt];
+        s.nSetupEntries = m;
+        auto ranges = [m](const std::vector<int>& entryOfObj, std::vector<int32_t>& start, std::vector<int32_t>& idx) {
+            start.assign(m + 1, 0);
+            for (int e : entryOfObj) start[e + 1]++;
+            for (int e = 0; e < m; ++e) start[e + 1] += start[e];
+            idx.assign(entryOfObj.size(), 0);
+            std::vector<int32_t> fill(start.begin(), start.end() - 1);
+            for (size_t k = 0; k < entryOfObj.size(); ++k) idx[fill[entryOfObj[k]]++] = (int32_t)k;
+        };
+        std::vector<int> pe, ve;
+        for (auto& p : s.pulses) pe.push_back(p.entry);
+        for (auto& v : s.vclamps) ve.push_back(v.entry);
+        ranges(pe, s.pulseStart, s.pulseIdx);
+        ranges(ve, s.vcStart, s.vcIdx);
     }
 
     // Leakage objects (their per-tick sends) and SynChans, grouped by

@@ -153,6 +153,10 @@ struct CellSetup {
     std::vector<HostChan2D> ch2d;
     std::vector<HostPulse> pulses;    // PulseGens on injectMsg, desc order
     std::vector<HostVClamp> vclamps;  // VClamps on injectMsg, desc order
+    // the injection entries at setup, and per entry the PulseGens and
+    // VClamps sending to it (indices, desc order within an entry)
+    int nSetupEntries = 0;
+    std::vector<int32_t> pulseStart, pulseIdx, vcStart, vcIdx;
     // Leakage sends and SynChans, sorted by compartment (desc order within
     // one), with their compartment -> objects indexes (n + 1)
     std::vector<HostLeak> leaks;

@@ -417,3 +417,39 @@ def test_gpu_vclamp_batch(pkg, oracle_mod, mode):
         np.testing.assert_allclose(eng.get(cid, "current", [0]), r.get("current", [0]), rtol=1e-6, atol=1e-18)
         assert eng.get(cid, "command", [0])[0] == -0.065
     eng.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["fused", "split"])
+def test_gpu_pulsegen_on_every_compartment(pkg, oracle_mod, mode):
+    """A PulseGen on every compartment (two on some), plus host injections
+    added later on others: the per-entry ranges and the parallel injection
+    stage give the oracle's traces."""
+    print(f"case: 300-compartment HH cell, 340 PulseGens, 500 steps, fp64, {mode}", flush=True)
+    rng = np.random.default_rng(91)
+    spec = workloads.random_hh_cell(rng, 300, inject=0.0)
+    recs = []
+    for c in range(300):
+        tr = dict(TRAINS[c % 3])
+        tr["level"] = [lv * 0.2 for lv in tr["level"]]
+        recs.append(train(c, **tr))
+    for c in range(0, 300, 7):
+        recs.append(train(c, **TRAINS[3]))
+    spec.pulsegens = np.array(recs, PULSEGEN_DT)
+    dt = 25e-6
+    eng = pkg.Engine("fp64", kernel_mode=mode)
+    cid = eng.add_cell(spec, dt)
+    eng.set_probes(cid, np.arange(300), 500)
+    eng.reinit(dt)
+    ref = oracle_mod.OracleCell(spec, dt)
+    ref.reinit(dt)
+    eng.step(200)
+    ref_tr = [ref.run(dt, 200, np.arange(300))]
+    for c in (3, 150):
+        eng.add_inject(cid, c, 0.05e-9)
+        ref.add_inject(c, 0.05e-9)
+    eng.step(300)
+    ref_tr.append(ref.run(dt, 300, np.arange(300)))
+    got = eng.read_probes(cid)
+    eng.close()
+    assert np.abs(got - np.concatenate(ref_tr)).max() <= FP64_TOL
