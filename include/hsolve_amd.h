@@ -420,6 +420,11 @@ typedef struct hsg_cell_info {
     int32_t max_group;        /* largest junction group (members)            */
     int64_t device_bytes;     /* device memory attributed to the cell        */
     int64_t steps_done;
+    /* the lookup grids setup derived from the gates, which the reference
+     * writes back into the HSolve's vMin/vMax/vDiv and caMin/caMax/caDiv
+     * fields (hsolve/HSolveActiveSetup.cpp:366-409) */
+    double  vMin, vMax, caMin, caMax;
+    int32_t vDiv, caDiv;
 } hsg_cell_info;
 int hsg_cell_info_get(hsg_engine* e, int32_t cell, hsg_cell_info* out);
 

@@ -91,7 +91,8 @@ class CellInfo(C.Structure):
                 ("n_gates", C.c_int32), ("n_pools", C.c_int32),
                 ("n_spikegens", C.c_int32), ("n_levels", C.c_int32),
                 ("max_group", C.c_int32), ("device_bytes", C.c_int64),
-                ("steps_done", C.c_int64)]
+                ("steps_done", C.c_int64), ("vMin", C.c_double), ("vMax", C.c_double),
+                ("caMin", C.c_double), ("caMax", C.c_double), ("vDiv", C.c_int32), ("caDiv", C.c_int32)]
 
 
 # numpy record layouts identical to the C structs (natural alignment)

@@ -2334,6 +2334,12 @@ int hsg_cell_info_get(hsg_engine* e, int32_t cell, hsg_cell_info* out) {
     out->device_bytes = hc->region.bytes + std::max<int64_t>(0, hc->inj.bytes) +
                         std::max<int64_t>(0, hc->probe.bytes) + std::max<int64_t>(0, hc->extBlk.bytes);
     out->steps_done = hc->stepsDone;
+    out->vMin = s.tables.v.min;
+    out->vMax = s.tables.v.max;
+    out->vDiv = (int32_t)s.tables.v.nDivs;
+    out->caMin = s.tables.ca.min;
+    out->caMax = s.tables.ca.max;
+    out->caDiv = (int32_t)s.tables.ca.nDivs;
     return 0;
 }
 

@@ -1029,6 +1029,11 @@ class HSolve(Neutral):
         cell = eng.add_cell(spec, float(self.dt), int(self.caAdvance))
         object.__setattr__(self, "_engine", eng)
         object.__setattr__(self, "_cell", cell)
+        # setup overwrites the lookup-grid fields with the grids it derived
+        # from the gates (hsolve/HSolveActiveSetup.cpp:366-409)
+        info = eng.cell_info(cell)
+        for f in ("vMin", "vMax", "vDiv", "caMin", "caMax", "caDiv"):
+            object.__setattr__(self, f, info[f])
         object.__setattr__(self, "_objs", objs)
         object.__setattr__(self, "_spec", spec)
         # zombify (hsolve/HSolve.cpp:235-262): fields now live in the solver

@@ -392,6 +392,13 @@ def test_reference_model_gpu(pkg, name, precision):
         hs = moose.HSolve("/model/elec/hsolve")
         hs.dt = DT
         hs.target = "/model/elec/soma"
+        if name == "squid_axon":
+            # setup's union grid over Na (-110..50 mV) and K (-120..40 mV),
+            # 3000 divisions each, written back to the HSolve's fields; no
+            # Ca gates leaves the Ca grid at its empty-set values
+            dx = min((50e-3 - -110e-3) / 3000, (40e-3 - -120e-3) / 3000)
+            assert (hs.vMin, hs.vMax, hs.vDiv) == (-120e-3, 50e-3, int((50e-3 - -120e-3) / dx + 0.5))
+            assert (hs.caMin, hs.caMax, hs.caDiv) == (np.finfo(float).max, np.finfo(float).tiny, 0)
         moose.reinit()
         if name == "vclamp":
             moose.element("/model/elec/soma/vclamp").command = expr(0.0)   # the Function's reinit
