@@ -40,10 +40,10 @@ HBM_PEAK_GBS = 8000.0
 # CmByDt/EmByRm/diag/offdiag 32 (FP64), w = 8 (FP64) or 4 (mixed)
 BYTES_PER_CS = {"fp64": 16 + 2 * 8 * 3 + 2 * 8 * 2 + 32, "mixed": 16 + 2 * 4 * 3 + 2 * 4 * 2 + 32}
 # the large Ca cells: six gates and four HHChannels, a CaConc (Ca r+w, its
-# activation r+w, basal 8: 40) and an HHChannel2D (its 136-byte record, Z r+w,
-# Gk w, the external current w then r: 192) per compartment
-BYTES_PER_CS_LARGE = {"fp64": 16 + 2 * 8 * 6 + 2 * 8 * 4 + 32 + 40 + 192,
-                      "mixed": 16 + 2 * 4 * 6 + 2 * 4 * 4 + 32 + 40 + 192}
+# activation r+w, basal 8: 40) and an HHChannel2D (its 40-byte instance
+# record, Z r+w, Gk w, the external current w then r: 96) per compartment
+BYTES_PER_CS_LARGE = {"fp64": 16 + 2 * 8 * 6 + 2 * 8 * 4 + 32 + 40 + 96,
+                      "mixed": 16 + 2 * 4 * 6 + 2 * 4 * 4 + 32 + 40 + 96}
 
 
 def log(msg):

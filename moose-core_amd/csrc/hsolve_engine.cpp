@@ -474,6 +474,26 @@ int place_cell(hsg_engine* e, HostCell& hc) {
     const bool wantSplit = e->split || (e->opt.kernel_mode == HSG_KERNEL_AUTO && n >= kAutoSplitMin);
     const bool msgOn = wantSplit && s.msgOk && msgLds <= e->ldsLimit;
     const int64_t nX = (int64_t)(s.ch2d.size() + s.leaks.size() + s.syn.size());
+    // the HHChannel2D kinds of the cell: objects with the same powers,
+    // tables, axes and instant flags share one record
+    std::vector<int> kindOf(s.ch2d.size(), 0);
+    std::vector<size_t> kindRep;
+    {
+        std::map<std::vector<double>, int> kinds;
+        for (size_t k = 0; k < s.ch2d.size(); ++k) {
+            const HostChan2D& h = s.ch2d[k];
+            std::vector<double> key(h.pw, h.pw + 3);
+            for (int a = 0; a < 3; ++a) { key.push_back(h.tabA[a]); key.push_back(h.tabB[a]); }
+            for (int q = 0; q < 6; ++q) key.push_back(h.dep[q]);
+            key.push_back(h.instant);
+            auto it = kinds.find(key);
+            if (it == kinds.end()) {
+                it = kinds.emplace(key, (int)kindRep.size()).first;
+                kindRep.push_back(k);
+            }
+            kindOf[k] = it->second;
+        }
+    }
     std::vector<Item> items = {
         {&d.oV, 8 * n}, {&d.oInitVm, 8 * n}, {&d.oCmByDt, 8 * n}, {&d.oEmByRm, 8 * n}, {&d.oDbase, 8 * n}, {&d.oOff, 8 * n},
         {&d.oDefer, 8 * n},
@@ -501,6 +521,7 @@ int place_cell(hsg_engine* e, HostCell& hc) {
         // HHChannel2D objects
         {&d.o2dStart, s.ch2d.empty() ? 0 : 4 * ((int64_t)n + 1)},
         {&d.o2dChan, (int64_t)sizeof(Chan2DDev) * (int64_t)s.ch2d.size()},
+        {&d.o2dType, (int64_t)sizeof(Chan2DType) * (int64_t)kindRep.size()},
         {&d.o2dState, 24 * (int64_t)s.ch2d.size()},
         {&d.o2dGk, 8 * (int64_t)s.ch2d.size()},
         {&d.o2dExt, nX == 0 ? 0 : 16 * (int64_t)n},
@@ -622,25 +643,33 @@ int place_cell(hsg_engine* e, HostCell& hc) {
         std::vector<int> tid(s.tables2d.size(), -1);
         for (size_t k = 0; k < s.tables2d.size(); ++k)
             if ((rc = table2d_id(e, s.tables2d[k], &tid[k]))) return rc;
+        std::vector<Chan2DType> ty(kindRep.size());
+        for (size_t t = 0; t < kindRep.size(); ++t) {
+            const HostChan2D& h = s.ch2d[kindRep[t]];
+            Chan2DType& y = ty[t];
+            std::memset(&y, 0, sizeof y);
+            for (int a = 0; a < 3; ++a) {
+                y.pw[a] = h.pw[a];
+                y.tabA[a] = h.tabA[a] >= 0 ? tid[h.tabA[a]] : -1;
+                y.tabB[a] = h.tabB[a] >= 0 ? tid[h.tabB[a]] : -1;
+            }
+            for (int q = 0; q < 6; ++q) y.dep[q] = h.dep[q];
+            y.instant = h.instant;
+        }
         std::vector<Chan2DDev> ch(s.ch2d.size());
         std::vector<double> stt(3 * s.ch2d.size());
         for (size_t k = 0; k < s.ch2d.size(); ++k) {
             const HostChan2D& h = s.ch2d[k];
             Chan2DDev& c = ch[k];
             c.Gbar = h.Gbar; c.Ek = h.Ek; c.mod = h.mod;
-            for (int a = 0; a < 3; ++a) {
-                c.pw[a] = h.pw[a];
-                c.tabA[a] = h.tabA[a] >= 0 ? tid[h.tabA[a]] : -1;
-                c.tabB[a] = h.tabB[a] >= 0 ? tid[h.tabB[a]] : -1;
-                stt[3 * k + a] = h.st[a];
-            }
-            for (int q = 0; q < 6; ++q) c.dep[q] = h.dep[q];
+            for (int a = 0; a < 3; ++a) stt[3 * k + a] = h.st[a];
+            c.type = kindOf[k];
             c.conc1 = h.conc1; c.conc2 = h.conc2;
-            c.instant = h.instant;
             c.compt = h.compt;
         }
         st.put(d.o2dStart, s.ch2dStart);
         st.put(d.o2dChan, ch);
+        st.put(d.o2dType, ty);
         st.put(d.o2dState, stt);
     }
     if (!s.pulses.empty()) {

@@ -51,15 +51,23 @@ struct Table2DDev {
     int64_t off;                // f64 [nx * ny], x-major
 };
 
-// One HHChannel2D (biophysics/HHChannel2D.cpp) run beside the solver.
-struct Chan2DDev {
-    double  Gbar, Ek, mod;
+// What HHChannel2D objects of one kind share (biophysics/HHChannel2D.cpp):
+// gate powers, tables, table axes, instant flags.  One record per distinct
+// kind in the cell, read from cache by every instance.
+struct Chan2DType {
     double  pw[3];
     int32_t tabA[3], tabB[3];   // engine-wide Table2DDev ids (gate with power > 0)
     int32_t dep[6];             // table axes of gate g: dep[2g], dep[2g+1] (0 Vm, 1 conc1, 2 conc2, -1)
-    int32_t conc1, conc2;       // slot-major pool offsets of the concentrations, -1
     int32_t instant;
+    int32_t pad;
+};
+
+// One HHChannel2D run beside the solver: what varies per object.
+struct Chan2DDev {
+    double  Gbar, Ek, mod;
+    int32_t type;               // Chan2DType of the cell
     int32_t compt;              // Hines index
+    int32_t conc1, conc2;       // slot-major pool offsets of the concentrations, -1
 };
 
 // A free-running PulseGen feeding a compartment's injection
@@ -197,6 +205,7 @@ struct CellDev {
     int32_t values2dSent;   // 1 once the solver has sent Vm/[Ca] (reinit or step)
     int64_t o2dStart;       // int32[n + 1]: channels of compartment i
     int64_t o2dChan;        // Chan2DDev[n2d]
+    int64_t o2dType;        // Chan2DType[n2dType]
     int64_t o2dState;       // f64[3 * n2d]: X, Y, Z
     int64_t o2dGk;          // f64[n2d]: Gk of the last tick
     int64_t o2dExt;         // f64[2n]: external (Gk, Gk*Ek) of the last tick per compartment

@@ -350,27 +350,29 @@ __device__ __forceinline__ double2 external_sum(char* arena, const Table2DDev* _
     if (WITH2D && c.n2d > 0) {
         const int32_t* start = A_<const int32_t>(arena, c.o2dStart);
         const Chan2DDev* chans = A_<const Chan2DDev>(arena, c.o2dChan);
+        const Chan2DType* kinds = A_<const Chan2DType>(arena, c.o2dType);
         double* st = A_<double>(arena, c.o2dState);
         double* gk = A_<double>(arena, c.o2dGk);
         const bool sent = c.values2dSent != 0;
 #pragma unroll 1
         for (int k = start[i]; k < start[i + 1]; ++k) {
             const Chan2DDev& ch = chans[k];
+            const Chan2DType& ty = kinds[ch.type];
             double g = 0.0;
             g += ch.Gbar;
 #pragma unroll 1
             for (int ax = 0; ax < 3; ++ax) {
-                if (!(ch.pw[ax] > 0.0)) continue;
-                const double u = dep2d(ch.dep[2 * ax], sent, Vm, ch, poolCa);
-                const double w = dep2d(ch.dep[2 * ax + 1], sent, Vm, ch, poolCa);
-                const Table2DDev& ta = tabs2d[ch.tabA[ax]];
-                const Table2DDev& tb = tabs2d[ch.tabB[ax]];
+                if (!(ty.pw[ax] > 0.0)) continue;
+                const double u = dep2d(ty.dep[2 * ax], sent, Vm, ch, poolCa);
+                const double w = dep2d(ty.dep[2 * ax + 1], sent, Vm, ch, poolCa);
+                const Table2DDev& ta = tabs2d[ty.tabA[ax]];
+                const Table2DDev& tb = tabs2d[ty.tabB[ax]];
                 const double A = interp2d(ta, A_<const double>(arena, ta.off), u, w);
                 const double B = interp2d(tb, A_<const double>(arena, tb.off), u, w);
                 double x = st[3 * k + ax];
-                x = (ch.instant >> ax) & 1 ? A / B : integrate_ee(x, dt, A, B);
+                x = (ty.instant >> ax) & 1 ? A / B : integrate_ee(x, dt, A, B);
                 st[3 * k + ax] = x;
-                g *= take_power<double>(x, ch.pw[ax]);
+                g *= take_power<double>(x, ty.pw[ax]);
             }
             const double Gk = g * ch.mod;
             gk[k] = Gk;
@@ -442,6 +444,7 @@ __device__ __forceinline__ double2 external_current(char* arena, bool first, con
 // EPSILON stops that channel's reinit.
 __device__ __forceinline__ void chan2d_reinit(char* arena, const Table2DDev* __restrict__ tabs2d, const CellDev& c) {
     const Chan2DDev* chans = A_<const Chan2DDev>(arena, c.o2dChan);
+    const Chan2DType* kinds = A_<const Chan2DType>(arena, c.o2dType);
     double* st = A_<double>(arena, c.o2dState);
     double* gk = A_<double>(arena, c.o2dGk);
     const double* V = A_<const double>(arena, c.oV);
@@ -449,15 +452,16 @@ __device__ __forceinline__ void chan2d_reinit(char* arena, const Table2DDev* __r
     const bool sent = c.values2dSent != 0;
     for (int k = threadIdx.x; k < c.n2d; k += blockDim.x) {
         const Chan2DDev& ch = chans[k];
+        const Chan2DType& ty = kinds[ch.type];
         double g = ch.Gbar;
         bool stop = false;
 #pragma unroll 1
         for (int ax = 0; ax < 3 && !stop; ++ax) {
-            if (!(ch.pw[ax] > 0.0)) continue;
-            const double u = dep2d(ch.dep[2 * ax], sent, V[ch.compt], ch, poolCa);
-            const double w = dep2d(ch.dep[2 * ax + 1], sent, V[ch.compt], ch, poolCa);
-            const Table2DDev& ta = tabs2d[ch.tabA[ax]];
-            const Table2DDev& tb = tabs2d[ch.tabB[ax]];
+            if (!(ty.pw[ax] > 0.0)) continue;
+            const double u = dep2d(ty.dep[2 * ax], sent, V[ch.compt], ch, poolCa);
+            const double w = dep2d(ty.dep[2 * ax + 1], sent, V[ch.compt], ch, poolCa);
+            const Table2DDev& ta = tabs2d[ty.tabA[ax]];
+            const Table2DDev& tb = tabs2d[ty.tabB[ax]];
             const double A = interp2d(ta, A_<const double>(arena, ta.off), u, w);
             const double B = interp2d(tb, A_<const double>(arena, tb.off), u, w);
             if (B < 1.0e-10) {
@@ -465,7 +469,7 @@ __device__ __forceinline__ void chan2d_reinit(char* arena, const Table2DDev* __r
                 break;
             }
             st[3 * k + ax] = A / B;
-            g *= take_power<double>(st[3 * k + ax], ch.pw[ax]);
+            g *= take_power<double>(st[3 * k + ax], ty.pw[ax]);
         }
         if (!stop) gk[k] = g * ch.mod;
     }
