@@ -263,12 +263,14 @@ struct hsg_engine {
         int64_t sumN = 0;           // compartments in the bin
         int lds = 0;
         int threads = 256;
+        bool pairNext = false;      // first half of a split bin whose second half is the next bin
     };
     std::vector<Bin> bins;
     bool split = false;          // channel phase as its own kernel
     static constexpr int kMaxBins = 32;
     hipStream_t binStream[kMaxBins] = {};
     hipEvent_t evFork = nullptr, evJoin[kMaxBins] = {};
+    hipEvent_t evPairA[kMaxBins] = {}, evPairB[kMaxBins] = {};   // hand-offs inside a pipelined pair
     // SpikeGen spikeOut -> Synapse addSpike messages between cells: (source
     // cell, spikegen desc index) -> (target cell, synchan desc index,
     // synapse index within it), in connection order
@@ -1012,6 +1014,21 @@ int make_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg_
         cur.cells.push_back(id);
     }
     if (!cur.cells.empty()) bins.push_back(cur);
+    // a split bin of few cells leaves most of the chip idle during its solve:
+    // its two halves (cells alternating by size) run as a pipelined pair, the
+    // bandwidth-bound kernels of one half beside the solve of the other
+    for (size_t b = 0; b < bins.size() && bins.size() < (size_t)hsg_engine::kMaxBins; ++b) {
+        const size_t nb = bins[b].cells.size();
+        if (!bins[b].split || nb < 2 || nb > (size_t)kPairMaxCells) continue;
+        hsg_engine::Bin second = bins[b];
+        std::vector<int32_t> all;
+        all.swap(bins[b].cells);
+        second.cells.clear();
+        for (size_t i = 0; i < nb; ++i) (i % 2 ? second.cells : bins[b].cells).push_back(all[i]);
+        bins[b].pairNext = true;
+        bins.insert(bins.begin() + b + 1, second);
+        ++b;
+    }
     for (auto& b : bins) {
         int maxN = 0, maxSlots = 0;
         bool allUniform = true;
@@ -1265,8 +1282,33 @@ int fused_fast(const hsg_engine::Bin& b) {
     return perThread >= HSG_FAST_FUSED_MIN ? 2 : 0;
 }
 
+// Bandwidth phase of one split tick: the HHChannel2D objects (bins that have
+// them), then the channel kernel over the bin's tiles.
+int split_wide(hsg_engine* e, const hsg_engine::Bin& bin, LaunchArgs& a, hipStream_t st) {
+    if (bin.hasExt) HIPCHK(hsg_launch_external(&a, bin.dTiles, bin.nTiles, st));
+    if (bin.chanMS > 0) HIPCHK(hsg_launch_channel_fast(&a, bin.dTiles, bin.nTiles, e->fp32, bin.chanMS, st));
+    else HIPCHK(hsg_launch_channel(&a, bin.dTiles, bin.nTiles, e->fp32, st));
+    return 0;
+}
+
+// Solve phase of one split tick: the solve stops at VMid; V, Ca and the
+// tick's outputs follow over the batch.
+int split_solve(hsg_engine* e, const hsg_engine::Bin& bin, LaunchArgs& a, hipStream_t st) {
+    const int nb = (int)bin.cells.size();
+    a.deferD = 1;
+    if (bin.msg) HIPCHK(hsg_launch_msg(&a, nb, bin.threads, e->fp32, st));
+    else HIPCHK(hsg_launch_solve(&a, nb, bin.threads, e->fp32, bin.slow, st));
+    HIPCHK(hsg_launch_update(&a, bin.dTiles, bin.nTiles, e->fp32, st));
+    HIPCHK(hsg_launch_outputs(&a, nb, st));
+    return 0;
+}
+
 // One round: every bin launched for `chunk` steps on its own stream, forked
-// from and joined back into the engine stream.
+// from and joined back into the engine stream. The halves A, B of a
+// pipelined pair alternate their bandwidth phases through two events: B's
+// phase of tick k follows A's, and A's phase of tick k+1 follows B's of tick
+// k, so each half's solve overlaps the other half's channel work. Every
+// cell still sees its own kernels in tick order.
 int launch_round(hsg_engine* e, std::vector<hsg_engine::Bin>& bins, int chunk, int first) {
     HIPCHK(hipEventRecord(e->evFork, e->stream));
     for (size_t b = 0; b < bins.size(); ++b) {
@@ -1274,29 +1316,45 @@ int launch_round(hsg_engine* e, std::vector<hsg_engine::Bin>& bins, int chunk, i
         HIPCHK(hipStreamWaitEvent(st, e->evFork, 0));
         const int nb = (int)bins[b].cells.size();
         const bool split = bins[b].split;
+        if (split && bins[b].pairNext && b + 1 < bins.size()) {
+            hipStream_t st2 = e->binStream[b + 1];
+            HIPCHK(hipStreamWaitEvent(st2, e->evFork, 0));
+            const hsg_engine::Bin& A = bins[b];
+            const hsg_engine::Bin& B = bins[b + 1];
+            for (int k = 0; k < chunk; ++k) {
+                LaunchArgs a = make_args(e, A.dOrder, 1, first && k == 0);
+                a.ldsBytes = A.lds;
+                if (k > 0) HIPCHK(hipStreamWaitEvent(st, e->evPairB[b], 0));
+                if (int rc = split_wide(e, A, a, st)) return rc;
+                HIPCHK(hipEventRecord(e->evPairA[b], st));
+                if (int rc = split_solve(e, A, a, st)) return rc;
+                LaunchArgs c = make_args(e, B.dOrder, 1, first && k == 0);
+                c.ldsBytes = B.lds;
+                HIPCHK(hipStreamWaitEvent(st2, e->evPairA[b], 0));
+                if (int rc = split_wide(e, B, c, st2)) return rc;
+                HIPCHK(hipEventRecord(e->evPairB[b], st2));
+                if (int rc = split_solve(e, B, c, st2)) return rc;
+            }
+            HIPCHK(hipEventRecord(e->evJoin[b], st));
+            HIPCHK(hipStreamWaitEvent(e->stream, e->evJoin[b], 0));
+            HIPCHK(hipEventRecord(e->evJoin[b + 1], st2));
+            HIPCHK(hipStreamWaitEvent(e->stream, e->evJoin[b + 1], 0));
+            ++b;
+            continue;
+        }
         if (split || bins[b].hasExt) {
-            // tick by tick: the HHChannel2D objects (bins that have them),
-            // then the channel kernel over the bin's tiles and the solve, or
-            // one fused tick
+            // tick by tick: the bandwidth phase and the solve, or the
+            // HHChannel2D objects and one fused tick
             for (int k = 0; k < chunk; ++k) {
                 LaunchArgs a = make_args(e, bins[b].dOrder, 1, first && k == 0);
                 a.ldsBytes = bins[b].lds;
-                if (bins[b].hasExt) HIPCHK(hsg_launch_external(&a, bins[b].dTiles, bins[b].nTiles, st));
                 if (!split) {
+                    HIPCHK(hsg_launch_external(&a, bins[b].dTiles, bins[b].nTiles, st));
                     HIPCHK(hsg_launch_step(&a, nb, bins[b].threads, e->fp32, bins[b].slow, fused_fast(bins[b]), st));
                     continue;
                 }
-                if (bins[b].chanMS > 0)
-                    HIPCHK(hsg_launch_channel_fast(&a, bins[b].dTiles, bins[b].nTiles, e->fp32, bins[b].chanMS, st));
-                else
-                    HIPCHK(hsg_launch_channel(&a, bins[b].dTiles, bins[b].nTiles, e->fp32, st));
-                // the solve stops at VMid; V, Ca and the tick's outputs
-                // follow over the batch
-                a.deferD = 1;
-                if (bins[b].msg) HIPCHK(hsg_launch_msg(&a, nb, bins[b].threads, e->fp32, st));
-                else HIPCHK(hsg_launch_solve(&a, nb, bins[b].threads, e->fp32, bins[b].slow, st));
-                HIPCHK(hsg_launch_update(&a, bins[b].dTiles, bins[b].nTiles, e->fp32, st));
-                HIPCHK(hsg_launch_outputs(&a, nb, st));
+                if (int rc = split_wide(e, bins[b], a, st)) return rc;
+                if (int rc = split_solve(e, bins[b], a, st)) return rc;
             }
         } else {
             LaunchArgs a = make_args(e, bins[b].dOrder, chunk, first);
@@ -1502,6 +1560,8 @@ int hsg_create(const hsg_options* opts, hsg_engine** out) {
     for (int b = 0; b < hsg_engine::kMaxBins; ++b) {
         HIPCHK(hipStreamCreateWithFlags(&e->binStream[b], hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&e->evJoin[b], hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&e->evPairA[b], hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&e->evPairB[b], hipEventDisableTiming));
     }
     HIPCHK(hipEventCreateWithFlags(&e->evFork, hipEventDisableTiming));
     int64_t cap = e->opt.mem_budget > 0 ? e->opt.mem_budget : (int64_t)64 << 20;
@@ -1535,6 +1595,8 @@ int hsg_destroy(hsg_engine* e) {
     for (int b = 0; b < hsg_engine::kMaxBins; ++b) {
         if (e->binStream[b]) hipStreamDestroy(e->binStream[b]);
         if (e->evJoin[b]) hipEventDestroy(e->evJoin[b]);
+        if (e->evPairA[b]) hipEventDestroy(e->evPairA[b]);
+        if (e->evPairB[b]) hipEventDestroy(e->evPairB[b]);
     }
     if (e->evFork) hipEventDestroy(e->evFork);
     if (e->stream) hipStreamDestroy(e->stream);
