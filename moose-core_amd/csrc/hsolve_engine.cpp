@@ -264,6 +264,7 @@ struct hsg_engine {
         int lds = 0;
         int threads = 256;
         bool pairNext = false;      // first half of a split bin whose second half is the next bin
+        int solveThreads = 256;     // threads per cell of the split solve kernel
     };
     std::vector<Bin> bins;
     bool split = false;          // channel phase as its own kernel
@@ -1046,6 +1047,9 @@ int make_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg_
         // workgroup (wider ones measured no faster, 512 slower)
         b.threads = e->opt.block_threads > 0 ? std::max(64, std::min(256, (e->opt.block_threads + 63) / 64 * 64))
                                              : default_threads(maxN);
+        // large cells' levels hold hundreds of units: their split solve
+        // (not the message-passing one) sweeps a level with 512 threads
+        b.solveThreads = b.split && !b.msg && e->opt.block_threads <= 0 && maxN >= kWideSolveMin ? 512 : b.threads;
         HIPCHK(hipMalloc(&b.dOrder, sizeof(int32_t) * b.cells.size()));
         HIPCHK(hipMemcpyAsync(b.dOrder, b.cells.data(), sizeof(int32_t) * b.cells.size(), hipMemcpyHostToDevice,
                               e->stream));
@@ -1297,7 +1301,7 @@ int split_solve(hsg_engine* e, const hsg_engine::Bin& bin, LaunchArgs& a, hipStr
     const int nb = (int)bin.cells.size();
     a.deferD = 1;
     if (bin.msg) HIPCHK(hsg_launch_msg(&a, nb, bin.threads, e->fp32, st));
-    else HIPCHK(hsg_launch_solve(&a, nb, bin.threads, e->fp32, bin.slow, st));
+    else HIPCHK(hsg_launch_solve(&a, nb, bin.solveThreads, e->fp32, bin.slow, st));
     HIPCHK(hsg_launch_update(&a, bin.dTiles, bin.nTiles, e->fp32, st));
     HIPCHK(hsg_launch_outputs(&a, nb, st));
     return 0;

@@ -276,6 +276,7 @@ constexpr int kDescGroupMax = 5;   // groups held in their descriptor (register 
 constexpr int kAutoSplitMin = 4096;  // automatic mode: cells from this size run split
 constexpr int kChannelTile = 256;  // compartments per workgroup of the channel kernel
 constexpr int kPairMaxCells = 512;  // split bins of up to this many cells run as pipelined pairs
+constexpr int kWideSolveMin = 4096;  // split bins with a cell from this size solve with 512 threads per cell
 constexpr int kDescInts = 12;
 
 struct SpikeRec {

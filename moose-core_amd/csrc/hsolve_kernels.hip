@@ -1379,6 +1379,19 @@ __global__ void __launch_bounds__(256) hsolve_solve_kernel(LaunchArgs a) {
     else cell_steps<GT, true, false, SLOWG, 0>(a, c, lds, sTypes);
 }
 
+// The same solve with up to 512 threads per cell, for large cells, whose
+// levels hold hundreds of units each: fewer dependent HBM round trips per
+// thread and level.  (1024 threads would cap the register forms of large
+// junction groups at 128 VGPRs, where they spill.)
+template <typename GT, bool SLOWG>
+__global__ void __launch_bounds__(512) hsolve_solve_wide_kernel(LaunchArgs a) {
+    extern __shared__ double lds[];
+    __shared__ ChanType sTypes[kMaxSlots];
+    const CellDev& c = a.cells[a.order[blockIdx.x]];
+    if (c.workInLds) cell_steps<GT, true, true, SLOWG, 0>(a, c, lds, sTypes);
+    else cell_steps<GT, true, false, SLOWG, 0>(a, c, lds, sTypes);
+}
+
 // Rows in HBM (split mode): the row's destination for compartment i.
 __device__ __forceinline__ void row_dest(char* arena, const CellDev& c, int i, double*& D, double*& B) {
     if (c.msg) {
@@ -1846,6 +1859,14 @@ extern "C" hipError_t hsg_launch_step(const hsg::LaunchArgs* a, int nblocks, int
 extern "C" hipError_t hsg_launch_solve(const hsg::LaunchArgs* a, int nblocks, int threads, int fp32, int slow,
                                       hipStream_t stream) {
     const dim3 g(nblocks), b(threads);
+    if (threads > 256) {
+        if (threads > 512) return hipErrorInvalidConfiguration;
+        if (fp32 && slow) hipLaunchKernelGGL((hsg::hsolve_solve_wide_kernel<float, true>), g, b, a->ldsBytes, stream, *a);
+        else if (fp32) hipLaunchKernelGGL((hsg::hsolve_solve_wide_kernel<float, false>), g, b, a->ldsBytes, stream, *a);
+        else if (slow) hipLaunchKernelGGL((hsg::hsolve_solve_wide_kernel<double, true>), g, b, a->ldsBytes, stream, *a);
+        else hipLaunchKernelGGL((hsg::hsolve_solve_wide_kernel<double, false>), g, b, a->ldsBytes, stream, *a);
+        return hipGetLastError();
+    }
     if (fp32 && slow) hipLaunchKernelGGL((hsg::hsolve_solve_kernel<float, true>), g, b, a->ldsBytes, stream, *a);
     else if (fp32) hipLaunchKernelGGL((hsg::hsolve_solve_kernel<float, false>), g, b, a->ldsBytes, stream, *a);
     else if (slow) hipLaunchKernelGGL((hsg::hsolve_solve_kernel<double, true>), g, b, a->ldsBytes, stream, *a);

@@ -124,9 +124,11 @@ def test_launch_shapes_and_split_runs(pkg, oracle_mod, opts, chunks):
         np.testing.assert_array_equal(gpu, ref)
 
 
-@pytest.mark.parametrize("mode", ["fused", "split"])
+@pytest.mark.parametrize("mode", ["fused", "split", "auto"])
 def test_large_cells_global_work(pkg, oracle_mod, mode):
-    """n = 2000 (work arrays in LDS) and n = 6000 (work arrays in HBM)."""
+    """n = 2000 (work arrays in LDS) and n = 6000 (work arrays in HBM);
+    in automatic mode the 6000-compartment cell runs split, its solve with
+    512 threads per cell."""
     print(f"case: large cells 2000/6000 compartments, 100 steps, fp64, {mode}", flush=True)
     rng = np.random.default_rng(9)
     specs = [workloads.random_hh_cell(rng, 2000, inject=2e-9), workloads.random_ca_cell(rng, 6000, inject=2e-9)]
