@@ -695,6 +695,108 @@ class SimpleSynHandler(Neutral):
         raise RuntimeError(f"{self.path}: its SynChan is not under an HSolve")
 
 
+class TimeTable(Neutral):
+    """A TimeTable (builtins/TimeTable.cpp): a list of spike times sent on
+    eventOut, typically to a Synapse's addSpike.  On each of its clock ticks
+    (8 by default, TimeTable.cpp:181-205) it sends the next listed time when
+    the tick's time has reached it, at most one time per tick, stamped with
+    the listed time (not the tick's).  Under a solver the sends are worked
+    out on the host before each run and queued on the synapses' handlers."""
+
+    def _init_fields(self):
+        self.__dict__.update(vector=np.zeros(0), filename="", method=4, state=0.0, tick=8, _pos=0)
+
+    def _set_local(self, name, value):
+        d = self.__dict__
+        if name == "vector":
+            d["vector"] = np.asarray(value, dtype=np.float64).reshape(-1).copy()
+        elif name == "filename":
+            # TimeTable::setFilename: whitespace-separated numbers up to the
+            # first token that is not one
+            vals = []
+            with open(value) as f:
+                for tok in f.read().split():
+                    try:
+                        vals.append(float(tok))
+                    except ValueError:
+                        break
+            d["filename"] = str(value)
+            d["vector"] = np.array(vals, dtype=np.float64)
+        elif name == "method":
+            if int(value) != 4:
+                raise ValueError("TimeTable: only method 4 (loading from file) is supported")
+            d["method"] = 4
+        elif name == "state":
+            raise AttributeError("TimeTable.state is read-only")
+        else:
+            object.__setattr__(self, name, value)
+
+    def clearVec(self):
+        self.__dict__["vector"] = np.zeros(0)
+
+
+def _timetable_sends(vec, pos, state, every, dt, s0, nsteps):
+    """TimeTable::process (builtins/TimeTable.cpp:187-205) on the ticks of
+    one run: the tick whose time is dt * u runs when u is a multiple of
+    `every`, for s0 < u <= s0 + nsteps.  Returns the sends as (listed time,
+    u), the position after the run and the state after its last tick."""
+    sends = []
+    u = (s0 // every + 1) * every
+    last = (s0 + nsteps) // every * every
+    if u > last:
+        return sends, pos, state
+    while pos < len(vec) and u <= last:
+        tau = float(vec[pos])
+        if tau > dt * u:
+            # the first tick whose time reaches the listed time
+            k = max(u, int(math.ceil(tau / dt / every)) * every)
+            while k - every >= u and dt * (k - every) >= tau:
+                k -= every
+            while dt * k < tau:
+                k += every
+            u = k
+            if u > last:
+                break
+        sends.append((tau, u))
+        pos += 1
+        u += every
+    state = 1.0 if sends and sends[-1][1] == last else 0.0
+    return sends, pos, state
+
+
+def _feed_timetables(group, dt, nsteps):
+    """TimeTable eventOut -> Synapse addSpike messages into the solvers of
+    one engine: the events the TimeTables send during the next `nsteps`
+    steps, queued on the handlers before the run.  A send on the tick of
+    step u is seen by the handler from step u + 1 on.  Queued early, an event
+    (t + delay) later than the tick that sends it pops on the same tick as in
+    the reference; an earlier one is given the time of the middle of step
+    u + 1, where the reference's handler first sees it."""
+    ours = {id(hs) for hs in group}
+    for tt in [o for o in _objects.values() if isinstance(o, TimeTable)]:
+        dests = []
+        for syn in _targets(tt, "eventOut", "Synapse"):
+            try:
+                hs, k = syn.handler._solver_of()
+            except RuntimeError:
+                continue
+            if id(hs) in ours:
+                dests.append((hs, k, syn))
+        if not dests:
+            continue
+        tdt = _clock_dt.get(tt.tick, 0.0)
+        every = int(round(tdt / dt)) if tdt > 0 else 0
+        if every < 1 or abs(every * dt - tdt) > 1e-9 * tdt:
+            raise ValueError(f"{tt.path}: its tick's dt ({tdt}) is not a multiple of the solver dt ({dt})")
+        s0 = dests[0][0].__dict__.get("_steps", 0)
+        d = tt.__dict__
+        sends, d["_pos"], d["state"] = _timetable_sends(d["vector"], d["_pos"], d["state"], every, dt, s0, nsteps)
+        for tau, u in sends:
+            for hs, k, syn in dests:
+                when = tau if tau + syn.delay > dt * u else dt * (u + 0.5) - syn.delay
+                hs._engine.add_spike(hs._cell, k, syn.index, when)
+
+
 class SynChan(Neutral):
     """A SynChan (biophysics/SynChan.cpp): dual-exponential conductance
     (tau1, tau2; normalised so the peak of a unit activation is Gbar),
@@ -1497,9 +1599,9 @@ def getClock(tick):
 
 
 def useClock(tick, path, field="process"):
-    """Puts the Table(s) at `path` on clock `tick`."""
+    """Puts the Table or TimeTable at `path` on clock `tick`."""
     obj = element(path)
-    if isinstance(obj, Table):
+    if isinstance(obj, (Table, TimeTable)):
         obj.__dict__["tick"] = int(tick)
 
 
@@ -1543,6 +1645,8 @@ def reinit():
     for o in list(_objects.values()):
         if isinstance(o, Leakage):
             o.__dict__["Gk"] = o.__dict__["Gbar"] * o.__dict__["modulation"]
+        if isinstance(o, TimeTable):
+            o.__dict__.update(_pos=0, state=0.0)
     _wire_spikes()
     done = set()
     for hs in _solvers:
@@ -1597,6 +1701,7 @@ def start(runtime):
             hs._push_pulsegens()
             hs._push_leaks()
             hs._push_synapses()
+        _feed_timetables(group, dt, nsteps)
         # tables of other fields: stepped in stretches ending on their
         # sampling steps, read by the host
         ftabs = [(hs, r) for hs in group for r in hs._field_recorders(dt)]

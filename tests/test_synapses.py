@@ -15,6 +15,9 @@ GPU tests compare the HIP path (through the C ABI) with the oracle: FP64
 within 1e-6 V with identical spike steps, mixed within 0.1 mV and one step;
 networks against oracle cells stepped one tick at a time, each spike at
 t = dt * step delivered to its targets as the event (t + delay, weight).
+TimeTable sources (builtins/TimeTable.cpp:187-205): the host's send plan
+against a tick-by-tick restatement on the CPU, and the object layer against
+an oracle cell that receives each send after the tick that makes it.
 """
 import math
 
@@ -590,3 +593,110 @@ def test_gpu_probes_of_other_fields(pkg, oracle_mod, mode):
     want = np.array(want)
     assert got.shape == want.shape
     np.testing.assert_allclose(got, want, rtol=1e-6, atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# TimeTable sources (builtins/TimeTable.cpp)
+# ---------------------------------------------------------------------------
+def timetable_ticks(vec, every, dt, nsteps):
+    """Brute force: TimeTable::process on every step whose time is a
+    multiple of the tick's, from the start."""
+    pos, sends, states = 0, [], []
+    for u in range(1, nsteps + 1):
+        if u % every:
+            continue
+        state = 0.0
+        if pos < len(vec) and dt * u >= vec[pos]:
+            sends.append((float(vec[pos]), u))
+            pos += 1
+            state = 1.0
+        states.append(state)
+    return sends, pos, (states[-1] if states else 0.0)
+
+
+@pytest.mark.parametrize("every", [1, 4])
+def test_timetable_sends_match_tick_by_tick(every):
+    from moose_core_amd.moose import _timetable_sends
+    dt = 25e-6
+    # two times inside one tick (the second waits a tick), one on a tick,
+    # time 0, and a late one
+    vec = np.array([0.0, 1e-3, 1.01e-3, 1.02e-3, dt * 80, 3.3e-3, 9.0e-3])
+    nsteps = 300
+    want = timetable_ticks(vec, every, dt, nsteps)
+    assert _timetable_sends(vec, 0, 0.0, every, dt, 0, nsteps) == want
+    # the same run in pieces (a piece may hold no tick of the TimeTable)
+    got, pos, state, s0 = [], 0, 0.0, 0
+    for piece in (3, 1, 77, 2, 150, 67):
+        sends, pos, state = _timetable_sends(vec, pos, state, every, dt, s0, piece)
+        got += sends
+        s0 += piece
+    assert (got, pos, state) == want
+    # time 0 goes out on the first tick; the three times inside one tick
+    # leave on consecutive ticks
+    assert want[0][0] == (0.0, every)
+    u1 = [u for t, u in want[0] if t in (1e-3, 1.01e-3, 1.02e-3)]
+    assert u1[1] - u1[0] == every and u1[2] - u1[1] == every
+    # at most one send per tick, never before its listed time
+    us = [u for _, u in want[0]]
+    assert len(set(us)) == len(us)
+    assert all(dt * u >= t for t, u in want[0])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tt_dt", [25e-6, 1e-4])
+def test_gpu_layer_timetable_to_synapses(pkg, oracle_mod, tt_dt):
+    """TimeTable eventOut -> Synapse addSpike through the object layer
+    (delays 0, 0.04 ms and 3 ms), against an oracle cell stepped one tick at
+    a time, the TimeTable's sends applied after the tick that makes them."""
+    print(f"case: object layer, TimeTable -> 3 synapses, TimeTable dt {tt_dt}, fp64", flush=True)
+    from moose_core_amd import moose
+    moose.reset()
+    moose.Neutral("/m")
+    c = moose.Compartment("/m/soma")
+    c.Rm, c.Cm, c.Em, c.initVm, c.Vm = 1e9, 1e-11, -0.065, -0.065, -0.065
+    syn = moose.SynChan("/m/soma/syn")
+    syn.Gbar, syn.Ek, syn.tau1, syn.tau2 = 5e-9, 0.0, 0.5e-3, 2e-3
+    moose.connect(syn, "channel", c, "channel")
+    h = moose.SimpleSynHandler("/m/soma/syn/handler")
+    h.numSynapses = 3
+    moose.connect(h, "activationOut", syn, "activation")
+    delays = (0.0, 0.04e-3, 3e-3)
+    weights = (1.0, 0.5, 2.0)
+    for y, dly, w in zip(h.synapse, delays, weights):
+        y.delay, y.weight = dly, w
+    tt = moose.TimeTable("/m/tt")
+    times = [0.0, 2e-3, 2.01e-3, 2.02e-3, 5e-3, 5.5e-3, 11.3e-3]
+    tt.vector = times
+    moose.setClock(tt.tick, tt_dt)
+    for y in h.synapse:
+        moose.connect(tt, "eventOut", y, "addSpike")
+    tab = moose.Table("/m/vm")
+    moose.connect(tab, "requestOut", c, "getVm")
+    moose.useClock(7, "/m/vm")
+    dt = 25e-6
+    moose.setClock(7, dt)
+    hs = moose.HSolve("/m/hsolve")
+    hs.dt = dt
+    hs.target = "/m"
+    import copy
+    spec = copy.deepcopy(hs._spec)
+    moose.reinit()
+    moose.start(0.006)
+    moose.start(0.014)
+    nsteps = int(round(0.02 / dt))
+    every = int(round(tt_dt / dt))
+    ref = oracle_mod.OracleCell(spec, dt)
+    ref.reinit(dt)
+    pos, tr = 0, []
+    for u in range(1, nsteps + 1):
+        tr.append(ref.run(dt, 1, [0])[0, 0])
+        if u % every == 0 and pos < len(times) and dt * u >= times[pos]:
+            for j in range(3):
+                ref.add_spike(0, j, times[pos])
+            pos += 1
+    v = tab.vector[1:]
+    assert len(v) == nsteps
+    assert np.abs(v - np.array(tr)).max() <= FP64_TOL
+    assert np.ptp(tr) > 1e-3          # the synaptic input moved Vm
+    assert pos == len(times)
+    moose.reset()
