@@ -657,10 +657,22 @@ class Synapse:
     def isA(self, name):
         return name == "Synapse"
 
+    def _target(self):
+        """(solver, SynChan index, synapse index within the SynChan): the
+        SynChan's synapses are those of its handlers in message order."""
+        hs, k = self.handler._solver_of()
+        sc = hs._objs["synchans"][k]
+        off = 0
+        for h in _sources(sc, "activation", "SimpleSynHandler"):
+            if h is self.handler:
+                return hs, k, off + self.index
+            off += len(h.synapse)
+        raise RuntimeError(f"{self.path}: its handler does not feed the SynChan")
+
     def addSpike(self, time):
         """Synapse::addSpike from the host (a spike at `time` seconds)."""
-        hs, k = self.handler._solver_of()
-        hs._engine.add_spike(hs._cell, k, self.index, float(time))
+        hs, k, j = self._target()
+        hs._engine.add_spike(hs._cell, k, j, float(time))
 
 
 class SimpleSynHandler(Neutral):
@@ -777,11 +789,11 @@ def _feed_timetables(group, dt, nsteps):
         dests = []
         for syn in _targets(tt, "eventOut", "Synapse"):
             try:
-                hs, k = syn.handler._solver_of()
+                hs, k, j = syn._target()
             except RuntimeError:
                 continue
             if id(hs) in ours:
-                dests.append((hs, k, syn))
+                dests.append((hs, k, j, syn))
         if not dests:
             continue
         tdt = _clock_dt.get(tt.tick, 0.0)
@@ -792,9 +804,9 @@ def _feed_timetables(group, dt, nsteps):
         d = tt.__dict__
         sends, d["_pos"], d["state"] = _timetable_sends(d["vector"], d["_pos"], d["state"], every, dt, s0, nsteps)
         for tau, u in sends:
-            for hs, k, syn in dests:
+            for hs, k, j, syn in dests:
                 when = tau if tau + syn.delay > dt * u else dt * (u + 0.5) - syn.delay
-                hs._engine.add_spike(hs._cell, k, syn.index, when)
+                hs._engine.add_spike(hs._cell, k, j, when)
 
 
 class SynChan(Neutral):
@@ -1077,14 +1089,13 @@ class HSolve(Neutral):
         for k, rec in enumerate(leaks):
             lrec[k] = rec
         # SynChans on the compartments, in message order, each with the
-        # synapses of the handler feeding it
+        # synapses of the handlers feeding it (handlers in message order: one
+        # queue holds their events, so several handlers' activations reach
+        # the SynChan as one sum per tick)
         syns, synrec, synapses = [], [], []
         for c in comps:
             for sc in _channels(c, "SynChan"):
-                hs_ = _sources(sc, "activation", "SimpleSynHandler")
-                if len(hs_) > 1:
-                    raise ValueError(f"{sc.path}: more than one SynHandler feeds it")
-                syn = hs_[0].synapse if hs_ else []
+                syn = [y for h in _sources(sc, "activation", "SimpleSynHandler") for y in h.synapse]
                 d = sc.__dict__
                 synrec.append((cidx[c.id], len(syn), d["Gbar"], d["Ek"], d["tau1"], d["tau2"], d["modulation"]))
                 synapses += [(y.weight, y.delay) for y in syn]
@@ -1171,8 +1182,7 @@ class HSolve(Neutral):
         sent = self._spec.synapses
         j = 0
         for sc in self._objs["synchans"]:
-            hs_ = _sources(sc, "activation", "SimpleSynHandler")
-            for y in (hs_[0].synapse if hs_ else []):
+            for y in [y for h in _sources(sc, "activation", "SimpleSynHandler") for y in h.synapse]:
                 for f, name in (("weight", "weight"), ("delay", "delay")):
                     v = getattr(y, f)
                     if v != sent[j][f]:
@@ -1622,7 +1632,7 @@ def _wire_spikes():
         if sf != "spikeOut" or df != "addSpike" or not isinstance(d, Synapse):
             continue
         try:
-            hs_dst, k = d.handler._solver_of()
+            hs_dst, k, j = d._target()
         except RuntimeError:
             continue
         srcs = where.get(s.id)
@@ -1634,7 +1644,7 @@ def _wire_spikes():
             key = (s.id, d.id, hs_src._cell, hs_dst._cell)
             wired = hs_src._engine.__dict__.setdefault("_wired", set())
             if key not in wired:
-                hs_src._engine.connect_spike(hs_src._cell, g, hs_dst._cell, k, d.index)
+                hs_src._engine.connect_spike(hs_src._cell, g, hs_dst._cell, k, j)
                 wired.add(key)
 
 

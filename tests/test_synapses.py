@@ -700,3 +700,82 @@ def test_gpu_layer_timetable_to_synapses(pkg, oracle_mod, tt_dt):
     assert np.ptp(tr) > 1e-3          # the synaptic input moved Vm
     assert pos == len(times)
     moose.reset()
+
+
+def two_handler_model(moose):
+    """One compartment whose SynChan is fed by two SimpleSynHandlers (2 and 1
+    synapses); a TimeTable drives the second handler's synapse and the
+    first handler's second synapse."""
+    moose.reset()
+    moose.Neutral("/m")
+    c = moose.Compartment("/m/soma")
+    c.Rm, c.Cm, c.Em, c.initVm, c.Vm = 1e9, 1e-11, -0.065, -0.065, -0.065
+    syn = moose.SynChan("/m/soma/syn")
+    syn.Gbar, syn.Ek, syn.tau1, syn.tau2 = 5e-9, 0.0, 0.5e-3, 2e-3
+    moose.connect(syn, "channel", c, "channel")
+    hs_ = []
+    for name, n, pars in (("h1", 2, ((0.5e-3, 1.0), (1e-3, 0.5))), ("h2", 1, ((0.0, 2.0),))):
+        h = moose.SimpleSynHandler(f"/m/soma/syn/{name}")
+        h.numSynapses = n
+        moose.connect(h, "activationOut", syn, "activation")
+        for y, (dly, w) in zip(h.synapse, pars):
+            y.delay, y.weight = dly, w
+        hs_.append(h)
+    tt = moose.TimeTable("/m/tt")
+    tt.vector = [1e-3, 3e-3, 3.05e-3, 7e-3]
+    moose.connect(tt, "eventOut", hs_[1].synapse[0], "addSpike")
+    moose.connect(tt, "eventOut", hs_[0].synapse[1], "addSpike")
+    return c, hs_, tt
+
+
+def test_layer_synchan_with_two_handlers(pkg):
+    from moose_core_amd import moose
+    c, (h1, h2), _ = two_handler_model(moose)
+    hs = moose.HSolve("/m/hsolve")
+    hs.dt = 25e-6
+    spec, objs = hs.build_spec(moose._deep_search_compartment(moose.element("/m")))
+    assert spec.synchans[0]["n_synapses"] == 3
+    np.testing.assert_array_equal(spec.synapses["weight"], [1.0, 0.5, 2.0])
+    np.testing.assert_array_equal(spec.synapses["delay"], [0.5e-3, 1e-3, 0.0])
+    moose.reset()
+
+
+@pytest.mark.gpu
+def test_gpu_layer_synchan_with_two_handlers(pkg, oracle_mod):
+    """Two handlers on one SynChan through the object layer (a TimeTable and
+    a host addSpike as sources) against an oracle cell holding the handlers'
+    synapses in message order as one list.  The reference adds the two
+    handlers' sums to the SynChan's activation one after the other; one
+    queue sums all due events in time order, which can differ in the last
+    bit of the activation, well inside the FP64 bound."""
+    print("case: object layer, SynChan fed by 2 SimpleSynHandlers, fp64", flush=True)
+    from moose_core_amd import moose
+    c, (h1, h2), tt = two_handler_model(moose)
+    tab = moose.Table("/m/vm")
+    moose.connect(tab, "requestOut", c, "getVm")
+    dt = 25e-6
+    moose.setClock(8, dt)
+    hs = moose.HSolve("/m/hsolve")
+    hs.dt = dt
+    hs.target = "/m"
+    import copy
+    spec = copy.deepcopy(hs._spec)
+    moose.reinit()
+    h1.synapse[0].addSpike(4e-3)
+    moose.start(0.012)
+    nsteps = int(round(0.012 / dt))
+    ref = oracle_mod.OracleCell(spec, dt)
+    ref.reinit(dt)
+    ref.add_spike(0, 0, 4e-3)
+    times, pos, tr = list(tt.vector), 0, []
+    for u in range(1, nsteps + 1):
+        tr.append(ref.run(dt, 1, [0])[0, 0])
+        if pos < len(times) and dt * u >= times[pos]:
+            ref.add_spike(0, 2, times[pos])     # h2.synapse[0]
+            ref.add_spike(0, 1, times[pos])     # h1.synapse[1]
+            pos += 1
+    v = tab.vector[1:]
+    assert len(v) == nsteps
+    assert np.abs(v - np.array(tr)).max() <= FP64_TOL
+    assert np.ptp(tr) > 1e-3
+    moose.reset()
