@@ -2,7 +2,7 @@
 
 FP64 form: bit-for-bit equal to the oracle (which is itself bit-exact with the
 reference's hsolve sources), i.e. max|dVm| = 0 and identical spike steps; the
-north-star bound is max|dVm| <= 1e-6 V with identical spike steps.
+required bound is max|dVm| <= 1e-6 V with identical spike steps.
 Mixed form (FP32 gates/tables/currents, FP64 Hines): max|dVm| <= 1e-4 V
 (0.1 mV) and spike steps within one step of the oracle's.
 """
@@ -13,7 +13,7 @@ from moose_core_amd import workloads
 
 pytestmark = pytest.mark.gpu
 
-FP64_TOL = 1e-6     # V, north-star bound for the FP64 form (observed: 0)
+FP64_TOL = 1e-6     # V, required bound for the FP64 form (observed: 0)
 MIXED_TOL = 1e-4    # V, 0.1 mV
 
 
