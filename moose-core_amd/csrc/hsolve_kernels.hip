@@ -1380,9 +1380,9 @@ __global__ void __launch_bounds__(256) hsolve_solve_kernel(LaunchArgs a) {
 }
 
 // The same solve with up to 512 threads per cell, for large cells, whose
-// levels hold hundreds of units each: fewer dependent HBM round trips per
-// thread and level.  (1024 threads would cap the register forms of large
-// junction groups at 128 VGPRs, where they spill.)
+// levels hold hundreds of units each, spread over twice as many threads.
+// (1024 threads would cap the register forms of large junction groups at
+// 128 VGPRs, where they spill.)
 template <typename GT, bool SLOWG>
 __global__ void __launch_bounds__(512) hsolve_solve_wide_kernel(LaunchArgs a) {
     extern __shared__ double lds[];
