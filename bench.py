@@ -11,12 +11,20 @@ Timed: `--steps` ticks of the whole ensemble (default 5000, the full run),
 after `--warmup` untimed ticks, bracketed by device synchronisation.  The
 headline `value` is the mixed-precision form (FP32 gates/tables/currents, FP64
 Hines solve and Vm); the FP64 form (bit-identical to the oracle) is run the
-same way and reported beside it, with the accuracy of the mixed form against
-it (soma Vm of every cell, every step).
+same way and reported beside it.
+
+Accuracy of the mixed form against the FP64 form comes from a separate,
+untimed run of `--accuracy-steps` ticks (default: the workload's full length)
+after a fresh reinit of both engines, whatever `--steps` is: the soma Vm of
+every cell at every step and every spike step.
+
+The default line also carries a short serving leg (Poisson arrivals of Ca
+cells with the 2-D K-AHP, continuous batching): p50/p99 request latency and
+sustained compartment-steps/s (`--serving-requests 0` leaves it out).
 
 The CPU baseline is the oracle (the bit-exact C restatement of the reference
 HSolve) on a bounded sample of the same cells, one cell after another on one
-core and sharded over all host cores.
+core and sharded over the GPU's share of host cores.
 """
 import argparse
 import json
@@ -35,15 +43,27 @@ pkg = __graft_entry__.import_package()
 from moose_core_amd import workloads  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes per compartment-step (state-resident between steps,
-# tables excluded): V r+w 16, gate states r+w 2*w*3, Gbar/Ek 2*w*2,
-# CmByDt/EmByRm/diag/offdiag 32 (FP64), w = 8 (FP64) or 4 (mixed)
-BYTES_PER_CS = {"fp64": 16 + 2 * 8 * 3 + 2 * 8 * 2 + 32, "mixed": 16 + 2 * 4 * 3 + 2 * 4 * 2 + 32}
-# the large Ca cells: six gates and four HHChannels, a CaConc (Ca r+w, its
-# activation r+w, basal 8: 40) and an HHChannel2D (its 40-byte instance
-# record, Z r+w, Gk w, the external current w then r: 96) per compartment
-BYTES_PER_CS_LARGE = {"fp64": 16 + 2 * 8 * 6 + 2 * 8 * 4 + 32 + 40 + 96,
-                      "mixed": 16 + 2 * 4 * 6 + 2 * 4 * 4 + 32 + 40 + 96}
+
+
+def bytes_per_cs(w_gate, w_2d, gates, chans, gates2d=0, chans2d=0, pools=0):
+    """Algorithmic bytes per compartment-step: the compulsory traffic with
+    the state resident between steps, lookup tables and elimination
+    temporaries excluded (they stay in LDS/L2).  V read+write 16; every gate
+    state read+written (w bytes each); Gbar and Ek of every channel read (w
+    each); CmByDt, EmByRm, the diagonal base and the off-diagonal (FP64) 32;
+    per Ca pool its state read+written, f1 and f2 read (FP64) 32.  w = 8 in
+    the FP64 form, 4 for the HHChannels' states and parameters in the mixed
+    form; the HHChannel2D objects stay FP64 in both (w_2d = 8)."""
+    return (16 + 2 * w_gate * gates + 2 * w_gate * chans + 2 * w_2d * gates2d + 2 * w_2d * chans2d
+            + 32 + 32 * pools)
+
+
+# squid HH cells: Na m^3 h and K n^4 (3 gates, 2 channels) -> 128 / 88 B
+BYTES_PER_CS = {"fp64": bytes_per_cs(8, 8, 3, 2), "mixed": bytes_per_cs(4, 8, 3, 2)}
+# the large Ca cells: Na m^3 h, Kdr n^4, KA a^3 b, CaL s^2 (6 gates, 4
+# channels), the 2-D K-AHP (1 gate, 1 channel, FP64 in both forms) and one
+# CaConc per compartment -> 272 / 192 B
+BYTES_PER_CS_LARGE = {"fp64": bytes_per_cs(8, 8, 6, 4, 1, 1, 1), "mixed": bytes_per_cs(4, 8, 6, 4, 1, 1, 1)}
 
 
 def log(msg):
@@ -120,7 +140,11 @@ WORKLOADS = {
 }
 
 
-def run_form(precision, packed, dt, steps, warmup, spl, graph, threads, mode="fused", conns=()):
+def run_form(precision, packed, dt, steps, warmup, spl, graph, threads, mode="fused", conns=(), acc_steps=0,
+             plans=None):
+    """Times `steps` ticks after `warmup` untimed ones; then, untimed, a
+    fresh reinit and `acc_steps` ticks recording the soma Vm of every cell
+    and the spikes (the accuracy leg)."""
     eng = pkg.Engine(precision, steps_per_launch=spl, graph_chunk=graph, block_threads=threads,
                      kernel_mode=mode)
     t = time.time()
@@ -129,7 +153,7 @@ def run_form(precision, packed, dt, steps, warmup, spl, graph, threads, mode="fu
         eng.connect_spike(ids[a], g, ids[b], k, j)
     log(f"[bench] {precision}: {len(ids)} cells set up in {time.time() - t:.1f}s")
     for cid in ids:
-        eng.set_probes(cid, [0], steps)
+        eng.set_probes(cid, [0], max(steps, acc_steps))
     eng.reinit(dt)
     if warmup:
         eng.step(warmup)
@@ -142,22 +166,37 @@ def run_form(precision, packed, dt, steps, warmup, spl, graph, threads, mode="fu
     eng.sync()
     wall = time.perf_counter() - t0
     launches, total_ms, mean_ms = eng.last_timing()
-    soma = np.stack([eng.read_probes(cid)[:, 0] for cid in ids], axis=1)
-    spikes = eng.read_spikes()
-    eng.close()
+    if plans is not None:
+        plans[precision] = eng.plan_info()
     log(f"[bench] {precision}: {steps} steps in {wall:.3f}s, {launches} launches, mean launch {mean_ms:.3f} ms")
+    soma = spikes = None
+    if acc_steps:
+        for cid in ids:
+            eng.read_probes(cid)
+        eng.read_spikes()
+        eng.reinit(dt)
+        t1 = time.perf_counter()
+        eng.step(acc_steps)
+        eng.sync()
+        log(f"[bench] {precision}: accuracy leg, {acc_steps} steps from reinit in {time.perf_counter() - t1:.3f}s")
+        soma = np.stack([eng.read_probes(cid)[:, 0] for cid in ids], axis=1)
+        spikes = eng.read_spikes()
+    eng.close()
     return wall, launches, total_ms, mean_ms, soma, spikes
 
 
 def measured_traffic(key, compartment_steps):
-    """HBM bytes per launch from the committed FETCH_SIZE/WRITE_SIZE profile
-    of this kernel and workload (profiles/traffic.json), or None."""
+    """HBM bytes per launch round from the committed FETCH_SIZE/WRITE_SIZE
+    profile of this workload and form (profiles/traffic.json: rocprofv3 PMC
+    runs, scaled by a plain device copy of known size), and the profile it
+    comes from; (None, None) when no profile of this workload exists."""
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
             t = json.load(f)[key]
     except (OSError, KeyError, ValueError):
-        return None
-    return (t["fetch_bytes_per_compartment_step"] + t["write_bytes_per_compartment_step"]) * compartment_steps
+        return None, None
+    per_cs = t["fetch_bytes_per_compartment_step"] + t["write_bytes_per_compartment_step"]
+    return per_cs * compartment_steps, f"{t['source']} ({per_cs:.1f} B per compartment-step)"
 
 
 def spike_steps(spikes, ncells):
@@ -167,12 +206,19 @@ def spike_steps(spikes, ncells):
     return out
 
 
+# host cores timed beside one GPU: the box's CPU share of one GPU
+HOST_CORES_PER_GPU = 16
+
+
 def cpu_baseline(cells, dt, budget_s=15.0):
-    """Oracle on the host: single core (cells one after another) and all
-    cores (cells sharded over threads; ctypes releases the GIL)."""
+    """Oracle on the host: single core (cells one after another) and the
+    GPU's share of host cores (cells sharded over threads; ctypes releases
+    the GIL).  The sample is every 16th cell of the workload (sizes spread
+    like the whole), stepped for as many ticks as ~budget_s seconds of one
+    core's work allow."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    sample = cells[:64]
+    sample = cells[::16]
     probe_steps = 20
     # calibrate: compartment-steps/s on one core
     c0 = oracle.OracleCell(sample[0], dt)
@@ -191,7 +237,9 @@ def cpu_baseline(cells, dt, budget_s=15.0):
     for o in objs:
         o.run(dt, nsteps)
     single = sum(c.n for c in sample) * nsteps / (time.perf_counter() - t)
-    ncores = max(1, min(16, os.cpu_count() or 1))   # the host share of one GPU
+    ncores = max(1, min(HOST_CORES_PER_GPU, os.cpu_count() or 1))
+    for o in objs:
+        o.reinit(dt)
     chunks = [objs[i::ncores] for i in range(ncores)]
     ths = [threading.Thread(target=lambda ch: [o.run(dt, nsteps) for o in ch], args=(ch,)) for ch in chunks]
     t = time.perf_counter()
@@ -203,40 +251,56 @@ def cpu_baseline(cells, dt, budget_s=15.0):
     return {
         "value": multi, "unit": "compartment-steps/s", "cores": ncores, "kind": "port",
         "single_core_value": single,
-        "sample": f"{len(sample)} of the ensemble's cells ({sum(c.n for c in sample)} compartments) "
-                  f"x {nsteps} steps, oracle (bit-exact C restatement of HSolve), -O2 -ffp-contract=off",
+        "sample": f"every 16th cell of the workload ({len(sample)} cells, {sum(c.n for c in sample)} "
+                  f"compartments) x {nsteps} steps, oracle (bit-exact C restatement of HSolve), "
+                  f"-O3 -ffp-contract=off, {ncores} threads (one GPU's share of the host)",
     }
 
 
-def run_serving(args):
-    """Poisson arrivals of simulation requests into one engine (continuous
-    batching, admission by device-memory footprint): each request a seeded
-    random Ca cell with a 2-D K-AHP (the large-cell channel set) of n ~
-    log-U[100, 30000] compartments, for a duration log-U[10 ms, 1 s] at
-    dt = 25 us, mixed precision.  Prints one JSON line with the p50/p99
-    request latency (arrival to result) and the sustained throughput."""
+def serving_requests(rng, count, rate, dt):
+    """`count` requests with Poisson arrivals at `rate` per second: a seeded
+    random Ca cell with the 2-D K-AHP of n ~ log-U[100, 30000] compartments
+    for a duration log-U[10 ms, 1 s] at dt."""
     from moose_core_amd import serving
-    dt = 25e-6
-    rng = np.random.default_rng(args.seed)
-    count = args.cells
-    t = time.time()
     reqs, arrival = [], 0.0
     for k in range(count):
-        arrival += rng.exponential(1.0 / args.rate)
+        arrival += rng.exponential(1.0 / rate)
         n = int(round(np.exp(rng.uniform(np.log(100), np.log(30000)))))
         steps = int(round(np.exp(rng.uniform(np.log(0.010 / dt), np.log(1.0 / dt)))))
         cell = workloads.random_ca_cell(rng, n, kahp2d=True).pack()
         reqs.append(serving.Request(k, cell, steps, arrival))
+    return reqs
+
+
+def serving_leg(count, rate, seed, quantum=100, mem_budget_gib=64, kernel_mode="auto"):
+    """The serving loop (continuous batching, admission by device-memory
+    footprint) over `count` Poisson requests, mixed precision, dt = 25 us.
+    Returns the summary (p50/p99 latency, sustained compartment-steps/s)."""
+    from moose_core_amd import serving
+    dt = 25e-6
+    rng = np.random.default_rng(seed)
+    t = time.time()
+    reqs = serving_requests(rng, count, rate, dt)
     log(f"[bench] serving: {count} requests built in {time.time() - t:.1f}s "
         f"({sum(r.spec.n for r in reqs)} compartments, {sum(r.nsteps for r in reqs)} request-steps)")
-    srv = serving.Server("mixed", dt, mem_budget=args.mem_budget << 30, quantum=args.quantum,
-                         kernel_mode=args.kernel_mode)
+    srv = serving.Server("mixed", dt, mem_budget=mem_budget_gib << 30, quantum=quantum, kernel_mode=kernel_mode)
     # warm-up: one small request through the loop (kernels loaded, arena grown)
     srv.run([serving.Request(-1, workloads.random_ca_cell(rng, 200, kahp2d=True).pack(), 200, 0.0)],
             realtime=False)
     res = srv.run(reqs, realtime=True)
     summ = serving.summarize(res, srv.wall, srv.cs)
+    summ.update({"rate_per_s": rate, "quantum_steps": quantum, "mem_budget_gib": mem_budget_gib,
+                 "max_active": srv.max_active, "quanta": srv.quanta})
     srv.close()
+    log(f"[bench] serving: p50 {summ['p50_latency_s']:.2f}s p99 {summ['p99_latency_s']:.2f}s, "
+        f"{summ['sustained_cs_per_s']:.3e} compartment-steps/s")
+    return summ
+
+
+def run_serving(args):
+    """The serving workload on its own: one JSON line with the p50/p99
+    request latency (arrival to result) and the sustained throughput."""
+    summ = serving_leg(args.cells, args.rate, args.seed, args.quantum, args.mem_budget, args.kernel_mode)
     out = {
         "metric": "serving: sustained compartment-steps/s with p50/p99 request latency",
         "value": summ["sustained_cs_per_s"], "unit": "compartment-steps/s",
@@ -245,8 +309,8 @@ def run_serving(args):
         "dtype": "f32-channels/f64-hines", "data": "synthetic (seeded random morphologies, Poisson arrivals)",
         "config": {"workload": "serving loop: Poisson arrivals, n~logU[100,30000] Ca cells with 2-D K-AHP, "
                                "duration logU[10ms,1s], dt=25us, mixed precision",
-                   "requests": count, "rate_per_s": args.rate, "quantum_steps": args.quantum,
-                   "mem_budget_gib": args.mem_budget, "max_active": srv.max_active, "quanta": srv.quanta},
+                   "requests": args.cells, "rate_per_s": args.rate, "quantum_steps": args.quantum,
+                   "mem_budget_gib": args.mem_budget, "max_active": summ["max_active"], "quanta": summ["quanta"]},
         "p50_latency_s": summ["p50_latency_s"], "p99_latency_s": summ["p99_latency_s"],
         "wall_s": summ["wall_s"], "compartment_steps": summ["compartment_steps"],
     }
@@ -265,6 +329,36 @@ def copy_probe(mib):
     print(json.dumps({"copy_probe_bytes": n * 4, "launches": 5}))
 
 
+def accuracy_block(soma, soma64, spikes, spk64, ncells, steps):
+    """Mixed form against the FP64 form over the accuracy leg: per cell the
+    max |dVm| of the soma trace, and spike steps within one step."""
+    d = soma - soma64
+    s_mixed = spike_steps(spikes, ncells)
+    s_64 = spike_steps(spk64, ncells)
+    worst = 0
+    mismatched = 0
+    for a, b in zip(s_mixed, s_64):
+        if len(a) != len(b):
+            mismatched += 1
+            continue
+        if a:
+            worst = max(worst, int(np.max(np.abs(np.array(a) - np.array(b)))))
+    per_cell = np.abs(d).max(axis=0)
+    spike_ok = np.array([len(a) == len(b) and (not a or np.max(np.abs(np.array(a) - np.array(b))) <= 1)
+                         for a, b in zip(s_mixed, s_64)])
+    bad = np.nonzero((per_cell > 1e-4) | ~spike_ok)[0]
+    return {
+        "steps": steps, "max_abs_dVm": float(per_cell.max()), "rms_dVm": float(np.sqrt(np.mean(d * d))),
+        "per_cell_max_abs_dVm_p50": float(np.percentile(per_cell, 50)),
+        "per_cell_max_abs_dVm_p99": float(np.percentile(per_cell, 99)),
+        "cells_within_0.1mV": int(np.sum(per_cell <= 1e-4)),
+        "cells_within_0.1mV_and_spikes_within_1dt": int(np.sum((per_cell <= 1e-4) & spike_ok)),
+        "cells": ncells, "cells_outside_bound": [int(c) for c in bad[:16]],
+        "spike_step_max_err": worst, "cells_with_spike_count_mismatch": mismatched,
+        "spikes": int(sum(len(a) for a in s_64)),
+        "probe": "soma of every cell, every step, from reinit (untimed run)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -278,6 +372,11 @@ def main():
     ap.add_argument("--mem-budget", type=int, default=64, help="serving: device memory budget, GiB")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--accuracy-steps", type=int, default=None,
+                    help="ticks of the untimed accuracy leg (default: the workload's full length; 0: none)")
+    ap.add_argument("--serving-requests", type=int, default=None,
+                    help="requests of the serving leg added to the line (default 24 for the ensemble, "
+                         "0 otherwise)")
     ap.add_argument("--cells", type=int, default=None)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--steps-per-launch", type=int, default=None)
@@ -306,6 +405,10 @@ def main():
     args.steps = dsteps if args.steps is None else args.steps
     args.steps_per_launch = dspl if args.steps_per_launch is None else args.steps_per_launch
     args.graph_chunk = dgraph if args.graph_chunk is None else args.graph_chunk
+    if args.accuracy_steps is None:
+        args.accuracy_steps = dsteps if "mixed" in args.forms and "fp64" in args.forms else 0
+    if args.serving_requests is None:
+        args.serving_requests = 24 if args.workload == "ensemble" else 0
     conns = ()
     if args.workload == "ensemble":
         cells = build_cells(args.cells, args.seed)
@@ -318,20 +421,26 @@ def main():
     packed = [c.pack() for c in cells]
     ncomp = sum(c.n for c in cells)
     res = {}
+    plans = {}
     for form in args.forms.split(","):
         res[form] = run_form(form, packed, dt, args.steps, args.warmup, args.steps_per_launch,
-                             args.graph_chunk, args.threads, args.kernel_mode, conns)
+                             args.graph_chunk, args.threads, args.kernel_mode, conns, args.accuracy_steps,
+                             plans)
     head = "mixed" if "mixed" in res else list(res)[0]
     wall, launches, total_ms, mean_ms, soma, spikes = res[head]
     value = ncomp * args.steps / wall
-    # dominant kernel: the fused step kernel; algorithmic bytes per launch
+    # one launch round: every size bin's dispatch, concurrently on its own
+    # stream; algorithmic bytes of the round over its duration (HIP events
+    # on the engine stream, which forks to and joins the bin streams)
     bpcs = BYTES_PER_CS_LARGE if args.workload == "large_ca" else BYTES_PER_CS
     steps_per_launch = args.steps / max(1, launches)
     bytes_per_launch = bpcs[head] * ncomp * steps_per_launch
     achieved = bytes_per_launch / (mean_ms * 1e-3) / 1e9
     wl_key = {"ensemble": f"hh_ensemble_{args.cells}", "large_ca": f"large_ca_{args.cells}",
               "hh_2048": f"hh_2048_{args.cells}", "network": f"network_{args.cells}"}[args.workload]
-    traffic = measured_traffic(f"hsolve_step_kernel/{head}/{wl_key}", ncomp * steps_per_launch)
+    traffic, traffic_src = measured_traffic(f"{head}/{wl_key}", ncomp * steps_per_launch)
+    bins = plans[head]
+    dom = max(bins, key=lambda b: b["round_ms"]) if bins else None
     out = {
         "metric": "compartment-steps/s (batched cells, FP32-channel mixed form)",
         "value": value,
@@ -347,42 +456,30 @@ def main():
         "data": "synthetic (seeded random morphologies)",
         "config": {"workload": desc, "cells": args.cells, "compartments": ncomp, "dt": dt,
                    "steps_per_launch": args.steps_per_launch, "graph_chunk": args.graph_chunk,
-                   "parallelism": "1 GPU, one workgroup per cell"}
+                   "kernel_mode": args.kernel_mode,
+                   "bins": [{k: b[k] for k in ("kernel", "cells", "compartments", "threads", "solve_threads",
+                                               "lds_bytes", "split", "fast_slots", "round_ms")} for b in bins]}
                   | ({"connections": len(conns), "spikes_routed": int(len(spikes))} if conns else {}),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "hsolve_step_kernel", "bytes_per_compartment_step": bpcs[head],
-                     "mean_launch_ms": mean_ms},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+                     "kernel": (f"{dom['kernel']} (the longest of the {len(bins)} concurrent size-bin "
+                                f"dispatches of one launch round)") if dom else None,
+                     "launch": "one launch round: every size bin's dispatch on its own stream",
+                     "bytes_per_compartment_step": bpcs[head], "mean_launch_ms": mean_ms},
     }
     if "fp64" in res and head != "fp64":
         w64, l64, t64, m64, soma64, spk64 = res["fp64"]
         out["fp64_value"] = ncomp * args.steps / w64
         out["mixed_over_fp64"] = value / out["fp64_value"]
-        d = soma - soma64
-        s_mixed = spike_steps(spikes, len(cells))
-        s_64 = spike_steps(spk64, len(cells))
-        worst = 0
-        mismatched = 0
-        for a, b in zip(s_mixed, s_64):
-            if len(a) != len(b):
-                mismatched += 1
-                continue
-            if a:
-                worst = max(worst, int(np.max(np.abs(np.array(a) - np.array(b)))))
-        per_cell = np.abs(d).max(axis=0)
-        spike_ok = np.array([len(a) == len(b) and (not a or np.max(np.abs(np.array(a) - np.array(b))) <= 1)
-                             for a, b in zip(s_mixed, s_64)])
-        out["accuracy_mixed_vs_fp64"] = {
-            "max_abs_dVm": float(per_cell.max()), "rms_dVm": float(np.sqrt(np.mean(d * d))),
-            "per_cell_max_abs_dVm_p50": float(np.percentile(per_cell, 50)),
-            "per_cell_max_abs_dVm_p99": float(np.percentile(per_cell, 99)),
-            "cells_within_0.1mV": int(np.sum(per_cell <= 1e-4)),
-            "cells_within_0.1mV_and_spikes_within_1dt": int(np.sum((per_cell <= 1e-4) & spike_ok)),
-            "cells": len(cells),
-            "spike_step_max_err": worst, "cells_with_spike_count_mismatch": mismatched,
-            "spikes": int(sum(len(a) for a in s_64)), "probe": "soma of every cell, every step"}
         out["fp64_roofline_frac"] = (bpcs["fp64"] * ncomp * (args.steps / max(1, l64))
                                      / (m64 * 1e-3) / 1e9) / HBM_PEAK_GBS
+        if soma is not None and soma64 is not None:
+            out["accuracy_mixed_vs_fp64"] = accuracy_block(soma, soma64, spikes, spk64, len(cells),
+                                                           args.accuracy_steps)
+    if args.serving_requests > 0:
+        out["serving"] = serving_leg(args.serving_requests, args.rate, args.seed, args.quantum, args.mem_budget)
+        out["serving"]["requests_workload"] = ("Poisson arrivals, n~logU[100,30000] Ca cells with 2-D K-AHP, "
+                                               "duration logU[10ms,1s], dt=25us, mixed precision")
     if not args.no_cpu:
         log("[bench] CPU baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(cells, dt)

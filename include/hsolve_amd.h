@@ -432,6 +432,30 @@ int hsg_cell_info_get(hsg_engine* e, int32_t cell, hsg_cell_info* out);
  * HSolvePassive::walkTree (hsolve/HSolvePassive.cpp:45-98). */
 int hsg_hines_order(hsg_engine* e, int32_t cell, int32_t* out);
 
+/* The launch plan of the live set (no counterpart in the reference, which
+ * steps each cell on its own): the size bins the engine launches
+ * concurrently, one per stream, with their launch shape and the duration of
+ * each bin's dispatches in the last launch round of the most recent
+ * hsg_step (0 where that round was a graph replay).  *n_out receives the
+ * number of bins (at most cap are written). */
+typedef struct hsg_bin_info {
+    int32_t cells;
+    int32_t threads;          /* threads per cell workgroup                     */
+    int32_t solve_threads;    /* split bins: threads per cell of the solve      */
+    int32_t lds_bytes;        /* dynamic LDS per workgroup                      */
+    int32_t split;            /* 1: channel kernel + per-cell solve every tick  */
+    int32_t msg;              /* 1: message-passing solve (split)               */
+    int32_t fast_slots;       /* channel pass for uniform cells of up to this
+                                 many slots (0: the generic pass)                */
+    int32_t slow_groups;      /* 1: cells with junction groups of 4+ members    */
+    int32_t ext;              /* 1: HHChannel2D objects (external kernel/tick)  */
+    int32_t round_steps;      /* ticks in the last timed round                  */
+    int64_t compartments;
+    double  round_ms;         /* duration of the bin's dispatches in that round */
+    char    kernel[64];       /* the per-cell kernel the bin launches           */
+} hsg_bin_info;
+int hsg_plan_info(hsg_engine* e, hsg_bin_info* out, int32_t cap, int32_t* n_out);
+
 /* Blocks until all queued device work of the engine is done. */
 int hsg_sync(hsg_engine* e);
 

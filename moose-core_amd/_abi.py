@@ -95,6 +95,14 @@ class CellInfo(C.Structure):
                 ("caMin", C.c_double), ("caMax", C.c_double), ("vDiv", C.c_int32), ("caDiv", C.c_int32)]
 
 
+class BinInfo(C.Structure):
+    _fields_ = [("cells", C.c_int32), ("threads", C.c_int32), ("solve_threads", C.c_int32),
+                ("lds_bytes", C.c_int32), ("split", C.c_int32), ("msg", C.c_int32),
+                ("fast_slots", C.c_int32), ("slow_groups", C.c_int32), ("ext", C.c_int32),
+                ("round_steps", C.c_int32), ("compartments", C.c_int64), ("round_ms", C.c_double),
+                ("kernel", C.c_char * 64)]
+
+
 # numpy record layouts identical to the C structs (natural alignment)
 COMPARTMENT_DT = np.dtype([(k, "f8") for k in ("Ra", "Rm", "Cm", "Em", "initVm", "Vm", "inject")])
 CHANNEL_DT = np.dtype([("compartment", "i4"), ("instant", "i4"), ("useConcentration", "i4"),

@@ -197,6 +197,7 @@ struct HostCell {
     int tableSet = 0;
     std::vector<int> probeCompt;
     int64_t probeCap = 0;
+    int64_t probeBase = 0;     // the device copy sits at dev.oProbeBase
     int64_t stepsDone = 0;
     bool injDirty = false;
     // Gk as of the last step for channels whose X/Y/Z, Gbar or modulation was
@@ -231,8 +232,6 @@ struct hsg_engine {
     // device-side tables of the launch
     CellDev* dCells = nullptr;
     int dCellsCap = 0;
-    int32_t* dOrder = nullptr;
-    int dOrderCap = 0;
     ChanType* dTypes = nullptr;
     int dTypesCap = 0;
     TableSetDev* dTsets = nullptr;
@@ -240,7 +239,12 @@ struct hsg_engine {
     SpikeRec* dSpikes = nullptr;
     unsigned long long* dSpikeCount = nullptr;
     int64_t spikesDropped = 0;
+    // tablesDirty: the cells' CellDev records or the engine's type and table
+    // lists changed and go to the device before the next launch (copies into
+    // the existing buffers).  planDirty: the live set, or a property that
+    // decides a cell's bin, changed, so the launch plan is rebuilt too.
     bool tablesDirty = true;
+    bool planDirty = true;
     std::vector<int32_t> order;
     int ldsLimit = 65536;
     double dt = 0;                // clock tick
@@ -252,8 +256,8 @@ struct hsg_engine {
     // size bins of the live set, each launched on its own stream
     struct Bin {
         std::vector<int32_t> cells;
-        int32_t* dOrder = nullptr;
-        int2* dTiles = nullptr;     // channel-kernel tiles (split mode)
+        int32_t* dOrder = nullptr;  // in the plan's device buffer
+        int2* dTiles = nullptr;     // channel-kernel tiles (split mode), same buffer
         int nTiles = 0;
         bool msg = false;           // message-passing solve kernel (split mode)
         bool hasExt = false;        // cells with HHChannel2D objects: the external kernel before every tick
@@ -265,13 +269,30 @@ struct hsg_engine {
         int threads = 256;
         bool pairNext = false;      // first half of a split bin whose second half is the next bin
         int solveThreads = 256;     // threads per cell of the split solve kernel
+        double roundMs = 0;         // its dispatches' duration in the last timed round
+        int roundSteps = 0;         // ticks of that round
     };
-    std::vector<Bin> bins;
+    // A launch plan: the bins of a set of cells, with every bin's cell order
+    // and channel tiles in one device buffer (owned by the plan; `external`
+    // buffers belong to someone else and are never freed here).
+    struct Plan {
+        std::vector<Bin> bins;
+        char* dBuf = nullptr;
+        int64_t cap = 0;
+        bool external = false;
+    };
+    Plan plan;                   // the live set's plan (hsg_step)
+    // plans of cell subsets stepped by hsg_step_cells with unequal step
+    // counts, keyed by the subset, most recent last; dropped with the main
+    // plan whenever it is rebuilt
+    std::vector<std::pair<std::vector<int32_t>, Plan>> subPlans;
+    static constexpr size_t kMaxSubPlans = 8;
     bool split = false;          // channel phase as its own kernel
     static constexpr int kMaxBins = 32;
     hipStream_t binStream[kMaxBins] = {};
     hipEvent_t evFork = nullptr, evJoin[kMaxBins] = {};
     hipEvent_t evPairA[kMaxBins] = {}, evPairB[kMaxBins] = {};   // hand-offs inside a pipelined pair
+    hipEvent_t evBin0[kMaxBins] = {}, evBin1[kMaxBins] = {};     // each bin's dispatches in a timed round
     // SpikeGen spikeOut -> Synapse addSpike messages between cells: (source
     // cell, spikegen desc index) -> (target cell, synchan desc index,
     // synapse index within it), in connection order
@@ -891,32 +912,45 @@ double chan_gk(const HostCell& hc, int dk, const std::vector<double>& state) {
     return (double)((GT)s.Gbar[o] * fraction);
 }
 
-int make_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg_engine::Bin>& bins);
+int make_bins(hsg_engine* e, const std::vector<int32_t>& cells, hsg_engine::Plan& plan);
+void free_plan(hsg_engine::Plan& plan);
 
+// Drops the plans of cell subsets (their cells' bins may have changed).
+void drop_sub_plans(hsg_engine* e) {
+    for (auto& sp : e->subPlans) free_plan(sp.second);
+    e->subPlans.clear();
+}
+
+// Brings the device copies of the cell records and engine tables up to date
+// before a launch: a copy of each list into its existing buffer when only
+// contents changed; the launch plan is rebuilt only when the live set or a
+// cell's bin changed (planDirty).  A captured graph holds the buffers'
+// addresses, so it stays valid unless a buffer moves or the plan changes.
 int upload_tables(hsg_engine* e) {
-    if (!e->tablesDirty) return 0;
+    if (!e->tablesDirty && !e->planDirty) return 0;
     const int nc = (int)e->cells.size();
     std::vector<CellDev> devs(std::max(1, nc));
     std::memset(devs.data(), 0, sizeof(CellDev) * devs.size());
-    e->order.clear();
+    std::vector<int32_t> live;
     for (int i = 0; i < nc; ++i) {
         if (!e->cells[i] || !e->cells[i]->live) continue;
         devs[i] = e->cells[i]->dev;
-        e->order.push_back(i);
+        live.push_back(i);
     }
-    std::stable_sort(e->order.begin(), e->order.end(),
-                     [&](int a, int b) { return devs[a].n > devs[b].n; });
+    if (e->planDirty) {
+        std::stable_sort(live.begin(), live.end(), [&](int a, int b) { return devs[a].n > devs[b].n; });
+        e->order = live;
+    }
+    const void* before[4] = {e->dCells, e->dTypes, e->dTsets, e->dTabs2d};
     int rc;
     if ((rc = grow_dev(&e->dCells, &e->dCellsCap, (int)devs.size())) ||
-        (rc = grow_dev(&e->dOrder, &e->dOrderCap, std::max<int>(1, (int)e->order.size()))) ||
         (rc = grow_dev(&e->dTypes, &e->dTypesCap, std::max<int>(1, (int)e->types.size()))) ||
         (rc = grow_dev(&e->dTsets, &e->dTsetsCap, std::max<int>(1, (int)e->tsets.size()))) ||
         (rc = grow_dev(&e->dTabs2d, &e->dTabs2dCap, std::max<int>(1, (int)e->tabs2d.size()))))
         return rc;
+    const void* after[4] = {e->dCells, e->dTypes, e->dTsets, e->dTabs2d};
+    if (std::memcmp(before, after, sizeof before) != 0) e->graphValid = false;
     HIPCHK(hipMemcpyAsync(e->dCells, devs.data(), sizeof(CellDev) * devs.size(), hipMemcpyHostToDevice, e->stream));
-    if (!e->order.empty())
-        HIPCHK(hipMemcpyAsync(e->dOrder, e->order.data(), sizeof(int32_t) * e->order.size(),
-                              hipMemcpyHostToDevice, e->stream));
     if (!e->types.empty())
         HIPCHK(hipMemcpyAsync(e->dTypes, e->types.data(), sizeof(ChanType) * e->types.size(),
                               hipMemcpyHostToDevice, e->stream));
@@ -927,9 +961,13 @@ int upload_tables(hsg_engine* e) {
         HIPCHK(hipMemcpyAsync(e->dTabs2d, e->tabs2d.data(), sizeof(Table2DDev) * e->tabs2d.size(),
                               hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
-    if ((rc = make_bins(e, e->order, e->bins))) return rc;
+    if (e->planDirty) {
+        drop_sub_plans(e);
+        if ((rc = make_bins(e, e->order, e->plan))) return rc;
+        e->planDirty = false;
+        e->graphValid = false;
+    }
     e->tablesDirty = false;
-    e->graphValid = false;
     return 0;
 }
 
@@ -968,13 +1006,73 @@ int default_threads(int n) { return n >= 1024 ? 256 : n >= 256 ? 128 : 64; }
 // Splits cells (largest first) into bins by kernel form and LDS need; each
 // bin is one launch (per round or per tick) on its own stream, with its own
 // dynamic LDS size and workgroup size.
-int make_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg_engine::Bin>& bins) {
-    for (auto& b : bins) {
-        if (b.dOrder) hipFree(b.dOrder);
-        if (b.dTiles) hipFree(b.dTiles);
+void free_plan(hsg_engine::Plan& plan) {
+    if (plan.dBuf && !plan.external) hipFree(plan.dBuf);
+    plan.dBuf = nullptr;
+    plan.cap = 0;
+    plan.bins.clear();
+}
+
+// The bins of `cells` (host side only: cell lists and launch shapes; no
+// device buffer is touched).
+void plan_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg_engine::Bin>& bins);
+
+// Bytes of the device tables of a plan's bins: per bin its cell order, then
+// its channel tiles, each 256-byte aligned.
+int64_t plan_bytes(hsg_engine* e, const std::vector<hsg_engine::Bin>& bins) {
+    int64_t total = 0;
+    for (const auto& b : bins) {
+        int64_t nt = 0;
+        for (int id : b.cells) nt += (e->cells[id]->dev.n + kChannelTile - 1) / kChannelTile;
+        total += align_up(4 * (int64_t)std::max<size_t>(1, b.cells.size())) + align_up(8 * std::max<int64_t>(1, nt));
     }
+    return std::max<int64_t>(total, kAlign);
+}
+
+// Writes the bins' device tables into dBuf (plan_bytes(bins) bytes, device
+// memory) with one copy, and points every bin at its part.
+int write_plan_tables(hsg_engine* e, std::vector<hsg_engine::Bin>& bins, char* dBuf, int64_t bytes) {
+    std::vector<char> img((size_t)bytes, 0);
+    int64_t off = 0;
+    for (auto& b : bins) {
+        std::memcpy(img.data() + off, b.cells.data(), 4 * b.cells.size());
+        b.dOrder = reinterpret_cast<int32_t*>(dBuf + off);
+        off += align_up(4 * (int64_t)std::max<size_t>(1, b.cells.size()));
+        // channel-kernel tiles: (cell, first compartment), kChannelTile each
+        std::vector<int2> tiles;
+        for (int id : b.cells)
+            for (int i0 = 0; i0 < e->cells[id]->dev.n; i0 += kChannelTile) tiles.push_back(make_int2(id, i0));
+        b.nTiles = (int)tiles.size();
+        if (!tiles.empty()) std::memcpy(img.data() + off, tiles.data(), sizeof(int2) * tiles.size());
+        b.dTiles = reinterpret_cast<int2*>(dBuf + off);
+        off += align_up(8 * std::max<int64_t>(1, (int64_t)tiles.size()));
+    }
+    if (off > bytes) return fail(HSG_E_STATE, "launch plan larger than its buffer");
+    HIPCHK(hipMemcpyAsync(dBuf, img.data(), (size_t)off, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+// The plan of `cells`: bins, then their device tables in the plan's own
+// buffer (grown by one allocation when too small).
+int make_bins(hsg_engine* e, const std::vector<int32_t>& cells, hsg_engine::Plan& plan) {
+    plan_bins(e, cells, plan.bins);
+    if (plan.bins.empty()) return 0;
+    const int64_t need = plan_bytes(e, plan.bins);
+    if (plan.cap < need) {
+        if (plan.dBuf && !plan.external) HIPCHK(hipFree(plan.dBuf));
+        plan.dBuf = nullptr;
+        plan.cap = 0;
+        plan.external = false;
+        HIPCHK(hipMalloc(&plan.dBuf, (size_t)need));
+        plan.cap = need;
+    }
+    return write_plan_tables(e, plan.bins, plan.dBuf, plan.cap);
+}
+
+void plan_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg_engine::Bin>& bins) {
     bins.clear();
-    if (cells.empty()) return 0;
+    if (cells.empty()) return;
     // split mode: message-passing cells by workgroup-size class (largest
     // cell >= 1024: 256 threads, >= 256: 128, else 64); other cells (and
     // every cell in fused mode) by LDS need (> 32 KB, > 16 KB, the rest)
@@ -1050,20 +1148,7 @@ int make_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg_
         // large cells' levels hold hundreds of units: their split solve
         // (not the message-passing one) sweeps a level with 512 threads
         b.solveThreads = b.split && !b.msg && e->opt.block_threads <= 0 && maxN >= kWideSolveMin ? 512 : b.threads;
-        HIPCHK(hipMalloc(&b.dOrder, sizeof(int32_t) * b.cells.size()));
-        HIPCHK(hipMemcpyAsync(b.dOrder, b.cells.data(), sizeof(int32_t) * b.cells.size(), hipMemcpyHostToDevice,
-                              e->stream));
-        // channel-kernel tiles: (cell, first compartment), kChannelTile each
-        std::vector<int2> tiles;
-        for (int id : b.cells)
-            for (int i0 = 0; i0 < e->cells[id]->dev.n; i0 += kChannelTile) tiles.push_back(make_int2(id, i0));
-        b.nTiles = (int)tiles.size();
-        HIPCHK(hipMalloc(&b.dTiles, sizeof(int2) * std::max<size_t>(1, tiles.size())));
-        HIPCHK(hipMemcpyAsync(b.dTiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, e->stream));
-        HIPCHK(hipStreamSynchronize(e->stream));
     }
-    HIPCHK(hipStreamSynchronize(e->stream));
-    return 0;
 }
 
 LaunchArgs make_args(hsg_engine* e, const int32_t* order, int nsteps, int first) {
@@ -1313,16 +1398,21 @@ int split_solve(hsg_engine* e, const hsg_engine::Bin& bin, LaunchArgs& a, hipStr
 // phase of tick k follows A's, and A's phase of tick k+1 follows B's of tick
 // k, so each half's solve overlaps the other half's channel work. Every
 // cell still sees its own kernels in tick order.
-int launch_round(hsg_engine* e, std::vector<hsg_engine::Bin>& bins, int chunk, int first) {
+int launch_round(hsg_engine* e, std::vector<hsg_engine::Bin>& bins, int chunk, int first, bool timed = false) {
+    // one stream and one set of events per bin: the plan never holds more
+    // bins than the engine has streams
+    if (bins.size() > (size_t)hsg_engine::kMaxBins) return fail(HSG_E_STATE, "launch plan has more bins than streams");
     HIPCHK(hipEventRecord(e->evFork, e->stream));
     for (size_t b = 0; b < bins.size(); ++b) {
         hipStream_t st = e->binStream[b];
         HIPCHK(hipStreamWaitEvent(st, e->evFork, 0));
+        if (timed) HIPCHK(hipEventRecord(e->evBin0[b], st));
         const int nb = (int)bins[b].cells.size();
         const bool split = bins[b].split;
         if (split && bins[b].pairNext && b + 1 < bins.size()) {
             hipStream_t st2 = e->binStream[b + 1];
             HIPCHK(hipStreamWaitEvent(st2, e->evFork, 0));
+            if (timed) HIPCHK(hipEventRecord(e->evBin0[b + 1], st2));
             const hsg_engine::Bin& A = bins[b];
             const hsg_engine::Bin& B = bins[b + 1];
             for (int k = 0; k < chunk; ++k) {
@@ -1338,6 +1428,10 @@ int launch_round(hsg_engine* e, std::vector<hsg_engine::Bin>& bins, int chunk, i
                 if (int rc = split_wide(e, B, c, st2)) return rc;
                 HIPCHK(hipEventRecord(e->evPairB[b], st2));
                 if (int rc = split_solve(e, B, c, st2)) return rc;
+            }
+            if (timed) {
+                HIPCHK(hipEventRecord(e->evBin1[b], st));
+                HIPCHK(hipEventRecord(e->evBin1[b + 1], st2));
             }
             HIPCHK(hipEventRecord(e->evJoin[b], st));
             HIPCHK(hipStreamWaitEvent(e->stream, e->evJoin[b], 0));
@@ -1365,9 +1459,41 @@ int launch_round(hsg_engine* e, std::vector<hsg_engine::Bin>& bins, int chunk, i
             a.ldsBytes = bins[b].lds;
             HIPCHK(hsg_launch_step(&a, nb, bins[b].threads, e->fp32, bins[b].slow, fused_fast(bins[b]), st));
         }
+        if (timed) HIPCHK(hipEventRecord(e->evBin1[b], st));
         HIPCHK(hipEventRecord(e->evJoin[b], st));
         HIPCHK(hipStreamWaitEvent(e->stream, e->evJoin[b], 0));
     }
+    return 0;
+}
+
+// The plan of a subset of the live cells (hsg_step_cells with unequal step
+// counts, or the cells left once some have finished): built once per subset
+// and kept until the main plan changes, so a serving loop that steps the
+// same cells quantum after quantum plans them once.
+int sub_plan(hsg_engine* e, const std::vector<int32_t>& cells, hsg_engine::Plan** out) {
+    for (size_t k = 0; k < e->subPlans.size(); ++k)
+        if (e->subPlans[k].first == cells) {
+            if (k + 1 != e->subPlans.size()) {
+                auto hit = std::move(e->subPlans[k]);
+                e->subPlans.erase(e->subPlans.begin() + (std::ptrdiff_t)k);
+                e->subPlans.push_back(std::move(hit));
+            }
+            *out = &e->subPlans.back().second;
+            return 0;
+        }
+    hsg_engine::Plan p;
+    if (e->subPlans.size() >= hsg_engine::kMaxSubPlans) {
+        // reuse the least recently used plan's buffer
+        p = std::move(e->subPlans.front().second);
+        e->subPlans.erase(e->subPlans.begin());
+    }
+    int rc = make_bins(e, cells, p);
+    if (rc) {
+        free_plan(p);
+        return rc;
+    }
+    e->subPlans.emplace_back(cells, std::move(p));
+    *out = &e->subPlans.back().second;
     return 0;
 }
 
@@ -1394,23 +1520,21 @@ int run_launches(hsg_engine* e, const std::vector<int32_t>& order, const std::ve
         }
         return {e->evStart[k], e->evStop[k]};
     };
-    std::vector<hsg_engine::Bin> tmpBins;
-    std::vector<int32_t> activePrev;
     bool first = true;
     int rc = 0;
+    int timedSteps = 0;   // ticks of the last round timed per bin (whole set, no graph)
+    for (auto& b : e->plan.bins) { b.roundMs = 0; b.roundSteps = 0; }
     while (true) {
         std::vector<int32_t> active;
         int64_t chunk = K;
         for (size_t i = 0; i < order.size(); ++i)
             if (left[i] > 0) { active.push_back(order[i]); chunk = std::min(chunk, left[i]); }
         if (active.empty()) break;
-        std::vector<hsg_engine::Bin>* bins = &e->bins;
+        std::vector<hsg_engine::Bin>* bins = &e->plan.bins;
         if (!whole) {
-            if (active != activePrev) {
-                if ((rc = make_bins(e, active, tmpBins))) break;
-                activePrev = active;
-            }
-            bins = &tmpBins;
+            hsg_engine::Plan* p = nullptr;
+            if ((rc = sub_plan(e, active, &p))) break;
+            bins = &p->bins;
         }
         // steady state of the whole set: replay a captured graph when asked
         const int G = e->opt.graph_chunk;
@@ -1428,7 +1552,7 @@ int run_launches(hsg_engine* e, const std::vector<int32_t>& order, const std::ve
                 hipGraph_t g;
                 hipError_t he = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
                 if (he != hipSuccess) { rc = fail(HSG_E_DEVICE, hipGetErrorString(he)); break; }
-                for (int r = 0; r < G && !rc; ++r) rc = launch_round(e, e->bins, K, 0);
+                for (int r = 0; r < G && !rc; ++r) rc = launch_round(e, e->plan.bins, K, 0);
                 he = hipStreamEndCapture(e->stream, &g);
                 if (rc) break;
                 if (he != hipSuccess) { rc = fail(HSG_E_DEVICE, hipGetErrorString(he)); break; }
@@ -1443,7 +1567,8 @@ int run_launches(hsg_engine* e, const std::vector<int32_t>& order, const std::ve
             for (size_t i = 0; i < order.size(); ++i) left[i] -= (int64_t)G * K;
             e->lastLaunches += G;
         } else {
-            if ((rc = launch_round(e, *bins, (int)chunk, first ? 1 : 0))) break;
+            if ((rc = launch_round(e, *bins, (int)chunk, first ? 1 : 0, whole))) break;
+            if (whole) timedSteps = (int)chunk;
             for (size_t i = 0; i < order.size(); ++i) if (left[i] > 0) left[i] -= chunk;
             e->lastLaunches += 1;
         }
@@ -1457,7 +1582,14 @@ int run_launches(hsg_engine* e, const std::vector<int32_t>& order, const std::ve
         float ms = 0;
         if (hipEventElapsedTime(&ms, ev.first, ev.second) == hipSuccess) e->lastTotalMs += ms;
     }
-    for (auto& b : tmpBins) if (b.dOrder) hipFree(b.dOrder);
+    if (!rc && timedSteps > 0)
+        for (size_t b = 0; b < e->plan.bins.size(); ++b) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, e->evBin0[b], e->evBin1[b]) == hipSuccess) {
+                e->plan.bins[b].roundMs = ms;
+                e->plan.bins[b].roundSteps = timedSteps;
+            }
+        }
     return rc;
 }
 
@@ -1514,7 +1646,11 @@ int reinit_cells_host(hsg_engine* e, HostCell& hc) {
         std::vector<int32_t> zero(s.syn.size(), 0);
         if ((rc = upload(e, hc.dev.oActCur, zero)) || (rc = upload(e, hc.dev.oActEnd, zero))) return rc;
     }
-    hc.dev.probeBase = 0;
+    if (hc.probe.off >= 0) {
+        hc.probeBase = 0;
+        int rc = put1<int64_t>(e, hc.dev.oProbeBase, 0, 0);
+        if (rc) return rc;
+    }
     e->tablesDirty = true;
     hc.ext.clear();
     hc.ica.clear();
@@ -1566,6 +1702,8 @@ int hsg_create(const hsg_options* opts, hsg_engine** out) {
         HIPCHK(hipEventCreateWithFlags(&e->evJoin[b], hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&e->evPairA[b], hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&e->evPairB[b], hipEventDisableTiming));
+        HIPCHK(hipEventCreate(&e->evBin0[b]));
+        HIPCHK(hipEventCreate(&e->evBin1[b]));
     }
     HIPCHK(hipEventCreateWithFlags(&e->evFork, hipEventDisableTiming));
     int64_t cap = e->opt.mem_budget > 0 ? e->opt.mem_budget : (int64_t)64 << 20;
@@ -1589,18 +1727,20 @@ int hsg_destroy(hsg_engine* e) {
     for (auto ev : e->evStop) hipEventDestroy(ev);
     hipFree(e->arena.base);
     hipFree(e->dCells);
-    hipFree(e->dOrder);
     hipFree(e->dTypes);
     hipFree(e->dTsets);
     hipFree(e->dTabs2d);
     hipFree(e->dSpikes);
     hipFree(e->dSpikeCount);
-    for (auto& b : e->bins) if (b.dOrder) hipFree(b.dOrder);
+    free_plan(e->plan);
+    drop_sub_plans(e);
     for (int b = 0; b < hsg_engine::kMaxBins; ++b) {
         if (e->binStream[b]) hipStreamDestroy(e->binStream[b]);
         if (e->evJoin[b]) hipEventDestroy(e->evJoin[b]);
         if (e->evPairA[b]) hipEventDestroy(e->evPairA[b]);
         if (e->evPairB[b]) hipEventDestroy(e->evPairB[b]);
+        if (e->evBin0[b]) hipEventDestroy(e->evBin0[b]);
+        if (e->evBin1[b]) hipEventDestroy(e->evBin1[b]);
     }
     if (e->evFork) hipEventDestroy(e->evFork);
     if (e->stream) hipStreamDestroy(e->stream);
@@ -1648,6 +1788,7 @@ int hsg_add_cell(hsg_engine* e, const hsg_cell_desc* desc, double dt, int32_t ca
     hc->live = true;
     e->cells[id] = std::move(hc);
     e->tablesDirty = true;
+    e->planDirty = true;
     *cell_id = id;
     return 0;
 }
@@ -1672,6 +1813,7 @@ int hsg_remove_cell(hsg_engine* e, int32_t cell) {
     }
     e->cells[cell].reset();
     e->tablesDirty = true;
+    e->planDirty = true;
     return 0;
 }
 
@@ -1709,6 +1851,7 @@ int hsg_reinit(hsg_engine* e, double dt) {
     if (!e) return fail(HSG_E_ARG, "null engine");
     if (!(dt > 0.0)) return fail(HSG_E_ARG, "reinit: dt must be positive");
     hipSetDevice(e->device);
+    if (dt != e->dt) e->graphValid = false;   // the graph's launches carry dt
     e->dt = dt;
     e->reinitDone = true;
     std::vector<int32_t> ids;
@@ -1763,8 +1906,11 @@ int hsg_step_cells(hsg_engine* e, const int32_t* cells, const int64_t* nsteps, i
     hipSetDevice(e->device);
     std::vector<int32_t> ord;
     std::vector<int64_t> st;
+    std::vector<char> seen(e->cells.size(), 0);
     for (int i = 0; i < n; ++i) {
         if (!get_cell(e, cells[i])) return fail(HSG_E_ARG, "no such cell");
+        if (seen[cells[i]]) return fail(HSG_E_ARG, "a cell is listed twice");
+        seen[cells[i]] = 1;
         if (nsteps[i] < 0) return fail(HSG_E_ARG, "negative step count");
         if (nsteps[i] > 0) { ord.push_back(cells[i]); st.push_back(nsteps[i]); }
     }
@@ -2066,6 +2212,7 @@ int hsg_set_field(hsg_engine* e, int32_t cell, int32_t field, const int32_t* idx
                     hc->dev.fastOk = 0;
                 }
                 e->tablesDirty = true;
+                e->planDirty = true;
                 break;
             }
             default: return fail(HSG_E_ARG, "unknown field");
@@ -2333,12 +2480,12 @@ static int set_probe_offsets(hsg_engine* e, HostCell* hc, const std::vector<int6
     hc->probe.bytes = bytes;
     hc->dev.oProbeOff = off;
     hc->dev.oProbeTrace = off + align_up(8 * (int64_t)n);
-    hc->dev.oProbeCount = hc->dev.oProbeTrace + align_up(8 * (int64_t)n * capacity);
+    hc->dev.oProbeBase = hc->dev.oProbeTrace + align_up(8 * (int64_t)n * capacity);
     if ((rc = upload(e, hc->dev.oProbeOff, offs))) return rc;
-    HIPCHK(hipStreamSynchronize(e->stream));
+    if ((rc = put1<int64_t>(e, hc->dev.oProbeBase, 0, hc->stepsDone))) return rc;
     hc->dev.nProbe = n;
     hc->dev.probeCap = capacity;
-    hc->dev.probeBase = hc->stepsDone;
+    hc->probeBase = hc->stepsDone;
     hc->probeCompt.assign(n, 0);
     hc->probeCap = capacity;
     return 0;
@@ -2395,15 +2542,19 @@ int hsg_read_probes(hsg_engine* e, int32_t cell, double* out, int64_t cap_sample
     *n_samples = 0;
     if (hc->dev.nProbe == 0) return 0;
     hipSetDevice(e->device);
-    // sample k is the (k+1)-th multiple of the stride after probeBase,
-    // written while below capacity
+    // sample k is the (k+1)-th multiple of the stride after the trace's
+    // base step, written while below capacity
     const int64_t st = hc->dev.probeStride;
-    const int64_t cnt = hc->stepsDone / st - hc->dev.probeBase / st;
+    const int64_t cnt = hc->stepsDone / st - hc->probeBase / st;
     int64_t k = std::min(std::min(cnt, hc->dev.probeCap), cap_samples);
     int rc;
     if (k > 0 && out && (rc = download(e, hc->dev.oProbeTrace, (size_t)(k * hc->dev.nProbe), out))) return rc;
-    hc->dev.probeBase = hc->stepsDone;
-    e->tablesDirty = true;
+    // the next trace starts here: only the device's base word changes, the
+    // cell records and the launch plan stay as they are
+    if (hc->probeBase != hc->stepsDone) {
+        hc->probeBase = hc->stepsDone;
+        if ((rc = put1<int64_t>(e, hc->dev.oProbeBase, 0, hc->stepsDone))) return rc;
+    }
     *n_samples = std::max<int64_t>(0, k);
     return 0;
 }
@@ -2442,6 +2593,40 @@ int hsg_hines_order(hsg_engine* e, int32_t cell, int32_t* out) {
     HostCell* hc = get_cell(e, cell);
     if (!hc || !out) return fail(HSG_E_ARG, "no such cell");
     for (size_t i = 0; i < hc->s.descToHines.size(); ++i) out[i] = hc->s.descToHines[i];
+    return 0;
+}
+
+int hsg_plan_info(hsg_engine* e, hsg_bin_info* out, int32_t cap, int32_t* n_out) {
+    if (!e || !n_out || (cap > 0 && !out)) return fail(HSG_E_ARG, "null argument");
+    hipSetDevice(e->device);
+    int rc = upload_tables(e);   // the plan of the current live set
+    if (rc) return rc;
+    const auto& bins = e->plan.bins;
+    *n_out = (int32_t)bins.size();
+    for (int32_t b = 0; b < cap && b < (int32_t)bins.size(); ++b) {
+        const hsg_engine::Bin& x = bins[b];
+        hsg_bin_info& o = out[b];
+        std::memset(&o, 0, sizeof o);
+        o.cells = (int32_t)x.cells.size();
+        o.threads = x.threads;
+        o.solve_threads = x.split ? (x.msg ? x.threads : x.solveThreads) : 0;
+        o.lds_bytes = x.lds;
+        o.split = x.split;
+        o.msg = x.msg;
+        o.fast_slots = x.split ? x.chanMS : fused_fast(x);
+        o.slow_groups = x.slow;
+        o.ext = x.hasExt;
+        o.round_steps = x.roundSteps;
+        o.compartments = x.sumN;
+        o.round_ms = x.roundMs;
+        const char* prec = e->fp32 ? "float" : "double";
+        if (x.split)
+            snprintf(o.kernel, sizeof o.kernel, "%s<%s>", x.msg ? "hsolve_msg_kernel"
+                     : x.solveThreads > 256 ? "hsolve_solve_wide_kernel" : "hsolve_solve_kernel", prec);
+        else
+            snprintf(o.kernel, sizeof o.kernel, "%s<%s>", x.slow ? "hsolve_step_slow_kernel"
+                     : fused_fast(x) ? "hsolve_step_fast_kernel" : "hsolve_step_kernel", prec);
+    }
     return 0;
 }
 

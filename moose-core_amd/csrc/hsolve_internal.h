@@ -150,7 +150,6 @@ struct CellDev {
     int32_t nGroups;
     int32_t tableSet;       // the cell's lookup-table set
     int64_t probeCap;       // samples per probe
-    int64_t probeBase;      // step count at which the current trace started
     int64_t probeStride;    // a sample every probeStride steps (>= 1)
     // compartment arrays (f64 unless noted)
     int64_t oV, oInitVm, oCmByDt, oEmByRm, oDbase, oOff;
@@ -180,8 +179,9 @@ struct CellDev {
     // spike generators: int32 compt, int32 edge, f64 thr, f64 refr, f64 last, int32 fired
     int64_t oSpkCompt, oSpkEdge, oSpkThr, oSpkRefr, oSpkLast, oSpkFired;
     // probes: int64 arena offset of the sampled double [nProbe], f64
-    // trace[probeCap*nProbe], int64 count
-    int64_t oProbeOff, oProbeTrace, oProbeCount;
+    // trace[probeCap*nProbe], int64 base: the step count at which the
+    // current trace started (written by the host when it reads the trace)
+    int64_t oProbeOff, oProbeTrace, oProbeBase;
     // per-call external currents: int32 idx + f64 Gk + f64 GkEk (nExt entries)
     int64_t oExtIdx, oExtG, oExtGE;
     int32_t nExt;

@@ -993,9 +993,10 @@ __device__ __forceinline__ void tick_outputs(const LaunchArgs& a, const CellDev&
     }
     // probes sample every probeStride-th step (a Table on a slower clock
     // tick, Table::process after the solver's tick), sample 0 being the
-    // first such step after probeBase
+    // first such step after the trace's base step
     if (c.nProbe > 0 && stepNo % c.probeStride == 0) {
-        const int64_t sample = stepNo / c.probeStride - c.probeBase / c.probeStride - 1;
+        const int64_t base = *A_<const int64_t>(arena, c.oProbeBase);
+        const int64_t sample = stepNo / c.probeStride - base / c.probeStride - 1;
         if (sample >= 0 && sample < c.probeCap) {
             const int64_t* poff = A_<const int64_t>(arena, c.oProbeOff);
             double* tr = A_<double>(arena, c.oProbeTrace);

@@ -22,7 +22,7 @@ EXPORTS = [
     "hsg_reinit_cell", "hsg_step", "hsg_step_cells", "hsg_get_field", "hsg_set_field",
     "hsg_add_inject", "hsg_add_gkek", "hsg_add_conc", "hsg_add_ica", "hsg_set_pulsegen", "hsg_add_spike", "hsg_connect_spike", "hsg_read_spikes",
     "hsg_set_probes", "hsg_set_probes_every", "hsg_set_probe_fields", "hsg_read_probes", "hsg_cell_info_get", "hsg_hines_order",
-    "hsg_sync", "hsg_last_step_timing", "hsg_last_error",
+    "hsg_sync", "hsg_last_step_timing", "hsg_plan_info", "hsg_last_error",
 ]
 
 
@@ -72,6 +72,7 @@ def load_library(path=None):
         "hsg_hines_order": ([vp, i32, P(i32)], C.c_int),
         "hsg_sync": ([vp], C.c_int),
         "hsg_last_step_timing": ([vp, P(i32), P(dbl), P(dbl)], C.c_int),
+        "hsg_plan_info": ([vp, P(_abi.BinInfo), i32, P(i32)], C.c_int),
         "hsg_last_error": ([], C.c_char_p),
     }
     for name, (args, res) in sig.items():
@@ -265,6 +266,21 @@ class Engine:
         out = np.zeros(ncomp, dtype=np.int32)
         self._check(self.lib.hsg_hines_order(self.h, int(cell),
                                              out.ctypes.data_as(C.POINTER(C.c_int32))))
+        return out
+
+    def plan_info(self):
+        """The launch plan of the live set: one dict per size bin (launch
+        shape, kernel, and its dispatches' duration in the last timed
+        launch round)."""
+        cap = 64
+        arr = (_abi.BinInfo * cap)()
+        n = C.c_int32()
+        self._check(self.lib.hsg_plan_info(self.h, arr, cap, C.byref(n)))
+        out = []
+        for b in arr[: min(n.value, cap)]:
+            d = {k: getattr(b, k) for k, _ in _abi.BinInfo._fields_}
+            d["kernel"] = b.kernel.decode()
+            out.append(d)
         return out
 
     def last_timing(self):

@@ -69,6 +69,8 @@ int main(void) {
   P(hsg_synchan_desc, Gbar); P(hsg_synchan_desc, tau2); P(hsg_synchan_desc, modulation);
   P(hsg_vclamp_desc, tau); P(hsg_vclamp_desc, gain);
   P(hsg_cell_desc, synchans); P(hsg_cell_desc, synapses); P(hsg_cell_desc, vclamps);
+  printf("hsg_bin_info %zu\n", sizeof(hsg_bin_info));
+  P(hsg_bin_info, compartments); P(hsg_bin_info, round_ms); P(hsg_bin_info, kernel);
   return 0;
 }
 '''
@@ -118,6 +120,9 @@ int main(void) {
         assert sizes["hsg_vclamp_desc." + f] == _abi.VCLAMP_DT.fields[f][1]
     for f in ("synchans", "synapses", "vclamps"):
         assert sizes["hsg_cell_desc." + f] == getattr(_abi.CellDesc, f).offset
+    assert sizes["hsg_bin_info"] == ctypes.sizeof(_abi.BinInfo)
+    for f in ("compartments", "round_ms", "kernel"):
+        assert sizes["hsg_bin_info." + f] == getattr(_abi.BinInfo, f).offset
 
 
 def test_engine_without_gpu_raises_cleanly():

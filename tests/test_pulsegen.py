@@ -374,6 +374,115 @@ def test_vclamp_holds_the_command(oracle_mod):
         assert i > 0 and abs(o.get("Im", np.arange(spec.n)).sum() + i) < 1e-6 * i
 
 
+def pid_closed_form(mode, sensed, initvm, Kp, a, b):
+    """The clamp current after each of N ticks, from the Vm the clamp sensed
+    on each tick (sensed[n-1] on tick n) with its command held at initVm.
+
+    The PID increments of VClamp::process telescope.  With e_n = command -
+    s_n, the sensed values s_n, and the histories the reinit clears (e_0 =
+    s_0 = s_-1 = 0):
+      mode 0: sum (1+a+b) e_n - (1+2b) e_n-1 + b e_n-2
+              = a sum e + e_N + b (e_N - e_N-1)
+      mode 1: sum (1+a) e_n - e_n-1 + b (s_n - 2 s_n-1 + s_n-2)
+              = a sum e + e_N + b (s_N - s_N-1)
+      mode 2: sum (s_n - s_n-1) + a e_n + b (s_n - 2 s_n-1 + s_n-2)
+              = s_N + a sum e + b (s_N - s_N-1)
+    (a = dt/ti, b = td/dt).  Only the sums of the increments are used, so a
+    history kept in the wrong variable (the error where the sensed Vm
+    belongs, or one tick late) does not telescope and fails the check."""
+    s = np.asarray(sensed, dtype=np.float64)
+    e = initvm - s
+    se = np.cumsum(e)
+    s_prev = np.concatenate([[0.0], s[:-1]])
+    e_prev = np.concatenate([[0.0], e[:-1]])
+    if mode == 0:
+        return Kp * (a * se + e + b * (e - e_prev))
+    if mode == 1:
+        return Kp * (a * se + e + b * (s - s_prev))
+    return Kp * (s + a * se + b * (s - s_prev))
+
+
+def pid_history_slip(mode, sensed, initvm, Kp, a, b):
+    """The same loop with the derivative history slipped (e2 takes the last
+    error instead of the last sensed Vm): a plausible mistake the closed form
+    must tell apart from VClamp::process."""
+    cur, e1, e2, v1, out = 0.0, 0.0, 0.0, 0.0, []
+    for v in sensed:
+        e = initvm - v
+        if mode == 1:
+            cur += Kp * ((1 + a) * e - e1 + b * (v - 2 * v1 + e2))
+            e2, v1, e1 = e1, v, e
+        else:
+            cur += Kp * (v - v1 + a * e + b * (v - 2 * v1 + e2))
+            e2, v1 = e1, v
+        out.append(cur)
+    return np.array(out)
+
+
+def clamp_cell(mode, ti, td, gain_scale):
+    spec = workloads.random_hh_cell(np.random.default_rng(44), 15, inject=0.0)
+    spec.spikegens = spec.spikegens[:0]
+    c = 3
+    Cm = spec.compartments["Cm"][c]
+    spec.vclamps = vclamps([(c, mode, 0.2e-3, ti, td, Cm * gain_scale)])
+    return spec, c, float(spec.compartments["initVm"][c]), Cm * gain_scale
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_vclamp_pid_modes_closed_form(oracle_mod, mode):
+    """The oracle's clamp current, tick by tick, equals the telescoped sum of
+    its PID increments computed from the Vm it sensed, in modes 0, 1 and 2
+    (ti finite, td > 0, so every term of the increment contributes); the
+    slipped-history variant does not."""
+    dt, N = 25e-6, 300
+    ti, td = 0.5e-3, 5e-6
+    spec, c, initvm, Kp = clamp_cell(mode, ti, td, 1e2 if mode == 2 else 1e3)
+    o = oracle_mod.OracleCell(spec, dt)
+    o.reinit(dt)
+    sensed, cur = [], []
+    v_prev = o.get("Vm", [c])[0]
+    assert v_prev == initvm
+    for _ in range(N):
+        sensed.append(v_prev)          # what the clamp reads on this tick
+        v_prev = o.run(dt, 1, [c])[0, 0]
+        cur.append(o.get("current", [0])[0])
+    cur = np.array(cur)
+    a, b = dt / ti, td / dt
+    want = pid_closed_form(mode, sensed, initvm, Kp, a, b)
+    scale = np.abs(cur).max()
+    assert scale > 0
+    np.testing.assert_allclose(cur, want, rtol=0, atol=1e-10 * scale)
+    # the clamp is doing work: the sensed Vm moves, so the derivative terms
+    # are not zero
+    assert np.ptp(sensed) > 1e-4
+    if mode in (1, 2):
+        slip = pid_history_slip(mode, sensed, initvm, Kp, a, b)
+        assert np.abs(slip - want).max() > 1e-4 * scale
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_gpu_vclamp_pid_modes_closed_form(pkg, mode):
+    """The GPU path's clamp current (sampled on the device every tick with
+    the compartment's Vm) equals the telescoped PID sum of the Vm it sensed,
+    in modes 0, 1 and 2."""
+    print(f"case: VClamp mode {mode} closed form, 300 steps, fp64", flush=True)
+    dt, N = 25e-6, 300
+    ti, td = 0.5e-3, 5e-6
+    spec, c, initvm, Kp = clamp_cell(mode, ti, td, 1e2 if mode == 2 else 1e3)
+    eng = pkg.Engine("fp64")
+    cid = eng.add_cell(spec, dt)
+    eng.set_probe_fields(cid, ["Vm", "current"], [c, 0], N)
+    eng.reinit(dt)
+    eng.step(N)
+    tr = eng.read_probes(cid)
+    eng.close()
+    v, cur = tr[:, 0], tr[:, 1]
+    sensed = np.concatenate([[initvm], v[:-1]])
+    want = pid_closed_form(mode, sensed, initvm, Kp, dt / ti, td / dt)
+    np.testing.assert_allclose(cur, want, rtol=0, atol=1e-10 * np.abs(cur).max())
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("mode", ["fused", "split"])
 def test_gpu_vclamp_batch(pkg, oracle_mod, mode):
