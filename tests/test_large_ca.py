@@ -1,0 +1,83 @@
+"""The large-cell configurations on the GPU against the oracle: Ca cells of
+6000 and 15000 compartments (branching 2-4, Na/Kdr/KA/CaL, one CaConc per
+compartment, the K-AHP as an HHChannel2D on a 3000 x 500 Vm x [Ca] table),
+injected so that each fires, dt = 20 us, 1000 steps, in the engine's
+automatic mode: both cells take the split path (batch-wide channel kernel,
+per-cell solve), run as a pipelined pair with the 512-thread wide solve,
+here also replayed as captured graphs of 2 x 50 steps.
+
+FP64 form: within 1e-6 V of the oracle at every sampled compartment and step
+(not bit for bit: the HHChannel2D gates use exp), identical spike steps.
+Mixed form (FP32 HHChannel gates, tables and currents; FP64 Hines solve):
+within 1e-4 V, spike steps within one step.
+"""
+import numpy as np
+import pytest
+
+from moose_core_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-6
+MIXED_TOL = 1e-4
+DT, NSTEPS = 20e-6, 1000
+
+
+def _cells():
+    out = []
+    for n, inj, seed in ((6000, 6e-9, 71), (15000, 8e-9, 72)):
+        rng = np.random.default_rng(seed)
+        out.append(workloads.random_ca_cell(rng, n, inject=inj, kahp2d=True))
+    return out
+
+
+def _probes(spec):
+    return np.unique(np.concatenate([[0], np.arange(0, spec.n, max(1, spec.n // 150)), [spec.n - 1]]))
+
+
+@pytest.fixture(scope="module")
+def reference(oracle_mod):
+    specs = _cells()
+    out = []
+    for s in specs:
+        c = oracle_mod.OracleCell(s, DT)
+        c.reinit(DT)
+        tr = c.run(DT, NSTEPS, _probes(s))
+        g, st = c.read_spikes()
+        out.append((tr, list(st)))
+    return specs, out
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_large_ca_cells_auto_mode(pkg, reference, precision):
+    print(f"case: Ca cells 6000 + 15000 compartments with 2-D K-AHP, {NSTEPS} steps, {precision}, auto, "
+          f"graphs", flush=True)
+    specs, ref = reference
+    eng = pkg.Engine(precision, kernel_mode="auto", steps_per_launch=50, graph_chunk=2)
+    ids = [eng.add_cell(s, DT) for s in specs]
+    for cid, s in zip(ids, specs):
+        eng.set_probes(cid, _probes(s), NSTEPS)
+    eng.reinit(DT)
+    plan = eng.plan_info()
+    # the path under test: split bins, a pipelined pair, the wide solve
+    assert all(b["split"] for b in plan) and len(plan) == 2
+    assert all(b["solve_threads"] == 512 for b in plan)
+    for k in (300, 700):
+        eng.step(k)
+    spikes = eng.read_spikes()
+    tol = FP64_TOL if precision == "fp64" else MIXED_TOL
+    nspk = 0
+    for cid, s, (rtr, rsp) in zip(ids, specs, ref):
+        got = eng.read_probes(cid)
+        err = float(np.abs(got - rtr).max())
+        print(f"cell {cid}: n = {s.n}, max|dVm| = {err:.3e} V, spikes {rsp}")
+        assert err <= tol, (cid, err)
+        mine = list(spikes[spikes["cell"] == cid]["step"])
+        assert len(mine) == len(rsp), (cid, mine, rsp)
+        nspk += len(rsp)
+        if precision == "fp64":
+            assert mine == rsp
+        else:
+            assert all(abs(a - b) <= 1 for a, b in zip(mine, rsp))
+    assert nspk >= 2
+    eng.close()
