@@ -283,6 +283,24 @@ int hsg_step(hsg_engine* e, int64_t nsteps);
 int hsg_step_cells(hsg_engine* e, const int32_t* cells, const int64_t* nsteps,
                    int32_t n);
 
+/* hsg_step_cells for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc);
+ * it has no counterpart in the reference.  Same arguments, semantics and
+ * results as hsg_step_cells, but it calls no hipMalloc, hipFree or other
+ * allocating routine: its launch tables (per size bin, the cell order and
+ * the channel-kernel tiles) go to a static device array of 1 MiB per GPU,
+ * held until the process ends, instead of a plan buffer; it never captures
+ * a graph.  Fails with HSG_E_ARG when a call's launch tables exceed that
+ * area (about 120k tiles of 256 compartments, i.e. some 30 M compartments),
+ * and with HSG_E_NOMEM / HSG_E_STATE when the call would have to grow the
+ * engine's own memory (arena, cell or type tables): step once with an
+ * allocating call after adding cells (that first call is also the first work
+ * on the engine's streams, whose queues the HIP runtime sets up on first
+ * use).  Calls that overlap on one GPU (several engines on several host
+ * threads) race on the array. */
+int hsg_step_cells_noalloc(hsg_engine* e, const int32_t* cells, const int64_t* nsteps,
+                           int32_t n);
+
 /* ---- zombie fields ----------------------------------------------------- */
 /* Compartment fields (index = compartment) */
 #define HSG_F_VM        1   /* getVm/setVm           HSolveInterface.cpp:66-79   */

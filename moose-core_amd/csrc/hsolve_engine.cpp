@@ -40,6 +40,7 @@ extern "C" hipError_t hsg_launch_update(const hsg::LaunchArgs* a, const int2* ti
 extern "C" hipError_t hsg_launch_outputs(const hsg::LaunchArgs* a, int nblocks, hipStream_t stream);
 extern "C" hipError_t hsg_launch_msg(const hsg::LaunchArgs* a, int nblocks, int threads, int fp32,
                                     hipStream_t stream);
+extern "C" hipError_t hsg_scratch_address(int device, void** ptr, long long* bytes);
 
 using namespace hsg;
 
@@ -306,6 +307,11 @@ struct hsg_engine {
     hipGraphExec_t graphExec = nullptr;
     int graphSteps = 0;
     bool graphValid = false;
+    // the static launch-table area of hsg_step_cells_noalloc on this engine's
+    // device, and whether the call in progress may allocate device memory
+    char* scratch = nullptr;
+    int64_t scratchBytes = 0;
+    bool noAlloc = false;
 };
 
 namespace {
@@ -369,6 +375,9 @@ int grow_arena(hsg_engine* e, int64_t need) {
 int arena_alloc(hsg_engine* e, int64_t bytes, int64_t* off) {
     int64_t o = e->arena.alloc(bytes);
     if (o < 0) {
+        if (e->noAlloc)
+            return fail(HSG_E_NOMEM, "the engine's device arena is full and this call allocates no device memory "
+                                     "(an allocating call such as hsg_step_cells grows it)");
         int rc = grow_arena(e, bytes);
         if (rc) return rc;
         o = e->arena.alloc(bytes);
@@ -927,7 +936,10 @@ void drop_sub_plans(hsg_engine* e) {
 // cell's bin changed (planDirty).  A captured graph holds the buffers'
 // addresses, so it stays valid unless a buffer moves or the plan changes.
 int upload_tables(hsg_engine* e) {
-    if (!e->tablesDirty && !e->planDirty) return 0;
+    // a call that allocates nothing uploads the records but leaves the
+    // launch plan to the next allocating call
+    const bool doPlan = e->planDirty && !e->noAlloc;
+    if (!e->tablesDirty && !doPlan) return 0;
     const int nc = (int)e->cells.size();
     std::vector<CellDev> devs(std::max(1, nc));
     std::memset(devs.data(), 0, sizeof(CellDev) * devs.size());
@@ -937,10 +949,15 @@ int upload_tables(hsg_engine* e) {
         devs[i] = e->cells[i]->dev;
         live.push_back(i);
     }
-    if (e->planDirty) {
+    if (doPlan) {
         std::stable_sort(live.begin(), live.end(), [&](int a, int b) { return devs[a].n > devs[b].n; });
         e->order = live;
     }
+    if (e->noAlloc && ((int)devs.size() > e->dCellsCap || std::max<int>(1, (int)e->types.size()) > e->dTypesCap ||
+                       std::max<int>(1, (int)e->tsets.size()) > e->dTsetsCap ||
+                       std::max<int>(1, (int)e->tabs2d.size()) > e->dTabs2dCap))
+        return fail(HSG_E_STATE, "the engine's cell and type tables must grow, and this call allocates no device "
+                                 "memory: step once with hsg_step or hsg_step_cells after adding cells");
     const void* before[4] = {e->dCells, e->dTypes, e->dTsets, e->dTabs2d};
     int rc;
     if ((rc = grow_dev(&e->dCells, &e->dCellsCap, (int)devs.size())) ||
@@ -961,7 +978,7 @@ int upload_tables(hsg_engine* e) {
         HIPCHK(hipMemcpyAsync(e->dTabs2d, e->tabs2d.data(), sizeof(Table2DDev) * e->tabs2d.size(),
                               hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
-    if (e->planDirty) {
+    if (doPlan) {
         drop_sub_plans(e);
         if ((rc = make_bins(e, e->order, e->plan))) return rc;
         e->planDirty = false;
@@ -1593,6 +1610,53 @@ int run_launches(hsg_engine* e, const std::vector<int32_t>& order, const std::ve
     return rc;
 }
 
+// run_launches for hsg_step_cells_noalloc: the same rounds (no graph), with
+// the launch tables of each active set written into the caller-independent
+// static area `buf` (cap bytes) instead of a plan buffer the library
+// allocates.  A new active set's tables are copied on the engine stream
+// after the previous round has joined it, so no round in flight reads them.
+int run_launches_static(hsg_engine* e, const std::vector<int32_t>& order, const std::vector<int64_t>& steps,
+                        char* buf, int64_t cap) {
+    const int K = e->opt.steps_per_launch > 0 ? e->opt.steps_per_launch : 50;
+    std::vector<int64_t> left(steps);
+    hsg_engine::Plan p;
+    p.dBuf = buf;
+    p.cap = cap;
+    p.external = true;
+    std::vector<int32_t> activePrev;
+    bool first = true;
+    int rc = 0;
+    HIPCHK(hipEventRecord(e->evStart[0], e->stream));
+    while (true) {
+        std::vector<int32_t> active;
+        int64_t chunk = K;
+        for (size_t i = 0; i < order.size(); ++i)
+            if (left[i] > 0) { active.push_back(order[i]); chunk = std::min(chunk, left[i]); }
+        if (active.empty()) break;
+        if (active != activePrev) {
+            plan_bins(e, active, p.bins);
+            const int64_t need = plan_bytes(e, p.bins);
+            if (need > cap) {
+                rc = fail(HSG_E_ARG, "launch tables of " + std::to_string(need) + " bytes exceed the " +
+                                         std::to_string(cap) + "-byte static area of hsg_step_cells_noalloc");
+                break;
+            }
+            if ((rc = write_plan_tables(e, p.bins, buf, cap))) break;
+            activePrev = active;
+        }
+        if ((rc = launch_round(e, p.bins, (int)chunk, first ? 1 : 0))) break;
+        for (size_t i = 0; i < order.size(); ++i) if (left[i] > 0) left[i] -= chunk;
+        e->lastLaunches += 1;
+        first = false;
+    }
+    HIPCHK(hipEventRecord(e->evStop[0], e->stream));
+    hipError_t he = hipStreamSynchronize(e->stream);
+    if (he != hipSuccess && !rc) rc = fail(HSG_E_DEVICE, std::string("step: ") + hipGetErrorString(he));
+    float ms = 0;
+    if (!rc && hipEventElapsedTime(&ms, e->evStart[0], e->evStop[0]) == hipSuccess) e->lastTotalMs += ms;
+    return rc;
+}
+
 int reinit_cells_host(hsg_engine* e, HostCell& hc) {
     // the device state (V, Ca pools, spike generators, gates, PulseGen
     // outputs) is reset by hsolve_reinit_kernel; here only the host mirrors
@@ -1706,6 +1770,21 @@ int hsg_create(const hsg_options* opts, hsg_engine** out) {
         HIPCHK(hipEventCreate(&e->evBin1[b]));
     }
     HIPCHK(hipEventCreateWithFlags(&e->evFork, hipEventDisableTiming));
+    {
+        // one timing pair ahead of any step call (the no-allocation entry
+        // point creates nothing), and the static launch-table area of this
+        // device, looked up once per device with the device current
+        hipEvent_t a, b;
+        HIPCHK(hipEventCreate(&a));
+        HIPCHK(hipEventCreate(&b));
+        e->evStart.push_back(a);
+        e->evStop.push_back(b);
+        void* sp = nullptr;
+        long long sb = 0;
+        HIPCHK(hsg_scratch_address(e->device, &sp, &sb));
+        e->scratch = static_cast<char*>(sp);
+        e->scratchBytes = sb;
+    }
     int64_t cap = e->opt.mem_budget > 0 ? e->opt.mem_budget : (int64_t)64 << 20;
     HIPCHK(hipMalloc(&e->arena.base, cap));
     e->arena.cap = cap;
@@ -1929,6 +2008,46 @@ int hsg_step_cells(hsg_engine* e, const int32_t* cells, const int64_t* nsteps, i
     e->lastTotalMs = 0;
     if ((rc = upload_tables(e))) return rc;
     if (!ord.empty() && (rc = run_launches(e, ord, st))) return rc;
+    for (size_t i = 0; i < ord.size(); ++i) consume_inputs(e, *e->cells[ord[i]], st[i]);
+    return 0;
+}
+
+int hsg_step_cells_noalloc(hsg_engine* e, const int32_t* cells, const int64_t* nsteps, int32_t n) {
+    if (!e || (n > 0 && (!cells || !nsteps))) return fail(HSG_E_ARG, "null argument");
+    if (!(e->dt > 0.0)) return fail(HSG_E_STATE, "call hsg_reinit before stepping");
+    if (!e->scratch) return fail(HSG_E_STATE, "no static launch-table area on this device");
+    hipSetDevice(e->device);
+    std::vector<int32_t> ord;
+    std::vector<int64_t> st;
+    std::vector<char> seen(e->cells.size(), 0);
+    for (int i = 0; i < n; ++i) {
+        if (!get_cell(e, cells[i])) return fail(HSG_E_ARG, "no such cell");
+        if (seen[cells[i]]) return fail(HSG_E_ARG, "a cell is listed twice");
+        seen[cells[i]] = 1;
+        if (nsteps[i] < 0) return fail(HSG_E_ARG, "negative step count");
+        if (nsteps[i] > 0) { ord.push_back(cells[i]); st.push_back(nsteps[i]); }
+    }
+    if (!e->conns.empty()) return fail(HSG_E_UNSUPPORTED, "cells joined by spike connections step with hsg_step");
+    // every allocation path below fails instead of allocating
+    struct Guard {
+        hsg_engine* e;
+        ~Guard() { e->noAlloc = false; }
+    } guard{e};
+    e->noAlloc = true;
+    int rc;
+    if ((rc = flush_inputs(e))) return rc;
+    std::vector<HostCell*> synCells;
+    std::vector<int64_t> synSteps;
+    for (size_t i = 0; i < ord.size(); ++i)
+        if (!e->cells[ord[i]]->s.syn.empty()) {
+            synCells.push_back(e->cells[ord[i]].get());
+            synSteps.push_back(st[i]);
+        }
+    if ((rc = write_activations(e, synCells, synSteps))) return rc;
+    e->lastLaunches = 0;
+    e->lastTotalMs = 0;
+    if ((rc = upload_tables(e))) return rc;
+    if (!ord.empty() && (rc = run_launches_static(e, ord, st, e->scratch, e->scratchBytes))) return rc;
     for (size_t i = 0; i < ord.size(); ++i) consume_inputs(e, *e->cells[ord[i]], st[i]);
     return 0;
 }

@@ -32,6 +32,8 @@
 // FP64).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "hsolve_internal.h"
 
 namespace hsg {
@@ -1841,6 +1843,37 @@ __global__ void __launch_bounds__(256) hsolve_chan2d_reinit_kernel(LaunchArgs a)
 }
 
 }  // namespace hsg
+
+// Launch tables of hsg_step_cells_noalloc (per bin: its cell order, then its
+// channel-kernel tiles) live in this array instead of memory the library
+// allocates.  One copy per GPU, kept until the process ends; every call on a
+// GPU writes the tables it needs before its launches, and calls that overlap
+// on one GPU (several engines, several host threads) race on it.
+constexpr long long kScratchBytes = 1LL << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+namespace {
+constexpr int kMaxScratchDevices = 64;
+void* g_scratchPtr[kMaxScratchDevices] = {};
+std::mutex g_scratchMu;
+}  // namespace
+
+// The device address of `scratch` on `device` (which must be the current
+// device), looked up once per device with hipGetSymbolAddress: in host code
+// the array's name is only a stand-in, never a device address.
+extern "C" hipError_t hsg_scratch_address(int device, void** ptr, long long* bytes) {
+    if (device < 0 || device >= kMaxScratchDevices || !ptr || !bytes) return hipErrorInvalidValue;
+    std::lock_guard<std::mutex> lock(g_scratchMu);
+    if (!g_scratchPtr[device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratchPtr[device] = p;
+    }
+    *ptr = g_scratchPtr[device];
+    *bytes = kScratchBytes;
+    return hipSuccess;
+}
 
 // host-side launchers (called from hsolve_engine.cpp)
 extern "C" hipError_t hsg_launch_step(const hsg::LaunchArgs* a, int nblocks, int threads, int fp32, int slow,

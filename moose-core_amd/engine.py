@@ -19,7 +19,7 @@ _LIB = None
 # exported entry points of include/hsolve_amd.h
 EXPORTS = [
     "hsg_create", "hsg_destroy", "hsg_add_cell", "hsg_remove_cell", "hsg_reinit",
-    "hsg_reinit_cell", "hsg_step", "hsg_step_cells", "hsg_get_field", "hsg_set_field",
+    "hsg_reinit_cell", "hsg_step", "hsg_step_cells", "hsg_step_cells_noalloc", "hsg_get_field", "hsg_set_field",
     "hsg_add_inject", "hsg_add_gkek", "hsg_add_conc", "hsg_add_ica", "hsg_set_pulsegen", "hsg_add_spike", "hsg_connect_spike", "hsg_read_spikes",
     "hsg_set_probes", "hsg_set_probes_every", "hsg_set_probe_fields", "hsg_read_probes", "hsg_cell_info_get", "hsg_hines_order",
     "hsg_sync", "hsg_last_step_timing", "hsg_plan_info", "hsg_last_error",
@@ -54,6 +54,7 @@ def load_library(path=None):
         "hsg_reinit_cell": ([vp, i32], C.c_int),
         "hsg_step": ([vp, i64], C.c_int),
         "hsg_step_cells": ([vp, P(i32), P(i64), i32], C.c_int),
+        "hsg_step_cells_noalloc": ([vp, P(i32), P(i64), i32], C.c_int),
         "hsg_get_field": ([vp, i32, i32, P(i32), i32, P(dbl)], C.c_int),
         "hsg_set_field": ([vp, i32, i32, P(i32), i32, P(dbl)], C.c_int),
         "hsg_add_inject": ([vp, i32, i32, dbl], C.c_int),
@@ -167,6 +168,14 @@ class Engine:
         s = np.ascontiguousarray(np.broadcast_to(np.asarray(nsteps, dtype=np.int64), c.shape))
         self._check(self.lib.hsg_step_cells(self.h, cp, s.ctypes.data_as(C.POINTER(C.c_int64)),
                                             len(c)))
+
+    def step_cells_noalloc(self, cells, nsteps):
+        """step_cells without device allocations (hsg_step_cells_noalloc):
+        launch tables in the library's static per-GPU area."""
+        c, cp = _i32(cells)
+        s = np.ascontiguousarray(np.broadcast_to(np.asarray(nsteps, dtype=np.int64), c.shape))
+        self._check(self.lib.hsg_step_cells_noalloc(self.h, cp, s.ctypes.data_as(C.POINTER(C.c_int64)),
+                                                    len(c)))
 
     def sync(self):
         self._check(self.lib.hsg_sync(self.h))
