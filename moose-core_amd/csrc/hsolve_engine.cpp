@@ -41,6 +41,10 @@ extern "C" hipError_t hsg_launch_outputs(const hsg::LaunchArgs* a, int nblocks, 
 extern "C" hipError_t hsg_launch_msg(const hsg::LaunchArgs* a, int nblocks, int threads, int fp32,
                                     hipStream_t stream);
 extern "C" hipError_t hsg_scratch_address(int device, void** ptr, long long* bytes);
+extern "C" hipError_t hsg_launch_block(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, int slow, int fwd,
+                                      int lds, hipStream_t stream);
+extern "C" hipError_t hsg_launch_top(const hsg::LaunchArgs* a, int nblocks, int threads, int slow, int lds,
+                                    hipStream_t stream);
 
 using namespace hsg;
 
@@ -272,6 +276,11 @@ struct hsg_engine {
         int solveThreads = 256;     // threads per cell of the split solve kernel
         double roundMs = 0;         // its dispatches' duration in the last timed round
         int roundSteps = 0;         // ticks of that round
+        bool blocks = false;        // split solve in block form (every cell has blocks)
+        int2* dBlkTiles = nullptr;  // (cell, block) per block workgroup, in the plan's buffer
+        int nBlkTiles = 0;
+        int blockLds = 0;           // dynamic LDS of the block kernels
+        int topLds = 0;             // dynamic LDS of the top kernel
     };
     // A launch plan: the bins of a set of cells, with every bin's cell order
     // and channel tiles in one device buffer (owned by the plan; `external`
@@ -506,6 +515,19 @@ int place_cell(hsg_engine* e, HostCell& hc) {
     // channel phase needs more threads than one workgroup brings
     const bool wantSplit = e->split || (e->opt.kernel_mode == HSG_KERNEL_AUTO && n >= kAutoSplitMin);
     const bool msgOn = wantSplit && s.msgOk && msgLds <= e->ldsLimit;
+    // the block form of the split solve: large cells without the message
+    // form whose schedule splits into blocks that fit LDS
+    bool blockOn = wantSplit && !msgOn && s.blocks.size() >= 2;
+    for (const auto& B : s.blocks) blockOn = blockOn && B.ldsBytes <= e->ldsLimit;
+    int64_t blockBytes = 0;
+    if (blockOn) {
+        blockBytes = align_up((int64_t)sizeof(BlockDev) * (int64_t)s.blocks.size());
+        for (const auto& B : s.blocks)
+            blockBytes += align_up(4 * (int64_t)B.rows.size() + 4) + align_up(4 * (int64_t)B.tExt.size() + 4) +
+                          align_up(4 * (int64_t)B.tImp.size() + 4) + align_up(4 * (int64_t)B.fills.size() + 4) +
+                          align_up(4 * (int64_t)B.hj.size() + 4) +
+                          align_up(4 * (int64_t)B.level.size()) + align_up(4 * (int64_t)B.desc.size());
+    }
     const int64_t nX = (int64_t)(s.ch2d.size() + s.leaks.size() + s.syn.size());
     // the HHChannel2D kinds of the cell: objects with the same powers,
     // tables, axes and instant flags share one record
@@ -551,6 +573,8 @@ int place_cell(hsg_engine* e, HostCell& hc) {
         {&d.oBA, msgOn ? 8 * (2 * (int64_t)s.units.size() + 3) : 0},
         {&d.oFillU, msgOn ? 8 * ((int64_t)s.units.size() + 1) : 0},
         {&d.oVMid, msgOn ? 8 * ((int64_t)n + 1) : 0},
+        // block form (large cells, split mode)
+        {&d.oBlocks, blockBytes},
         // HHChannel2D objects
         {&d.o2dStart, s.ch2d.empty() ? 0 : 4 * ((int64_t)n + 1)},
         {&d.o2dChan, (int64_t)sizeof(Chan2DDev) * (int64_t)s.ch2d.size()},
@@ -671,6 +695,40 @@ int place_cell(hsg_engine* e, HostCell& hc) {
     if (msgOn) {
         st.put(d.oMsgDesc, s.mdesc);
         st.put(d.oRowPos, s.rowPos);
+    }
+    if (blockOn) {
+        // the BlockDev records (the blocks, then the top), then each one's
+        // arrays
+        std::vector<BlockDev> recs(s.blocks.size());
+        int64_t off = d.oBlocks + align_up((int64_t)sizeof(BlockDev) * (int64_t)s.blocks.size());
+        int32_t maxLds = 0;
+        for (size_t k = 0; k < s.blocks.size(); ++k) {
+            const auto& B = s.blocks[k];
+            BlockDev& r = recs[k];
+            std::memset(&r, 0, sizeof r);
+            r.nRows = B.nRows;
+            r.nExt = B.nExt;
+            r.nT = B.nT;
+            r.nTExt = (int32_t)B.tExt.size();
+            r.nTImp = (int32_t)B.tImp.size();
+            r.nHJ = (int32_t)B.hj.size();
+            r.nF = (int32_t)B.fills.size();
+            r.nLevels = (int32_t)(B.level.size() - 1) / kBlockForms;
+            r.nUnits = (int32_t)(B.desc.size() / kDescInts);
+            r.ldsBytes = (int32_t)B.ldsBytes;
+            if (k + 1 < s.blocks.size()) maxLds = std::max(maxLds, r.ldsBytes);
+            r.oRows = off;  st.put(off, B.rows);  off += align_up(4 * (int64_t)B.rows.size() + 4);
+            r.oTExt = off;  st.put(off, B.tExt);  off += align_up(4 * (int64_t)B.tExt.size() + 4);
+            r.oTImp = off;  st.put(off, B.tImp);  off += align_up(4 * (int64_t)B.tImp.size() + 4);
+            r.oFill = off;  st.put(off, B.fills); off += align_up(4 * (int64_t)B.fills.size() + 4);
+            r.oHJ = off;    st.put(off, B.hj);    off += align_up(4 * (int64_t)B.hj.size() + 4);
+            r.oLevel = off; st.put(off, B.level); off += align_up(4 * (int64_t)B.level.size());
+            r.oDesc = off;  st.put(off, B.desc);  off += align_up(4 * (int64_t)B.desc.size());
+        }
+        st.put(d.oBlocks, recs);
+        d.nBlocks = (int32_t)s.blocks.size() - 1;
+        d.blockLds = maxLds;
+        d.topLds = (int32_t)s.blocks.back().ldsBytes;
     }
     if (!s.ch2d.empty()) {
         std::vector<int> tid(s.tables2d.size(), -1);
@@ -1039,9 +1097,13 @@ void plan_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg
 int64_t plan_bytes(hsg_engine* e, const std::vector<hsg_engine::Bin>& bins) {
     int64_t total = 0;
     for (const auto& b : bins) {
-        int64_t nt = 0;
-        for (int id : b.cells) nt += (e->cells[id]->dev.n + kChannelTile - 1) / kChannelTile;
-        total += align_up(4 * (int64_t)std::max<size_t>(1, b.cells.size())) + align_up(8 * std::max<int64_t>(1, nt));
+        int64_t nt = 0, nb = 0;
+        for (int id : b.cells) {
+            nt += (e->cells[id]->dev.n + kChannelTile - 1) / kChannelTile;
+            nb += b.blocks ? e->cells[id]->dev.nBlocks : 0;
+        }
+        total += align_up(4 * (int64_t)std::max<size_t>(1, b.cells.size())) + align_up(8 * std::max<int64_t>(1, nt)) +
+                 align_up(8 * std::max<int64_t>(1, nb));
     }
     return std::max<int64_t>(total, kAlign);
 }
@@ -1063,6 +1125,15 @@ int write_plan_tables(hsg_engine* e, std::vector<hsg_engine::Bin>& bins, char* d
         if (!tiles.empty()) std::memcpy(img.data() + off, tiles.data(), sizeof(int2) * tiles.size());
         b.dTiles = reinterpret_cast<int2*>(dBuf + off);
         off += align_up(8 * std::max<int64_t>(1, (int64_t)tiles.size()));
+        // block-kernel tiles: (cell, block)
+        std::vector<int2> btiles;
+        if (b.blocks)
+            for (int id : b.cells)
+                for (int k = 0; k < e->cells[id]->dev.nBlocks; ++k) btiles.push_back(make_int2(id, k));
+        b.nBlkTiles = (int)btiles.size();
+        if (!btiles.empty()) std::memcpy(img.data() + off, btiles.data(), sizeof(int2) * btiles.size());
+        b.dBlkTiles = reinterpret_cast<int2*>(dBuf + off);
+        off += align_up(8 * std::max<int64_t>(1, (int64_t)btiles.size()));
     }
     if (off > bytes) return fail(HSG_E_STATE, "launch plan larger than its buffer");
     HIPCHK(hipMemcpyAsync(dBuf, img.data(), (size_t)off, hipMemcpyHostToDevice, e->stream));
@@ -1106,7 +1177,8 @@ void plan_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg
         const int s = (d.slowHJ ? 8 : 0) + (autoMode && !d.msg && d.n >= kAutoSplitMin ? 16 : 0);
         if (d.n2d > 0) return s + (d.msg ? 6 : 7);
         if (d.msg) return s + (d.n >= 1024 ? 0 : d.n >= 256 ? 1 : 2);
-        return s + 3 + (d.ldsBytes > 32768 ? 0 : d.ldsBytes > 16384 ? 1 : 2);
+        const int64_t wk = work_lds_bytes(d) + level_bytes(d);
+        return s + 3 + (wk > 32768 ? 0 : wk > 16384 ? 1 : 2);
     };
     std::vector<int32_t> sorted(cells);
     std::stable_sort(sorted.begin(), sorted.end(), [&](int a, int b) {
@@ -1165,6 +1237,16 @@ void plan_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg
         // large cells' levels hold hundreds of units: their split solve
         // (not the message-passing one) sweeps a level with 512 threads
         b.solveThreads = b.split && !b.msg && e->opt.block_threads <= 0 && maxN >= kWideSolveMin ? 512 : b.threads;
+        // the block form when every cell of a split bin has it
+        b.blocks = b.split && !b.msg;
+        b.blockLds = 0;
+        b.topLds = 0;
+        for (int id : b.cells) {
+            const CellDev& d = e->cells[id]->dev;
+            b.blocks = b.blocks && d.nBlocks > 0;
+            b.blockLds = std::max(b.blockLds, (int)d.blockLds);
+            b.topLds = std::max(b.topLds, (int)d.topLds);
+        }
     }
 }
 
@@ -1402,8 +1484,16 @@ int split_wide(hsg_engine* e, const hsg_engine::Bin& bin, LaunchArgs& a, hipStre
 int split_solve(hsg_engine* e, const hsg_engine::Bin& bin, LaunchArgs& a, hipStream_t st) {
     const int nb = (int)bin.cells.size();
     a.deferD = 1;
-    if (bin.msg) HIPCHK(hsg_launch_msg(&a, nb, bin.threads, e->fp32, st));
-    else HIPCHK(hsg_launch_solve(&a, nb, bin.solveThreads, e->fp32, bin.slow, st));
+    if (bin.msg) {
+        HIPCHK(hsg_launch_msg(&a, nb, bin.threads, e->fp32, st));
+    } else if (bin.blocks) {
+        // blocks forward, the cells' tops, blocks backward
+        if (bin.nBlkTiles) HIPCHK(hsg_launch_block(&a, bin.dBlkTiles, bin.nBlkTiles, bin.slow, 1, bin.blockLds, st));
+        HIPCHK(hsg_launch_top(&a, nb, 256, bin.slow, bin.topLds, st));
+        if (bin.nBlkTiles) HIPCHK(hsg_launch_block(&a, bin.dBlkTiles, bin.nBlkTiles, bin.slow, 0, bin.blockLds, st));
+    } else {
+        HIPCHK(hsg_launch_solve(&a, nb, bin.solveThreads, e->fp32, bin.slow, st));
+    }
     HIPCHK(hsg_launch_update(&a, bin.dTiles, bin.nTiles, e->fp32, st));
     HIPCHK(hsg_launch_outputs(&a, nb, st));
     return 0;
@@ -2728,7 +2818,7 @@ int hsg_plan_info(hsg_engine* e, hsg_bin_info* out, int32_t cap, int32_t* n_out)
         std::memset(&o, 0, sizeof o);
         o.cells = (int32_t)x.cells.size();
         o.threads = x.threads;
-        o.solve_threads = x.split ? (x.msg ? x.threads : x.solveThreads) : 0;
+        o.solve_threads = x.split ? (x.msg ? x.threads : x.blocks ? 256 : x.solveThreads) : 0;
         o.lds_bytes = x.lds;
         o.split = x.split;
         o.msg = x.msg;
@@ -2739,7 +2829,9 @@ int hsg_plan_info(hsg_engine* e, hsg_bin_info* out, int32_t cap, int32_t* n_out)
         o.compartments = x.sumN;
         o.round_ms = x.roundMs;
         const char* prec = e->fp32 ? "float" : "double";
-        if (x.split)
+        if (x.split && x.blocks)
+            snprintf(o.kernel, sizeof o.kernel, "hsolve_block_fwd/top/bwd<%s>", prec);
+        else if (x.split)
             snprintf(o.kernel, sizeof o.kernel, "%s<%s>", x.msg ? "hsolve_msg_kernel"
                      : x.solveThreads > 256 ? "hsolve_solve_wide_kernel" : "hsolve_solve_kernel", prec);
         else

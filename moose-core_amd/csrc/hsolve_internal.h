@@ -114,6 +114,30 @@ struct ActDev {
     double  value;
 };
 
+// One block of a large cell's elimination schedule (split mode): a set of
+// subtrees of the unit tree, eliminated by one workgroup with its rows and
+// descriptors in LDS.  Local rows 0..nRows-1 are the rows its units
+// eliminate, nRows..nRows+nExt-1 the rows its subtrees' roots feed
+// (eliminated later by the cell's top units); local term pairs 0..nT-1 stay
+// inside the block, nT..nT+nTExt-1 are the pairs its roots leave for the
+// top; nF fill entries; nHJ junction coefficients of its groups of four
+// or five members.  The cell's top (the units above its blocks, ending
+// at the root row, its only fed row) is one more such record after the
+// blocks, whose nTImp term pairs (all of its pairs) are staged from HBM,
+// where the blocks left theirs.  Arrays are arena offsets; descriptors use
+// local rows, pairs and fills.
+struct BlockDev {
+    int32_t nRows, nExt, nT, nTExt, nF, nLevels, nUnits, ldsBytes, nTImp, nHJ;
+    int64_t oRows;      // int32[nRows + nExt]: global row of each local row
+    int64_t oTExt;      // int32[nTExt]: global term pair of each exported pair
+    int64_t oTImp;      // int32[nTImp]: global term pair of each local pair (top)
+    int64_t oFill;      // int32[nF]: global fill entry of each local fill
+    int64_t oHJ;        // int32[nHJ]: global junction coefficient of each local one
+    int64_t oLevel;     // int32[kBlockForms * nLevels + 1]: per level the start of
+                        // each form (two or three members, four, five), then the end
+    int64_t oDesc;      // int32[kDescInts * nUnits]: local descriptors, level order
+};
+
 // Per-cell descriptor (read by every workgroup working on the cell).
 // One channel slot of a uniform cell for the fast channel pass, packed in
 // three words (read with scalar loads):
@@ -236,6 +260,13 @@ struct CellDev {
     // idx[start[e] .. start[e+1]) (desc order)
     int64_t oPulseStart, oPulseIdx, oVcStart, oVcIdx;
     int64_t oInjOf;         // int32[n]: injection entry of compartment i, -1
+    // block form of the split solve (large cells): nBlocks blocks and the
+    // top (BlockDev records at oBlocks, the top last)
+    int32_t nBlocks;
+    int32_t blockLds;       // largest LDS of its blocks
+    int32_t topLds;         // LDS of its top
+    int32_t pad4;
+    int64_t oBlocks;
 };
 
 // Elimination unit descriptor: 12 int32 (48 bytes), read as three int4 loads.
@@ -278,6 +309,8 @@ constexpr int kChannelTile = 256;  // compartments per workgroup of the channel 
 constexpr int kPairMaxCells = 512;  // split bins of up to this many cells run as pipelined pairs
 constexpr int kWideSolveMin = 4096;  // split bins with a cell from this size solve with 512 threads per cell
 constexpr int kDescInts = 12;
+constexpr int kBlockMaxLds = 64 * 1024;  // LDS of one block (rows, exported rows, terms, fills, levels)
+constexpr int kBlockForms = 3;           // unit forms per block level: 2-3 members, 4, 5
 
 struct SpikeRec {
     int32_t cell;

@@ -129,7 +129,8 @@ __device__ __forceinline__ GT gate_step(GT x, GT C1, GT C2, GT dt, bool instant)
 struct Work {
     double* D;
     double* B;     // becomes VMid during back-substitution
-    double* HJ;    // working junction coefficients of slow-path groups (HBM)
+    double* HJ;    // working junction coefficients of slow-path groups (HBM, or LDS in blocks)
+    const double* HJ0;   // their initial values (HJCopy_): HBM, or the same LDS array in blocks
     double* T;     // deferred terms: pairs (tD, tB)
     double* F;     // fill entry A[1][2] of each three-member unit
 };
@@ -611,7 +612,7 @@ __device__ __forceinline__ void group_members(const Desc& u, int (&m)[S]) {
 }
 
 template <int S>
-__device__ __forceinline__ void fwd_group_desc(const Desc& u, const double* __restrict__ hj0, Work w) {
+__device__ __forceinline__ void fwd_group_desc(const Desc& u, const double* hj0, Work w) {
     int m[S];
     group_members<S>(u, m);
     const int deferLast = u.a.y;
@@ -705,7 +706,7 @@ __device__ __noinline__ void fwd_group_mem(char* arena, const CellDev& c, Desc u
     const int32_t* mem = A_<const int32_t>(arena, c.oGrpMem) + gi.x;
     const int s = gi.y;
     double* HJ = w.HJ;
-    const double* hj0 = A_<const double>(arena, c.oHJ0);
+    const double* hj0 = w.HJ0;
     for (int k = 0; k < s * (s - 1); ++k) HJ[gi.z + k] = hj0[gi.z + k];
     const int deferLast = gi.w;
     for (int p = 0; p + 1 < s; ++p) {
@@ -763,9 +764,8 @@ __device__ __noinline__ void bwd_group_mem(char* arena, const CellDev& c, Desc u
 // have such groups carry them (their registers would lower the occupancy of
 // every other cell's kernel).
 __device__ __noinline__ void fwd_group_regs(char* arena, const CellDev& c, Desc u, Work w) {
-    const double* hj0 = A_<const double>(arena, c.oHJ0);
-    if ((u.a.x & 15) == 4) fwd_group_desc<4>(u, hj0, w);
-    else fwd_group_desc<5>(u, hj0, w);
+    if ((u.a.x & 15) == 4) fwd_group_desc<4>(u, w.HJ0, w);
+    else fwd_group_desc<5>(u, w.HJ0, w);
 }
 
 __device__ __noinline__ void bwd_group_regs(Desc u, Work w) {
@@ -1169,6 +1169,7 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
         w.F = gwork + 2 * n + 2 * nTerm;
     }
     w.HJ = gwork + nWork;
+    w.HJ0 = A_<const double>(arena, c.oHJ0);
     // level table: in LDS after the work arrays, or read from HBM
     const int32_t* gLevel = A_<const int32_t>(arena, c.oLevelStart);
     const int32_t* level;
@@ -1393,6 +1394,254 @@ __global__ void __launch_bounds__(512) hsolve_solve_wide_kernel(LaunchArgs a) {
     const CellDev& c = a.cells[a.order[blockIdx.x]];
     if (c.workInLds) cell_steps<GT, true, true, SLOWG, 0>(a, c, lds, sTypes);
     else cell_steps<GT, true, false, SLOWG, 0>(a, c, lds, sTypes);
+}
+
+// The block form of the split solve for large cells (rows in HBM, HHChannel
+// and 2-D channel rows already written by the tick's channel kernels):
+//   hsolve_block_fwd_kernel  one workgroup per block: its rows (and the rows
+//                            its subtrees feed) and descriptors staged into
+//                            LDS, its units eliminated level by level with
+//                            LDS-only barriers, the rows, exported term
+//                            pairs and fills written back;
+//   hsolve_block_top_kernel  one workgroup per cell: the top block (the units
+//                            above the blocks) in LDS with the pairs the
+//                            blocks exported, forward, the root, backward;
+//                            the VMid of its rows written back;
+//   hsolve_block_bwd_kernel  one workgroup per block: back-substitution from
+//                            the VMid of the rows its subtrees feed.
+// Every unit does the arithmetic of the level schedule on the same values in
+// the same order (hsolve/HSolvePassive.cpp:246-401), so the result is
+// identical to it, bit for bit.
+// LDS of a block: descriptors (48 B per unit, first, 16-byte aligned), D and
+// B of its rows, term pairs, fills, level table.
+struct BlockView {
+    const BlockDev* b;
+    const int32_t* rows;
+    double *gD, *gB, *gT, *gF, *gHJ;
+    const double* gHJ0;
+    const int32_t* hjMap;   // global junction coefficient of each local one
+    Work w;
+    int32_t* level;
+    int4* desc;
+};
+
+__device__ __forceinline__ BlockView block_view(char* arena, const CellDev& c, int blk, double* lds) {
+    BlockView v;
+    v.b = A_<const BlockDev>(arena, c.oBlocks) + blk;
+    const BlockDev& b = *v.b;
+    const int R = b.nRows + b.nExt, P = b.nT + b.nTExt;
+    double* gwork = A_<double>(arena, c.oWork);
+    v.gD = gwork;
+    v.gB = gwork + c.n;
+    v.gT = gwork + 2 * c.n;
+    v.gF = v.gT + 2 * c.nTerm;
+    v.rows = A_<const int32_t>(arena, b.oRows);
+    v.desc = reinterpret_cast<int4*>(lds);
+    double* p = lds + (size_t)6 * b.nUnits;
+    v.w.D = p;
+    v.w.B = p + R;
+    v.w.T = p + 2 * R;
+    v.w.F = v.w.T + 2 * P;
+    // the junction coefficients of its groups: in LDS, the initial values
+    // staged in and overwritten in place by the working ones
+    v.w.HJ = v.w.F + b.nF;
+    v.w.HJ0 = v.w.HJ;
+    v.gHJ0 = A_<const double>(arena, c.oHJ0);
+    v.gHJ = v.gF + c.nFill;
+    v.hjMap = A_<const int32_t>(arena, b.oHJ);
+    v.level = reinterpret_cast<int32_t*>(v.w.HJ + b.nHJ);   // kBlockForms * nLevels + 1 entries
+    return v;
+}
+
+// Staging copies with eight loads in flight per thread before their
+// stores (a plain strided loop waits out one memory latency per element).
+constexpr int kStage = 8;
+
+template <typename T>
+__device__ __forceinline__ void stage_copy(T* __restrict__ dst, const T* __restrict__ src, int count) {
+    const int nt = blockDim.x;
+    for (int k0 = threadIdx.x; k0 < count; k0 += kStage * nt) {
+        T v[kStage];
+#pragma unroll
+        for (int j = 0; j < kStage; ++j) {
+            const int k = k0 + j * nt;
+            v[j] = src[k < count ? k : 0];
+        }
+#pragma unroll
+        for (int j = 0; j < kStage; ++j) {
+            const int k = k0 + j * nt;
+            if (k < count) dst[k] = v[j];
+        }
+    }
+}
+
+// dst[k] = src[idx[k]] (and dst2[k] = src2[idx[k]] when given)
+__device__ __forceinline__ void stage_gather(double* __restrict__ dst, const double* __restrict__ src,
+                                             double* __restrict__ dst2, const double* __restrict__ src2,
+                                             const int32_t* __restrict__ idx, int count) {
+    const int nt = blockDim.x;
+    for (int k0 = threadIdx.x; k0 < count; k0 += kStage * nt) {
+        int r[kStage];
+#pragma unroll
+        for (int j = 0; j < kStage; ++j) {
+            const int k = k0 + j * nt;
+            r[j] = idx[k < count ? k : 0];
+        }
+        double a[kStage], b[kStage];
+#pragma unroll
+        for (int j = 0; j < kStage; ++j) {
+            a[j] = src[r[j]];
+            b[j] = dst2 ? src2[r[j]] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < kStage; ++j) {
+            const int k = k0 + j * nt;
+            if (k < count) {
+                dst[k] = a[j];
+                if (dst2) dst2[k] = b[j];
+            }
+        }
+    }
+}
+
+// the block's level table and descriptors into LDS
+__device__ __forceinline__ void stage_schedule(char* arena, const BlockView& v) {
+    const BlockDev& b = *v.b;
+    stage_copy(v.level, A_<const int32_t>(arena, b.oLevel), kBlockForms * b.nLevels + 1);
+    stage_copy(v.desc, A_<const int4>(arena, b.oDesc), 3 * b.nUnits);
+}
+
+// A block's units with their work arrays in LDS.  The groups of four or
+// five members are inlined here (unlike the step kernels' out-of-line
+// forms), so that their row, term and coefficient accesses stay LDS
+// instructions rather than generic ones.  A level's units come sorted by
+// form (two or three members, four, five); each form's run is laid on
+// whole wavefronts (its lanes padded to a multiple of 64), so a wavefront
+// runs one form's code rather than all three under masks.
+template <bool FWD, bool SLOWG>
+__device__ __forceinline__ void block_level(const BlockView& v, int l) {
+    const int32_t* lv = v.level + kBlockForms * l;
+    const int s0 = lv[0], s1 = lv[1], s2 = lv[2], e = lv[3];
+    const int p0 = (s1 - s0 + 63) & ~63, p1 = (s2 - s1 + 63) & ~63, p2 = (e - s2 + 63) & ~63;
+    for (int q = threadIdx.x; q < p0 + p1 + p2; q += blockDim.x) {
+        if (q < p0) {
+            if (s0 + q < s1) {
+                const Desc d = load_desc(v.desc, s0 + q);
+                if (FWD) fwd_small(d, v.w);
+                else bwd_small(d, v.w);
+            }
+        } else if (q < p0 + p1) {
+            if (SLOWG && s1 + q - p0 < s2) {
+                const Desc d = load_desc(v.desc, s1 + q - p0);
+                if (FWD) fwd_group_desc<4>(d, v.w.HJ0, v.w);
+                else bwd_group_desc<4>(d, v.w);
+            }
+        } else if (SLOWG && s2 + q - p0 - p1 < e) {
+            const Desc d = load_desc(v.desc, s2 + q - p0 - p1);
+            if (FWD) fwd_group_desc<5>(d, v.w.HJ0, v.w);
+            else bwd_group_desc<5>(d, v.w);
+        }
+    }
+}
+
+template <bool SLOWG>
+__device__ __forceinline__ void block_forward(char* arena, const CellDev& c, const BlockView& v) {
+    for (int l = 0; l < v.b->nLevels; ++l) {
+        block_level<true, SLOWG>(v, l);
+        lds_barrier();
+    }
+}
+
+template <bool SLOWG>
+__device__ __forceinline__ void block_backward(char* arena, const CellDev& c, const BlockView& v) {
+    for (int l = v.b->nLevels - 1; l >= 0; --l) {
+        block_level<false, SLOWG>(v, l);
+        lds_barrier();
+    }
+}
+
+template <bool SLOWG>
+__global__ void __launch_bounds__(256) hsolve_block_fwd_kernel(LaunchArgs a, const int2* __restrict__ tiles) {
+    extern __shared__ double lds[];
+    const int2 t = tiles[blockIdx.x];
+    const CellDev& c = a.cells[t.x];
+    char* arena = a.arena;
+    BlockView v = block_view(arena, c, t.y, lds);
+    const BlockDev& b = *v.b;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int R = b.nRows + b.nExt;
+    stage_gather(v.w.D, v.gD, v.w.B, v.gB, v.rows, R);
+    stage_gather(v.w.HJ, v.gHJ0, nullptr, nullptr, v.hjMap, b.nHJ);
+    stage_schedule(arena, v);
+    __syncthreads();
+    block_forward<SLOWG>(arena, c, v);
+    // the rows (eliminated, for the way back; fed, for the top), the term
+    // pairs left for the top, the fills, the working junction coefficients
+    for (int k = tid; k < R; k += nt) {
+        const int r = v.rows[k];
+        v.gD[r] = v.w.D[k];
+        v.gB[r] = v.w.B[k];
+    }
+    const int32_t* tx = A_<const int32_t>(arena, b.oTExt);
+    for (int j = tid; j < b.nTExt; j += nt) {
+        const int g = tx[j];
+        v.gT[2 * g] = v.w.T[2 * (b.nT + j)];
+        v.gT[2 * g + 1] = v.w.T[2 * (b.nT + j) + 1];
+    }
+    const int32_t* fl = A_<const int32_t>(arena, b.oFill);
+    for (int j = tid; j < b.nF; j += nt) v.gF[fl[j]] = v.w.F[j];
+    for (int k = tid; k < b.nHJ; k += nt) v.gHJ[v.hjMap[k]] = v.w.HJ[k];
+}
+
+template <bool SLOWG>
+__global__ void __launch_bounds__(256) hsolve_block_bwd_kernel(LaunchArgs a, const int2* __restrict__ tiles) {
+    extern __shared__ double lds[];
+    const int2 t = tiles[blockIdx.x];
+    const CellDev& c = a.cells[t.x];
+    char* arena = a.arena;
+    BlockView v = block_view(arena, c, t.y, lds);
+    const BlockDev& b = *v.b;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    stage_gather(v.w.D, v.gD, v.w.B, v.gB, v.rows, b.nRows);
+    // the rows the subtrees feed hold their VMid (the top solved them)
+    stage_gather(v.w.B + b.nRows, v.gB, nullptr, nullptr, v.rows + b.nRows, b.nExt);
+    stage_gather(v.w.F, v.gF, nullptr, nullptr, A_<const int32_t>(arena, b.oFill), b.nF);
+    stage_gather(v.w.HJ, v.gHJ, nullptr, nullptr, v.hjMap, b.nHJ);
+    stage_schedule(arena, v);
+    __syncthreads();
+    block_backward<SLOWG>(arena, c, v);
+    for (int k = tid; k < b.nRows; k += nt) v.gB[v.rows[k]] = v.w.B[k];
+}
+
+// The top of each cell in `order` (its record follows the blocks'): the rows
+// and every term pair it uses staged in, forward, the root row (its only fed
+// row, VMid = B / D), backward; VMid of its rows written back.  As the solve
+// kernel of the tick, it advances the cell's step counter.
+template <bool SLOWG>
+__global__ void __launch_bounds__(256) hsolve_block_top_kernel(LaunchArgs a) {
+    extern __shared__ double lds[];
+    const CellDev& c = a.cells[a.order[blockIdx.x]];
+    char* arena = a.arena;
+    BlockView v = block_view(arena, c, c.nBlocks, lds);
+    const BlockDev& b = *v.b;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int R = b.nRows + b.nExt;
+    stage_gather(v.w.D, v.gD, v.w.B, v.gB, v.rows, R);
+    const int32_t* im = A_<const int32_t>(arena, b.oTImp);
+    for (int j = tid; j < b.nTImp; j += nt) {
+        const int g = im[j];
+        v.w.T[2 * j] = v.gT[2 * g];
+        v.w.T[2 * j + 1] = v.gT[2 * g + 1];
+    }
+    stage_gather(v.w.HJ, v.gHJ0, nullptr, nullptr, v.hjMap, b.nHJ);
+    stage_schedule(arena, v);
+    __syncthreads();
+    block_forward<SLOWG>(arena, c, v);
+    if (tid == 0) v.w.B[b.nRows] = v.w.B[b.nRows] / v.w.D[b.nRows];
+    lds_barrier();
+    block_backward<SLOWG>(arena, c, v);
+    for (int k = tid; k < R; k += nt) v.gB[v.rows[k]] = v.w.B[k];
+    if (tid == 0) *A_<int64_t>(arena, c.oSteps) += a.nsteps;
 }
 
 // Rows in HBM (split mode): the row's destination for compartment i.
@@ -1978,14 +2227,38 @@ extern "C" hipError_t hsg_launch_outputs(const hsg::LaunchArgs* a, int nblocks, 
     return hipGetLastError();
 }
 
+extern "C" hipError_t hsg_launch_block(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, int slow, int fwd,
+                                      int lds, hipStream_t stream) {
+    const dim3 g(ntiles), b(256);
+    if (fwd) {
+        if (slow) hipLaunchKernelGGL(hsg::hsolve_block_fwd_kernel<true>, g, b, lds, stream, *a, tiles);
+        else hipLaunchKernelGGL(hsg::hsolve_block_fwd_kernel<false>, g, b, lds, stream, *a, tiles);
+    } else {
+        if (slow) hipLaunchKernelGGL(hsg::hsolve_block_bwd_kernel<true>, g, b, lds, stream, *a, tiles);
+        else hipLaunchKernelGGL(hsg::hsolve_block_bwd_kernel<false>, g, b, lds, stream, *a, tiles);
+    }
+    return hipGetLastError();
+}
+
+extern "C" hipError_t hsg_launch_top(const hsg::LaunchArgs* a, int nblocks, int threads, int slow, int lds,
+                                    hipStream_t stream) {
+    const dim3 g(nblocks), b(threads);
+    if (slow) hipLaunchKernelGGL(hsg::hsolve_block_top_kernel<true>, g, b, lds, stream, *a);
+    else hipLaunchKernelGGL(hsg::hsolve_block_top_kernel<false>, g, b, lds, stream, *a);
+    return hipGetLastError();
+}
+
 extern "C" hipError_t hsg_set_max_lds(int bytes) {
-    const void* kernels[12] = {
+    const void* kernels[18] = {
         (const void*)hsg::hsolve_step_kernel<double>, (const void*)hsg::hsolve_step_kernel<float>,
         (const void*)hsg::hsolve_step_slow_kernel<double>, (const void*)hsg::hsolve_step_slow_kernel<float>,
         (const void*)hsg::hsolve_step_fast_kernel<double>, (const void*)hsg::hsolve_step_fast_kernel<float>,
         (const void*)hsg::hsolve_solve_kernel<double, false>, (const void*)hsg::hsolve_solve_kernel<float, false>,
         (const void*)hsg::hsolve_solve_kernel<double, true>, (const void*)hsg::hsolve_solve_kernel<float, true>,
-        (const void*)hsg::hsolve_msg_kernel<double>, (const void*)hsg::hsolve_msg_kernel<float>};
+        (const void*)hsg::hsolve_msg_kernel<double>, (const void*)hsg::hsolve_msg_kernel<float>,
+        (const void*)hsg::hsolve_block_fwd_kernel<false>, (const void*)hsg::hsolve_block_fwd_kernel<true>,
+        (const void*)hsg::hsolve_block_bwd_kernel<false>, (const void*)hsg::hsolve_block_bwd_kernel<true>,
+        (const void*)hsg::hsolve_block_top_kernel<false>, (const void*)hsg::hsolve_block_top_kernel<true>};
     for (const void* k : kernels) {
         hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
         if (e != hipSuccess) return e;

@@ -19,6 +19,8 @@
 #include "hsolve_internal.h"
 
 #include <algorithm>
+#include <map>
+#include <array>
 #include <cmath>
 #include <cfloat>
 #include <cstring>
@@ -432,6 +434,272 @@ static void schedule_msg(CellSetup& s) {
     s.rootCnt = cnt_of(root);
 }
 
+// ---------------------------------------------------------------------------
+// Block form of the schedule (large cells, split mode).  The units form a tree
+// (a unit's parent eliminates the row it feeds), so any subtree can be
+// eliminated on its own as long as it runs before its ancestors: every row
+// still receives the same updates in the same order (the deferred-term slots
+// keep the sibling order), and the result is the level schedule's, bit for
+// bit.  The largest subtrees that fit kBlockMaxLds bytes of LDS are packed
+// into blocks of at most that size: one workgroup per block runs them with its rows and descriptors in LDS
+// (rows, deferred terms and fills renumbered locally; the rows and term
+// pairs the subtrees' roots feed are exported).  The remaining units (the
+// top: the ancestors of the blocks, a few dozen units) form one more such
+// block, fed only into the root row, run after every block (forward, the
+// root, backward) with the pairs the blocks exported staged in; then the
+// blocks back-substitute.
+// ---------------------------------------------------------------------------
+// (the level table holds kBlockForms starts per level, then the end)
+static int64_t block_lds(int64_t rows, int64_t pairs, int64_t fills, int64_t levels, int64_t units, int64_t hj) {
+    return 16 * rows + 16 * pairs + 8 * fills + 8 * hj + (4 * (kBlockForms * levels + 1) + 15) / 16 * 16 +
+           4 * kDescInts * units;
+}
+
+namespace {
+struct UnitInfo {
+    std::vector<std::array<int, 5>> mem;
+    std::vector<int> sz, level, readPairs, fill3, lastOf, hj;
+};
+}  // namespace
+
+// One block from a level-ordered unit list: rows (eliminated, then `fed`),
+// term pairs (read inside, then exported by `roots`, or, for the top, every
+// pair staged from HBM), fills, local levels and descriptors.  False when a
+// local index would not fit.
+static bool build_block(const CellSetup& s, const UnitInfo& U, const std::vector<int>& us,
+                        const std::vector<int>& roots, const std::vector<int>& fed, bool top,
+                        CellSetup::Block& B) {
+    auto dsc = [&](int u) { return &s.udesc[(size_t)kDescInts * u]; };
+    std::vector<int32_t> elim;
+    for (int u : us)
+        for (int p = 0; p + 1 < U.sz[u]; ++p) elim.push_back(U.mem[u][p]);
+    std::sort(elim.begin(), elim.end());
+    std::map<int, int> local;
+    for (size_t k = 0; k < elim.size(); ++k) local[elim[k]] = (int)k;
+    B.nRows = (int)elim.size();
+    B.rows = elim;
+    for (int r : fed) {
+        local[r] = (int)B.rows.size();
+        B.rows.push_back(r);
+    }
+    B.nExt = (int)fed.size();
+    std::map<int, int> pair;
+    auto map_range = [&](int base, int cnt) {
+        for (int t = 0; t < cnt; ++t)
+            if (!pair.count(base + t)) pair[base + t] = (int)pair.size();
+    };
+    for (int u : us) {
+        const int32_t* d = dsc(u);
+        if (d[0] & 32) {
+            for (int k = 0; k < 3; ++k)
+                if (d[6 + k] >= 0) map_range(d[6 + k] & 0xffffff, (d[6 + k] >> 24) & 0x7f);
+        } else if (U.readPairs[u] > 0) {
+            map_range(d[2], U.readPairs[u]);
+        }
+    }
+    B.nT = (int)pair.size();
+    for (int r : roots) {
+        const int32_t* d = dsc(r);
+        if (d[1] < 0) continue;
+        for (int t = 0; t < U.sz[r] - 1; ++t) {
+            pair[d[1] + t] = (int)pair.size();
+            B.tExt.push_back(d[1] + t);
+        }
+    }
+    // every unit's output pairs are read by its parent inside (roots: exported)
+    for (int u : us)
+        if (dsc(u)[1] >= 0 && !pair.count(dsc(u)[1])) return false;
+    if (top) {
+        B.tImp.assign(pair.size(), 0);
+        for (auto& kv : pair) B.tImp[kv.second] = kv.first;
+    }
+    int lo = U.level[us.front()], hi = U.level[us.back()];
+    // local levels: the units' own levels, empty ones dropped; within a
+    // level the units by form (two or three members, four, five), each
+    // form's start in the level table, so that a wavefront runs one form
+    std::vector<int> lvOf(hi - lo + 1, -1);
+    int nLv = 0;
+    for (int u : us)
+        if (lvOf[U.level[u] - lo] < 0) lvOf[U.level[u] - lo] = nLv++;
+    auto form = [&](int u) { return (dsc(u)[0] & 32) ? U.sz[u] - 3 : 0; };   // 0, or 1-2: register-form groups
+    B.level.assign((size_t)kBlockForms * nLv + 1, 0);
+    for (int u : us) B.level[(size_t)kBlockForms * lvOf[U.level[u] - lo] + form(u) + 1]++;
+    for (size_t k = 0; k + 1 < B.level.size(); ++k) B.level[k + 1] += B.level[k];
+    B.desc.assign((size_t)kDescInts * us.size(), 0);
+    std::vector<int> pos(B.level.begin(), B.level.end() - 1);
+    for (int u : us) {
+        int32_t* o = &B.desc[(size_t)kDescInts * pos[(size_t)kBlockForms * lvOf[U.level[u] - lo] + form(u)]++];
+        const int32_t* d = dsc(u);
+        std::memcpy(o, d, sizeof(int32_t) * kDescInts);
+        auto L = [&](int m) { return local.at(m); };
+        const auto& m = U.mem[u];
+        o[1] = d[1] >= 0 ? pair.at(d[1]) : -1;
+        if (d[0] & 32) {
+            // the group's junction coefficients, renumbered into the block's
+            o[2] = (int)B.hj.size();
+            for (int k = 0; k < U.sz[u] * (U.sz[u] - 1); ++k) B.hj.push_back(d[2] + k);
+            o[3] = (L(m[0]) & 0xffff) | (L(m[1]) << 16);
+            o[4] = (L(m[2]) & 0xffff) | (L(m[3]) << 16);
+            o[5] = U.sz[u] == 5 ? L(m[4]) : 0;
+            for (int k = 0; k < 3; ++k)
+                if (d[6 + k] >= 0) o[6 + k] = pair.at(d[6 + k] & 0xffffff) | (d[6 + k] & ~0xffffff);
+        } else {
+            o[2] = U.readPairs[u] > 0 ? pair.at(d[2]) : 0;
+            o[3] = (L(m[0]) & 0xffff) | (L(m[1]) << 16);
+            o[4] = U.sz[u] == 3 ? L(m[2]) : L(m[1]);
+            if (U.sz[u] == 3) {
+                o[5] = (int)B.fills.size();
+                B.fills.push_back(d[5]);
+            } else {
+                o[5] = -1;
+            }
+        }
+    }
+    const int nr = (int)B.rows.size(), np = (int)pair.size(), nf = (int)B.fills.size();
+    B.ldsBytes = block_lds(nr, np, nf, nLv, (int64_t)us.size(), (int64_t)B.hj.size());
+    // every local index in range (rows, term pairs, fills): the block kernels
+    // read and write exactly these
+    bool ok = nr <= 65535 && B.ldsBytes <= kBlockMaxLds;
+    for (size_t k = 0; ok && k < us.size(); ++k) {
+        const int32_t* o = &B.desc[(size_t)kDescInts * k];
+        const int S = o[0] & 15;
+        int mm[5] = {o[3] & 0xffff, (o[3] >> 16) & 0xffff, 0, 0, 0};
+        if (o[0] & 32) {
+            mm[2] = o[4] & 0xffff; mm[3] = (o[4] >> 16) & 0xffff; mm[4] = o[5];
+            for (int q = 0; q < 3; ++q)
+                if (o[6 + q] >= 0) ok = ok && (o[6 + q] & 0xffffff) + ((o[6 + q] >> 24) & 0x7f) <= np;
+        } else {
+            mm[2] = o[4];
+            if (S == 3) ok = ok && o[5] >= 0 && o[5] < nf && o[2] + ((o[0] >> 8) & 15) <= np;
+        }
+        for (int q = 0; q < S; ++q) ok = ok && mm[q] >= 0 && mm[q] < nr;
+        ok = ok && ((o[0] & 32) ? (S == 4 || S == 5) : S <= 3);
+        if (o[1] >= 0) ok = ok && o[1] + S - 1 <= np;
+    }
+    return ok;
+}
+
+static void schedule_blocks(CellSetup& s) {
+    s.blocks.clear();
+    const int n = s.n;
+    const int nU = (int)s.units.size();
+    if (n < kAutoSplitMin || nU == 0) return;
+    const int nL = (int)s.levelStart.size() - 1;
+    auto dsc = [&](int u) { return &s.udesc[(size_t)kDescInts * u]; };
+    UnitInfo U;
+    U.mem.resize(nU);
+    U.sz.assign(nU, 0);
+    U.level.assign(nU, 0);
+    U.readPairs.assign(nU, 0);
+    U.fill3.assign(nU, 0);
+    U.lastOf.assign(nU, 0);
+    U.hj.assign(nU, 0);
+    for (int l = 0; l < nL; ++l)
+        for (int u = s.levelStart[l]; u < s.levelStart[l + 1]; ++u) U.level[u] = l;
+    for (int u = 0; u < nU; ++u) {
+        const int32_t* d = dsc(u);
+        if (d[0] & 16) return;   // a group kept in memory: no block form
+        const int S = d[0] & 15;
+        U.sz[u] = S;
+        auto& m = U.mem[u];
+        m = {d[3] & 0xffff, (d[3] >> 16) & 0xffff, 0, 0, 0};
+        if (d[0] & 32) {
+            m[2] = d[4] & 0xffff;
+            m[3] = (d[4] >> 16) & 0xffff;
+            m[4] = d[5];
+            for (int k = 0; k < 3; ++k)
+                if (d[6 + k] >= 0) U.readPairs[u] += (d[6 + k] >> 24) & 0x7f;
+            U.hj[u] = S * (S - 1);
+        } else if (S == 3) {
+            m[2] = d[4];
+            U.readPairs[u] = (d[0] >> 8) & 15;
+            U.fill3[u] = 1;
+        }
+        U.lastOf[u] = m[S - 1];
+    }
+    std::vector<int> elimUnit(n, -1), parent(nU);
+    for (int u = 0; u < nU; ++u)
+        for (int p = 0; p + 1 < U.sz[u]; ++p) elimUnit[U.mem[u][p]] = u;
+    for (int u = 0; u < nU; ++u) parent[u] = U.lastOf[u] == n - 1 ? -1 : elimUnit[U.lastOf[u]];
+    for (int u = 0; u < nU; ++u)
+        if (parent[u] >= 0 && U.level[parent[u]] <= U.level[u]) return;   // not a unit tree in level order
+    // subtree totals (children come before parents in level order)
+    std::vector<int64_t> rowsSub(nU, 0), pairsSub(nU, 0), fillsSub(nU, 0), unitsSub(nU, 0), hjSub(nU, 0);
+    for (int u = 0; u < nU; ++u) {
+        rowsSub[u] += U.sz[u] - 1;
+        pairsSub[u] += U.readPairs[u];
+        fillsSub[u] += U.fill3[u];
+        unitsSub[u] += 1;
+        hjSub[u] += U.hj[u];
+        if (parent[u] >= 0) {
+            rowsSub[parent[u]] += rowsSub[u];
+            pairsSub[parent[u]] += pairsSub[u];
+            fillsSub[parent[u]] += fillsSub[u];
+            unitsSub[parent[u]] += unitsSub[u];
+            hjSub[parent[u]] += hjSub[u];
+        }
+    }
+    // a subtree fits a block when its rows, the row it feeds, its term pairs
+    // (read inside, then exported), fills, levels, descriptors and junction
+    // coefficients fit the block's LDS
+    auto fits = [&](int u) {
+        return block_lds(rowsSub[u] + 1, pairsSub[u] + (U.sz[u] - 1), fillsSub[u], U.level[u] + 1, unitsSub[u],
+                         hjSub[u]) <= kBlockMaxLds;
+    };
+    std::vector<int> roots;
+    for (int u = 0; u < nU; ++u)
+        if (fits(u) && (parent[u] < 0 || !fits(parent[u]))) {
+            if (parent[u] < 0) return;   // the whole cell is one block: nothing to gain
+            roots.push_back(u);
+        }
+    if (roots.size() < 2) return;
+    std::vector<int> sub(nU, -1);
+    for (int u : roots) sub[u] = u;
+    for (int u = nU - 1; u >= 0; --u)
+        if (sub[u] < 0 && parent[u] >= 0 && sub[parent[u]] >= 0) sub[u] = sub[parent[u]];
+    // pack subtrees into blocks, in the order of the rows they feed
+    std::sort(roots.begin(), roots.end(), [&](int a, int b) { return U.lastOf[a] < U.lastOf[b]; });
+    std::vector<int> packOf(nU, -1);
+    std::vector<std::vector<int>> packs;
+    {
+        int64_t r = 0, pr = 0, f = 0, lv = 0, nu = 0, hj = 0;
+        for (int u : roots) {
+            const int64_t nr = r + rowsSub[u] + 1, np = pr + pairsSub[u] + (U.sz[u] - 1), nf = f + fillsSub[u];
+            const int64_t nlv = std::max<int64_t>(lv, U.level[u] + 1), nnu = nu + unitsSub[u], nhj = hj + hjSub[u];
+            if (packs.empty() || block_lds(nr, np, nf, nlv, nnu, nhj) > kBlockMaxLds) {
+                packs.push_back({});
+                r = rowsSub[u] + 1;
+                pr = pairsSub[u] + (U.sz[u] - 1);
+                f = fillsSub[u];
+                lv = U.level[u] + 1;
+                nu = unitsSub[u];
+                hj = hjSub[u];
+            } else {
+                r = nr; pr = np; f = nf; lv = nlv; nu = nnu; hj = nhj;
+            }
+            packs.back().push_back(u);
+            packOf[u] = (int)packs.size() - 1;
+        }
+    }
+    std::vector<std::vector<int>> unitsOf(packs.size());
+    std::vector<int> topUnits;
+    for (int u = 0; u < nU; ++u) {
+        if (sub[u] >= 0) unitsOf[packOf[sub[u]]].push_back(u);   // level order kept
+        else topUnits.push_back(u);
+    }
+    s.blocks.resize(packs.size() + 1);
+    for (size_t b = 0; b < packs.size(); ++b) {
+        std::vector<int> fed;
+        for (int r : packs[b]) fed.push_back(U.lastOf[r]);
+        if (!build_block(s, U, unitsOf[b], packs[b], fed, false, s.blocks[b])) { s.blocks.clear(); return; }
+    }
+    if (topUnits.empty() ||
+        !build_block(s, U, topUnits, {}, {n - 1}, true, s.blocks.back())) {
+        s.blocks.clear();
+        return;
+    }
+}
+
 int setup_cell(const hsg_cell_desc* d, double dt, int caAdvance, CellSetup& s, std::string& err) {
     if (!d) { err = "null cell description"; return HSG_E_ARG; }
     if (!(dt > 0.0)) { err = "HSolve: 'dt' must be set (positive) before 'target'"; return HSG_E_ARG; }
@@ -539,6 +807,7 @@ int setup_cell(const hsg_cell_desc* d, double dt, int caAdvance, CellSetup& s, s
     rc = schedule(s, junctionGroup, err);
     if (rc) return rc;
     schedule_msg(s);
+    schedule_blocks(s);
 
     // readHHChannels / readGates
     std::vector<std::vector<int>> chans(nd);

@@ -169,6 +169,21 @@ struct CellSetup {
     std::vector<int32_t> ch2dStart;
     std::vector<int> ch2dOfDesc;             // desc index -> position in ch2d
     std::vector<HostTable2D> tables2d;
+    // block form of the elimination schedule (large cells, split mode):
+    // blocks of subtrees of the unit tree with local rows/terms/fills, and
+    // the remaining (top) units in the ordinary format
+    struct Block {
+        std::vector<int32_t> rows;     // global rows: eliminated (nRows), then fed (nExt)
+        int nRows = 0, nExt = 0, nT = 0;
+        std::vector<int32_t> tExt;     // global term pair of each exported pair
+        std::vector<int32_t> tImp;     // global term pair of each local pair (the top)
+        std::vector<int32_t> fills;    // global fill entry of each local fill
+        std::vector<int32_t> hj;       // global junction coefficient of each local one
+        std::vector<int32_t> level;    // local level table
+        std::vector<int32_t> desc;     // local descriptors, level order
+        int64_t ldsBytes = 0;
+    };
+    std::vector<Block> blocks;         // the blocks, then the top
     std::vector<HostChanType> localTypes;    // indexed by local type id
     std::vector<int> chTypeLocal;            // [k*n+i] local type id
     HostTableSet tables;

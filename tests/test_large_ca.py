@@ -3,8 +3,9 @@
 compartment, the K-AHP as an HHChannel2D on a 3000 x 500 Vm x [Ca] table),
 injected so that each fires, dt = 20 us, 1000 steps, in the engine's
 automatic mode: both cells take the split path (batch-wide channel kernel,
-per-cell solve), run as a pipelined pair with the 512-thread wide solve,
-here also replayed as captured graphs of 2 x 50 steps.
+then the block form of the solve: blocks of subtrees eliminated by their own
+workgroups in LDS, the cell's top units, the blocks' back-substitution), run
+as a pipelined pair, here also replayed as captured graphs of 2 x 50 steps.
 
 FP64 form: within 1e-6 V of the oracle at every sampled compartment and step
 (not bit for bit: the HHChannel2D gates use exp), identical spike steps.
@@ -59,9 +60,10 @@ def test_large_ca_cells_auto_mode(pkg, reference, precision):
         eng.set_probes(cid, _probes(s), NSTEPS)
     eng.reinit(DT)
     plan = eng.plan_info()
-    # the path under test: split bins, a pipelined pair, the wide solve
+    # the path under test: split bins, a pipelined pair, the block form of
+    # the solve
     assert all(b["split"] for b in plan) and len(plan) == 2
-    assert all(b["solve_threads"] == 512 for b in plan)
+    assert all("block" in b["kernel"] for b in plan)
     for k in (300, 700):
         eng.step(k)
     spikes = eng.read_spikes()
