@@ -45,24 +45,23 @@ from moose_core_amd import workloads  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 
 
-def bytes_per_cs(w_gate, w_2d, gates, chans, gates2d=0, chans2d=0, pools=0):
+def bytes_per_cs(w_chan, w_2d, gates, chans, gates2d=0, chans2d=0, pools=0):
     """Algorithmic bytes per compartment-step: the compulsory traffic with
     the state resident between steps, lookup tables and elimination
     temporaries excluded (they stay in LDS/L2).  V read+write 16; every gate
-    state read+written (w bytes each); Gbar and Ek of every channel read (w
-    each); CmByDt, EmByRm, the diagonal base and the off-diagonal (FP64) 32;
-    per Ca pool its state read+written, f1 and f2 read (FP64) 32.  w = 8 in
-    the FP64 form, 4 for the HHChannels' states and parameters in the mixed
-    form; the HHChannel2D objects stay FP64 in both (w_2d = 8)."""
-    return (16 + 2 * w_gate * gates + 2 * w_gate * chans + 2 * w_2d * gates2d + 2 * w_2d * chans2d
-            + 32 + 32 * pools)
+    state read+written (FP64 in both forms) 16; Gbar and Ek of every channel
+    read (w each); CmByDt, EmByRm, the diagonal base and the off-diagonal
+    (FP64) 32; per Ca pool its state read+written, f1 and f2 read (FP64) 32.
+    w = 8 in the FP64 form and 4 for the HHChannels in the mixed form; the
+    HHChannel2D objects stay FP64 in both (w_2d = 8)."""
+    return 16 + 16 * gates + 2 * w_chan * chans + 16 * gates2d + 2 * w_2d * chans2d + 32 + 32 * pools
 
 
-# squid HH cells: Na m^3 h and K n^4 (3 gates, 2 channels) -> 128 / 88 B
+# squid HH cells: Na m^3 h and K n^4 (3 gates, 2 channels) -> 128 / 112 B
 BYTES_PER_CS = {"fp64": bytes_per_cs(8, 8, 3, 2), "mixed": bytes_per_cs(4, 8, 3, 2)}
 # the large Ca cells: Na m^3 h, Kdr n^4, KA a^3 b, CaL s^2 (6 gates, 4
 # channels), the 2-D K-AHP (1 gate, 1 channel, FP64 in both forms) and one
-# CaConc per compartment -> 272 / 192 B
+# CaConc per compartment -> 272 / 240 B
 BYTES_PER_CS_LARGE = {"fp64": bytes_per_cs(8, 8, 6, 4, 1, 1, 1), "mixed": bytes_per_cs(4, 8, 6, 4, 1, 1, 1)}
 
 

@@ -50,8 +50,10 @@ extern "C" {
 
 /* ---- precision forms --------------------------------------------------- */
 #define HSG_FP64   0   /* everything in FP64 (the reference's arithmetic)           */
-#define HSG_MIXED  1   /* gate tables, gate states, channel currents in FP32;
-                          Hines elimination, back-substitution and Vm in FP64      */
+#define HSG_MIXED  1   /* rate tables and their interpolation, Gbar, Ek, gate
+                          powers and channel currents in FP32; gate states and
+                          their update, Hines elimination, back-substitution and
+                          Vm in FP64                                               */
 
 /* ---- model description (MOOSE object level) ---------------------------- */
 

@@ -553,7 +553,7 @@ int place_cell(hsg_engine* e, HostCell& hc) {
         {&d.oV, 8 * n}, {&d.oInitVm, 8 * n}, {&d.oCmByDt, 8 * n}, {&d.oEmByRm, 8 * n}, {&d.oDbase, 8 * n}, {&d.oOff, 8 * n},
         {&d.oDefer, 8 * n},
         {&d.oChType, 4 * CS}, {&d.oGbar, gt * CS}, {&d.oEk, gt * CS}, {&d.oMod, 8 * CS}, {&d.oExtCa, 8 * CS},
-        {&d.oState, gt * GS},
+        {&d.oState, 8 * GS},
         {&d.oPoolC, 8 * PS}, {&d.oPoolCa, 8 * PS}, {&d.oPoolBasal, 8 * PS}, {&d.oPoolF1, 8 * PS},
         {&d.oPoolF2, 8 * PS}, {&d.oPoolCeil, 8 * PS}, {&d.oPoolFloor, 8 * PS}, {&d.oPoolICa, 8 * PS},
         {&d.oPoolCount, 4 * n},
@@ -673,7 +673,7 @@ int place_cell(hsg_engine* e, HostCell& hc) {
     put_gt(d.oGbar, s.Gbar);
     put_gt(d.oEk, s.Ek);
     st.put(d.oMod, s.mod);
-    put_gt(d.oState, s.state);
+    st.put(d.oState, s.state);
     st.put(d.oPoolC, s.pC);
     st.put(d.oPoolCa, s.pCa);
     st.put(d.oPoolBasal, s.pBasal);
@@ -934,19 +934,6 @@ HostCell* get_cell(hsg_engine* e, int32_t cell) {
     if (!e || cell < 0 || cell >= (int32_t)e->cells.size() || !e->cells[cell] || !e->cells[cell]->live)
         return nullptr;
     return e->cells[cell].get();
-}
-
-// state arrays of a cell as doubles
-int download_gt(hsg_engine* e, int64_t off, size_t count, std::vector<double>& out) {
-    out.assign(count, 0.0);
-    if (e->fp32) {
-        std::vector<float> f(count);
-        int rc = download(e, off, count, f.data());
-        if (rc) return rc;
-        for (size_t i = 0; i < count; ++i) out[i] = f[i];
-        return 0;
-    }
-    return download(e, off, count, out.data());
 }
 
 int put_gt(hsg_engine* e, int64_t off, size_t index, double v) {
@@ -2154,7 +2141,10 @@ int hsg_get_field(hsg_engine* e, int32_t cell, int32_t field, const int32_t* idx
     int rc;
     auto needV = [&]() -> int { if (V.empty()) { V.resize(nc); return download(e, hc->dev.oV, nc, V.data()); } return 0; };
     auto needState = [&]() -> int {
-        if (state.empty() && s.gateSlots > 0) return download_gt(e, hc->dev.oState, (size_t)s.gateSlots * nc, state);
+        if (state.empty() && s.gateSlots > 0) {
+            state.resize((size_t)s.gateSlots * nc);
+            return download(e, hc->dev.oState, state.size(), state.data());
+        }
         return 0;
     };
     for (int q = 0; q < n; ++q) {
@@ -2381,16 +2371,17 @@ int hsg_set_field(hsg_engine* e, int32_t cell, int32_t field, const int32_t* idx
             if ((field == HSG_F_GBAR || field == HSG_F_X || field == HSG_F_Y || field == HSG_F_Z ||
                  field == HSG_F_MODULATION) && !hc->gkOverride.count(i)) {
                 std::vector<double> st;
-                if (s.gateSlots > 0 && (rc = download_gt(e, hc->dev.oState, (size_t)s.gateSlots * nc, st))) return rc;
+                st.resize((size_t)s.gateSlots * nc);
+                if (s.gateSlots > 0 && (rc = download(e, hc->dev.oState, st.size(), st.data()))) return rc;
                 hc->gkOverride[i] = e->fp32 ? chan_gk<float>(*hc, i, st) : chan_gk<double>(*hc, i, st);
             }
             switch (field) {
             case HSG_F_GBAR: s.Gbar[o] = x; if ((rc = put_gt(e, hc->dev.oGbar, o, x))) return rc; break;
             case HSG_F_EK: s.Ek[o] = x; if ((rc = put_gt(e, hc->dev.oEk, o, x))) return rc; break;
             case HSG_F_GK: case HSG_F_IK: return fail(HSG_E_UNSUPPORTED, "Gk/Ik are recomputed every step");
-            case HSG_F_X: if (s.chanXp[i] != 0.0 && (rc = put_gt(e, hc->dev.oState, stateIdx(0), x))) return rc; break;
-            case HSG_F_Y: if (s.chanYp[i] != 0.0 && (rc = put_gt(e, hc->dev.oState, stateIdx(1), x))) return rc; break;
-            case HSG_F_Z: if (s.chanZp[i] != 0.0 && (rc = put_gt(e, hc->dev.oState, stateIdx(2), x))) return rc; break;
+            case HSG_F_X: if (s.chanXp[i] != 0.0 && (rc = put1<double>(e, hc->dev.oState, stateIdx(0), x))) return rc; break;
+            case HSG_F_Y: if (s.chanYp[i] != 0.0 && (rc = put1<double>(e, hc->dev.oState, stateIdx(1), x))) return rc; break;
+            case HSG_F_Z: if (s.chanZp[i] != 0.0 && (rc = put1<double>(e, hc->dev.oState, stateIdx(2), x))) return rc; break;
             case HSG_F_MODULATION:
                 if (x > 0.0) {   // HSolve::setHHmodulation ignores non-positive values
                     s.mod[o] = x;

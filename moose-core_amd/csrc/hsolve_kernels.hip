@@ -27,9 +27,12 @@
 // fields and table parameters sit in registers, and the channel types of a
 // uniform cell sit in LDS.
 //
-// GT is the gate/channel type: double (FP64 form) or float (mixed form, where
-// gate states, tables and channel currents are FP32 and the Hines solve stays
-// FP64).
+// GT is the channel type: double (FP64 form) or float (mixed form, where the
+// rate tables and their interpolation, Gbar, Ek, the gate powers and the
+// channel currents are FP32).  Gate states and their trapezoidal update, the
+// Hines solve and Vm are FP64 in both forms: a gate state stored in FP32
+// loses the per-step increments that are below half its ulp (near rest they
+// are), and the bias this leaves moves spike times by several steps.
 #include <hip/hip_runtime.h>
 
 #include <mutex>
@@ -143,7 +146,7 @@ struct View {
     const double* __restrict__ CmByDt;
     const double* __restrict__ EmByRm;
     const double* __restrict__ Dbase;
-    GT* __restrict__ state;
+    double* __restrict__ state;   // gate states: FP64 in both forms
     const GT* __restrict__ Gbar;
     const GT* __restrict__ Ek;
     const double* __restrict__ mod;
@@ -167,7 +170,7 @@ __device__ __forceinline__ View<GT> make_view(char* arena, const CellDev& c, con
     v.CmByDt = A_<const double>(arena, c.oCmByDt);
     v.EmByRm = A_<const double>(arena, c.oEmByRm);
     v.Dbase = A_<const double>(arena, c.oDbase);
-    v.state = A_<GT>(arena, c.oState);
+    v.state = A_<double>(arena, c.oState);
     v.Gbar = A_<const GT>(arena, c.oGbar);
     v.Ek = A_<const GT>(arena, c.oEk);
     v.mod = A_<const double>(arena, c.oMod);
@@ -187,7 +190,7 @@ __device__ __forceinline__ View<GT> make_view(char* arena, const CellDev& c, con
 // ---------------------------------------------------------------------------
 template <typename GT, bool UNIFORM>
 __device__ __forceinline__ void channel_row(const View<GT>& v, const ChanType* __restrict__ types, int i,
-                                            GT dtg, Work w, bool reinitOnly, int dst) {
+                                            double dtg, Work w, bool reinitOnly, int dst) {
     const int n = v.n;
     const double V = v.V[i];
     // pools of this compartment; the Ca row of a pool (caRowCompt_) is
@@ -207,23 +210,25 @@ __device__ __forceinline__ void channel_row(const View<GT>& v, const ChanType* _
         GT C1, C2;
         GT fraction = v.hasModulation ? (GT)v.mod[(size_t)k * n + i] : (GT)1.0;
         if (ct.Xpower > 0.0) {
-            GT* sp = v.state + (size_t)js * n + i;
+            double* sp = v.state + (size_t)js * n + i;
             table_lookup<GT>(v.vt, vBase, ct.colX, frac, C1, C2);
-            GT x = reinitOnly ? C1 / C2 : gate_step<GT>(*sp, C1, C2, dtg, ct.instant & 1);
+            const double x = reinitOnly ? (double)C1 / (double)C2
+                                        : gate_step<double>(*sp, C1, C2, dtg, ct.instant & 1);
             *sp = x;
-            fraction *= take_power<GT>(x, ct.Xpower);
+            fraction *= take_power<GT>((GT)x, ct.Xpower);
             ++js;
         }
         if (ct.Ypower > 0.0) {
-            GT* sp = v.state + (size_t)js * n + i;
+            double* sp = v.state + (size_t)js * n + i;
             table_lookup<GT>(v.vt, vBase, ct.colY, frac, C1, C2);
-            GT x = reinitOnly ? C1 / C2 : gate_step<GT>(*sp, C1, C2, dtg, ct.instant & 2);
+            const double x = reinitOnly ? (double)C1 / (double)C2
+                                        : gate_step<double>(*sp, C1, C2, dtg, ct.instant & 2);
             *sp = x;
-            fraction *= take_power<GT>(x, ct.Ypower);
+            fraction *= take_power<GT>((GT)x, ct.Ypower);
             ++js;
         }
         if (ct.Zpower > 0.0) {
-            GT* sp = v.state + (size_t)js * n + i;
+            double* sp = v.state + (size_t)js * n + i;
             if (ct.caDepLocal >= 0 && ct.caDepLocal < nPools) {
                 int cb;
                 double cf;
@@ -240,9 +245,10 @@ __device__ __forceinline__ void channel_row(const View<GT>& v, const ChanType* _
                     table_lookup<GT>(v.vt, vBase, ct.colZ, frac, C1, C2);
                 }
             }
-            GT x = reinitOnly ? C1 / C2 : gate_step<GT>(*sp, C1, C2, dtg, ct.instant & 4);
+            const double x = reinitOnly ? (double)C1 / (double)C2
+                                        : gate_step<double>(*sp, C1, C2, dtg, ct.instant & 4);
             *sp = x;
-            fraction *= take_power<GT>(x, ct.Zpower);
+            fraction *= take_power<GT>((GT)x, ct.Zpower);
             ++js;
         }
         if (!reinitOnly) {
@@ -850,9 +856,9 @@ __device__ __forceinline__ void calcium_row(char* arena, const CellDev& c, const
             if (ct.caTgtLocal == q) {
                 GT fraction = v.hasModulation ? (GT)v.mod[(size_t)k * n + i] : (GT)1.0;
                 int jj = js;
-                if (ct.Xpower > 0.0) fraction *= take_power<GT>(v.state[(size_t)(jj++) * n + i], ct.Xpower);
-                if (ct.Ypower > 0.0) fraction *= take_power<GT>(v.state[(size_t)(jj++) * n + i], ct.Ypower);
-                if (ct.Zpower > 0.0) fraction *= take_power<GT>(v.state[(size_t)(jj++) * n + i], ct.Zpower);
+                if (ct.Xpower > 0.0) fraction *= take_power<GT>((GT)v.state[(size_t)(jj++) * n + i], ct.Xpower);
+                if (ct.Ypower > 0.0) fraction *= take_power<GT>((GT)v.state[(size_t)(jj++) * n + i], ct.Ypower);
+                if (ct.Zpower > 0.0) fraction *= take_power<GT>((GT)v.state[(size_t)(jj++) * n + i], ct.Zpower);
                 const double Gk = (double)(v.Gbar[(size_t)k * n + i] * fraction);
                 const double ek = (double)v.Ek[(size_t)k * n + i];
                 if (v.caAdvance == 1) act += Gk * (ek - vmid);
@@ -1034,7 +1040,7 @@ __device__ __forceinline__ GT power_code(GT x, int code) {
 }
 
 template <typename GT, int MS>
-__device__ __forceinline__ void channel_row_fast(const View<GT>& v, const CellDev& c, int i, GT dtg, double& Dout,
+__device__ __forceinline__ void channel_row_fast(const View<GT>& v, const CellDev& c, int i, double dtg, double& Dout,
                                                  double& Bout) {
     const int n = v.n;
     const int S = v.nChanSlots;
@@ -1043,8 +1049,9 @@ __device__ __forceinline__ void channel_row_fast(const View<GT>& v, const CellDe
     const double Dbase = v.Dbase[i], CmByDt = v.CmByDt[i], EmByRm = v.EmByRm[i];
     const int nPools = v.hasPools ? v.poolCount[i] : 0;
     const double* __restrict__ caSrc = v.hasPools ? v.poolCa : v.V;
-    const GT* __restrict__ stSrc = c.nGateSlots > 0 ? v.state : reinterpret_cast<const GT*>(v.V);
-    GT st[MS][3], gb[MS], ek[MS];
+    const double* __restrict__ stSrc = c.nGateSlots > 0 ? v.state : v.V;
+    double st[MS][3];
+    GT gb[MS], ek[MS];
     double md[MS], cav[MS], ext[MS];
 #pragma unroll
     for (int k = 0; k < MS; ++k) {
@@ -1116,9 +1123,9 @@ __device__ __forceinline__ void channel_row_fast(const View<GT>& v, const CellDe
             const GT fr = ax == 2 ? zf[k] : frac;
             const GT C1 = raw[k][ax][0] + (raw[k][ax][1] - raw[k][ax][0]) * fr;
             const GT C2 = raw[k][ax][2] + (raw[k][ax][3] - raw[k][ax][2]) * fr;
-            const GT x = gate_step<GT>(st[k][ax], C1, C2, dtg, (w2 >> (9 + ax)) & 1);
+            const double x = gate_step<double>(st[k][ax], C1, C2, dtg, (w2 >> (9 + ax)) & 1);
             st[k][ax] = x;
-            fraction *= power_code<GT>(x, code);
+            fraction *= power_code<GT>((GT)x, code);
         }
         const GT Gk = gb[k] * fraction;
         GkSum += (double)Gk;
@@ -1181,7 +1188,7 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
         level = gLevel;
     }
     const ChanType* types = stage_types(c, a.types, sTypes);
-    const GT dtg = (GT)a.dt;
+    const double dtg = a.dt;
     const int4* desc = A_<const int4>(arena, c.oDesc);
     int64_t* stepCounter = A_<int64_t>(arena, c.oSteps);
     const int64_t step0 = *stepCounter;
@@ -1715,8 +1722,8 @@ __global__ void __launch_bounds__(256) hsolve_channel_kernel(LaunchArgs a, const
         w.D = gwork;
         w.B = gwork + v.n;
     }
-    if (v.uniform) channel_row<GT, true>(v, types, i, (GT)a.dt, w, false, dst);
-    else channel_row<GT, false>(v, types, i, (GT)a.dt, w, false, dst);
+    if (v.uniform) channel_row<GT, true>(v, types, i, a.dt, w, false, dst);
+    else channel_row<GT, false>(v, types, i, a.dt, w, false, dst);
     row_inputs(a, c, i, w.D[dst], w.B[dst]);
 }
 
@@ -1729,7 +1736,7 @@ __global__ void __launch_bounds__(256) hsolve_channel_fast_kernel(LaunchArgs a, 
     const int i = tile.y + threadIdx.x;
     if (i >= v.n) return;
     double D, B;
-    channel_row_fast<GT, MS>(v, c, i, (GT)a.dt, D, B);
+    channel_row_fast<GT, MS>(v, c, i, a.dt, D, B);
     row_inputs(a, c, i, D, B);
     double *Dp, *Bp;
     row_dest(a.arena, c, i, Dp, Bp);
@@ -2062,9 +2069,9 @@ __global__ void __launch_bounds__(256) hsolve_reinit_kernel(LaunchArgs a) {
     __syncthreads();
     Work w{};
     if (v.uniform)
-        for (int i = threadIdx.x; i < v.n; i += blockDim.x) channel_row<GT, true>(v, types, i, (GT)a.dt, w, true, i);
+        for (int i = threadIdx.x; i < v.n; i += blockDim.x) channel_row<GT, true>(v, types, i, a.dt, w, true, i);
     else
-        for (int i = threadIdx.x; i < v.n; i += blockDim.x) channel_row<GT, false>(v, types, i, (GT)a.dt, w, true, i);
+        for (int i = threadIdx.x; i < v.n; i += blockDim.x) channel_row<GT, false>(v, types, i, a.dt, w, true, i);
 }
 
 // The objects sending (Gk, Ek) every tick (Leakage, HHChannel2D, SynChan)
