@@ -40,7 +40,7 @@ sys.path.insert(0, ROOT)
 import __graft_entry__  # noqa: E402
 
 pkg = __graft_entry__.import_package()
-from moose_core_amd import workloads  # noqa: E402
+from moose_core_amd import autotune, workloads  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
@@ -140,9 +140,10 @@ WORKLOADS = {
 
 
 def run_form(precision, packed, dt, steps, warmup, spl, graph, threads, mode="fused", conns=(), acc_steps=0,
-             plans=None):
-    """Times `steps` ticks after `warmup` untimed ones; then, untimed, a
-    fresh reinit and `acc_steps` ticks recording the soma Vm of every cell
+             plans=None, tune=True, tuned=None):
+    """Tunes the launch plan bin by bin (autotune.tune_bins, untimed) and
+    reinits; times `steps` ticks after `warmup` untimed ones; then, untimed,
+    a fresh reinit and `acc_steps` ticks recording the soma Vm of every cell
     and the spikes (the accuracy leg)."""
     eng = pkg.Engine(precision, steps_per_launch=spl, graph_chunk=graph, block_threads=threads,
                      kernel_mode=mode)
@@ -151,6 +152,14 @@ def run_form(precision, packed, dt, steps, warmup, spl, graph, threads, mode="fu
     for (a, g, b, k, j) in conns:
         eng.connect_spike(ids[a], g, ids[b], k, j)
     log(f"[bench] {precision}: {len(ids)} cells set up in {time.time() - t:.1f}s")
+    if tune and threads == 0:
+        eng.reinit(dt)
+        t = time.time()
+        res = autotune.tune_bins(eng, steps=max(10, spl), repeats=2)
+        log(f"[bench] {precision}: autotuned {len(res['bins'])} bins in {time.time() - t:.1f}s: "
+            f"{res['ms_per_step_default']:.4f} -> {res['ms_per_step_tuned']:.4f} ms/step")
+        if tuned is not None:
+            tuned[precision] = res
     for cid in ids:
         eng.set_probes(cid, [0], max(steps, acc_steps))
     eng.reinit(dt)
@@ -383,6 +392,7 @@ def main():
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--kernel-mode", default="auto", choices=["fused", "split", "auto"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the engine's own launch shapes")
     ap.add_argument("--forms", default="mixed,fp64")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (calibrates "
@@ -421,10 +431,11 @@ def main():
     ncomp = sum(c.n for c in cells)
     res = {}
     plans = {}
+    tuned = {}
     for form in args.forms.split(","):
         res[form] = run_form(form, packed, dt, args.steps, args.warmup, args.steps_per_launch,
                              args.graph_chunk, args.threads, args.kernel_mode, conns, args.accuracy_steps,
-                             plans)
+                             plans, not args.no_autotune, tuned)
     head = "mixed" if "mixed" in res else list(res)[0]
     wall, launches, total_ms, mean_ms, soma, spikes = res[head]
     value = ncomp * args.steps / wall
@@ -457,8 +468,16 @@ def main():
                    "steps_per_launch": args.steps_per_launch, "graph_chunk": args.graph_chunk,
                    "kernel_mode": args.kernel_mode,
                    "bins": [{k: b[k] for k in ("kernel", "cells", "compartments", "threads", "solve_threads",
-                                               "lds_bytes", "split", "fast_slots", "round_ms")} for b in bins]}
-                  | ({"connections": len(conns), "spikes_routed": int(len(spikes))} if conns else {}),
+                                               "lds_bytes", "split", "fast_slots", "round_ms")}
+                            | ({"tuned": tuned[head]["bins"][i]["chosen"]} if head in tuned else {})
+                            for i, b in enumerate(bins)]}
+                  | ({"connections": len(conns), "spikes_routed": int(len(spikes))} if conns else {})
+                  | ({"autotune": {"ms_per_step_default": tuned[head]["ms_per_step_default"],
+                                   "ms_per_step_tuned": tuned[head]["ms_per_step_tuned"],
+                                   "trials": len(tuned[head]["trials"]),
+                                   "method": "per-bin threads and channel pass, each bin's candidates timed on the "
+                                             "whole round with HIP events (tune_bins), then reinit"}}
+                     if head in tuned else {}),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": (f"{dom['kernel']} (the longest of the {len(bins)} concurrent size-bin "

@@ -476,6 +476,17 @@ typedef struct hsg_bin_info {
 } hsg_bin_info;
 int hsg_plan_info(hsg_engine* e, hsg_bin_info* out, int32_t cap, int32_t* n_out);
 
+/* Launch shape of bin `bin` of the current plan (its index in
+ * hsg_plan_info), for a caller that has measured the alternatives (the
+ * autotuner): threads per cell workgroup (64, 128 or 256; 0: the engine's
+ * choice by cell size) and the fast channel pass (1 on, 0 off, -1 the
+ * engine's choice; on only where the bin's cells allow it: uniform cells
+ * with integer powers).  The choice stays with the bin's class (kernel form,
+ * LDS class, pair half) across later plans until changed; bin = -1 clears
+ * every choice.  The next step re-plans.  No counterpart in the reference,
+ * which has one cell per solver and no launch shapes (hsolve/HSolve.cpp:221). */
+int hsg_set_bin_shape(hsg_engine* e, int32_t bin, int32_t threads, int32_t fast_pass);
+
 /* Blocks until all queued device work of the engine is done. */
 int hsg_sync(hsg_engine* e);
 

@@ -281,7 +281,17 @@ struct hsg_engine {
         int nBlkTiles = 0;
         int blockLds = 0;           // dynamic LDS of the block kernels
         int topLds = 0;             // dynamic LDS of the top kernel
+        int key = 0;                // the bin's class (second half of a pair: + 32)
+        int fastSel = -1;           // fast channel pass: -1 the engine's choice, 0 off, 1 on
     };
+    // Launch shapes set by the caller per bin class (hsg_set_bin_shape):
+    // threads per cell workgroup (0: the engine's choice) and the fast
+    // channel pass (-1: the engine's choice).
+    struct BinShape {
+        int threads = 0;
+        int fast = -1;
+    };
+    std::map<int, BinShape> shapes;
     // A launch plan: the bins of a set of cells, with every bin's cell order
     // and channel tiles in one device buffer (owned by the plan; `external`
     // buffers belong to someone else and are never freed here).
@@ -1182,6 +1192,7 @@ void plan_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg
             cur = hsg_engine::Bin();
         }
         curCls = k;
+        cur.key = k;
         cur.msg = (k & 7) < 3 || (k & 7) == 6;
         cur.hasExt = (k & 7) >= 6;
         cur.slow = (k & 8) != 0;
@@ -1201,6 +1212,7 @@ void plan_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg
         second.cells.clear();
         for (size_t i = 0; i < nb; ++i) (i % 2 ? second.cells : bins[b].cells).push_back(all[i]);
         bins[b].pairNext = true;
+        second.key = bins[b].key + 32;
         bins.insert(bins.begin() + b + 1, second);
         ++b;
     }
@@ -1221,9 +1233,16 @@ void plan_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg
         // workgroup (wider ones measured no faster, 512 slower)
         b.threads = e->opt.block_threads > 0 ? std::max(64, std::min(256, (e->opt.block_threads + 63) / 64 * 64))
                                              : default_threads(maxN);
+        // a shape the caller chose for this bin class (it measured it)
+        const auto sh = e->shapes.find(b.key);
+        if (sh != e->shapes.end()) {
+            if (sh->second.threads > 0) b.threads = sh->second.threads;
+            b.fastSel = sh->second.fast;
+        }
         // large cells' levels hold hundreds of units: their split solve
         // (not the message-passing one) sweeps a level with 512 threads
-        b.solveThreads = b.split && !b.msg && e->opt.block_threads <= 0 && maxN >= kWideSolveMin ? 512 : b.threads;
+        b.solveThreads = b.split && !b.msg && e->opt.block_threads <= 0 && (sh == e->shapes.end() || sh->second.threads <= 0) &&
+                                 maxN >= kWideSolveMin ? 512 : b.threads;
         // the block form when every cell of a split bin has it
         b.blocks = b.split && !b.msg;
         b.blockLds = 0;
@@ -1452,7 +1471,8 @@ int64_t routed_stretch(hsg_engine* e) {
 #define HSG_FAST_FUSED_MIN 6
 #endif
 int fused_fast(const hsg_engine::Bin& b) {
-    if (b.chanMS != 2 || b.slow || b.cells.empty()) return 0;
+    if (b.chanMS != 2 || b.slow || b.cells.empty() || b.fastSel == 0) return 0;
+    if (b.fastSel == 1) return 2;
     const double perThread = (double)b.sumN / (double)b.cells.size() / (double)b.threads;
     return perThread >= HSG_FAST_FUSED_MIN ? 2 : 0;
 }
@@ -1461,7 +1481,8 @@ int fused_fast(const hsg_engine::Bin& b) {
 // them), then the channel kernel over the bin's tiles.
 int split_wide(hsg_engine* e, const hsg_engine::Bin& bin, LaunchArgs& a, hipStream_t st) {
     if (bin.hasExt) HIPCHK(hsg_launch_external(&a, bin.dTiles, bin.nTiles, st));
-    if (bin.chanMS > 0) HIPCHK(hsg_launch_channel_fast(&a, bin.dTiles, bin.nTiles, e->fp32, bin.chanMS, st));
+    if (bin.chanMS > 0 && bin.fastSel != 0)
+        HIPCHK(hsg_launch_channel_fast(&a, bin.dTiles, bin.nTiles, e->fp32, bin.chanMS, st));
     else HIPCHK(hsg_launch_channel(&a, bin.dTiles, bin.nTiles, e->fp32, st));
     return 0;
 }
@@ -1496,6 +1517,17 @@ int launch_round(hsg_engine* e, std::vector<hsg_engine::Bin>& bins, int chunk, i
     // one stream and one set of events per bin: the plan never holds more
     // bins than the engine has streams
     if (bins.size() > (size_t)hsg_engine::kMaxBins) return fail(HSG_E_STATE, "launch plan has more bins than streams");
+    // every stream and event indexed below exists (created with the engine;
+    // a null handle would silently mean the null stream), and a pair's
+    // second half is a bin of this plan
+    if (!e->evFork) return fail(HSG_E_STATE, "engine streams not set up");
+    for (size_t b = 0; b < bins.size(); ++b) {
+        if (!e->binStream[b] || !e->evJoin[b] || !e->evPairA[b] || !e->evPairB[b] ||
+            (timed && (!e->evBin0[b] || !e->evBin1[b])))
+            return fail(HSG_E_STATE, "engine streams not set up");
+        if (bins[b].pairNext && bins[b].split && b + 1 >= bins.size())
+            return fail(HSG_E_STATE, "pipelined pair without its second half");
+    }
     HIPCHK(hipEventRecord(e->evFork, e->stream));
     for (size_t b = 0; b < bins.size(); ++b) {
         hipStream_t st = e->binStream[b];
@@ -2813,7 +2845,7 @@ int hsg_plan_info(hsg_engine* e, hsg_bin_info* out, int32_t cap, int32_t* n_out)
         o.lds_bytes = x.lds;
         o.split = x.split;
         o.msg = x.msg;
-        o.fast_slots = x.split ? x.chanMS : fused_fast(x);
+        o.fast_slots = x.split ? (x.fastSel == 0 ? 0 : x.chanMS) : fused_fast(x);
         o.slow_groups = x.slow;
         o.ext = x.hasExt;
         o.round_steps = x.roundSteps;
@@ -2829,6 +2861,32 @@ int hsg_plan_info(hsg_engine* e, hsg_bin_info* out, int32_t cap, int32_t* n_out)
             snprintf(o.kernel, sizeof o.kernel, "%s<%s>", x.slow ? "hsolve_step_slow_kernel"
                      : fused_fast(x) ? "hsolve_step_fast_kernel" : "hsolve_step_kernel", prec);
     }
+    return 0;
+}
+
+int hsg_set_bin_shape(hsg_engine* e, int32_t bin, int32_t threads, int32_t fast_pass) {
+    if (!e) return fail(HSG_E_ARG, "null engine");
+    if (bin == -1) {
+        e->shapes.clear();
+        e->planDirty = true;
+        return 0;
+    }
+    hipSetDevice(e->device);
+    int rc = upload_tables(e);   // the plan the index refers to
+    if (rc) return rc;
+    if (bin < 0 || bin >= (int32_t)e->plan.bins.size()) return fail(HSG_E_ARG, "no such bin in the launch plan");
+    if (threads != 0 && threads != 64 && threads != 128 && threads != 256)
+        return fail(HSG_E_ARG, "threads per cell workgroup: 0, 64, 128 or 256");
+    if (fast_pass < -1 || fast_pass > 1) return fail(HSG_E_ARG, "fast_pass: -1, 0 or 1");
+    const hsg_engine::Bin& b = e->plan.bins[bin];
+    if (threads != 0 && b.split && !b.msg && b.blocks)
+        return fail(HSG_E_ARG, "the block-form kernels have a fixed workgroup size");
+    const bool fastOk = b.split ? b.chanMS > 0 : (b.chanMS == 2 && !b.slow);
+    if (fast_pass == 1 && !fastOk) return fail(HSG_E_ARG, "the fast channel pass does not apply to this bin's cells");
+    hsg_engine::BinShape& sh = e->shapes[b.key];
+    sh.threads = threads;
+    sh.fast = fast_pass;
+    e->planDirty = true;
     return 0;
 }
 

@@ -22,7 +22,7 @@ EXPORTS = [
     "hsg_reinit_cell", "hsg_step", "hsg_step_cells", "hsg_step_cells_noalloc", "hsg_get_field", "hsg_set_field",
     "hsg_add_inject", "hsg_add_gkek", "hsg_add_conc", "hsg_add_ica", "hsg_set_pulsegen", "hsg_add_spike", "hsg_connect_spike", "hsg_read_spikes",
     "hsg_set_probes", "hsg_set_probes_every", "hsg_set_probe_fields", "hsg_read_probes", "hsg_cell_info_get", "hsg_hines_order",
-    "hsg_sync", "hsg_last_step_timing", "hsg_plan_info", "hsg_last_error",
+    "hsg_sync", "hsg_last_step_timing", "hsg_plan_info", "hsg_set_bin_shape", "hsg_last_error",
 ]
 
 
@@ -74,6 +74,7 @@ def load_library(path=None):
         "hsg_sync": ([vp], C.c_int),
         "hsg_last_step_timing": ([vp, P(i32), P(dbl), P(dbl)], C.c_int),
         "hsg_plan_info": ([vp, P(_abi.BinInfo), i32, P(i32)], C.c_int),
+        "hsg_set_bin_shape": ([vp, i32, i32, i32], C.c_int),
         "hsg_last_error": ([], C.c_char_p),
     }
     for name, (args, res) in sig.items():
@@ -291,6 +292,12 @@ class Engine:
             d["kernel"] = b.kernel.decode()
             out.append(d)
         return out
+
+    def set_bin_shape(self, bin, threads=0, fast_pass=-1):
+        """Launch shape of plan bin `bin` (hsg_set_bin_shape): threads per
+        cell workgroup (0: the engine's choice) and the fast channel pass
+        (-1: the engine's choice).  bin = -1 clears every choice."""
+        self._check(self.lib.hsg_set_bin_shape(self.h, int(bin), int(threads), int(fast_pass)))
 
     def last_timing(self):
         n = C.c_int32()

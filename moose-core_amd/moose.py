@@ -28,6 +28,7 @@ gate order).  Only models run by an HSolve advance: `start` steps every
 HSolve's cells on the GPU; there is no per-object CPU integration here.
 """
 import math
+import sys
 
 import numpy as np
 
@@ -970,18 +971,29 @@ class HSolve(Neutral):
 
     @target.setter
     def target(self, path):
-        if not self.dt > 0:
-            raise ValueError("HSolve::setPath(): Must set 'dt' first.")
-        seed = _deep_search_compartment(element(path))
+        # HSolve::setPath (hsolve/HSolve.cpp:315-334): without dt, or with no
+        # compartment at or below the path, it prints and returns, leaving
+        # the solver as it was (on no compartment, the seed cleared)
+        if self.dt == 0.0:
+            print("Error: HSolve::setPath(): Must set 'dt' first.", file=sys.stderr)
+            return
+        try:
+            seed = _deep_search_compartment(element(path))
+        except ValueError:   # no element there: no compartment found
+            seed = None
         if seed is None:
-            raise ValueError(f"HSolve::setPath(): No compartments found at or below '{path}'.")
+            object.__setattr__(self, "seed", None)
+            print(f"Warning: HSolve::setPath(): No compartments found at or below '{path}'.", file=sys.stderr)
+            return
         object.__setattr__(self, "_target", path)
         object.__setattr__(self, "seed", seed)
         self._setup(seed)
 
     def _set_local(self, name, value):
         if name == "dt" and value < 0:
-            raise ValueError("HSolve: 'dt' must be positive.")
+            # HSolve::setDt (hsolve/HSolve.cpp:409-418): prints, keeps the old dt
+            print("Error: HSolve: 'dt' must be positive.", file=sys.stderr)
+            return
         if name == "target":
             HSolve.target.fset(self, value)
             return
