@@ -234,6 +234,7 @@ struct hsg_engine {
     std::map<uint64_t, int> tab2dKey;
     Table2DDev* dTabs2d = nullptr;
     int dTabs2dCap = 0;
+    std::map<std::pair<int, int>, int> pair2dKey;  // (A, B) table ids -> pair record id
     // device-side tables of the launch
     CellDev* dCells = nullptr;
     int dCellsCap = 0;
@@ -449,6 +450,40 @@ int table_set_id(hsg_engine* e, const HostTableSet& t, int* id) {
     *id = (int)e->tsets.size();
     e->tsets.push_back(ts);
     e->tsetKey[t.key] = *id;
+    e->tablesDirty = true;
+    return 0;
+}
+
+// Engine-wide id of the (A, B) pair record of two 2-D tables on one grid:
+// the two tables interleaved point by point, in f64 and in f32, so that a
+// lookup of both reads two neighbouring pairs per grid row.  -1 in *id when
+// the grids differ.
+int pair2d_id(hsg_engine* e, int idA, int idB, const HostTable2D& a, const HostTable2D& b, int* id) {
+    *id = -1;
+    if (a.nx != b.nx || a.ny != b.ny || a.xmin != b.xmin || a.xmax != b.xmax || a.ymin != b.ymin ||
+        a.ymax != b.ymax || a.invDx != b.invDx || a.invDy != b.invDy || a.table.size() != b.table.size())
+        return 0;
+    const auto key = std::make_pair(idA, idB);
+    auto it = e->pair2dKey.find(key);
+    if (it != e->pair2dKey.end()) { *id = it->second; return 0; }
+    Table2DDev d = e->tabs2d[idA];
+    const size_t n = a.table.size();
+    std::vector<double> p64(2 * std::max<size_t>(n, 1), 0.0);
+    std::vector<float> p32(2 * std::max<size_t>(n, 1), 0.0f);
+    for (size_t k = 0; k < n; ++k) {
+        p64[2 * k] = a.table[k];
+        p64[2 * k + 1] = b.table[k];
+        p32[2 * k] = (float)a.table[k];
+        p32[2 * k + 1] = (float)b.table[k];
+    }
+    int rc = arena_alloc(e, (int64_t)p64.size() * 8, &d.off);
+    if (rc) return rc;
+    if ((rc = arena_alloc(e, (int64_t)p32.size() * 4, &d.off32))) return rc;
+    if ((rc = upload(e, d.off, p64)) || (rc = upload(e, d.off32, p32))) return rc;
+    HIPCHK(hipStreamSynchronize(e->stream));
+    *id = (int)e->tabs2d.size();
+    e->tabs2d.push_back(d);
+    e->pair2dKey[key] = *id;
     e->tablesDirty = true;
     return 0;
 }
@@ -753,6 +788,11 @@ int place_cell(hsg_engine* e, HostCell& hc) {
                 y.pw[a] = h.pw[a];
                 y.tabA[a] = h.tabA[a] >= 0 ? tid[h.tabA[a]] : -1;
                 y.tabB[a] = h.tabB[a] >= 0 ? tid[h.tabB[a]] : -1;
+                y.pair[a] = -1;
+                if (y.tabA[a] >= 0 && y.tabB[a] >= 0 &&
+                    (rc = pair2d_id(e, y.tabA[a], y.tabB[a], s.tables2d[h.tabA[a]], s.tables2d[h.tabB[a]],
+                                    &y.pair[a])))
+                    return rc;
             }
             for (int q = 0; q < 6; ++q) y.dep[q] = h.dep[q];
             y.instant = h.instant;
@@ -1264,6 +1304,7 @@ LaunchArgs make_args(hsg_engine* e, const int32_t* order, int nsteps, int first)
     a.types = e->dTypes;
     a.tsets = e->dTsets;
     a.tabs2d = e->dTabs2d;
+    a.fp32 = e->fp32 ? 1 : 0;
     a.spikes = e->dSpikes;
     a.spikeCount = e->dSpikeCount;
     a.spikeCap = kSpikeCap;

@@ -48,7 +48,8 @@ struct TableSetDev {
 struct Table2DDev {
     double  xmin, xmax, ymin, ymax, invDx, invDy;
     int32_t nx, ny;             // xdivs + 1, ydivs + 1
-    int64_t off;                // f64 [nx * ny], x-major
+    int64_t off;                // f64 [nx * ny], x-major; a pair record: (A, B) as f64 pairs
+    int64_t off32;              // a pair record: (A, B) as f32 pairs; a single table: 0
 };
 
 // What HHChannel2D objects of one kind share (biophysics/HHChannel2D.cpp):
@@ -59,7 +60,8 @@ struct Chan2DType {
     int32_t tabA[3], tabB[3];   // engine-wide Table2DDev ids (gate with power > 0)
     int32_t dep[6];             // table axes of gate g: dep[2g], dep[2g+1] (0 Vm, 1 conc1, 2 conc2, -1)
     int32_t instant;
-    int32_t pad;
+    int32_t pair[3];            // Table2DDev id of gate g's (A, B) pair record where the two
+                                // tables share one grid, else -1
 };
 
 // One HHChannel2D run beside the solver: what varies per object.
@@ -334,7 +336,7 @@ struct LaunchArgs {
     double         dt;          // clock tick dt
     int32_t        deferD;      // split mode: the solve kernels stop at VMid; the
                                 // update and outputs kernels finish the tick
-    int32_t        pad;
+    int32_t        fp32;        // mixed form: FP32 channel tables and currents
 };
 
 }  // namespace hsg

@@ -36,6 +36,7 @@
 #include <hip/hip_runtime.h>
 
 #include <mutex>
+#include <type_traits>
 
 #include "hsolve_internal.h"
 
@@ -304,6 +305,50 @@ __device__ __forceinline__ double interp2d(const Table2DDev& t, const double* __
     return z00 * (1 - xF - yF + xFyF) + z10 * (xF - xFyF) + z01 * (yF - xFyF) + z11 * xFyF;
 }
 
+// Both tables of a gate at once from its pair record (the A and B tables
+// share one grid, so Interpol2D's cell and weights are shared): the same
+// index arithmetic as interp2d, corner pairs and the weighted sums in GT
+// (FP64: the same operations as two interp2d calls; mixed form: FP32 values
+// and weights).
+template <typename GT>
+__device__ __forceinline__ void interp2d_pair(const Table2DDev& t, const char* arena, double x, double y, double& A,
+                                              double& B) {
+    using V2 = typename std::conditional<std::is_same<GT, float>::value, float2, double2>::type;
+    const V2* __restrict__ tab = A_<const V2>(const_cast<char*>(arena), std::is_same<GT, float>::value ? t.off32 : t.off);
+    if (x < t.xmin) x = t.xmin;
+    if (x > t.xmax) x = t.xmax;
+    if (y < t.ymin) y = t.ymin;
+    if (y > t.ymax) y = t.ymax;
+    const unsigned long nx = (unsigned long)t.nx, ny = (unsigned long)t.ny;
+    const double xv = (x - t.xmin) * t.invDx;
+    unsigned long xi = (unsigned long)xv;
+    if (xi >= nx) xi = nx - 1;
+    const bool endX = xi == nx - 1;
+    const double xFd = xv - xi;
+    const double yv = (y - t.ymin) * t.invDy;
+    unsigned long yi = (unsigned long)yv;
+    if (yi >= ny) yi = ny - 1;
+    const bool endY = yi == ny - 1;
+    const double yFd = yv - yi;
+    const GT xF = (GT)xFd, yF = (GT)yFd;
+    const GT xFyF = xF * yF;
+    const V2 z00 = tab[xi * ny + yi];
+    V2 z01, z10, z11;
+    z01.x = z01.y = z10.x = z10.y = z11.x = z11.y = (GT)0;
+    if (!endX) {
+        z10 = tab[(xi + 1) * ny + yi];
+        if (!endY) {
+            z11 = tab[(xi + 1) * ny + yi + 1];
+            z01 = tab[xi * ny + yi + 1];
+        }
+    } else if (!endY) {
+        z01 = tab[xi * ny + yi + 1];
+    }
+    const GT w00 = 1 - xF - yF + xFyF, w10 = xF - xFyF, w01 = yF - xFyF;
+    A = (double)(z00.x * w00 + z10.x * w10 + z01.x * w01 + z11.x * xFyF);
+    B = (double)(z00.y * w00 + z10.y * w10 + z01.y * w01 + z11.y * xFyF);
+}
+
 // HHChannelBase::integrate (biophysics/HHChannelBase.cpp:268-274)
 __device__ __forceinline__ double integrate_ee(double state, double dt, double A, double B) {
     if (B > 1.0e-10) {
@@ -333,8 +378,10 @@ __device__ __forceinline__ double dep2d(int dep, bool sent, double Vm, const Cha
 // their handlers sent on tick 1), each kind in description order.  The sum
 // is kept in o2dExt for the row and for Im.  Run by its own kernel before
 // the tick's solver kernels, so the step kernels carry none of its
-// registers.
-template <bool WITH2D>
+// registers.  F32 (mixed form): the HHChannel2D tables are read as FP32
+// pairs and Gk is formed in FP32; the gate state and its integration stay
+// FP64.
+template <bool WITH2D, bool F32 = false>
 __device__ __forceinline__ double2 external_sum(char* arena, const Table2DDev* __restrict__ tabs2d, double dt,
                                                 bool first, const CellDev& c, int i, double Vm,
                                                 const double* __restrict__ poolCa, int64_t stepNo) {
@@ -367,23 +414,29 @@ __device__ __forceinline__ double2 external_sum(char* arena, const Table2DDev* _
         for (int k = start[i]; k < start[i + 1]; ++k) {
             const Chan2DDev& ch = chans[k];
             const Chan2DType& ty = kinds[ch.type];
-            double g = 0.0;
-            g += ch.Gbar;
+            using CT = typename std::conditional<F32, float, double>::type;
+            CT g = 0.0;
+            g += (CT)ch.Gbar;
 #pragma unroll 1
             for (int ax = 0; ax < 3; ++ax) {
                 if (!(ty.pw[ax] > 0.0)) continue;
                 const double u = dep2d(ty.dep[2 * ax], sent, Vm, ch, poolCa);
                 const double w = dep2d(ty.dep[2 * ax + 1], sent, Vm, ch, poolCa);
-                const Table2DDev& ta = tabs2d[ty.tabA[ax]];
-                const Table2DDev& tb = tabs2d[ty.tabB[ax]];
-                const double A = interp2d(ta, A_<const double>(arena, ta.off), u, w);
-                const double B = interp2d(tb, A_<const double>(arena, tb.off), u, w);
+                double A, B;
+                if (ty.pair[ax] >= 0) {
+                    interp2d_pair<CT>(tabs2d[ty.pair[ax]], arena, u, w, A, B);
+                } else {
+                    const Table2DDev& ta = tabs2d[ty.tabA[ax]];
+                    const Table2DDev& tb = tabs2d[ty.tabB[ax]];
+                    A = interp2d(ta, A_<const double>(arena, ta.off), u, w);
+                    B = interp2d(tb, A_<const double>(arena, tb.off), u, w);
+                }
                 double x = st[3 * k + ax];
                 x = (ty.instant >> ax) & 1 ? A / B : integrate_ee(x, dt, A, B);
                 st[3 * k + ax] = x;
-                g *= take_power<double>(x, ty.pw[ax]);
+                g *= take_power<CT>((CT)x, ty.pw[ax]);
             }
-            const double Gk = g * ch.mod;
+            const double Gk = (double)(g * (CT)ch.mod);
             gk[k] = Gk;
             G += Gk;
             GE += Gk * ch.Ek;
@@ -422,10 +475,11 @@ __device__ __forceinline__ double2 external_sum(char* arena, const Table2DDev* _
 
 // The external kernel's form (every kind of object).
 __device__ __forceinline__ void external_objects(char* arena, const Table2DDev* __restrict__ tabs2d, double dt,
-                                                 bool first, const CellDev& c, int i, double Vm,
+                                                 bool first, bool f32, const CellDev& c, int i, double Vm,
                                                  const double* __restrict__ poolCa, int64_t stepNo) {
     if (!A_<const int32_t>(arena, c.oXFlag)[i]) return;
-    external_sum<true>(arena, tabs2d, dt, first, c, i, Vm, poolCa, stepNo);
+    if (f32) external_sum<true, true>(arena, tabs2d, dt, first, c, i, Vm, poolCa, stepNo);
+    else external_sum<true, false>(arena, tabs2d, dt, first, c, i, Vm, poolCa, stepNo);
 }
 
 // The step kernels' form, for cells without HHChannel2D objects (Leakage
@@ -1371,8 +1425,12 @@ __global__ void __launch_bounds__(256) hsolve_step_slow_kernel(LaunchArgs a) {
 // through channel_row_fast (more registers, fewer workgroups per CU, the
 // channel loads of a compartment in two round trips instead of one per
 // gate).
+// Three waves per SIMD (at most 168 VGPRs, a small spill): with the
+// 52 KB of LDS of the cells it serves (n ~ 1300-2000), three workgroups fit
+// on a CU; measured 3 % faster on the ensemble than two waves without
+// spills, and faster than four (128 VGPRs, heavy spills).
 template <typename GT>
-__global__ void __launch_bounds__(256) hsolve_step_fast_kernel(LaunchArgs a) {
+__global__ void __launch_bounds__(256, 3) hsolve_step_fast_kernel(LaunchArgs a) {
     extern __shared__ double lds[];
     __shared__ ChanType sTypes[kMaxSlots];
     const CellDev& c = a.cells[a.order[blockIdx.x]];
@@ -2088,7 +2146,7 @@ __global__ void __launch_bounds__(256) hsolve_external_kernel(LaunchArgs a, cons
     // this tick ends at step (steps done + 1); the tick's solver kernel
     // advances the counter after it
     const int64_t stepNo = *A_<const int64_t>(a.arena, c.oSteps) + 1;
-    external_objects(a.arena, a.tabs2d, a.dt, a.firstOfCall != 0, c, i, V[i], poolCa, stepNo);
+    external_objects(a.arena, a.tabs2d, a.dt, a.firstOfCall != 0, a.fp32 != 0, c, i, V[i], poolCa, stepNo);
 }
 
 // HHChannel2D::vReinit of the cells in `order`, launched before the
