@@ -184,16 +184,29 @@ typedef struct hsg_chan2d_desc {
 
 /* A PulseGen whose outputOut feeds a compartment's injectMsg
  * (device/PulseGen.cpp:359-433; HSolve::addInject,
- * hsolve/HSolveInterface.cpp:213-219): free-running pulses (trigMode 0) of
- * n_pulses (delay, width, level) entries over baseLevel, evaluated on the
- * device before each tick at the tick's end time (PulseGen runs on clock tick
- * 0, before HSolve's tick 6), and sent once at reinit (baseLevel). */
+ * hsolve/HSolveInterface.cpp:213-219): n_pulses (delay, width, level)
+ * entries over baseLevel, evaluated on the device before each tick at the
+ * tick's end time (PulseGen runs on clock tick 0, before HSolve's tick 6), in
+ * description order, and sent once at reinit (baseLevel).
+ *
+ * trigMode 0 runs free; 1 (EXT_TRIG) and 2 (EXT_GATE) follow the input: the
+ * value the host writes (HSG_F_PULSE_INPUT, PulseGen::input; reinit sets it
+ * to 0), or, with input_src >= 0, the output of that PulseGen of the cell (an
+ * "output" -> "input" message), delivered as the reference's messages are:
+ * the same tick's output when the source comes earlier in the list, the
+ * previous tick's (at the first tick, its reinit baseLevel) when later.  A
+ * source must run free with pulses that cover its period (else
+ * HSG_E_UNSUPPORTED); compartment may be -1 for a PulseGen that only drives
+ * another's input. */
 #define HSG_MAX_PULSES 4
 typedef struct hsg_pulsegen_desc {
-    int32_t compartment;
+    int32_t compartment;     /* target of injectMsg, or -1 (input source only) */
     int32_t n_pulses;        /* 1..HSG_MAX_PULSES (PulseGen::count)          */
     double  baseLevel;
     double  delay[HSG_MAX_PULSES], width[HSG_MAX_PULSES], level[HSG_MAX_PULSES];
+    int32_t trigMode;        /* 0 FREE_RUN, 1 EXT_TRIG, 2 EXT_GATE            */
+    int32_t input_src;       /* trigMode 1-2: PulseGen of this cell driving
+                                the input, or -1; ignored in trigMode 0      */
 } hsg_pulsegen_desc;
 
 /* One cell.  axial[2*k], axial[2*k+1] = (parent, child) of the k-th
@@ -354,6 +367,9 @@ int hsg_step_cells_noalloc(hsg_engine* e, const int32_t* cells, const int64_t* n
 /* VClamp fields (index = vclamps entry; biophysics/VClamp.cpp:212-285) */
 #define HSG_F_VCLAMP_COMMAND 100   /* commandIn: the command voltage input */
 #define HSG_F_VCLAMP_CURRENT 101   /* read-only: the current last sent */
+/* PulseGen fields (index = pulsegens entry; device/PulseGen.cpp:196-204) */
+#define HSG_F_PULSE_OUTPUT 110     /* read-only: output_ (the level last set) */
+#define HSG_F_PULSE_INPUT  111     /* input_ (a PulseGen without input_src)  */
 /* Leakage fields (index = leaks entry): the per-tick send; Gk and the
  * fold into EmByRm stay as set up */
 #define HSG_F_LEAK_GBAR 95

@@ -36,6 +36,7 @@ FIELDS = {
     "XSyn": 86, "YSyn": 87, "weight": 90, "delay": 91,
     "GbarLeak": 95, "EkLeak": 96, "modulationLeak": 97,
     "command": 100, "current": 101,
+    "pulseOutput": 110, "pulseInput": 111,
 }
 COMPARTMENT_FIELDS = {k for k, v in FIELDS.items() if v < 20}
 CHANNEL_FIELDS = {k for k, v in FIELDS.items() if 20 <= v < 40}
@@ -44,7 +45,8 @@ CHAN2D_FIELDS = {k for k, v in FIELDS.items() if 60 <= v < 80}
 SYNCHAN_FIELDS = {k for k, v in FIELDS.items() if 80 <= v < 90}
 SYNAPSE_FIELDS = {k for k, v in FIELDS.items() if 90 <= v < 95}
 LEAK_FIELDS = {k for k, v in FIELDS.items() if 95 <= v < 100}
-VCLAMP_FIELDS = {k for k, v in FIELDS.items() if v >= 100}
+VCLAMP_FIELDS = {k for k, v in FIELDS.items() if 100 <= v < 110}
+PULSE_FIELDS = {k for k, v in FIELDS.items() if v >= 110}
 
 
 class GateDesc(C.Structure):
@@ -128,10 +130,10 @@ assert SYNCHAN_DT.itemsize == 48 and SYNAPSE_DT.itemsize == 16 and VCLAMP_DT.ite
 MAX_PULSES = 4
 PULSEGEN_DT = np.dtype([("compartment", "i4"), ("n_pulses", "i4"), ("baseLevel", "f8"),
                         ("delay", "f8", (MAX_PULSES,)), ("width", "f8", (MAX_PULSES,)),
-                        ("level", "f8", (MAX_PULSES,))], align=True)
+                        ("level", "f8", (MAX_PULSES,)), ("trigMode", "i4"), ("input_src", "i4")], align=True)
 
 assert CHAN2D_DT.itemsize == 136
-assert PULSEGEN_DT.itemsize == 112
+assert PULSEGEN_DT.itemsize == 120
 
 
 def _ptr(a):

@@ -822,8 +822,12 @@ int place_cell(hsg_engine* e, HostCell& hc) {
             q.compt = h.compt;
             q.entry = h.entry;
             q.n = h.n;
+            q.trigMode = h.trigMode;
+            q.src = h.src;
+            q.srcBefore = h.src >= 0 && h.src < (int)k;
             q.base = h.base;
             q.out = h.base;
+            q.trigTime = -1.0;
             for (int j = 0; j < h.n; ++j) {
                 q.delay[j] = h.delay[j];
                 q.width[j] = h.width[j];
@@ -2376,6 +2380,20 @@ int hsg_get_field(hsg_engine* e, int32_t cell, int32_t field, const int32_t* idx
             const int64_t rec = hc->dev.oVClamp + (int64_t)sizeof(VClampDev) * i;
             const int64_t off = field == HSG_F_VCLAMP_COMMAND ? offsetof(VClampDev, cmdIn) : offsetof(VClampDev, current);
             if ((rc = download(e, rec + off, 1, &out[q]))) return rc;
+        } else if (field == HSG_F_PULSE_OUTPUT || field == HSG_F_PULSE_INPUT) {
+            if (i < 0 || i >= (int)s.pulses.size()) return fail(HSG_E_ARG, "unknown PulseGen index");
+            if (field == HSG_F_PULSE_OUTPUT && s.pulses[i].entry < 0) {
+                // a free-running input source the device never steps: its
+                // output at the last tick (its baseLevel before the first)
+                int64_t steps = 0;
+                if ((rc = download(e, hc->dev.oSteps, 1, &steps))) return rc;
+                out[q] = steps == 0 ? s.pulses[i].base
+                                    : pulse_free_output(s.pulses[i], e->dt * (double)(unsigned long long)steps);
+                continue;
+            }
+            const int64_t rec = hc->dev.oPulse + (int64_t)sizeof(PulseDev) * i;
+            const int64_t off = field == HSG_F_PULSE_OUTPUT ? offsetof(PulseDev, out) : offsetof(PulseDev, input);
+            if ((rc = download(e, rec + off, 1, &out[q]))) return rc;
         } else {
             return fail(HSG_E_ARG, "unknown field");
         }
@@ -2578,6 +2596,13 @@ int hsg_set_field(hsg_engine* e, int32_t cell, int32_t field, const int32_t* idx
             if (i < 0 || i >= (int)s.vclamps.size()) return fail(HSG_E_ARG, "unknown VClamp index");
             const int64_t rec = hc->dev.oVClamp + (int64_t)sizeof(VClampDev) * i;
             if ((rc = put1<double>(e, rec + offsetof(VClampDev, cmdIn), 0, x))) return rc;
+        } else if (field == HSG_F_PULSE_INPUT) {
+            // PulseGen::input (device/PulseGen.cpp:355-358), where no
+            // PulseGen's output drives it
+            if (i < 0 || i >= (int)s.pulses.size()) return fail(HSG_E_ARG, "unknown PulseGen index");
+            if (s.pulses[i].src >= 0) return fail(HSG_E_ARG, "this PulseGen's input comes from another PulseGen");
+            const int64_t rec = hc->dev.oPulse + (int64_t)sizeof(PulseDev) * i;
+            if ((rc = put1<double>(e, rec + offsetof(PulseDev, input), 0, x))) return rc;
         } else {
             return fail(HSG_E_ARG, "unknown field");
         }
@@ -2609,9 +2634,30 @@ int hsg_set_pulsegen(hsg_engine* e, int32_t cell, int32_t pulsegen, const hsg_pu
     if (desc->n_pulses < 1 || desc->n_pulses > HSG_MAX_PULSES)
         return fail(HSG_E_ARG, "PulseGen: 1 to HSG_MAX_PULSES pulses");
     HostPulse& p = s.pulses[pulsegen];
-    if (desc->compartment < 0 || desc->compartment >= (int)s.descToHines.size() ||
-        s.descToHines[desc->compartment] != p.compt)
+    const int compt = desc->compartment >= 0 && desc->compartment < (int)s.descToHines.size()
+                          ? s.descToHines[desc->compartment] : -2;
+    if (desc->compartment < -1 || (desc->compartment == -1 ? p.compt != -1 : compt != p.compt))
         return fail(HSG_E_ARG, "a PulseGen keeps the compartment it was added with");
+    if (desc->trigMode < 0 || desc->trigMode > 2) return fail(HSG_E_ARG, "PulseGen: trigMode 0, 1 or 2");
+    const int src = desc->trigMode != 0 ? desc->input_src : -1;
+    if (src < -1 || src >= (int)s.pulses.size() || src == pulsegen)
+        return fail(HSG_E_ARG, "PulseGen: input_src is -1 or another PulseGen of the cell");
+    // a source must stay one (this one, if others read it, and its own)
+    HostPulse next = p;
+    next.n = desc->n_pulses;
+    next.trigMode = desc->trigMode;
+    next.base = desc->baseLevel;
+    for (int j = 0; j < HSG_MAX_PULSES; ++j) {
+        next.delay[j] = j < next.n ? desc->delay[j] : 0.0;
+        next.width[j] = j < next.n ? desc->width[j] : 0.0;
+    }
+    for (size_t k = 0; k < s.pulses.size(); ++k)
+        if ((int)k != pulsegen && s.pulses[k].src == pulsegen && !pulse_source_ok(next))
+            return fail(HSG_E_UNSUPPORTED, "PulseGen: an input source must run free with pulses that cover its period");
+    if (src >= 0 && !pulse_source_ok(s.pulses[src]))
+        return fail(HSG_E_UNSUPPORTED, "PulseGen: an input source must run free with pulses that cover its period");
+    p.trigMode = desc->trigMode;
+    p.src = src;
     p.n = desc->n_pulses;
     p.base = desc->baseLevel;
     for (int j = 0; j < HSG_MAX_PULSES; ++j) {
@@ -2620,12 +2666,16 @@ int hsg_set_pulsegen(hsg_engine* e, int32_t cell, int32_t pulsegen, const hsg_pu
         p.level[j] = j < p.n ? desc->level[j] : 0.0;
     }
     hipSetDevice(e->device);
-    // everything but the output level, which the device holds
+    // everything but the state the device holds (output, input, trigger
+    // history)
     PulseDev q{};
     const int64_t off = hc->dev.oPulse + (int64_t)sizeof(PulseDev) * pulsegen;
     HIPCHK(hipMemcpyAsync(&q, e->arena.base + off, sizeof q, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     q.n = p.n;
+    q.trigMode = p.trigMode;
+    q.src = p.src;
+    q.srcBefore = p.src >= 0 && p.src < pulsegen;
     q.base = p.base;
     for (int j = 0; j < HSG_MAX_PULSES; ++j) {
         q.delay[j] = p.delay[j];

@@ -75,12 +75,15 @@ struct Chan2DDev {
 // A free-running PulseGen feeding a compartment's injection
 // (device/PulseGen.cpp:359-433).
 struct PulseDev {
-    int32_t compt;              // Hines index
-    int32_t entry;              // its injection entry (oInjIdx)
+    int32_t compt;              // Hines index, -1
+    int32_t entry;              // its injection entry (oInjIdx), -1
     int32_t n;                  // pulses (1..HSG_MAX_PULSES)
-    int32_t pad;
+    int32_t trigMode;           // 0 free running, 1 external trigger, 2 external gate
     double  base, out;          // baseLevel_, output_ (state)
     double  delay[HSG_MAX_PULSES], width[HSG_MAX_PULSES], level[HSG_MAX_PULSES];
+    int32_t src;                // trigMode 1-2: the free-running PulseGen driving input_, -1
+    int32_t srcBefore;          // 1: src comes first in tick order (this tick's output), 0: after
+    double  input, prevInput, trigTime;   // input_ (host-set where src < 0), prevInput_, trigTime_
 };
 
 // A VClamp (biophysics/VClamp.cpp:287-347): constants set at reinit, then

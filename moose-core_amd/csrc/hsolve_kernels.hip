@@ -938,30 +938,64 @@ __device__ __forceinline__ const ChanType* stage_types(const CellDev& c, const C
     return sTypes;
 }
 
-// PulseGen::process in FREE_RUN mode (device/PulseGen.cpp:359-424) at time
-// t: updates the output state; false when the reference sends nothing
-// (phase past the period).
-__device__ __noinline__ bool pulse_tick(PulseDev& p, double t) {
+// PulseGen's period (device/PulseGen.cpp:362-369).
+__device__ __forceinline__ double pulse_period(const PulseDev& p) {
     double period = p.width[0] + p.delay[0];
     for (int ii = 1; ii < p.n && (p.width[ii] > 0.0 || p.delay[ii] > 0.0); ++ii) {
         const double incr = p.delay[ii] + p.width[ii] - p.width[ii - 1];
         if (incr > 0) period += incr;
     }
-    double phase = fmod(t, period);
+    return period;
+}
+
+// The level for a phase inside the period (device/PulseGen.cpp:408-421);
+// `out` where no pulse or gap holds the phase (the reference keeps output_).
+__device__ __forceinline__ double pulse_level(const PulseDev& p, double phase, double out) {
+    for (int ii = 0; ii < p.n; ++ii) {
+        if (phase < p.delay[ii]) return p.base;
+        if (phase < p.delay[ii] + p.width[ii]) return p.level[ii];
+        phase -= p.delay[ii];
+    }
+    return out;
+}
+
+// The output at time t of a free-running PulseGen that drives another's
+// input: a function of t (the setup admits only sources whose pulses and
+// gaps cover the period, so the level is always set).
+__device__ __forceinline__ double pulse_free_output(const PulseDev& s, double t) {
+    return pulse_level(s, fmod(t, pulse_period(s)), s.base);
+}
+
+// PulseGen::process (device/PulseGen.cpp:359-424) at time t with input
+// `input` (trigger modes): updates the output and trigger state; false
+// when the reference sends nothing (phase past the period).
+__device__ __noinline__ bool pulse_tick(PulseDev& p, double t, double input) {
+    const double period = pulse_period(p);
+    double phase;
+    if (p.trigMode == 0) {
+        phase = fmod(t, period);
+    } else if (p.trigMode == 1) {          // EXT_TRIG
+        if (input == 0) {
+            phase = p.trigTime < 0 ? period : t - p.trigTime;
+        } else {
+            if (p.prevInput == 0) p.trigTime = t;
+            phase = t - p.trigTime;
+        }
+        p.prevInput = input;
+    } else {                               // EXT_GATE
+        if (input == 0) {
+            phase = period;
+        } else {
+            if (p.prevInput == 0) p.trigTime = t;
+            phase = fmod(t - p.trigTime, period);
+        }
+        p.prevInput = input;
+    }
     if (phase >= period) {
         p.out = p.base;
         return false;
     }
-    for (int ii = 0; ii < p.n; ++ii) {
-        if (phase < p.delay[ii]) {
-            p.out = p.base;
-            break;
-        } else if (phase < p.delay[ii] + p.width[ii]) {
-            p.out = p.level[ii];
-            break;
-        }
-        phase -= p.delay[ii];
-    }
+    p.out = pulse_level(p, phase, p.out);
     return true;
 }
 
@@ -991,11 +1025,15 @@ __device__ __noinline__ double vclamp_tick(VClampDev& c, double vIn) {
     return c.current;
 }
 
-// The varying injection of entry e for the tick ending at t: the call's host
-// additions (first tick), then what the entry's PulseGens and VClamps send
-// (clock tick 0), in their order (HSolve::addInject sums into
-// injectVarying).
-__device__ __forceinline__ double inject_varying(char* arena, const CellDev& c, int e, bool first, double t) {
+// The varying injection of entry e for the tick ending at t (tick stepNo,
+// 1 the first after reinit): the call's host additions (first tick), then
+// what the entry's PulseGens and VClamps send (clock tick 0), in their order
+// (HSolve::addInject sums into injectVarying).  A PulseGen driven by another
+// sees that one's output as the reference's immediate message delivers it:
+// this tick's when the source is processed first, else the previous tick's
+// (its reinit baseLevel at the first tick).
+__device__ __forceinline__ double inject_varying(char* arena, const CellDev& c, int e, bool first, double t,
+                                                 double dt, int64_t stepNo) {
     double* iv = A_<double>(arena, c.oInjVar);
     double var = first ? iv[e] : 0.0;
     if (first) iv[e] = 0.0;
@@ -1004,8 +1042,18 @@ __device__ __forceinline__ double inject_varying(char* arena, const CellDev& c, 
         PulseDev* pl = A_<PulseDev>(arena, c.oPulse);
         const int32_t* st = A_<const int32_t>(arena, c.oPulseStart);
         const int32_t* ix = A_<const int32_t>(arena, c.oPulseIdx);
-        for (int q = st[e]; q < st[e + 1]; ++q)
-            if (pulse_tick(pl[ix[q]], t)) var += pl[ix[q]].out;
+        for (int q = st[e]; q < st[e + 1]; ++q) {
+            PulseDev& p = pl[ix[q]];
+            double in = p.input;
+            if (p.src >= 0) {
+                const PulseDev& sp = pl[p.src];
+                in = p.srcBefore ? pulse_free_output(sp, t)
+                     : stepNo <= 1 ? sp.base : pulse_free_output(sp, dt * (double)(unsigned long long)(stepNo - 1));
+            }
+            if (pulse_tick(p, t, in)) var += p.out;
+            // input_ as read after the tick: the source's last message
+            if (p.src >= 0) p.input = p.srcBefore ? in : pulse_free_output(pl[p.src], t);
+        }
     }
     if (c.nVClamp > 0) {
         VClampDev* vc = A_<VClampDev>(arena, c.oVClamp);
@@ -1283,9 +1331,10 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
             {
                 const int32_t* ii = A_<const int32_t>(arena, c.oInjIdx);
                 const double* ib = A_<const double>(arena, c.oInjBasal);
-                const double t = a.dt * (double)(unsigned long long)(step0 + k + 1);
+                const int64_t stepNo = step0 + k + 1;
+                const double t = a.dt * (double)(unsigned long long)stepNo;
                 for (int e = tid; e < nInj; e += nt) {
-                    const double var = inject_varying(arena, c, e, first, t);
+                    const double var = inject_varying(arena, c, e, first, t, a.dt, stepNo);
                     w.B[ii[e]] += var + ib[e];
                 }
             }
@@ -1730,8 +1779,9 @@ __device__ __forceinline__ void row_inputs(const LaunchArgs& a, const CellDev& c
     if (e >= 0) {
         // this tick ends at step (steps done + 1); the solve kernel
         // advances the counter after it
-        const double t = a.dt * (double)(unsigned long long)(*A_<const int64_t>(a.arena, c.oSteps) + 1);
-        const double var = inject_varying(a.arena, c, e, a.firstOfCall != 0, t);
+        const int64_t stepNo = *A_<const int64_t>(a.arena, c.oSteps) + 1;
+        const double t = a.dt * (double)(unsigned long long)stepNo;
+        const double var = inject_varying(a.arena, c, e, a.firstOfCall != 0, t, a.dt, stepNo);
         B += var + A_<const double>(a.arena, c.oInjBasal)[e];
     }
     if (has2d) {
@@ -2121,7 +2171,15 @@ __global__ void __launch_bounds__(256) hsolve_reinit_kernel(LaunchArgs a) {
         const double* refr = A_<const double>(a.arena, c.oSpkRefr);
         for (int k = threadIdx.x; k < c.nSpike; k += blockDim.x) last[k] = -refr[k];
         PulseDev* pl = A_<PulseDev>(a.arena, c.oPulse);
-        for (int k = threadIdx.x; k < c.nPulse; k += blockDim.x) pl[k].out = pl[k].base;
+        // PulseGen::reinit (device/PulseGen.cpp:426-433)
+        for (int k = threadIdx.x; k < c.nPulse; k += blockDim.x) {
+            pl[k].out = pl[k].base;
+            // reinit clears input_; a source reinit after it then sends its
+            // baseLevel there
+            pl[k].input = pl[k].src >= 0 && !pl[k].srcBefore ? pl[pl[k].src].base : 0.0;
+            pl[k].prevInput = 0.0;
+            pl[k].trigTime = -1.0;
+        }
         if (threadIdx.x == 0) *A_<int64_t>(a.arena, c.oSteps) = 0;
     }
     __syncthreads();

@@ -1148,7 +1148,8 @@ int setup_cell(const hsg_cell_desc* d, double dt, int caAdvance, CellSetup& s, s
     s.pulses.clear();
     for (int k = 0; k < (d->n_pulsegens > 0 ? d->n_pulsegens : 0); ++k) {
         const hsg_pulsegen_desc& pd = d->pulsegens[k];
-        if (pd.compartment < 0 || pd.compartment >= nd || s.descToHines[pd.compartment] < 0) {
+        if (pd.compartment < -1 || pd.compartment >= nd ||
+            (pd.compartment >= 0 && s.descToHines[pd.compartment] < 0)) {
             err = "PulseGen on a compartment outside the cell";
             return HSG_E_MODEL;
         }
@@ -1156,17 +1157,38 @@ int setup_cell(const hsg_cell_desc* d, double dt, int caAdvance, CellSetup& s, s
             err = "PulseGen: 1 to HSG_MAX_PULSES pulses";
             return HSG_E_ARG;
         }
+        if (pd.trigMode < 0 || pd.trigMode > 2) {
+            err = "PulseGen: trigMode 0, 1 or 2";
+            return HSG_E_ARG;
+        }
+        const int np = d->n_pulsegens;
+        if (pd.trigMode != 0 && (pd.input_src < -1 || pd.input_src >= np || pd.input_src == k)) {
+            err = "PulseGen: input_src is -1 or another PulseGen of the cell";
+            return HSG_E_ARG;
+        }
         HostPulse p{};
-        p.compt = s.descToHines[pd.compartment];
+        p.compt = pd.compartment >= 0 ? s.descToHines[pd.compartment] : -1;
         p.n = pd.n_pulses;
+        p.trigMode = pd.trigMode;
+        p.src = pd.trigMode != 0 ? pd.input_src : -1;
         p.base = pd.baseLevel;
         for (int j = 0; j < p.n; ++j) {
             p.delay[j] = pd.delay[j];
             p.width[j] = pd.width[j];
             p.level[j] = pd.level[j];
         }
-        s.hasInject[p.compt] = 1;
+        if (p.compt >= 0) s.hasInject[p.compt] = 1;
         s.pulses.push_back(p);
+    }
+    for (const HostPulse& p : s.pulses) {
+        if (p.compt < 0 && p.trigMode != 0) {
+            err = "PulseGen: one without a compartment can only be a free-running input source";
+            return HSG_E_UNSUPPORTED;
+        }
+        if (p.src >= 0 && !pulse_source_ok(s.pulses[p.src])) {
+            err = "PulseGen: an input source must run free with pulses that cover its period";
+            return HSG_E_UNSUPPORTED;
+        }
     }
     // VClamps likewise (their currentOut goes to injectMsg)
     s.vclamps.clear();
@@ -1195,16 +1217,19 @@ int setup_cell(const hsg_cell_desc* d, double dt, int caAdvance, CellSetup& s, s
         int m = 0;
         for (int i = 0; i < n; ++i)
             if (s.hasInject[i]) entryOf[i] = m++;
-        for (auto& p : s.pulses) p.entry = entryOf[p.compt];
+        for (auto& p : s.pulses) p.entry = p.compt >= 0 ? entryOf[p.compt] : -1;
         for (auto& v : s.vclamps) v.entry = entryOf[v.compt];
         s.nSetupEntries = m;
+        // objects by entry (those with no entry are left out)
         auto ranges = [m](const std::vector<int>& entryOfObj, std::vector<int32_t>& start, std::vector<int32_t>& idx) {
             start.assign(m + 1, 0);
-            for (int e : entryOfObj) start[e + 1]++;
+            for (int e : entryOfObj)
+                if (e >= 0) start[e + 1]++;
             for (int e = 0; e < m; ++e) start[e + 1] += start[e];
-            idx.assign(entryOfObj.size(), 0);
+            idx.assign(start[m], 0);
             std::vector<int32_t> fill(start.begin(), start.end() - 1);
-            for (size_t k = 0; k < entryOfObj.size(); ++k) idx[fill[entryOfObj[k]]++] = (int32_t)k;
+            for (size_t k = 0; k < entryOfObj.size(); ++k)
+                if (entryOfObj[k] >= 0) idx[fill[entryOfObj[k]]++] = (int32_t)k;
         };
         std::vector<int> pe, ve;
         for (auto& p : s.pulses) pe.push_back(p.entry);
@@ -1294,6 +1319,40 @@ int setup_cell(const hsg_cell_desc* d, double dt, int caAdvance, CellSetup& s, s
             s.spkRefr.push_back(d->spikegens[k].refractT);
         }
     return 0;
+}
+
+double pulse_free_output(const HostPulse& p, double t) {
+    double period = p.width[0] + p.delay[0];
+    for (int ii = 1; ii < p.n && (p.width[ii] > 0.0 || p.delay[ii] > 0.0); ++ii) {
+        const double incr = p.delay[ii] + p.width[ii] - p.width[ii - 1];
+        if (incr > 0) period += incr;
+    }
+    double phase = std::fmod(t, period);
+    for (int ii = 0; ii < p.n; ++ii) {
+        if (phase < p.delay[ii]) return p.base;
+        if (phase < p.delay[ii] + p.width[ii]) return p.level[ii];
+        phase -= p.delay[ii];
+    }
+    return p.base;
+}
+
+bool pulse_source_ok(const HostPulse& p) {
+    if (p.trigMode != 0 || p.n < 1) return false;
+    // the period as PulseGen::process computes it (device/PulseGen.cpp:362-369)
+    double period = p.width[0] + p.delay[0];
+    for (int ii = 1; ii < p.n && (p.width[ii] > 0.0 || p.delay[ii] > 0.0); ++ii) {
+        const double incr = p.delay[ii] + p.width[ii] - p.width[ii - 1];
+        if (incr > 0) period += incr;
+    }
+    if (!(period > 0.0) || !std::isfinite(period)) return false;
+    // a phase escapes every pulse and gap only at or past
+    // max_ii (delay[0] + ... + delay[ii] + max(0, width[ii]))
+    double D = 0.0, reach = -INFINITY;
+    for (int ii = 0; ii < p.n; ++ii) {
+        D += p.delay[ii];
+        reach = std::max(reach, D + std::max(0.0, p.width[ii]));
+    }
+    return period <= reach;
 }
 
 }  // namespace hsg

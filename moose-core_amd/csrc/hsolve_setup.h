@@ -47,12 +47,24 @@ struct HostChan2D {
 };
 
 struct HostPulse {
-    int compt;                        // Hines index
-    int entry;                        // injection entry (Hines order of injected compartments)
+    int compt;                        // Hines index, -1 (drives another PulseGen's input only)
+    int entry;                        // injection entry (Hines order of injected compartments), -1
     int n;
+    int trigMode;                     // 0 free running, 1 external trigger, 2 external gate
+    int src;                          // trigMode 1-2: the PulseGen driving the input, -1
     double base;
     double delay[HSG_MAX_PULSES], width[HSG_MAX_PULSES], level[HSG_MAX_PULSES];
 };
+
+// Whether PulseGen p can drive another's input on the device: free running
+// with a positive period, and every phase in [0, period) inside one of its
+// pulses or the gaps before them, so that device/PulseGen.cpp:408-421 sets
+// the output on every tick and the output is a function of the time alone.
+bool pulse_source_ok(const HostPulse& p);
+
+// The output of such a source at time t (device/PulseGen.cpp:359-421 in
+// FREE_RUN mode), for a host read of one that feeds no compartment.
+double pulse_free_output(const HostPulse& p, double t);
 
 // A VClamp (biophysics/VClamp.cpp) sending to its compartment's injectMsg.
 struct HostVClamp {

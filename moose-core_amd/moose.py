@@ -556,14 +556,19 @@ class _PulseLookup:
 
 
 class PulseGen(Neutral):
-    """A free-running pulse train (trigMode 0, device/PulseGen.cpp:359-424)
-    sent through 'output' to a compartment's 'injectMsg'.  Under an HSolve it
-    is evaluated on the GPU before each tick."""
+    """A pulse train (device/PulseGen.cpp:359-424) sent through 'output' to a
+    compartment's 'injectMsg': free running (trigMode 0), or started by its
+    'input' (1, external trigger) or gated by it (2, external gate).  The
+    input is the 'input' field, or another PulseGen's 'output' connected to
+    it (which must run free).  Under an HSolve it is evaluated on the GPU
+    before each tick; 'output' and 'input' read from there."""
+    _zfields = ("output", "input")
+    _zmap = {"output": "pulseOutput", "input": "pulseInput"}
 
     def _init_fields(self):
         # device/PulseGen.cpp:148-165: two pulses, all zero
         self.__dict__.update(_level=[0.0, 0.0], _width=[0.0, 0.0], _delay=[0.0, 0.0],
-                             baseLevel=0.0, trigMode=0)
+                             baseLevel=0.0, trigMode=0, output=0.0, input=0.0)
 
     level = property(lambda self: _PulseLookup(self, "_level"))
     width = property(lambda self: _PulseLookup(self, "_width"))
@@ -590,8 +595,10 @@ class PulseGen(Neutral):
             if i < len(d[key]):             # the second* setters need count >= 2
                 d[key][i] = float(value)
             return
-        if name == "trigMode" and int(value) != 0:
-            raise NotImplementedError("PulseGen: only trigMode 0 (free running) is supported")
+        if name == "trigMode" and int(value) not in (0, 1, 2):
+            # PulseGen::process's default branch: the mode is kept but no
+            # pulse is produced; this layer refuses it instead
+            raise ValueError("PulseGen: trigMode is 0, 1 or 2")
         object.__setattr__(self, name, value)
 
     def __getattr__(self, name):
@@ -604,14 +611,37 @@ class PulseGen(Neutral):
             return vals[i] if i < len(vals) else 0.0
         return Neutral.__getattr__(self, name)
 
-    def _record(self, compartment):
+    def _record(self, compartment, input_src=-1):
         n = self.count
         if n > _abi.MAX_PULSES:
             raise ValueError(f"{self.path}: at most {_abi.MAX_PULSES} pulses under HSolve")
         r = np.zeros(1, _abi.PULSEGEN_DT)[0]
         r["compartment"], r["n_pulses"], r["baseLevel"] = compartment, n, self.baseLevel
         r["delay"][:n], r["width"][:n], r["level"][:n] = self._delay, self._width, self._level
+        r["trigMode"], r["input_src"] = int(self.trigMode), input_src
         return r
+
+
+def _pulse_list(cidx):
+    """The PulseGens a cell's solver runs, in creation order (the order clock
+    tick 0 processes them): those feeding its compartments' injectMsg, with
+    their compartment index, and the PulseGens whose 'output' drives one of
+    those's 'input', with -1 (they feed no compartment); and each one's
+    input source (an index in the list, or -1)."""
+    feeds = {s.id: (s, cidx[d.id]) for s, sf, d, df in _msgs
+             if sf == "output" and df == "injectMsg" and isinstance(s, PulseGen) and d.id in cidx}
+    src = {}
+    for pg, _ in list(feeds.values()):
+        drv = [s for s, sf, d, df in _msgs if d is pg and df == "input" and sf == "output" and isinstance(s, PulseGen)]
+        if len(drv) > 1:
+            raise ValueError(f"{pg.path}: more than one PulseGen drives its input under HSolve")
+        if drv:
+            src[pg.id] = drv[0]
+            if drv[0].id not in feeds:
+                feeds[drv[0].id] = (drv[0], -1)
+    order = sorted(feeds.values(), key=lambda x: x[0].id)
+    pos = {pg.id: k for k, (pg, _) in enumerate(order)}
+    return [(pg, ci, pos[src[pg.id].id] if pg.id in src else -1) for pg, ci in order]
 
 
 class VClamp(Neutral):
@@ -1126,21 +1156,20 @@ class HSolve(Neutral):
                 raise ValueError(f"{vc.path}: a VClamp under HSolve senses the compartment it drives")
             d = vc.__dict__
             vrec[k] = (ci, int(d["mode"]), d["tau"], d["ti"], d["td"], d["gain"])
-        # PulseGens feeding the compartments' injectMsg, in creation order
-        # (the order clock tick 0 processes them)
-        pgs = sorted(((s, cidx[d.id]) for s, sf, d, df in _msgs
-                      if sf == "output" and df == "injectMsg" and isinstance(s, PulseGen)
-                      and d.id in cidx), key=lambda x: x[0].id)
+        # PulseGens feeding the compartments' injectMsg and those driving
+        # their inputs, in creation order (the order clock tick 0 runs them)
+        pgs = _pulse_list(cidx)
         grec = np.zeros(len(pgs), _abi.PULSEGEN_DT)
-        for k, (pg, ci) in enumerate(pgs):
-            grec[k] = pg._record(ci)
+        for k, (pg, ci, src) in enumerate(pgs):
+            grec[k] = pg._record(ci, src)
         spec = CellSpec(ct, np.array(axial, dtype=np.int32).reshape(-1, 2), seed=cidx[seed.id],
                         channels=crec, gates=glist, pools=prec, spikegens=srec, leaks=lrec,
                         tables2d=tabs, chans2d=c2rec, pulsegens=grec, synchans=screc, synapses=sprec,
                         vclamps=vrec)
         objs = {"compartments": comps, "channels": [ch for _, ch in chans], "pools": pools,
                 "spikegens": [sg for _, sg in spikes], "chans2d": [ch for _, ch in ch2],
-                "pulsegens": pgs, "synchans": syns, "vclamps": [vc for vc, _ in vcs],
+                "pulsegens": [pg for pg, _, _ in pgs], "pulse_links": [(ci, src) for _, ci, src in pgs],
+                "synchans": syns, "vclamps": [vc for vc, _ in vcs],
                 "leaks": [lk for c in comps for lk in _channels(c, "Leakage")]}
         return spec, objs
 
@@ -1162,7 +1191,7 @@ class HSolve(Neutral):
         object.__setattr__(self, "_objs", objs)
         object.__setattr__(self, "_spec", spec)
         # zombify (hsolve/HSolve.cpp:235-262): fields now live in the solver
-        for kind in ("compartments", "channels", "pools", "chans2d", "synchans", "vclamps"):
+        for kind in ("compartments", "channels", "pools", "chans2d", "synchans", "vclamps", "pulsegens"):
             for i, o in enumerate(objs[kind]):
                 o.__dict__["_solver"] = (self, i)
                 for f in type(o)._zfields:       # reads now reach the solver
@@ -1207,14 +1236,12 @@ class HSolve(Neutral):
         next step, as the reference's PulseGen reads them every tick."""
         sent = self._spec.pulsegens
         cidx = {c.id: i for i, c in enumerate(self._objs["compartments"])}
-        now = sorted(((s.id, cidx[d.id]) for s, sf, d, df in _msgs
-                      if sf == "output" and df == "injectMsg" and isinstance(s, PulseGen)
-                      and d.id in cidx))
-        if now != [(pg.id, ci) for pg, ci in self._objs["pulsegens"]]:
-            raise ValueError("PulseGen messages to the solver's compartments changed after "
-                             "HSolve.target was set; connect them first")
-        for k, (pg, ci) in enumerate(self._objs["pulsegens"]):
-            r = pg._record(ci)
+        now = [(pg.id, ci, src) for pg, ci, src in _pulse_list(cidx)]
+        if now != [(pg.id, ci, src) for pg, (ci, src) in zip(self._objs["pulsegens"], self._objs["pulse_links"])]:
+            raise ValueError("PulseGen messages to the solver's compartments or PulseGen inputs changed "
+                             "after HSolve.target was set; connect them first")
+        for k, (pg, (ci, src)) in enumerate(zip(self._objs["pulsegens"], self._objs["pulse_links"])):
+            r = pg._record(ci, src)
             if r.tobytes() != sent[k].tobytes():
                 self._engine.set_pulsegen(self._cell, k, r)
                 sent[k] = r

@@ -129,12 +129,15 @@ typedef struct {
     double  *injectVarying, *injectBasal;
 } or_hines;
 
-/* A free-running PulseGen (device/PulseGen.cpp) sending to a compartment's
- * injectMsg. */
+/* A PulseGen (device/PulseGen.cpp) sending to a compartment's injectMsg
+ * (ic >= 0) and/or to other PulseGens' input. */
 typedef struct {
-    int    ic;                        /* Hines index of the target */
+    int    ic;                        /* Hines index of the target, -1 */
     int    n;                         /* entries of delay/width/level */
+    int    trigMode;                  /* trigMode_ */
+    int    src;                       /* the PulseGen whose output drives input_, -1 */
     double base, out;                 /* baseLevel_, output_ */
+    double input, prevInput, trigTime;/* input_, prevInput_, trigTime_ */
     double delay[HSG_MAX_PULSES], width[HSG_MAX_PULSES], level[HSG_MAX_PULSES];
 } or_pulse;
 
@@ -1171,22 +1174,30 @@ int or_create(const hsg_cell_desc* d, double dt, int caAdvance, or_solver** out)
     s->pulse = (or_pulse*)calloc(s->nPulse + 1, sizeof(or_pulse));
     for (int k = 0; k < s->nPulse; ++k) {
         const hsg_pulsegen_desc* pd = &d->pulsegens[k];
-        if (pd->compartment < 0 || pd->compartment >= d->n_compartments || s->descToHines[pd->compartment] < 0 ||
-            pd->n_pulses < 1 || pd->n_pulses > HSG_MAX_PULSES) {
+        if ((pd->compartment >= 0 &&
+             (pd->compartment >= d->n_compartments || s->descToHines[pd->compartment] < 0)) ||
+            pd->compartment < -1 || pd->n_pulses < 1 || pd->n_pulses > HSG_MAX_PULSES || pd->trigMode < 0 ||
+            pd->trigMode > 2 ||
+            (pd->trigMode != 0 && (pd->input_src < -1 || pd->input_src >= s->nPulse || pd->input_src == k))) {
             or_destroy(s);
             return HSG_E_ARG;
         }
         or_pulse* p = &s->pulse[k];
-        p->ic = s->descToHines[pd->compartment];
+        p->ic = pd->compartment >= 0 ? s->descToHines[pd->compartment] : -1;
         p->n = pd->n_pulses;
+        p->trigMode = pd->trigMode;
+        p->src = pd->trigMode != 0 ? pd->input_src : -1;
         p->base = pd->baseLevel;
         p->out = 0.0;
+        p->input = 0.0;
+        p->prevInput = 0.0;
+        p->trigTime = -1.0;
         for (int j = 0; j < p->n; ++j) {
             p->delay[j] = pd->delay[j];
             p->width[j] = pd->width[j];
             p->level[j] = pd->level[j];
         }
-        s->h.hasInject[p->ic] = 1;
+        if (p->ic >= 0) s->h.hasInject[p->ic] = 1;
     }
     /* Leakage objects keep sending on their own tick */
     s->nLeak = d->n_leaks > 0 ? d->n_leaks : 0;
@@ -1278,16 +1289,35 @@ static void vclamp_process(or_vclamp* c) {
     }
 }
 
-/* PulseGen::process in FREE_RUN mode (device/PulseGen.cpp:359-424): the
- * output at time t; returns 0 when the reference sends nothing (phase past
- * the period), leaving out at baseLevel. */
+/* PulseGen::process (device/PulseGen.cpp:359-424): the output at time t in
+ * the PulseGen's trigger mode, from its input_; returns 0 when the reference
+ * sends nothing (phase past the period), leaving out at baseLevel. */
 static int pulse_process(or_pulse* p, double t) {
     double period = p->width[0] + p->delay[0];
     for (int ii = 1; ii < p->n && (p->width[ii] > 0.0 || p->delay[ii] > 0.0); ++ii) {
         double incr = p->delay[ii] + p->width[ii] - p->width[ii - 1];
         if (incr > 0) period += incr;
     }
-    double phase = fmod(t, period);
+    double phase = 0.0;
+    if (p->trigMode == 0) {
+        phase = fmod(t, period);
+    } else if (p->trigMode == 1) {             /* EXT_TRIG */
+        if (p->input == 0) {
+            phase = p->trigTime < 0 ? period : t - p->trigTime;
+        } else {
+            if (p->prevInput == 0) p->trigTime = t;
+            phase = t - p->trigTime;
+        }
+        p->prevInput = p->input;
+    } else {                                   /* EXT_GATE */
+        if (p->input == 0) {
+            phase = period;
+        } else {
+            if (p->prevInput == 0) p->trigTime = t;
+            phase = fmod(t - p->trigTime, period);
+        }
+        p->prevInput = p->input;
+    }
     if (phase >= period) {
         p->out = p->base;
         return 0;
@@ -1303,6 +1333,16 @@ static int pulse_process(or_pulse* p, double t) {
         phase -= p->delay[ii];
     }
     return 1;
+}
+
+/* outputOut of PulseGen k (device/PulseGen.cpp:423, :432): to its
+ * compartment's injectMsg (HSolve::addInject) and to the input of every
+ * PulseGen it drives, at once, as the reference delivers messages. */
+static void pulse_send(or_solver* s, int k) {
+    const or_pulse* p = &s->pulse[k];
+    if (p->ic >= 0) s->h.injectVarying[p->ic] += p->out;
+    for (int j = 0; j < s->nPulse; ++j)
+        if (s->pulse[j].src == k) s->pulse[j].input = p->out;
 }
 
 void or_destroy(or_solver* s) {
@@ -1377,8 +1417,12 @@ void or_reinit(or_solver* s, double dt) {
      * baseLevel; the solver's reinit leaves injectVarying alone, so it is
      * added to the first step */
     for (int k = 0; k < s->nPulse; ++k) {
-        s->pulse[k].out = s->pulse[k].base;
-        h->injectVarying[s->pulse[k].ic] += s->pulse[k].base;
+        or_pulse* p = &s->pulse[k];
+        p->trigTime = -1.0;
+        p->prevInput = 0.0;
+        p->out = p->base;
+        p->input = 0.0;
+        pulse_send(s, k);
     }
     /* VClamp::reinit (VClamp.cpp:313-347, tick 0): the error terms and
      * the command filter restart at the compartment's initVm; the current
@@ -1694,7 +1738,7 @@ void or_step(or_solver* s, double dt, double currTime) {
     /* PulseGens (tick 0), then the HHChannel2D objects (tick 4), before the
      * solver (tick 6) */
     for (int k = 0; k < s->nPulse; ++k)
-        if (pulse_process(&s->pulse[k], currTime)) h->injectVarying[s->pulse[k].ic] += s->pulse[k].out;
+        if (pulse_process(&s->pulse[k], currTime)) pulse_send(s, k);
     /* VClamps (tick 0, after the PulseGens), from the Vm the solver sent at
      * the end of the previous tick */
     for (int k = 0; k < s->nVc; ++k) {
@@ -1822,6 +1866,13 @@ int or_get_field(const or_solver* s, int field, const int* idx, int n, double* o
             case HSG_F_INSTANT: out[k] = (double)s->instant[c]; break;
             default: return HSG_E_ARG;
             }
+        } else if (field >= 110) {
+            if (i < 0 || i >= s->nPulse) return HSG_E_ARG;
+            switch (field) {
+            case HSG_F_PULSE_OUTPUT: out[k] = s->pulse[i].out; break;
+            case HSG_F_PULSE_INPUT: out[k] = s->pulse[i].input; break;
+            default: return HSG_E_ARG;
+            }
         } else if (field >= 100) {
             if (i < 0 || i >= s->nVc) return HSG_E_ARG;
             switch (field) {
@@ -1919,6 +1970,11 @@ int or_set_field(or_solver* s, int field, const int* idx, int n, const double* v
             case HSG_F_INSTANT: s->instant[c] = (int)x; break;
             default: return HSG_E_ARG;
             }
+        } else if (field >= 110) {
+            /* PulseGen::input (device/PulseGen.cpp:355-358) where no
+             * PulseGen's output drives it */
+            if (i < 0 || i >= s->nPulse || field != HSG_F_PULSE_INPUT || s->pulse[i].src >= 0) return HSG_E_ARG;
+            s->pulse[i].input = x;
         } else if (field >= 100) {
             if (i < 0 || i >= s->nVc) return HSG_E_ARG;
             switch (field) {

@@ -90,7 +90,8 @@ FIELD = {"Vm": 1, "Im": 2, "inject": 3, "initVm": 4, "Cm": 5, "Em": 6, "Rm": 7, 
          "Gbar2D": 60, "Ek2D": 61, "Gk2D": 62, "X2D": 64, "Y2D": 65, "Z2D": 66, "modulation2D": 67,
          "GbarSyn": 80, "EkSyn": 81, "GkSyn": 82, "tau1": 83, "tau2": 84, "modulationSyn": 85,
          "XSyn": 86, "YSyn": 87, "weight": 90, "delay": 91,
-         "GbarLeak": 95, "EkLeak": 96, "modulationLeak": 97, "command": 100, "current": 101}
+         "GbarLeak": 95, "EkLeak": 96, "modulationLeak": 97, "command": 100, "current": 101,
+         "pulseOutput": 110, "pulseInput": 111}
 
 
 class OracleCell:

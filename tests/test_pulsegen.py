@@ -1,5 +1,5 @@
-"""PulseGen stimuli under the solver: free-running pulse trains
-(device/PulseGen.cpp:359-424, trigMode 0) sent through outputOut to a
+"""PulseGen stimuli under the solver: pulse trains (device/PulseGen.cpp:
+359-424; free running, externally triggered or gated) sent through outputOut to a
 compartment's injectMsg, which the solver sums into injectVarying
 (HSolve::addInject, hsolve/HSolveInterface.cpp:213-219).  PulseGens run on
 clock tick 0, before the solver's tick 6, at the tick's end time
@@ -121,6 +121,99 @@ def test_reinit_sends_base_level_once(oracle_mod):
 
 
 # ---------------------------------------------------------------------------
+# trigger modes (PulseGen::process, device/PulseGen.cpp:371-400)
+# ---------------------------------------------------------------------------
+# Whole-tick timing (dt = 2**-15 s, delays and widths whole ticks), so every
+# phase is exact.  The expected sequences below were worked out by hand from
+# the documented behaviour, one tick at a time: EXT_TRIG (1): a rise of the
+# input starts one train at that tick (phase 0); it runs to its end whatever
+# the input does; before the first rise and after the train's end nothing
+# is sent.  EXT_GATE (2): while the input is high the train runs, restarting
+# at each rise; while it is low nothing is sent.  A tick that sends nothing
+# leaves output at baseLevel.  None = nothing sent.
+TDT = 2.0 ** -15
+B, L = 0.05e-9, 0.8e-9
+# one pulse: delay 2 ticks, width 3 ticks (period 5 ticks)
+TRIG_TRAIN = dict(base=B, delay=[2 * TDT], width=[3 * TDT], level=[L])
+TRIG_INPUT = [0, 0, 0, 1, 1, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0]
+TRIG_SENT = [None, None, None, B, B, L, L, L, None, None, None, B, B, L, L, L, None, None]
+GATE_INPUT = [0, 1, 1, 1, 1, 1, 1, 1, 0, 0, 1, 1, 1, 0, 1, 1, 1, 1]
+GATE_SENT = [None, B, B, L, L, L, B, B, None, None, B, B, L, None, B, B, L, L]
+
+
+def _pulse_cell(rng_seed, trains, comps, modes, srcs=None):
+    spec = workloads.random_hh_cell(np.random.default_rng(rng_seed), 24, inject=0.0)
+    spec = cell_with_trains(spec, trains, comps)
+    spec.pulsegens["trigMode"] = modes
+    spec.pulsegens["input_src"] = srcs if srcs is not None else [-1] * len(modes)
+    return spec
+
+
+@pytest.mark.parametrize("mode,inputs,sent", [(1, TRIG_INPUT, TRIG_SENT), (2, GATE_INPUT, GATE_SENT)])
+def test_trigger_modes_tick_by_tick(oracle_mod, mode, inputs, sent):
+    """The oracle's output of a triggered PulseGen equals the hand-worked
+    sequence tick by tick, and its cell's trace equals a copy without the
+    PulseGen that gets baseLevel after reinit and, through addInject, each
+    value the sequence says is sent."""
+    a = oracle_mod.OracleCell(_pulse_cell(8, [TRIG_TRAIN], [5], [mode]), TDT)
+    b = oracle_mod.OracleCell(workloads.random_hh_cell(np.random.default_rng(8), 24, inject=0.0), TDT)
+    a.reinit(TDT)
+    b.reinit(TDT)
+    b.add_inject(5, B)
+    outs, ta, tb = [], [], []
+    for x, v in zip(inputs, sent):
+        a.set("pulseInput", [0], [x])
+        ta.append(a.run(TDT, 1, np.arange(24))[0])
+        outs.append(a.get("pulseOutput", [0])[0])
+        if v is not None:
+            b.add_inject(5, v)
+        tb.append(b.run(TDT, 1, np.arange(24))[0])
+    assert outs == [B if v is None else v for v in sent]
+    np.testing.assert_array_equal(np.array(ta), np.array(tb))
+
+
+# A free-running source (delay 1 tick, width 2, level 1, base 0: output
+# 1, 1, 0 repeating from tick 1) drives an EXT_GATE PulseGen (delay 1,
+# width 1: base then level while high).  The source's message reaches the
+# gate's input at once: when the source comes first it has already sent this
+# tick's output (inputs 1, 1, 0, ...); when it comes after, the gate sees
+# the previous tick's (inputs 0 = the source's reinit baseLevel, then 1, 1,
+# 0, ...).  Sent values worked by hand:
+DRIVEN_SENT = {True: [B, L, None] * 4, False: [None, B, L] * 4}
+
+
+@pytest.mark.parametrize("src_first", [True, False])
+def test_driven_input_follows_message_order(oracle_mod, src_first):
+    """The hand-worked sends of a PulseGen driven by another's output, in
+    both orders, and the trace of a copy of the cell that gets them through
+    addInject."""
+    src = dict(base=0.0, delay=[1 * TDT], width=[2 * TDT], level=[1.0])     # period 3 ticks
+    dst = dict(base=B, delay=[1 * TDT], width=[1 * TDT], level=[L])
+    if src_first:
+        spec = _pulse_cell(9, [src, dst], [-1, 4], [0, 2], [-1, 0])
+        si, di = 0, 1
+    else:
+        spec = _pulse_cell(9, [dst, src], [4, -1], [2, 0], [1, -1])
+        si, di = 1, 0
+    a = oracle_mod.OracleCell(spec, TDT)
+    b = oracle_mod.OracleCell(workloads.random_hh_cell(np.random.default_rng(9), 24, inject=0.0), TDT)
+    a.reinit(TDT)
+    b.reinit(TDT)
+    b.add_inject(4, B)
+    src_out, outs, ta, tb = [], [], [], []
+    for v in DRIVEN_SENT[src_first]:
+        ta.append(a.run(TDT, 1, np.arange(24))[0])
+        src_out.append(a.get("pulseOutput", [si])[0])
+        outs.append(a.get("pulseOutput", [di])[0])
+        if v is not None:
+            b.add_inject(4, v)
+        tb.append(b.run(TDT, 1, np.arange(24))[0])
+    assert src_out == [1.0, 1.0, 0.0] * 4
+    assert outs == [B if v is None else v for v in DRIVEN_SENT[src_first]]
+    np.testing.assert_array_equal(np.array(ta), np.array(tb))
+
+
+# ---------------------------------------------------------------------------
 # object layer
 # ---------------------------------------------------------------------------
 def test_pulsegen_fields(pkg):
@@ -144,8 +237,10 @@ def test_pulsegen_fields(pkg):
         pg.level[1] = 1.0
     with pytest.raises(ValueError):
         pg.count = 0
-    with pytest.raises(NotImplementedError):
-        pg.trigMode = 1
+    pg.trigMode = 1
+    pg.trigMode = 2
+    with pytest.raises(ValueError):
+        pg.trigMode = 3
     moose.reset()
 
 
@@ -174,10 +269,31 @@ def test_layer_collects_pulsegens(pkg):
     hs.dt = 50e-6
     spec, objs = hs.build_spec(moose._deep_search_compartment(moose.element("/cell")))
     pg = spec.pulsegens
-    assert [p for p, _ in objs["pulsegens"]] == [p1, p2]
+    assert objs["pulsegens"] == [p1, p2] and objs["pulse_links"] == [(0, -1), (5, -1)]
     assert list(pg["compartment"]) == [0, 5] and list(pg["n_pulses"]) == [2, 3]
     assert pg[0]["delay"][1] == 1e9 and pg[1]["baseLevel"] == 0.02e-9
     np.testing.assert_array_equal(pg[1]["level"][:3], [0.4e-9, -0.2e-9, 0.3e-9])
+    moose.reset()
+
+
+def test_layer_collects_trigger_sources(pkg):
+    """A PulseGen whose 'output' drives a triggered PulseGen's 'input' joins
+    the solver's list in creation order without a compartment, and the
+    triggered one names it as its input source."""
+    from moose_core_amd import moose
+    comps, _, (p1, p2) = build_pulse_model(moose)
+    trig = moose.PulseGen("/cell/trig")
+    trig.firstDelay, trig.firstWidth, trig.firstLevel = 1e-3, 1e-3, 1.0
+    p2.trigMode = 1
+    moose.connect(trig, "output", p2, "input")
+    hs = moose.HSolve("/cell/hsolve")
+    hs.dt = 50e-6
+    spec, objs = hs.build_spec(moose._deep_search_compartment(moose.element("/cell")))
+    pg = spec.pulsegens
+    assert objs["pulsegens"] == [p1, p2, trig]
+    assert objs["pulse_links"] == [(0, -1), (5, 2), (-1, -1)]
+    assert list(pg["compartment"]) == [0, 5, -1] and list(pg["trigMode"]) == [0, 1, 0]
+    assert list(pg["input_src"]) == [-1, 2, -1]
     moose.reset()
 
 
@@ -562,3 +678,82 @@ def test_gpu_pulsegen_on_every_compartment(pkg, oracle_mod, mode):
     got = eng.read_probes(cid)
     eng.close()
     assert np.abs(got - np.concatenate(ref_tr)).max() <= FP64_TOL
+
+
+# ---------------------------------------------------------------------------
+# trigger modes on the GPU
+# ---------------------------------------------------------------------------
+def _trigger_cells():
+    """Triggered PulseGens with host inputs (one EXT_TRIG, one EXT_GATE on
+    another compartment), and the two driven orders; ms-scale trains."""
+    dt = 25e-6
+    src = dict(base=0.0, delay=[0.4e-3], width=[0.6e-3], level=[1.0])           # period 1 ms
+    trig = dict(base=0.02e-9, delay=[0.3e-3], width=[1.2e-3], level=[0.9e-9])
+    gate = dict(base=0.0, delay=[0.2e-3, 0.5e-3], width=[0.3e-3, 0.4e-3], level=[0.7e-9, 1.1e-9])
+    host = _pulse_cell(21, [trig, gate], [0, 9], [1, 2])
+    first = _pulse_cell(22, [src, trig, gate], [-1, 0, 5], [0, 1, 2], [-1, 0, 0])
+    after = _pulse_cell(23, [trig, gate, src], [0, 5, -1], [1, 2, 0], [2, 2, -1])
+    return dt, [host, first, after]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["fused", "split"])
+def test_gpu_trigger_modes(pkg, oracle_mod, mode):
+    """EXT_TRIG and EXT_GATE PulseGens with inputs written by the host
+    between calls, and driven by a free-running PulseGen before and after
+    them, against the oracle (FP64): Vm within 1e-6 V at every sampled
+    step, the same outputs and inputs after every call."""
+    print(f"case: trigger-mode PulseGens (host inputs, driven both orders), fp64, {mode}", flush=True)
+    dt, specs = _trigger_cells()
+    eng = pkg.Engine("fp64", kernel_mode=mode, steps_per_launch=10)
+    ids = [eng.add_cell(s, dt) for s in specs]
+    refs = [oracle_mod.OracleCell(s, dt) for s in specs]
+    for cid, s in zip(ids, specs):
+        eng.set_probes(cid, np.arange(s.n), 2000)
+    eng.reinit(dt)
+    for r in refs:
+        r.reinit(dt)
+    traces = [[] for _ in specs]
+    # host inputs of the first cell's two PulseGens, per call
+    plan = [(40, 0, 0), (30, 1, 1), (70, 0, 1), (60, 0, 0), (25, 1, 1), (90, 1, 0), (50, 0, 1)]
+    for nsteps, x_trig, x_gate in plan:
+        eng.set(ids[0], "pulseInput", [0, 1], [x_trig, x_gate])
+        refs[0].set("pulseInput", [0, 1], [x_trig, x_gate])
+        eng.step(nsteps)
+        for k, (cid, r, s) in enumerate(zip(ids, refs, specs)):
+            traces[k].append(r.run(dt, nsteps, np.arange(s.n)))
+            npg = len(s.pulsegens)
+            assert list(eng.get(cid, "pulseOutput", np.arange(npg))) == list(r.get("pulseOutput", np.arange(npg)))
+            assert list(eng.get(cid, "pulseInput", np.arange(npg))) == list(r.get("pulseInput", np.arange(npg)))
+    for k, cid in enumerate(ids):
+        err = np.abs(eng.read_probes(cid) - np.concatenate(traces[k])).max()
+        print(f"cell {k}: max|dVm| {err:.3e} V")
+        assert err <= FP64_TOL
+    # the stimuli did something
+    assert np.abs(np.concatenate(traces[0])[:, 0] - np.concatenate(traces[0])[0, 0]).max() > 1e-3
+    eng.close()
+
+
+@pytest.mark.gpu
+def test_gpu_trigger_source_rules(pkg):
+    """Sources that do not run free or whose pulses leave part of the period
+    uncovered, and triggered PulseGens without a compartment, are refused
+    at setup; an input written to a driven PulseGen is refused."""
+    print("case: PulseGen trigger setup rules", flush=True)
+    from moose_core_amd.engine import HSolveError
+    ok = dict(base=0.0, delay=[1 * TDT], width=[2 * TDT], level=[1.0])
+    # the second pulse starts before the first ends: phases in [8, 9) ticks
+    # fall past every pulse, so the output would keep its last value
+    overlap = dict(base=0.0, delay=[1 * TDT, 1 * TDT, 5 * TDT], width=[3 * TDT, 1 * TDT, 1 * TDT],
+                   level=[1.0, 1.0, 1.0])
+    eng = pkg.Engine("fp64")
+    cid = eng.add_cell(_pulse_cell(9, [ok, TRIG_TRAIN], [-1, 4], [0, 2], [-1, 0]), TDT)
+    eng.reinit(TDT)
+    with pytest.raises(HSolveError):
+        eng.set(cid, "pulseInput", [1], [1.0])
+    for bad in (_pulse_cell(9, [TRIG_TRAIN, TRIG_TRAIN], [3, 4], [1, 2], [-1, 0]),     # source triggered
+                _pulse_cell(9, [overlap, TRIG_TRAIN], [-1, 4], [0, 1], [-1, 0]),       # source uncovered
+                _pulse_cell(9, [TRIG_TRAIN, ok], [-1, 4], [1, 0], [-1, -1])):          # triggered, no compartment
+        with pytest.raises(HSolveError):
+            eng.add_cell(bad, TDT)
+    eng.close()
