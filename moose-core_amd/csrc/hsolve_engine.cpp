@@ -23,7 +23,7 @@
 #include "hsolve_setup.h"
 
 extern "C" hipError_t hsg_launch_step(const hsg::LaunchArgs* a, int nblocks, int threads, int fp32, int slow,
-                                     int fastms, hipStream_t stream);
+                                     int fastms, int lean, hipStream_t stream);
 extern "C" hipError_t hsg_launch_reinit(const hsg::LaunchArgs* a, int nblocks, int fp32,
                                        hipStream_t stream);
 extern "C" hipError_t hsg_set_max_lds(int bytes);
@@ -284,6 +284,7 @@ struct hsg_engine {
         int topLds = 0;             // dynamic LDS of the top kernel
         int key = 0;                // the bin's class (second half of a pair: + 32)
         int fastSel = -1;           // fast channel pass: -1 the engine's choice, 0 off, 1 on
+        bool lean = false;          // no stimulus or external objects and no Ca pools in any cell
     };
     // Launch shapes set by the caller per bin class (hsg_set_bin_shape):
     // threads per cell workgroup (0: the engine's choice) and the fast
@@ -604,7 +605,9 @@ int place_cell(hsg_engine* e, HostCell& hc) {
         {&d.oPoolCount, 4 * n},
         {&d.oLevelStart, 4 * (int64_t)s.levelStart.size()}, {&d.oUnits, 4 * (int64_t)s.units.size()},
         {&d.oGrpMem, 4 * nMem}, {&d.oGrpInfo, 16 * nGroups}, {&d.oHJ0, 8 * (int64_t)s.HJ0.size()},
-        {&d.oDesc, 4 * (int64_t)s.udesc.size()},
+        // at least one descriptor: the step kernels load a level's
+        // descriptors ahead without a branch (clamped to the last unit)
+        {&d.oDesc, 4 * (int64_t)std::max<size_t>(s.udesc.size(), kDescInts)},
         {&d.oWork, workB},
         {&d.oSpkCompt, 4 * (int64_t)s.spkCompt.size()}, {&d.oSpkEdge, 4 * (int64_t)s.spkCompt.size()},
         {&d.oSpkThr, 8 * (int64_t)s.spkCompt.size()}, {&d.oSpkRefr, 8 * (int64_t)s.spkCompt.size()},
@@ -1263,12 +1266,14 @@ void plan_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg
     for (auto& b : bins) {
         int maxN = 0, maxSlots = 0;
         bool allUniform = true;
+        b.lean = true;
         for (int id : b.cells) {
             const CellDev& d = e->cells[id]->dev;
             b.lds = std::max<int>(b.lds, d.ldsBytes);
             maxN = std::max(maxN, d.n);
             maxSlots = std::max(maxSlots, d.nChanSlots);
             allUniform = allUniform && d.fastOk;
+            b.lean = b.lean && d.nPulse == 0 && d.nVClamp == 0 && d.nXObj == 0 && !d.hasPools;
         }
         b.chanMS = allUniform ? (maxSlots <= 2 ? 2 : maxSlots <= 4 ? 4 : 8) : 0;
         b.sumN = 0;
@@ -1619,7 +1624,8 @@ int launch_round(hsg_engine* e, std::vector<hsg_engine::Bin>& bins, int chunk, i
                 a.ldsBytes = bins[b].lds;
                 if (!split) {
                     HIPCHK(hsg_launch_external(&a, bins[b].dTiles, bins[b].nTiles, st));
-                    HIPCHK(hsg_launch_step(&a, nb, bins[b].threads, e->fp32, bins[b].slow, fused_fast(bins[b]), st));
+                    HIPCHK(hsg_launch_step(&a, nb, bins[b].threads, e->fp32, bins[b].slow, fused_fast(bins[b]),
+                                           false, st));
                     continue;
                 }
                 if (int rc = split_wide(e, bins[b], a, st)) return rc;
@@ -1628,7 +1634,8 @@ int launch_round(hsg_engine* e, std::vector<hsg_engine::Bin>& bins, int chunk, i
         } else {
             LaunchArgs a = make_args(e, bins[b].dOrder, chunk, first);
             a.ldsBytes = bins[b].lds;
-            HIPCHK(hsg_launch_step(&a, nb, bins[b].threads, e->fp32, bins[b].slow, fused_fast(bins[b]), st));
+            HIPCHK(hsg_launch_step(&a, nb, bins[b].threads, e->fp32, bins[b].slow, fused_fast(bins[b]),
+                                   bins[b].lean, st));
         }
         if (timed) HIPCHK(hipEventRecord(e->evBin1[b], st));
         HIPCHK(hipEventRecord(e->evJoin[b], st));
@@ -2950,7 +2957,8 @@ int hsg_plan_info(hsg_engine* e, hsg_bin_info* out, int32_t cap, int32_t* n_out)
                      : x.solveThreads > 256 ? "hsolve_solve_wide_kernel" : "hsolve_solve_kernel", prec);
         else
             snprintf(o.kernel, sizeof o.kernel, "%s<%s>", x.slow ? "hsolve_step_slow_kernel"
-                     : fused_fast(x) ? "hsolve_step_fast_kernel" : "hsolve_step_kernel", prec);
+                     : fused_fast(x) ? (x.lean && !x.hasExt ? "hsolve_step_lean_kernel" : "hsolve_step_fast_kernel")
+                     : "hsolve_step_kernel", prec);
     }
     return 0;
 }

@@ -122,12 +122,34 @@ __device__ __forceinline__ GT take_power(GT x, double p) {
     return (GT)power_general((double)x, p);
 }
 
-// trapezoidal gate update, dx/dt = A - B x (HSolveActive.cpp:259-268)
-template <typename GT>
-__device__ __forceinline__ GT gate_step(GT x, GT C1, GT C2, GT dt, bool instant) {
-    if (instant) return C1 / C2;
-    GT temp = (GT)1.0 + dt / (GT)2.0 * C2;
-    return (x * ((GT)2.0 - temp) + dt * C1) / temp;
+// Quotient a / b.  FAST = false: the IEEE quotient (the FP64 form reproduces
+// the reference bit for bit).  FAST = true (the mixed form): the reciprocal
+// refined by two Newton steps and the quotient corrected once, within one
+// ulp of the IEEE quotient for operands of ordinary range, in half the
+// dependent steps.
+template <bool FAST>
+__device__ __forceinline__ double quot(double a, double b) {
+    if constexpr (!FAST) {
+        return a / b;
+    } else {
+        double r = __builtin_amdgcn_rcp(b);
+        double e = fma(-b, r, 1.0);
+        r = fma(e, r, r);
+        e = fma(-b, r, 1.0);
+        r = fma(e, r, r);
+        double q = a * r;
+        e = fma(-b, q, a);
+        return fma(e, r, q);
+    }
+}
+
+// trapezoidal gate update, dx/dt = A - B x (HSolveActive.cpp:259-268), in
+// FP64; FAST (the mixed form) divides with quot<true>
+template <bool FAST>
+__device__ __forceinline__ double gate_step(double x, double C1, double C2, double dt, bool instant) {
+    if (instant) return quot<FAST>(C1, C2);
+    double temp = 1.0 + dt / 2.0 * C2;
+    return quot<FAST>(x * (2.0 - temp) + dt * C1, temp);
 }
 
 struct Work {
@@ -214,7 +236,7 @@ __device__ __forceinline__ void channel_row(const View<GT>& v, const ChanType* _
             double* sp = v.state + (size_t)js * n + i;
             table_lookup<GT>(v.vt, vBase, ct.colX, frac, C1, C2);
             const double x = reinitOnly ? (double)C1 / (double)C2
-                                        : gate_step<double>(*sp, C1, C2, dtg, ct.instant & 1);
+                                        : gate_step<std::is_same<GT, float>::value>(*sp, C1, C2, dtg, ct.instant & 1);
             *sp = x;
             fraction *= take_power<GT>((GT)x, ct.Xpower);
             ++js;
@@ -223,7 +245,7 @@ __device__ __forceinline__ void channel_row(const View<GT>& v, const ChanType* _
             double* sp = v.state + (size_t)js * n + i;
             table_lookup<GT>(v.vt, vBase, ct.colY, frac, C1, C2);
             const double x = reinitOnly ? (double)C1 / (double)C2
-                                        : gate_step<double>(*sp, C1, C2, dtg, ct.instant & 2);
+                                        : gate_step<std::is_same<GT, float>::value>(*sp, C1, C2, dtg, ct.instant & 2);
             *sp = x;
             fraction *= take_power<GT>((GT)x, ct.Ypower);
             ++js;
@@ -247,7 +269,7 @@ __device__ __forceinline__ void channel_row(const View<GT>& v, const ChanType* _
                 }
             }
             const double x = reinitOnly ? (double)C1 / (double)C2
-                                        : gate_step<double>(*sp, C1, C2, dtg, ct.instant & 4);
+                                        : gate_step<std::is_same<GT, float>::value>(*sp, C1, C2, dtg, ct.instant & 4);
             *sp = x;
             fraction *= take_power<GT>((GT)x, ct.Zpower);
             ++js;
@@ -573,36 +595,62 @@ __device__ __forceinline__ double desc_off(const Desc& u) { return desc_double(u
 // HinesMatrix::makeOperands, hsolve/HinesMatrix.cpp:225-365, and of
 // HSolvePassive::forwardEliminate, hsolve/HSolvePassive.cpp:262-298, ranks 1
 // and 2 and the linear run).
+template <bool FD = false>
 __device__ __forceinline__ void fwd_small(const Desc& u, Work w) {
     const bool three = (u.a.x & 15) == 3;
     const int m0 = u.a.w & 0xffff, m1 = (u.a.w >> 16) & 0xffff, m2 = u.b.x;
     const int deferLast = u.a.y;
+    const int cnt = three ? (u.a.x >> 8) & 15 : 0;
+    const int tb = u.a.z;
     const double a01 = desc_double(u, 6);
+    // Every row and term the unit reads, read before anything is written, so
+    // that one LDS round trip brings them all: no other unit of the level
+    // writes them (a row has one feeder and one consumer, at different
+    // levels), and the terms of member 1's feeder (at most two for a binary
+    // junction; more in the loop below) were written at earlier levels.
     const double D0 = w.D[m0], B0 = w.B[m0];
     double D1 = w.D[m1], B1 = w.B[m1];
-    const double l1 = a01 / D0;
+    double D2 = 0.0, B2 = 0.0, t0D = 0.0, t0B = 0.0, t1D = 0.0, t1B = 0.0;
+    if (three && deferLast < 0) {
+        D2 = w.D[m2];
+        B2 = w.B[m2];
+    }
+    if (cnt > 0) {
+        t0D = w.T[2 * tb];
+        t0B = w.T[2 * tb + 1];
+    }
+    if (cnt > 1) {
+        t1D = w.T[2 * tb + 2];
+        t1B = w.T[2 * tb + 3];
+    }
+    const double l1 = quot<FD>(a01, D0);
     const double tD1 = l1 * a01;
     const double tB1 = l1 * B0;
     // terms for the last member, in the reference's order
     double lD1 = tD1, lB1 = tB1, lD2 = 0.0, lB2 = 0.0;
-    int last = m1;
     if (three) {
         const double a02 = desc_double(u, 8), a12 = desc_double(u, 10);
         D1 -= tD1;
         B1 -= tB1;
         // member 1's feeder terms come after member 0's (deferred slot)
-        const int cnt = (u.a.x >> 8) & 15;
-        const int tb = u.a.z;
-        for (int t = 0; t < cnt; ++t) {
+        if (cnt > 0) {
+            D1 -= t0D;
+            B1 -= t0B;
+        }
+        if (cnt > 1) {
+            D1 -= t1D;
+            B1 -= t1B;
+        }
+        for (int t = 2; t < cnt; ++t) {
             D1 -= w.T[2 * (tb + t)];
             B1 -= w.T[2 * (tb + t) + 1];
         }
-        const double l2 = a02 / D0;
+        const double l2 = quot<FD>(a02, D0);
         const double tD2 = l2 * a02;
         const double tB2 = l2 * B0;
         const double a12u = a12 - l1 * a02;   // A[1][2] after member 0
         const double a12l = a12 - l2 * a01;   // A[2][1] after member 0
-        const double l = a12l / D1;
+        const double l = quot<FD>(a12l, D1);
         w.D[m1] = D1;
         w.B[m1] = B1;
         w.F[u.b.y] = a12u;
@@ -610,7 +658,6 @@ __device__ __forceinline__ void fwd_small(const Desc& u, Work w) {
         lB1 = tB2;
         lD2 = l * a12u;
         lB2 = l * B1;
-        last = m2;
     }
     if (deferLast >= 0) {
         w.T[2 * deferLast] = lD1;
@@ -620,14 +667,15 @@ __device__ __forceinline__ void fwd_small(const Desc& u, Work w) {
             w.T[2 * deferLast + 3] = lB2;
         }
     } else {
-        double Dl = three ? w.D[last] : D1;
-        double Bl = three ? w.B[last] : B1;
+        double Dl = three ? D2 : D1;
+        double Bl = three ? B2 : B1;
         Dl -= lD1;
         Bl -= lB1;
         if (three) {
             Dl -= lD2;
             Bl -= lB2;
         }
+        const int last = three ? m2 : m1;
         w.D[last] = Dl;
         w.B[last] = Bl;
     }
@@ -635,21 +683,29 @@ __device__ __forceinline__ void fwd_small(const Desc& u, Work w) {
 
 // Back-substitution of a unit of two or three members (HSolvePassive.cpp:
 // 344-369): VMid of the last member is known; B holds VMid once solved.
+// Every value the unit reads is read before it writes (one LDS round trip).
+template <bool FD = false>
 __device__ __forceinline__ void bwd_small(const Desc& u, Work w) {
     const bool three = (u.a.x & 15) == 3;
     const int m0 = u.a.w & 0xffff, m1 = (u.a.w >> 16) & 0xffff, m2 = u.b.x;
     const double a01 = desc_double(u, 6);
+    const double Bm0 = w.B[m0], Dm0 = w.D[m0], Bm1 = w.B[m1];
+    double v2 = 0.0, f = 0.0, Dm1 = 1.0;
+    if (three) {
+        v2 = w.B[m2];
+        f = w.F[u.b.y];
+        Dm1 = w.D[m1];
+    }
     double acc;
     if (three) {
         const double a02 = desc_double(u, 8);
-        const double v2 = w.B[m2];
-        const double v1 = (w.B[m1] - v2 * w.F[u.b.y]) / w.D[m1];
+        const double v1 = quot<FD>(Bm1 - v2 * f, Dm1);
         w.B[m1] = v1;
-        acc = (w.B[m0] - v2 * a02) - v1 * a01;
+        acc = (Bm0 - v2 * a02) - v1 * a01;
     } else {
-        acc = w.B[m0] - a01 * w.B[m1];
+        acc = Bm0 - a01 * Bm1;
     }
-    w.B[m0] = acc / w.D[m0];
+    w.B[m0] = quot<FD>(acc, Dm0);
 }
 
 // start of member p's block in a group's junction coefficients (reference
@@ -833,7 +889,7 @@ __device__ __noinline__ void bwd_group_regs(Desc u, Work w) {
     else bwd_group_desc<5>(u, w);
 }
 
-template <bool SLOWG>
+template <bool SLOWG, bool FD = false>
 __device__ __forceinline__ void forward_unit(char* arena, const CellDev& c, const Desc& u, Work w) {
     if constexpr (SLOWG) {
         if (u.a.x & 32) {
@@ -845,10 +901,10 @@ __device__ __forceinline__ void forward_unit(char* arena, const CellDev& c, cons
             return;
         }
     }
-    fwd_small(u, w);
+    fwd_small<FD>(u, w);
 }
 
-template <bool SLOWG>
+template <bool SLOWG, bool FD = false>
 __device__ __forceinline__ void backward_unit(char* arena, const CellDev& c, const Desc& u, Work w) {
     if constexpr (SLOWG) {
         if (u.a.x & 32) {
@@ -860,7 +916,24 @@ __device__ __forceinline__ void backward_unit(char* arena, const CellDev& c, con
             return;
         }
     }
-    bwd_small(u, w);
+    bwd_small<FD>(u, w);
+}
+
+// One level's units [s, e) in one direction: unit s + tid from `d` (loaded
+// ahead by the caller), later ones of a level wider than the workgroup as
+// they come.
+template <bool FWD, bool SLOWG, bool FD>
+__device__ __forceinline__ void sweep_level(char* arena, const CellDev& c, const int4* __restrict__ desc,
+                                            const Desc& d, int s, int e, Work w) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    if (s + tid < e) {
+        if constexpr (FWD) forward_unit<SLOWG, FD>(arena, c, d, w);
+        else backward_unit<SLOWG, FD>(arena, c, d, w);
+    }
+    for (int u = s + tid + nt; u < e; u += nt) {
+        if constexpr (FWD) forward_unit<SLOWG, FD>(arena, c, load_desc(desc, u), w);
+        else backward_unit<SLOWG, FD>(arena, c, load_desc(desc, u), w);
+    }
 }
 
 // CaConcStruct::process (hsolve/HSolveStruct.cpp:138-155)
@@ -1032,6 +1105,16 @@ __device__ __noinline__ double vclamp_tick(VClampDev& c, double vIn) {
 // sees that one's output as the reference's immediate message delivers it:
 // this tick's when the source is processed first, else the previous tick's
 // (its reinit baseLevel at the first tick).
+// The host's inputs of injection entry e (addInject since the last call),
+// taken on the call's first tick.
+__device__ __forceinline__ double inject_host(char* arena, const CellDev& c, int e, bool first) {
+    if (!first) return 0.0;
+    double* iv = A_<double>(arena, c.oInjVar);
+    const double var = iv[e];
+    iv[e] = 0.0;
+    return var;
+}
+
 __device__ __forceinline__ double inject_varying(char* arena, const CellDev& c, int e, bool first, double t,
                                                  double dt, int64_t stepNo) {
     double* iv = A_<double>(arena, c.oInjVar);
@@ -1141,16 +1224,19 @@ __device__ __forceinline__ GT power_code(GT x, int code) {
     return y * y;
 }
 
-template <typename GT, int MS>
-__device__ __forceinline__ void channel_row_fast(const View<GT>& v, const CellDev& c, int i, double dtg, double& Dout,
-                                                 double& Bout) {
+// V is the compartment's Vm, loaded by the caller (the fused kernel loads it
+// one compartment ahead, so that the table rows need no load of their own).
+// NOPOOL: the caller knows the cell has no Ca pools.
+template <typename GT, int MS, bool NOPOOL = false>
+__device__ __forceinline__ void channel_row_fast(const View<GT>& v, const CellDev& c, int i, const double V,
+                                                 double dtg, double& Dout, double& Bout) {
     const int n = v.n;
     const int S = v.nChanSlots;
+    const bool hasPools = !NOPOOL && v.hasPools;
     // ---- loads (absent entries read valid addresses and are not used)
-    const double V = v.V[i];
     const double Dbase = v.Dbase[i], CmByDt = v.CmByDt[i], EmByRm = v.EmByRm[i];
-    const int nPools = v.hasPools ? v.poolCount[i] : 0;
-    const double* __restrict__ caSrc = v.hasPools ? v.poolCa : v.V;
+    const int nPools = hasPools ? v.poolCount[i] : 0;
+    const double* __restrict__ caSrc = hasPools ? v.poolCa : v.V;
     const double* __restrict__ stSrc = c.nGateSlots > 0 ? v.state : v.V;
     double st[MS][3];
     GT gb[MS], ek[MS];
@@ -1169,7 +1255,7 @@ __device__ __forceinline__ void channel_row_fast(const View<GT>& v, const CellDe
         const bool hasZ = ((w2 >> 6) & 7) != 0;
         ext[k] = hasZ ? v.extCa[kk] : 0.0;
         const int caDep = ((w2 >> 12) & 15) - 1;
-        cav[k] = hasZ && caDep >= 0 ? caSrc[(size_t)(v.hasPools && caDep > 0 ? caDep : 0) * n + i] : 0.0;
+        cav[k] = hasZ && caDep >= 0 ? caSrc[(size_t)(hasPools && caDep > 0 ? caDep : 0) * n + i] : 0.0;
     }
     int vBase;
     double vFrac;
@@ -1225,7 +1311,7 @@ __device__ __forceinline__ void channel_row_fast(const View<GT>& v, const CellDe
             const GT fr = ax == 2 ? zf[k] : frac;
             const GT C1 = raw[k][ax][0] + (raw[k][ax][1] - raw[k][ax][0]) * fr;
             const GT C2 = raw[k][ax][2] + (raw[k][ax][3] - raw[k][ax][2]) * fr;
-            const double x = gate_step<double>(st[k][ax], C1, C2, dtg, (w2 >> (9 + ax)) & 1);
+            const double x = gate_step<std::is_same<GT, float>::value>(st[k][ax], C1, C2, dtg, (w2 >> (9 + ax)) & 1);
             st[k][ax] = x;
             fraction *= power_code<GT>((GT)x, code);
         }
@@ -1247,8 +1333,10 @@ __device__ __forceinline__ void channel_row_fast(const View<GT>& v, const CellDe
 }
 
 // FASTMS > 0: the cells are uniform with at most FASTMS channel slots and
-// phase A uses channel_row_fast.
-template <typename GT, bool SPLIT, bool WL, bool SLOWG, int FASTMS>
+// phase A uses channel_row_fast.  LEAN: the cells have no PulseGen, VClamp,
+// HHChannel2D, Leakage or SynChan objects and no Ca pools, so those paths
+// (and the registers they hold) are left out.
+template <typename GT, bool SPLIT, bool WL, bool SLOWG, int FASTMS, bool LEAN = false>
 __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c, double* lds, ChanType* sTypes) {
     char* arena = a.arena;
     const View<GT> v = make_view<GT>(arena, c, a.tsets);
@@ -1295,13 +1383,15 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
     int64_t* stepCounter = A_<int64_t>(arena, c.oSteps);
     const int64_t step0 = *stepCounter;
     const bool uniform = v.uniform;
+    // the mixed form divides with quot<true> in the elimination
+    constexpr bool FD = std::is_same<GT, float>::value;
     __syncthreads();
 
     for (int k = 0; k < a.nsteps; ++k) {
         const bool first = a.firstOfCall && k == 0;
         // first unit of level 0 for this thread, loaded while phase A runs
         Desc cur{};
-        if (nUnits > 0) cur = load_desc(desc, min(tid, nUnits - 1));
+        cur = load_desc(desc, min(tid, max(nUnits - 1, 0)));
         // ---- A: channels, currents, matrix rows
         if constexpr (SPLIT) {
             if constexpr (WL)
@@ -1310,8 +1400,15 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
                     w.B[i] = gwork[n + i];
                 }
         } else if constexpr (FASTMS > 0) {
+            // Vm one compartment ahead: the table rows of a compartment are
+            // known when its loads are issued, so they travel with them
+            double vNext = tid < n ? v.V[tid] : 0.0;
 #pragma unroll 1
-            for (int i = tid; i < n; i += nt) channel_row_fast<GT, FASTMS>(v, c, i, dtg, w.D[i], w.B[i]);
+            for (int i = tid; i < n; i += nt) {
+                const double Vi = vNext;
+                if (i + nt < n) vNext = v.V[i + nt];
+                channel_row_fast<GT, FASTMS, LEAN>(v, c, i, Vi, dtg, w.D[i], w.B[i]);
+            }
         } else {
             if (uniform)
                 for (int i = tid; i < n; i += nt) channel_row<GT, true>(v, types, i, dtg, w, false, i);
@@ -1319,10 +1416,10 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
                 for (int i = tid; i < n; i += nt) channel_row<GT, false>(v, types, i, dtg, w, false, i);
         }
         // (split mode: the channel kernel has added them to the rows)
-        if (!SPLIT && (nInj > 0 || (first && nExt > 0) || c.nXObj > 0)) {
+        if (!SPLIT && (nInj > 0 || (first && nExt > 0) || (!LEAN && c.nXObj > 0))) {
             // VClamps read V written by other threads in the last tick: a
             // full barrier then (its release covers those global stores)
-            if (c.nVClamp > 0) __syncthreads();
+            if (!LEAN && c.nVClamp > 0) __syncthreads();
             else work_barrier(WL);
             const int32_t* xflag = A_<const int32_t>(arena, c.oXFlag);
             // injections: HS[4i+3] += injectVarying + injectBasal
@@ -1334,7 +1431,8 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
                 const int64_t stepNo = step0 + k + 1;
                 const double t = a.dt * (double)(unsigned long long)stepNo;
                 for (int e = tid; e < nInj; e += nt) {
-                    const double var = inject_varying(arena, c, e, first, t, a.dt, stepNo);
+                    const double var = LEAN ? inject_host(arena, c, e, first)
+                                            : inject_varying(arena, c, e, first, t, a.dt, stepNo);
                     w.B[ii[e]] += var + ib[e];
                 }
             }
@@ -1346,12 +1444,12 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
                 const double* xg = A_<const double>(arena, c.oExtG);
                 const double* xge = A_<const double>(arena, c.oExtGE);
                 for (int e = tid; e < nExt; e += nt) {
-                    if (c.nXObj > 0 && xflag[xi[e]]) continue;   // in o2dExt
+                    if (!LEAN && c.nXObj > 0 && xflag[xi[e]]) continue;   // in o2dExt
                     w.D[xi[e]] += xg[e];
                     w.B[xi[e]] += xge[e];
                 }
             }
-            if (c.nXObj > 0) {
+            if (!LEAN && c.nXObj > 0) {
                 // compartments with objects sending every tick: the whole
                 // external current, after the injection
                 work_barrier(WL);
@@ -1366,58 +1464,63 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
         // full barrier: it also lands every outstanding global load, so the
         // level loops start from an empty queue
         __syncthreads();
-        // ---- F: forward elimination, level by level.  Two descriptor
-        // buffers alternate: while one level is processed from one, the next
-        // level's descriptors load into the other, so the loads stay in
-        // flight across the barrier (no register copy forces them to land
-        // before it).  Level bounds are read one level ahead.
+        // ---- F: forward elimination, level by level.  Three descriptor
+        // buffers rotate: the units of level l+2 load while level l is
+        // processed, so a descriptor has two levels' time to arrive (its
+        // loads stay in flight across the barriers; no register copy forces
+        // them to land early).  Level bounds are read a level ahead.
+        const int lastU = max(nUnits - 1, 0);
         {
-            int s0 = nLevels > 0 ? level[0] : 0;
-            int s1 = nLevels > 0 ? level[1] : 0;
-            Desc nxt{};
-            for (int l = 0; l < nLevels; l += 2) {
-                nxt = load_desc(desc, min(s1 + tid, nUnits - 1));
-                const int s2 = l + 2 <= nLevels ? level[l + 2] : s1;
-                if (s0 + tid < s1) forward_unit<SLOWG>(arena, c, cur, w);
-                for (int u = s0 + tid + nt; u < s1; u += nt) forward_unit<SLOWG>(arena, c, load_desc(desc, u), w);
+            int sA = nLevels > 0 ? level[0] : 0;
+            int sB = nLevels > 0 ? level[1] : 0;
+            Desc d0 = cur, d1 = load_desc(desc, min(sB + tid, lastU)), d2;
+            for (int l = 0; l < nLevels; l += 3) {
+                const int sC = l + 2 <= nLevels ? level[l + 2] : sB;
+                d2 = load_desc(desc, min(sC + tid, lastU));
+                sweep_level<true, SLOWG, FD>(arena, c, desc, d0, sA, sB, w);
                 work_barrier(WL);
                 if (l + 1 >= nLevels) break;
-                cur = load_desc(desc, min(s2 + tid, nUnits - 1));
-                const int s3 = l + 3 <= nLevels ? level[l + 3] : s2;
-                if (s1 + tid < s2) forward_unit<SLOWG>(arena, c, nxt, w);
-                for (int u = s1 + tid + nt; u < s2; u += nt) forward_unit<SLOWG>(arena, c, load_desc(desc, u), w);
+                const int sD = l + 3 <= nLevels ? level[l + 3] : sC;
+                d0 = load_desc(desc, min(sD + tid, lastU));
+                sweep_level<true, SLOWG, FD>(arena, c, desc, d1, sB, sC, w);
                 work_barrier(WL);
-                s0 = s2;
-                s1 = s3;
+                if (l + 2 >= nLevels) break;
+                const int sE = l + 4 <= nLevels ? level[l + 4] : sD;
+                d1 = load_desc(desc, min(sE + tid, lastU));
+                sweep_level<true, SLOWG, FD>(arena, c, desc, d2, sC, sD, w);
+                work_barrier(WL);
+                sA = sD;
+                sB = sE;
             }
         }
         // ---- B: back-substitution (VMid overwrites B), in reverse level
-        // order, with the same alternating prefetch
-        if (tid == 0) w.B[n - 1] = w.B[n - 1] / w.D[n - 1];
+        // order, with the same rotation
         {
-            int s1 = nLevels > 0 ? level[nLevels] : 0;
-            int s0 = nLevels > 0 ? level[nLevels - 1] : 0;
-            int p0 = nLevels > 1 ? level[nLevels - 2] : 0;
+            // level L-1 is [sA, eA), level L-2 [sB, sA)
+            int eA = nLevels > 0 ? level[nLevels] : 0;
+            int sA = nLevels > 0 ? level[nLevels - 1] : 0;
+            int sB = nLevels > 1 ? level[nLevels - 2] : 0;
+            Desc d0 = load_desc(desc, min(sA + tid, lastU)), d1 = load_desc(desc, min(sB + tid, lastU)), d2;
+            if (tid == 0) w.B[n - 1] = w.B[n - 1] / w.D[n - 1];
             __syncthreads();
-            if (nLevels > 0) cur = load_desc(desc, min(s0 + tid, nUnits - 1));
-            Desc nxt{};
-            for (int l = nLevels - 1; l >= 0; l -= 2) {
-                // level l from cur, level l-1 (units p0..s0) into nxt
-                nxt = load_desc(desc, min(p0 + tid, nUnits - 1));
-                const int pp = l >= 2 ? level[l - 2] : 0;
-                if (s0 + tid < s1) backward_unit<SLOWG>(arena, c, cur, w);
-                for (int u = s0 + tid + nt; u < s1; u += nt) backward_unit<SLOWG>(arena, c, load_desc(desc, u), w);
+            for (int l = nLevels - 1; l >= 0; l -= 3) {
+                const int sC = l >= 2 ? level[l - 2] : 0;
+                d2 = load_desc(desc, min(sC + tid, lastU));
+                sweep_level<false, SLOWG, FD>(arena, c, desc, d0, sA, eA, w);
                 work_barrier(WL);
                 if (l == 0) break;
-                // level l-1 from nxt, level l-2 (units pp..p0) into cur
-                cur = load_desc(desc, min(pp + tid, nUnits - 1));
-                const int ppp = l >= 3 ? level[l - 3] : 0;
-                if (p0 + tid < s0) backward_unit<SLOWG>(arena, c, nxt, w);
-                for (int u = p0 + tid + nt; u < s0; u += nt) backward_unit<SLOWG>(arena, c, load_desc(desc, u), w);
+                const int sD = l >= 3 ? level[l - 3] : 0;
+                d0 = load_desc(desc, min(sD + tid, lastU));
+                sweep_level<false, SLOWG, FD>(arena, c, desc, d1, sB, sA, w);
                 work_barrier(WL);
-                s1 = p0;
-                s0 = pp;
-                p0 = ppp;
+                if (l == 1) break;
+                const int sE = l >= 4 ? level[l - 4] : 0;
+                d1 = load_desc(desc, min(sE + tid, lastU));
+                sweep_level<false, SLOWG, FD>(arena, c, desc, d2, sC, sB, w);
+                work_barrier(WL);
+                eA = sC;
+                sA = sD;
+                sB = sE;
             }
         }
         // ---- D: Vm, calcium
@@ -1426,7 +1529,7 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
             if constexpr (WL)
                 for (int i = tid; i < n; i += nt) gwork[n + i] = w.B[i];
         } else {
-            if (v.hasPools)
+            if (!LEAN && v.hasPools)
                 for (int i = tid; i < n; i += nt) {
                     const double vmid = w.B[i];
                     const double vnew = 2 * vmid - v.V[i];
@@ -1434,7 +1537,16 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
                     calcium_row<GT>(arena, c, v, types, i, vmid, vnew, first);
                 }
             else
-                for (int i = tid; i < n; i += nt) v.V[i] = 2 * w.B[i] - v.V[i];
+                // Vm = 2 VMid - Vm; the old Vm of four compartments loaded
+                // together (clamped loads: every lane issues the same loads)
+                for (int i = tid; i < n; i += 4 * nt) {
+                    double vo[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) vo[u] = v.V[min(i + u * nt, n - 1)];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (i + u * nt < n) v.V[i + u * nt] = 2 * w.B[i + u * nt] - vo[u];
+                }
             const int64_t stepNo = step0 + k + 1;
             if (nSpike > 0 || nProbe > 0) {
                 __syncthreads();   // V written by other threads
@@ -1448,10 +1560,11 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
     if (tid == 0) *stepCounter = step0 + a.nsteps;
 }
 
-// The fused kernel.  Its register budget is that of five waves per SIMD
-// (measured: the occupancy gained outweighs the few spills it costs).
+// The fused kernel.  Its register budget is that of four waves per SIMD
+// (128 VGPRs: the three rotating descriptor buffers of the level sweeps fit
+// with few spills).
 template <typename GT>
-__global__ void __launch_bounds__(256, 5) hsolve_step_kernel(LaunchArgs a) {
+__global__ void __launch_bounds__(256, 4) hsolve_step_kernel(LaunchArgs a) {
     extern __shared__ double lds[];
     __shared__ ChanType sTypes[kMaxSlots];
     const CellDev& c = a.cells[a.order[blockIdx.x]];
@@ -1485,6 +1598,17 @@ __global__ void __launch_bounds__(256, 3) hsolve_step_fast_kernel(LaunchArgs a) 
     const CellDev& c = a.cells[a.order[blockIdx.x]];
     if (c.workInLds) cell_steps<GT, false, true, false, 2>(a, c, lds, sTypes);
     else cell_steps<GT, false, false, false, 2>(a, c, lds, sTypes);
+}
+
+// The same for cells with nothing beside the solver but host inputs
+// (no stimulus objects, external channel objects or Ca pools).
+template <typename GT>
+__global__ void __launch_bounds__(256, 3) hsolve_step_lean_kernel(LaunchArgs a) {
+    extern __shared__ double lds[];
+    __shared__ ChanType sTypes[kMaxSlots];
+    const CellDev& c = a.cells[a.order[blockIdx.x]];
+    if (c.workInLds) cell_steps<GT, false, true, false, 2, true>(a, c, lds, sTypes);
+    else cell_steps<GT, false, false, false, 2, true>(a, c, lds, sTypes);
 }
 
 // The solve half of a tick when the channel phase runs as its own kernel.
@@ -1844,7 +1968,7 @@ __global__ void __launch_bounds__(256) hsolve_channel_fast_kernel(LaunchArgs a, 
     const int i = tile.y + threadIdx.x;
     if (i >= v.n) return;
     double D, B;
-    channel_row_fast<GT, MS>(v, c, i, a.dt, D, B);
+    channel_row_fast<GT, MS>(v, c, i, v.V[i], a.dt, D, B);
     row_inputs(a, c, i, D, B);
     double *Dp, *Bp;
     row_dest(a.arena, c, i, Dp, Bp);
@@ -2249,10 +2373,13 @@ extern "C" hipError_t hsg_scratch_address(int device, void** ptr, long long* byt
 
 // host-side launchers (called from hsolve_engine.cpp)
 extern "C" hipError_t hsg_launch_step(const hsg::LaunchArgs* a, int nblocks, int threads, int fp32, int slow,
-                                     int fastms, hipStream_t stream) {
+                                     int fastms, int lean, hipStream_t stream) {
     const dim3 g(nblocks), b(threads);
     const size_t l = a->ldsBytes;
-    if (fastms == 2 && !slow) {
+    if (fastms == 2 && !slow && lean) {
+        if (fp32) hipLaunchKernelGGL((hsg::hsolve_step_lean_kernel<float>), g, b, l, stream, *a);
+        else hipLaunchKernelGGL((hsg::hsolve_step_lean_kernel<double>), g, b, l, stream, *a);
+    } else if (fastms == 2 && !slow) {
         if (fp32) hipLaunchKernelGGL((hsg::hsolve_step_fast_kernel<float>), g, b, l, stream, *a);
         else hipLaunchKernelGGL((hsg::hsolve_step_fast_kernel<double>), g, b, l, stream, *a);
     } else if (fp32 && slow) hipLaunchKernelGGL((hsg::hsolve_step_slow_kernel<float>), g, b, l, stream, *a);
@@ -2372,8 +2499,9 @@ extern "C" hipError_t hsg_launch_top(const hsg::LaunchArgs* a, int nblocks, int 
 }
 
 extern "C" hipError_t hsg_set_max_lds(int bytes) {
-    const void* kernels[18] = {
+    const void* kernels[20] = {
         (const void*)hsg::hsolve_step_kernel<double>, (const void*)hsg::hsolve_step_kernel<float>,
+        (const void*)hsg::hsolve_step_lean_kernel<double>, (const void*)hsg::hsolve_step_lean_kernel<float>,
         (const void*)hsg::hsolve_step_slow_kernel<double>, (const void*)hsg::hsolve_step_slow_kernel<float>,
         (const void*)hsg::hsolve_step_fast_kernel<double>, (const void*)hsg::hsolve_step_fast_kernel<float>,
         (const void*)hsg::hsolve_solve_kernel<double, false>, (const void*)hsg::hsolve_solve_kernel<float, false>,

@@ -273,9 +273,23 @@ static int schedule(CellSetup& s, const std::vector<int>& junctionGroup, std::st
         const int sz = (int)s.groups[-code - 1].mem.size();
         return sz > kFastGroupMax || n > 65535;
     };
+    // within a level (its units are independent): the fast units first, and
+    // among them the three-member units, then the two-member ones, each with
+    // the units that defer their last member's terms first -- so that the
+    // lanes of a wave mostly take the same path through the unit code
+    auto unit_class = [&](int code) {
+        if (is_slow(code)) return 4;
+        int sz = 2, dl = -1;
+        if (code >= 0) dl = s.defer[2 * (code + 1)];
+        else {
+            sz = (int)s.groups[-code - 1].mem.size();
+            dl = s.groups[-code - 1].deferLast;
+        }
+        return (sz == 3 ? 0 : 2) + (dl >= 0 ? 0 : 1);
+    };
     for (int l = 0; l + 1 < (int)s.levelStart.size(); ++l)
-        std::stable_partition(s.units.begin() + s.levelStart[l], s.units.begin() + s.levelStart[l + 1],
-                              [&](int code) { return !is_slow(code); });
+        std::stable_sort(s.units.begin() + s.levelStart[l], s.units.begin() + s.levelStart[l + 1],
+                         [&](int a, int b) { return unit_class(a) < unit_class(b); });
     for (size_t k = 0; k < s.units.size(); ++k) {
         int32_t* d = &s.udesc[kDescInts * k];
         const int code = s.units[k];
