@@ -4,7 +4,7 @@
 own clock (HIP events around the launches, hsg_last_step_timing):
 
   threads     threads per cell workgroup of the bin's per-cell kernel
-              (64 / 128 / 256; the block-form split solve has a fixed shape)
+              (64 / 128 / 192 / 256; the block-form split solve has a fixed shape)
   fast_pass   the fast channel pass (all loads of a compartment first) or
               the generic one, where the bin's cells allow both
 
@@ -30,7 +30,7 @@ DEFAULT_SPACE = {
     "steps_per_launch": (50,),
 }
 
-THREADS = (64, 128, 256)
+THREADS = (64, 128, 192, 256)
 
 
 def _ms_per_step(eng, steps, repeats):

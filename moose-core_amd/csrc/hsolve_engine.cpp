@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <set>
 #include <array>
 #include <cmath>
 #include <cstdio>
@@ -1120,7 +1121,21 @@ void fill_fast_slots(hsg_engine* e, CellDev& d) {
 }
 
 // Threads per cell workgroup for a bin whose largest cell has n compartments.
-int default_threads(int n) { return n >= 1024 ? 256 : n >= 256 ? 128 : 64; }
+// LDS classes of the per-cell bins: a class's cells need at most
+// kCuShare[k-1] and more than kCuShare[k] bytes (3, 4, 5, 6, 8 and 10 per CU)
+constexpr int kLdsClasses = 5;
+constexpr int64_t kCuShare[kLdsClasses] = {40960, 32768, 27306, 20480, 16384};
+
+// Threads per cell workgroup: by cell size, and no more than the waves per
+// CU of the fused kernels (12: three per SIMD) leave each of the
+// workgroups that the bin's LDS lets reside on a CU.
+int default_threads(int n, int lds) {
+    const int byN = n >= 1024 ? 256 : n >= 256 ? 128 : 64;
+    if (lds <= 0) return byN;
+    const int resident = std::max(1, (160 * 1024) / lds);
+    const int waves = std::max(1, 12 / resident);
+    return std::min(byN, std::min(4, waves) * 64);
+}
 
 // Splits cells (largest first) into bins by kernel form and LDS need; each
 // bin is one launch (per round or per tick) on its own stream, with its own
@@ -1216,14 +1231,27 @@ void plan_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg
     // and, in automatic mode, large cells without the message-passing form
     // in split bins of their own (16-31)
     const bool autoMode = e->opt.kernel_mode == HSG_KERNEL_AUTO;
+    // the fine LDS classes unless they would need more bins than the engine
+    // has streams (pairs double a split bin): then one class per kind
+    bool fine = true;
     auto cls = [&](int id) {
         const CellDev& d = e->cells[id]->dev;
         const int s = (d.slowHJ ? 8 : 0) + (autoMode && !d.msg && d.n >= kAutoSplitMin ? 16 : 0);
         if (d.n2d > 0) return s + (d.msg ? 6 : 7);
         if (d.msg) return s + (d.n >= 1024 ? 0 : d.n >= 256 ? 1 : 2);
+        // the rest by how many of their workgroups' LDS fit a CU's 160 KiB:
+        // three (more than 40 KiB), four, five, six, eight, ten or more (a
+        // bin's dynamic LDS is its largest cell's: fine classes waste little)
         const int64_t wk = work_lds_bytes(d) + level_bytes(d);
-        return s + 3 + (wk > 32768 ? 0 : wk > 16384 ? 1 : 2);
+        int sub = 0;
+        while (fine && sub < kLdsClasses && wk <= kCuShare[sub]) ++sub;
+        return s + 3 + (sub << 8);
     };
+    {
+        std::set<int> kinds;
+        for (int id : cells) kinds.insert(cls(id));
+        if (2 * kinds.size() > (size_t)hsg_engine::kMaxBins) fine = false;
+    }
     std::vector<int32_t> sorted(cells);
     std::stable_sort(sorted.begin(), sorted.end(), [&](int a, int b) {
         const int ca = cls(a), cb = cls(b);
@@ -1281,7 +1309,7 @@ void plan_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg
         // the per-cell kernels are built for at most 256 threads per
         // workgroup (wider ones measured no faster, 512 slower)
         b.threads = e->opt.block_threads > 0 ? std::max(64, std::min(256, (e->opt.block_threads + 63) / 64 * 64))
-                                             : default_threads(maxN);
+                                             : default_threads(maxN, b.split ? 0 : b.lds);
         // a shape the caller chose for this bin class (it measured it)
         const auto sh = e->shapes.find(b.key);
         if (sh != e->shapes.end()) {

@@ -1148,53 +1148,84 @@ __device__ __forceinline__ double inject_varying(char* arena, const CellDev& c, 
     return var;
 }
 
+// The probe schedule of one launch, worked out once at its start: probes
+// sample every probeStride-th step (a Table on a slower clock tick,
+// Table::process after the solver's tick), sample 0 being the first such
+// step after the trace's base step.  `next` is the next sampled step and
+// `sample` its index in the trace.
+struct ProbeClock {
+    int64_t next, stride, sample;
+};
+
+__device__ __forceinline__ ProbeClock probe_clock(char* arena, const CellDev& c, int64_t step0) {
+    ProbeClock p{-1, 1, 0};
+    if (c.nProbe <= 0) return p;
+    const int64_t base = *A_<const int64_t>(arena, c.oProbeBase);
+    p.stride = c.probeStride;
+    p.next = (step0 / p.stride + 1) * p.stride;   // the first multiple of the stride after step0
+    p.sample = p.next / p.stride - base / p.stride - 1;
+    return p;
+}
+
+// SpikeGen::process at t = dt * step for generator sg, given Vm
+// (biophysics/SpikeGen.cpp:202-216): its last spike time and fired flag are
+// read and written by the caller.
+__device__ __forceinline__ void spike_rule(const LaunchArgs& a, const CellDev& c, int sg, double Vm, double thr,
+                                          double refr, int edge, double& last, int& fired, int64_t stepNo) {
+    if (Vm > thr) {
+        const double t = a.dt * (double)(unsigned long long)stepNo;
+        if ((t + a.dt / 2.0) >= (last + refr)) {
+            if (!(edge && fired)) {
+                const unsigned long long slot = atomicAdd(a.spikeCount, 1ULL);
+                if (slot < (unsigned long long)a.spikeCap) {
+                    SpikeRec r;
+                    r.cell = c.cellId;
+                    r.gen = sg;
+                    r.step = stepNo;
+                    a.spikes[slot] = r;
+                }
+                last = t;
+                fired = 1;
+            }
+        }
+    } else {
+        fired = 0;
+    }
+}
+
 // Per-tick outputs once V of the tick is complete: spike generators
 // (SpikeGen::process at t = dt * step, biophysics/SpikeGen.cpp:202-216, fed
 // by HSolveActive::sendSpikes, hsolve/HSolveActive.cpp:342-347) into the
 // engine's spike ring, and Vm probes (Table sampling after the solver's tick).
 __device__ __forceinline__ void tick_outputs(const LaunchArgs& a, const CellDev& c, const double* __restrict__ V,
-                                             int64_t stepNo) {
+                                             int64_t stepNo, ProbeClock& pc) {
     char* arena = a.arena;
     const int tid = threadIdx.x, nt = blockDim.x;
     for (int sg = tid; sg < c.nSpike; sg += nt) {
         const int comp = A_<const int32_t>(arena, c.oSpkCompt)[sg];
         const double Vm = V[comp];
         const double thr = A_<const double>(arena, c.oSpkThr)[sg];
-        int32_t* fired = A_<int32_t>(arena, c.oSpkFired);
-        if (Vm > thr) {
-            const double t = a.dt * (double)(unsigned long long)stepNo;
-            double* last = A_<double>(arena, c.oSpkLast);
-            const double refr = A_<const double>(arena, c.oSpkRefr)[sg];
-            const int edge = A_<const int32_t>(arena, c.oSpkEdge)[sg];
-            if ((t + a.dt / 2.0) >= (last[sg] + refr)) {
-                if (!(edge && fired[sg])) {
-                    const unsigned long long slot = atomicAdd(a.spikeCount, 1ULL);
-                    if (slot < (unsigned long long)a.spikeCap) {
-                        SpikeRec r;
-                        r.cell = c.cellId;
-                        r.gen = sg;
-                        r.step = stepNo;
-                        a.spikes[slot] = r;
-                    }
-                    last[sg] = t;
-                    fired[sg] = 1;
-                }
-            }
-        } else {
-            fired[sg] = 0;
-        }
+        int32_t* firedP = A_<int32_t>(arena, c.oSpkFired);
+        double* lastP = A_<double>(arena, c.oSpkLast);
+        int fired = firedP[sg];
+        double last = lastP[sg];
+        const int fired0 = fired;
+        const double last0 = last;
+        spike_rule(a, c, sg, Vm, thr, A_<const double>(arena, c.oSpkRefr)[sg], A_<const int32_t>(arena, c.oSpkEdge)[sg],
+                   last, fired, stepNo);
+        if (fired != fired0) firedP[sg] = fired;
+        if (last != last0) lastP[sg] = last;
     }
-    // probes sample every probeStride-th step (a Table on a slower clock
-    // tick, Table::process after the solver's tick), sample 0 being the
-    // first such step after the trace's base step
-    if (c.nProbe > 0 && stepNo % c.probeStride == 0) {
-        const int64_t base = *A_<const int64_t>(arena, c.oProbeBase);
-        const int64_t sample = stepNo / c.probeStride - base / c.probeStride - 1;
+    // probes (ProbeClock)
+    if (c.nProbe > 0 && stepNo == pc.next) {
+        const int64_t sample = pc.sample;
         if (sample >= 0 && sample < c.probeCap) {
             const int64_t* poff = A_<const int64_t>(arena, c.oProbeOff);
             double* tr = A_<double>(arena, c.oProbeTrace);
             for (int j = tid; j < c.nProbe; j += nt) tr[sample * c.nProbe + j] = *A_<const double>(arena, poff[j]);
         }
+        pc.next += pc.stride;
+        ++pc.sample;
     }
 }
 
@@ -1385,6 +1416,7 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
     const bool uniform = v.uniform;
     // the mixed form divides with quot<true> in the elimination
     constexpr bool FD = std::is_same<GT, float>::value;
+    ProbeClock pc = probe_clock(arena, c, step0);
     __syncthreads();
 
     for (int k = 0; k < a.nsteps; ++k) {
@@ -1529,6 +1561,7 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
             if constexpr (WL)
                 for (int i = tid; i < n; i += nt) gwork[n + i] = w.B[i];
         } else {
+            const int64_t stepNo = step0 + k + 1;
             if (!LEAN && v.hasPools)
                 for (int i = tid; i < n; i += nt) {
                     const double vmid = w.B[i];
@@ -1547,10 +1580,9 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
                     for (int u = 0; u < 4; ++u)
                         if (i + u * nt < n) v.V[i + u * nt] = 2 * w.B[i + u * nt] - vo[u];
                 }
-            const int64_t stepNo = step0 + k + 1;
             if (nSpike > 0 || nProbe > 0) {
                 __syncthreads();   // V written by other threads
-                tick_outputs(a, c, v.V, stepNo);
+                tick_outputs(a, c, v.V, stepNo, pc);
             }
         }
         // the next step's phase A overwrites D/B that other threads may still
@@ -2123,6 +2155,7 @@ __global__ void __launch_bounds__(256) hsolve_msg_kernel(LaunchArgs a) {
     double* VMid = A_<double>(arena, c.oVMid);
     int64_t* stepCounter = A_<int64_t>(arena, c.oSteps);
     const int64_t step0 = *stepCounter;
+    ProbeClock pc = probe_clock(arena, c, step0);
     const int last = nUnits - 1;
     __syncthreads();
 
@@ -2227,7 +2260,7 @@ __global__ void __launch_bounds__(256) hsolve_msg_kernel(LaunchArgs a) {
         const int64_t stepNo = step0 + k + 1;
         if (!a.deferD && (c.nSpike > 0 || c.nProbe > 0)) {
             __syncthreads();   // V written by other threads
-            tick_outputs(a, c, V, stepNo);
+            tick_outputs(a, c, V, stepNo, pc);
         }
         __syncthreads();
     }
@@ -2266,7 +2299,8 @@ __global__ void __launch_bounds__(256) hsolve_outputs_kernel(LaunchArgs a) {
     const CellDev& c = a.cells[a.order[blockIdx.x]];
     if (c.nSpike == 0 && c.nProbe == 0) return;
     const int64_t stepNo = *A_<const int64_t>(a.arena, c.oSteps);
-    tick_outputs(a, c, A_<const double>(a.arena, c.oV), stepNo);
+    ProbeClock pc = probe_clock(a.arena, c, stepNo - 1);
+    tick_outputs(a, c, A_<const double>(a.arena, c.oV), stepNo, pc);
 }
 
 // HSolveActive::reinitChannels (hsolve/HSolveActiveSetup.cpp:79-160): gates at
