@@ -285,7 +285,8 @@ struct hsg_engine {
         int topLds = 0;             // dynamic LDS of the top kernel
         int key = 0;                // the bin's class (second half of a pair: + 32)
         int fastSel = -1;           // fast channel pass: -1 the engine's choice, 0 off, 1 on
-        bool lean = false;          // no stimulus or external objects and no Ca pools in any cell
+        int lean = 0;               // no stimulus or external objects and no Ca pools in any cell: 1,
+                                    // and every cell's channels the squid HH pair: 2
     };
     // Launch shapes set by the caller per bin class (hsg_set_bin_shape):
     // threads per cell workgroup (0: the engine's choice) and the fast
@@ -1150,6 +1151,7 @@ void free_plan(hsg_engine::Plan& plan) {
 // The bins of `cells` (host side only: cell lists and launch shapes; no
 // device buffer is touched).
 void plan_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg_engine::Bin>& bins);
+bool hh_pair(const CellDev& d);
 
 // Bytes of the device tables of a plan's bins: per bin its cell order, then
 // its channel tiles, each 256-byte aligned.
@@ -1294,15 +1296,17 @@ void plan_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg
     for (auto& b : bins) {
         int maxN = 0, maxSlots = 0;
         bool allUniform = true;
-        b.lean = true;
+        bool lean = true, hh = true;
         for (int id : b.cells) {
             const CellDev& d = e->cells[id]->dev;
             b.lds = std::max<int>(b.lds, d.ldsBytes);
             maxN = std::max(maxN, d.n);
             maxSlots = std::max(maxSlots, d.nChanSlots);
             allUniform = allUniform && d.fastOk;
-            b.lean = b.lean && d.nPulse == 0 && d.nVClamp == 0 && d.nXObj == 0 && !d.hasPools;
+            lean = lean && d.nPulse == 0 && d.nVClamp == 0 && d.nXObj == 0 && !d.hasPools;
+            hh = hh && hh_pair(d);
         }
+        b.lean = lean ? (hh ? 2 : 1) : 0;
         b.chanMS = allUniform ? (maxSlots <= 2 ? 2 : maxSlots <= 4 ? 4 : 8) : 0;
         b.sumN = 0;
         for (int id : b.cells) b.sumN += e->cells[id]->dev.n;
@@ -1548,6 +1552,13 @@ int64_t routed_stretch(hsg_engine* e) {
 #ifndef HSG_FAST_FUSED_MIN
 #define HSG_FAST_FUSED_MIN 6
 #endif
+// The squid HH pair on the fast pass: two live slots, m^3 h then n^4 (no Z
+// gate, no instant gate), without modulation (the kernels' kSigHH).
+bool hh_pair(const CellDev& d) {
+    return d.fastOk && d.nChanSlots == 2 && !d.hasModulation && d.fast[0][1] < 0 && d.fast[1][1] < 0 &&
+           d.fast[0][2] == (3 | 1 << 3) && d.fast[1][2] == 4;
+}
+
 int fused_fast(const hsg_engine::Bin& b) {
     if (b.chanMS != 2 || b.slow || b.cells.empty() || b.fastSel == 0) return 0;
     if (b.fastSel == 1) return 2;
@@ -2984,9 +2995,10 @@ int hsg_plan_info(hsg_engine* e, hsg_bin_info* out, int32_t cap, int32_t* n_out)
             snprintf(o.kernel, sizeof o.kernel, "%s<%s>", x.msg ? "hsolve_msg_kernel"
                      : x.solveThreads > 256 ? "hsolve_solve_wide_kernel" : "hsolve_solve_kernel", prec);
         else
-            snprintf(o.kernel, sizeof o.kernel, "%s<%s>", x.slow ? "hsolve_step_slow_kernel"
+            snprintf(o.kernel, sizeof o.kernel, "%s<%s%s>", x.slow ? "hsolve_step_slow_kernel"
                      : fused_fast(x) ? (x.lean && !x.hasExt ? "hsolve_step_lean_kernel" : "hsolve_step_fast_kernel")
-                     : "hsolve_step_kernel", prec);
+                     : "hsolve_step_kernel", prec,
+                     !x.slow && fused_fast(x) && x.lean && !x.hasExt ? (x.lean == 2 ? ", kSigHH" : ", 0") : "");
     }
     return 0;
 }

@@ -1257,12 +1257,23 @@ __device__ __forceinline__ GT power_code(GT x, int code) {
 
 // V is the compartment's Vm, loaded by the caller (the fused kernel loads it
 // one compartment ahead, so that the table rows need no load of their own).
-// NOPOOL: the caller knows the cell has no Ca pools.
-template <typename GT, int MS, bool NOPOOL = false>
+// NOPOOL: the caller knows the cell has no Ca pools.  SIG = kSigHH: the caller
+// knows the cell's two slots are the squid HH pair (m^3 h, then n^4; no Z
+// gate, no instant gate, no modulation), so the gate codes are constants and
+// every branch on them folds away; only the table columns and state slots
+// are read from the cell.
+constexpr int kSigHH = 1;
+constexpr int kSigHHWord[2] = {3 | 1 << 3, 4};
+
+template <typename GT, int MS, bool NOPOOL = false, int SIG = 0>
 __device__ __forceinline__ void channel_row_fast(const View<GT>& v, const CellDev& c, int i, const double V,
                                                  double dtg, double& Dout, double& Bout) {
+    static_assert(SIG == 0 || MS == 2, "the HH signature has two slots");
     const int n = v.n;
-    const int S = v.nChanSlots;
+    const int S = SIG == kSigHH ? 2 : v.nChanSlots;
+    // gate codes of slot k (fast word 2) and whether the slot is live
+    auto word2 = [&](int k) { return SIG == kSigHH ? kSigHHWord[k] : c.fast[k][2]; };
+    auto live = [&](int k) { return SIG == kSigHH ? true : (k < S && c.fast[k][1] < 0); };
     const bool hasPools = !NOPOOL && v.hasPools;
     // ---- loads (absent entries read valid addresses and are not used)
     const double Dbase = v.Dbase[i], CmByDt = v.CmByDt[i], EmByRm = v.EmByRm[i];
@@ -1274,14 +1285,15 @@ __device__ __forceinline__ void channel_row_fast(const View<GT>& v, const CellDe
     double md[MS], cav[MS], ext[MS];
 #pragma unroll
     for (int k = 0; k < MS; ++k) {
-        const int w1 = c.fast[k][1], w2 = c.fast[k][2];
+        const int w1 = c.fast[k][1], w2 = word2(k);
         const size_t kk = (size_t)(k < S ? k : S - 1) * n + i;
-        st[k][0] = stSrc[(size_t)((w1 >> 16) & 31) * n + i];
-        st[k][1] = stSrc[(size_t)((w1 >> 21) & 31) * n + i];
-        st[k][2] = stSrc[(size_t)((w1 >> 26) & 31) * n + i];
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax)
+            st[k][ax] = SIG != kSigHH || ((w2 >> (3 * ax)) & 7) != 0 ? stSrc[(size_t)((w1 >> (16 + 5 * ax)) & 31) * n + i]
+                                                                    : 0.0;
         gb[k] = v.Gbar[kk];
         ek[k] = v.Ek[kk];
-        md[k] = v.hasModulation ? v.mod[kk] : 1.0;
+        md[k] = SIG != kSigHH && v.hasModulation ? v.mod[kk] : 1.0;
         // Ca inputs only where a Z gate may read them (uniform condition)
         const bool hasZ = ((w2 >> 6) & 7) != 0;
         ext[k] = hasZ ? v.extCa[kk] : 0.0;
@@ -1300,8 +1312,8 @@ __device__ __forceinline__ void channel_row_fast(const View<GT>& v, const CellDe
     GT zf[MS];
 #pragma unroll
     for (int k = 0; k < MS; ++k) {
-        const int w0 = c.fast[k][0], w1 = c.fast[k][1], w2 = c.fast[k][2];
-        if (k >= S || w1 >= 0) continue;          // empty slot (live bit 31 clear)
+        const int w0 = c.fast[k][0], w1 = c.fast[k][1], w2 = word2(k);
+        if (!live(k)) continue;                   // empty slot (live bit 31 clear)
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax) {
             if (((w2 >> (3 * ax)) & 7) == 0) continue;
@@ -1331,9 +1343,9 @@ __device__ __forceinline__ void channel_row_fast(const View<GT>& v, const CellDe
     double GkSum = 0.0, GkEkSum = 0.0;
 #pragma unroll
     for (int k = 0; k < MS; ++k) {
-        const int w1 = c.fast[k][1], w2 = c.fast[k][2];
-        if (k >= S || w1 >= 0) continue;
-        GT fraction = v.hasModulation ? (GT)md[k] : (GT)1.0;
+        const int w2 = word2(k);
+        if (!live(k)) continue;
+        GT fraction = SIG != kSigHH && v.hasModulation ? (GT)md[k] : (GT)1.0;
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax) {
             const int code = (w2 >> (3 * ax)) & 7;
@@ -1353,8 +1365,8 @@ __device__ __forceinline__ void channel_row_fast(const View<GT>& v, const CellDe
     // ---- stores
 #pragma unroll
     for (int k = 0; k < MS; ++k) {
-        const int w1 = c.fast[k][1], w2 = c.fast[k][2];
-        if (k >= S || w1 >= 0) continue;
+        const int w1 = c.fast[k][1], w2 = word2(k);
+        if (!live(k)) continue;
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax)
             if (((w2 >> (3 * ax)) & 7) != 0) v.state[(size_t)((w1 >> (16 + 5 * ax)) & 31) * n + i] = st[k][ax];
@@ -1367,7 +1379,7 @@ __device__ __forceinline__ void channel_row_fast(const View<GT>& v, const CellDe
 // phase A uses channel_row_fast.  LEAN: the cells have no PulseGen, VClamp,
 // HHChannel2D, Leakage or SynChan objects and no Ca pools, so those paths
 // (and the registers they hold) are left out.
-template <typename GT, bool SPLIT, bool WL, bool SLOWG, int FASTMS, bool LEAN = false>
+template <typename GT, bool SPLIT, bool WL, bool SLOWG, int FASTMS, bool LEAN = false, int SIG = 0>
 __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c, double* lds, ChanType* sTypes) {
     char* arena = a.arena;
     const View<GT> v = make_view<GT>(arena, c, a.tsets);
@@ -1439,7 +1451,7 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
             for (int i = tid; i < n; i += nt) {
                 const double Vi = vNext;
                 if (i + nt < n) vNext = v.V[i + nt];
-                channel_row_fast<GT, FASTMS, LEAN>(v, c, i, Vi, dtg, w.D[i], w.B[i]);
+                channel_row_fast<GT, FASTMS, LEAN, SIG>(v, c, i, Vi, dtg, w.D[i], w.B[i]);
             }
         } else {
             if (uniform)
@@ -1633,14 +1645,15 @@ __global__ void __launch_bounds__(256, 3) hsolve_step_fast_kernel(LaunchArgs a) 
 }
 
 // The same for cells with nothing beside the solver but host inputs
-// (no stimulus objects, external channel objects or Ca pools).
-template <typename GT>
+// (no stimulus objects, external channel objects or Ca pools); SIG = kSigHH
+// for cells whose two channel slots are the squid HH pair.
+template <typename GT, int SIG>
 __global__ void __launch_bounds__(256, 3) hsolve_step_lean_kernel(LaunchArgs a) {
     extern __shared__ double lds[];
     __shared__ ChanType sTypes[kMaxSlots];
     const CellDev& c = a.cells[a.order[blockIdx.x]];
-    if (c.workInLds) cell_steps<GT, false, true, false, 2, true>(a, c, lds, sTypes);
-    else cell_steps<GT, false, false, false, 2, true>(a, c, lds, sTypes);
+    if (c.workInLds) cell_steps<GT, false, true, false, 2, true, SIG>(a, c, lds, sTypes);
+    else cell_steps<GT, false, false, false, 2, true, SIG>(a, c, lds, sTypes);
 }
 
 // The solve half of a tick when the channel phase runs as its own kernel.
@@ -2410,9 +2423,12 @@ extern "C" hipError_t hsg_launch_step(const hsg::LaunchArgs* a, int nblocks, int
                                      int fastms, int lean, hipStream_t stream) {
     const dim3 g(nblocks), b(threads);
     const size_t l = a->ldsBytes;
-    if (fastms == 2 && !slow && lean) {
-        if (fp32) hipLaunchKernelGGL((hsg::hsolve_step_lean_kernel<float>), g, b, l, stream, *a);
-        else hipLaunchKernelGGL((hsg::hsolve_step_lean_kernel<double>), g, b, l, stream, *a);
+    if (fastms == 2 && !slow && lean == 2) {
+        if (fp32) hipLaunchKernelGGL((hsg::hsolve_step_lean_kernel<float, hsg::kSigHH>), g, b, l, stream, *a);
+        else hipLaunchKernelGGL((hsg::hsolve_step_lean_kernel<double, hsg::kSigHH>), g, b, l, stream, *a);
+    } else if (fastms == 2 && !slow && lean) {
+        if (fp32) hipLaunchKernelGGL((hsg::hsolve_step_lean_kernel<float, 0>), g, b, l, stream, *a);
+        else hipLaunchKernelGGL((hsg::hsolve_step_lean_kernel<double, 0>), g, b, l, stream, *a);
     } else if (fastms == 2 && !slow) {
         if (fp32) hipLaunchKernelGGL((hsg::hsolve_step_fast_kernel<float>), g, b, l, stream, *a);
         else hipLaunchKernelGGL((hsg::hsolve_step_fast_kernel<double>), g, b, l, stream, *a);
@@ -2533,9 +2549,11 @@ extern "C" hipError_t hsg_launch_top(const hsg::LaunchArgs* a, int nblocks, int 
 }
 
 extern "C" hipError_t hsg_set_max_lds(int bytes) {
-    const void* kernels[20] = {
+    const void* kernels[22] = {
         (const void*)hsg::hsolve_step_kernel<double>, (const void*)hsg::hsolve_step_kernel<float>,
-        (const void*)hsg::hsolve_step_lean_kernel<double>, (const void*)hsg::hsolve_step_lean_kernel<float>,
+        (const void*)hsg::hsolve_step_lean_kernel<double, 0>, (const void*)hsg::hsolve_step_lean_kernel<float, 0>,
+        (const void*)hsg::hsolve_step_lean_kernel<double, hsg::kSigHH>,
+        (const void*)hsg::hsolve_step_lean_kernel<float, hsg::kSigHH>,
         (const void*)hsg::hsolve_step_slow_kernel<double>, (const void*)hsg::hsolve_step_slow_kernel<float>,
         (const void*)hsg::hsolve_step_fast_kernel<double>, (const void*)hsg::hsolve_step_fast_kernel<float>,
         (const void*)hsg::hsolve_solve_kernel<double, false>, (const void*)hsg::hsolve_solve_kernel<float, false>,
