@@ -63,7 +63,7 @@ __device__ __forceinline__ void work_barrier(bool ldsWork) {
 // One lookup table in registers (hsolve/RateLookup.h LookupTable).
 template <typename GT>
 struct Table {
-    double min, max, dx;
+    double min, max, dx, rdx;   // rdx = 1 / dx
     int nCol;
     const GT* __restrict__ tab;
 };
@@ -74,17 +74,21 @@ __device__ __forceinline__ Table<GT> table_view(char* arena, const TableDev& t) 
     v.min = t.min;
     v.max = t.max;
     v.dx = t.dx;
+    v.rdx = 1.0 / t.dx;
     v.nCol = t.nColumns;
     v.tab = reinterpret_cast<const GT*>(arena + (sizeof(GT) == 8 ? t.off64 : t.off32));
     return v;
 }
 
-// LookupTable::row (hsolve/RateLookup.cpp:63-76)
+// LookupTable::row (hsolve/RateLookup.cpp:63-76).  The mixed form (GT =
+// float) multiplies by 1 / dx instead of dividing: the row and fraction then
+// differ from the quotient's by at most an ulp of the position, which moves
+// the interpolated value by less than the FP32 table's own rounding.
 template <typename GT>
 __device__ __forceinline__ void table_row(const Table<GT>& t, double x, int& base, double& frac) {
     if (x < t.min) x = t.min;
     else if (x > t.max) x = t.max;
-    double div = (x - t.min) / t.dx;
+    double div = std::is_same<GT, float>::value ? (x - t.min) * t.rdx : (x - t.min) / t.dx;
     unsigned integer = (unsigned)div;
     frac = div - integer;
     base = (int)integer * t.nCol;
@@ -1265,35 +1269,50 @@ __device__ __forceinline__ GT power_code(GT x, int code) {
 constexpr int kSigHH = 1;
 constexpr int kSigHHWord[2] = {3 | 1 << 3, 4};
 
-template <typename GT, int MS, bool NOPOOL = false, int SIG = 0>
-__device__ __forceinline__ void channel_row_fast(const View<GT>& v, const CellDev& c, int i, const double V,
-                                                 double dtg, double& Dout, double& Bout) {
+// The loaded inputs of one compartment's fast channel pass.
+template <typename GT, int MS>
+struct RowIn {
+    double Dbase, CmByDt, EmByRm;
+    double st[MS][3];
+    GT gb[MS], ek[MS];
+    double md[MS];
+    GT raw[MS][3][4];
+    GT zf[MS];
+    GT frac;
+};
+
+// Every global load of compartment i's channel pass, issued together: gate
+// states, Gbar, Ek, modulation, Ca inputs, the passive terms, then the table
+// entries of every gate ((A, B) at the row and the next row; Z: the Ca row of
+// its pool, the channel's external Ca, or V, as channel_row chooses).
+// Absent entries read valid addresses and are not used.
+template <typename GT, int MS, bool NOPOOL, int SIG>
+__device__ __forceinline__ void row_load(const View<GT>& v, const CellDev& c, int i, const double V,
+                                         RowIn<GT, MS>& R) {
     static_assert(SIG == 0 || MS == 2, "the HH signature has two slots");
     const int n = v.n;
     const int S = SIG == kSigHH ? 2 : v.nChanSlots;
-    // gate codes of slot k (fast word 2) and whether the slot is live
     auto word2 = [&](int k) { return SIG == kSigHH ? kSigHHWord[k] : c.fast[k][2]; };
     auto live = [&](int k) { return SIG == kSigHH ? true : (k < S && c.fast[k][1] < 0); };
     const bool hasPools = !NOPOOL && v.hasPools;
-    // ---- loads (absent entries read valid addresses and are not used)
-    const double Dbase = v.Dbase[i], CmByDt = v.CmByDt[i], EmByRm = v.EmByRm[i];
+    R.Dbase = v.Dbase[i];
+    R.CmByDt = v.CmByDt[i];
+    R.EmByRm = v.EmByRm[i];
     const int nPools = hasPools ? v.poolCount[i] : 0;
     const double* __restrict__ caSrc = hasPools ? v.poolCa : v.V;
     const double* __restrict__ stSrc = c.nGateSlots > 0 ? v.state : v.V;
-    double st[MS][3];
-    GT gb[MS], ek[MS];
-    double md[MS], cav[MS], ext[MS];
+    double cav[MS], ext[MS];
 #pragma unroll
     for (int k = 0; k < MS; ++k) {
         const int w1 = c.fast[k][1], w2 = word2(k);
         const size_t kk = (size_t)(k < S ? k : S - 1) * n + i;
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax)
-            st[k][ax] = SIG != kSigHH || ((w2 >> (3 * ax)) & 7) != 0 ? stSrc[(size_t)((w1 >> (16 + 5 * ax)) & 31) * n + i]
-                                                                    : 0.0;
-        gb[k] = v.Gbar[kk];
-        ek[k] = v.Ek[kk];
-        md[k] = SIG != kSigHH && v.hasModulation ? v.mod[kk] : 1.0;
+            R.st[k][ax] = SIG != kSigHH || ((w2 >> (3 * ax)) & 7) != 0
+                              ? stSrc[(size_t)((w1 >> (16 + 5 * ax)) & 31) * n + i] : 0.0;
+        R.gb[k] = v.Gbar[kk];
+        R.ek[k] = v.Ek[kk];
+        R.md[k] = SIG != kSigHH && v.hasModulation ? v.mod[kk] : 1.0;
         // Ca inputs only where a Z gate may read them (uniform condition)
         const bool hasZ = ((w2 >> 6) & 7) != 0;
         ext[k] = hasZ ? v.extCa[kk] : 0.0;
@@ -1303,13 +1322,7 @@ __device__ __forceinline__ void channel_row_fast(const View<GT>& v, const CellDe
     int vBase;
     double vFrac;
     table_row(v.vt, V, vBase, vFrac);
-    const GT frac = (GT)vFrac;
-    // table entries of every gate: (A, B) at the row and the next row, loaded
-    // here and interpolated below, so the loads of all gates travel together
-    // (Z: the Ca row of its pool, the channel's external Ca, or V, as
-    // channel_row chooses)
-    GT raw[MS][3][4];
-    GT zf[MS];
+    R.frac = (GT)vFrac;
 #pragma unroll
     for (int k = 0; k < MS; ++k) {
         const int w0 = c.fast[k][0], w1 = c.fast[k][1], w2 = word2(k);
@@ -1321,58 +1334,76 @@ __device__ __forceinline__ void channel_row_fast(const View<GT>& v, const CellDe
             const GT* tab = v.vt.tab;
             int base = vBase, nCol = v.vt.nCol;
             if (ax == 2) {
-                zf[k] = frac;
+                R.zf[k] = R.frac;
                 const int caDep = ((w2 >> 12) & 15) - 1;
                 const bool fromPool = caDep >= 0 && caDep < nPools;
                 const bool fromExt = !fromPool && ext[k] > 0 && col + 1 < v.ct.nCol;
                 if (fromPool || fromExt) {
                     double cf;
                     table_row(v.ct, fromPool ? cav[k] : ext[k], base, cf);
-                    zf[k] = (GT)cf;
+                    R.zf[k] = (GT)cf;
                     tab = v.ct.tab;
                     nCol = v.ct.nCol;
                 }
             }
             const GT* ap = tab + base + col;
-            raw[k][ax][0] = ap[0];
-            raw[k][ax][1] = ap[nCol];
-            raw[k][ax][2] = ap[1];
-            raw[k][ax][3] = ap[nCol + 1];
+            R.raw[k][ax][0] = ap[0];
+            R.raw[k][ax][1] = ap[nCol];
+            R.raw[k][ax][2] = ap[1];
+            R.raw[k][ax][3] = ap[nCol + 1];
         }
     }
+}
+
+// The arithmetic of compartment i's channel pass (channel_row's, in the same
+// order) and its stores: the new gate states, and the row (D, B) returned.
+template <typename GT, int MS, bool NOPOOL, int SIG>
+__device__ __forceinline__ void row_finish(const View<GT>& v, const CellDev& c, int i, const double V, double dtg,
+                                           RowIn<GT, MS>& R, double& Dout, double& Bout) {
+    const int n = v.n;
+    const int S = SIG == kSigHH ? 2 : v.nChanSlots;
+    auto word2 = [&](int k) { return SIG == kSigHH ? kSigHHWord[k] : c.fast[k][2]; };
+    auto live = [&](int k) { return SIG == kSigHH ? true : (k < S && c.fast[k][1] < 0); };
     double GkSum = 0.0, GkEkSum = 0.0;
 #pragma unroll
     for (int k = 0; k < MS; ++k) {
         const int w2 = word2(k);
         if (!live(k)) continue;
-        GT fraction = SIG != kSigHH && v.hasModulation ? (GT)md[k] : (GT)1.0;
+        GT fraction = SIG != kSigHH && v.hasModulation ? (GT)R.md[k] : (GT)1.0;
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax) {
             const int code = (w2 >> (3 * ax)) & 7;
             if (code == 0) continue;
             // LookupTable::lookup (RateLookup.cpp:78-103)
-            const GT fr = ax == 2 ? zf[k] : frac;
-            const GT C1 = raw[k][ax][0] + (raw[k][ax][1] - raw[k][ax][0]) * fr;
-            const GT C2 = raw[k][ax][2] + (raw[k][ax][3] - raw[k][ax][2]) * fr;
-            const double x = gate_step<std::is_same<GT, float>::value>(st[k][ax], C1, C2, dtg, (w2 >> (9 + ax)) & 1);
-            st[k][ax] = x;
+            const GT fr = ax == 2 ? R.zf[k] : R.frac;
+            const GT C1 = R.raw[k][ax][0] + (R.raw[k][ax][1] - R.raw[k][ax][0]) * fr;
+            const GT C2 = R.raw[k][ax][2] + (R.raw[k][ax][3] - R.raw[k][ax][2]) * fr;
+            const double x = gate_step<std::is_same<GT, float>::value>(R.st[k][ax], C1, C2, dtg, (w2 >> (9 + ax)) & 1);
+            R.st[k][ax] = x;
             fraction *= power_code<GT>((GT)x, code);
         }
-        const GT Gk = gb[k] * fraction;
+        const GT Gk = R.gb[k] * fraction;
         GkSum += (double)Gk;
-        GkEkSum += (double)Gk * (double)ek[k];
+        GkEkSum += (double)Gk * (double)R.ek[k];
     }
-    // ---- stores
 #pragma unroll
     for (int k = 0; k < MS; ++k) {
         const int w1 = c.fast[k][1], w2 = word2(k);
         if (!live(k)) continue;
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax)
-            if (((w2 >> (3 * ax)) & 7) != 0) v.state[(size_t)((w1 >> (16 + 5 * ax)) & 31) * n + i] = st[k][ax];
+            if (((w2 >> (3 * ax)) & 7) != 0) v.state[(size_t)((w1 >> (16 + 5 * ax)) & 31) * n + i] = R.st[k][ax];
     }
-    Dout = Dbase + GkSum;
-    Bout = V * CmByDt + EmByRm + GkEkSum;
+    Dout = R.Dbase + GkSum;
+    Bout = V * R.CmByDt + R.EmByRm + GkEkSum;
+}
+
+template <typename GT, int MS, bool NOPOOL = false, int SIG = 0>
+__device__ __forceinline__ void channel_row_fast(const View<GT>& v, const CellDev& c, int i, const double V,
+                                                 double dtg, double& Dout, double& Bout) {
+    RowIn<GT, MS> R;
+    row_load<GT, MS, NOPOOL, SIG>(v, c, i, V, R);
+    row_finish<GT, MS, NOPOOL, SIG>(v, c, i, V, dtg, R, Dout, Bout);
 }
 
 // FASTMS > 0: the cells are uniform with at most FASTMS channel slots and
@@ -1444,14 +1475,41 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
                     w.B[i] = gwork[n + i];
                 }
         } else if constexpr (FASTMS > 0) {
-            // Vm one compartment ahead: the table rows of a compartment are
-            // known when its loads are issued, so they travel with them
-            double vNext = tid < n ? v.V[tid] : 0.0;
+            if constexpr (SIG == kSigHH) {
+                // two compartments per iteration: both compartments' loads
+                // travel together, then both are computed and stored (Vm
+                // loaded an iteration ahead, so the table rows need no wait)
+                double va = tid < n ? v.V[tid] : 0.0, vb = tid + nt < n ? v.V[tid + nt] : 0.0;
 #pragma unroll 1
-            for (int i = tid; i < n; i += nt) {
-                const double Vi = vNext;
-                if (i + nt < n) vNext = v.V[i + nt];
-                channel_row_fast<GT, FASTMS, LEAN, SIG>(v, c, i, Vi, dtg, w.D[i], w.B[i]);
+                for (int i = tid; i < n; i += 2 * nt) {
+                    const int j = min(i + nt, n - 1);
+                    const double Va = va, Vb = vb;
+                    va = v.V[min(i + 2 * nt, n - 1)];
+                    vb = v.V[min(i + 3 * nt, n - 1)];
+                    RowIn<GT, FASTMS> Ra, Rb;
+                    row_load<GT, FASTMS, LEAN, SIG>(v, c, i, Va, Ra);
+                    row_load<GT, FASTMS, LEAN, SIG>(v, c, j, Vb, Rb);
+                    double Da, Ba;
+                    row_finish<GT, FASTMS, LEAN, SIG>(v, c, i, Va, dtg, Ra, Da, Ba);
+                    w.D[i] = Da;
+                    w.B[i] = Ba;
+                    if (i + nt < n) {
+                        double Db, Bb;
+                        row_finish<GT, FASTMS, LEAN, SIG>(v, c, j, Vb, dtg, Rb, Db, Bb);
+                        w.D[j] = Db;
+                        w.B[j] = Bb;
+                    }
+                }
+            } else {
+                // Vm one compartment ahead: the table rows of a compartment
+                // are known when its loads are issued, so they travel with them
+                double vNext = tid < n ? v.V[tid] : 0.0;
+#pragma unroll 1
+                for (int i = tid; i < n; i += nt) {
+                    const double Vi = vNext;
+                    if (i + nt < n) vNext = v.V[i + nt];
+                    channel_row_fast<GT, FASTMS, LEAN, SIG>(v, c, i, Vi, dtg, w.D[i], w.B[i]);
+                }
             }
         } else {
             if (uniform)
