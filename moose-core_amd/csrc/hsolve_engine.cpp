@@ -34,7 +34,7 @@ extern "C" hipError_t hsg_launch_channel(const hsg::LaunchArgs* a, const int2* t
                                         hipStream_t stream);
 extern "C" hipError_t hsg_launch_external(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, hipStream_t stream);
 extern "C" hipError_t hsg_launch_channel_fast(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, int fp32,
-                                             int ms, hipStream_t stream);
+                                             int ms, int sig, hipStream_t stream);
 extern "C" hipError_t hsg_launch_chan2d_reinit(const hsg::LaunchArgs* a, int nblocks, hipStream_t stream);
 extern "C" hipError_t hsg_launch_update(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, int fp32,
                                        hipStream_t stream);
@@ -287,6 +287,7 @@ struct hsg_engine {
         int fastSel = -1;           // fast channel pass: -1 the engine's choice, 0 off, 1 on
         int lean = 0;               // no stimulus or external objects and no Ca pools in any cell: 1,
                                     // and every cell's channels the squid HH pair: 2
+        int sig = 0;                // every cell's channel slots Na, Kdr, KA, CaL: 2 (the split channel kernel)
     };
     // Launch shapes set by the caller per bin class (hsg_set_bin_shape):
     // threads per cell workgroup (0: the engine's choice) and the fast
@@ -1152,6 +1153,7 @@ void free_plan(hsg_engine::Plan& plan) {
 // device buffer is touched).
 void plan_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg_engine::Bin>& bins);
 bool hh_pair(const CellDev& d);
+bool ca_quartet(const CellDev& d);
 
 // Bytes of the device tables of a plan's bins: per bin its cell order, then
 // its channel tiles, each 256-byte aligned.
@@ -1296,7 +1298,7 @@ void plan_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg
     for (auto& b : bins) {
         int maxN = 0, maxSlots = 0;
         bool allUniform = true;
-        bool lean = true, hh = true;
+        bool lean = true, hh = true, ca4 = true;
         for (int id : b.cells) {
             const CellDev& d = e->cells[id]->dev;
             b.lds = std::max<int>(b.lds, d.ldsBytes);
@@ -1305,7 +1307,9 @@ void plan_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg
             allUniform = allUniform && d.fastOk;
             lean = lean && d.nPulse == 0 && d.nVClamp == 0 && d.nXObj == 0 && !d.hasPools;
             hh = hh && hh_pair(d);
+            ca4 = ca4 && ca_quartet(d);
         }
+        b.sig = ca4 ? 2 : 0;
         b.lean = lean ? (hh ? 2 : 1) : 0;
         b.chanMS = allUniform ? (maxSlots <= 2 ? 2 : maxSlots <= 4 ? 4 : 8) : 0;
         b.sumN = 0;
@@ -1559,6 +1563,15 @@ bool hh_pair(const CellDev& d) {
            d.fast[0][2] == (3 | 1 << 3) && d.fast[1][2] == 4;
 }
 
+// Na, Kdr, KA and CaL on the fast pass: four live slots m^3 h, n^4, a^3 b,
+// s^2 (no Z gate, no instant gate), without modulation (kSigCa4).
+bool ca_quartet(const CellDev& d) {
+    const int w[4] = {3 | 1 << 3, 4, 3 | 1 << 3, 2};
+    bool ok = d.fastOk && d.nChanSlots == 4 && !d.hasModulation;
+    for (int k = 0; k < 4 && ok; ++k) ok = d.fast[k][1] < 0 && d.fast[k][2] == w[k];
+    return ok;
+}
+
 int fused_fast(const hsg_engine::Bin& b) {
     if (b.chanMS != 2 || b.slow || b.cells.empty() || b.fastSel == 0) return 0;
     if (b.fastSel == 1) return 2;
@@ -1571,7 +1584,7 @@ int fused_fast(const hsg_engine::Bin& b) {
 int split_wide(hsg_engine* e, const hsg_engine::Bin& bin, LaunchArgs& a, hipStream_t st) {
     if (bin.hasExt) HIPCHK(hsg_launch_external(&a, bin.dTiles, bin.nTiles, st));
     if (bin.chanMS > 0 && bin.fastSel != 0)
-        HIPCHK(hsg_launch_channel_fast(&a, bin.dTiles, bin.nTiles, e->fp32, bin.chanMS, st));
+        HIPCHK(hsg_launch_channel_fast(&a, bin.dTiles, bin.nTiles, e->fp32, bin.chanMS, bin.sig, st));
     else HIPCHK(hsg_launch_channel(&a, bin.dTiles, bin.nTiles, e->fp32, st));
     return 0;
 }

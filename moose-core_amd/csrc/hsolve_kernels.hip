@@ -731,7 +731,7 @@ __device__ __forceinline__ void group_members(const Desc& u, int (&m)[S]) {
     if constexpr (S == 5) m[4] = u.b.y;
 }
 
-template <int S>
+template <int S, bool FD = false>
 __device__ __forceinline__ void fwd_group_desc(const Desc& u, const double* hj0, Work w) {
     int m[S];
     group_members<S>(u, m);
@@ -762,8 +762,9 @@ __device__ __forceinline__ void fwd_group_desc(const Desc& u, const double* hj0,
 #pragma unroll
         for (int q = p + 1; q < S; ++q) {
             const double lower = HJ[base + 2 * (q - p - 1) + 1];
-            const double tD = lower / pivot * HJ[base + 2 * (q - p - 1)];
-            const double tB = lower / pivot * Bc;
+            const double lp = quot<FD>(lower, pivot);
+            const double tD = lp * HJ[base + 2 * (q - p - 1)];
+            const double tB = lp * Bc;
             if (q == S - 1 && deferLast >= 0) {
                 w.T[2 * (deferLast + p)] = tD;
                 w.T[2 * (deferLast + p) + 1] = tB;
@@ -776,7 +777,7 @@ __device__ __forceinline__ void fwd_group_desc(const Desc& u, const double* hj0,
                 if (z == q) continue;
                 const double above = HJ[base + 2 * (z - p - 1)];
                 const int idx = q < z ? hb(S, q) + 2 * (z - q - 1) : hb(S, z) + 2 * (q - z - 1) + 1;
-                HJ[idx] -= lower / pivot * above;
+                HJ[idx] -= lp * above;
             }
         }
     }
@@ -791,7 +792,7 @@ __device__ __forceinline__ void fwd_group_desc(const Desc& u, const double* hj0,
     for (int k = 0; k < S * (S - 1); ++k) w.HJ[hjOff + k] = HJ[k];
 }
 
-template <int S>
+template <int S, bool FD = false>
 __device__ __forceinline__ void bwd_group_desc(const Desc& u, Work w) {
     int m[S];
     group_members<S>(u, m);
@@ -811,7 +812,7 @@ __device__ __forceinline__ void bwd_group_desc(const Desc& u, Work w) {
         double acc = V[p];
 #pragma unroll
         for (int q = S - 1; q > p; --q) acc -= V[q] * HJ[base + 2 * (q - p - 1)];
-        V[p] = acc / D[p];
+        V[p] = quot<FD>(acc, D[p]);
     }
 #pragma unroll
     for (int p = 0; p + 1 < S; ++p) w.B[m[p]] = V[p];
@@ -1261,13 +1262,16 @@ __device__ __forceinline__ GT power_code(GT x, int code) {
 
 // V is the compartment's Vm, loaded by the caller (the fused kernel loads it
 // one compartment ahead, so that the table rows need no load of their own).
-// NOPOOL: the caller knows the cell has no Ca pools.  SIG = kSigHH: the caller
-// knows the cell's two slots are the squid HH pair (m^3 h, then n^4; no Z
-// gate, no instant gate, no modulation), so the gate codes are constants and
-// every branch on them folds away; only the table columns and state slots
-// are read from the cell.
+// NOPOOL: the caller knows the cell has no Ca pools.  SIG > 0: the caller
+// knows the cell's channel slots (no Z gate, no instant gate, no
+// modulation), so the gate codes are constants and every branch on them
+// folds away; only the table columns and state slots are read from the cell.
+// kSigHH: the squid HH pair (m^3 h, then n^4); kSigCa4: m^3 h, n^4, a^3 b,
+// s^2 (Na, Kdr, KA, CaL).
 constexpr int kSigHH = 1;
-constexpr int kSigHHWord[2] = {3 | 1 << 3, 4};
+constexpr int kSigCa4 = 2;
+constexpr int kSigSlots[3] = {0, 2, 4};
+constexpr int kSigWord[3][4] = {{0, 0, 0, 0}, {3 | 1 << 3, 4, 0, 0}, {3 | 1 << 3, 4, 3 | 1 << 3, 2}};
 
 // The loaded inputs of one compartment's fast channel pass.
 template <typename GT, int MS>
@@ -1289,11 +1293,11 @@ struct RowIn {
 template <typename GT, int MS, bool NOPOOL, int SIG>
 __device__ __forceinline__ void row_load(const View<GT>& v, const CellDev& c, int i, const double V,
                                          RowIn<GT, MS>& R) {
-    static_assert(SIG == 0 || MS == 2, "the HH signature has two slots");
+    static_assert(SIG == 0 || MS == kSigSlots[SIG], "a signature fixes the slot count");
     const int n = v.n;
-    const int S = SIG == kSigHH ? 2 : v.nChanSlots;
-    auto word2 = [&](int k) { return SIG == kSigHH ? kSigHHWord[k] : c.fast[k][2]; };
-    auto live = [&](int k) { return SIG == kSigHH ? true : (k < S && c.fast[k][1] < 0); };
+    const int S = SIG ? kSigSlots[SIG] : v.nChanSlots;
+    auto word2 = [&](int k) { return SIG ? kSigWord[SIG][k] : c.fast[k][2]; };
+    auto live = [&](int k) { return SIG ? k < kSigSlots[SIG] : (k < S && c.fast[k][1] < 0); };
     const bool hasPools = !NOPOOL && v.hasPools;
     R.Dbase = v.Dbase[i];
     R.CmByDt = v.CmByDt[i];
@@ -1308,11 +1312,11 @@ __device__ __forceinline__ void row_load(const View<GT>& v, const CellDev& c, in
         const size_t kk = (size_t)(k < S ? k : S - 1) * n + i;
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax)
-            R.st[k][ax] = SIG != kSigHH || ((w2 >> (3 * ax)) & 7) != 0
+            R.st[k][ax] = SIG == 0 || ((w2 >> (3 * ax)) & 7) != 0
                               ? stSrc[(size_t)((w1 >> (16 + 5 * ax)) & 31) * n + i] : 0.0;
         R.gb[k] = v.Gbar[kk];
         R.ek[k] = v.Ek[kk];
-        R.md[k] = SIG != kSigHH && v.hasModulation ? v.mod[kk] : 1.0;
+        R.md[k] = SIG == 0 && v.hasModulation ? v.mod[kk] : 1.0;
         // Ca inputs only where a Z gate may read them (uniform condition)
         const bool hasZ = ((w2 >> 6) & 7) != 0;
         ext[k] = hasZ ? v.extCa[kk] : 0.0;
@@ -1361,15 +1365,15 @@ template <typename GT, int MS, bool NOPOOL, int SIG>
 __device__ __forceinline__ void row_finish(const View<GT>& v, const CellDev& c, int i, const double V, double dtg,
                                            RowIn<GT, MS>& R, double& Dout, double& Bout) {
     const int n = v.n;
-    const int S = SIG == kSigHH ? 2 : v.nChanSlots;
-    auto word2 = [&](int k) { return SIG == kSigHH ? kSigHHWord[k] : c.fast[k][2]; };
-    auto live = [&](int k) { return SIG == kSigHH ? true : (k < S && c.fast[k][1] < 0); };
+    const int S = SIG ? kSigSlots[SIG] : v.nChanSlots;
+    auto word2 = [&](int k) { return SIG ? kSigWord[SIG][k] : c.fast[k][2]; };
+    auto live = [&](int k) { return SIG ? k < kSigSlots[SIG] : (k < S && c.fast[k][1] < 0); };
     double GkSum = 0.0, GkEkSum = 0.0;
 #pragma unroll
     for (int k = 0; k < MS; ++k) {
         const int w2 = word2(k);
         if (!live(k)) continue;
-        GT fraction = SIG != kSigHH && v.hasModulation ? (GT)R.md[k] : (GT)1.0;
+        GT fraction = SIG == 0 && v.hasModulation ? (GT)R.md[k] : (GT)1.0;
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax) {
             const int code = (w2 >> (3 * ax)) & 7;
@@ -1859,7 +1863,7 @@ __device__ __forceinline__ void stage_schedule(char* arena, const BlockView& v) 
 // form (two or three members, four, five); each form's run is laid on
 // whole wavefronts (its lanes padded to a multiple of 64), so a wavefront
 // runs one form's code rather than all three under masks.
-template <bool FWD, bool SLOWG>
+template <bool FWD, bool SLOWG, bool FD>
 __device__ __forceinline__ void block_level(const BlockView& v, int l) {
     const int32_t* lv = v.level + kBlockForms * l;
     const int s0 = lv[0], s1 = lv[1], s2 = lv[2], e = lv[3];
@@ -1868,35 +1872,37 @@ __device__ __forceinline__ void block_level(const BlockView& v, int l) {
         if (q < p0) {
             if (s0 + q < s1) {
                 const Desc d = load_desc(v.desc, s0 + q);
-                if (FWD) fwd_small(d, v.w);
-                else bwd_small(d, v.w);
+                if (FWD) fwd_small<FD>(d, v.w);
+                else bwd_small<FD>(d, v.w);
             }
         } else if (q < p0 + p1) {
             if (SLOWG && s1 + q - p0 < s2) {
                 const Desc d = load_desc(v.desc, s1 + q - p0);
-                if (FWD) fwd_group_desc<4>(d, v.w.HJ0, v.w);
-                else bwd_group_desc<4>(d, v.w);
+                if (FWD) fwd_group_desc<4, FD>(d, v.w.HJ0, v.w);
+                else bwd_group_desc<4, FD>(d, v.w);
             }
         } else if (SLOWG && s2 + q - p0 - p1 < e) {
             const Desc d = load_desc(v.desc, s2 + q - p0 - p1);
-            if (FWD) fwd_group_desc<5>(d, v.w.HJ0, v.w);
-            else bwd_group_desc<5>(d, v.w);
+            if (FWD) fwd_group_desc<5, FD>(d, v.w.HJ0, v.w);
+            else bwd_group_desc<5, FD>(d, v.w);
         }
     }
 }
 
+// (Both forms divide exactly here: the refined reciprocal of the mixed form
+// measured no faster in the block kernels.)
 template <bool SLOWG>
-__device__ __forceinline__ void block_forward(char* arena, const CellDev& c, const BlockView& v) {
+__device__ __forceinline__ void block_forward(const LaunchArgs& a, const BlockView& v) {
     for (int l = 0; l < v.b->nLevels; ++l) {
-        block_level<true, SLOWG>(v, l);
+        block_level<true, SLOWG, false>(v, l);
         lds_barrier();
     }
 }
 
 template <bool SLOWG>
-__device__ __forceinline__ void block_backward(char* arena, const CellDev& c, const BlockView& v) {
+__device__ __forceinline__ void block_backward(const LaunchArgs& a, const BlockView& v) {
     for (int l = v.b->nLevels - 1; l >= 0; --l) {
-        block_level<false, SLOWG>(v, l);
+        block_level<false, SLOWG, false>(v, l);
         lds_barrier();
     }
 }
@@ -1915,7 +1921,7 @@ __global__ void __launch_bounds__(256) hsolve_block_fwd_kernel(LaunchArgs a, con
     stage_gather(v.w.HJ, v.gHJ0, nullptr, nullptr, v.hjMap, b.nHJ);
     stage_schedule(arena, v);
     __syncthreads();
-    block_forward<SLOWG>(arena, c, v);
+    block_forward<SLOWG>(a, v);
     // the rows (eliminated, for the way back; fed, for the top), the term
     // pairs left for the top, the fills, the working junction coefficients
     for (int k = tid; k < R; k += nt) {
@@ -1950,7 +1956,7 @@ __global__ void __launch_bounds__(256) hsolve_block_bwd_kernel(LaunchArgs a, con
     stage_gather(v.w.HJ, v.gHJ, nullptr, nullptr, v.hjMap, b.nHJ);
     stage_schedule(arena, v);
     __syncthreads();
-    block_backward<SLOWG>(arena, c, v);
+    block_backward<SLOWG>(a, v);
     for (int k = tid; k < b.nRows; k += nt) v.gB[v.rows[k]] = v.w.B[k];
 }
 
@@ -1977,10 +1983,10 @@ __global__ void __launch_bounds__(256) hsolve_block_top_kernel(LaunchArgs a) {
     stage_gather(v.w.HJ, v.gHJ0, nullptr, nullptr, v.hjMap, b.nHJ);
     stage_schedule(arena, v);
     __syncthreads();
-    block_forward<SLOWG>(arena, c, v);
+    block_forward<SLOWG>(a, v);
     if (tid == 0) v.w.B[b.nRows] = v.w.B[b.nRows] / v.w.D[b.nRows];
     lds_barrier();
-    block_backward<SLOWG>(arena, c, v);
+    block_backward<SLOWG>(a, v);
     for (int k = tid; k < R; k += nt) v.gB[v.rows[k]] = v.w.B[k];
     if (tid == 0) *A_<int64_t>(arena, c.oSteps) += a.nsteps;
 }
@@ -2062,8 +2068,9 @@ __global__ void __launch_bounds__(256) hsolve_channel_kernel(LaunchArgs a, const
     row_inputs(a, c, i, w.D[dst], w.B[dst]);
 }
 
-// The channel kernel for bins of uniform cells with at most MS slots.
-template <typename GT, int MS>
+// The channel kernel for bins of uniform cells with at most MS slots (SIG:
+// their slot signature, 0 for any).
+template <typename GT, int MS, int SIG = 0>
 __global__ void __launch_bounds__(256) hsolve_channel_fast_kernel(LaunchArgs a, const int2* __restrict__ tiles) {
     const int2 tile = tiles[blockIdx.x];
     const CellDev& c = a.cells[tile.x];
@@ -2071,7 +2078,7 @@ __global__ void __launch_bounds__(256) hsolve_channel_fast_kernel(LaunchArgs a, 
     const int i = tile.y + threadIdx.x;
     if (i >= v.n) return;
     double D, B;
-    channel_row_fast<GT, MS>(v, c, i, v.V[i], a.dt, D, B);
+    channel_row_fast<GT, MS, false, SIG>(v, c, i, v.V[i], a.dt, D, B);
     row_inputs(a, c, i, D, B);
     double *Dp, *Bp;
     row_dest(a.arena, c, i, Dp, Bp);
@@ -2545,9 +2552,12 @@ extern "C" hipError_t hsg_launch_reinit(const hsg::LaunchArgs* a, int nblocks, i
 }
 
 extern "C" hipError_t hsg_launch_channel_fast(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, int fp32,
-                                             int ms, hipStream_t stream) {
+                                             int ms, int sig, hipStream_t stream) {
     const dim3 g(ntiles), b(hsg::kChannelTile);
-    if (fp32) {
+    if (sig == hsg::kSigCa4 && ms == 4) {
+        if (fp32) hipLaunchKernelGGL((hsg::hsolve_channel_fast_kernel<float, 4, hsg::kSigCa4>), g, b, 0, stream, *a, tiles);
+        else hipLaunchKernelGGL((hsg::hsolve_channel_fast_kernel<double, 4, hsg::kSigCa4>), g, b, 0, stream, *a, tiles);
+    } else if (fp32) {
         if (ms <= 2) hipLaunchKernelGGL((hsg::hsolve_channel_fast_kernel<float, 2>), g, b, 0, stream, *a, tiles);
         else if (ms <= 4) hipLaunchKernelGGL((hsg::hsolve_channel_fast_kernel<float, 4>), g, b, 0, stream, *a, tiles);
         else hipLaunchKernelGGL((hsg::hsolve_channel_fast_kernel<float, 8>), g, b, 0, stream, *a, tiles);
