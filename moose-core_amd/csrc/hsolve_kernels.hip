@@ -127,23 +127,20 @@ __device__ __forceinline__ GT take_power(GT x, double p) {
 }
 
 // Quotient a / b.  FAST = false: the IEEE quotient (the FP64 form reproduces
-// the reference bit for bit).  FAST = true (the mixed form): the reciprocal
-// refined by two Newton steps and the quotient corrected once, within one
-// ulp of the IEEE quotient for operands of ordinary range, in half the
-// dependent steps.
+// the reference bit for bit).  FAST = true (the mixed form): the hardware
+// reciprocal (relative error up to 2^-24 on gfx950, measured over 1e-14..1e14)
+// refined by one Newton step (below 2^-48 measured) times a: within about
+// ten ulps of the IEEE quotient for operands of ordinary range, in four
+// dependent steps instead of about ten.
 template <bool FAST>
 __device__ __forceinline__ double quot(double a, double b) {
     if constexpr (!FAST) {
         return a / b;
     } else {
         double r = __builtin_amdgcn_rcp(b);
-        double e = fma(-b, r, 1.0);
+        const double e = fma(-b, r, 1.0);
         r = fma(e, r, r);
-        e = fma(-b, r, 1.0);
-        r = fma(e, r, r);
-        double q = a * r;
-        e = fma(-b, q, a);
-        return fma(e, r, q);
+        return a * r;
     }
 }
 
