@@ -539,7 +539,10 @@ int type_id(hsg_engine* e, const HostChanType& t, int tableSet) {
 int64_t work_lds_bytes(const CellDev& c) {
     return 8 * (2 * (int64_t)c.n + 2 * (int64_t)c.nTerm + (int64_t)c.nFill);
 }
-int64_t level_bytes(const CellDev& c) { return 4 * ((int64_t)c.nLevels + 1); }
+// The level table and, 16-byte aligned after it, the chain's descriptors.
+int64_t level_bytes(const CellDev& c) {
+    return 4 * ((int64_t)c.nLevels + 1) + 16 + 4 * kDescInts * ((int64_t)c.nLevels - c.chainLevel);
+}
 
 // Lays out and uploads one cell's main region.
 void fill_fast_slots(hsg_engine* e, CellDev& d);
@@ -674,6 +677,13 @@ int place_cell(hsg_engine* e, HostCell& hc) {
     d.caAdvance = s.caAdvance;
     d.nLevels = (int)s.levelStart.size() - 1;
     d.nUnits = (int)s.units.size();
+    {
+        // the trailing levels of one unit each, when there are at least two
+        // (at most the last kMaxChain of them: their descriptors take LDS)
+        int l = d.nLevels;
+        while (l > 0 && s.levelStart[l] - s.levelStart[l - 1] == 1 && d.nLevels - l < kMaxChain) --l;
+        d.chainLevel = d.nLevels - l >= 2 ? l : d.nLevels;
+    }
     d.nHJ = (int)s.HJ0.size();
     d.nTerm = s.nTerm;
     d.nFill = s.nFill;

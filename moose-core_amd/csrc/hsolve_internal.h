@@ -168,6 +168,11 @@ struct CellDev {
     int32_t caAdvance;
     int32_t nLevels;
     int32_t nUnits;
+    // levels [chainLevel, nLevels) hold one unit each (the trunk next to the
+    // root): the fused kernels run them as one chain on one thread, their
+    // descriptors staged in LDS after the level table (chainLevel == nLevels:
+    // no chain)
+    int32_t chainLevel;
     int32_t nHJ;            // doubles of junction coefficients
     int32_t nTerm;          // deferred elimination terms (pairs)
     int32_t nFill;          // fill slots (one per three-member group)
@@ -314,6 +319,7 @@ constexpr int kChannelTile = 256;  // compartments per workgroup of the channel 
 constexpr int kPairMaxCells = 512;  // split bins of up to this many cells run as pipelined pairs
 constexpr int kWideSolveMin = 4096;  // split bins with a cell from this size solve with 512 threads per cell
 constexpr int kDescInts = 12;
+constexpr int kMaxChain = 64;       // units of a cell's one-thread chain (their descriptors sit in LDS)
 constexpr int kBlockMaxLds = 64 * 1024;  // LDS of one block (rows, exported rows, terms, fills, levels)
 constexpr int kBlockForms = 3;           // unit forms per block level: 2-3 members, 4, 5
 

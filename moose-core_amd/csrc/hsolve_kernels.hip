@@ -1445,10 +1445,22 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
     // level table: in LDS after the work arrays, or read from HBM
     const int32_t* gLevel = A_<const int32_t>(arena, c.oLevelStart);
     const int32_t* level;
+    // the chain: levels [Lc, nLevels) of one unit each (the trunk next to
+    // the root), eliminated and back-substituted by thread 0 alone, one unit
+    // after another with no barrier between them; its descriptors in LDS,
+    // 16-byte aligned after the level table (work arrays in LDS only)
+    const int Lc = WL ? c.chainLevel : nLevels;
+    const int nChain = nLevels - Lc;
+    const int4* sChain = nullptr;
     if constexpr (WL) {
         int32_t* sLevel = reinterpret_cast<int32_t*>(lds + nWork);
         for (int l = tid; l <= nLevels; l += nt) sLevel[l] = gLevel[l];
         level = sLevel;
+        const size_t off = ((size_t)8 * nWork + 4 * ((size_t)nLevels + 1) + 15) & ~(size_t)15;
+        int4* sc = reinterpret_cast<int4*>(reinterpret_cast<char*>(lds) + off);
+        const int4* gd = A_<const int4>(arena, c.oDesc) + (nChain > 0 ? 3 * (size_t)gLevel[Lc] : 0);
+        for (int k = tid; k < 3 * nChain; k += nt) sc[k] = gd[k];
+        sChain = sc;
     } else {
         level = gLevel;
     }
@@ -1567,28 +1579,29 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
         // full barrier: it also lands every outstanding global load, so the
         // level loops start from an empty queue
         __syncthreads();
-        // ---- F: forward elimination, level by level.  Three descriptor
+        // ---- F: forward elimination of the levels below the chain, level by
+        // level.  Three descriptor
         // buffers rotate: the units of level l+2 load while level l is
         // processed, so a descriptor has two levels' time to arrive (its
         // loads stay in flight across the barriers; no register copy forces
         // them to land early).  Level bounds are read a level ahead.
         const int lastU = max(nUnits - 1, 0);
         {
-            int sA = nLevels > 0 ? level[0] : 0;
-            int sB = nLevels > 0 ? level[1] : 0;
+            int sA = Lc > 0 ? level[0] : 0;
+            int sB = Lc > 0 ? level[1] : 0;
             Desc d0 = cur, d1 = load_desc(desc, min(sB + tid, lastU)), d2;
-            for (int l = 0; l < nLevels; l += 3) {
-                const int sC = l + 2 <= nLevels ? level[l + 2] : sB;
+            for (int l = 0; l < Lc; l += 3) {
+                const int sC = l + 2 <= Lc ? level[l + 2] : sB;
                 d2 = load_desc(desc, min(sC + tid, lastU));
                 sweep_level<true, SLOWG, FD>(arena, c, desc, d0, sA, sB, w);
                 work_barrier(WL);
-                if (l + 1 >= nLevels) break;
-                const int sD = l + 3 <= nLevels ? level[l + 3] : sC;
+                if (l + 1 >= Lc) break;
+                const int sD = l + 3 <= Lc ? level[l + 3] : sC;
                 d0 = load_desc(desc, min(sD + tid, lastU));
                 sweep_level<true, SLOWG, FD>(arena, c, desc, d1, sB, sC, w);
                 work_barrier(WL);
-                if (l + 2 >= nLevels) break;
-                const int sE = l + 4 <= nLevels ? level[l + 4] : sD;
+                if (l + 2 >= Lc) break;
+                const int sE = l + 4 <= Lc ? level[l + 4] : sD;
                 d1 = load_desc(desc, min(sE + tid, lastU));
                 sweep_level<true, SLOWG, FD>(arena, c, desc, d2, sC, sD, w);
                 work_barrier(WL);
@@ -1596,17 +1609,33 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
                 sB = sE;
             }
         }
-        // ---- B: back-substitution (VMid overwrites B), in reverse level
-        // order, with the same rotation
+        // ---- the chain (thread 0): forward, the root, and back
+        if (tid == 0) {
+            Desc dn{};
+            if (nChain > 0) dn = load_desc(sChain, 0);
+            for (int k = 0; k < nChain; ++k) {
+                const Desc d = dn;
+                if (k + 1 < nChain) dn = load_desc(sChain, k + 1);
+                forward_unit<SLOWG, FD>(arena, c, d, w);
+            }
+            w.B[n - 1] = w.B[n - 1] / w.D[n - 1];
+            if (nChain > 0) dn = load_desc(sChain, nChain - 1);
+            for (int k = nChain - 1; k >= 0; --k) {
+                const Desc d = dn;
+                if (k > 0) dn = load_desc(sChain, k - 1);
+                backward_unit<SLOWG, FD>(arena, c, d, w);
+            }
+        }
+        // ---- B: back-substitution of the levels below the chain (VMid
+        // overwrites B), in reverse level order, with the same rotation
         {
-            // level L-1 is [sA, eA), level L-2 [sB, sA)
-            int eA = nLevels > 0 ? level[nLevels] : 0;
-            int sA = nLevels > 0 ? level[nLevels - 1] : 0;
-            int sB = nLevels > 1 ? level[nLevels - 2] : 0;
+            // level Lc-1 is [sA, eA), level Lc-2 [sB, sA)
+            int eA = Lc > 0 ? level[Lc] : 0;
+            int sA = Lc > 0 ? level[Lc - 1] : 0;
+            int sB = Lc > 1 ? level[Lc - 2] : 0;
             Desc d0 = load_desc(desc, min(sA + tid, lastU)), d1 = load_desc(desc, min(sB + tid, lastU)), d2;
-            if (tid == 0) w.B[n - 1] = w.B[n - 1] / w.D[n - 1];
             __syncthreads();
-            for (int l = nLevels - 1; l >= 0; l -= 3) {
+            for (int l = Lc - 1; l >= 0; l -= 3) {
                 const int sC = l >= 2 ? level[l - 2] : 0;
                 d2 = load_desc(desc, min(sC + tid, lastU));
                 sweep_level<false, SLOWG, FD>(arena, c, desc, d0, sA, eA, w);
