@@ -1132,7 +1132,6 @@ void fill_fast_slots(hsg_engine* e, CellDev& d) {
     if (!d.fastOk) std::memset(d.fast, 0, sizeof d.fast);
 }
 
-// Threads per cell workgroup for a bin whose largest cell has n compartments.
 // LDS classes of the per-cell bins: a class's cells need at most
 // kCuShare[k-1] and more than kCuShare[k] bytes (3, 4, 5, 6, 8 and 10 per CU)
 constexpr int kLdsClasses = 5;

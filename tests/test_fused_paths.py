@@ -1,0 +1,127 @@
+"""The fused step kernel's specialised paths against the oracle, with the launch
+plan checked to be the path under test:
+
+* the lean kernel for the squid HH pair (`hsolve_step_lean_kernel<.., kSigHH>`:
+  cells with nothing beside the solver but host inputs, whose two channel
+  slots are m^3 h then n^4) on cells of 50-2000 compartments, in every LDS
+  class of the engine's bins;
+* the generic lean kernel (`hsolve_step_lean_kernel<.., 0>`) on cells whose
+  second channel is n^3 instead of n^4;
+* the one-thread chain that runs the trunk of single-unit levels next to the
+  root: unbranched cables of 40 compartments (the whole schedule is the
+  chain) and 300 compartments (longer than the chain's 64-unit cap, so the
+  level-parallel sweep runs below it).
+
+FP64 form: identical bit for bit to the oracle (which is bit-exact with the
+reference's hsolve sources) at every compartment and step, identical spike
+steps.  Mixed form: within 1e-4 V of the oracle, spike steps within one step.
+"""
+import numpy as np
+import pytest
+
+from moose_core_amd import workloads
+from moose_core_amd.cellspec import CellSpec
+from moose_core_amd._abi import SPIKEGEN_DT
+
+pytestmark = pytest.mark.gpu
+
+MIXED_TOL = 1e-4    # V, 0.1 mV
+
+
+def _run(pkg, oracle_mod, specs, dt, nsteps, precision):
+    """Every bin on the fast channel pass (so on a lean kernel wherever the
+    cells allow one); GPU traces and spikes checked against the oracle.
+    Returns the plan's kernel names and the oracle's spike count."""
+    eng = pkg.Engine(precision, kernel_mode="fused")
+    ids = [eng.add_cell(s, dt) for s in specs]
+    eng.reinit(dt)
+    eng.step(1)                                  # the plan, then its shapes
+    for b in range(len(eng.plan_info())):
+        eng.set_bin_shape(b, 0, 1)
+    for cid, s in zip(ids, specs):
+        eng.set_probes(cid, np.arange(s.n), nsteps)
+    eng.reinit(dt)
+    eng.step(nsteps)
+    kernels = [b["kernel"] for b in eng.plan_info()]
+    traces = [eng.read_probes(cid) for cid in ids]
+    spikes = eng.read_spikes()
+    eng.close()
+    total = 0
+    for cid, spec, gpu in zip(ids, specs, traces):
+        c = oracle_mod.OracleCell(spec, dt)
+        c.reinit(dt)
+        ref = c.run(dt, nsteps, np.arange(spec.n))
+        _, ref_spk = c.read_spikes()
+        mine = list(spikes[spikes["cell"] == cid]["step"])
+        total += len(ref_spk)
+        if precision == "fp64":
+            np.testing.assert_array_equal(gpu, ref)
+            assert mine == list(ref_spk)
+        else:
+            assert np.abs(gpu - ref).max() <= MIXED_TOL
+            assert len(mine) == len(ref_spk) and all(abs(a - b) <= 1 for a, b in zip(mine, ref_spk))
+    return kernels, total
+
+
+def _cable(n, inject, k_power=4.0):
+    """An unbranched cable of n compartments (soma first), HH Na (m^3 h) and K
+    (n^k_power) everywhere, a SpikeGen on the soma."""
+    length = np.full(n, 10e-6)
+    dia = np.full(n, 1e-6)
+    length[0] = dia[0] = 20e-6
+    comp, area = workloads.passive_compartments(length, dia)
+    comp["inject"][0] = inject
+    axial = np.stack([np.arange(n - 1), np.arange(1, n)], axis=1).astype(np.int32)
+    spk = np.zeros(1, SPIKEGEN_DT)
+    spk["compartment"] = 0
+    spk["edgeTriggered"] = 1
+    return CellSpec(comp, axial, seed=0, channels=_hh_pair(n, area, k_power), gates=workloads.hh_gates(),
+                    spikegens=spk)
+
+
+def _hh_pair(n, area, k_power):
+    comp = np.arange(n, dtype=np.int32)
+    na = workloads.channel_records(n, comp, 1200.0 * area, 0.050, Xpower=3.0, Ypower=1.0, gates=(0, 1, -1))
+    k = workloads.channel_records(n, comp, 360.0 * area, -0.077, Xpower=k_power, gates=(2, -1, -1))
+    return workloads.interleave_channels([na, k])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_hh_lean_kernel_all_classes(pkg, oracle_mod, precision):
+    """HH cells of 50-2000 compartments: every bin on the HH lean kernel."""
+    print(f"case: HH cells 50-2000 compartments, 300 steps, {precision}, lean HH kernel", flush=True)
+    rng = np.random.default_rng(41)
+    sizes = (50, 180, 420, 700, 950, 1250, 1500, 1750, 2000)
+    specs = [workloads.random_hh_cell(rng, n, inject=float(rng.uniform(1.0e-9, 3.0e-9))) for n in sizes]
+    kernels, spikes = _run(pkg, oracle_mod, specs, 25e-6, 300, precision)
+    assert len(kernels) >= 3 and all("hsolve_step_lean_kernel" in k and "kSigHH" in k for k in kernels), kernels
+    assert spikes > 0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_generic_lean_kernel(pkg, oracle_mod, precision):
+    """Two-slot cells whose K channel is n^3: the lean kernel without the HH
+    signature."""
+    print(f"case: n^3 K cells, 300 steps, {precision}, generic lean kernel", flush=True)
+    rng = np.random.default_rng(42)
+    specs = []
+    for n in (300, 1200, 1900):
+        s = workloads.random_hh_cell(rng, n, inject=float(rng.uniform(1.0e-9, 3.0e-9)))
+        area = np.asarray(s.channels["Gbar"][s.channels["Xpower"] == 4.0]) / 360.0
+        s.channels = _hh_pair(n, area, 3.0)
+        specs.append(s)
+    kernels, spikes = _run(pkg, oracle_mod, specs, 25e-6, 300, precision)
+    assert all("hsolve_step_lean_kernel" in k and ", 0>" in k for k in kernels), kernels
+    assert spikes > 0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_trunk_chain_cables(pkg, oracle_mod, precision):
+    """Unbranched cables: the schedule is single-unit levels only, run as
+    the one-thread chain (40 compartments: all of it; 300: its last 64
+    units, the level-parallel sweep below)."""
+    print(f"case: cables of 40 and 300 compartments, 400 steps, {precision}, trunk chain", flush=True)
+    specs = [_cable(40, 1.0e-9), _cable(300, 1.0e-9)]
+    kernels, spikes = _run(pkg, oracle_mod, specs, 25e-6, 400, precision)
+    assert all("hsolve_step_lean_kernel" in k for k in kernels), kernels
+    assert spikes > 0
