@@ -320,7 +320,9 @@ constexpr int kPairMaxCells = 512;  // split bins of up to this many cells run a
 constexpr int kWideSolveMin = 4096;  // split bins with a cell from this size solve with 512 threads per cell
 constexpr int kDescInts = 12;
 constexpr int kMaxChain = 64;       // units of a cell's one-thread chain (their descriptors sit in LDS)
-constexpr int kBlockMaxLds = 64 * 1024;  // LDS of one block (rows, exported rows, terms, fills, levels)
+// LDS of one block (rows, exported rows, terms, fills, levels): three per CU
+// (156 of 160 KiB), with room for the static LDS of the kernels beside them
+constexpr int kBlockMaxLds = 52 * 1024;
 constexpr int kBlockForms = 3;           // unit forms per block level: 2-3 members, 4, 5
 
 struct SpikeRec {
