@@ -1418,6 +1418,10 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
     const int n = v.n;
     const int tid = threadIdx.x;
     const int nt = blockDim.x;
+    // phase D's old Vm of four compartments per thread loaded ahead (the
+    // mixed HH lean kernel, whose registers leave room for it: eight spill)
+    constexpr bool VPRE = LEAN && SIG == kSigHH && std::is_same<GT, float>::value;
+    constexpr int kVPre = VPRE ? 4 : 1;
     const int nTerm = c.nTerm;
     const int nLevels = c.nLevels;
     const int nUnits = c.nUnits;
@@ -1609,6 +1613,14 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
                 sB = sE;
             }
         }
+        // Vm of this thread's first compartments for phase D, loaded before
+        // the chain (mixed lean kernels): they land while thread 0 runs it,
+        // so phase D stores without waiting on loads
+        double vPre[kVPre];
+        if constexpr (VPRE) {
+#pragma unroll
+            for (int u = 0; u < kVPre; ++u) vPre[u] = v.V[min(tid + u * nt, n - 1)];
+        }
         // ---- the chain (thread 0): forward, the root, and back
         if (tid == 0) {
             Desc dn{};
@@ -1669,10 +1681,17 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
                     v.V[i] = vnew;
                     calcium_row<GT>(arena, c, v, types, i, vmid, vnew, first);
                 }
-            else
+            else {
+                int i0 = tid;
+                if constexpr (VPRE) {
+#pragma unroll
+                    for (int u = 0; u < kVPre; ++u)
+                        if (tid + u * nt < n) v.V[tid + u * nt] = 2 * w.B[tid + u * nt] - vPre[u];
+                    i0 = tid + kVPre * nt;
+                }
                 // Vm = 2 VMid - Vm; the old Vm of four compartments loaded
                 // together (clamped loads: every lane issues the same loads)
-                for (int i = tid; i < n; i += 4 * nt) {
+                for (int i = i0; i < n; i += 4 * nt) {
                     double vo[4];
 #pragma unroll
                     for (int u = 0; u < 4; ++u) vo[u] = v.V[min(i + u * nt, n - 1)];
@@ -1680,6 +1699,7 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
                     for (int u = 0; u < 4; ++u)
                         if (i + u * nt < n) v.V[i + u * nt] = 2 * w.B[i + u * nt] - vo[u];
                 }
+            }
             if (nSpike > 0 || nProbe > 0) {
                 __syncthreads();   // V written by other threads
                 tick_outputs(a, c, v.V, stepNo, pc);
