@@ -22,14 +22,21 @@ The default line also carries a short serving leg (Poisson arrivals of Ca
 cells with the 2-D K-AHP, continuous batching): p50/p99 request latency and
 sustained compartment-steps/s (`--serving-requests 0` leaves it out).
 
-The CPU baseline is the oracle (the bit-exact C restatement of the reference
-HSolve) on a bounded sample of the same cells, one cell after another on one
-core and sharded over the GPU's share of host cores.
+The CPU baseline is the reference's own HSolve step (hsolve/HSolveActive.cpp
+and the files it calls, compiled from the reference's sources into
+oracle/_ref/hsolve_ref, on the oracle's setup of each cell) on a bounded
+sample of the same cells, one cell after another on one core and sharded
+over processes on the GPU's share of host cores.  Where that build is absent,
+or the workload has objects beside the solver (HHChannel2D, SynChan), which
+the reference runs outside HSolve, the oracle (the bit-exact C restatement)
+is timed instead.
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
+import tempfile
 import threading
 import time
 
@@ -269,6 +276,58 @@ def cpu_baseline(cells, dt, budget_s=15.0):
     }
 
 
+def _ref_time(binary, nsteps, dt, flats):
+    """One hsolve_ref process timing its cells: (compartments, seconds)."""
+    out = subprocess.run([binary, "--time", str(int(nsteps)), repr(float(dt))] + flats, capture_output=True,
+                         text=True, check=True).stdout.split()
+    return int(out[0]), float(out[1])
+
+
+def cpu_baseline_reference(cells, dt, budget_s=15.0):
+    """The reference's own HSolveActive::step (oracle/_ref/hsolve_ref, built
+    from hsolve/*.cpp by oracle/Makefile) on the oracle's setup of every 16th
+    cell: one process stepping the sample on one core, then the sample
+    sharded over processes on the GPU's share of host cores (wall time from
+    their start to the last one's end).  None when the build is absent."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    if not oracle.ref_available():
+        return None
+    sample = cells[::16]
+    ncomp = sum(c.n for c in sample)
+    ncores = max(1, min(HOST_CORES_PER_GPU, os.cpu_count() or 1))
+    with tempfile.TemporaryDirectory() as td:
+        flats = []
+        for k, c in enumerate(sample):
+            o = oracle.OracleCell(c, dt)
+            path = os.path.join(td, f"cell{k}.bin")
+            o.dump_flat(path)
+            o.close()
+            flats.append(path)
+        n0, s0 = _ref_time(oracle.REF_BIN, 20, dt, flats[:1])
+        rate1 = n0 * 20 / max(s0, 1e-9)
+        nsteps = int(max(10, min(5000, budget_s * rate1 / 2 / ncomp)))
+        n1, s1 = _ref_time(oracle.REF_BIN, nsteps, dt, flats)
+        single = n1 * nsteps / s1
+        shards = [flats[i::ncores] for i in range(ncores) if flats[i::ncores]]
+        t = time.perf_counter()
+        procs = [subprocess.Popen([oracle.REF_BIN, "--time", str(nsteps), repr(float(dt))] + sh,
+                                  stdout=subprocess.PIPE, text=True) for sh in shards]
+        for p in procs:
+            p.communicate()
+        wall = time.perf_counter() - t
+        if any(p.returncode != 0 for p in procs):
+            raise RuntimeError("hsolve_ref failed")
+    return {
+        "value": ncomp * nsteps / wall, "unit": "compartment-steps/s", "cores": len(shards), "kind": "reference",
+        "single_core_value": single,
+        "sample": f"every 16th cell of the workload ({len(sample)} cells, {ncomp} compartments) x {nsteps} steps, "
+                  f"the reference's own HSolveActive::step (hsolve/*.cpp built from its sources, -O3 "
+                  f"-ffp-contract=off) on the oracle's setup of each cell, {len(shards)} processes (one GPU's "
+                  f"share of the host), wall time including process start",
+    }
+
+
 def serving_requests(rng, count, rate, dt):
     """`count` requests with Poisson arrivals at `rate` per second: a seeded
     random Ca cell with the 2-D K-AHP of n ~ log-U[100, 30000] compartments
@@ -503,8 +562,16 @@ def main():
         out["serving"]["requests_workload"] = ("Poisson arrivals, n~logU[100,30000] Ca cells with 2-D K-AHP, "
                                                "duration logU[10ms,1s], dt=25us, mixed precision")
     if not args.no_cpu:
-        log("[bench] CPU baseline (oracle) ...")
-        out["cpu_baseline"] = cpu_baseline(cells, dt)
+        # objects beside the solver run outside the reference's HSolve: the
+        # oracle covers them
+        beside = any(len(c.chans2d) or len(c.synchans) for c in cells)
+        ref = None if beside else cpu_baseline_reference(cells, dt)
+        if ref is not None:
+            log("[bench] CPU baseline: the reference's HSolve step")
+            out["cpu_baseline"] = ref
+        else:
+            log("[bench] CPU baseline (oracle) ...")
+            out["cpu_baseline"] = cpu_baseline(cells, dt)
     print(json.dumps(out))
 
 

@@ -15,10 +15,17 @@
 // usage: hsolve_ref FLAT.bin NSTEPS DT OUT.bin [REINIT(1)]
 // OUT.bin: int64 n, int64 nsteps, then V (n doubles) after every step,
 //          then VMid (n), state (nState), ca (nPool), Gk (nChan).
+// usage: hsolve_ref --time NSTEPS DT FLAT.bin...
+//        the CPU baseline: every cell reinit, then NSTEPS steps of each,
+//        nothing recorded; prints the compartments and the seconds the
+//        step loops took.
 #include "hsolve/HSolveActive.h"
 
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
+#include <memory>
 #include <vector>
 
 namespace {
@@ -166,11 +173,52 @@ public:
         fwrite(ca_.data(), sizeof(double), ca_.size(), out);
         for (const CurrentStruct& c : current_) fwrite(&c.Gk, sizeof(double), 1, out);
     }
+
+    long compartments() const { return (long)V_.size(); }
+
+    void start(double dt) {
+        ProcInfo p;
+        p.dt = dt;
+        p.currTime = 0.0;
+        reinit(&p);
+    }
+
+    void steps(long nsteps, double dt) {
+        ProcInfo p;
+        p.dt = dt;
+        for (long k = 0; k < nsteps; ++k) {
+            p.currTime = dt * (double)(unsigned long)(k + 1);
+            step(&p);
+        }
+    }
 };
+
+int time_cells(int argc, char** argv) {
+    const long nsteps = atol(argv[2]);
+    const double dt = atof(argv[3]);
+    std::vector<std::unique_ptr<Harness>> cells;
+    long n = 0;
+    for (int a = 4; a < argc; ++a) {
+        FILE* f = fopen(argv[a], "rb");
+        if (!f) { perror(argv[a]); return 2; }
+        Reader r{f};
+        cells.push_back(std::make_unique<Harness>());
+        cells.back()->load(r);
+        fclose(f);
+        n += cells.back()->compartments();
+    }
+    for (auto& h : cells) h->start(dt);
+    const auto t0 = std::chrono::steady_clock::now();
+    for (auto& h : cells) h->steps(nsteps, dt);
+    const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    printf("%ld %.9f\n", n, s);
+    return 0;
+}
 
 }  // namespace
 
 int main(int argc, char** argv) {
+    if (argc >= 5 && strcmp(argv[1], "--time") == 0) return time_cells(argc, argv);
     if (argc < 5) {
         fprintf(stderr, "usage: %s FLAT.bin NSTEPS DT OUT.bin [REINIT]\n", argv[0]);
         return 2;

@@ -76,3 +76,23 @@ def test_oracle_matches_reference_with_inputs(oracle_mod):
     tr = cell.run(dt, 300, np.arange(spec.n))
     np.testing.assert_array_equal(tr, V[:, order])
     np.testing.assert_array_equal(cell.ca(), ca)
+
+
+@needs_ref
+def test_reference_timing_mode(oracle_mod, tmp_path):
+    """The reference harness's CPU-baseline mode (bench.py's cpu_baseline):
+    it loads every cell given, steps them and reports their compartments and
+    a positive loop time."""
+    import subprocess
+    rng = np.random.default_rng(8)
+    cells = [workloads.random_hh_cell(rng, n, inject=1e-10) for n in (60, 250)]
+    flats = []
+    for k, c in enumerate(cells):
+        o = oracle_mod.OracleCell(c, 25e-6)
+        flats.append(str(tmp_path / f"c{k}.bin"))
+        o.dump_flat(flats[-1])
+        o.close()
+    out = subprocess.run([oracle_mod.REF_BIN, "--time", "50", repr(25e-6)] + flats, capture_output=True,
+                         text=True, check=True).stdout.split()
+    assert int(out[0]) == sum(c.n for c in cells)
+    assert float(out[1]) > 0.0
