@@ -1582,7 +1582,10 @@ bool ca_quartet(const CellDev& d) {
 }
 
 int fused_fast(const hsg_engine::Bin& b) {
-    if (b.chanMS != 2 || b.slow || b.cells.empty() || b.fastSel == 0) return 0;
+    if (b.cells.empty() || b.fastSel == 0) return 0;
+    // Na/Kdr/KA/CaL cells (junction groups or not): hsolve_step_ca_kernel
+    if (b.chanMS == 4 && b.sig == 2) return 4;
+    if (b.chanMS != 2 || b.slow) return 0;
     if (b.fastSel == 1) return 2;
     const double perThread = (double)b.sumN / (double)b.cells.size() / (double)b.threads;
     return perThread >= HSG_FAST_FUSED_MIN ? 2 : 0;
@@ -3016,6 +3019,8 @@ int hsg_plan_info(hsg_engine* e, hsg_bin_info* out, int32_t cap, int32_t* n_out)
         else if (x.split)
             snprintf(o.kernel, sizeof o.kernel, "%s<%s>", x.msg ? "hsolve_msg_kernel"
                      : x.solveThreads > 256 ? "hsolve_solve_wide_kernel" : "hsolve_solve_kernel", prec);
+        else if (fused_fast(x) == 4)
+            snprintf(o.kernel, sizeof o.kernel, "hsolve_step_ca_kernel<%s, %s>", prec, x.slow ? "true" : "false");
         else
             snprintf(o.kernel, sizeof o.kernel, "%s<%s%s>", x.slow ? "hsolve_step_slow_kernel"
                      : fused_fast(x) ? (x.lean && !x.hasExt ? "hsolve_step_lean_kernel" : "hsolve_step_fast_kernel")
@@ -3042,7 +3047,7 @@ int hsg_set_bin_shape(hsg_engine* e, int32_t bin, int32_t threads, int32_t fast_
     const hsg_engine::Bin& b = e->plan.bins[bin];
     if (threads != 0 && b.split && !b.msg && b.blocks)
         return fail(HSG_E_ARG, "the block-form kernels have a fixed workgroup size");
-    const bool fastOk = b.split ? b.chanMS > 0 : (b.chanMS == 2 && !b.slow);
+    const bool fastOk = b.split ? b.chanMS > 0 : ((b.chanMS == 2 && !b.slow) || (b.chanMS == 4 && b.sig == 2));
     if (fast_pass == 1 && !fastOk) return fail(HSG_E_ARG, "the fast channel pass does not apply to this bin's cells");
     hsg_engine::BinShape& sh = e->shapes[b.key];
     sh.threads = threads;

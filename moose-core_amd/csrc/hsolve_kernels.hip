@@ -1735,6 +1735,19 @@ __global__ void __launch_bounds__(256) hsolve_step_slow_kernel(LaunchArgs a) {
     else cell_steps<GT, false, false, true, 0>(a, c, lds, sTypes);
 }
 
+// Cells whose four channel slots are Na, Kdr, KA and CaL (kSigCa4), with
+// Ca pools and (SLOWG) junction groups of four or five members: phase A
+// through the fast channel pass, every load of a compartment issued
+// together, instead of one round trip per gate.
+template <typename GT, bool SLOWG>
+__global__ void __launch_bounds__(256) hsolve_step_ca_kernel(LaunchArgs a) {
+    extern __shared__ double lds[];
+    __shared__ ChanType sTypes[kMaxSlots];
+    const CellDev& c = a.cells[a.order[blockIdx.x]];
+    if (c.workInLds) cell_steps<GT, false, true, SLOWG, 4, false, kSigCa4>(a, c, lds, sTypes);
+    else cell_steps<GT, false, false, SLOWG, 4, false, kSigCa4>(a, c, lds, sTypes);
+}
+
 // Uniform two-slot cells with many compartments per thread: phase A
 // through channel_row_fast (more registers, fewer workgroups per CU, the
 // channel loads of a compartment in two round trips instead of one per
@@ -2534,7 +2547,12 @@ extern "C" hipError_t hsg_launch_step(const hsg::LaunchArgs* a, int nblocks, int
                                      int fastms, int lean, hipStream_t stream) {
     const dim3 g(nblocks), b(threads);
     const size_t l = a->ldsBytes;
-    if (fastms == 2 && !slow && lean == 2) {
+    if (fastms == 4) {
+        if (fp32 && slow) hipLaunchKernelGGL((hsg::hsolve_step_ca_kernel<float, true>), g, b, l, stream, *a);
+        else if (fp32) hipLaunchKernelGGL((hsg::hsolve_step_ca_kernel<float, false>), g, b, l, stream, *a);
+        else if (slow) hipLaunchKernelGGL((hsg::hsolve_step_ca_kernel<double, true>), g, b, l, stream, *a);
+        else hipLaunchKernelGGL((hsg::hsolve_step_ca_kernel<double, false>), g, b, l, stream, *a);
+    } else if (fastms == 2 && !slow && lean == 2) {
         if (fp32) hipLaunchKernelGGL((hsg::hsolve_step_lean_kernel<float, hsg::kSigHH>), g, b, l, stream, *a);
         else hipLaunchKernelGGL((hsg::hsolve_step_lean_kernel<double, hsg::kSigHH>), g, b, l, stream, *a);
     } else if (fastms == 2 && !slow && lean) {
@@ -2663,7 +2681,7 @@ extern "C" hipError_t hsg_launch_top(const hsg::LaunchArgs* a, int nblocks, int 
 }
 
 extern "C" hipError_t hsg_set_max_lds(int bytes) {
-    const void* kernels[22] = {
+    const void* kernels[] = {
         (const void*)hsg::hsolve_step_kernel<double>, (const void*)hsg::hsolve_step_kernel<float>,
         (const void*)hsg::hsolve_step_lean_kernel<double, 0>, (const void*)hsg::hsolve_step_lean_kernel<float, 0>,
         (const void*)hsg::hsolve_step_lean_kernel<double, hsg::kSigHH>,
@@ -2675,7 +2693,9 @@ extern "C" hipError_t hsg_set_max_lds(int bytes) {
         (const void*)hsg::hsolve_msg_kernel<double>, (const void*)hsg::hsolve_msg_kernel<float>,
         (const void*)hsg::hsolve_block_fwd_kernel<false>, (const void*)hsg::hsolve_block_fwd_kernel<true>,
         (const void*)hsg::hsolve_block_bwd_kernel<false>, (const void*)hsg::hsolve_block_bwd_kernel<true>,
-        (const void*)hsg::hsolve_block_top_kernel<false>, (const void*)hsg::hsolve_block_top_kernel<true>};
+        (const void*)hsg::hsolve_block_top_kernel<false>, (const void*)hsg::hsolve_block_top_kernel<true>,
+        (const void*)hsg::hsolve_step_ca_kernel<double, false>, (const void*)hsg::hsolve_step_ca_kernel<float, false>,
+        (const void*)hsg::hsolve_step_ca_kernel<double, true>, (const void*)hsg::hsolve_step_ca_kernel<float, true>};
     for (const void* k : kernels) {
         hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
         if (e != hipSuccess) return e;
