@@ -1001,6 +1001,73 @@ __device__ __forceinline__ void calcium_row(char* arena, const CellDev& c, const
     }
 }
 
+// advanceCalcium (calcium_row's operations, in its order) for a cell whose
+// four channel slots are Na, Kdr, KA and CaL (kSigCa4: gate states at 0-1,
+// 2, 3-4 and 5; powers 3/1, 4, 3/1 and 2; no modulation) with at most one
+// pool per compartment.  `feed`: bit k set when slot k's channel feeds the
+// pool (from the staged types; the same for every compartment).  The loads
+// of a compartment (its old Vm, the pool's inputs, and the states, Gbar and
+// Ek of the feeding slots) are issued together by ca4_load; ca4_finish does
+// the arithmetic and the stores.
+template <typename GT>
+struct Ca4In {
+    double V, pc, basal, f1, f2, ceil, floor, ica;
+    double st[6];
+    GT gb[4], ek[4];
+    int np;
+};
+
+template <typename GT>
+__device__ __forceinline__ void ca4_load(char* arena, const CellDev& c, const View<GT>& v, int feed, int i,
+                                         bool first, Ca4In<GT>& R) {
+    constexpr int base[4] = {0, 2, 3, 5}, ng[4] = {2, 1, 2, 1};
+    const int n = v.n;
+    R.V = v.V[i];
+    R.np = v.poolCount[i];
+    R.pc = A_<const double>(arena, c.oPoolC)[i];
+    R.basal = A_<const double>(arena, c.oPoolBasal)[i];
+    R.f1 = A_<const double>(arena, c.oPoolF1)[i];
+    R.f2 = A_<const double>(arena, c.oPoolF2)[i];
+    R.ceil = A_<const double>(arena, c.oPoolCeil)[i];
+    R.floor = A_<const double>(arena, c.oPoolFloor)[i];
+    R.ica = first ? A_<const double>(arena, c.oPoolICa)[i] : 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (!((feed >> k) & 1)) continue;
+#pragma unroll
+        for (int g = 0; g < ng[k]; ++g) R.st[base[k] + g] = v.state[(size_t)(base[k] + g) * n + i];
+        R.gb[k] = v.Gbar[(size_t)k * n + i];
+        R.ek[k] = v.Ek[(size_t)k * n + i];
+    }
+}
+
+template <typename GT>
+__device__ __forceinline__ void ca4_finish(char* arena, const CellDev& c, const View<GT>& v, int feed, int i,
+                                           double vmid, bool first, Ca4In<GT>& R) {
+    constexpr int base[4] = {0, 2, 3, 5}, ng[4] = {2, 1, 2, 1};
+    constexpr double pw[4][2] = {{3.0, 1.0}, {4.0, 0.0}, {3.0, 1.0}, {2.0, 0.0}};
+    const double vnew = 2 * vmid - R.V;
+    v.V[i] = vnew;
+    if (R.np < 1) return;
+    double act = 0.0;
+    if (first) act = 0.0 + R.ica;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (!((feed >> k) & 1)) continue;
+        GT fraction = (GT)1.0;
+#pragma unroll
+        for (int g = 0; g < ng[k]; ++g) fraction *= take_power<GT>((GT)R.st[base[k] + g], pw[k][g]);
+        const double Gk = (double)(R.gb[k] * fraction);
+        const double ek = (double)R.ek[k];
+        if (v.caAdvance == 1) act += Gk * (ek - vmid);
+        else act += Gk * (ek - (2 * vmid - vnew));
+    }
+    double cst = R.pc;
+    v.poolCa[i] = caconc_process(cst, R.basal, R.f1, R.f2, R.ceil, R.floor, act);
+    A_<double>(arena, c.oPoolC)[i] = cst;
+    if (first) A_<double>(arena, c.oPoolICa)[i] = 0.0;
+}
+
 // Copies the slot types of a uniform cell to LDS (empty slots get
 // nGates = -1).  Returns the table to read channel types from.
 __device__ __forceinline__ const ChanType* stage_types(const CellDev& c, const ChanType* types, ChanType* sTypes) {
@@ -1478,6 +1545,10 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
     constexpr bool FD = std::is_same<GT, float>::value;
     ProbeClock pc = probe_clock(arena, c, step0);
     __syncthreads();
+    // kSigCa4 cells: the slots whose channel feeds the compartment's pool
+    int ca4Feed = 0;
+    if constexpr (SIG == kSigCa4)
+        for (int q = 0; q < 4; ++q) ca4Feed |= (types[q].caTgtLocal == 0 ? 1 : 0) << q;
 
     for (int k = 0; k < a.nsteps; ++k) {
         const bool first = a.firstOfCall && k == 0;
@@ -1674,7 +1745,19 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
                 for (int i = tid; i < n; i += nt) gwork[n + i] = w.B[i];
         } else {
             const int64_t stepNo = step0 + k + 1;
-            if (!LEAN && v.hasPools)
+            if (SIG == kSigCa4 && v.hasPools && c.nPoolSlots == 1) {
+                // two compartments per iteration, each one's loads issued
+                // together
+#pragma unroll 1
+                for (int i = tid; i < n; i += 2 * nt) {
+                    const int j = min(i + nt, n - 1);
+                    Ca4In<GT> Ra, Rb;
+                    ca4_load<GT>(arena, c, v, ca4Feed, i, first, Ra);
+                    ca4_load<GT>(arena, c, v, ca4Feed, j, first, Rb);
+                    ca4_finish<GT>(arena, c, v, ca4Feed, i, w.B[i], first, Ra);
+                    if (i + nt < n) ca4_finish<GT>(arena, c, v, ca4Feed, j, w.B[j], first, Rb);
+                }
+            } else if (!LEAN && v.hasPools)
                 for (int i = tid; i < n; i += nt) {
                     const double vmid = w.B[i];
                     const double vnew = 2 * vmid - v.V[i];

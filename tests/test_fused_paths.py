@@ -134,15 +134,22 @@ def test_trunk_chain_cables(pkg, oracle_mod, precision):
     assert spikes > 0
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_ca_kernel_fast_pass(pkg, oracle_mod, precision):
-    """Ca cells (branching 2-4, Na/Kdr/KA/CaL, CaConc, the 2-D K-AHP) below
-    the split threshold: the fused Ca kernel with junction groups."""
-    print(f"case: Ca cells 300-3000 compartments with 2-D K-AHP, 400 steps, {precision}, fused Ca kernel",
+@pytest.mark.parametrize("precision,kahp2d", [("fp64", True), ("mixed", True), ("fp64", False), ("mixed", False)])
+def test_ca_kernel_fast_pass(pkg, oracle_mod, precision, kahp2d):
+    """Ca cells (branching 2-4, Na/Kdr/KA/CaL, one CaConc per compartment fed
+    by CaL) below the split threshold: the fused Ca kernel with junction
+    groups, its advanceCalcium with each compartment's loads issued
+    together.  With the 2-D K-AHP (kahp2d), or without any K-AHP, so that
+    the FP64 form is held bit for bit."""
+    print(f"case: Ca cells 300-3000 compartments, 2-D K-AHP {kahp2d}, 400 steps, {precision}, fused Ca kernel",
           flush=True)
     rng = np.random.default_rng(43)
     specs = [workloads.random_ca_cell(rng, n, inject=inj, kahp2d=True)
              for n, inj in ((300, 2e-9), (1200, 4e-9), (3000, 6e-9))]
-    kernels, spikes = _run(pkg, oracle_mod, specs, 20e-6, 400, precision, fp64_tol=1e-6)
+    if not kahp2d:
+        for sp in specs:
+            sp.chans2d = sp.chans2d[:0]
+            sp.tables2d = []
+    kernels, spikes = _run(pkg, oracle_mod, specs, 20e-6, 400, precision, fp64_tol=1e-6 if kahp2d else 0.0)
     assert all("hsolve_step_ca_kernel" in k and "true" in k for k in kernels), kernels
     assert spikes >= 3
