@@ -878,24 +878,15 @@ __device__ __noinline__ void bwd_group_mem(char* arena, const CellDev& c, Desc u
     }
 }
 
-// The descriptor-form groups out of line.  Only the kernels for cells that
-// have such groups carry them (their registers would lower the occupancy of
-// every other cell's kernel).
-__device__ __noinline__ void fwd_group_regs(char* arena, const CellDev& c, Desc u, Work w) {
-    if ((u.a.x & 15) == 4) fwd_group_desc<4>(u, w.HJ0, w);
-    else fwd_group_desc<5>(u, w.HJ0, w);
-}
-
-__device__ __noinline__ void bwd_group_regs(Desc u, Work w) {
-    if ((u.a.x & 15) == 4) bwd_group_desc<4>(u, w);
-    else bwd_group_desc<5>(u, w);
-}
-
 template <bool SLOWG, bool FD = false>
 __device__ __forceinline__ void forward_unit(char* arena, const CellDev& c, const Desc& u, Work w) {
     if constexpr (SLOWG) {
         if (u.a.x & 32) {
-            fwd_group_regs(arena, c, u, w);
+            // inlined, so that their row and term accesses stay LDS
+            // instructions (through a call they become flat ones, each
+            // waiting for every global load in flight)
+            if ((u.a.x & 15) == 4) fwd_group_desc<4, FD>(u, w.HJ0, w);
+            else fwd_group_desc<5, FD>(u, w.HJ0, w);
             return;
         }
         if (u.a.x & 16) {
@@ -910,7 +901,8 @@ template <bool SLOWG, bool FD = false>
 __device__ __forceinline__ void backward_unit(char* arena, const CellDev& c, const Desc& u, Work w) {
     if constexpr (SLOWG) {
         if (u.a.x & 32) {
-            bwd_group_regs(u, w);
+            if ((u.a.x & 15) == 4) bwd_group_desc<4, FD>(u, w);
+            else bwd_group_desc<5, FD>(u, w);
             return;
         }
         if (u.a.x & 16) {
@@ -1808,7 +1800,7 @@ __global__ void __launch_bounds__(256, 4) hsolve_step_kernel(LaunchArgs a) {
 }
 
 // The fused kernel for cells with junction groups of four or more members
-// (see fwd_group_regs).
+// (the group forms inlined: only the SLOWG kernels carry their registers).
 template <typename GT>
 __global__ void __launch_bounds__(256) hsolve_step_slow_kernel(LaunchArgs a) {
     extern __shared__ double lds[];
