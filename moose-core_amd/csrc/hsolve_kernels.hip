@@ -1061,15 +1061,17 @@ __device__ __forceinline__ void ca4_finish(char* arena, const CellDev& c, const 
 }
 
 // Copies the slot types of a uniform cell to LDS (empty slots get
-// nGates = -1).  Returns the table to read channel types from.
-__device__ __forceinline__ const ChanType* stage_types(const CellDev& c, const ChanType* types, ChanType* sTypes) {
-    if (!c.uniform) return types;
+// nGates = -1).  Callers then read a uniform cell's types from sTypes and
+// other cells' from the global table, each through its own pointer, so the
+// compiler knows the address space of either (a pointer that may be either
+// makes every read a flat access, which waits for every load in flight).
+__device__ __forceinline__ void stage_types(const CellDev& c, const ChanType* types, ChanType* sTypes) {
+    if (!c.uniform) return;
     for (int k = threadIdx.x; k < c.nChanSlots; k += blockDim.x) {
         const int t = c.slotType[k];
         if (t >= 0) sTypes[k] = types[t];
         else sTypes[k].nGates = -1;
     }
-    return sTypes;
 }
 
 // PulseGen's period (device/PulseGen.cpp:362-369).
@@ -1527,7 +1529,7 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
     } else {
         level = gLevel;
     }
-    const ChanType* types = stage_types(c, a.types, sTypes);
+    stage_types(c, a.types, sTypes);
     const double dtg = a.dt;
     const int4* desc = A_<const int4>(arena, c.oDesc);
     int64_t* stepCounter = A_<int64_t>(arena, c.oSteps);
@@ -1537,10 +1539,11 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
     constexpr bool FD = std::is_same<GT, float>::value;
     ProbeClock pc = probe_clock(arena, c, step0);
     __syncthreads();
-    // kSigCa4 cells: the slots whose channel feeds the compartment's pool
+    // kSigCa4 cells (uniform: their types staged in LDS): the slots whose
+    // channel feeds the compartment's pool
     int ca4Feed = 0;
     if constexpr (SIG == kSigCa4)
-        for (int q = 0; q < 4; ++q) ca4Feed |= (types[q].caTgtLocal == 0 ? 1 : 0) << q;
+        for (int q = 0; q < 4; ++q) ca4Feed |= (sTypes[q].caTgtLocal == 0 ? 1 : 0) << q;
 
     for (int k = 0; k < a.nsteps; ++k) {
         const bool first = a.firstOfCall && k == 0;
@@ -1593,9 +1596,9 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
             }
         } else {
             if (uniform)
-                for (int i = tid; i < n; i += nt) channel_row<GT, true>(v, types, i, dtg, w, false, i);
+                for (int i = tid; i < n; i += nt) channel_row<GT, true>(v, sTypes, i, dtg, w, false, i);
             else
-                for (int i = tid; i < n; i += nt) channel_row<GT, false>(v, types, i, dtg, w, false, i);
+                for (int i = tid; i < n; i += nt) channel_row<GT, false>(v, a.types, i, dtg, w, false, i);
         }
         // (split mode: the channel kernel has added them to the rows)
         if (!SPLIT && (nInj > 0 || (first && nExt > 0) || (!LEAN && c.nXObj > 0))) {
@@ -1754,7 +1757,8 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
                     const double vmid = w.B[i];
                     const double vnew = 2 * vmid - v.V[i];
                     v.V[i] = vnew;
-                    calcium_row<GT>(arena, c, v, types, i, vmid, vnew, first);
+                    if (uniform) calcium_row<GT>(arena, c, v, sTypes, i, vmid, vnew, first);
+                    else calcium_row<GT>(arena, c, v, a.types, i, vmid, vnew, first);
                 }
             else {
                 int i0 = tid;
@@ -2182,7 +2186,7 @@ __global__ void __launch_bounds__(256) hsolve_channel_kernel(LaunchArgs a, const
     const int2 tile = tiles[blockIdx.x];
     const CellDev& c = a.cells[tile.x];
     const View<GT> v = make_view<GT>(a.arena, c, a.tsets);
-    const ChanType* types = stage_types(c, a.types, sTypes);
+    stage_types(c, a.types, sTypes);
     __syncthreads();
     const int i = tile.y + threadIdx.x;
     if (i >= v.n) return;
@@ -2197,8 +2201,8 @@ __global__ void __launch_bounds__(256) hsolve_channel_kernel(LaunchArgs a, const
         w.D = gwork;
         w.B = gwork + v.n;
     }
-    if (v.uniform) channel_row<GT, true>(v, types, i, a.dt, w, false, dst);
-    else channel_row<GT, false>(v, types, i, a.dt, w, false, dst);
+    if (v.uniform) channel_row<GT, true>(v, sTypes, i, a.dt, w, false, dst);
+    else channel_row<GT, false>(v, a.types, i, a.dt, w, false, dst);
     row_inputs(a, c, i, w.D[dst], w.B[dst]);
 }
 
@@ -2458,13 +2462,14 @@ __global__ void __launch_bounds__(256) hsolve_msg_kernel(LaunchArgs a) {
             // the update kernel does D over the whole batch
         } else if (c.hasPools) {
             const View<GT> v = make_view<GT>(arena, c, a.tsets);
-            const ChanType* types = stage_types(c, a.types, sTypes);
+            stage_types(c, a.types, sTypes);
             __syncthreads();
             for (int i = tid; i < n; i += nt) {
                 const double vmid = VMid[i];
                 const double vnew = 2 * vmid - V[i];
                 V[i] = vnew;
-                calcium_row<GT>(arena, c, v, types, i, vmid, vnew, first);
+                if (c.uniform) calcium_row<GT>(arena, c, v, sTypes, i, vmid, vnew, first);
+                else calcium_row<GT>(arena, c, v, a.types, i, vmid, vnew, first);
             }
         } else {
             for (int i = tid; i < n; i += nt) V[i] = 2 * VMid[i] - V[i];
@@ -2490,7 +2495,7 @@ __global__ void __launch_bounds__(256) hsolve_update_kernel(LaunchArgs a, const 
     __shared__ ChanType sTypes[kMaxSlots];
     const int2 tile = tiles[blockIdx.x];
     const CellDev& c = a.cells[tile.x];
-    const ChanType* types = stage_types(c, a.types, sTypes);
+    stage_types(c, a.types, sTypes);
     __syncthreads();
     const int i = tile.y + threadIdx.x;
     if (i >= c.n) return;
@@ -2501,7 +2506,8 @@ __global__ void __launch_bounds__(256) hsolve_update_kernel(LaunchArgs a, const 
     V[i] = vnew;
     if (c.hasPools) {
         const View<GT> v = make_view<GT>(a.arena, c, a.tsets);
-        calcium_row<GT>(a.arena, c, v, types, i, vmid, vnew, a.firstOfCall != 0);
+        if (c.uniform) calcium_row<GT>(a.arena, c, v, sTypes, i, vmid, vnew, a.firstOfCall != 0);
+        else calcium_row<GT>(a.arena, c, v, a.types, i, vmid, vnew, a.firstOfCall != 0);
     }
 }
 
@@ -2522,7 +2528,7 @@ __global__ void __launch_bounds__(256) hsolve_reinit_kernel(LaunchArgs a) {
     __shared__ ChanType sTypes[kMaxSlots];
     const CellDev& c = a.cells[a.order[blockIdx.x]];
     const View<GT> v = make_view<GT>(a.arena, c, a.tsets);
-    const ChanType* types = stage_types(c, a.types, sTypes);
+    stage_types(c, a.types, sTypes);
     // HSolveActive::reinit order (hsolve/HSolveActiveSetup.cpp:43-76):
     // spike generators, compartments, Ca pools, then the gates at the new V/Ca
     {
@@ -2555,9 +2561,9 @@ __global__ void __launch_bounds__(256) hsolve_reinit_kernel(LaunchArgs a) {
     __syncthreads();
     Work w{};
     if (v.uniform)
-        for (int i = threadIdx.x; i < v.n; i += blockDim.x) channel_row<GT, true>(v, types, i, a.dt, w, true, i);
+        for (int i = threadIdx.x; i < v.n; i += blockDim.x) channel_row<GT, true>(v, sTypes, i, a.dt, w, true, i);
     else
-        for (int i = threadIdx.x; i < v.n; i += blockDim.x) channel_row<GT, false>(v, types, i, a.dt, w, true, i);
+        for (int i = threadIdx.x; i < v.n; i += blockDim.x) channel_row<GT, false>(v, a.types, i, a.dt, w, true, i);
 }
 
 // The objects sending (Gk, Ek) every tick (Leakage, HHChannel2D, SynChan)
