@@ -36,7 +36,7 @@ extern "C" hipError_t hsg_launch_external(const hsg::LaunchArgs* a, const int2* 
 extern "C" hipError_t hsg_launch_channel_fast(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, int fp32,
                                              int ms, int sig, hipStream_t stream);
 extern "C" hipError_t hsg_launch_chan2d_reinit(const hsg::LaunchArgs* a, int nblocks, hipStream_t stream);
-extern "C" hipError_t hsg_launch_update(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, int fp32,
+extern "C" hipError_t hsg_launch_update(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, int fp32, int sig,
                                        hipStream_t stream);
 extern "C" hipError_t hsg_launch_outputs(const hsg::LaunchArgs* a, int nblocks, hipStream_t stream);
 extern "C" hipError_t hsg_launch_msg(const hsg::LaunchArgs* a, int nblocks, int threads, int fp32,
@@ -1616,7 +1616,7 @@ int split_solve(hsg_engine* e, const hsg_engine::Bin& bin, LaunchArgs& a, hipStr
     } else {
         HIPCHK(hsg_launch_solve(&a, nb, bin.solveThreads, e->fp32, bin.slow, st));
     }
-    HIPCHK(hsg_launch_update(&a, bin.dTiles, bin.nTiles, e->fp32, st));
+    HIPCHK(hsg_launch_update(&a, bin.dTiles, bin.nTiles, e->fp32, bin.sig, st));
     HIPCHK(hsg_launch_outputs(&a, nb, st));
     return 0;
 }

@@ -2490,7 +2490,7 @@ __global__ void __launch_bounds__(256) hsolve_msg_kernel(LaunchArgs a) {
 // (HSolveActive.cpp:153-216) for one compartment per thread over the bin's
 // 256-compartment tiles, so that a large cell's sweep is spread over the
 // whole GPU rather than one workgroup.
-template <typename GT>
+template <typename GT, int SIG = 0>
 __global__ void __launch_bounds__(256) hsolve_update_kernel(LaunchArgs a, const int2* __restrict__ tiles) {
     __shared__ ChanType sTypes[kMaxSlots];
     const int2 tile = tiles[blockIdx.x];
@@ -2501,6 +2501,21 @@ __global__ void __launch_bounds__(256) hsolve_update_kernel(LaunchArgs a, const 
     if (i >= c.n) return;
     const double* VMid = c.msg ? A_<const double>(a.arena, c.oVMid) : A_<const double>(a.arena, c.oWork) + c.n;
     double* V = A_<double>(a.arena, c.oV);
+    if constexpr (SIG == kSigCa4) {
+        // Na/Kdr/KA/CaL cells with one pool per compartment: every load of
+        // the compartment issued together (ca4_load), then calcium_row's
+        // arithmetic (ca4_finish)
+        if (c.hasPools && c.nPoolSlots == 1) {
+            int feed = 0;
+            for (int q = 0; q < 4; ++q) feed |= (sTypes[q].caTgtLocal == 0 ? 1 : 0) << q;
+            const View<GT> v = make_view<GT>(a.arena, c, a.tsets);
+            const double vm = VMid[i];
+            Ca4In<GT> R;
+            ca4_load<GT>(a.arena, c, v, feed, i, a.firstOfCall != 0, R);
+            ca4_finish<GT>(a.arena, c, v, feed, i, vm, a.firstOfCall != 0, R);
+            return;
+        }
+    }
     const double vmid = VMid[i];
     const double vnew = 2 * vmid - V[i];
     V[i] = vnew;
@@ -2724,14 +2739,17 @@ extern "C" hipError_t hsg_launch_chan2d_reinit(const hsg::LaunchArgs* a, int nbl
     return hipGetLastError();
 }
 
-extern "C" hipError_t hsg_launch_update(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, int fp32,
+extern "C" hipError_t hsg_launch_update(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, int fp32, int sig,
                                        hipStream_t stream) {
-    if (fp32)
-        hipLaunchKernelGGL(hsg::hsolve_update_kernel<float>, dim3(ntiles), dim3(hsg::kChannelTile), 0, stream, *a,
-                           tiles);
-    else
-        hipLaunchKernelGGL(hsg::hsolve_update_kernel<double>, dim3(ntiles), dim3(hsg::kChannelTile), 0, stream, *a,
-                           tiles);
+    const dim3 g(ntiles), b(hsg::kChannelTile);
+    if (sig == hsg::kSigCa4) {
+        if (fp32) hipLaunchKernelGGL((hsg::hsolve_update_kernel<float, hsg::kSigCa4>), g, b, 0, stream, *a, tiles);
+        else hipLaunchKernelGGL((hsg::hsolve_update_kernel<double, hsg::kSigCa4>), g, b, 0, stream, *a, tiles);
+    } else if (fp32) {
+        hipLaunchKernelGGL(hsg::hsolve_update_kernel<float>, g, b, 0, stream, *a, tiles);
+    } else {
+        hipLaunchKernelGGL(hsg::hsolve_update_kernel<double>, g, b, 0, stream, *a, tiles);
+    }
     return hipGetLastError();
 }
 
