@@ -891,6 +891,15 @@ int place_cell(hsg_engine* e, HostCell& hc) {
     d.oAct = 0;
     d.msg = msgOn ? 1 : 0;
     d.n2d = (int32_t)s.ch2d.size();
+    // one HHChannel2D on every compartment, in compartment order, of one
+    // kind, and nothing else beside the solver: the external kernel's
+    // direct path
+    d.uni2d = 0;
+    if (d.n2d == n && kindRep.size() == 1 && s.leaks.empty() && s.syn.empty()) {
+        bool inOrder = true;
+        for (int i = 0; i <= n && inOrder; ++i) inOrder = s.ch2dStart[i] == i;
+        d.uni2d = inOrder ? 1 : 0;
+    }
     d.nPulse = (int32_t)s.pulses.size();
     d.nVClamp = (int32_t)s.vclamps.size();
     d.nSetupEntries = s.nSetupEntries;

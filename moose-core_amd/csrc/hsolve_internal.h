@@ -206,6 +206,9 @@ struct CellDev {
     int64_t oDesc;          // int32[8*nUnits] unit descriptors, level order
     int32_t slowHJ;         // 1: some group uses the slow path (HJ scratch in oWork)
     int32_t ldsBytes;       // LDS this cell needs (work + level table)
+    int32_t uni2d;          // 1: the objects beside the solver are one HHChannel2D per
+                            // compartment (instance i on compartment i), all of one kind
+    int32_t pad2d;
     // work (global fallback when not in LDS): D[n], B[n], HJ[nHJ], term[2*nTerm]
     int64_t oWork;
     // injections: int32 idx[nInj] + f64 basal[nInj] + f64 varying[nInj]

@@ -496,10 +496,79 @@ __device__ __forceinline__ double2 external_sum(char* arena, const Table2DDev* _
     return make_double2(G, GE);
 }
 
+// external_sum for a cell whose objects beside the solver are one
+// HHChannel2D per compartment, instance i on compartment i, all of one kind
+// (c.uni2d): the same arithmetic in the same order, with the kind read once
+// (every lane reads the same record) and the instance, its gate states and
+// the compartment's own pool concentration loaded together, so that the
+// table corners are the only loads that wait on others.
+template <bool F32>
+__device__ __forceinline__ void external_uni2d(char* arena, const Table2DDev* __restrict__ tabs2d, double dt,
+                                               bool first, const CellDev& c, int i, double Vm,
+                                               const double* __restrict__ poolCa) {
+    using CT = typename std::conditional<F32, float, double>::type;
+    const Chan2DType& ty = A_<const Chan2DType>(arena, c.o2dType)[0];
+    const Chan2DDev ch = A_<const Chan2DDev>(arena, c.o2dChan)[i];
+    double* st = A_<double>(arena, c.o2dState);
+    double x3[3] = {st[3 * i], st[3 * i + 1], st[3 * i + 2]};
+    // conc1 / conc2 are the compartment's own pool in the usual case
+    const double own = c.nPoolSlots > 0 ? poolCa[i] : 0.0;
+    const bool sent = c.values2dSent != 0;
+    auto dep = [&](int d) -> double {
+        if (!sent) return 0.0;
+        if (d == 0) return Vm;
+        const int o = d == 1 ? ch.conc1 : d == 2 ? ch.conc2 : -1;
+        if (o < 0) return 0.0;
+        return o == i ? own : poolCa[o];
+    };
+    double G = 0.0, GE = 0.0;
+    if (first) {
+        const int32_t* xi = A_<const int32_t>(arena, c.oExtIdx);
+        for (int e = 0; e < c.nExt; ++e)
+            if (xi[e] == i) {
+                G += A_<const double>(arena, c.oExtG)[e];
+                GE += A_<const double>(arena, c.oExtGE)[e];
+            }
+    }
+    CT g = 0.0;
+    g += (CT)ch.Gbar;
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        if (!(ty.pw[ax] > 0.0)) continue;
+        const double u = dep(ty.dep[2 * ax]);
+        const double w = dep(ty.dep[2 * ax + 1]);
+        double A, B;
+        if (ty.pair[ax] >= 0) {
+            interp2d_pair<CT>(tabs2d[ty.pair[ax]], arena, u, w, A, B);
+        } else {
+            const Table2DDev& ta = tabs2d[ty.tabA[ax]];
+            const Table2DDev& tb = tabs2d[ty.tabB[ax]];
+            A = interp2d(ta, A_<const double>(arena, ta.off), u, w);
+            B = interp2d(tb, A_<const double>(arena, tb.off), u, w);
+        }
+        double x = x3[ax];
+        x = (ty.instant >> ax) & 1 ? A / B : integrate_ee(x, dt, A, B);
+        st[3 * i + ax] = x;
+        g *= take_power<CT>((CT)x, ty.pw[ax]);
+    }
+    const double Gk = (double)(g * (CT)ch.mod);
+    A_<double>(arena, c.o2dGk)[i] = Gk;
+    G += Gk;
+    GE += Gk * ch.Ek;
+    double* ext = A_<double>(arena, c.o2dExt);
+    ext[2 * i] = G;
+    ext[2 * i + 1] = GE;
+}
+
 // The external kernel's form (every kind of object).
 __device__ __forceinline__ void external_objects(char* arena, const Table2DDev* __restrict__ tabs2d, double dt,
                                                  bool first, bool f32, const CellDev& c, int i, double Vm,
                                                  const double* __restrict__ poolCa, int64_t stepNo) {
+    if (c.uni2d) {
+        if (f32) external_uni2d<true>(arena, tabs2d, dt, first, c, i, Vm, poolCa);
+        else external_uni2d<false>(arena, tabs2d, dt, first, c, i, Vm, poolCa);
+        return;
+    }
     if (!A_<const int32_t>(arena, c.oXFlag)[i]) return;
     if (f32) external_sum<true, true>(arena, tabs2d, dt, first, c, i, Vm, poolCa, stepNo);
     else external_sum<true, false>(arena, tabs2d, dt, first, c, i, Vm, poolCa, stepNo);
