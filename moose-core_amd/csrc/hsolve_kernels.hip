@@ -2285,8 +2285,25 @@ __global__ void __launch_bounds__(256) hsolve_channel_fast_kernel(LaunchArgs a, 
     const int i = tile.y + threadIdx.x;
     if (i >= v.n) return;
     double D, B;
-    channel_row_fast<GT, MS, false, SIG>(v, c, i, v.V[i], a.dt, D, B);
-    row_inputs(a, c, i, D, B);
+    if (c.uni2d) {
+        // one HHChannel2D per compartment: its external current (o2dExt,
+        // host entries included) and the injection entry loaded with the
+        // row, then row_inputs' additions in its order
+        const double2 g = reinterpret_cast<const double2*>(A_<const double>(a.arena, c.o2dExt))[i];
+        const int e = c.nInj > 0 ? A_<const int32_t>(a.arena, c.oInjOf)[i] : -1;
+        channel_row_fast<GT, MS, false, SIG>(v, c, i, v.V[i], a.dt, D, B);
+        if (e >= 0) {
+            const int64_t stepNo = *A_<const int64_t>(a.arena, c.oSteps) + 1;
+            const double t = a.dt * (double)(unsigned long long)stepNo;
+            const double var = inject_varying(a.arena, c, e, a.firstOfCall != 0, t, a.dt, stepNo);
+            B += var + A_<const double>(a.arena, c.oInjBasal)[e];
+        }
+        D += g.x;
+        B += g.y;
+    } else {
+        channel_row_fast<GT, MS, false, SIG>(v, c, i, v.V[i], a.dt, D, B);
+        row_inputs(a, c, i, D, B);
+    }
     double *Dp, *Bp;
     row_dest(a.arena, c, i, Dp, Bp);
     *Dp = D;
