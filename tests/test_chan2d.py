@@ -221,6 +221,12 @@ def kahp2d_batch():
              for n in (40, 150, 333)]
     # cells without 2-D channels in the same engine
     specs += [workloads.random_ca_cell(rng, 90, inject=0.7e-9), workloads.hh_ball_tree(inject=0.6e-9)]
+    # a cell with the 2-D channel on every other compartment only (the
+    # external kernel's general path; the cells above, with one on every
+    # compartment, take its direct path)
+    half = workloads.random_ca_cell(rng, 120, inject=0.8e-9, kahp2d=True)
+    half.chans2d = half.chans2d[::2]
+    specs.append(half)
     return specs
 
 
