@@ -539,6 +539,11 @@ int type_id(hsg_engine* e, const HostChanType& t, int tableSet) {
 int64_t work_lds_bytes(const CellDev& c) {
     return 8 * (2 * (int64_t)c.n + 2 * (int64_t)c.nTerm + (int64_t)c.nFill);
 }
+// The groups' junction coefficients when they live in LDS (after the work
+// arrays, before the level table).
+int64_t hj_lds_bytes(const CellDev& c) {
+    return c.hjInLds ? 8 * (int64_t)c.nHJ : 0;
+}
 // The level table and, 16-byte aligned after it, the chain's descriptors.
 int64_t level_bytes(const CellDev& c) {
     return 4 * ((int64_t)c.nLevels + 1) + 16 + 4 * kDescInts * ((int64_t)c.nLevels - c.chainLevel);
@@ -908,14 +913,17 @@ int place_cell(hsg_engine* e, HostCell& hc) {
     d.rootSlot = s.rootSlot;
     d.rootCnt = s.rootCnt;
     d.slowHJ = s.anySlowGroup ? 1 : 0;
-    // work arrays (and the level table) in LDS when they fit
+    // work arrays (and the level table) in LDS when they fit, and the
+    // groups' junction coefficients with them when those fit too
     const int64_t wl = work_lds_bytes(d), lb = level_bytes(d);
+    d.hjInLds = 0;
     if (msgOn) {
         d.workInLds = 1;
         d.ldsBytes = (int32_t)msgLds;
     } else if (wl + lb <= e->ldsLimit) {
         d.workInLds = 1;
-        d.ldsBytes = (int32_t)align_up(wl + lb);
+        d.hjInLds = s.anySlowGroup && wl + 8 * (int64_t)d.nHJ + lb <= e->ldsLimit ? 1 : 0;
+        d.ldsBytes = (int32_t)align_up(wl + hj_lds_bytes(d) + lb);
     } else {
         d.workInLds = 0;   // work arrays and level table read from HBM
         d.ldsBytes = 0;
@@ -1264,7 +1272,7 @@ void plan_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg
         // the rest by how many of their workgroups' LDS fit a CU's 160 KiB:
         // three (more than 40 KiB), four, five, six, eight, ten or more (a
         // bin's dynamic LDS is its largest cell's: fine classes waste little)
-        const int64_t wk = work_lds_bytes(d) + level_bytes(d);
+        const int64_t wk = work_lds_bytes(d) + hj_lds_bytes(d) + level_bytes(d);
         int sub = 0;
         while (fine && sub < kLdsClasses && wk <= kCuShare[sub]) ++sub;
         return s + 3 + (sub << 8);

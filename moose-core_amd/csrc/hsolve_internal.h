@@ -208,7 +208,9 @@ struct CellDev {
     int32_t ldsBytes;       // LDS this cell needs (work + level table)
     int32_t uni2d;          // 1: the objects beside the solver are one HHChannel2D per
                             // compartment (instance i on compartment i), all of one kind
-    int32_t pad2d;
+    int32_t hjInLds;        // 1 (with workInLds): the groups' junction coefficients are
+                            // staged into LDS after the work arrays each tick and
+                            // worked on in place there
     // work (global fallback when not in LDS): D[n], B[n], HJ[nHJ], term[2*nTerm]
     int64_t oWork;
     // injections: int32 idx[nInj] + f64 basal[nInj] + f64 varying[nInj]

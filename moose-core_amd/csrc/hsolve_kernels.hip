@@ -1576,6 +1576,15 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
     }
     w.HJ = gwork + nWork;
     w.HJ0 = A_<const double>(arena, c.oHJ0);
+    // the groups' junction coefficients in LDS (hjInLds): staged from their
+    // initial values at the start of every tick and worked on in place
+    const bool hjLds = SLOWG && WL && c.hjInLds;
+    const int nHJL = hjLds ? c.nHJ : 0;
+    const double* gHJ0 = w.HJ0;
+    if (hjLds) {
+        w.HJ = lds + nWork;
+        w.HJ0 = w.HJ;
+    }
     // level table: in LDS after the work arrays, or read from HBM
     const int32_t* gLevel = A_<const int32_t>(arena, c.oLevelStart);
     const int32_t* level;
@@ -1587,10 +1596,10 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
     const int nChain = nLevels - Lc;
     const int4* sChain = nullptr;
     if constexpr (WL) {
-        int32_t* sLevel = reinterpret_cast<int32_t*>(lds + nWork);
+        int32_t* sLevel = reinterpret_cast<int32_t*>(lds + nWork + nHJL);
         for (int l = tid; l <= nLevels; l += nt) sLevel[l] = gLevel[l];
         level = sLevel;
-        const size_t off = ((size_t)8 * nWork + 4 * ((size_t)nLevels + 1) + 15) & ~(size_t)15;
+        const size_t off = ((size_t)8 * (nWork + nHJL) + 4 * ((size_t)nLevels + 1) + 15) & ~(size_t)15;
         int4* sc = reinterpret_cast<int4*>(reinterpret_cast<char*>(lds) + off);
         const int4* gd = A_<const int4>(arena, c.oDesc) + (nChain > 0 ? 3 * (size_t)gLevel[Lc] : 0);
         for (int k = tid; k < 3 * nChain; k += nt) sc[k] = gd[k];
@@ -1616,6 +1625,18 @@ __device__ __forceinline__ void cell_steps(const LaunchArgs& a, const CellDev& c
 
     for (int k = 0; k < a.nsteps; ++k) {
         const bool first = a.firstOfCall && k == 0;
+        // the junction coefficients' initial values, sixteen loads in flight
+        // per thread before their stores (the forward sweep reads them after
+        // the barrier that follows phase A)
+        if (hjLds)
+            for (int k0 = tid; k0 < nHJL; k0 += 16 * nt) {
+                double hv[16];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) hv[j] = gHJ0[min(k0 + j * nt, nHJL - 1)];
+#pragma unroll
+                for (int j = 0; j < 16; ++j)
+                    if (k0 + j * nt < nHJL) w.HJ[k0 + j * nt] = hv[j];
+            }
         // first unit of level 0 for this thread, loaded while phase A runs
         Desc cur{};
         cur = load_desc(desc, min(tid, max(nUnits - 1, 0)));
