@@ -2077,6 +2077,65 @@ __device__ __forceinline__ void stage_gather(double* __restrict__ dst, const dou
     }
 }
 
+// A block's staging with every load in flight before any LDS store: the
+// first kStageAll entries per thread of each array (the index loads, then
+// the data they address, then the stores), which covers the blocks' usual
+// sizes; the rest of a longer array follows in the loops of stage_gather /
+// stage_copy.  Rows: D and B of the first `nR` local rows; hj0: the initial
+// junction coefficients (forward) or the working ones (backward); with
+// fills (backward): the fills.  Then the level table and descriptors.
+constexpr int kStageAll = 4;
+
+__device__ __forceinline__ void stage_block(char* arena, const BlockView& v, int nR, const double* hjSrc,
+                                            bool fills) {
+    const BlockDev& b = *v.b;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int32_t* fl = A_<const int32_t>(arena, b.oFill);
+    const int32_t* gLv = A_<const int32_t>(arena, b.oLevel);
+    const int4* gDesc = A_<const int4>(arena, b.oDesc);
+    const int nLv = kBlockForms * b.nLevels + 1, nDesc = 3 * b.nUnits, nF = fills ? b.nF : 0;
+    const int last = nR > 0 ? nR - 1 : 0;
+    int ri[kStageAll], hi[kStageAll], fi[kStageAll];
+#pragma unroll
+    for (int j = 0; j < kStageAll; ++j) {
+        const int k = tid + j * nt;
+        ri[j] = v.rows[min(k, last)];
+        hi[j] = b.nHJ > 0 ? v.hjMap[min(k, b.nHJ - 1)] : 0;
+        fi[j] = nF > 0 ? fl[min(k, nF - 1)] : 0;
+    }
+    double dv[kStageAll], bv[kStageAll], hv[kStageAll], fv[kStageAll];
+    int4 dd[kStageAll];
+    int lv[kStageAll];
+#pragma unroll
+    for (int j = 0; j < kStageAll; ++j) {
+        const int k = tid + j * nt;
+        dv[j] = v.gD[ri[j]];
+        bv[j] = v.gB[ri[j]];
+        hv[j] = b.nHJ > 0 ? hjSrc[hi[j]] : 0.0;
+        fv[j] = nF > 0 ? v.gF[fi[j]] : 0.0;
+        dd[j] = gDesc[min(k, nDesc - 1)];
+        lv[j] = gLv[min(k, nLv - 1)];
+    }
+#pragma unroll
+    for (int j = 0; j < kStageAll; ++j) {
+        const int k = tid + j * nt;
+        if (k < nR) {
+            v.w.D[k] = dv[j];
+            v.w.B[k] = bv[j];
+        }
+        if (k < b.nHJ) v.w.HJ[k] = hv[j];
+        if (k < nF) v.w.F[k] = fv[j];
+        if (k < nDesc) v.desc[k] = dd[j];
+        if (k < nLv) v.level[k] = lv[j];
+    }
+    const int off = kStageAll * nt;
+    if (nR > off) stage_gather(v.w.D + off, v.gD, v.w.B + off, v.gB, v.rows + off, nR - off);
+    if (b.nHJ > off) stage_gather(v.w.HJ + off, hjSrc, nullptr, nullptr, v.hjMap + off, b.nHJ - off);
+    if (nF > off) stage_gather(v.w.F + off, v.gF, nullptr, nullptr, fl + off, nF - off);
+    if (nDesc > off) stage_copy(v.desc + off, gDesc + off, nDesc - off);
+    if (nLv > off) stage_copy(v.level + off, gLv + off, nLv - off);
+}
+
 // the block's level table and descriptors into LDS
 __device__ __forceinline__ void stage_schedule(char* arena, const BlockView& v) {
     const BlockDev& b = *v.b;
@@ -2145,9 +2204,7 @@ __global__ void __launch_bounds__(256) hsolve_block_fwd_kernel(LaunchArgs a, con
     const BlockDev& b = *v.b;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int R = b.nRows + b.nExt;
-    stage_gather(v.w.D, v.gD, v.w.B, v.gB, v.rows, R);
-    stage_gather(v.w.HJ, v.gHJ0, nullptr, nullptr, v.hjMap, b.nHJ);
-    stage_schedule(arena, v);
+    stage_block(arena, v, R, v.gHJ0, false);
     __syncthreads();
     block_forward<SLOWG>(a, v);
     // the rows (eliminated, for the way back; fed, for the top), the term
@@ -2177,12 +2234,9 @@ __global__ void __launch_bounds__(256) hsolve_block_bwd_kernel(LaunchArgs a, con
     BlockView v = block_view(arena, c, t.y, lds);
     const BlockDev& b = *v.b;
     const int tid = threadIdx.x, nt = blockDim.x;
-    stage_gather(v.w.D, v.gD, v.w.B, v.gB, v.rows, b.nRows);
-    // the rows the subtrees feed hold their VMid (the top solved them)
-    stage_gather(v.w.B + b.nRows, v.gB, nullptr, nullptr, v.rows + b.nRows, b.nExt);
-    stage_gather(v.w.F, v.gF, nullptr, nullptr, A_<const int32_t>(arena, b.oFill), b.nF);
-    stage_gather(v.w.HJ, v.gHJ, nullptr, nullptr, v.hjMap, b.nHJ);
-    stage_schedule(arena, v);
+    // the rows the subtrees feed hold their VMid (the top solved them):
+    // staged with the rest (their D is staged too, and not read)
+    stage_block(arena, v, b.nRows + b.nExt, v.gHJ, true);
     __syncthreads();
     block_backward<SLOWG>(a, v);
     for (int k = tid; k < b.nRows; k += nt) v.gB[v.rows[k]] = v.w.B[k];
