@@ -2087,7 +2087,7 @@ __device__ __forceinline__ void stage_gather(double* __restrict__ dst, const dou
 constexpr int kStageAll = 4;
 
 __device__ __forceinline__ void stage_block(char* arena, const BlockView& v, int nR, const double* hjSrc,
-                                            bool fills) {
+                                            bool fills, bool imports = false) {
     const BlockDev& b = *v.b;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int32_t* fl = A_<const int32_t>(arena, b.oFill);
@@ -2095,15 +2095,19 @@ __device__ __forceinline__ void stage_block(char* arena, const BlockView& v, int
     const int4* gDesc = A_<const int4>(arena, b.oDesc);
     const int nLv = kBlockForms * b.nLevels + 1, nDesc = 3 * b.nUnits, nF = fills ? b.nF : 0;
     const int last = nR > 0 ? nR - 1 : 0;
-    int ri[kStageAll], hi[kStageAll], fi[kStageAll];
+    // (imports, the top: every term pair the blocks left, by global pair)
+    const int32_t* im = A_<const int32_t>(arena, b.oTImp);
+    const int nI = imports ? b.nTImp : 0;
+    int ri[kStageAll], hi[kStageAll], fi[kStageAll], ii[kStageAll];
 #pragma unroll
     for (int j = 0; j < kStageAll; ++j) {
         const int k = tid + j * nt;
         ri[j] = v.rows[min(k, last)];
         hi[j] = b.nHJ > 0 ? v.hjMap[min(k, b.nHJ - 1)] : 0;
         fi[j] = nF > 0 ? fl[min(k, nF - 1)] : 0;
+        ii[j] = nI > 0 ? im[min(k, nI - 1)] : 0;
     }
-    double dv[kStageAll], bv[kStageAll], hv[kStageAll], fv[kStageAll];
+    double dv[kStageAll], bv[kStageAll], hv[kStageAll], fv[kStageAll], tv[kStageAll], uv[kStageAll];
     int4 dd[kStageAll];
     int lv[kStageAll];
 #pragma unroll
@@ -2113,6 +2117,8 @@ __device__ __forceinline__ void stage_block(char* arena, const BlockView& v, int
         bv[j] = v.gB[ri[j]];
         hv[j] = b.nHJ > 0 ? hjSrc[hi[j]] : 0.0;
         fv[j] = nF > 0 ? v.gF[fi[j]] : 0.0;
+        tv[j] = nI > 0 ? v.gT[2 * ii[j]] : 0.0;
+        uv[j] = nI > 0 ? v.gT[2 * ii[j] + 1] : 0.0;
         dd[j] = gDesc[min(k, nDesc - 1)];
         lv[j] = gLv[min(k, nLv - 1)];
     }
@@ -2125,10 +2131,19 @@ __device__ __forceinline__ void stage_block(char* arena, const BlockView& v, int
         }
         if (k < b.nHJ) v.w.HJ[k] = hv[j];
         if (k < nF) v.w.F[k] = fv[j];
+        if (k < nI) {
+            v.w.T[2 * k] = tv[j];
+            v.w.T[2 * k + 1] = uv[j];
+        }
         if (k < nDesc) v.desc[k] = dd[j];
         if (k < nLv) v.level[k] = lv[j];
     }
     const int off = kStageAll * nt;
+    for (int k = off + tid; k < nI; k += nt) {
+        const int g = im[k];
+        v.w.T[2 * k] = v.gT[2 * g];
+        v.w.T[2 * k + 1] = v.gT[2 * g + 1];
+    }
     if (nR > off) stage_gather(v.w.D + off, v.gD, v.w.B + off, v.gB, v.rows + off, nR - off);
     if (b.nHJ > off) stage_gather(v.w.HJ + off, hjSrc, nullptr, nullptr, v.hjMap + off, b.nHJ - off);
     if (nF > off) stage_gather(v.w.F + off, v.gF, nullptr, nullptr, fl + off, nF - off);
@@ -2136,12 +2151,6 @@ __device__ __forceinline__ void stage_block(char* arena, const BlockView& v, int
     if (nLv > off) stage_copy(v.level + off, gLv + off, nLv - off);
 }
 
-// the block's level table and descriptors into LDS
-__device__ __forceinline__ void stage_schedule(char* arena, const BlockView& v) {
-    const BlockDev& b = *v.b;
-    stage_copy(v.level, A_<const int32_t>(arena, b.oLevel), kBlockForms * b.nLevels + 1);
-    stage_copy(v.desc, A_<const int4>(arena, b.oDesc), 3 * b.nUnits);
-}
 
 // A block's units with their work arrays in LDS.  The groups of four or
 // five members are inlined here (unlike the step kernels' out-of-line
@@ -2255,15 +2264,7 @@ __global__ void __launch_bounds__(256) hsolve_block_top_kernel(LaunchArgs a) {
     const BlockDev& b = *v.b;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int R = b.nRows + b.nExt;
-    stage_gather(v.w.D, v.gD, v.w.B, v.gB, v.rows, R);
-    const int32_t* im = A_<const int32_t>(arena, b.oTImp);
-    for (int j = tid; j < b.nTImp; j += nt) {
-        const int g = im[j];
-        v.w.T[2 * j] = v.gT[2 * g];
-        v.w.T[2 * j + 1] = v.gT[2 * g + 1];
-    }
-    stage_gather(v.w.HJ, v.gHJ0, nullptr, nullptr, v.hjMap, b.nHJ);
-    stage_schedule(arena, v);
+    stage_block(arena, v, R, v.gHJ0, false, true);
     __syncthreads();
     block_forward<SLOWG>(a, v);
     if (tid == 0) v.w.B[b.nRows] = v.w.B[b.nRows] / v.w.D[b.nRows];
