@@ -2185,25 +2185,25 @@ __device__ __forceinline__ void block_level(const BlockView& v, int l) {
     }
 }
 
-// (Both forms divide exactly here: the refined reciprocal of the mixed form
-// measured no faster in the block kernels.)
-template <bool SLOWG>
+// FD: the mixed form divides with the refined reciprocal (quot<true>), the
+// FP64 form exactly.
+template <bool SLOWG, bool FD>
 __device__ __forceinline__ void block_forward(const LaunchArgs& a, const BlockView& v) {
     for (int l = 0; l < v.b->nLevels; ++l) {
-        block_level<true, SLOWG, false>(v, l);
+        block_level<true, SLOWG, FD>(v, l);
         lds_barrier();
     }
 }
 
-template <bool SLOWG>
+template <bool SLOWG, bool FD>
 __device__ __forceinline__ void block_backward(const LaunchArgs& a, const BlockView& v) {
     for (int l = v.b->nLevels - 1; l >= 0; --l) {
-        block_level<false, SLOWG, false>(v, l);
+        block_level<false, SLOWG, FD>(v, l);
         lds_barrier();
     }
 }
 
-template <bool SLOWG>
+template <bool SLOWG, bool FD>
 __global__ void __launch_bounds__(256) hsolve_block_fwd_kernel(LaunchArgs a, const int2* __restrict__ tiles) {
     extern __shared__ double lds[];
     const int2 t = tiles[blockIdx.x];
@@ -2215,7 +2215,7 @@ __global__ void __launch_bounds__(256) hsolve_block_fwd_kernel(LaunchArgs a, con
     const int R = b.nRows + b.nExt;
     stage_block(arena, v, R, v.gHJ0, false);
     __syncthreads();
-    block_forward<SLOWG>(a, v);
+    block_forward<SLOWG, FD>(a, v);
     // the rows (eliminated, for the way back; fed, for the top), the term
     // pairs left for the top, the fills, the working junction coefficients
     for (int k = tid; k < R; k += nt) {
@@ -2234,7 +2234,7 @@ __global__ void __launch_bounds__(256) hsolve_block_fwd_kernel(LaunchArgs a, con
     for (int k = tid; k < b.nHJ; k += nt) v.gHJ[v.hjMap[k]] = v.w.HJ[k];
 }
 
-template <bool SLOWG>
+template <bool SLOWG, bool FD>
 __global__ void __launch_bounds__(256) hsolve_block_bwd_kernel(LaunchArgs a, const int2* __restrict__ tiles) {
     extern __shared__ double lds[];
     const int2 t = tiles[blockIdx.x];
@@ -2247,7 +2247,7 @@ __global__ void __launch_bounds__(256) hsolve_block_bwd_kernel(LaunchArgs a, con
     // staged with the rest (their D is staged too, and not read)
     stage_block(arena, v, b.nRows + b.nExt, v.gHJ, true);
     __syncthreads();
-    block_backward<SLOWG>(a, v);
+    block_backward<SLOWG, FD>(a, v);
     for (int k = tid; k < b.nRows; k += nt) v.gB[v.rows[k]] = v.w.B[k];
 }
 
@@ -2255,7 +2255,7 @@ __global__ void __launch_bounds__(256) hsolve_block_bwd_kernel(LaunchArgs a, con
 // and every term pair it uses staged in, forward, the root row (its only fed
 // row, VMid = B / D), backward; VMid of its rows written back.  As the solve
 // kernel of the tick, it advances the cell's step counter.
-template <bool SLOWG>
+template <bool SLOWG, bool FD>
 __global__ void __launch_bounds__(256) hsolve_block_top_kernel(LaunchArgs a) {
     extern __shared__ double lds[];
     const CellDev& c = a.cells[a.order[blockIdx.x]];
@@ -2266,10 +2266,10 @@ __global__ void __launch_bounds__(256) hsolve_block_top_kernel(LaunchArgs a) {
     const int R = b.nRows + b.nExt;
     stage_block(arena, v, R, v.gHJ0, false, true);
     __syncthreads();
-    block_forward<SLOWG>(a, v);
+    block_forward<SLOWG, FD>(a, v);
     if (tid == 0) v.w.B[b.nRows] = v.w.B[b.nRows] / v.w.D[b.nRows];
     lds_barrier();
-    block_backward<SLOWG>(a, v);
+    block_backward<SLOWG, FD>(a, v);
     for (int k = tid; k < R; k += nt) v.gB[v.rows[k]] = v.w.B[k];
     if (tid == 0) *A_<int64_t>(arena, c.oSteps) += a.nsteps;
 }
@@ -2920,24 +2920,36 @@ extern "C" hipError_t hsg_launch_outputs(const hsg::LaunchArgs* a, int nblocks, 
     return hipGetLastError();
 }
 
-extern "C" hipError_t hsg_launch_block(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, int slow, int fwd,
-                                      int lds, hipStream_t stream) {
+template <bool FD>
+static void launch_block(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, int slow, int fwd, int lds,
+                         hipStream_t stream) {
     const dim3 g(ntiles), b(256);
     if (fwd) {
-        if (slow) hipLaunchKernelGGL(hsg::hsolve_block_fwd_kernel<true>, g, b, lds, stream, *a, tiles);
-        else hipLaunchKernelGGL(hsg::hsolve_block_fwd_kernel<false>, g, b, lds, stream, *a, tiles);
+        if (slow) hipLaunchKernelGGL((hsg::hsolve_block_fwd_kernel<true, FD>), g, b, lds, stream, *a, tiles);
+        else hipLaunchKernelGGL((hsg::hsolve_block_fwd_kernel<false, FD>), g, b, lds, stream, *a, tiles);
     } else {
-        if (slow) hipLaunchKernelGGL(hsg::hsolve_block_bwd_kernel<true>, g, b, lds, stream, *a, tiles);
-        else hipLaunchKernelGGL(hsg::hsolve_block_bwd_kernel<false>, g, b, lds, stream, *a, tiles);
+        if (slow) hipLaunchKernelGGL((hsg::hsolve_block_bwd_kernel<true, FD>), g, b, lds, stream, *a, tiles);
+        else hipLaunchKernelGGL((hsg::hsolve_block_bwd_kernel<false, FD>), g, b, lds, stream, *a, tiles);
     }
+}
+
+extern "C" hipError_t hsg_launch_block(const hsg::LaunchArgs* a, const int2* tiles, int ntiles, int slow, int fwd,
+                                      int lds, hipStream_t stream) {
+    if (a->fp32) launch_block<true>(a, tiles, ntiles, slow, fwd, lds, stream);
+    else launch_block<false>(a, tiles, ntiles, slow, fwd, lds, stream);
     return hipGetLastError();
 }
 
 extern "C" hipError_t hsg_launch_top(const hsg::LaunchArgs* a, int nblocks, int threads, int slow, int lds,
                                     hipStream_t stream) {
     const dim3 g(nblocks), b(threads);
-    if (slow) hipLaunchKernelGGL(hsg::hsolve_block_top_kernel<true>, g, b, lds, stream, *a);
-    else hipLaunchKernelGGL(hsg::hsolve_block_top_kernel<false>, g, b, lds, stream, *a);
+    if (a->fp32) {
+        if (slow) hipLaunchKernelGGL((hsg::hsolve_block_top_kernel<true, true>), g, b, lds, stream, *a);
+        else hipLaunchKernelGGL((hsg::hsolve_block_top_kernel<false, true>), g, b, lds, stream, *a);
+    } else {
+        if (slow) hipLaunchKernelGGL((hsg::hsolve_block_top_kernel<true, false>), g, b, lds, stream, *a);
+        else hipLaunchKernelGGL((hsg::hsolve_block_top_kernel<false, false>), g, b, lds, stream, *a);
+    }
     return hipGetLastError();
 }
 
@@ -2952,9 +2964,12 @@ extern "C" hipError_t hsg_set_max_lds(int bytes) {
         (const void*)hsg::hsolve_solve_kernel<double, false>, (const void*)hsg::hsolve_solve_kernel<float, false>,
         (const void*)hsg::hsolve_solve_kernel<double, true>, (const void*)hsg::hsolve_solve_kernel<float, true>,
         (const void*)hsg::hsolve_msg_kernel<double>, (const void*)hsg::hsolve_msg_kernel<float>,
-        (const void*)hsg::hsolve_block_fwd_kernel<false>, (const void*)hsg::hsolve_block_fwd_kernel<true>,
-        (const void*)hsg::hsolve_block_bwd_kernel<false>, (const void*)hsg::hsolve_block_bwd_kernel<true>,
-        (const void*)hsg::hsolve_block_top_kernel<false>, (const void*)hsg::hsolve_block_top_kernel<true>,
+        (const void*)hsg::hsolve_block_fwd_kernel<false, false>, (const void*)hsg::hsolve_block_fwd_kernel<true, false>,
+        (const void*)hsg::hsolve_block_bwd_kernel<false, false>, (const void*)hsg::hsolve_block_bwd_kernel<true, false>,
+        (const void*)hsg::hsolve_block_top_kernel<false, false>, (const void*)hsg::hsolve_block_top_kernel<true, false>,
+        (const void*)hsg::hsolve_block_fwd_kernel<false, true>, (const void*)hsg::hsolve_block_fwd_kernel<true, true>,
+        (const void*)hsg::hsolve_block_bwd_kernel<false, true>, (const void*)hsg::hsolve_block_bwd_kernel<true, true>,
+        (const void*)hsg::hsolve_block_top_kernel<false, true>, (const void*)hsg::hsolve_block_top_kernel<true, true>,
         (const void*)hsg::hsolve_step_ca_kernel<double, false>, (const void*)hsg::hsolve_step_ca_kernel<float, false>,
         (const void*)hsg::hsolve_step_ca_kernel<double, true>, (const void*)hsg::hsolve_step_ca_kernel<float, true>};
     for (const void* k : kernels) {
