@@ -268,6 +268,7 @@ struct hsg_engine {
         int nTiles = 0;
         bool msg = false;           // message-passing solve kernel (split mode)
         bool hasExt = false;        // cells with HHChannel2D objects: the external kernel before every tick
+        bool uni2dAll = false;      // every cell has one HHChannel2D per compartment and nothing else beside
         bool split = false;         // channel kernel + solve kernel per tick
         bool slow = false;          // cells with junction groups of four or more members
         int chanMS = 0;             // channel kernel's fast pass for up to chanMS slots (0: generic)
@@ -1324,7 +1325,7 @@ void plan_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg
     for (auto& b : bins) {
         int maxN = 0, maxSlots = 0;
         bool allUniform = true;
-        bool lean = true, hh = true, ca4 = true;
+        bool lean = true, hh = true, ca4 = true, uni2d = !b.cells.empty();
         for (int id : b.cells) {
             const CellDev& d = e->cells[id]->dev;
             b.lds = std::max<int>(b.lds, d.ldsBytes);
@@ -1334,7 +1335,9 @@ void plan_bins(hsg_engine* e, const std::vector<int32_t>& cells, std::vector<hsg
             lean = lean && d.nPulse == 0 && d.nVClamp == 0 && d.nXObj == 0 && !d.hasPools;
             hh = hh && hh_pair(d);
             ca4 = ca4 && ca_quartet(d);
+            uni2d = uni2d && d.uni2d != 0;
         }
+        b.uni2dAll = uni2d;
         b.sig = ca4 ? 2 : 0;
         b.lean = lean ? (hh ? 2 : 1) : 0;
         b.chanMS = allUniform ? (maxSlots <= 2 ? 2 : maxSlots <= 4 ? 4 : 8) : 0;
@@ -1609,12 +1612,17 @@ int fused_fast(const hsg_engine::Bin& b) {
 }
 
 // Bandwidth phase of one split tick: the HHChannel2D objects (bins that have
-// them), then the channel kernel over the bin's tiles.
+// them), then the channel kernel over the bin's tiles.  On the fast pass,
+// bins whose cells all have one HHChannel2D per compartment (and nothing
+// else beside the solver) advance it inside the channel kernel instead.
 int split_wide(hsg_engine* e, const hsg_engine::Bin& bin, LaunchArgs& a, hipStream_t st) {
-    if (bin.hasExt) HIPCHK(hsg_launch_external(&a, bin.dTiles, bin.nTiles, st));
-    if (bin.chanMS > 0 && bin.fastSel != 0)
-        HIPCHK(hsg_launch_channel_fast(&a, bin.dTiles, bin.nTiles, e->fp32, bin.chanMS, bin.sig, st));
+    const bool fast = bin.chanMS > 0 && bin.fastSel != 0;
+    const bool inRow = fast && bin.hasExt && bin.uni2dAll;
+    if (bin.hasExt && !inRow) HIPCHK(hsg_launch_external(&a, bin.dTiles, bin.nTiles, st));
+    a.ext2dInRow = inRow ? 1 : 0;
+    if (fast) HIPCHK(hsg_launch_channel_fast(&a, bin.dTiles, bin.nTiles, e->fp32, bin.chanMS, bin.sig, st));
     else HIPCHK(hsg_launch_channel(&a, bin.dTiles, bin.nTiles, e->fp32, st));
+    a.ext2dInRow = 0;
     return 0;
 }
 

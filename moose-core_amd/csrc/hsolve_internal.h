@@ -353,6 +353,9 @@ struct LaunchArgs {
     int32_t        deferD;      // split mode: the solve kernels stop at VMid; the
                                 // update and outputs kernels finish the tick
     int32_t        fp32;        // mixed form: FP32 channel tables and currents
+    int32_t        ext2dInRow;  // split mode, cells with one HHChannel2D per
+                                // compartment (uni2d): the channel kernel
+                                // advances it in place of the external kernel
 };
 
 }  // namespace hsg

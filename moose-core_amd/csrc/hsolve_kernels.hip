@@ -503,9 +503,9 @@ __device__ __forceinline__ double2 external_sum(char* arena, const Table2DDev* _
 // the compartment's own pool concentration loaded together, so that the
 // table corners are the only loads that wait on others.
 template <bool F32>
-__device__ __forceinline__ void external_uni2d(char* arena, const Table2DDev* __restrict__ tabs2d, double dt,
-                                               bool first, const CellDev& c, int i, double Vm,
-                                               const double* __restrict__ poolCa) {
+__device__ __forceinline__ double2 external_uni2d(char* arena, const Table2DDev* __restrict__ tabs2d, double dt,
+                                                  bool first, const CellDev& c, int i, double Vm,
+                                                  const double* __restrict__ poolCa) {
     using CT = typename std::conditional<F32, float, double>::type;
     const Chan2DType& ty = A_<const Chan2DType>(arena, c.o2dType)[0];
     const Chan2DDev ch = A_<const Chan2DDev>(arena, c.o2dChan)[i];
@@ -558,6 +558,7 @@ __device__ __forceinline__ void external_uni2d(char* arena, const Table2DDev* __
     double* ext = A_<double>(arena, c.o2dExt);
     ext[2 * i] = G;
     ext[2 * i + 1] = GE;
+    return make_double2(G, GE);
 }
 
 // The external kernel's form (every kind of object).
@@ -2362,11 +2363,16 @@ __global__ void __launch_bounds__(256) hsolve_channel_fast_kernel(LaunchArgs a, 
     if (i >= v.n) return;
     double D, B;
     if (c.uni2d) {
-        // one HHChannel2D per compartment: its external current (o2dExt,
-        // host entries included) and the injection entry loaded with the
-        // row, then row_inputs' additions in its order
-        const double2 g = reinterpret_cast<const double2*>(A_<const double>(a.arena, c.o2dExt))[i];
+        // one HHChannel2D per compartment: its external current (host
+        // entries included), advanced here (ext2dInRow: the bin's external
+        // kernel is not launched) or read from o2dExt, and the injection
+        // entry loaded with the row, then row_inputs' additions in its order
         const int e = c.nInj > 0 ? A_<const int32_t>(a.arena, c.oInjOf)[i] : -1;
+        const double2 g =
+            a.ext2dInRow ? external_uni2d<std::is_same<GT, float>::value>(
+                               a.arena, a.tabs2d, a.dt, a.firstOfCall != 0, c, i, v.V[i],
+                               A_<const double>(a.arena, c.oPoolCa))
+                         : reinterpret_cast<const double2*>(A_<const double>(a.arena, c.o2dExt))[i];
         channel_row_fast<GT, MS, false, SIG>(v, c, i, v.V[i], a.dt, D, B);
         if (e >= 0) {
             const int64_t stepNo = *A_<const int64_t>(a.arena, c.oSteps) + 1;
