@@ -61,8 +61,10 @@ def test_fp64_results_equal_the_oracle(pkg, oracle_mod):
 
 def _large_2d_requests(serving):
     """A 5000-compartment Ca cell with the 2-D K-AHP (split mode, block
-    form of the solve, a request of its own bin), a 400-compartment one, and
-    an HH cell, with staggered arrivals."""
+    form of the solve, a request of its own bin; one HHChannel2D per
+    compartment, so the channel kernel advances it without the external
+    kernel), a 400-compartment one, and an HH cell, with staggered
+    arrivals."""
     rng = np.random.default_rng(33)
     specs = [workloads.random_ca_cell(rng, 5000, inject=6e-9, kahp2d=True),
              workloads.random_ca_cell(rng, 400, inject=3e-9, kahp2d=True),
